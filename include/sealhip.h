@@ -1,0 +1,77 @@
+/*
+ * sealhip.h -- C ABI of the MI355X (gfx950) RNS polynomial core.
+ *
+ * The reference has no C ABI: its GPU operations are reached through the C++
+ * classes of src/troy_cuda.cuh and, one level down, through the kernel_util::k*
+ * launchers (src/kernelutils.cuh:448-706).  This header is the thin layer that the
+ * C++ facade (seal-cuda_amd/troy/) and any FFI (ctypes, cgo, JNI) bind; each entry
+ * point names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Device data uses the reference layout: a ciphertext is [poly][limb][coeff] of
+ *     uint64, N coefficients per limb (src/ciphertext_cuda.cuh:60-67).
+ *   - Residues are fully reduced, in [0, q_i), on entry and on return.
+ *   - Pointers named d_* are device pointers; the library never allocates or frees
+ *     them.  Work is ordered on `stream` (a hipStream_t; NULL = default stream).
+ *   - Every function returns SHC_OK or a negative status.  shc_last_error() gives
+ *     the message of the calling thread's last failure.  Status values map to the
+ *     reference's exception types.
+ */
+#ifndef SEALHIP_H
+#define SEALHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SHC_OK = 0,
+    SHC_INVALID_ARGUMENT = -1, /* std::invalid_argument */
+    SHC_LOGIC_ERROR = -2,      /* std::logic_error      */
+    SHC_OUT_OF_RANGE = -3,     /* std::out_of_range     */
+    SHC_RUNTIME_ERROR = -4,    /* std::runtime_error    */
+    SHC_HIP_ERROR = -5         /* HIP runtime failure (reported as std::runtime_error) */
+};
+
+enum { SHC_SCHEME_NONE = 0, SHC_SCHEME_BFV = 1, SHC_SCHEME_CKKS = 2, SHC_SCHEME_BGV = 3 };
+
+typedef struct shc_context shc_context;
+
+/* Message of this thread's last failed call ("" if none). */
+const char *shc_last_error(void);
+
+/* Select the device and initialise the runtime.
+ * Replaces KernelProvider::initialize (src/kernelprovider.cuh:18-21). */
+int shc_init(int device);
+
+/* Build and upload the tables of one parameter set: NTT tables of every prime of
+ * `coeff_modulus` (key level, special prime last), and the scheme's RNS tables.
+ * Replaces SEALContextCuda(parms) (src/context_cuda.cuh:172-189, src/context_cuda.cu:5-32)
+ * together with the CPU SEALContext it is built from (src/context.cpp:138-522). */
+int shc_context_create(int scheme, uint32_t poly_modulus_degree, const uint64_t *coeff_modulus,
+                       uint32_t coeff_modulus_size, uint64_t plain_modulus, shc_context **out);
+void shc_context_destroy(shc_context *ctx);
+
+/* CoeffModulus::Create (src/modulus.cpp:80-165): primes = 1 mod 2N (times t when
+ * plain_modulus != 0) of the requested bit sizes, written to out[count]. */
+int shc_coeff_modulus_create(uint32_t poly_modulus_degree, const int *bit_sizes, uint32_t count,
+                             uint64_t plain_modulus, uint64_t *out);
+
+/* In-place negacyclic NTT of d_data[poly_count][limb_count][N]; limb i is reduced
+ * modulo coeff_modulus[i].  Natural order in, bit-reversed out.
+ * Replaces kernel_util::kNttNegacyclicHarvey (src/kernelutils.cuh:578-603). */
+int shc_ntt_forward(const shc_context *ctx, uint64_t *d_data, uint32_t poly_count, uint32_t limb_count,
+                    void *stream);
+
+/* Inverse of shc_ntt_forward (bit-reversed in, natural out, times N^-1).
+ * Replaces kernel_util::kInverseNttNegacyclicHarvey (src/kernelutils.cuh:607-632). */
+int shc_ntt_inverse(const shc_context *ctx, uint64_t *d_data, uint32_t poly_count, uint32_t limb_count,
+                    void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,88 @@
+/*
+ * oracle.h -- CPU restatement of the reference's RNS polynomial core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is the checker that the HIP path is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product library never links or calls it.
+ *
+ * Every function restates the mathematics of the reference CPU path (a SEAL-4.0
+ * port under the reference src/ directory) and cites the file:line it follows.  Arithmetic is done
+ * with exact 128-bit integers (unsigned __int128) and fully reduced residues: the
+ * reference's lazy ranges never change a fully reduced word, so the restatement
+ * reproduces the reference's outputs word for word.
+ *
+ * Layouts follow the reference: a polynomial in RNS form is [limb][coeff], a
+ * ciphertext is [poly][limb][coeff], all uint64.
+ *
+ * Parity status of each group is stated in its section below ("pinned by").
+ */
+#ifndef SEALHIP_ORACLE_H
+#define SEALHIP_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- arithmetic
+ * Pinned by: closed forms (Barrett ratio floor(2^128/q), Shoup floor(w 2^64/q)),
+ * brute force in tests, reference KATs (test/utils/uintarithsmallmod.cpp).     */
+uint64_t or_mulmod(uint64_t a, uint64_t b, uint64_t q);
+uint64_t or_powmod(uint64_t a, uint64_t e, uint64_t q);
+/* returns 0 when not invertible (reference tryInvertUintMod, uintarithsmallmod.h:311) */
+uint64_t or_invmod(uint64_t a, uint64_t q);
+uint64_t or_shoup(uint64_t w, uint64_t q);              /* floor(w * 2^64 / q) */
+void or_barrett_ratio(uint64_t q, uint64_t out[3]);     /* modulus.cpp:15-42 */
+uint32_t or_reverse_bits(uint32_t x, int bits);
+
+/* ---------------------------------------------------------------- primes
+ * Pinned by: reference KAT test/modulus.cpp (CoeffModTest), BFVDefault table,
+ * and the reference library (oracle/_ref) in tests.                            */
+int or_is_prime(uint64_t v);
+/* getPrimes (utils/numth.cpp:261-283): descending from (2^b-1)/factor*factor+1 */
+int or_get_primes(uint64_t factor, int bit_size, size_t count, uint64_t *out);
+/* CoeffModulus::Create (modulus.cpp:80-121); plain_modulus 0 = the variant without t */
+int or_coeff_modulus_create(size_t n, const int *bit_sizes, size_t count, uint64_t plain_modulus,
+                            uint64_t *out);
+/* tryMinimalPrimitiveRoot (utils/numth.cpp:335-362) */
+int or_minimal_primitive_root(uint64_t degree, uint64_t q, uint64_t *root);
+
+/* ---------------------------------------------------------------- NTT
+ * Pinned by: reference KATs (test/utils/ntt.cpp:52-99), O(N^2) negacyclic DFT
+ * at psi^(2 bitrev(k)+1), round trip, and oracle/_ref.                        */
+typedef struct {
+    int logn;
+    uint64_t n;
+    uint64_t q;
+    uint64_t root, inv_root;
+    uint64_t *root_powers;       /* [bitrev(i)] = psi^i, [0] = 1           (utils/ntt.cpp:36-44) */
+    uint64_t *root_quot;         /* Shoup quotients of root_powers                              */
+    uint64_t *inv_root_powers;   /* [bitrev(i-1)+1] = psi^-i, [0] = 1       (utils/ntt.cpp:47-54) */
+    uint64_t *inv_root_quot;
+    uint64_t inv_n, inv_n_quot;
+} or_ntt_tables;
+
+int or_ntt_tables_init(or_ntt_tables *t, int logn, uint64_t q);
+void or_ntt_tables_free(or_ntt_tables *t);
+/* nttNegacyclicHarvey (utils/ntt.cpp:165-187): natural order in, bit-reversed out, in [0,q) */
+void or_ntt_forward(const or_ntt_tables *t, uint64_t *a);
+/* inverseNttNegacyclicHarvey (utils/ntt.cpp:195-213): bit-reversed in, natural out, in [0,q) */
+void or_ntt_inverse(const or_ntt_tables *t, uint64_t *a);
+/* convenience: count polynomials of length 2^logn, all modulo q */
+int or_ntt_batch(int logn, uint64_t q, uint64_t *data, size_t count, int inverse);
+/* copies of the tables (root powers, their Shoup quotients, n^-1 and its quotient) */
+int or_ntt_tables_export(int logn, uint64_t q, uint64_t *rp, uint64_t *rq, uint64_t *irp, uint64_t *irq,
+                         uint64_t *inv_n);
+
+/* ---------------------------------------------------------------- Galois
+ * Pinned by: reference KATs (test/utils/galois.cpp:26-109) and oracle/_ref.    */
+uint32_t or_galois_elt_from_step(int logn, int step);        /* utils/galois.cpp:44-86   */
+void or_apply_galois(int logn, uint32_t elt, uint64_t q, const uint64_t *in, uint64_t *out); /* :143-162 */
+void or_apply_galois_ntt(int logn, uint32_t elt, const uint64_t *in, uint64_t *out);         /* :13-30, 164-175 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

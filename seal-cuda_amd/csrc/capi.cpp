@@ -1,0 +1,94 @@
+// capi.cpp -- extern "C" entry points declared in include/sealhip.h.
+#include "../../include/sealhip.h"
+#include "context.h"
+#include "ntt.h"
+#include <string>
+
+struct shc_context {
+    std::unique_ptr<shc::Context> impl;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return SHC_OK;
+    } catch (const shc::Error &e) {
+        g_last_error = e.what();
+        return e.kind == shc::ERR_HIP ? SHC_HIP_ERROR : (int)e.kind;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return SHC_RUNTIME_ERROR;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return SHC_RUNTIME_ERROR;
+    }
+}
+
+const shc::Context &ctx_of(const shc_context *c) {
+    if (!c || !c->impl) shc::fail(shc::ERR_INVALID_ARGUMENT, "context is null");
+    return *c->impl;
+}
+
+void check_limbs(const shc::Context &c, uint32_t limbs) {
+    if (limbs == 0 || limbs > c.key_count())
+        shc::fail(shc::ERR_INVALID_ARGUMENT, "limb_count must be in [1, coeff_modulus_size]");
+}
+}  // namespace
+
+extern "C" {
+
+const char *shc_last_error(void) { return g_last_error.c_str(); }
+
+int shc_init(int device) {
+    return guarded([&] {
+        int count = 0;
+        SHC_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) shc::fail(shc::ERR_INVALID_ARGUMENT, "no such device");
+        SHC_HIP(hipSetDevice(device));
+        SHC_HIP(hipFree(nullptr));
+    });
+}
+
+int shc_context_create(int scheme, uint32_t n, const uint64_t *moduli, uint32_t count, uint64_t plain,
+                       shc_context **out) {
+    return guarded([&] {
+        if (!out || !moduli) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        if (n < 2 || (n & (n - 1))) shc::fail(shc::ERR_INVALID_ARGUMENT, "poly_modulus_degree must be a power of two");
+        int logn = 0;
+        while ((1u << logn) < n) logn++;
+        auto impl = shc::make_context((shc::Scheme)scheme, logn, std::vector<uint64_t>(moduli, moduli + count), plain);
+        *out = new shc_context{std::move(impl)};
+    });
+}
+
+void shc_context_destroy(shc_context *ctx) { delete ctx; }
+
+int shc_coeff_modulus_create(uint32_t n, const int *bits, uint32_t count, uint64_t plain, uint64_t *out) {
+    return guarded([&] {
+        auto v = shc::hm::coeff_modulus_create(n, std::vector<int>(bits, bits + count), plain);
+        for (size_t i = 0; i < v.size(); i++) out[i] = v[i];
+    });
+}
+
+int shc_ntt_forward(const shc_context *ctx, uint64_t *d, uint32_t polys, uint32_t limbs, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ntt_forward(d, polys * limbs, limbs, c.primes(), c.logn, shc::NTT_FULL, (hipStream_t)stream);
+    });
+}
+
+int shc_ntt_inverse(const shc_context *ctx, uint64_t *d, uint32_t polys, uint32_t limbs, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ntt_inverse(d, polys * limbs, limbs, c.primes(), c.logn, shc::NTT_FULL, (hipStream_t)stream);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// common.h -- shared types of the HIP library: device-resident prime descriptors
+// and error plumbing between kernels, the context and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdexcept>
+#include <string>
+
+namespace shc {
+
+// One RNS prime as the kernels see it.  `fwd[i]` = (root_powers[i], Shoup quotient),
+// bit-reversed powers of the minimal primitive 2N-th root; `inv[i]` the "scrambled"
+// inverse powers (layouts of NTTTables, src/utils/ntt.cpp:36-54).
+struct alignas(64) DevPrime {
+    uint64_t q;
+    uint64_t two_q;
+    uint64_t ratio_lo, ratio_hi;     // floor(2^128 / q)
+    uint64_t inv_n, inv_n_shoup;     // N^-1 mod q as a Shoup operand
+    const ulonglong2 *fwd;           // [N]  (w, floor(w 2^64 / q))
+    const ulonglong2 *inv;           // [N]
+};
+
+// Error raised by host code; carries the exception family of the reference
+// (std::invalid_argument / logic_error / out_of_range / runtime_error).
+enum ErrKind { ERR_OK = 0, ERR_INVALID_ARGUMENT = -1, ERR_LOGIC = -2, ERR_OUT_OF_RANGE = -3, ERR_RUNTIME = -4,
+               ERR_HIP = -5 };
+
+struct Error : std::runtime_error {
+    ErrKind kind;
+    Error(ErrKind k, const std::string &m) : std::runtime_error(m), kind(k) {}
+};
+
+[[noreturn]] inline void fail(ErrKind k, const std::string &m) { throw Error(k, m); }
+
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) fail(ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define SHC_HIP(call) ::shc::hip_check((call), #call)
+
+inline void launch_check(const char *what) { hip_check(hipGetLastError(), what); }
+
+}  // namespace shc

@@ -1,0 +1,67 @@
+// context.cpp -- builds and uploads the per-parameter-set tables.
+#include "context.h"
+#include <cstring>
+
+namespace shc {
+
+DevBuf::DevBuf(size_t b) : bytes(b) {
+    if (b) SHC_HIP(hipMalloc(&ptr, b));
+}
+
+DevBuf::~DevBuf() {
+    if (ptr) (void)hipFree(ptr);
+}
+
+static DevPrime upload_prime(Context &c, const hm::NttTables &t) {
+    DevPrime p{};
+    p.q = t.q;
+    p.two_q = t.q << 1;
+    hm::barrett_ratio(t.q, p.ratio_lo, p.ratio_hi);
+    p.inv_n = t.inv_n;
+    p.inv_n_shoup = t.inv_n_shoup;
+    const size_t bytes = t.fwd.size() * sizeof(uint64_t);
+    DevBuf buf(2 * bytes);
+    SHC_HIP(hipMemcpy(buf.ptr, t.fwd.data(), bytes, hipMemcpyHostToDevice));
+    SHC_HIP(hipMemcpy(static_cast<char *>(buf.ptr) + bytes, t.inv.data(), bytes, hipMemcpyHostToDevice));
+    p.fwd = buf.as<const ulonglong2>();
+    p.inv = reinterpret_cast<const ulonglong2 *>(static_cast<char *>(buf.ptr) + bytes);
+    c.table_bufs.push_back(std::move(buf));
+    return p;
+}
+
+std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector<uint64_t> &moduli, uint64_t plain) {
+    if (logn < 1 || logn > 17) fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree must be a power of two in [2, 131072]");
+    if (moduli.empty() || moduli.size() > 64) fail(ERR_INVALID_ARGUMENT, "coeff_modulus size is invalid");
+    for (size_t i = 0; i < moduli.size(); i++) {
+        if (moduli[i] >> 60 || moduli[i] < 2) fail(ERR_INVALID_ARGUMENT, "coeff_modulus bit count is invalid");
+        for (size_t j = 0; j < i; j++)
+            if (moduli[i] == moduli[j]) fail(ERR_INVALID_ARGUMENT, "coeff_modulus must be pairwise coprime");
+    }
+    auto c = std::make_unique<Context>();
+    SHC_HIP(hipGetDevice(&c->device));
+    c->scheme = scheme;
+    c->logn = logn;
+    c->n = uint64_t(1) << logn;
+    c->moduli = moduli;
+    c->plain = plain;
+    for (uint64_t q : moduli) c->host_tables.push_back(hm::make_ntt_tables(logn, q));
+    for (auto &t : c->host_tables) c->host_primes.push_back(upload_prime(*c, t));
+
+    const size_t K = moduli.size();
+    c->d_primes = DevBuf(c->host_primes.size() * sizeof(DevPrime));
+    SHC_HIP(hipMemcpy(c->d_primes.ptr, c->host_primes.data(), c->host_primes.size() * sizeof(DevPrime),
+                      hipMemcpyHostToDevice));
+    if (K >= 2) {
+        std::vector<DevPrime> ks((K - 1) * K);
+        for (size_t L = 1; L < K; L++) {
+            DevPrime *row = ks.data() + (L - 1) * K;
+            for (size_t i = 0; i < L; i++) row[i] = c->host_primes[i];
+            row[L] = c->host_primes[K - 1];
+        }
+        c->d_keyswitch_primes = DevBuf(ks.size() * sizeof(DevPrime));
+        SHC_HIP(hipMemcpy(c->d_keyswitch_primes.ptr, ks.data(), ks.size() * sizeof(DevPrime), hipMemcpyHostToDevice));
+    }
+    return c;
+}
+
+}  // namespace shc

@@ -1,0 +1,89 @@
+"""ctypes binding of the C ABI in include/sealhip.h (libsealhip.so, built in-tree).
+
+Device buffers are passed as raw device pointers; callers typically hold them in
+torch tensors (torch is only the allocator and stream provider here).  Every call
+raises SealHipError with the library's message when the status is not SHC_OK, and
+loading fails loudly when the HIP library has not been built.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsealhip.so")
+
+SCHEME_BFV, SCHEME_CKKS, SCHEME_BGV = 1, 2, 3
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class SealHipError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"sealhip status {status}: {msg}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH}; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.shc_last_error.restype = ctypes.c_char_p
+        L.shc_init.argtypes = [ctypes.c_int]
+        L.shc_context_create.argtypes = [ctypes.c_int, ctypes.c_uint32, _u64p, ctypes.c_uint32, ctypes.c_uint64,
+                                         ctypes.POINTER(_vp)]
+        L.shc_context_destroy.argtypes = [_vp]
+        L.shc_context_destroy.restype = None
+        L.shc_coeff_modulus_create.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), ctypes.c_uint32,
+                                               ctypes.c_uint64, _u64p]
+        for name in ("shc_ntt_forward", "shc_ntt_inverse"):
+            getattr(L, name).argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp]
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        raise SealHipError(status, lib().shc_last_error().decode())
+
+
+def init(device=0):
+    check(lib().shc_init(device))
+
+
+def coeff_modulus_create(n, bits, plain_modulus=0):
+    out = (ctypes.c_uint64 * len(bits))()
+    check(lib().shc_coeff_modulus_create(n, (ctypes.c_int * len(bits))(*bits), len(bits), plain_modulus, out))
+    return list(out)
+
+
+class Context:
+    """Device-resident tables for one parameter set (SEALContextCuda analogue)."""
+
+    def __init__(self, scheme, n, coeff_modulus, plain_modulus=0):
+        self.scheme, self.n = scheme, n
+        self.moduli = [int(q) for q in coeff_modulus]
+        self.plain_modulus = plain_modulus
+        arr = (ctypes.c_uint64 * len(self.moduli))(*self.moduli)
+        h = _vp()
+        check(lib().shc_context_create(scheme, n, arr, len(self.moduli), plain_modulus, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().shc_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ntt_forward(self, d_ptr, polys, limbs, stream=None):
+        check(lib().shc_ntt_forward(self.handle, d_ptr, polys, limbs, stream))
+
+    def ntt_inverse(self, d_ptr, polys, limbs, stream=None):
+        check(lib().shc_ntt_inverse(self.handle, d_ptr, polys, limbs, stream))

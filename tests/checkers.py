@@ -1,0 +1,138 @@
+"""ctypes access to the two CPU checkers: the oracle restatement (oracle/liboracle.so)
+and, where it was built in this container, the reference CPU path itself
+(oracle/_ref/libtroyref.so, built by oracle/ref.mk from the reference sources).
+
+Only tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libtroyref.so")
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+_oracle = None
+_ref = None
+
+
+def ptr(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(u64p)
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle not built: {ORACLE_SO} (run __graft_entry__.build())")
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.or_mulmod.restype = ctypes.c_uint64
+        lib.or_mulmod.argtypes = [ctypes.c_uint64] * 3
+        lib.or_powmod.restype = ctypes.c_uint64
+        lib.or_powmod.argtypes = [ctypes.c_uint64] * 3
+        lib.or_invmod.restype = ctypes.c_uint64
+        lib.or_invmod.argtypes = [ctypes.c_uint64] * 2
+        lib.or_shoup.restype = ctypes.c_uint64
+        lib.or_shoup.argtypes = [ctypes.c_uint64] * 2
+        lib.or_is_prime.argtypes = [ctypes.c_uint64]
+        lib.or_get_primes.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t, u64p]
+        lib.or_coeff_modulus_create.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
+                                                ctypes.c_size_t, ctypes.c_uint64, u64p]
+        lib.or_minimal_primitive_root.argtypes = [ctypes.c_uint64, ctypes.c_uint64, u64p]
+        lib.or_ntt_batch.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, ctypes.c_size_t, ctypes.c_int]
+        lib.or_ntt_tables_export.argtypes = [ctypes.c_int, ctypes.c_uint64] + [u64p] * 5
+        lib.or_galois_elt_from_step.restype = ctypes.c_uint32
+        lib.or_galois_elt_from_step.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.or_apply_galois.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, u64p, u64p]
+        lib.or_apply_galois_ntt.argtypes = [ctypes.c_int, ctypes.c_uint32, u64p, u64p]
+        _oracle = lib
+    return _oracle
+
+
+def ref():
+    """The reference CPU library, or None where it was not built (e.g. on a GPU box)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        lib = ctypes.CDLL(REF_SO)
+        lib.ref_coeff_modulus_create.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.c_uint64, u64p]
+        lib.ref_get_primes.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64, u64p]
+        lib.ref_ntt_tables.argtypes = [ctypes.c_int, ctypes.c_uint64] + [u64p] * 5
+        lib.ref_ntt.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, ctypes.c_uint64, ctypes.c_int]
+        lib.ref_galois_elt_from_step.restype = ctypes.c_int64
+        lib.ref_galois_elt_from_step.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.ref_apply_galois.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, u64p, u64p,
+                                         ctypes.c_int]
+        _ref = lib
+    return _ref
+
+
+# ------------------------------------------------------------------ oracle helpers
+
+def coeff_modulus_create(n, bits, plain_modulus=0):
+    out = np.zeros(len(bits), dtype=np.uint64)
+    b = (ctypes.c_int * len(bits))(*bits)
+    rc = oracle().or_coeff_modulus_create(n, b, len(bits), plain_modulus, ptr(out))
+    if rc:
+        raise ValueError("coeff_modulus_create failed")
+    return [int(x) for x in out]
+
+
+def get_primes(factor, bit_size, count):
+    out = np.zeros(count, dtype=np.uint64)
+    if oracle().or_get_primes(factor, bit_size, count, ptr(out)):
+        raise ValueError("not enough primes")
+    return [int(x) for x in out]
+
+
+def ntt(data, q, logn, inverse=False):
+    """Fully reduced negacyclic NTT (or inverse) of every length-2^logn row of `data`."""
+    a = np.ascontiguousarray(data, dtype=np.uint64).copy()
+    n = 1 << logn
+    assert a.size % n == 0
+    if oracle().or_ntt_batch(logn, q, ptr(a), a.size // n, 1 if inverse else 0):
+        raise ValueError("bad modulus for NTT")
+    return a
+
+
+def ntt_rns(data, moduli, logn, inverse=False):
+    """NTT of a [..., limb, N] array where limb i is modulo moduli[i]."""
+    a = np.ascontiguousarray(data, dtype=np.uint64).copy()
+    L = len(moduli)
+    n = 1 << logn
+    v = a.reshape(-1, L, n)
+    for i, q in enumerate(moduli):
+        v[:, i, :] = ntt(v[:, i, :], q, logn, inverse)
+    return a
+
+
+# ------------------------------------------------------------------ seeded inputs
+
+def splitmix64(seed):
+    """splitmix64 stream; the same generator the bench uses for synthetic inputs."""
+    state = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield z ^ (z >> 31)
+
+
+def uniform_residues(rng, shape, moduli):
+    """Array of shape (..., len(moduli), N) with limb i uniform in [0, moduli[i])."""
+    a = np.empty(shape, dtype=np.uint64)
+    L = len(moduli)
+    assert shape[-2] == L
+    v = a.reshape(-1, L, shape[-1])
+    for i, q in enumerate(moduli):
+        v[:, i, :] = rng.integers(0, q, size=v[:, i, :].shape, dtype=np.uint64)
+    return a
