@@ -1,15 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X RNS polynomial core.
 
-Default workload (`--workload ntt`): batched negacyclic NTT, N = 2^16, 24 primes of
-60 bits (= 1 mod 2^17), 128 ciphertexts x 2 polys x 24 limbs, forward then inverse.
-One step = one forward + one inverse transform of the whole batch, inputs resident
-in HBM.  Prints one JSON line.
+Default workload (`--workload mulrelin`): CKKS multiply + relinearize at N = 2^16 with
+25 primes of 60 bits (24 data primes + 1 special prime, CoeffModulus::Create(65536,
+{60 x 25})), a batch of B = 128 ciphertext pairs in NTT form.  One step = tensor
+product of all 128 pairs + relinearization of all 128 size-3 results, inputs and the
+relinearization key resident in HBM.  The same work is also timed through the plain
+per-ciphertext path (one call per ciphertext).
+
+`--workload ntt`: batched negacyclic NTT, N = 2^16, 24 primes of 60 bits,
+128 ciphertexts x 2 polys x 24 limbs, forward then inverse.
+
+Prints one JSON line.  Data are seeded uniform residues (synthetic) and injected
+uniform key words; every word is fully reduced.
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -18,22 +25,168 @@ sys.path.insert(0, os.path.join(HERE, "seal-cuda_amd"))
 sys.path.insert(0, os.path.join(HERE, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+CKKS = 2
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="ntt", choices=["ntt"])
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="mulrelin", choices=["mulrelin", "ntt"])
+    p.add_argument("--batch", type=int, default=128)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-plain", action="store_true", help="skip the plain per-ciphertext leg")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
     return p.parse_args()
 
 
+def fill_uniform(torch, t, moduli, limb_dim, gen):
+    """Fill a [..., L, N] int64 device tensor with uniform residues of moduli[limb]."""
+    for i, q in enumerate(moduli):
+        sl = t.select(limb_dim, i)
+        sl.copy_(torch.randint(0, q, sl.shape, generator=gen, device="cuda", dtype=torch.int64))
+
+
+class Timer:
+    def __init__(self, torch):
+        self.torch = torch
+        self.a = torch.cuda.Event(enable_timing=True)
+        self.b = torch.cuda.Event(enable_timing=True)
+
+    def __enter__(self):
+        self.a.record()
+        return self
+
+    def __exit__(self, *exc):
+        self.b.record()
+        self.torch.cuda.synchronize()
+        self.ms = self.a.elapsed_time(self.b)
+
+
+def cpu_baseline_mulrelin(moduli, logn, budget_s):
+    """The reference CPU path (oracle/_ref, src/evaluator.cpp multiply + relinearize) when
+    it was shipped with this tree, else the oracle restatement, on one host thread."""
+    import numpy as np
+    import checkers as ck
+    n, K = 1 << logn, len(moduli)
+    L = K - 1
+    rng = np.random.default_rng(5)
+    a = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    b = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    key = ck.random_key(rng, moduli, logn)
+    kind = "reference" if ck.ref() is not None else "port"
+    sess = ck.RefSession(CKKS, n, moduli) if kind == "reference" else None
+    ops = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s or ops == 0:
+        if sess is not None:
+            ct3 = sess.multiply(a, b, L)
+            sess.relinearize(ct3, key, L)
+        else:
+            ct3 = ck.ckks_multiply(a, b, moduli[:L], logn)
+            ck.relinearize(CKKS, ct3, key, moduli, L, logn)
+        ops += 1
+    dt = time.perf_counter() - t0
+    return {"value": ops / dt, "unit": "mul+relin ops/s", "cores": 1, "kind": kind,
+            "sample": f"{ops} CKKS mul+relin at N=2^{logn}, L={L} (+1 special), one thread, {dt:.1f} s"}
+
+
+def bench_mulrelin(args):
+    import torch
+    import sealhip
+    sealhip.init(0)
+    logn, n = 16, 1 << 16
+    K, B = 25, args.batch
+    L = K - 1
+    moduli = sealhip.coeff_modulus_create(n, [60] * K)
+    ctx = sealhip.Context(sealhip.SCHEME_CKKS, n, moduli)
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    a = torch.empty((B, 2, L, n), dtype=torch.int64, device="cuda")
+    b = torch.empty((B, 2, L, n), dtype=torch.int64, device="cuda")
+    key = torch.empty((K - 1, 2, K, n), dtype=torch.int64, device="cuda")
+    fill_uniform(torch, a, moduli[:L], 2, gen)
+    fill_uniform(torch, b, moduli[:L], 2, gen)
+    fill_uniform(torch, key, moduli, 2, gen)
+    out = torch.empty((B, 3, L, n), dtype=torch.int64, device="cuda")
+    ws_bytes = ctx.keyswitch_workspace_size(B, L)
+    ws = torch.empty(((ws_bytes + 7) // 8,), dtype=torch.int64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+
+    def step_batched():
+        ctx.ckks_multiply(a.data_ptr(), b.data_ptr(), out.data_ptr(), B, L, st)
+        ctx.relinearize(out.data_ptr(), key.data_ptr(), B, L, ws.data_ptr(), ws_bytes, st)
+
+    ws1_bytes = ctx.keyswitch_workspace_size(1, L)
+    stride3 = 3 * L * n * 8
+    stride2 = 2 * L * n * 8
+
+    def step_plain():
+        for i in range(B):
+            ctx.ckks_multiply(a.data_ptr() + i * stride2, b.data_ptr() + i * stride2, out.data_ptr() + i * stride3,
+                              1, L, st)
+            ctx.relinearize(out.data_ptr() + i * stride3, key.data_ptr(), 1, L, ws.data_ptr(), ws1_bytes, st)
+
+    for _ in range(args.warmup):
+        step_batched()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with Timer(torch) as tm:
+        for _ in range(args.steps):
+            step_batched()
+    wall = time.perf_counter() - t0
+    ms = tm.ms / args.steps
+    ops_s = B / (ms * 1e-3)
+
+    plain = None
+    if not args.no_plain:
+        step_plain()
+        torch.cuda.synchronize()
+        with Timer(torch) as tp:
+            step_plain()
+        plain = {"ops_s": B / (tp.ms * 1e-3), "ms_per_step": tp.ms, "calls": 2 * B}
+
+    # Dominant kernel: the forward NTT over the ModUp limbs (B x L x (L+1) limbs), timed
+    # alone on the same stream with the same shape.
+    tlimbs = B * L * (L + 1)
+    tbuf = ws  # the ModUp region of the workspace holds valid residues of the same primes
+    with Timer(torch) as tn:
+        ctx.ntt_forward(tbuf.data_ptr(), tlimbs // K, K, st)
+    # the same limb count in K-limb polys: same kernels, same per-limb work
+    ntt_bytes = 2 * tlimbs * n * 8 + K * n * 16
+
+    limb_bytes = n * 8
+    unit_bytes = 6 * L * limb_bytes + (2 * (K - 1) * K * limb_bytes) / B  # inputs 4L + output 2L + key share
+    step_bytes = unit_bytes * B
+    achieved_step = step_bytes / (ms * 1e-3) / 1e9
+    achieved_ntt = ntt_bytes / (tn.ms * 1e-3) / 1e9
+    out_json = {
+        "metric": "HE mul+relin ops/s at N=2^16 batched vs plain; p50/p99 served latency",
+        "value": ops_s,
+        "unit": "mul+relin ops/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (seeded uniform residues; injected uniform relinearization key)",
+        "config": {"workload": f"CKKS multiply+relinearize, N=2^16, 24 data + 1 special 60-bit primes, "
+                               f"batch of {B} ciphertext pairs, NTT form, key resident",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved_ntt, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_ntt / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"forward NTT (columns+rows pass) over the ModUp limbs, {tlimbs} limbs",
+                     "algorithmic_bytes": ntt_bytes, "avg_ms": tn.ms},
+        "step_roofline": {"achieved": achieved_step, "unit": "GB/s", "frac": achieved_step / HBM_PEAK_GBS,
+                          "algorithmic_bytes_per_op": unit_bytes},
+        "plain": plain,
+        "batched_over_plain": (ops_s / plain["ops_s"]) if plain else None,
+        "served_latency": None,
+        "wall_s": wall,
+    }
+    if not args.no_cpu:
+        out_json["cpu_baseline"] = cpu_baseline_mulrelin(moduli, logn, args.cpu_seconds)
+    return out_json
+
+
 def cpu_baseline_ntt(moduli, logn, budget_s):
-    """Reference CPU path (oracle/_ref) when present, else the oracle restatement,
-    timed on this host: forward+inverse of single limbs until the budget is spent."""
     import numpy as np
     import checkers as ck
     n = 1 << logn
@@ -60,15 +213,13 @@ def cpu_baseline_ntt(moduli, logn, budget_s):
 def bench_ntt(args):
     import torch
     import sealhip
-    import checkers as ck  # noqa: F401  (cpu baseline only)
     sealhip.init(0)
     logn, n, L, polys = 16, 1 << 16, 24, 256
     moduli = sealhip.coeff_modulus_create(n, [60] * L)
     ctx = sealhip.Context(sealhip.SCHEME_CKKS, n, moduli)
     gen = torch.Generator(device="cuda").manual_seed(1)
     d = torch.empty((polys, L, n), dtype=torch.int64, device="cuda")
-    for i, q in enumerate(moduli):
-        d[:, i, :] = torch.randint(0, q, (polys, n), generator=gen, device="cuda", dtype=torch.int64)
+    fill_uniform(torch, d, moduli, 1, gen)
     st = torch.cuda.current_stream().cuda_stream
 
     def step():
@@ -78,36 +229,22 @@ def bench_ntt(args):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    # forward-only timing of the whole forward transform (both passes) for the roofline
-    fw0, fw1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    ev0.record()
-    for _ in range(args.steps):
-        step()
-    ev1.record()
-    torch.cuda.synchronize()
+    with Timer(torch) as tm:
+        for _ in range(args.steps):
+            step()
     wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1) / args.steps
-    fw0.record()
-    ctx.ntt_forward(d.data_ptr(), polys, L, st)
-    fw1.record()
-    torch.cuda.synchronize()
-    fwd_ms = fw0.elapsed_time(fw1)
+    ms = tm.ms / args.steps
+    with Timer(torch) as tf:
+        ctx.ntt_forward(d.data_ptr(), polys, L, st)
     ctx.ntt_inverse(d.data_ptr(), polys, L, st)
     torch.cuda.synchronize()
-
     limbs = polys * L
-    limb_bytes = n * 8
-    # compulsory traffic of one forward transform: read + write every limb once,
-    # plus one read of each prime's forward twiddle table (N x 16 B)
-    fwd_bytes = 2 * limbs * limb_bytes + L * n * 16
-    step_bytes = 2 * fwd_bytes
-    achieved = fwd_bytes / (fwd_ms * 1e-3) / 1e9
+    fwd_bytes = 2 * limbs * n * 8 + L * n * 16
+    achieved = fwd_bytes / (tf.ms * 1e-3) / 1e9
     out = {
         "metric": "batched NTT limbs/s (forward+inverse), N=2^16, 24x60-bit primes",
-        "value": limbs / (ms * 1e-3),
-        "unit": "limb fwd+inv/s",
+        "value": limbs / (ms * 1e-3), "unit": "limb fwd+inv/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (uniform residues, seeded)",
@@ -116,9 +253,8 @@ def bench_ntt(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "forward NTT (columns + rows pass)", "algorithmic_bytes": fwd_bytes,
-                     "avg_ms": fwd_ms},
+                     "avg_ms": tf.ms},
         "wall_s": wall,
-        "step_algorithmic_bytes": step_bytes,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_ntt(moduli, logn, args.cpu_seconds)
@@ -131,7 +267,7 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus N > 1 is not supported "
                                    "(independent ciphertexts shard across GPUs with no exchange)"}))
         sys.exit(2)
-    out = bench_ntt(args)
+    out = bench_mulrelin(args) if args.workload == "mulrelin" else bench_ntt(args)
     print(json.dumps(out), flush=True)
 
 

@@ -71,6 +71,48 @@ int shc_ntt_forward(const shc_context *ctx, uint64_t *d_data, uint32_t poly_coun
 int shc_ntt_inverse(const shc_context *ctx, uint64_t *d_data, uint32_t poly_count, uint32_t limb_count,
                     void *stream);
 
+/* ------------------------------------------------------------------ evaluator
+ * Batched entry points: `count` ciphertexts packed back to back, all at the same
+ * level with `limb_count` = L data primes.  Scratch memory is caller-owned; the
+ * *_workspace_size queries give its size in bytes for a given shape. */
+
+/* CKKS tensor product of size-2 ciphertexts in NTT form:
+ * d_out[c] = (a0 b0, a0 b1 + a1 b0, a1 b1) for c < count; d_a, d_b: [count][2][L][N],
+ * d_out: [count][3][L][N].
+ * Replaces EvaluatorCuda::ckksMultiply (src/evaluator_cuda.cu:384-432). */
+int shc_ckks_multiply(const shc_context *ctx, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out,
+                      uint32_t count, uint32_t limb_count, void *stream);
+
+int shc_keyswitch_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limb_count, size_t *bytes);
+
+/* Key switching: for each ciphertext c, (c0, c1) += ModDown(sum_j ModUp(d_j) * key[j]),
+ * where d_j are the limbs of the target polynomial (CKKS: NTT form; BFV: coefficient
+ * form).  d_target + c * target_stride and d_ct + c * ct_stride (strides in words) give
+ * ciphertext c's target and its (c0, c1).  d_key is a key-switching key at key level,
+ * [K-1][2][K][N] in NTT form (KSwitchKeys::data(), src/kswitchkeys_cuda.cuh).
+ * Replaces EvaluatorCuda::switchKeyInplace (src/evaluator_cuda.cu:1164-1363). */
+int shc_switch_key(const shc_context *ctx, const uint64_t *d_target, size_t target_stride, uint64_t *d_ct,
+                   size_t ct_stride, const uint64_t *d_key, uint32_t count, uint32_t limb_count, void *d_workspace,
+                   size_t workspace_bytes, void *stream);
+
+/* Relinearize size-3 ciphertexts d_ct[count][3][L][N] in place: the size-2 result is
+ * left in polys 0 and 1 of each ciphertext.
+ * Replaces EvaluatorCuda::relinearizeInplace (src/evaluator_cuda.cuh:101-105,
+ * src/evaluator_cuda.cu:703-744). */
+int shc_relinearize(const shc_context *ctx, uint64_t *d_ct, const uint64_t *d_relin_key, uint32_t count,
+                    uint32_t limb_count, void *d_workspace, size_t workspace_bytes, void *stream);
+
+int shc_rescale_workspace_size(const shc_context *ctx, uint32_t count, uint32_t poly_count, uint32_t limb_count,
+                               size_t *bytes);
+
+/* Divide-and-round by the last prime, dropping one limb: CKKS rescale (NTT form) or
+ * BFV modulus switch (coefficient form).  d_in: [count][poly_count][L][N],
+ * d_out: [count][poly_count][L-1][N] (must not overlap d_in).
+ * Replaces EvaluatorCuda::rescaleToNext / modSwitchToNext (src/evaluator_cuda.cu:749-824). */
+int shc_rescale_to_next(const shc_context *ctx, const uint64_t *d_in, uint64_t *d_out, uint32_t count,
+                        uint32_t poly_count, uint32_t limb_count, void *d_workspace, size_t workspace_bytes,
+                        void *stream);
+
 #ifdef __cplusplus
 }
 #endif

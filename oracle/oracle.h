@@ -82,6 +82,24 @@ uint32_t or_galois_elt_from_step(int logn, int step);        /* utils/galois.cpp
 void or_apply_galois(int logn, uint32_t elt, uint64_t q, const uint64_t *in, uint64_t *out); /* :143-162 */
 void or_apply_galois_ntt(int logn, uint32_t elt, const uint64_t *in, uint64_t *out);         /* :13-30, 164-175 */
 
+/* ---------------------------------------------------------------- evaluator (evaluator.c)
+ * Pinned by: oracle/_ref (the reference's own Evaluator) on identical injected
+ * ciphertexts and keys at N = 2^10..2^13, decrypt-equality with real keys, and the
+ * closed forms in the tests (tensor = schoolbook product; rescale = CRT round).     */
+/* Evaluator::ckksMultiply (src/evaluator.cpp:626-702); out has a_size + b_size - 1 polys */
+int or_ckks_multiply(int logn, const uint64_t *moduli, size_t L, const uint64_t *a, size_t a_size,
+                     const uint64_t *b, size_t b_size, uint64_t *out);
+/* Evaluator::switchKeyInplace (src/evaluator.cpp:2310-2653); scheme 1 = BFV, 2 = CKKS.
+   key: [K-1][2][K][N]; target: L limbs; ct: [2][L][N] updated in place. */
+int or_switch_key(int scheme, int logn, const uint64_t *key_moduli, size_t K, size_t L, const uint64_t *target,
+                  const uint64_t *key, uint64_t *ct);
+/* Evaluator::relinearizeInternal (src/evaluator.cpp:1113-1163): ct3 [3][L][N] -> first 2 polys */
+int or_relinearize(int scheme, int logn, const uint64_t *key_moduli, size_t K, size_t L, uint64_t *ct3,
+                   const uint64_t *relin_key);
+/* Evaluator::modSwitchScaleToNext (src/evaluator.cpp:1165-1240): in [polys][L][N] -> out [polys][L-1][N] */
+int or_rescale_to_next(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *in, size_t polys,
+                       uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,4 +89,111 @@ int ref_apply_galois(int logn, uint32_t elt, uint64_t q, const uint64_t* in, uin
     } catch (...) { return -1; }
 }
 
+// ---------------------------------------------------------------- evaluator with injected words
+
+struct RefSession {
+    std::unique_ptr<SEALContext> context;
+    std::unique_ptr<Evaluator> evaluator;
+    SchemeType scheme;
+    size_t n;
+    size_t K;
+};
+
+// EncryptionParameters + SEALContext(parms, expand_mod_chain = true, SecurityLevel::none)
+// (src/context.cpp:458-522).  scheme: 1 = BFV, 2 = CKKS.
+void *ref_session_create(int scheme, uint64_t n, const uint64_t *moduli, uint64_t K, uint64_t plain) {
+    try {
+        auto s = new RefSession();
+        s->scheme = scheme == 1 ? SchemeType::bfv : SchemeType::ckks;
+        EncryptionParameters parms(s->scheme);
+        parms.setPolyModulusDegree(n);
+        std::vector<Modulus> mods;
+        for (size_t i = 0; i < K; i++) mods.emplace_back(moduli[i]);
+        parms.setCoeffModulus(mods);
+        if (scheme == 1) parms.setPlainModulus(plain);
+        s->context.reset(new SEALContext(parms, true, SecurityLevel::none));
+        if (!s->context->parametersSet()) { delete s; return nullptr; }
+        s->evaluator.reset(new Evaluator(*s->context));
+        s->n = n;
+        s->K = K;
+        return s;
+    } catch (...) { return nullptr; }
+}
+
+void ref_session_destroy(void *h) { delete static_cast<RefSession *>(h); }
+
+static ParmsID parms_for_level(const RefSession &s, size_t L) {
+    auto cd = s.context->firstContextData();
+    while (cd && cd->parms().coeffModulus().size() != L) cd = cd->nextContextData();
+    if (!cd) throw std::invalid_argument("no level with that many primes");
+    return cd->parmsID();
+}
+
+static void load_ct(const RefSession &s, Ciphertext &ct, size_t L, size_t size, const uint64_t *data, double scale) {
+    ct.resize(*s.context, parms_for_level(s, L), size);
+    ct.isNttForm() = (s.scheme == SchemeType::ckks);
+    ct.scale() = (s.scheme == SchemeType::ckks) ? scale : 1.0;
+    std::memcpy(ct.data(), data, size * L * s.n * 8);
+}
+
+// KSwitchKeys with one key vector of K-1 size-2 key-level ciphertexts (NTT form),
+// filled from words laid out [K-1][2][K][N] (KeyGenerator::generateOneKswitchKey layout,
+// src/keygenerator.cpp:294-329).
+static void load_kswitch(const RefSession &s, KSwitchKeys &ks, size_t index, const uint64_t *words) {
+    auto key_parms = s.context->keyParmsID();
+    ks.parmsID() = key_parms;
+    ks.data().resize(index + 1);
+    auto &vec = ks.data()[index];
+    vec.clear();
+    const size_t dnum = s.K - 1, per = 2 * s.K * s.n;
+    for (size_t j = 0; j < dnum; j++) {
+        PublicKey pk;
+        pk.data().resize(*s.context, key_parms, 2);
+        pk.data().isNttForm() = true;
+        std::memcpy(pk.data().data(), words + j * per, per * 8);
+        pk.parmsID() = key_parms;
+        vec.push_back(std::move(pk));
+    }
+}
+
+// Evaluator::multiplyInplace (src/evaluator.cpp:355-383): size-2 x size-2 -> size 3.
+int ref_multiply(void *h, uint64_t L, const uint64_t *a, const uint64_t *b, uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        Ciphertext x, y;
+        load_ct(s, x, L, 2, a, 1024.0);
+        load_ct(s, y, L, 2, b, 1024.0);
+        s.evaluator->multiplyInplace(x, y);
+        std::memcpy(out, x.data(), 3 * L * s.n * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_multiply: %s\n", e.what()); return -1; }
+}
+
+// Evaluator::relinearizeInplace (src/evaluator.cpp:1113-1163): size 3 -> size 2.
+int ref_relinearize(void *h, uint64_t L, const uint64_t *ct3, const uint64_t *key, uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        Ciphertext x;
+        load_ct(s, x, L, 3, ct3, 1024.0);
+        RelinKeys rk;
+        load_kswitch(s, rk, 0, key);
+        s.evaluator->relinearizeInplace(x, rk);
+        std::memcpy(out, x.data(), 2 * L * s.n * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_relinearize: %s\n", e.what()); return -1; }
+}
+
+// Evaluator::rescaleToNext (CKKS) / modSwitchToNext (BFV) (src/evaluator.cpp:1165-1240).
+int ref_rescale(void *h, uint64_t L, uint64_t size, const uint64_t *in, uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        Ciphertext x, y;
+        load_ct(s, x, L, size, in, 1024.0 * 1024.0 * 1024.0);
+        if (s.scheme == SchemeType::ckks) s.evaluator->rescaleToNext(x, y);
+        else s.evaluator->modSwitchToNext(x, y);
+        std::memcpy(out, y.data(), size * (L - 1) * s.n * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_rescale: %s\n", e.what()); return -1; }
+}
+
 }  // extern "C"

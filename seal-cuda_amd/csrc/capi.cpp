@@ -2,6 +2,7 @@
 #include "../../include/sealhip.h"
 #include "context.h"
 #include "ntt.h"
+#include "rns_ops.h"
 #include <string>
 
 struct shc_context {
@@ -88,6 +89,60 @@ int shc_ntt_inverse(const shc_context *ctx, uint64_t *d, uint32_t polys, uint32_
         const auto &c = ctx_of(ctx);
         check_limbs(c, limbs);
         shc::ntt_inverse(d, polys * limbs, limbs, c.primes(), c.logn, shc::NTT_FULL, (hipStream_t)stream);
+    });
+}
+
+int shc_ckks_multiply(const shc_context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t count,
+                      uint32_t limbs, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ckks_tensor(c, a, b, out, count, limbs, (hipStream_t)stream);
+    });
+}
+
+int shc_keyswitch_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = shc::keyswitch_workspace(c, count, limbs);
+    });
+}
+
+int shc_switch_key(const shc_context *ctx, const uint64_t *target, size_t target_stride, uint64_t *ct,
+                   size_t ct_stride, const uint64_t *key, uint32_t count, uint32_t limbs, void *ws, size_t ws_bytes,
+                   void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::switch_key(c, target, target_stride, ct, ct_stride, key, count, limbs, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_relinearize(const shc_context *ctx, uint64_t *ct, const uint64_t *key, uint32_t count, uint32_t limbs,
+                    void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        const size_t poly = (size_t)limbs * c.n;
+        shc::switch_key(c, ct + 2 * poly, 3 * poly, ct, 3 * poly, key, count, limbs, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_rescale_workspace_size(const shc_context *ctx, uint32_t count, uint32_t polys, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = shc::rescale_workspace(c, count, polys, limbs);
+    });
+}
+
+int shc_rescale_to_next(const shc_context *ctx, const uint64_t *in, uint64_t *out, uint32_t count, uint32_t polys,
+                        uint32_t limbs, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::rescale_to_next(c, in, out, count, polys, limbs, ws, ws_bytes, (hipStream_t)stream);
     });
 }
 

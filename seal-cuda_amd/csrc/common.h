@@ -20,6 +20,14 @@ struct alignas(64) DevPrime {
     const ulonglong2 *inv;           // [N]
 };
 
+// Constants of a divide-and-round by a prime p into prime q_i (p = the special prime
+// for key switching, q_last for rescale / modulus switching).
+struct alignas(32) DivRoundConst {
+    uint64_t inv, inv_shoup;   // p^-1 mod q_i as a Shoup operand
+    uint64_t half_mod;         // floor(p/2) mod q_i
+    uint64_t pad;
+};
+
 // Error raised by host code; carries the exception family of the reference
 // (std::invalid_argument / logic_error / out_of_range / runtime_error).
 enum ErrKind { ERR_OK = 0, ERR_INVALID_ARGUMENT = -1, ERR_LOGIC = -2, ERR_OUT_OF_RANGE = -3, ERR_RUNTIME = -4,

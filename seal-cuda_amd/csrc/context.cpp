@@ -61,6 +61,24 @@ std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector
         c->d_keyswitch_primes = DevBuf(ks.size() * sizeof(DevPrime));
         SHC_HIP(hipMemcpy(c->d_keyswitch_primes.ptr, ks.data(), ks.size() * sizeof(DevPrime), hipMemcpyHostToDevice));
     }
+    // divide-and-round constants: RNSTool::inv_q_last_mod_q_ (src/utils/rns.cpp:778-790)
+    // for every level, and the special prime's for key switching
+    auto divconst = [](uint64_t p, uint64_t qi) {
+        DivRoundConst d{};
+        d.inv = hm::invmod(p % qi, qi);
+        if (!d.inv) fail(ERR_LOGIC, "invalid rns bases");
+        d.inv_shoup = hm::shoup(d.inv, qi);
+        d.half_mod = (p >> 1) % qi;
+        return d;
+    };
+    std::vector<DivRoundConst> ksc(K), rsc((K + 1) * K);
+    for (size_t i = 0; i + 1 < K; i++) ksc[i] = divconst(moduli[K - 1], moduli[i]);
+    for (size_t L = 2; L <= K; L++)
+        for (size_t i = 0; i + 1 < L; i++) rsc[L * K + i] = divconst(moduli[L - 1], moduli[i]);
+    c->d_ks_consts = DevBuf(ksc.size() * sizeof(DivRoundConst));
+    SHC_HIP(hipMemcpy(c->d_ks_consts.ptr, ksc.data(), ksc.size() * sizeof(DivRoundConst), hipMemcpyHostToDevice));
+    c->d_rescale_consts = DevBuf(rsc.size() * sizeof(DivRoundConst));
+    SHC_HIP(hipMemcpy(c->d_rescale_consts.ptr, rsc.data(), rsc.size() * sizeof(DivRoundConst), hipMemcpyHostToDevice));
     return c;
 }
 

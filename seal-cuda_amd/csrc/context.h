@@ -52,8 +52,15 @@ struct Context {
     std::vector<DevBuf> table_bufs;
     DevBuf d_primes;                   // DevPrime[total]
     DevBuf d_keyswitch_primes;         // DevPrime[(K-1) * K]: row L-1 = level L's targets
+    DevBuf d_ks_consts;                // DivRoundConst[K]: division by the special prime, entry i < K-1
+    DevBuf d_rescale_consts;           // DivRoundConst[(K+1) * K]: row L = division by q_{L-1}, entries i < L-1
 
     int device = 0;
+
+    const DivRoundConst *ks_consts() const { return d_ks_consts.as<DivRoundConst>(); }
+    const DivRoundConst *rescale_consts(size_t level) const {
+        return d_rescale_consts.as<DivRoundConst>() + level * moduli.size();
+    }
 
     size_t key_count() const { return moduli.size(); }
     const DevPrime *primes() const { return d_primes.as<DevPrime>(); }

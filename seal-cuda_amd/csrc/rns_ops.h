@@ -1,0 +1,30 @@
+// rns_ops.h -- elementwise RNS kernels and the key-switching pipeline.
+#pragma once
+#include "common.h"
+#include "context.h"
+
+namespace shc {
+
+// out_k = sum_{i+j=k} a_i * b_j (dyadic, NTT form) for `count` pairs of size-2
+// ciphertexts with L limbs; a, b: [count][2][L][N], out: [count][3][L][N].
+void ckks_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t count, uint32_t L,
+                 hipStream_t st);
+
+// Bytes of scratch device memory switch_key needs for `count` ciphertexts at level L.
+size_t keyswitch_workspace(const Context &c, uint32_t count, uint32_t L);
+
+// ct_k += ModDown(sum_j NTT(ModUp(d_j)) * key[j][k]) for k = 0, 1 (src/evaluator.cpp:2310-2653).
+//   target: c2 of each ciphertext, element stride `target_stride` words between ciphertexts
+//           (CKKS: NTT form; BFV: coefficient form), L limbs each.
+//   ct:     c0,c1 of each ciphertext, stride `ct_stride` words between ciphertexts.
+//   key:    [K-1][2][K][N] key-switching key at key level (NTT form).
+void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
+                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st);
+
+// Divide-and-round by the last prime of level L, dropping it (CKKS rescale in NTT form,
+// BFV modulus switch in coefficient form); in: [count][polys][L][N], out: [count][polys][L-1][N].
+size_t rescale_workspace(const Context &c, uint32_t count, uint32_t polys, uint32_t L);
+void rescale_to_next(const Context &c, const uint64_t *in, uint64_t *out, uint32_t count, uint32_t polys, uint32_t L,
+                     void *ws, size_t ws_bytes, hipStream_t st);
+
+}  // namespace shc

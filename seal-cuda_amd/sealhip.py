@@ -40,6 +40,13 @@ def lib():
                                                ctypes.c_uint64, _u64p]
         for name in ("shc_ntt_forward", "shc_ntt_inverse"):
             getattr(L, name).argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp]
+        u32, sz = ctypes.c_uint32, ctypes.c_size_t
+        L.shc_ckks_multiply.argtypes = [_vp, _vp, _vp, _vp, u32, u32, _vp]
+        L.shc_keyswitch_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
+        L.shc_switch_key.argtypes = [_vp, _vp, sz, _vp, sz, _vp, u32, u32, _vp, sz, _vp]
+        L.shc_relinearize.argtypes = [_vp, _vp, _vp, u32, u32, _vp, sz, _vp]
+        L.shc_rescale_workspace_size.argtypes = [_vp, u32, u32, u32, ctypes.POINTER(sz)]
+        L.shc_rescale_to_next.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
         _lib = L
     return _lib
 
@@ -87,3 +94,26 @@ class Context:
 
     def ntt_inverse(self, d_ptr, polys, limbs, stream=None):
         check(lib().shc_ntt_inverse(self.handle, d_ptr, polys, limbs, stream))
+
+    def ckks_multiply(self, d_a, d_b, d_out, count, limbs, stream=None):
+        check(lib().shc_ckks_multiply(self.handle, d_a, d_b, d_out, count, limbs, stream))
+
+    def keyswitch_workspace_size(self, count, limbs):
+        b = ctypes.c_size_t()
+        check(lib().shc_keyswitch_workspace_size(self.handle, count, limbs, ctypes.byref(b)))
+        return b.value
+
+    def switch_key(self, d_target, target_stride, d_ct, ct_stride, d_key, count, limbs, d_ws, ws_bytes, stream=None):
+        check(lib().shc_switch_key(self.handle, d_target, target_stride, d_ct, ct_stride, d_key, count, limbs, d_ws,
+                                   ws_bytes, stream))
+
+    def relinearize(self, d_ct, d_key, count, limbs, d_ws, ws_bytes, stream=None):
+        check(lib().shc_relinearize(self.handle, d_ct, d_key, count, limbs, d_ws, ws_bytes, stream))
+
+    def rescale_workspace_size(self, count, polys, limbs):
+        b = ctypes.c_size_t()
+        check(lib().shc_rescale_workspace_size(self.handle, count, polys, limbs, ctypes.byref(b)))
+        return b.value
+
+    def rescale_to_next(self, d_in, d_out, count, polys, limbs, d_ws, ws_bytes, stream=None):
+        check(lib().shc_rescale_to_next(self.handle, d_in, d_out, count, polys, limbs, d_ws, ws_bytes, stream))

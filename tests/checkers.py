@@ -53,6 +53,11 @@ def oracle():
         lib.or_galois_elt_from_step.argtypes = [ctypes.c_int, ctypes.c_int]
         lib.or_apply_galois.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, u64p, u64p]
         lib.or_apply_galois_ntt.argtypes = [ctypes.c_int, ctypes.c_uint32, u64p, u64p]
+        sz = ctypes.c_size_t
+        lib.or_ckks_multiply.argtypes = [ctypes.c_int, u64p, sz, u64p, sz, u64p, sz, u64p]
+        lib.or_switch_key.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, sz, u64p, u64p, u64p]
+        lib.or_relinearize.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, sz, u64p, u64p]
+        lib.or_rescale_to_next.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, u64p, sz, u64p]
         _oracle = lib
     return _oracle
 
@@ -71,6 +76,13 @@ def ref():
         lib.ref_galois_elt_from_step.argtypes = [ctypes.c_int, ctypes.c_int]
         lib.ref_apply_galois.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, u64p, u64p,
                                          ctypes.c_int]
+        u64 = ctypes.c_uint64
+        lib.ref_session_create.restype = ctypes.c_void_p
+        lib.ref_session_create.argtypes = [ctypes.c_int, u64, u64p, u64, u64]
+        lib.ref_session_destroy.argtypes = [ctypes.c_void_p]
+        lib.ref_multiply.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
+        lib.ref_relinearize.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
+        lib.ref_rescale.argtypes = [ctypes.c_void_p, u64, u64, u64p, u64p]
         _ref = lib
     return _ref
 
@@ -112,6 +124,77 @@ def ntt_rns(data, moduli, logn, inverse=False):
     for i, q in enumerate(moduli):
         v[:, i, :] = ntt(v[:, i, :], q, logn, inverse)
     return a
+
+
+def _arr(moduli):
+    return np.ascontiguousarray(np.array(moduli, dtype=np.uint64))
+
+
+def ckks_multiply(a, b, moduli, logn):
+    """a, b: [2][L][N] -> [3][L][N]"""
+    L = len(moduli)
+    out = np.zeros((3, L, 1 << logn), dtype=np.uint64)
+    m = _arr(moduli)
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    assert oracle().or_ckks_multiply(logn, ptr(m), L, ptr(a), 2, ptr(b), 2, ptr(out)) == 0
+    return out
+
+
+def relinearize(scheme, ct3, key, key_moduli, L, logn):
+    """ct3: [3][L][N] (CKKS NTT form / BFV coefficient form); key [K-1][2][K][N] -> [2][L][N]"""
+    c = np.ascontiguousarray(ct3, dtype=np.uint64).copy()
+    m = _arr(key_moduli)
+    k = np.ascontiguousarray(key, dtype=np.uint64)
+    assert oracle().or_relinearize(scheme, logn, ptr(m), len(key_moduli), L, ptr(c), ptr(k)) == 0
+    return c[:2].copy()
+
+
+def rescale(scheme, ct, moduli, logn):
+    """ct: [polys][L][N] with moduli[:L] -> [polys][L-1][N]"""
+    ct = np.ascontiguousarray(ct, dtype=np.uint64)
+    polys, L, n = ct.shape
+    out = np.zeros((polys, L - 1, n), dtype=np.uint64)
+    m = _arr(moduli[:L])
+    assert oracle().or_rescale_to_next(scheme, logn, ptr(m), L, ptr(ct), polys, ptr(out)) == 0
+    return out
+
+
+class RefSession:
+    """The reference's own Evaluator (oracle/_ref) on injected words."""
+
+    def __init__(self, scheme, n, moduli, plain=0):
+        m = _arr(moduli)
+        self.h = ref().ref_session_create(scheme, n, ptr(m), len(moduli), plain)
+        assert self.h, "reference rejected the parameters"
+        self.n, self.K = n, len(moduli)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref().ref_session_destroy(self.h)
+
+    def multiply(self, a, b, L):
+        out = np.zeros((3, L, self.n), dtype=np.uint64)
+        assert ref().ref_multiply(self.h, L, ptr(np.ascontiguousarray(a)), ptr(np.ascontiguousarray(b)), ptr(out)) == 0
+        return out
+
+    def relinearize(self, ct3, key, L):
+        out = np.zeros((2, L, self.n), dtype=np.uint64)
+        assert ref().ref_relinearize(self.h, L, ptr(np.ascontiguousarray(ct3)), ptr(np.ascontiguousarray(key)),
+                                     ptr(out)) == 0
+        return out
+
+    def rescale(self, ct, L):
+        size = ct.shape[0]
+        out = np.zeros((size, L - 1, self.n), dtype=np.uint64)
+        assert ref().ref_rescale(self.h, L, size, ptr(np.ascontiguousarray(ct)), ptr(out)) == 0
+        return out
+
+
+def random_key(rng, moduli, logn):
+    """Uniform key-switching key words [K-1][2][K][N] at key level (injected keys)."""
+    K = len(moduli)
+    return uniform_residues(rng, (K - 1, 2, K, 1 << logn), moduli)
 
 
 # ------------------------------------------------------------------ seeded inputs

@@ -1,0 +1,128 @@
+"""Pins the oracle's tensor product, key switching and rescale.
+
+Against the reference's own Evaluator (oracle/_ref) on identical injected words, and
+against closed forms that a dropped term or wrong index would break: the tensor
+product equals a schoolbook negacyclic product, rescale equals CRT-compose /
+divide-and-round / decompose with Python integers, and key switching is linear in
+the target and maps a zero key to an unchanged ciphertext.
+"""
+import numpy as np
+import pytest
+
+import checkers as ck
+
+BFV, CKKS = 1, 2
+ref_only = pytest.mark.skipif(ck.ref() is None, reason="reference CPU library oracle/_ref not built here")
+
+
+def params(logn, bits):
+    n = 1 << logn
+    return n, ck.coeff_modulus_create(n, bits)
+
+
+def test_tensor_is_negacyclic_product():
+    logn = 5
+    n, moduli = params(logn, [40, 40, 50])
+    L = 2
+    rng = np.random.default_rng(1)
+    a = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    b = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    out = ck.ckks_multiply(a, b, moduli[:L], logn)
+    coef = lambda x: ck.ntt_rns(x, moduli[:L], logn, inverse=True)
+    A, B, O = coef(a), coef(b), coef(out)
+    for i, q in enumerate(moduli[:L]):
+        def pm(x, y):
+            r = [0] * n
+            for s in range(n):
+                for t in range(n):
+                    v = int(x[s]) * int(y[t])
+                    if s + t >= n:
+                        r[s + t - n] -= v
+                    else:
+                        r[s + t] += v
+            return [v % q for v in r]
+        c1 = [(u + v) % q for u, v in zip(pm(A[0, i], B[1, i]), pm(A[1, i], B[0, i]))]
+        assert [int(x) for x in O[0, i]] == pm(A[0, i], B[0, i])
+        assert [int(x) for x in O[1, i]] == c1
+        assert [int(x) for x in O[2, i]] == pm(A[1, i], B[1, i])
+
+
+@pytest.mark.parametrize("scheme", [BFV, CKKS])
+def test_rescale_is_crt_divide_and_round(scheme):
+    logn = 4
+    n, moduli = params(logn, [30, 30, 40, 50])
+    L = 3
+    rng = np.random.default_rng(2)
+    ct = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    out = ck.rescale(scheme, ct, moduli, logn)
+    coef_in = ck.ntt_rns(ct, moduli[:L], logn, inverse=True) if scheme == CKKS else ct
+    coef_out = ck.ntt_rns(out, moduli[:L - 1], logn, inverse=True) if scheme == CKKS else out
+    Q = 1
+    for q in moduli[:L]:
+        Q *= q
+    qL = moduli[L - 1]
+    for p in range(2):
+        for c in range(n):
+            x = 0
+            for i, q in enumerate(moduli[:L]):
+                Qi = Q // q
+                x += int(coef_in[p, i, c]) * Qi * pow(Qi, -1, q)
+            x %= Q
+            # SEAL rounds as floor((x + floor(qL/2)) / qL) on the representative in [0, Q)
+            y = (x + qL // 2) // qL
+            for i, q in enumerate(moduli[:L - 1]):
+                assert int(coef_out[p, i, c]) == y % q
+
+
+@pytest.mark.parametrize("scheme", [BFV, CKKS])
+def test_switch_key_linearity_and_zero_key(scheme):
+    logn = 5
+    n, moduli = params(logn, [40, 40, 40, 50])
+    K, L = len(moduli), 3
+    rng = np.random.default_rng(3)
+    key = ck.random_key(rng, moduli, logn)
+    base = ck.uniform_residues(rng, (3, L, n), moduli[:L])
+    zero_key = np.zeros_like(key)
+    # zero key: ModDown of zero plus rounding leaves (c0, c1) unchanged
+    assert np.array_equal(ck.relinearize(scheme, base, zero_key, moduli, L, logn), base[:2])
+    # the update is linear in the key up to the ModDown rounding (at most 1 per
+    # coefficient): f(k1) + f(k2) - f(k1 + k2) - (c0, c1) in {-1, 0, 1}
+    key2 = ck.random_key(rng, moduli, logn)
+    ksum = key.copy()
+    for i, q in enumerate(moduli):
+        ksum[:, :, i] = (key[:, :, i] + key2[:, :, i]) % np.uint64(q)
+    fx = ck.relinearize(scheme, base, key, moduli, L, logn)
+    fy = ck.relinearize(scheme, base, key2, moduli, L, logn)
+    fs = ck.relinearize(scheme, base, ksum, moduli, L, logn)
+    assert not np.array_equal(fx, base[:2])
+    conv = (lambda v: ck.ntt_rns(v, moduli[:L], logn, inverse=True)) if scheme == CKKS else (lambda v: v)
+    d = conv(fx).astype(object) + conv(fy).astype(object) - conv(fs).astype(object) - conv(base[:2]).astype(object)
+    for i, q in enumerate(moduli[:L]):
+        r = np.mod(d[:, i], q)
+        assert np.all((r <= 1) | (r >= q - 1)), "key switching is not linear up to rounding"
+
+
+@ref_only
+@pytest.mark.ref
+@pytest.mark.parametrize("scheme,logn,bits,L", [
+    (CKKS, 10, [60, 40, 40, 60], 3),
+    (CKKS, 11, [60, 40, 40, 40, 60], 2),
+    (BFV, 10, [40, 40, 50], 2),
+    (BFV, 12, [43, 43, 44, 44, 44], 4),
+])
+def test_evaluator_matches_reference(scheme, logn, bits, L):
+    n, moduli = params(logn, bits)
+    plain = 65537 if scheme == BFV else 0
+    if scheme == BFV:
+        moduli = ck.coeff_modulus_create(n, bits)
+    sess = ck.RefSession(scheme, n, moduli, plain)
+    rng = np.random.default_rng(logn + L)
+    a = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    b = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    ct3 = ck.uniform_residues(rng, (3, L, n), moduli[:L])
+    key = ck.random_key(rng, moduli, logn)
+    if scheme == CKKS:
+        assert np.array_equal(ck.ckks_multiply(a, b, moduli[:L], logn), sess.multiply(a, b, L))
+    assert np.array_equal(ck.relinearize(scheme, ct3, key, moduli, L, logn), sess.relinearize(ct3, key, L))
+    if L >= 2:
+        assert np.array_equal(ck.rescale(scheme, a, moduli, logn), sess.rescale(a, L))
