@@ -140,11 +140,18 @@ def bench_mulrelin(args):
 
     plain = None
     if not args.no_plain:
+        # the batch must give each ciphertext the same words as the per-ciphertext path
+        probe = [0, B // 2 + 1, B - 1]
+        batched_words = out[probe, :2].clone()
         step_plain()
         torch.cuda.synchronize()
         with Timer(torch) as tp:
             step_plain()
-        plain = {"ops_s": B / (tp.ms * 1e-3), "ms_per_step": tp.ms, "calls": 2 * B}
+        same = bool(torch.equal(out[probe, :2], batched_words))
+        if not same:
+            raise SystemExit("bench self-check failed: batched and per-ciphertext results differ")
+        plain = {"ops_s": B / (tp.ms * 1e-3), "ms_per_step": tp.ms, "calls": 2 * B,
+                 "self_check": f"batched == per-ciphertext words on ciphertexts {probe}"}
 
     # Dominant kernel: the forward NTT over the ModUp limbs (B x L x (L+1) limbs), timed
     # alone on the same stream with the same shape.

@@ -83,6 +83,15 @@ int shc_ntt_inverse(const shc_context *ctx, uint64_t *d_data, uint32_t poly_coun
 int shc_ckks_multiply(const shc_context *ctx, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out,
                       uint32_t count, uint32_t limb_count, void *stream);
 
+int shc_multiply_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limb_count, size_t *bytes);
+
+/* Scheme-dispatched multiply of size-2 ciphertexts: CKKS tensor in NTT form, or BFV
+ * BEHZ multiplication in coefficient form (same layouts as shc_ckks_multiply).
+ * Replaces EvaluatorCuda::multiplyInplace (src/evaluator_cuda.cu:262-281) with
+ * bfvMultiply (src/evaluator_cuda.cu:283-382). */
+int shc_multiply(const shc_context *ctx, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out, uint32_t count,
+                 uint32_t limb_count, void *d_workspace, size_t workspace_bytes, void *stream);
+
 int shc_keyswitch_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limb_count, size_t *bytes);
 
 /* Key switching: for each ciphertext c, (c0, c1) += ModDown(sum_j ModUp(d_j) * key[j]),

@@ -96,6 +96,15 @@ int or_switch_key(int scheme, int logn, const uint64_t *key_moduli, size_t K, si
 /* Evaluator::relinearizeInternal (src/evaluator.cpp:1113-1163): ct3 [3][L][N] -> first 2 polys */
 int or_relinearize(int scheme, int logn, const uint64_t *key_moduli, size_t K, size_t L, uint64_t *ct3,
                    const uint64_t *relin_key);
+/* ---------------------------------------------------------------- BFV (behz.c)
+ * Pinned by: oracle/_ref on injected words (BFV multiply words), decrypt-equality. */
+/* RNSTool bases (src/utils/rns.cpp:587-646): m_sk, gamma and B for base q of size L */
+int or_behz_bases(int logn, const uint64_t *q, size_t L, uint64_t t, uint64_t *m_sk, uint64_t *gamma, uint64_t *B,
+                  size_t *B_size);
+/* Evaluator::bfvMultiply (src/evaluator.cpp:385-624): a, b [2][L][N] coefficient form -> out [3][L][N] */
+int or_bfv_multiply(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *a, const uint64_t *b,
+                    uint64_t *out);
+
 /* Evaluator::modSwitchScaleToNext (src/evaluator.cpp:1165-1240): in [polys][L][N] -> out [polys][L-1][N] */
 int or_rescale_to_next(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *in, size_t polys,
                        uint64_t *out);

@@ -1,0 +1,289 @@
+// behz.hip -- BFV multiplication by the BEHZ method on gfx950.
+//
+// Same arithmetic as Evaluator::bfvMultiply (src/evaluator.cpp:385-624) with the
+// RNSTool steps (src/utils/rns.cpp:849-960); GPU twins src/evaluator_cuda.cu:283-382,
+// src/utils/rns_cuda.cu:300-443.  Per coefficient, each base conversion is a short
+// dot product over the input base: the thread stages the scaled inputs in LDS and
+// accumulates every output prime with 128-bit sums, so intermediate residues
+// never leave the kernel.  Scalar multiplies that precede a conversion in the
+// reference (x m_tilde, x t) are folded into the conversion's first constant.
+#include "behz.h"
+#include "context.h"
+#include "modarith.h"
+#include "ntt.h"
+#include "rns_ops.h"
+
+namespace shc {
+namespace {
+
+constexpr int BT = 128;   // threads per block of the conversion kernels
+constexpr int MAXB = 64;  // LDS rows per thread (>= |q| and >= |B|)
+
+inline uint32_t grid_for(size_t n) {
+    const size_t b = (n + BT - 1) / BT;
+    return (uint32_t)(b < (size_t(1) << 20) ? b : (size_t(1) << 20));
+}
+
+__device__ __forceinline__ uint64_t shoup_pair(uint64_t x, const uint64_t *pair, uint64_t q) {
+    return shoup(x, pair[0], pair[1], q);
+}
+
+// (1)-(2): fastbconvmTilde + smMrq.  x: [P][L][N] base q -> out: [P][bsk][N] base Bsk.
+__global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restrict__ x, uint64_t *__restrict__ out,
+                                                         BehzDev B, const DevPrime *__restrict__ qp, int logn,
+                                                         size_t total) {
+    extern __shared__ uint64_t ys[];  // [rows][BT], rows = max(|q|, |B|)
+    const size_t n = (size_t)1 << logn;
+    const int tid = threadIdx.x;
+    for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
+        const size_t p = idx >> logn, c = idx & (n - 1);
+        for (uint32_t j = 0; j < B.L; j++)
+            ys[j * BT + tid] = shoup_pair(x[(p * B.L + j) * n + c], B.mt_qhat_inv + 2 * j, qp[j].q);
+        // conversion into m_tilde = 2^32: sums wrap modulo 2^64, a multiple of 2^32
+        uint64_t s_mt = 0;
+        const uint64_t *row_mt = B.qhat_mod_bskm + (size_t)B.bsk * B.L;
+        for (uint32_t j = 0; j < B.L; j++) s_mt += ys[j * BT + tid] * row_mt[j];
+        const uint64_t r = ((s_mt & 0xffffffffull) * B.m_tilde_neg_inv_q) & 0xffffffffull;
+        for (uint32_t i = 0; i < B.bsk; i++) {
+            const DevPrime &P = B.bsk_primes[i];
+            const uint64_t *row = B.qhat_mod_bskm + (size_t)i * B.L;
+            u128 acc{0, 0};
+            for (uint32_t j = 0; j < B.L; j++) add128(acc, mul64x64(ys[j * BT + tid], row[j]));
+            const uint64_t s = barrett128(acc, P.q, P.ratio_lo, P.ratio_hi);
+            // centred r (m_tilde is a power of two, so r >= m_tilde/2 is negative)
+            const uint64_t rc = r >= 0x80000000ull ? r + P.q - 0x100000000ull : r;
+            u128 v = mul64x64(rc, B.q_mod_bsk[i]);
+            add128(v, u128{s, 0});
+            const uint64_t w = barrett128(v, P.q, P.ratio_lo, P.ratio_hi);
+            out[(p * B.bsk + i) * n + c] = shoup_pair(w, B.inv_mt_bsk + 2 * i, P.q);
+        }
+    }
+}
+
+// (6)-(7): multiply by t and fastFloor.  Tq: [P][L][N], Tb: [P][bsk][N] -> Z: [P][bsk][N].
+__global__ __launch_bounds__(BT) void behz_floor_kernel(const uint64_t *__restrict__ Tq, const uint64_t *__restrict__ Tb,
+                                                        uint64_t *__restrict__ Z, BehzDev B,
+                                                        const DevPrime *__restrict__ qp, int logn, size_t total) {
+    extern __shared__ uint64_t ws[];  // [rows][BT], rows = max(|q|, |B|)
+    const size_t n = (size_t)1 << logn;
+    const int tid = threadIdx.x;
+    for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
+        const size_t p = idx >> logn, c = idx & (n - 1);
+        for (uint32_t j = 0; j < B.L; j++)
+            ws[j * BT + tid] = shoup_pair(Tq[(p * B.L + j) * n + c], B.t_qhat_inv + 2 * j, qp[j].q);
+        for (uint32_t i = 0; i < B.bsk; i++) {
+            const DevPrime &P = B.bsk_primes[i];
+            const uint64_t *row = B.qhat_mod_bskm + (size_t)i * B.L;
+            u128 acc{0, 0};
+            for (uint32_t j = 0; j < B.L; j++) add128(acc, mul64x64(ws[j * BT + tid], row[j]));
+            const uint64_t conv = barrett128(acc, P.q, P.ratio_lo, P.ratio_hi);
+            const uint64_t tb = shoup_pair(Tb[(p * B.bsk + i) * n + c], B.t_bsk + 2 * i, P.q);
+            Z[(p * B.bsk + i) * n + c] = shoup_pair(tb + P.q - conv, B.inv_q_bsk + 2 * i, P.q);
+        }
+    }
+}
+
+// (8): fastbconvSk.  Z: [P][bsk][N] -> out: [P][L][N] base q.
+__global__ __launch_bounds__(BT) void behz_sk_kernel(const uint64_t *__restrict__ Z, uint64_t *__restrict__ out,
+                                                     BehzDev B, const DevPrime *__restrict__ qp, int logn,
+                                                     size_t total) {
+    extern __shared__ uint64_t zs[];  // [rows][BT], rows = max(|q|, |B|)
+    const size_t n = (size_t)1 << logn;
+    const int tid = threadIdx.x;
+    const DevPrime &Msk = B.bsk_primes[B.bsz];
+    for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
+        const size_t p = idx >> logn, c = idx & (n - 1);
+        for (uint32_t j = 0; j < B.bsz; j++)
+            zs[j * BT + tid] = shoup_pair(Z[(p * B.bsk + j) * n + c], B.bhat_inv + 2 * j, B.bsk_primes[j].q);
+        u128 acc{0, 0};
+        for (uint32_t j = 0; j < B.bsz; j++) add128(acc, mul64x64(zs[j * BT + tid], B.bhat_mod_msk[j]));
+        const uint64_t conv = barrett128(acc, Msk.q, Msk.ratio_lo, Msk.ratio_hi);
+        const uint64_t zmsk = Z[(p * B.bsk + B.bsz) * n + c];
+        const uint64_t alpha = shoup(conv + B.m_sk - zmsk, B.inv_prod_B_msk, B.inv_prod_B_msk_shoup, B.m_sk);
+        const bool neg = alpha > B.m_sk_half;
+        for (uint32_t i = 0; i < B.L; i++) {
+            const DevPrime &Q = qp[i];
+            const uint64_t *row = B.bhat_mod_q + (size_t)i * B.bsz;
+            u128 s{0, 0};
+            for (uint32_t j = 0; j < B.bsz; j++) add128(s, mul64x64(zs[j * BT + tid], row[j]));
+            const uint64_t o = barrett128(s, Q.q, Q.ratio_lo, Q.ratio_hi);
+            // Shenoy-Kumaresan correction with the centred alpha_sk
+            const uint64_t pb = B.prod_B_mod_q[i];
+            u128 v = neg ? mul64x64(B.m_sk - alpha, pb) : mul64x64(alpha, Q.q - pb);
+            add128(v, u128{o, 0});
+            out[(p * B.L + i) * n + c] = barrett128(v, Q.q, Q.ratio_lo, Q.ratio_hi);
+        }
+    }
+}
+
+struct BfvLayout {
+    size_t aq, bq, ab, bb, tq, tb, z, total;
+};
+
+BfvLayout bfv_layout(const Context &c, uint32_t count, uint32_t L, uint32_t bsk) {
+    const size_t limb = c.n * 8;
+    BfvLayout l;
+    size_t off = 0;
+    auto take = [&](size_t limbs) { size_t o = off; off += (limbs * limb + 255) & ~size_t(255); return o; };
+    l.aq = take((size_t)count * 2 * L);
+    l.bq = take((size_t)count * 2 * L);
+    l.ab = take((size_t)count * 2 * bsk);
+    l.bb = take((size_t)count * 2 * bsk);
+    l.tq = take((size_t)count * 3 * L);
+    l.tb = take((size_t)count * 3 * bsk);
+    l.z = take((size_t)count * 3 * bsk);
+    l.total = off;
+    return l;
+}
+
+}  // namespace
+
+void build_behz_level(Context &c, uint32_t L, BehzLevel &out) {
+    using namespace hm;
+    const std::vector<uint64_t> q(c.moduli.begin(), c.moduli.begin() + L);
+    const uint64_t t = c.plain;
+    if (L > MAXB - 2) fail(ERR_INVALID_ARGUMENT, "too many primes for BEHZ tables");
+    // RNSTool::initialize (src/utils/rns.cpp:587-646)
+    size_t bsz = L;
+    if (32 + bit_count(t) + big_bit_count(big_product(q)) >= 61 * (int)L + 61) bsz++;
+    const std::vector<uint64_t> pr = get_primes(2 * c.n, 61, bsz + 2);
+    const uint64_t msk = pr[0];
+    out.gamma = pr[1];
+    std::vector<uint64_t> Bv(pr.begin() + 2, pr.end());
+    std::vector<uint64_t> Bsk = Bv;
+    Bsk.push_back(msk);
+    out.bsk_moduli = Bsk;
+    const uint64_t mt = uint64_t(1) << 32;
+    const uint32_t bsk = (uint32_t)Bsk.size();
+
+    auto prod_mod = [](const std::vector<uint64_t> &base, size_t skip, uint64_t p) {
+        uint64_t r = 1 % p;
+        for (size_t k = 0; k < base.size(); k++)
+            if (k != skip) r = mulmod(r, base[k] % p, p);
+        return r;
+    };
+    const size_t NONE = (size_t)-1;
+    std::vector<uint64_t> words;
+    auto put = [&](std::vector<uint64_t> v) { size_t o = words.size(); words.insert(words.end(), v.begin(), v.end()); return o; };
+    auto shoup_vec = [](const std::vector<uint64_t> &w, const std::vector<uint64_t> &mods) {
+        std::vector<uint64_t> r;
+        for (size_t i = 0; i < w.size(); i++) { r.push_back(w[i]); r.push_back(shoup(w[i], mods[i])); }
+        return r;
+    };
+    std::vector<uint64_t> qhat_inv(L), v;
+    for (uint32_t j = 0; j < L; j++) qhat_inv[j] = invmod(prod_mod(q, j, q[j]), q[j]);
+    v.clear();
+    for (uint32_t j = 0; j < L; j++) v.push_back(mulmod(mt % q[j], qhat_inv[j], q[j]));
+    const size_t o_mt_qhat_inv = put(shoup_vec(v, q));
+    v.clear();
+    for (uint32_t i = 0; i < bsk; i++)
+        for (uint32_t j = 0; j < L; j++) v.push_back(prod_mod(q, j, Bsk[i]));
+    for (uint32_t j = 0; j < L; j++) v.push_back(prod_mod(q, j, mt));
+    const size_t o_qhat = put(v);
+    v.clear();
+    for (uint32_t i = 0; i < bsk; i++) v.push_back(prod_mod(q, NONE, Bsk[i]));
+    const size_t o_q_bsk = put(v);
+    v.clear();
+    for (uint32_t i = 0; i < bsk; i++) v.push_back(invmod(mt % Bsk[i], Bsk[i]));
+    const size_t o_inv_mt = put(shoup_vec(v, Bsk));
+    v.clear();
+    for (uint32_t j = 0; j < L; j++) v.push_back(mulmod(t % q[j], qhat_inv[j], q[j]));
+    const size_t o_t_qhat_inv = put(shoup_vec(v, q));
+    v.clear();
+    for (uint32_t i = 0; i < bsk; i++) v.push_back(t % Bsk[i]);
+    const size_t o_t_bsk = put(shoup_vec(v, Bsk));
+    v.clear();
+    for (uint32_t i = 0; i < bsk; i++) v.push_back(invmod(prod_mod(q, NONE, Bsk[i]), Bsk[i]));
+    const size_t o_inv_q_bsk = put(shoup_vec(v, Bsk));
+    v.clear();
+    for (size_t j = 0; j < bsz; j++) v.push_back(invmod(prod_mod(Bv, j, Bv[j]), Bv[j]));
+    const size_t o_bhat_inv = put(shoup_vec(v, Bv));
+    v.clear();
+    for (uint32_t i = 0; i < L; i++)
+        for (size_t j = 0; j < bsz; j++) v.push_back(prod_mod(Bv, j, q[i]));
+    const size_t o_bhat_q = put(v);
+    v.clear();
+    for (size_t j = 0; j < bsz; j++) v.push_back(prod_mod(Bv, j, msk));
+    const size_t o_bhat_msk = put(v);
+    v.clear();
+    for (uint32_t i = 0; i < L; i++) v.push_back(prod_mod(Bv, NONE, q[i]));
+    const size_t o_prodB_q = put(v);
+
+    out.tables = DevBuf(words.size() * 8);
+    SHC_HIP(hipMemcpy(out.tables.ptr, words.data(), words.size() * 8, hipMemcpyHostToDevice));
+    std::vector<DevPrime> bp;
+    for (uint64_t b : Bsk) bp.push_back(c.aux_prime(b));
+    out.bsk_primes = DevBuf(bp.size() * sizeof(DevPrime));
+    SHC_HIP(hipMemcpy(out.bsk_primes.ptr, bp.data(), out.bsk_primes.bytes, hipMemcpyHostToDevice));
+
+    const uint64_t *base = out.tables.as<const uint64_t>();
+    BehzDev &d = out.dev;
+    d.L = L;
+    d.bsz = (uint32_t)bsz;
+    d.bsk = bsk;
+    d.m_sk = msk;
+    d.m_sk_half = msk >> 1;
+    const uint64_t inv_q_mt = invmod(prod_mod(q, NONE, mt), mt);
+    d.m_tilde_neg_inv_q = (mt - inv_q_mt) % mt;
+    d.inv_prod_B_msk = invmod(prod_mod(Bv, NONE, msk), msk);
+    d.inv_prod_B_msk_shoup = shoup(d.inv_prod_B_msk, msk);
+    d.t = t;
+    d.bsk_primes = out.bsk_primes.as<const DevPrime>();
+    d.mt_qhat_inv = base + o_mt_qhat_inv;
+    d.qhat_mod_bskm = base + o_qhat;
+    d.q_mod_bsk = base + o_q_bsk;
+    d.inv_mt_bsk = base + o_inv_mt;
+    d.t_qhat_inv = base + o_t_qhat_inv;
+    d.t_bsk = base + o_t_bsk;
+    d.inv_q_bsk = base + o_inv_q_bsk;
+    d.bhat_inv = base + o_bhat_inv;
+    d.bhat_mod_q = base + o_bhat_q;
+    d.bhat_mod_msk = base + o_bhat_msk;
+    d.prod_B_mod_q = base + o_prodB_q;
+}
+
+size_t bfv_multiply_workspace(const Context &c, uint32_t count, uint32_t L) {
+    return bfv_layout(c, count, L, c.behz(L).dev.bsk).total;
+}
+
+void bfv_multiply(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t count, uint32_t L,
+                  void *ws, size_t ws_bytes, hipStream_t st) {
+    if (c.scheme != SCHEME_BFV) fail(ERR_LOGIC, "BFV multiplication needs a BFV context");
+    if (!count) return;
+    const BehzLevel &lv = c.behz(L);
+    const BehzDev &B = lv.dev;
+    const uint32_t bsk = B.bsk;
+    const BfvLayout lay = bfv_layout(c, count, L, bsk);
+    if (!ws || ws_bytes < lay.total) fail(ERR_INVALID_ARGUMENT, "workspace too small for BFV multiplication");
+    char *w = static_cast<char *>(ws);
+    auto at = [&](size_t off) { return reinterpret_cast<uint64_t *>(w + off); };
+    uint64_t *Aq = at(lay.aq), *Bq = at(lay.bq), *Ab = at(lay.ab), *Bb = at(lay.bb);
+    uint64_t *Tq = at(lay.tq), *Tb = at(lay.tb), *Z = at(lay.z);
+    const int logn = c.logn;
+    const size_t n = c.n;
+    const DevPrime *qp = c.primes();
+    const DevPrime *bp = B.bsk_primes;
+    const size_t qwords = (size_t)count * 2 * L * n;
+    SHC_HIP(hipMemcpyAsync(Aq, a, qwords * 8, hipMemcpyDeviceToDevice, st));
+    SHC_HIP(hipMemcpyAsync(Bq, b, qwords * 8, hipMemcpyDeviceToDevice, st));
+    const size_t ext = (size_t)count * 2 * n;
+    const size_t lds = (size_t)(L > B.bsz ? L : B.bsz) * BT * 8;
+    behz_extend_kernel<<<grid_for(ext), BT, lds, st>>>(a, Ab, B, qp, logn, ext);
+    behz_extend_kernel<<<grid_for(ext), BT, lds, st>>>(b, Bb, B, qp, logn, ext);
+    launch_check("behz_extend");
+    ntt_forward(Aq, count * 2 * L, L, qp, logn, NTT_FULL, st);
+    ntt_forward(Bq, count * 2 * L, L, qp, logn, NTT_FULL, st);
+    ntt_forward(Ab, count * 2 * bsk, bsk, bp, logn, NTT_FULL, st);
+    ntt_forward(Bb, count * 2 * bsk, bsk, bp, logn, NTT_FULL, st);
+    tensor_strided(qp, Aq, 2 * L * n, Bq, 2 * L * n, Tq, 3 * L * n, count, L, logn, st);
+    tensor_strided(bp, Ab, 2 * bsk * n, Bb, 2 * bsk * n, Tb, 3 * bsk * n, count, bsk, logn, st);
+    ntt_inverse(Tq, count * 3 * L, L, qp, logn, NTT_FULL, st);
+    ntt_inverse(Tb, count * 3 * bsk, bsk, bp, logn, NTT_FULL, st);
+    const size_t fl = (size_t)count * 3 * n;
+    behz_floor_kernel<<<grid_for(fl), BT, lds, st>>>(Tq, Tb, Z, B, qp, logn, fl);
+    launch_check("behz_floor");
+    behz_sk_kernel<<<grid_for(fl), BT, lds, st>>>(Z, out, B, qp, logn, fl);
+    launch_check("behz_sk");
+}
+
+}  // namespace shc

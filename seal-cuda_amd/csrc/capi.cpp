@@ -101,6 +101,28 @@ int shc_ckks_multiply(const shc_context *ctx, const uint64_t *a, const uint64_t 
     });
 }
 
+int shc_multiply_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = c.scheme == shc::SCHEME_BFV ? shc::bfv_multiply_workspace(c, count, limbs) : 0;
+    });
+}
+
+int shc_multiply(const shc_context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t count,
+                 uint32_t limbs, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        if (c.scheme == shc::SCHEME_BFV)
+            shc::bfv_multiply(c, a, b, out, count, limbs, ws, ws_bytes, (hipStream_t)stream);
+        else if (c.scheme == shc::SCHEME_CKKS)
+            shc::ckks_tensor(c, a, b, out, count, limbs, (hipStream_t)stream);
+        else
+            shc::fail(shc::ERR_INVALID_ARGUMENT, "unsupported scheme");
+    });
+}
+
 int shc_keyswitch_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limbs, size_t *bytes) {
     return guarded([&] {
         const auto &c = ctx_of(ctx);

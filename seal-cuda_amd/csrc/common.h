@@ -48,4 +48,34 @@ inline void hip_check(hipError_t e, const char *what) {
 
 inline void launch_check(const char *what) { hip_check(hipGetLastError(), what); }
 
+// RAII device buffer (hipMalloc / hipFree).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b) : bytes(b) {
+        if (b) SHC_HIP(hipMalloc(&ptr, b));
+    }
+    ~DevBuf() {
+        if (ptr) (void)hipFree(ptr);
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            if (ptr) (void)hipFree(ptr);
+            ptr = o.ptr;
+            bytes = o.bytes;
+            o.ptr = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
 }  // namespace shc

@@ -4,15 +4,8 @@
 
 namespace shc {
 
-DevBuf::DevBuf(size_t b) : bytes(b) {
-    if (b) SHC_HIP(hipMalloc(&ptr, b));
-}
-
-DevBuf::~DevBuf() {
-    if (ptr) (void)hipFree(ptr);
-}
-
-static DevPrime upload_prime(Context &c, const hm::NttTables &t) {
+DevPrime upload_prime(Context &c, uint64_t q) {
+    const hm::NttTables t = hm::make_ntt_tables(c.logn, q);
     DevPrime p{};
     p.q = t.q;
     p.two_q = t.q << 1;
@@ -29,6 +22,27 @@ static DevPrime upload_prime(Context &c, const hm::NttTables &t) {
     return p;
 }
 
+DevPrime Context::aux_prime(uint64_t q) {
+    auto it = aux.find(q);
+    if (it != aux.end()) return it->second;
+    for (size_t i = 0; i < moduli.size(); i++)
+        if (moduli[i] == q) return aux[q] = host_primes[i];
+    return aux[q] = upload_prime(*this, q);
+}
+
+const BehzLevel &Context::behz(size_t level) const {
+    if (level >= behz_levels.size() || !behz_levels[level])
+        fail(ERR_LOGIC, "BEHZ tables are only built for BFV data levels");
+    return *behz_levels[level];
+}
+
+template <class T>
+static DevBuf upload(const std::vector<T> &v) {
+    DevBuf b(v.size() * sizeof(T));
+    if (!v.empty()) SHC_HIP(hipMemcpy(b.ptr, v.data(), b.bytes, hipMemcpyHostToDevice));
+    return b;
+}
+
 std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector<uint64_t> &moduli, uint64_t plain) {
     if (logn < 1 || logn > 17) fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree must be a power of two in [2, 131072]");
     if (moduli.empty() || moduli.size() > 64) fail(ERR_INVALID_ARGUMENT, "coeff_modulus size is invalid");
@@ -37,6 +51,7 @@ std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector
         for (size_t j = 0; j < i; j++)
             if (moduli[i] == moduli[j]) fail(ERR_INVALID_ARGUMENT, "coeff_modulus must be pairwise coprime");
     }
+    if (scheme == SCHEME_BFV && (plain < 2 || plain >> 60)) fail(ERR_INVALID_ARGUMENT, "plain_modulus is invalid");
     auto c = std::make_unique<Context>();
     SHC_HIP(hipGetDevice(&c->device));
     c->scheme = scheme;
@@ -44,13 +59,10 @@ std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector
     c->n = uint64_t(1) << logn;
     c->moduli = moduli;
     c->plain = plain;
-    for (uint64_t q : moduli) c->host_tables.push_back(hm::make_ntt_tables(logn, q));
-    for (auto &t : c->host_tables) c->host_primes.push_back(upload_prime(*c, t));
+    for (uint64_t q : moduli) c->host_primes.push_back(upload_prime(*c, q));
 
     const size_t K = moduli.size();
-    c->d_primes = DevBuf(c->host_primes.size() * sizeof(DevPrime));
-    SHC_HIP(hipMemcpy(c->d_primes.ptr, c->host_primes.data(), c->host_primes.size() * sizeof(DevPrime),
-                      hipMemcpyHostToDevice));
+    c->d_primes = upload(c->host_primes);
     if (K >= 2) {
         std::vector<DevPrime> ks((K - 1) * K);
         for (size_t L = 1; L < K; L++) {
@@ -58,8 +70,7 @@ std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector
             for (size_t i = 0; i < L; i++) row[i] = c->host_primes[i];
             row[L] = c->host_primes[K - 1];
         }
-        c->d_keyswitch_primes = DevBuf(ks.size() * sizeof(DevPrime));
-        SHC_HIP(hipMemcpy(c->d_keyswitch_primes.ptr, ks.data(), ks.size() * sizeof(DevPrime), hipMemcpyHostToDevice));
+        c->d_keyswitch_primes = upload(ks);
     }
     // divide-and-round constants: RNSTool::inv_q_last_mod_q_ (src/utils/rns.cpp:778-790)
     // for every level, and the special prime's for key switching
@@ -75,10 +86,26 @@ std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector
     for (size_t i = 0; i + 1 < K; i++) ksc[i] = divconst(moduli[K - 1], moduli[i]);
     for (size_t L = 2; L <= K; L++)
         for (size_t i = 0; i + 1 < L; i++) rsc[L * K + i] = divconst(moduli[L - 1], moduli[i]);
-    c->d_ks_consts = DevBuf(ksc.size() * sizeof(DivRoundConst));
-    SHC_HIP(hipMemcpy(c->d_ks_consts.ptr, ksc.data(), ksc.size() * sizeof(DivRoundConst), hipMemcpyHostToDevice));
-    c->d_rescale_consts = DevBuf(rsc.size() * sizeof(DivRoundConst));
-    SHC_HIP(hipMemcpy(c->d_rescale_consts.ptr, rsc.data(), rsc.size() * sizeof(DivRoundConst), hipMemcpyHostToDevice));
+    c->d_ks_consts = upload(ksc);
+    c->d_rescale_consts = upload(rsc);
+
+    if (scheme == SCHEME_BFV) {
+        for (uint64_t q : moduli)
+            if (hm::invmod(plain % q, q) == 0) fail(ERR_INVALID_ARGUMENT, "plain_modulus must be coprime to coeff_modulus");
+        // BEHZ tables for every data level (the first data level drops the special prime)
+        const size_t top = K > 1 ? K - 1 : 1;
+        c->behz_levels.resize(top + 1);
+        for (size_t L = 1; L <= top; L++) {
+            c->behz_levels[L] = std::make_unique<BehzLevel>();
+            build_behz_level(*c, (uint32_t)L, *c->behz_levels[L]);
+        }
+        // batching needs t = 1 mod 2N (plain NTT tables, src/context.cpp:246-254)
+        uint64_t root;
+        if (hm::is_prime(plain) && hm::minimal_primitive_root(2 * c->n, plain, root)) {
+            std::vector<DevPrime> tp{c->aux_prime(plain)};
+            c->d_plain_prime = upload(tp);
+        }
+    }
     return c;
 }
 

@@ -212,7 +212,8 @@ void launch_ntt(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *pr
 }
 
 template <bool INV, bool FULL>
-void dispatch(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn, hipStream_t st) {
+void dispatch_chunk(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn,
+                    hipStream_t st) {
     switch (logn) {
 #define SHC_NTT_CASE(L) \
     case L: launch_ntt<L, INV, FULL>(data, count, lpp, primes, st); break;
@@ -223,6 +224,20 @@ void dispatch(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *prim
         default: fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree must be 2^1 .. 2^17");
     }
     launch_check("ntt launch");
+}
+
+// Splits very large batches so one launch stays well below 2^31 work-items (every
+// pass runs N/16 work-items per limb); chunks hold whole polynomials so that limb k
+// keeps prime k % lpp.
+template <bool INV, bool FULL>
+void dispatch(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn, hipStream_t st) {
+    uint32_t max_limbs = (uint32_t)((uint64_t(1) << 34) >> logn);
+    max_limbs = max_limbs < lpp ? lpp : max_limbs - max_limbs % lpp;
+    for (uint32_t done = 0; done < count;) {
+        const uint32_t c = count - done < max_limbs ? count - done : max_limbs;
+        dispatch_chunk<INV, FULL>(data + ((size_t)done << logn), c, lpp, primes, logn, st);
+        done += c;
+    }
 }
 
 }  // namespace

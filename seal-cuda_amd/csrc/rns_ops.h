@@ -10,6 +10,11 @@ namespace shc {
 void ckks_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t count, uint32_t L,
                  hipStream_t st);
 
+// The same tensor product with explicit per-ciphertext strides (in words) and an
+// arbitrary prime list (limb l uses primes[l]).
+void tensor_strided(const DevPrime *primes, const uint64_t *a, size_t a_stride, const uint64_t *b, size_t b_stride,
+                    uint64_t *out, size_t out_stride, uint32_t count, uint32_t L, int logn, hipStream_t st);
+
 // Bytes of scratch device memory switch_key needs for `count` ciphertexts at level L.
 size_t keyswitch_workspace(const Context &c, uint32_t count, uint32_t L);
 

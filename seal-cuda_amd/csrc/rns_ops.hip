@@ -21,32 +21,36 @@ namespace {
 
 constexpr int TPB = 256;
 
-inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + TPB - 1) / TPB); }
+// Elementwise kernels run grid-stride loops; the grid is capped so that a launch stays
+// far below the 2^32 work-item limit however large the batch is.
+inline uint32_t blocks_for(size_t n) {
+    const size_t b = (n + TPB - 1) / TPB;
+    return (uint32_t)(b < (size_t(1) << 20) ? b : (size_t(1) << 20));
+}
 
-__global__ void tensor_kernel(const uint64_t *__restrict__ a, const uint64_t *__restrict__ b,
-                              uint64_t *__restrict__ out, const DevPrime *__restrict__ primes, uint32_t L, int logn,
-                              size_t total) {
-    size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (idx >= total) return;
+__global__ void tensor_kernel(const uint64_t *__restrict__ a, size_t a_stride, const uint64_t *__restrict__ b,
+                              size_t b_stride, uint64_t *__restrict__ out, size_t out_stride,
+                              const DevPrime *__restrict__ primes, uint32_t L, int logn, size_t total) {
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t limbwords = (size_t)L << logn;
     const size_t ct = idx / limbwords, rem = idx % limbwords;
     const DevPrime &P = primes[rem >> logn];
-    const uint64_t *pa = a + ct * 2 * limbwords + rem, *pb = b + ct * 2 * limbwords + rem;
+    const uint64_t *pa = a + ct * a_stride + rem, *pb = b + ct * b_stride + rem;
     const uint64_t a0 = pa[0], a1 = pa[limbwords], b0 = pb[0], b1 = pb[limbwords];
-    uint64_t *po = out + ct * 3 * limbwords + rem;
+    uint64_t *po = out + ct * out_stride + rem;
     po[0] = mulmod(a0, b0, P.q, P.ratio_lo, P.ratio_hi);
     u128 cross = mul64x64(a0, b1);
     add128(cross, mul64x64(a1, b0));
     po[limbwords] = barrett128(cross, P.q, P.ratio_lo, P.ratio_hi);
     po[2 * limbwords] = mulmod(a1, b1, P.q, P.ratio_lo, P.ratio_hi);
+    }
 }
 
 // T[ct][j][i][c] = d[ct][j][c] mod q_i, i in [0, L] (target i == L is the special prime).
 __global__ void modup_gather_kernel(const uint64_t *__restrict__ d, size_t d_stride, uint64_t *__restrict__ T,
                                     const DevPrime *__restrict__ targets, const DevPrime *__restrict__ digits,
                                     uint32_t L, int logn, size_t total) {
-    size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (idx >= total) return;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t n = (size_t)1 << logn;
     const size_t c = idx & (n - 1);
     size_t r = idx >> logn;
@@ -57,6 +61,7 @@ __global__ void modup_gather_kernel(const uint64_t *__restrict__ d, size_t d_str
     const uint64_t v = d[ct * d_stride + ((size_t)j << logn) + c];
     const DevPrime &Q = targets[i];
     T[idx] = digits[j].q <= Q.q ? v : barrett64(v, Q.q, Q.ratio_hi);
+    }
 }
 
 // P[ct][k][i][c] = sum_j T[ct][j][i][c] * key[j][k][ki][c] mod q_i for i < L, and the
@@ -69,9 +74,9 @@ __global__ void inner_product_kernel(const uint64_t *__restrict__ T, const uint6
                                      uint32_t count) {
     const size_t n = (size_t)1 << logn;
     const size_t per_group = (size_t)(L + 1) << logn;
-    size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     const size_t groups = (count + G - 1) / G;
-    if (idx >= groups * per_group) return;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < groups * per_group;
+         idx += (size_t)gridDim.x * TPB) {
     const size_t grp = idx / per_group;
     const size_t rem = idx % per_group;
     const uint32_t i = (uint32_t)(rem >> logn);
@@ -111,6 +116,7 @@ __global__ void inner_product_kernel(const uint64_t *__restrict__ T, const uint6
             }
         }
     }
+    }
 }
 
 // U[g][i][c] = ([S[g][c] + floor(p/2)]_p mod q_i - floor(p/2) mod q_i) mod q_i
@@ -118,8 +124,7 @@ __global__ void inner_product_kernel(const uint64_t *__restrict__ T, const uint6
 __global__ void divround_prep_kernel(const uint64_t *__restrict__ S, uint64_t *__restrict__ U,
                                      const DevPrime *__restrict__ primes, const DivRoundConst *__restrict__ dc,
                                      uint64_t p, uint32_t Lout, int logn, size_t total) {
-    size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (idx >= total) return;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t n = (size_t)1 << logn;
     const size_t c = idx & (n - 1);
     const size_t r = idx >> logn;
@@ -130,14 +135,14 @@ __global__ void divround_prep_kernel(const uint64_t *__restrict__ S, uint64_t *_
     v = v >= p ? v - p : v;
     v = barrett64(v, Q.q, Q.ratio_hi);
     U[idx] = submod(v, dc[i].half_mod, Q.q);
+    }
 }
 
 // ct[ct][k][i] += (P[ct][k][i] - U[ct][k][i]) * p^-1 mod q_i
 __global__ void moddown_finish_kernel(const uint64_t *__restrict__ P, const uint64_t *__restrict__ U,
                                       uint64_t *__restrict__ ct, size_t ct_stride, const DevPrime *__restrict__ primes,
                                       const DivRoundConst *__restrict__ dc, uint32_t L, int logn, size_t total) {
-    size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (idx >= total) return;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t per_ct = (size_t)2 * L << logn;
     const size_t cti = idx / per_ct, rem = idx % per_ct;
     const uint32_t i = (uint32_t)((rem >> logn) % L);
@@ -146,14 +151,14 @@ __global__ void moddown_finish_kernel(const uint64_t *__restrict__ P, const uint
     const uint64_t v = shoup(x, dc[i].inv, dc[i].inv_shoup, Q.q);
     uint64_t *dst = ct + cti * ct_stride + rem;
     *dst = addmod(*dst, v, Q.q);
+    }
 }
 
 // out[g][i] = (in[g][i] - U[g][i]) * q_last^-1 mod q_i, in has Lin = Lout + 1 limbs per group.
 __global__ void rescale_finish_kernel(const uint64_t *__restrict__ in, const uint64_t *__restrict__ U,
                                       uint64_t *__restrict__ out, const DevPrime *__restrict__ primes,
                                       const DivRoundConst *__restrict__ dc, uint32_t Lout, int logn, size_t total) {
-    size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (idx >= total) return;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t n = (size_t)1 << logn;
     const size_t c = idx & (n - 1);
     const size_t r = idx >> logn;
@@ -162,25 +167,26 @@ __global__ void rescale_finish_kernel(const uint64_t *__restrict__ in, const uin
     const DevPrime &Q = primes[i];
     const uint64_t x = submod(in[(g * (Lout + 1) + i) * n + c], U[idx], Q.q);
     out[idx] = shoup(x, dc[i].inv, dc[i].inv_shoup, Q.q);
+    }
 }
 
 // S[g][c] = in[g][Lin-1][c]: gathers the last limb of every polynomial.
 __global__ void last_limb_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ S, uint32_t Lin, int logn,
                                  size_t total) {
-    size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (idx >= total) return;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t n = (size_t)1 << logn;
     const size_t g = idx >> logn, c = idx & (n - 1);
     S[idx] = in[(g * Lin + Lin - 1) * n + c];
+    }
 }
 
 // dst[g][limbs][N] = src[g * src_stride ...] (strided gather of c2 polynomials)
 __global__ void gather_polys_kernel(const uint64_t *__restrict__ src, size_t src_stride, uint64_t *__restrict__ dst,
                                     size_t poly_words, size_t total) {
-    size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (idx >= total) return;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t g = idx / poly_words, r = idx % poly_words;
     dst[idx] = src[g * src_stride + r];
+    }
 }
 
 struct KsLayout {
@@ -205,9 +211,16 @@ KsLayout ks_layout(const Context &c, uint32_t count, uint32_t L) {
 
 void ckks_tensor(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t count, uint32_t L,
                  hipStream_t st) {
-    const size_t total = (size_t)count * L * c.n;
+    const size_t lw = (size_t)L * c.n;
+    tensor_strided(c.primes(), a, 2 * lw, b, 2 * lw, out, 3 * lw, count, L, c.logn, st);
+}
+
+void tensor_strided(const DevPrime *primes, const uint64_t *a, size_t a_stride, const uint64_t *b, size_t b_stride,
+                    uint64_t *out, size_t out_stride, uint32_t count, uint32_t L, int logn, hipStream_t st) {
+    const size_t total = ((size_t)count * L) << logn;
     if (!total) return;
-    tensor_kernel<<<blocks_for(total), TPB, 0, st>>>(a, b, out, c.primes(), L, c.logn, total);
+    tensor_kernel<<<blocks_for(total), TPB, 0, st>>>(a, a_stride, b, b_stride, out, out_stride, primes, L, logn,
+                                                     total);
     launch_check("tensor");
 }
 

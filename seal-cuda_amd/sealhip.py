@@ -42,6 +42,8 @@ def lib():
             getattr(L, name).argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp]
         u32, sz = ctypes.c_uint32, ctypes.c_size_t
         L.shc_ckks_multiply.argtypes = [_vp, _vp, _vp, _vp, u32, u32, _vp]
+        L.shc_multiply_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
+        L.shc_multiply.argtypes = [_vp, _vp, _vp, _vp, u32, u32, _vp, sz, _vp]
         L.shc_keyswitch_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
         L.shc_switch_key.argtypes = [_vp, _vp, sz, _vp, sz, _vp, u32, u32, _vp, sz, _vp]
         L.shc_relinearize.argtypes = [_vp, _vp, _vp, u32, u32, _vp, sz, _vp]
@@ -97,6 +99,14 @@ class Context:
 
     def ckks_multiply(self, d_a, d_b, d_out, count, limbs, stream=None):
         check(lib().shc_ckks_multiply(self.handle, d_a, d_b, d_out, count, limbs, stream))
+
+    def multiply_workspace_size(self, count, limbs):
+        b = ctypes.c_size_t()
+        check(lib().shc_multiply_workspace_size(self.handle, count, limbs, ctypes.byref(b)))
+        return b.value
+
+    def multiply(self, d_a, d_b, d_out, count, limbs, d_ws, ws_bytes, stream=None):
+        check(lib().shc_multiply(self.handle, d_a, d_b, d_out, count, limbs, d_ws, ws_bytes, stream))
 
     def keyswitch_workspace_size(self, count, limbs):
         b = ctypes.c_size_t()

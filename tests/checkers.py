@@ -58,6 +58,9 @@ def oracle():
         lib.or_switch_key.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, sz, u64p, u64p, u64p]
         lib.or_relinearize.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, sz, u64p, u64p]
         lib.or_rescale_to_next.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, u64p, sz, u64p]
+        lib.or_behz_bases.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p,
+                                      ctypes.POINTER(sz)]
+        lib.or_bfv_multiply.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p]
         _oracle = lib
     return _oracle
 
@@ -139,6 +142,26 @@ def ckks_multiply(a, b, moduli, logn):
     b = np.ascontiguousarray(b, dtype=np.uint64)
     assert oracle().or_ckks_multiply(logn, ptr(m), L, ptr(a), 2, ptr(b), 2, ptr(out)) == 0
     return out
+
+
+def bfv_multiply(a, b, moduli, t, logn):
+    """a, b: [2][L][N] coefficient form -> [3][L][N] (BEHZ)"""
+    L = len(moduli)
+    out = np.zeros((3, L, 1 << logn), dtype=np.uint64)
+    m = _arr(moduli)
+    assert oracle().or_bfv_multiply(logn, ptr(m), L, t, ptr(np.ascontiguousarray(a, dtype=np.uint64)),
+                                    ptr(np.ascontiguousarray(b, dtype=np.uint64)), ptr(out)) == 0
+    return out
+
+
+def behz_bases(moduli, t, logn):
+    m = _arr(moduli)
+    msk, gamma = np.zeros(1, dtype=np.uint64), np.zeros(1, dtype=np.uint64)
+    B = np.zeros(70, dtype=np.uint64)
+    bsz = ctypes.c_size_t()
+    assert oracle().or_behz_bases(logn, ptr(m), len(moduli), t, ptr(msk), ptr(gamma), ptr(B),
+                                  ctypes.byref(bsz)) == 0
+    return int(msk[0]), int(gamma[0]), [int(x) for x in B[:bsz.value]]
 
 
 def relinearize(scheme, ct3, key, key_moduli, L, logn):

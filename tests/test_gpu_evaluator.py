@@ -111,6 +111,42 @@ def test_rescale(scheme, logn, bits, L):
         np.testing.assert_array_equal(got[c], ck.rescale(scheme, ct[c], moduli, logn))
 
 
+@pytest.mark.parametrize("logn,bits,L,t,count", [
+    (10, [40, 40, 50], 2, 65537, 2),
+    (13, [43, 43, 44, 44, 44], 4, 65537, 2),   # BFV plain path: BFVDefault-sized primes, t = 65537
+    (11, [60, 60, 60], 2, 1024, 1),
+    (10, [30, 30, 30, 40], 3, 786433, 3),
+    (4, [30, 30, 40], 2, 97, 2),
+])
+def test_bfv_multiply(logn, bits, L, t, count):
+    ctx, moduli = make_ctx(BFV, logn, bits, t)
+    n = 1 << logn
+    rng = np.random.default_rng(300 + logn)
+    a = ck.uniform_residues(rng, (count, 2, L, n), moduli[:L])
+    b = ck.uniform_residues(rng, (count, 2, L, n), moduli[:L])
+    da, db, do = to_dev(a), to_dev(b), empty_dev((count, 3, L, n))
+    ws_bytes = ctx.multiply_workspace_size(count, L)
+    ws = empty_dev((ws_bytes + 7) // 8)
+    ctx.multiply(da.data_ptr(), db.data_ptr(), do.data_ptr(), count, L, ws.data_ptr(), ws_bytes, stream())
+    got = from_dev(do)
+    for c in range(count):
+        np.testing.assert_array_equal(got[c], ck.bfv_multiply(a[c], b[c], moduli[:L], t, logn))
+
+
+def test_bfv_multiply_target_shape():
+    """N = 2^16, 24 data primes of 60 bits, t = 65537 (one pair)."""
+    logn, L, t = 16, 24, 65537
+    ctx, moduli = make_ctx(BFV, logn, [60] * 25, t)
+    rng = np.random.default_rng(8)
+    a = ck.uniform_residues(rng, (1, 2, L, 1 << logn), moduli[:L])
+    b = ck.uniform_residues(rng, (1, 2, L, 1 << logn), moduli[:L])
+    da, db, do = to_dev(a), to_dev(b), empty_dev((1, 3, L, 1 << logn))
+    ws_bytes = ctx.multiply_workspace_size(1, L)
+    ws = empty_dev((ws_bytes + 7) // 8)
+    ctx.multiply(da.data_ptr(), db.data_ptr(), do.data_ptr(), 1, L, ws.data_ptr(), ws_bytes, stream())
+    np.testing.assert_array_equal(from_dev(do)[0], ck.bfv_multiply(a[0], b[0], moduli[:L], t, logn))
+
+
 def test_keyswitch_rejects_small_workspace():
     ctx, moduli = make_ctx(CKKS, 10, [60, 40, 60])
     d = to_dev(np.zeros((1, 3, 2, 1024), dtype=np.uint64))

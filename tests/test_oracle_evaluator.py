@@ -126,3 +126,80 @@ def test_evaluator_matches_reference(scheme, logn, bits, L):
     assert np.array_equal(ck.relinearize(scheme, ct3, key, moduli, L, logn), sess.relinearize(ct3, key, L))
     if L >= 2:
         assert np.array_equal(ck.rescale(scheme, a, moduli, logn), sess.rescale(a, L))
+
+
+def test_behz_bases_size_rule():
+    # RNSTool::initialize: |B| = |q| unless 32 + bits(t) + bits(q) >= 61 (|q| + 1)
+    n = 1 << 12
+    q = ck.coeff_modulus_create(n, [60, 60])
+    msk, gamma, B = ck.behz_bases(q, 65537, 12)
+    assert len(B) == 2
+    q3 = ck.coeff_modulus_create(n, [30, 30, 30])
+    assert len(ck.behz_bases(q3, 65537, 12)[2]) == 3
+    # 61-bit primes = 1 mod 2N, descending: m_sk > gamma > B_0 > B_1
+    primes = ck.get_primes(2 * n, 61, 4)
+    assert [msk, gamma] + B == primes
+
+
+def test_bfv_multiply_approximates_exact_scaled_product():
+    """BEHZ output = round(t/q * (a (x) b)) mod q up to a small base-conversion error."""
+    logn = 3
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [30, 30])
+    t = 17
+    Q = moduli[0] * moduli[1]
+    rng = np.random.default_rng(4)
+    a = ck.uniform_residues(rng, (2, 2, n), moduli)
+    b = ck.uniform_residues(rng, (2, 2, n), moduli)
+    out = ck.bfv_multiply(a, b, moduli, t, logn)
+
+    def compose(x):  # [L][n] -> centred integers in (-Q/2, Q/2]; BEHZ's Montgomery
+        r = []       # lift (smMrq) yields the centred representative
+        for c in range(n):
+            v = 0
+            for i, q in enumerate(moduli):
+                Qi = Q // q
+                v += int(x[i, c]) * Qi * pow(Qi, -1, q)
+            v %= Q
+            r.append(v - Q if v > Q // 2 else v)
+        return r
+
+    A = [compose(a[p]) for p in range(2)]
+    Bq = [compose(b[p]) for p in range(2)]
+
+    def pm(x, y):
+        r = [0] * n
+        for s in range(n):
+            for u in range(n):
+                if s + u >= n:
+                    r[s + u - n] -= x[s] * y[u]
+                else:
+                    r[s + u] += x[s] * y[u]
+        return r
+
+    exact = [pm(A[0], Bq[0]), [u + v for u, v in zip(pm(A[0], Bq[1]), pm(A[1], Bq[0]))], pm(A[1], Bq[1])]
+    for d in range(3):
+        got = compose(out[d])
+        for c in range(n):
+            want = (t * exact[d][c]) // Q
+            diff = (got[c] - want) % Q
+            diff = min(diff, Q - diff)
+            assert diff <= 8, (d, c, diff)
+
+
+@ref_only
+@pytest.mark.ref
+@pytest.mark.parametrize("logn,bits,L,t", [
+    (10, [40, 40, 50], 2, 65537),
+    (12, [43, 43, 44, 44, 44], 4, 65537),
+    (11, [60, 60, 60], 2, 1024),
+    (10, [30, 30, 30, 40], 3, 786433),
+])
+def test_bfv_multiply_matches_reference(logn, bits, L, t):
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, bits)
+    sess = ck.RefSession(BFV, n, moduli, t)
+    rng = np.random.default_rng(logn + t)
+    a = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    b = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    assert np.array_equal(ck.bfv_multiply(a, b, moduli[:L], t, logn), sess.multiply(a, b, L))
