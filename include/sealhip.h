@@ -111,6 +111,28 @@ int shc_switch_key(const shc_context *ctx, const uint64_t *d_target, size_t targ
 int shc_relinearize(const shc_context *ctx, uint64_t *d_ct, const uint64_t *d_relin_key, uint32_t count,
                     uint32_t limb_count, void *d_workspace, size_t workspace_bytes, void *stream);
 
+/* GaloisTool::getEltFromStep (src/utils/galois_cuda.cu:44-86): 3^step mod 2N, a
+ * negative step s maps to N/2 - |s|, step 0 gives 2N - 1 (complex conjugation). */
+int shc_galois_elt_from_step(uint32_t poly_modulus_degree, int step, uint32_t *elt);
+
+/* d_out[g] = sigma_elt(d_in[g]) for `poly_count` polynomials of L limbs, in NTT form
+ * (ntt_form != 0) or coefficient form.  d_in and d_out must not overlap.
+ * Replaces GaloisToolCuda::applyGaloisNtt / applyGalois (src/utils/galois_cuda.cuh:21-25). */
+int shc_apply_galois(const shc_context *ctx, const uint64_t *d_in, uint64_t *d_out, uint32_t poly_count,
+                     uint32_t limb_count, uint32_t galois_elt, int ntt_form, void *stream);
+
+int shc_galois_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limb_count, size_t *bytes);
+
+/* Apply sigma_elt to size-2 ciphertexts d_ct[count][2][L][N] in place and switch back to
+ * the original key with d_galois_key ([K-1][2][K][N], the key of element `galois_elt`,
+ * i.e. GaloisKeys::data()[(elt - 1) / 2]).  CKKS ciphertexts are in NTT form, BFV in
+ * coefficient form.
+ * Replaces EvaluatorCuda::applyGaloisInplace (src/evaluator_cuda.cu:2011-2104); rotate
+ * rows / vector compose it with shc_galois_elt_from_step (src/evaluator_cuda.cu:2106-2163). */
+int shc_apply_galois_keyswitch(const shc_context *ctx, uint64_t *d_ct, const uint64_t *d_galois_key,
+                               uint32_t galois_elt, uint32_t count, uint32_t limb_count, void *d_workspace,
+                               size_t workspace_bytes, void *stream);
+
 int shc_rescale_workspace_size(const shc_context *ctx, uint32_t count, uint32_t poly_count, uint32_t limb_count,
                                size_t *bytes);
 

@@ -107,6 +107,29 @@ int or_relinearize(int scheme, int logn, const uint64_t *key_moduli, size_t K, s
     return or_switch_key(scheme, logn, key_moduli, K, L, ct3 + 2 * L * n, relin_key, ct3);
 }
 
+int or_apply_galois_keyswitch(int scheme, int logn, const uint64_t *key_moduli, size_t K, size_t L, uint64_t *ct,
+                              uint32_t elt, const uint64_t *galois_key) {
+    /* Evaluator::applyGaloisInplace (src/evaluator.cpp:2210-2250): c0 <- sigma(c0),
+       target <- sigma(c1), c1 <- 0, switchKeyInplace(target, galois key).  ct [2][L][N]. */
+    size_t n = (size_t)1 << logn;
+    uint64_t *tmp = malloc(L * n * 8), *target = malloc(L * n * 8);
+    for (size_t j = 0; j < L; j++) {
+        if (scheme == 2) {
+            or_apply_galois_ntt(logn, elt, ct + j * n, tmp + j * n);
+            or_apply_galois_ntt(logn, elt, ct + (L + j) * n, target + j * n);
+        } else {
+            or_apply_galois(logn, elt, key_moduli[j], ct + j * n, tmp + j * n);
+            or_apply_galois(logn, elt, key_moduli[j], ct + (L + j) * n, target + j * n);
+        }
+    }
+    memcpy(ct, tmp, L * n * 8);
+    memset(ct + L * n, 0, L * n * 8);
+    int rc = or_switch_key(scheme, logn, key_moduli, K, L, target, galois_key, ct);
+    free(tmp);
+    free(target);
+    return rc;
+}
+
 int or_rescale_to_next(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *in, size_t polys,
                        uint64_t *out) {
     /* Evaluator::modSwitchScaleToNext (src/evaluator.cpp:1165-1240) via

@@ -93,6 +93,9 @@ int or_ckks_multiply(int logn, const uint64_t *moduli, size_t L, const uint64_t 
    key: [K-1][2][K][N]; target: L limbs; ct: [2][L][N] updated in place. */
 int or_switch_key(int scheme, int logn, const uint64_t *key_moduli, size_t K, size_t L, const uint64_t *target,
                   const uint64_t *key, uint64_t *ct);
+/* Evaluator::applyGaloisInplace (src/evaluator.cpp:2210-2250): ct [2][L][N] in place */
+int or_apply_galois_keyswitch(int scheme, int logn, const uint64_t *key_moduli, size_t K, size_t L, uint64_t *ct,
+                              uint32_t elt, const uint64_t *galois_key);
 /* Evaluator::relinearizeInternal (src/evaluator.cpp:1113-1163): ct3 [3][L][N] -> first 2 polys */
 int or_relinearize(int scheme, int logn, const uint64_t *key_moduli, size_t K, size_t L, uint64_t *ct3,
                    const uint64_t *relin_key);

@@ -183,6 +183,22 @@ int ref_relinearize(void *h, uint64_t L, const uint64_t *ct3, const uint64_t *ke
     } catch (const std::exception &e) { fprintf(stderr, "ref_relinearize: %s\n", e.what()); return -1; }
 }
 
+// Evaluator::applyGaloisInplace (src/evaluator.cpp:2210-2250) with one Galois key
+// stored at GaloisKeys::getIndex(elt) = (elt - 1) / 2.
+int ref_apply_galois_keyswitch(void *h, uint64_t L, const uint64_t *ct, uint32_t elt, const uint64_t *key,
+                               uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        Ciphertext x;
+        load_ct(s, x, L, 2, ct, 1024.0);
+        GaloisKeys gk;
+        load_kswitch(s, gk, (elt - 1) >> 1, key);
+        s.evaluator->applyGaloisInplace(x, elt, gk);
+        std::memcpy(out, x.data(), 2 * L * s.n * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_apply_galois: %s\n", e.what()); return -1; }
+}
+
 // Evaluator::rescaleToNext (CKKS) / modSwitchToNext (BFV) (src/evaluator.cpp:1165-1240).
 int ref_rescale(void *h, uint64_t L, uint64_t size, const uint64_t *in, uint64_t *out) {
     try {

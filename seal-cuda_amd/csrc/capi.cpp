@@ -2,6 +2,7 @@
 #include "../../include/sealhip.h"
 #include "context.h"
 #include "ntt.h"
+#include "galois.h"
 #include "rns_ops.h"
 #include <string>
 
@@ -148,6 +149,40 @@ int shc_relinearize(const shc_context *ctx, uint64_t *ct, const uint64_t *key, u
         check_limbs(c, limbs);
         const size_t poly = (size_t)limbs * c.n;
         shc::switch_key(c, ct + 2 * poly, 3 * poly, ct, 3 * poly, key, count, limbs, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_galois_elt_from_step(uint32_t n, int step, uint32_t *elt) {
+    return guarded([&] {
+        if (n < 2 || (n & (n - 1))) shc::fail(shc::ERR_INVALID_ARGUMENT, "poly_modulus_degree must be a power of two");
+        *elt = shc::hm::galois_elt_from_step(n, step);
+    });
+}
+
+int shc_apply_galois(const shc_context *ctx, const uint64_t *in, uint64_t *out, uint32_t polys, uint32_t limbs,
+                     uint32_t elt, int ntt_form, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        const size_t poly = (size_t)limbs * c.n;
+        shc::apply_galois(c, in, poly, out, poly, polys, limbs, elt, ntt_form != 0, (hipStream_t)stream);
+    });
+}
+
+int shc_galois_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = shc::galois_keyswitch_workspace(c, count, limbs);
+    });
+}
+
+int shc_apply_galois_keyswitch(const shc_context *ctx, uint64_t *ct, const uint64_t *key, uint32_t elt, uint32_t count,
+                               uint32_t limbs, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::apply_galois_keyswitch(c, ct, count, limbs, elt, key, ws, ws_bytes, (hipStream_t)stream);
     });
 }
 

@@ -127,6 +127,17 @@ bool minimal_primitive_root(uint64_t degree, uint64_t q, uint64_t &root) {
     return true;
 }
 
+uint32_t galois_elt_from_step(uint64_t n, int step) {
+    const uint64_t m = 2 * n;
+    if (step == 0) return (uint32_t)(m - 1);
+    const uint64_t pos = (uint64_t)(step < 0 ? -(int64_t)step : step);
+    if (pos >= (n >> 1)) fail(ERR_INVALID_ARGUMENT, "step count too large");
+    uint64_t e = step < 0 ? (n >> 1) - pos : pos;
+    uint64_t g = 1;
+    while (e--) g = (g * 3) & (m - 1);
+    return (uint32_t)g;
+}
+
 NttTables make_ntt_tables(int logn, uint64_t q) {
     NttTables t;
     const uint64_t n = uint64_t(1) << logn;

@@ -28,6 +28,9 @@ std::vector<uint64_t> coeff_modulus_create(size_t n, const std::vector<int> &bit
 // tryMinimalPrimitiveRoot (src/utils/numth.cpp:335-362)
 bool minimal_primitive_root(uint64_t degree, uint64_t q, uint64_t &root);
 
+// GaloisTool::getEltFromStep (src/utils/galois.cpp:44-86); throws on |step| >= N/2.
+uint32_t galois_elt_from_step(uint64_t n, int step);
+
 struct NttTables {
     uint64_t q = 0, root = 0;
     std::vector<uint64_t> fwd;  // interleaved (w, shoup(w)) pairs, [2N]

@@ -47,6 +47,10 @@ def lib():
         L.shc_keyswitch_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
         L.shc_switch_key.argtypes = [_vp, _vp, sz, _vp, sz, _vp, u32, u32, _vp, sz, _vp]
         L.shc_relinearize.argtypes = [_vp, _vp, _vp, u32, u32, _vp, sz, _vp]
+        L.shc_galois_elt_from_step.argtypes = [u32, ctypes.c_int, ctypes.POINTER(u32)]
+        L.shc_apply_galois.argtypes = [_vp, _vp, _vp, u32, u32, u32, ctypes.c_int, _vp]
+        L.shc_galois_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
+        L.shc_apply_galois_keyswitch.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
         L.shc_rescale_workspace_size.argtypes = [_vp, u32, u32, u32, ctypes.POINTER(sz)]
         L.shc_rescale_to_next.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
         _lib = L
@@ -119,6 +123,22 @@ class Context:
 
     def relinearize(self, d_ct, d_key, count, limbs, d_ws, ws_bytes, stream=None):
         check(lib().shc_relinearize(self.handle, d_ct, d_key, count, limbs, d_ws, ws_bytes, stream))
+
+    def galois_elt_from_step(self, step):
+        e = ctypes.c_uint32()
+        check(lib().shc_galois_elt_from_step(self.n, step, ctypes.byref(e)))
+        return e.value
+
+    def apply_galois(self, d_in, d_out, polys, limbs, elt, ntt_form, stream=None):
+        check(lib().shc_apply_galois(self.handle, d_in, d_out, polys, limbs, elt, 1 if ntt_form else 0, stream))
+
+    def galois_workspace_size(self, count, limbs):
+        b = ctypes.c_size_t()
+        check(lib().shc_galois_workspace_size(self.handle, count, limbs, ctypes.byref(b)))
+        return b.value
+
+    def apply_galois_keyswitch(self, d_ct, d_key, elt, count, limbs, d_ws, ws_bytes, stream=None):
+        check(lib().shc_apply_galois_keyswitch(self.handle, d_ct, d_key, elt, count, limbs, d_ws, ws_bytes, stream))
 
     def rescale_workspace_size(self, count, polys, limbs):
         b = ctypes.c_size_t()

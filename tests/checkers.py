@@ -58,6 +58,8 @@ def oracle():
         lib.or_switch_key.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, sz, u64p, u64p, u64p]
         lib.or_relinearize.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, sz, u64p, u64p]
         lib.or_rescale_to_next.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, u64p, sz, u64p]
+        lib.or_apply_galois_keyswitch.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, sz, u64p, ctypes.c_uint32,
+                                                  u64p]
         lib.or_behz_bases.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p,
                                       ctypes.POINTER(sz)]
         lib.or_bfv_multiply.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p]
@@ -86,6 +88,7 @@ def ref():
         lib.ref_multiply.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
         lib.ref_relinearize.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
         lib.ref_rescale.argtypes = [ctypes.c_void_p, u64, u64, u64p, u64p]
+        lib.ref_apply_galois_keyswitch.argtypes = [ctypes.c_void_p, u64, u64p, ctypes.c_uint32, u64p, u64p]
         _ref = lib
     return _ref
 
@@ -173,6 +176,19 @@ def relinearize(scheme, ct3, key, key_moduli, L, logn):
     return c[:2].copy()
 
 
+def apply_galois_keyswitch(scheme, ct, elt, key, key_moduli, L, logn):
+    """ct: [2][L][N] -> sigma_elt then key switch back (applyGaloisInplace)"""
+    c = np.ascontiguousarray(ct, dtype=np.uint64).copy()
+    m = _arr(key_moduli)
+    assert oracle().or_apply_galois_keyswitch(scheme, logn, ptr(m), len(key_moduli), L, ptr(c), elt,
+                                              ptr(np.ascontiguousarray(key, dtype=np.uint64))) == 0
+    return c
+
+
+def galois_elt(logn, step):
+    return int(oracle().or_galois_elt_from_step(logn, step))
+
+
 def rescale(scheme, ct, moduli, logn):
     """ct: [polys][L][N] with moduli[:L] -> [polys][L-1][N]"""
     ct = np.ascontiguousarray(ct, dtype=np.uint64)
@@ -205,6 +221,12 @@ class RefSession:
         out = np.zeros((2, L, self.n), dtype=np.uint64)
         assert ref().ref_relinearize(self.h, L, ptr(np.ascontiguousarray(ct3)), ptr(np.ascontiguousarray(key)),
                                      ptr(out)) == 0
+        return out
+
+    def apply_galois(self, ct, elt, key, L):
+        out = np.zeros((2, L, self.n), dtype=np.uint64)
+        assert ref().ref_apply_galois_keyswitch(self.h, L, ptr(np.ascontiguousarray(ct)), elt,
+                                                ptr(np.ascontiguousarray(key)), ptr(out)) == 0
         return out
 
     def rescale(self, ct, L):
