@@ -18,4 +18,11 @@ void ntt_forward(uint64_t *data, uint32_t count, uint32_t limbs_per_poly, const 
 void ntt_inverse(uint64_t *data, uint32_t count, uint32_t limbs_per_poly, const DevPrime *primes, int logn,
                  NttOut out, hipStream_t stream);
 
+// Key-switching ModUp fused with the forward NTT: T[ct][j][i] = NTT_i(d[ct][j] mod t_i)
+// for digits j < L and targets i <= L (targets[i]), d[ct] at d + ct * d_ct_stride with
+// L limbs.  Output fully reduced.  Returns false (and does nothing) for N < 2^13, where
+// the caller gathers first and runs the plain transform.
+bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
+                       const DevPrime *targets, int logn, hipStream_t stream);
+
 }  // namespace shc

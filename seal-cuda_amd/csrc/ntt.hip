@@ -179,6 +179,38 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
     }
 }
 
+// ModUp columns pass: limb m = (ct, j, i) of the output T[ct][j][i] is digit j of
+// ciphertext ct, read from d[ct][j], reduced modulo target prime i on load, then run
+// through the first K layers (the key switch's gather + forward NTT pass 1 in one).
+template <int LOGN, int K, int E, int SUBS>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const uint64_t *__restrict__ d,
+                                                                        size_t d_ct_stride,
+                                                                        uint64_t *__restrict__ T,
+                                                                        const DevPrime *__restrict__ targets,
+                                                                        uint32_t L) {
+    constexpr int SZ = 1 << K;
+    constexpr int C = 1 << (LOGN - K);
+    constexpr int TILES = C / SUBS;
+    __shared__ uint64_t tile[lds_pad(SUBS * SZ)];
+    const int tid = threadIdx.x;
+    const int col = tid % SUBS, t = tid / SUBS;
+    const uint32_t limb = blockIdx.x / TILES;
+    const int c0 = (int)(blockIdx.x % TILES) * SUBS;
+    const uint32_t i = limb % (L + 1);
+    const uint32_t r = limb / (L + 1);
+    const uint32_t j = r % L;
+    const size_t ct = r / L;
+    const uint64_t *in = d + ct * d_ct_stride + ((size_t)j << LOGN) + c0;
+    uint64_t *out = T + ((size_t)limb << LOGN) + c0;
+    const DevPrime &P = targets[i];
+    for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E))
+        tile[lds_pad(e)] = barrett64(in[(e / SUBS) * C + (e % SUBS)], P.q, P.ratio_hi);
+    __syncthreads();
+    auto addr = [col](int idx) { return lds_pad(idx * SUBS + col); };
+    run_phases<LOGN, K, E, false>(tile, addr, t, 0, 0, P.fwd, P.q, P.two_q, true);
+    for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E)) out[(e / SUBS) * C + (e % SUBS)] = tile[lds_pad(e)];
+}
+
 template <int LOGN, bool INV, bool FULL>
 void launch_ntt(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, hipStream_t st) {
     if constexpr (LOGN <= 12) {
@@ -240,7 +272,46 @@ void dispatch(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *prim
     }
 }
 
+template <int LOGN>
+void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts, uint32_t L, const DevPrime *targets,
+                  hipStream_t st) {
+    constexpr int K1 = LOGN / 2;
+    constexpr int K2 = LOGN - K1;
+    constexpr int E = 4;
+    constexpr int CS = 16;
+    constexpr int RS = 4096 >> K2;
+    const uint32_t limbs = cts * L * (L + 1);
+    ntt_cols_modup_kernel<LOGN, K1, E, CS>
+        <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, L);
+    const uint32_t rows = limbs << K1;
+    ntt_rows_kernel<LOGN, K2, E, RS, false, true, true>
+        <<<(rows + RS - 1) / RS, RS << (K2 - E), 0, st>>>(T, targets, L + 1, rows);
+}
+
 }  // namespace
+
+bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
+                       const DevPrime *targets, int logn, hipStream_t st) {
+    if (logn < 13 || logn > 17 || count == 0) return false;
+    const uint64_t per_ct = (uint64_t)L * (L + 1);
+    uint32_t max_cts = (uint32_t)(((uint64_t(1) << 34) >> logn) / per_ct);
+    if (max_cts == 0) max_cts = 1;
+    for (uint32_t done = 0; done < count;) {
+        const uint32_t c = count - done < max_cts ? count - done : max_cts;
+        const uint64_t *dd = d + (size_t)done * d_ct_stride;
+        uint64_t *TT = T + ((size_t)done * per_ct << logn);
+        switch (logn) {
+            case 13: launch_modup<13>(dd, d_ct_stride, TT, c, L, targets, st); break;
+            case 14: launch_modup<14>(dd, d_ct_stride, TT, c, L, targets, st); break;
+            case 15: launch_modup<15>(dd, d_ct_stride, TT, c, L, targets, st); break;
+            case 16: launch_modup<16>(dd, d_ct_stride, TT, c, L, targets, st); break;
+            case 17: launch_modup<17>(dd, d_ct_stride, TT, c, L, targets, st); break;
+        }
+        launch_check("ntt modup launch");
+        done += c;
+    }
+    return true;
+}
 
 void ntt_forward(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn, NttOut out,
                  hipStream_t st) {

@@ -251,10 +251,13 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
     launch_check("gather");
     if (ckks) ntt_inverse(d, count * L, L, c.primes(), logn, NTT_FULL, st);
     // 2-3. ModUp
-    const size_t twords = (size_t)count * L * (L + 1) * n;
-    modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(d, (size_t)L * n, T, targets, c.primes(), L, logn, twords);
-    launch_check("modup_gather");
-    ntt_forward(T, count * L * (L + 1), L + 1, targets, logn, NTT_FULL, st);
+    if (!ntt_forward_modup(d, (size_t)L * n, T, count, L, targets, logn, st)) {
+        const size_t twords = (size_t)count * L * (L + 1) * n;
+        modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(d, (size_t)L * n, T, targets, c.primes(), L, logn,
+                                                               twords);
+        launch_check("modup_gather");
+        ntt_forward(T, count * L * (L + 1), L + 1, targets, logn, NTT_FULL, st);
+    }
     // 4. inner product with the key
     constexpr int G = 4;
     const size_t ip_threads = ((size_t)(count + G - 1) / G) * (L + 1) * n;
