@@ -133,6 +133,63 @@ int shc_apply_galois_keyswitch(const shc_context *ctx, uint64_t *d_ct, const uin
                                uint32_t galois_elt, uint32_t count, uint32_t limb_count, void *d_workspace,
                                size_t workspace_bytes, void *stream);
 
+/* ------------------------------------------------------------------ keys, encryption
+ * Sampling is counter based: word i of seed s is splitmix64's i-th output, so a seed
+ * reproduces a polynomial exactly.  The reference samples with Blake2xb (CPU) or
+ * curand (GPU); distributions match (ternary; centred binomial, sigma ~ 3.24;
+ * uniform mod q_i), streams do not.  Every random input can also be passed
+ * explicitly (shc_encrypt_zero_asymmetric). */
+enum { SHC_SAMPLE_TERNARY = 0, SHC_SAMPLE_CBD = 1, SHC_SAMPLE_UNIFORM = 2 };
+
+/* d_out[limb][N] (coefficient form) for the first limb_count key-level primes.
+ * Replaces samplePolyTernary / samplePolyCbd / samplePolyUniform (src/utils/rlwe_cuda.cu:25-150). */
+int shc_sample_poly(const shc_context *ctx, int kind, uint64_t seed, uint32_t limb_count, uint64_t *d_out,
+                    void *stream);
+
+/* Secret key [K][N] in NTT form.  Replaces KeyGeneratorCuda(context) (src/keygenerator_cuda.cuh). */
+int shc_keygen_secret(const shc_context *ctx, uint64_t seed, uint64_t *d_secret_key, void *stream);
+/* Public key [2][K][N] in NTT form.  Replaces KeyGeneratorCuda::createPublicKey. */
+int shc_keygen_public(const shc_context *ctx, const uint64_t *d_secret_key, uint64_t seed, uint64_t *d_public_key,
+                      void *stream);
+int shc_keygen_workspace_size(const shc_context *ctx, size_t *bytes);
+/* Relinearization key [K-1][2][K][N] (key for s^2).  Replaces KeyGeneratorCuda::createRelinKeys. */
+int shc_keygen_relin(const shc_context *ctx, const uint64_t *d_secret_key, uint64_t seed, uint64_t *d_relin_key,
+                     void *d_workspace, size_t workspace_bytes, void *stream);
+/* Galois key of one element [K-1][2][K][N].  Replaces KeyGeneratorCuda::createGaloisKeys. */
+int shc_keygen_galois(const shc_context *ctx, const uint64_t *d_secret_key, uint32_t galois_elt, uint64_t seed,
+                      uint64_t *d_galois_key, void *d_workspace, size_t workspace_bytes, void *stream);
+
+int shc_encrypt_workspace_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
+/* Encryption of zero at level L (limb_count) from explicit randomness u, e0, e1
+ * ([L+1][N] coefficient form: one level above, or L when L is the key level), then
+ * divide-and-round to L limbs; CKKS output in NTT form, BFV in coefficient form.
+ * Replaces EncryptorCuda::encryptZeroInternal (src/encryptor_cuda.cu:92-160). */
+int shc_encrypt_zero_asymmetric(const shc_context *ctx, const uint64_t *d_public_key, const uint64_t *d_u,
+                                const uint64_t *d_e0, const uint64_t *d_e1, uint32_t limb_count, uint64_t *d_out,
+                                void *d_workspace, size_t workspace_bytes, void *stream);
+/* Public-key encryption of d_plain (BFV: [N] coefficients mod t; CKKS: [L][N] in NTT
+ * form; NULL encrypts zero) with randomness drawn from `seed`.
+ * Replaces EncryptorCuda::encrypt (src/encryptor_cuda.cuh:123, src/encryptor_cuda.cu:162-237). */
+int shc_encrypt(const shc_context *ctx, const uint64_t *d_public_key, const uint64_t *d_plain, uint32_t limb_count,
+                uint64_t seed, uint64_t *d_out, void *d_workspace, size_t workspace_bytes, void *stream);
+/* Symmetric encryption of zero with the secret key (utils/rlwe_cuda.cu:262-331). */
+int shc_encrypt_zero_symmetric(const shc_context *ctx, const uint64_t *d_secret_key, uint32_t limb_count,
+                               uint64_t seed, uint64_t *d_out, void *stream);
+
+int shc_decrypt_workspace_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
+/* Decrypt a size-2 or size-3 ciphertext d_ct[size][L][N].  BFV: d_out [N] mod t;
+ * CKKS: d_out [L][N], the NTT-form plaintext.
+ * Replaces DecryptorCuda::decrypt (src/decryptor_cuda.cuh:66, src/decryptor_cuda.cu:92-328). */
+int shc_decrypt(const shc_context *ctx, const uint64_t *d_ct, uint32_t ct_size, const uint64_t *d_secret_key,
+                uint32_t limb_count, uint64_t *d_out, void *d_workspace, size_t workspace_bytes, void *stream);
+
+/* BatchEncoder: d_values[count <= N] -> d_plain[N] mod t, and back (d_scratch: N words).
+ * Replaces BatchEncoderCuda::encode / decode (src/batchencoder_cuda.cuh:61-75). */
+int shc_batch_encode(const shc_context *ctx, const uint64_t *d_values, uint32_t count, uint64_t *d_plain,
+                     void *stream);
+int shc_batch_decode(const shc_context *ctx, const uint64_t *d_plain, uint64_t *d_values, uint64_t *d_scratch,
+                     void *stream);
+
 int shc_rescale_workspace_size(const shc_context *ctx, uint32_t count, uint32_t poly_count, uint32_t limb_count,
                                size_t *bytes);
 

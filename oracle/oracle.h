@@ -108,6 +108,27 @@ int or_behz_bases(int logn, const uint64_t *q, size_t L, uint64_t t, uint64_t *m
 int or_bfv_multiply(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *a, const uint64_t *b,
                     uint64_t *out);
 
+/* ---------------------------------------------------------------- encryption (crypto.c)
+ * Randomness is an explicit input (see crypto.c).  Pinned by: decrypt-equality with
+ * the reference Decryptor (oracle/_ref), the reference Decryptor and BatchEncoder
+ * words, closed forms (phase = Delta m + small noise).                             */
+int or_encrypt_zero_asymmetric(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *pk,
+                               size_t pk_limbs, const uint64_t *u, const uint64_t *e0, const uint64_t *e1,
+                               uint64_t *out);
+int or_encrypt_zero_symmetric(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *sk,
+                              size_t sk_limbs, const uint64_t *a, const uint64_t *e, uint64_t *out);
+int or_generate_kswitch_key(int logn, const uint64_t *key_moduli, size_t K, const uint64_t *sk,
+                            const uint64_t *new_key, const uint64_t *a, const uint64_t *e, uint64_t *out);
+int or_decrypt_phase(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *ct, size_t size,
+                     const uint64_t *sk, size_t sk_limbs, uint64_t *out);
+int or_bfv_decrypt_scale_round(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *phase,
+                               uint64_t *out);
+int or_bfv_add_plain_scaled(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *plain,
+                            size_t plain_count, uint64_t *c0);
+void or_batch_index_map(int logn, uint64_t *map);
+int or_batch_encode(int logn, uint64_t t, const uint64_t *values, size_t count, uint64_t *plain);
+int or_batch_decode(int logn, uint64_t t, const uint64_t *plain, uint64_t *values);
+
 /* Evaluator::modSwitchScaleToNext (src/evaluator.cpp:1165-1240): in [polys][L][N] -> out [polys][L-1][N] */
 int or_rescale_to_next(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *in, size_t polys,
                        uint64_t *out);

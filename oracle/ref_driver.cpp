@@ -199,6 +199,99 @@ int ref_apply_galois_keyswitch(void *h, uint64_t L, const uint64_t *ct, uint32_t
     } catch (const std::exception &e) { fprintf(stderr, "ref_apply_galois: %s\n", e.what()); return -1; }
 }
 
+static SecretKey load_sk(const RefSession &s, const uint64_t *sk) {
+    SecretKey key;
+    key.data().parmsID() = parmsIDZero;
+    key.data().resize(s.K * s.n);
+    std::memcpy(key.data().data(), sk, s.K * s.n * 8);
+    key.parmsID() = s.context->keyParmsID();
+    return key;
+}
+
+// Decryptor::decrypt (src/decryptor.cpp:79-153) with an injected key-level secret key
+// (NTT form, [K][N]).  CKKS: out = NTT-form plaintext [L][N]; BFV: out = [N] mod t.
+int ref_decrypt(void *h, uint64_t L, const uint64_t *ct, const uint64_t *sk, uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        Ciphertext x;
+        load_ct(s, x, L, 2, ct, 1024.0);
+        SecretKey key = load_sk(s, sk);
+        Decryptor dec(*s.context, key);
+        Plaintext p;
+        dec.decrypt(x, p);
+        if (s.scheme == SchemeType::ckks) {
+            std::memcpy(out, p.data(), L * s.n * 8);
+        } else {
+            std::memset(out, 0, s.n * 8);
+            std::memcpy(out, p.data(), p.coeffCount() * 8);
+        }
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_decrypt: %s\n", e.what()); return -1; }
+}
+
+// BatchEncoder::encode / decode (src/batchencoder.cpp:104-200)
+int ref_batch_encode(void *h, const uint64_t *values, uint64_t count, uint64_t *plain) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        BatchEncoder enc(*s.context);
+        Plaintext p;
+        enc.encode(std::vector<uint64_t>(values, values + count), p);
+        std::memset(plain, 0, s.n * 8);
+        std::memcpy(plain, p.data(), p.coeffCount() * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_batch_encode: %s\n", e.what()); return -1; }
+}
+
+int ref_batch_decode(void *h, const uint64_t *plain, uint64_t *values) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        BatchEncoder enc(*s.context);
+        Plaintext p;
+        p.resize(s.n);
+        std::memcpy(p.data(), plain, s.n * 8);
+        std::vector<uint64_t> v;
+        enc.decode(p, v);
+        std::memcpy(values, v.data(), s.n * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_batch_decode: %s\n", e.what()); return -1; }
+}
+
+// CKKSEncoder::encode (src/ckks.cpp:98-277) of `count` complex values (re, im pairs)
+// at the level with L primes; out: NTT-form plaintext [L][N].
+int ref_ckks_encode(void *h, uint64_t L, const double *values, uint64_t count, double scale, uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        CKKSEncoder enc(*s.context);
+        std::vector<std::complex<double>> v(count);
+        for (size_t i = 0; i < count; i++) v[i] = {values[2 * i], values[2 * i + 1]};
+        Plaintext p;
+        enc.encode(v, parms_for_level(s, L), scale, p);
+        std::memcpy(out, p.data(), L * s.n * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_ckks_encode: %s\n", e.what()); return -1; }
+}
+
+// CKKSEncoder::decode (src/ckks.cpp:474-580): NTT-form plaintext [L][N] -> N/2 complex
+int ref_ckks_decode(void *h, uint64_t L, const uint64_t *plain, double scale, double *values) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        CKKSEncoder enc(*s.context);
+        Plaintext p;
+        p.parmsID() = parmsIDZero;
+        p.resize(L * s.n);
+        std::memcpy(p.data(), plain, L * s.n * 8);
+        p.parmsID() = parms_for_level(s, L);
+        p.scale() = scale;
+        std::vector<std::complex<double>> v;
+        enc.decode(p, v);
+        for (size_t i = 0; i < v.size(); i++) {
+            values[2 * i] = v[i].real();
+            values[2 * i + 1] = v[i].imag();
+        }
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_ckks_decode: %s\n", e.what()); return -1; }
+}
+
 // Evaluator::rescaleToNext (CKKS) / modSwitchToNext (BFV) (src/evaluator.cpp:1165-1240).
 int ref_rescale(void *h, uint64_t L, uint64_t size, const uint64_t *in, uint64_t *out) {
     try {

@@ -31,6 +31,17 @@ struct BehzDev {
     const uint64_t *bhat_mod_q;      // [L][bsz] Bhat_j mod q_i
     const uint64_t *bhat_mod_msk;    // [bsz]    Bhat_j mod m_sk
     const uint64_t *prod_B_mod_q;    // [L]      prod(B) mod q_i
+    // decryptScaleAndRound (src/utils/rns.cpp:1039-1095): base q -> {t, gamma}
+    uint64_t gamma;
+    uint64_t neg_inv_q_t, neg_inv_q_t_shoup, neg_inv_q_g, neg_inv_q_g_shoup;
+    uint64_t inv_gamma_t, inv_gamma_t_shoup;
+    uint64_t t_ratio_hi, t_ratio_lo, g_ratio_hi, g_ratio_lo;  // floor(2^128/t), floor(2^128/gamma)
+    const uint64_t *tg_qhat_inv;     // [L][2]   (t gamma qhat_j^-1) mod q_j
+    const uint64_t *qhat_mod_t;      // [L]
+    const uint64_t *qhat_mod_g;      // [L]
+    // multiplyAddPlainWithScalingVariant (src/utils/scalingvariant.cpp:48-86)
+    uint64_t q_mod_t;                // prod(q) mod t
+    const uint64_t *delta;           // [L][2]   floor(prod(q)/t) mod q_j
 };
 
 struct BehzLevel {

@@ -208,6 +208,25 @@ void build_behz_level(Context &c, uint32_t L, BehzLevel &out) {
     v.clear();
     for (uint32_t i = 0; i < L; i++) v.push_back(prod_mod(Bv, NONE, q[i]));
     const size_t o_prodB_q = put(v);
+    // decrypt: base q -> {t, gamma} (RNSTool::initialize, src/utils/rns.cpp:721-760)
+    const uint64_t g = out.gamma;
+    v.clear();
+    for (uint32_t j = 0; j < L; j++) v.push_back(mulmod(mulmod(t % q[j], g % q[j], q[j]), qhat_inv[j], q[j]));
+    const size_t o_tg_qhat_inv = put(shoup_vec(v, q));
+    v.clear();
+    for (uint32_t j = 0; j < L; j++) v.push_back(prod_mod(q, j, t));
+    const size_t o_qhat_t = put(v);
+    v.clear();
+    for (uint32_t j = 0; j < L; j++) v.push_back(prod_mod(q, j, g));
+    const size_t o_qhat_g = put(v);
+    // Delta = floor(prod(q) / t) mod q_j and prod(q) mod t (SEALContext::validate,
+    // src/context.cpp:264-284)
+    uint64_t q_mod_t = 0;
+    const Big qd = big_div_word(big_product(q), t, q_mod_t);
+    v.clear();
+    std::vector<uint64_t> dl;
+    for (uint32_t j = 0; j < L; j++) dl.push_back(big_mod_word(qd, q[j]));
+    const size_t o_delta = put(shoup_vec(dl, q));
 
     out.tables = DevBuf(words.size() * 8);
     SHC_HIP(hipMemcpy(out.tables.ptr, words.data(), words.size() * 8, hipMemcpyHostToDevice));
@@ -240,6 +259,22 @@ void build_behz_level(Context &c, uint32_t L, BehzLevel &out) {
     d.bhat_mod_q = base + o_bhat_q;
     d.bhat_mod_msk = base + o_bhat_msk;
     d.prod_B_mod_q = base + o_prodB_q;
+    d.gamma = g;
+    uint64_t inv = invmod(prod_mod(q, NONE, t), t);
+    d.neg_inv_q_t = inv ? t - inv : 0;
+    d.neg_inv_q_t_shoup = shoup(d.neg_inv_q_t, t);
+    inv = invmod(prod_mod(q, NONE, g), g);
+    d.neg_inv_q_g = inv ? g - inv : 0;
+    d.neg_inv_q_g_shoup = shoup(d.neg_inv_q_g, g);
+    d.inv_gamma_t = invmod(g % t, t);
+    d.inv_gamma_t_shoup = shoup(d.inv_gamma_t, t);
+    barrett_ratio(t, d.t_ratio_lo, d.t_ratio_hi);
+    barrett_ratio(g, d.g_ratio_lo, d.g_ratio_hi);
+    d.tg_qhat_inv = base + o_tg_qhat_inv;
+    d.qhat_mod_t = base + o_qhat_t;
+    d.qhat_mod_g = base + o_qhat_g;
+    d.q_mod_t = q_mod_t;
+    d.delta = base + o_delta;
 }
 
 size_t bfv_multiply_workspace(const Context &c, uint32_t count, uint32_t L) {

@@ -2,6 +2,7 @@
 #include "../../include/sealhip.h"
 #include "context.h"
 #include "ntt.h"
+#include "crypto.h"
 #include "galois.h"
 #include "rns_ops.h"
 #include <string>
@@ -184,6 +185,100 @@ int shc_apply_galois_keyswitch(const shc_context *ctx, uint64_t *ct, const uint6
         check_limbs(c, limbs);
         shc::apply_galois_keyswitch(c, ct, count, limbs, elt, key, ws, ws_bytes, (hipStream_t)stream);
     });
+}
+
+int shc_sample_poly(const shc_context *ctx, int kind, uint64_t seed, uint32_t limbs, uint64_t *out, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        if (kind < 0 || kind > 2) shc::fail(shc::ERR_INVALID_ARGUMENT, "unknown sample kind");
+        shc::sample_poly(c, (shc::SampleKind)kind, seed, c.primes(), limbs, out, (hipStream_t)stream);
+    });
+}
+
+int shc_keygen_secret(const shc_context *ctx, uint64_t seed, uint64_t *sk, void *stream) {
+    return guarded([&] { shc::keygen_secret(ctx_of(ctx), seed, sk, (hipStream_t)stream); });
+}
+
+int shc_keygen_public(const shc_context *ctx, const uint64_t *sk, uint64_t seed, uint64_t *pk, void *stream) {
+    return guarded([&] { shc::keygen_public(ctx_of(ctx), sk, seed, pk, (hipStream_t)stream); });
+}
+
+int shc_keygen_workspace_size(const shc_context *ctx, size_t *bytes) {
+    return guarded([&] { *bytes = shc::keygen_workspace(ctx_of(ctx)); });
+}
+
+int shc_keygen_relin(const shc_context *ctx, const uint64_t *sk, uint64_t seed, uint64_t *key, void *ws,
+                     size_t ws_bytes, void *stream) {
+    return guarded([&] { shc::keygen_relin(ctx_of(ctx), sk, seed, key, ws, ws_bytes, (hipStream_t)stream); });
+}
+
+int shc_keygen_galois(const shc_context *ctx, const uint64_t *sk, uint32_t elt, uint64_t seed, uint64_t *key, void *ws,
+                      size_t ws_bytes, void *stream) {
+    return guarded([&] { shc::keygen_galois(ctx_of(ctx), sk, elt, seed, key, ws, ws_bytes, (hipStream_t)stream); });
+}
+
+int shc_encrypt_workspace_size(const shc_context *ctx, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = shc::encrypt_workspace(c, limbs);
+    });
+}
+
+int shc_encrypt_zero_asymmetric(const shc_context *ctx, const uint64_t *pk, const uint64_t *u, const uint64_t *e0,
+                                const uint64_t *e1, uint32_t limbs, uint64_t *out, void *ws, size_t ws_bytes,
+                                void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::encrypt_zero_asymmetric(c, pk, u, e0, e1, limbs, out, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_encrypt(const shc_context *ctx, const uint64_t *pk, const uint64_t *plain, uint32_t limbs, uint64_t seed,
+                uint64_t *out, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::encrypt(c, pk, plain, limbs, seed, out, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_encrypt_zero_symmetric(const shc_context *ctx, const uint64_t *sk, uint32_t limbs, uint64_t seed,
+                               uint64_t *out, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::encrypt_zero_symmetric(c, sk, c.primes(), limbs, c.scheme == shc::SCHEME_CKKS, seed, out,
+                                    (hipStream_t)stream);
+    });
+}
+
+int shc_decrypt_workspace_size(const shc_context *ctx, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = shc::decrypt_workspace(c, limbs);
+    });
+}
+
+int shc_decrypt(const shc_context *ctx, const uint64_t *ct, uint32_t size, const uint64_t *sk, uint32_t limbs,
+                uint64_t *out, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::decrypt(c, ct, size, sk, limbs, out, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_batch_encode(const shc_context *ctx, const uint64_t *values, uint32_t count, uint64_t *plain, void *stream) {
+    return guarded([&] { shc::batch_encode(ctx_of(ctx), values, count, plain, (hipStream_t)stream); });
+}
+
+int shc_batch_decode(const shc_context *ctx, const uint64_t *plain, uint64_t *values, uint64_t *scratch,
+                     void *stream) {
+    return guarded([&] { shc::batch_decode(ctx_of(ctx), plain, values, scratch, (hipStream_t)stream); });
 }
 
 int shc_rescale_workspace_size(const shc_context *ctx, uint32_t count, uint32_t polys, uint32_t limbs, size_t *bytes) {

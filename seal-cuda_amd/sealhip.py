@@ -51,6 +51,21 @@ def lib():
         L.shc_apply_galois.argtypes = [_vp, _vp, _vp, u32, u32, u32, ctypes.c_int, _vp]
         L.shc_galois_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
         L.shc_apply_galois_keyswitch.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
+        u64 = ctypes.c_uint64
+        L.shc_sample_poly.argtypes = [_vp, ctypes.c_int, u64, u32, _vp, _vp]
+        L.shc_keygen_secret.argtypes = [_vp, u64, _vp, _vp]
+        L.shc_keygen_public.argtypes = [_vp, _vp, u64, _vp, _vp]
+        L.shc_keygen_workspace_size.argtypes = [_vp, ctypes.POINTER(sz)]
+        L.shc_keygen_relin.argtypes = [_vp, _vp, u64, _vp, _vp, sz, _vp]
+        L.shc_keygen_galois.argtypes = [_vp, _vp, u32, u64, _vp, _vp, sz, _vp]
+        L.shc_encrypt_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
+        L.shc_encrypt_zero_asymmetric.argtypes = [_vp, _vp, _vp, _vp, _vp, u32, _vp, _vp, sz, _vp]
+        L.shc_encrypt.argtypes = [_vp, _vp, _vp, u32, u64, _vp, _vp, sz, _vp]
+        L.shc_encrypt_zero_symmetric.argtypes = [_vp, _vp, u32, u64, _vp, _vp]
+        L.shc_decrypt_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
+        L.shc_decrypt.argtypes = [_vp, _vp, u32, _vp, u32, _vp, _vp, sz, _vp]
+        L.shc_batch_encode.argtypes = [_vp, _vp, u32, _vp, _vp]
+        L.shc_batch_decode.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.shc_rescale_workspace_size.argtypes = [_vp, u32, u32, u32, ctypes.POINTER(sz)]
         L.shc_rescale_to_next.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
         _lib = L
@@ -139,6 +154,55 @@ class Context:
 
     def apply_galois_keyswitch(self, d_ct, d_key, elt, count, limbs, d_ws, ws_bytes, stream=None):
         check(lib().shc_apply_galois_keyswitch(self.handle, d_ct, d_key, elt, count, limbs, d_ws, ws_bytes, stream))
+
+    # ---- keys, encryption, decryption, batch encoding
+    def _size(self, fn, *args):
+        b = ctypes.c_size_t()
+        check(fn(self.handle, *args, ctypes.byref(b)))
+        return b.value
+
+    def sample_poly(self, kind, seed, limbs, d_out, stream=None):
+        check(lib().shc_sample_poly(self.handle, kind, seed, limbs, d_out, stream))
+
+    def keygen_secret(self, seed, d_sk, stream=None):
+        check(lib().shc_keygen_secret(self.handle, seed, d_sk, stream))
+
+    def keygen_public(self, d_sk, seed, d_pk, stream=None):
+        check(lib().shc_keygen_public(self.handle, d_sk, seed, d_pk, stream))
+
+    def keygen_workspace_size(self):
+        return self._size(lib().shc_keygen_workspace_size)
+
+    def keygen_relin(self, d_sk, seed, d_key, d_ws, ws_bytes, stream=None):
+        check(lib().shc_keygen_relin(self.handle, d_sk, seed, d_key, d_ws, ws_bytes, stream))
+
+    def keygen_galois(self, d_sk, elt, seed, d_key, d_ws, ws_bytes, stream=None):
+        check(lib().shc_keygen_galois(self.handle, d_sk, elt, seed, d_key, d_ws, ws_bytes, stream))
+
+    def encrypt_workspace_size(self, limbs):
+        return self._size(lib().shc_encrypt_workspace_size, limbs)
+
+    def encrypt_zero_asymmetric(self, d_pk, d_u, d_e0, d_e1, limbs, d_out, d_ws, ws_bytes, stream=None):
+        check(lib().shc_encrypt_zero_asymmetric(self.handle, d_pk, d_u, d_e0, d_e1, limbs, d_out, d_ws, ws_bytes,
+                                                stream))
+
+    def encrypt(self, d_pk, d_plain, limbs, seed, d_out, d_ws, ws_bytes, stream=None):
+        check(lib().shc_encrypt(self.handle, d_pk, d_plain, limbs, seed, d_out, d_ws, ws_bytes, stream))
+
+    def encrypt_zero_symmetric(self, d_sk, limbs, seed, d_out, stream=None):
+        check(lib().shc_encrypt_zero_symmetric(self.handle, d_sk, limbs, seed, d_out, stream))
+
+    def decrypt_workspace_size(self, limbs):
+        return self._size(lib().shc_decrypt_workspace_size, limbs)
+
+    def decrypt(self, d_ct, size, d_sk, limbs, d_out, d_ws, ws_bytes, stream=None):
+        check(lib().shc_decrypt(self.handle, d_ct, size, d_sk, limbs, d_out, d_ws, ws_bytes, stream))
+
+    def batch_encode(self, d_values, count, d_plain, stream=None):
+        check(lib().shc_batch_encode(self.handle, d_values, count, d_plain, stream))
+
+    def batch_decode(self, d_plain, d_values, d_scratch, stream=None):
+        check(lib().shc_batch_decode(self.handle, d_plain, d_values, d_scratch, stream))
 
     def rescale_workspace_size(self, count, polys, limbs):
         b = ctypes.c_size_t()

@@ -60,6 +60,16 @@ def oracle():
         lib.or_rescale_to_next.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, u64p, sz, u64p]
         lib.or_apply_galois_keyswitch.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, sz, u64p, ctypes.c_uint32,
                                                   u64p]
+        lib.or_encrypt_zero_asymmetric.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, u64p, sz, u64p, u64p,
+                                                   u64p, u64p]
+        lib.or_encrypt_zero_symmetric.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, u64p, sz, u64p, u64p, u64p]
+        lib.or_generate_kswitch_key.argtypes = [ctypes.c_int, u64p, sz, u64p, u64p, u64p, u64p, u64p]
+        lib.or_decrypt_phase.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, u64p, sz, u64p, sz, u64p]
+        lib.or_bfv_decrypt_scale_round.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p]
+        lib.or_bfv_add_plain_scaled.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, sz, u64p]
+        lib.or_batch_index_map.argtypes = [ctypes.c_int, u64p]
+        lib.or_batch_encode.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, sz, u64p]
+        lib.or_batch_decode.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, u64p]
         lib.or_behz_bases.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p,
                                       ctypes.POINTER(sz)]
         lib.or_bfv_multiply.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p]
@@ -89,6 +99,12 @@ def ref():
         lib.ref_relinearize.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
         lib.ref_rescale.argtypes = [ctypes.c_void_p, u64, u64, u64p, u64p]
         lib.ref_apply_galois_keyswitch.argtypes = [ctypes.c_void_p, u64, u64p, ctypes.c_uint32, u64p, u64p]
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.ref_decrypt.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
+        lib.ref_batch_encode.argtypes = [ctypes.c_void_p, u64p, u64, u64p]
+        lib.ref_batch_decode.argtypes = [ctypes.c_void_p, u64p, u64p]
+        lib.ref_ckks_encode.argtypes = [ctypes.c_void_p, u64, dp, u64, ctypes.c_double, u64p]
+        lib.ref_ckks_decode.argtypes = [ctypes.c_void_p, u64, u64p, ctypes.c_double, dp]
         _ref = lib
     return _ref
 
@@ -185,6 +201,80 @@ def apply_galois_keyswitch(scheme, ct, elt, key, key_moduli, L, logn):
     return c
 
 
+def encrypt_zero_asymmetric(scheme, pk, u, e0, e1, moduli, logn):
+    """pk: [2][K][N] NTT form; u, e0, e1: [L][N] coefficient form -> [2][L][N]"""
+    L = len(moduli)
+    out = np.zeros((2, L, 1 << logn), dtype=np.uint64)
+    m = _arr(moduli)
+    c = lambda x: ptr(np.ascontiguousarray(x, dtype=np.uint64))
+    assert oracle().or_encrypt_zero_asymmetric(scheme, logn, ptr(m), L, c(pk), pk.shape[1], c(u), c(e0), c(e1),
+                                               ptr(out)) == 0
+    return out
+
+
+def encrypt_zero_symmetric(scheme, sk, a, e, moduli, logn):
+    L = len(moduli)
+    out = np.zeros((2, L, 1 << logn), dtype=np.uint64)
+    m = _arr(moduli)
+    c = lambda x: ptr(np.ascontiguousarray(x, dtype=np.uint64))
+    assert oracle().or_encrypt_zero_symmetric(scheme, logn, ptr(m), L, c(sk), sk.shape[0], c(a), c(e),
+                                              ptr(out)) == 0
+    return out
+
+
+def generate_kswitch_key(sk, new_key, a, e, moduli, logn):
+    K = len(moduli)
+    out = np.zeros((K - 1, 2, K, 1 << logn), dtype=np.uint64)
+    m = _arr(moduli)
+    c = lambda x: ptr(np.ascontiguousarray(x, dtype=np.uint64))
+    assert oracle().or_generate_kswitch_key(logn, ptr(m), K, c(sk), c(new_key), c(a), c(e), ptr(out)) == 0
+    return out
+
+
+def decrypt_phase(scheme, ct, sk, moduli, logn):
+    size, L, n = ct.shape
+    out = np.zeros((L, n), dtype=np.uint64)
+    m = _arr(moduli[:L])
+    c = lambda x: ptr(np.ascontiguousarray(x, dtype=np.uint64))
+    assert oracle().or_decrypt_phase(scheme, logn, ptr(m), L, c(ct), size, c(sk), sk.shape[0], ptr(out)) == 0
+    return out
+
+
+def bfv_decrypt(ct, sk, moduli, t, logn):
+    L = ct.shape[1]
+    phase = decrypt_phase(1, ct, sk, moduli, logn)
+    out = np.zeros(1 << logn, dtype=np.uint64)
+    assert oracle().or_bfv_decrypt_scale_round(logn, ptr(_arr(moduli[:L])), L, t, ptr(phase), ptr(out)) == 0
+    return out
+
+
+def bfv_add_plain_scaled(c0, plain, moduli, t, logn):
+    c = np.ascontiguousarray(c0, dtype=np.uint64).copy()
+    L = c.shape[0]
+    p = np.ascontiguousarray(plain, dtype=np.uint64)
+    assert oracle().or_bfv_add_plain_scaled(logn, ptr(_arr(moduli[:L])), L, t, ptr(p), len(p), ptr(c)) == 0
+    return c
+
+
+def batch_encode(values, t, logn):
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    out = np.zeros(1 << logn, dtype=np.uint64)
+    assert oracle().or_batch_encode(logn, t, ptr(v), len(v), ptr(out)) == 0
+    return out
+
+
+def batch_decode(plain, t, logn):
+    out = np.zeros(1 << logn, dtype=np.uint64)
+    assert oracle().or_batch_decode(logn, t, ptr(np.ascontiguousarray(plain, dtype=np.uint64)), ptr(out)) == 0
+    return out
+
+
+def rns_of_small(values, moduli):
+    """[N] signed integers -> [L][N] residues (one integer reduced into every limb)."""
+    v = np.asarray(values, dtype=object)
+    return np.array([[int(x) % q for x in v] for q in moduli], dtype=np.uint64)
+
+
 def galois_elt(logn, step):
     return int(oracle().or_galois_elt_from_step(logn, step))
 
@@ -206,7 +296,7 @@ class RefSession:
         m = _arr(moduli)
         self.h = ref().ref_session_create(scheme, n, ptr(m), len(moduli), plain)
         assert self.h, "reference rejected the parameters"
-        self.n, self.K = n, len(moduli)
+        self.n, self.K, self.scheme = n, len(moduli), scheme
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -228,6 +318,36 @@ class RefSession:
         assert ref().ref_apply_galois_keyswitch(self.h, L, ptr(np.ascontiguousarray(ct)), elt,
                                                 ptr(np.ascontiguousarray(key)), ptr(out)) == 0
         return out
+
+    def decrypt(self, ct, sk, L):
+        shape = (L, self.n) if self.scheme == 2 else (self.n,)
+        out = np.zeros(shape, dtype=np.uint64)
+        assert ref().ref_decrypt(self.h, L, ptr(np.ascontiguousarray(ct)), ptr(np.ascontiguousarray(sk)),
+                                 ptr(out)) == 0
+        return out
+
+    def batch_encode(self, values):
+        v = np.ascontiguousarray(values, dtype=np.uint64)
+        out = np.zeros(self.n, dtype=np.uint64)
+        assert ref().ref_batch_encode(self.h, ptr(v), len(v), ptr(out)) == 0
+        return out
+
+    def batch_decode(self, plain):
+        out = np.zeros(self.n, dtype=np.uint64)
+        assert ref().ref_batch_decode(self.h, ptr(np.ascontiguousarray(plain, dtype=np.uint64)), ptr(out)) == 0
+        return out
+
+    def ckks_encode(self, values, L, scale):
+        v = np.ascontiguousarray(np.stack([np.real(values), np.imag(values)], axis=1).reshape(-1), dtype=np.float64)
+        out = np.zeros((L, self.n), dtype=np.uint64)
+        assert ref().ref_ckks_encode(self.h, L, dptr(v), len(values), scale, ptr(out)) == 0
+        return out
+
+    def ckks_decode(self, plain, L, scale):
+        out = np.zeros(self.n, dtype=np.float64)
+        assert ref().ref_ckks_decode(self.h, L, ptr(np.ascontiguousarray(plain, dtype=np.uint64)), scale,
+                                     dptr(out)) == 0
+        return out[0::2] + 1j * out[1::2]
 
     def rescale(self, ct, L):
         size = ct.shape[0]

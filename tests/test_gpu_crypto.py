@@ -1,0 +1,227 @@
+"""GPU parity and end-to-end correctness of sampling, key generation, encryption,
+decryption and the BatchEncoder.
+
+Word-level parity: decryption and batch encoding are compared word for word with the
+oracle on arbitrary inputs; encryption is compared on the same sampled randomness.
+End to end (the reference's decrypt-KAT style, test/evaluator_cuda.cu): the BFV plain
+path of the CPU-runnable configuration (N = 8192, 43,43,44,44,44-bit primes,
+t = 65537): encode -> encrypt -> multiply -> relinearize -> decrypt -> decode gives
+the slot-wise product mod t.
+"""
+import numpy as np
+import pytest
+
+import checkers as ck
+from gpu_util import empty_dev, from_dev, hip, stream, to_dev
+
+pytestmark = pytest.mark.gpu
+BFV, CKKS = 1, 2
+T = 65537
+
+
+def bfv_ctx(logn=13, bits=(43, 43, 44, 44, 44), t=T):
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, list(bits))
+    return hip().Context(BFV, n, moduli, t), moduli
+
+
+def ws(nbytes):
+    return empty_dev((max(nbytes, 8) + 7) // 8)
+
+
+def centred(x, q):
+    x = x.astype(object)
+    return np.where(x > q // 2, x - q, x)
+
+
+def test_samplers_distributions():
+    ctx, moduli = bfv_ctx()
+    n, K = ctx.n, len(moduli)
+    d = empty_dev((K, n))
+    ctx.sample_poly(0, 11, K, d.data_ptr(), stream())
+    t = from_dev(d)
+    vals = centred(t[0], moduli[0])
+    assert set(np.unique(vals)) <= {-1, 0, 1}
+    for i, q in enumerate(moduli):
+        assert np.array_equal(centred(t[i], q), vals)
+    assert abs(np.mean(vals == 0) - 1 / 3) < 0.03
+    ctx.sample_poly(1, 12, K, d.data_ptr(), stream())
+    e = centred(from_dev(d)[0], moduli[0]).astype(np.int64)
+    assert np.abs(e).max() <= 21 and abs(e.var() - 10.5) < 1.0
+    ctx.sample_poly(2, 13, K, d.data_ptr(), stream())
+    u = from_dev(d)
+    for i, q in enumerate(moduli):
+        assert u[i].max() < q and u[i].min() >= 0
+    # a seed reproduces its words
+    d2 = empty_dev((K, n))
+    ctx.sample_poly(2, 13, K, d2.data_ptr(), stream())
+    assert np.array_equal(from_dev(d2), u)
+
+
+@pytest.mark.parametrize("logn,bits,L", [(13, (43, 43, 44, 44, 44), 4), (13, (43, 43, 44, 44, 44), 2),
+                                         (10, (40, 40, 50), 2)])
+def test_decrypt_and_batch_words_match_oracle(logn, bits, L):
+    ctx, moduli = bfv_ctx(logn, bits)
+    n = ctx.n
+    rng = np.random.default_rng(logn + L)
+    sk = ck.ntt_rns(ck.rns_of_small(rng.integers(-1, 2, size=n), moduli), moduli, logn)
+    ct = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    out = empty_dev((n,))
+    w = ctx.decrypt_workspace_size(L)
+    wb = ws(w)
+    ctx.decrypt(to_dev(ct).data_ptr(), 2, to_dev(sk).data_ptr(), L, out.data_ptr(), wb.data_ptr(), w, stream())
+    np.testing.assert_array_equal(from_dev(out), ck.bfv_decrypt(ct, sk[:L], moduli, T, logn))
+    values = rng.integers(0, T, size=n, dtype=np.uint64)
+    plain = empty_dev((n,))
+    ctx.batch_encode(to_dev(values).data_ptr(), n, plain.data_ptr(), stream())
+    np.testing.assert_array_equal(from_dev(plain), ck.batch_encode(values, T, logn))
+    back, scratch = empty_dev((n,)), empty_dev((n,))
+    ctx.batch_decode(plain.data_ptr(), back.data_ptr(), scratch.data_ptr(), stream())
+    np.testing.assert_array_equal(from_dev(back), values)
+
+
+def test_ckks_decrypt_words_match_oracle():
+    logn = 12
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [60, 40, 40, 60])
+    ctx = hip().Context(CKKS, n, moduli)
+    rng = np.random.default_rng(5)
+    sk = ck.ntt_rns(ck.rns_of_small(rng.integers(-1, 2, size=n), moduli), moduli, logn)
+    for size in (2, 3):
+        ct = ck.uniform_residues(rng, (size, 3, n), moduli[:3])
+        out = empty_dev((3, n))
+        ctx.decrypt(to_dev(ct).data_ptr(), size, to_dev(sk).data_ptr(), 3, out.data_ptr(), 0, 0, stream())
+        np.testing.assert_array_equal(from_dev(out), ck.decrypt_phase(CKKS, ct, sk[:3], moduli, logn))
+
+
+@pytest.mark.parametrize("scheme", [BFV, CKKS])
+def test_encrypt_words_match_oracle_on_same_randomness(scheme):
+    logn = 12
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [50, 40, 40, 50] if scheme == CKKS else [43, 43, 44, 44])
+    ctx = hip().Context(scheme, n, moduli, T if scheme == BFV else 0)
+    K = len(moduli)
+    sk, pk = empty_dev((K, n)), empty_dev((2, K, n))
+    ctx.keygen_secret(1, sk.data_ptr(), stream())
+    ctx.keygen_public(sk.data_ptr(), 2, pk.data_ptr(), stream())
+    for L in (K - 1, 1):
+        L1 = L + 1
+        u, e0, e1 = empty_dev((L1, n)), empty_dev((L1, n)), empty_dev((L1, n))
+        ctx.sample_poly(0, 21, L1, u.data_ptr(), stream())
+        ctx.sample_poly(1, 22, L1, e0.data_ptr(), stream())
+        ctx.sample_poly(1, 23, L1, e1.data_ptr(), stream())
+        out = empty_dev((2, L, n))
+        w = ctx.encrypt_workspace_size(L)
+        wb = ws(w)
+        ctx.encrypt_zero_asymmetric(pk.data_ptr(), u.data_ptr(), e0.data_ptr(), e1.data_ptr(), L, out.data_ptr(),
+                                    wb.data_ptr(), w, stream())
+        z = ck.encrypt_zero_asymmetric(scheme, from_dev(pk), from_dev(u), from_dev(e0), from_dev(e1),
+                                       moduli[:L1], logn)
+        want = ck.rescale(scheme, z, moduli[:L1], logn)
+        np.testing.assert_array_equal(from_dev(out), want)
+
+
+def test_keys_have_the_kswitch_structure():
+    """Each relinearization key digit decrypts to P * s^2 (limb j) minus a CBD noise."""
+    logn = 11
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [40, 40, 40, 50])
+    ctx = hip().Context(CKKS, n, moduli)
+    K = len(moduli)
+    sk, rk = empty_dev((K, n)), empty_dev((K - 1, 2, K, n))
+    ctx.keygen_secret(3, sk.data_ptr(), stream())
+    w = ctx.keygen_workspace_size()
+    wb = ws(w)
+    ctx.keygen_relin(sk.data_ptr(), 4, rk.data_ptr(), wb.data_ptr(), w, stream())
+    s = from_dev(sk)
+    s2 = (s.astype(object) * s.astype(object) % np.array(moduli, dtype=object)[:, None]).astype(np.uint64)
+    key = from_dev(rk)
+    P = moduli[-1]
+    for j in range(K - 1):
+        phase = ck.ntt_rns(ck.decrypt_phase(CKKS, key[j], s, moduli, logn), moduli, logn, inverse=True)
+        s2c = ck.ntt_rns(s2, moduli, logn, inverse=True)
+        for i, q in enumerate(moduli):
+            noise = phase[i].astype(object) - (P * s2c[i].astype(object) if i == j else 0)
+            noise = centred(np.mod(noise, q), q)
+            assert np.abs(noise.astype(np.int64)).max() <= 21
+
+
+def test_bfv_pipeline_end_to_end():
+    """encode -> encrypt -> multiply -> relinearize -> decrypt -> decode (config-1 parameters)."""
+    ctx, moduli = bfv_ctx()
+    n, K = ctx.n, len(moduli)
+    L = K - 1
+    sk, pk, rk = empty_dev((K, n)), empty_dev((2, K, n)), empty_dev((K - 1, 2, K, n))
+    ctx.keygen_secret(101, sk.data_ptr(), stream())
+    ctx.keygen_public(sk.data_ptr(), 102, pk.data_ptr(), stream())
+    kw = ctx.keygen_workspace_size()
+    kb = ws(kw)
+    ctx.keygen_relin(sk.data_ptr(), 103, rk.data_ptr(), kb.data_ptr(), kw, stream())
+    rng = np.random.default_rng(1)
+    x = rng.integers(0, T, size=n, dtype=np.uint64)
+    y = rng.integers(0, T, size=n, dtype=np.uint64)
+    px, py = empty_dev((n,)), empty_dev((n,))
+    ctx.batch_encode(to_dev(x).data_ptr(), n, px.data_ptr(), stream())
+    ctx.batch_encode(to_dev(y).data_ptr(), n, py.data_ptr(), stream())
+    cts = empty_dev((2, 2, L, n))
+    ew = ctx.encrypt_workspace_size(L)
+    eb = ws(ew)
+    ctx.encrypt(pk.data_ptr(), px.data_ptr(), L, 7, cts[0].data_ptr(), eb.data_ptr(), ew, stream())
+    ctx.encrypt(pk.data_ptr(), py.data_ptr(), L, 8, cts[1].data_ptr(), eb.data_ptr(), ew, stream())
+    prod = empty_dev((1, 3, L, n))
+    mw = ctx.multiply_workspace_size(1, L)
+    mb = ws(mw)
+    ctx.multiply(cts[0].data_ptr(), cts[1].data_ptr(), prod.data_ptr(), 1, L, mb.data_ptr(), mw, stream())
+    # decrypting the size-3 product already works (phase with s^2)
+    dw = ctx.decrypt_workspace_size(L)
+    db = ws(dw)
+    out, vals, scratch = empty_dev((n,)), empty_dev((n,)), empty_dev((n,))
+    ctx.decrypt(prod.data_ptr(), 3, sk.data_ptr(), L, out.data_ptr(), db.data_ptr(), dw, stream())
+    ctx.batch_decode(out.data_ptr(), vals.data_ptr(), scratch.data_ptr(), stream())
+    want = (x.astype(object) * y.astype(object) % T).astype(np.uint64)
+    np.testing.assert_array_equal(from_dev(vals), want)
+    kw2 = ctx.keyswitch_workspace_size(1, L)
+    kb2 = ws(kw2)
+    ctx.relinearize(prod.data_ptr(), rk.data_ptr(), 1, L, kb2.data_ptr(), kw2, stream())
+    ctx.decrypt(prod.data_ptr(), 2, sk.data_ptr(), L, out.data_ptr(), db.data_ptr(), dw, stream())
+    ctx.batch_decode(out.data_ptr(), vals.data_ptr(), scratch.data_ptr(), stream())
+    np.testing.assert_array_equal(from_dev(vals), want)
+    # the GPU ciphertext words after multiply+relinearize equal the oracle's on the same inputs
+    c0 = from_dev(cts)
+    o3 = ck.bfv_multiply(c0[0], c0[1], moduli[:L], T, 13)
+    o2 = ck.relinearize(BFV, o3, from_dev(rk), moduli, L, 13)
+    np.testing.assert_array_equal(from_dev(prod)[0, :2], o2)
+
+
+def test_bfv_rotate_rows_end_to_end():
+    ctx, moduli = bfv_ctx(12, (43, 43, 44, 44))
+    n, K = ctx.n, len(moduli)
+    L = K - 1
+    sk, pk, gk = empty_dev((K, n)), empty_dev((2, K, n)), empty_dev((K - 1, 2, K, n))
+    ctx.keygen_secret(201, sk.data_ptr(), stream())
+    ctx.keygen_public(sk.data_ptr(), 202, pk.data_ptr(), stream())
+    kw = ctx.keygen_workspace_size()
+    kb = ws(kw)
+    elt = ctx.galois_elt_from_step(1)
+    ctx.keygen_galois(sk.data_ptr(), elt, 203, gk.data_ptr(), kb.data_ptr(), kw, stream())
+    rng = np.random.default_rng(2)
+    x = rng.integers(0, T, size=n, dtype=np.uint64)
+    px = empty_dev((n,))
+    ctx.batch_encode(to_dev(x).data_ptr(), n, px.data_ptr(), stream())
+    ct = empty_dev((1, 2, L, n))
+    ew = ctx.encrypt_workspace_size(L)
+    eb = ws(ew)
+    ctx.encrypt(pk.data_ptr(), px.data_ptr(), L, 9, ct.data_ptr(), eb.data_ptr(), ew, stream())
+    gw = ctx.galois_workspace_size(1, L)
+    gb = ws(gw)
+    ctx.apply_galois_keyswitch(ct.data_ptr(), gk.data_ptr(), elt, 1, L, gb.data_ptr(), gw, stream())
+    dw = ctx.decrypt_workspace_size(L)
+    db = ws(dw)
+    out, vals, scratch = empty_dev((n,)), empty_dev((n,)), empty_dev((n,))
+    ctx.decrypt(ct.data_ptr(), 2, sk.data_ptr(), L, out.data_ptr(), db.data_ptr(), dw, stream())
+    ctx.batch_decode(out.data_ptr(), vals.data_ptr(), scratch.data_ptr(), stream())
+    got = from_dev(vals)
+    row = n // 2
+    # rotate_rows(1): each row of the 2 x N/2 matrix shifts left by one
+    want = np.concatenate([np.roll(x[:row], -1), np.roll(x[row:], -1)])
+    np.testing.assert_array_equal(got, want)
