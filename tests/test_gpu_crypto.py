@@ -69,11 +69,13 @@ def test_decrypt_and_batch_words_match_oracle(logn, bits, L):
     out = empty_dev((n,))
     w = ctx.decrypt_workspace_size(L)
     wb = ws(w)
-    ctx.decrypt(to_dev(ct).data_ptr(), 2, to_dev(sk).data_ptr(), L, out.data_ptr(), wb.data_ptr(), w, stream())
+    dct, dsk = to_dev(ct), to_dev(sk)  # keep the tensors alive across the call
+    ctx.decrypt(dct.data_ptr(), 2, dsk.data_ptr(), L, out.data_ptr(), wb.data_ptr(), w, stream())
     np.testing.assert_array_equal(from_dev(out), ck.bfv_decrypt(ct, sk[:L], moduli, T, logn))
     values = rng.integers(0, T, size=n, dtype=np.uint64)
     plain = empty_dev((n,))
-    ctx.batch_encode(to_dev(values).data_ptr(), n, plain.data_ptr(), stream())
+    dv = to_dev(values)
+    ctx.batch_encode(dv.data_ptr(), n, plain.data_ptr(), stream())
     np.testing.assert_array_equal(from_dev(plain), ck.batch_encode(values, T, logn))
     back, scratch = empty_dev((n,)), empty_dev((n,))
     ctx.batch_decode(plain.data_ptr(), back.data_ptr(), scratch.data_ptr(), stream())
@@ -90,7 +92,8 @@ def test_ckks_decrypt_words_match_oracle():
     for size in (2, 3):
         ct = ck.uniform_residues(rng, (size, 3, n), moduli[:3])
         out = empty_dev((3, n))
-        ctx.decrypt(to_dev(ct).data_ptr(), size, to_dev(sk).data_ptr(), 3, out.data_ptr(), 0, 0, stream())
+        dct, dsk = to_dev(ct), to_dev(sk)
+        ctx.decrypt(dct.data_ptr(), size, dsk.data_ptr(), 3, out.data_ptr(), 0, 0, stream())
         np.testing.assert_array_equal(from_dev(out), ck.decrypt_phase(CKKS, ct, sk[:3], moduli, logn))
 
 
@@ -161,8 +164,9 @@ def test_bfv_pipeline_end_to_end():
     x = rng.integers(0, T, size=n, dtype=np.uint64)
     y = rng.integers(0, T, size=n, dtype=np.uint64)
     px, py = empty_dev((n,)), empty_dev((n,))
-    ctx.batch_encode(to_dev(x).data_ptr(), n, px.data_ptr(), stream())
-    ctx.batch_encode(to_dev(y).data_ptr(), n, py.data_ptr(), stream())
+    dx, dy = to_dev(x), to_dev(y)
+    ctx.batch_encode(dx.data_ptr(), n, px.data_ptr(), stream())
+    ctx.batch_encode(dy.data_ptr(), n, py.data_ptr(), stream())
     cts = empty_dev((2, 2, L, n))
     ew = ctx.encrypt_workspace_size(L)
     eb = ws(ew)
@@ -207,7 +211,8 @@ def test_bfv_rotate_rows_end_to_end():
     rng = np.random.default_rng(2)
     x = rng.integers(0, T, size=n, dtype=np.uint64)
     px = empty_dev((n,))
-    ctx.batch_encode(to_dev(x).data_ptr(), n, px.data_ptr(), stream())
+    dx = to_dev(x)
+    ctx.batch_encode(dx.data_ptr(), n, px.data_ptr(), stream())
     ct = empty_dev((1, 2, L, n))
     ew = ctx.encrypt_workspace_size(L)
     eb = ws(ew)
