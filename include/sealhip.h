@@ -190,6 +190,17 @@ int shc_batch_encode(const shc_context *ctx, const uint64_t *d_values, uint32_t 
 int shc_batch_decode(const shc_context *ctx, const uint64_t *d_plain, uint64_t *d_values, uint64_t *d_scratch,
                      void *stream);
 
+/* CKKSEncoder on the device (fp64): d_values holds `count` (re, im) double pairs,
+ * count <= N/2; d_plain is the NTT-form plaintext [L][N] at the level with
+ * limb_count primes.  Encode rounds to integers (coefficients up to 128 bits).
+ * Replaces CKKSEncoderCuda::encode / decode (src/ckks_cuda.cuh:21-84). */
+int shc_ckks_encode_workspace_size(const shc_context *ctx, size_t *bytes);
+int shc_ckks_encode(const shc_context *ctx, const double *d_values, uint32_t count, double scale, uint32_t limb_count,
+                    uint64_t *d_plain, void *d_workspace, size_t workspace_bytes, void *stream);
+int shc_ckks_decode_workspace_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
+int shc_ckks_decode(const shc_context *ctx, const uint64_t *d_plain, uint32_t limb_count, double scale,
+                    double *d_values, void *d_workspace, size_t workspace_bytes, void *stream);
+
 int shc_rescale_workspace_size(const shc_context *ctx, uint32_t count, uint32_t poly_count, uint32_t limb_count,
                                size_t *bytes);
 

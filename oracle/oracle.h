@@ -129,6 +129,13 @@ void or_batch_index_map(int logn, uint64_t *map);
 int or_batch_encode(int logn, uint64_t t, const uint64_t *values, size_t count, uint64_t *plain);
 int or_batch_decode(int logn, uint64_t t, const uint64_t *plain, uint64_t *values);
 
+/* ---------------------------------------------------------------- CKKS encoder (ckks_encoder.c)
+ * Pinned by: the reference CKKSEncoder (oracle/_ref): encode words equal up to rare
+ * +-1 rounding cases, decode within 2^-30 relative; round trip encode -> decode.     */
+int or_ckks_encode(int logn, const uint64_t *moduli, size_t L, const double *values, size_t count, double scale,
+                   uint64_t *out);
+int or_ckks_decode(int logn, const uint64_t *moduli, size_t L, const uint64_t *plain, double scale, double *values);
+
 /* Evaluator::modSwitchScaleToNext (src/evaluator.cpp:1165-1240): in [polys][L][N] -> out [polys][L-1][N] */
 int or_rescale_to_next(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *in, size_t polys,
                        uint64_t *out);

@@ -2,6 +2,7 @@
 #include "../../include/sealhip.h"
 #include "context.h"
 #include "ntt.h"
+#include "ckks_encoder.h"
 #include "crypto.h"
 #include "galois.h"
 #include "rns_ops.h"
@@ -279,6 +280,36 @@ int shc_batch_encode(const shc_context *ctx, const uint64_t *values, uint32_t co
 int shc_batch_decode(const shc_context *ctx, const uint64_t *plain, uint64_t *values, uint64_t *scratch,
                      void *stream) {
     return guarded([&] { shc::batch_decode(ctx_of(ctx), plain, values, scratch, (hipStream_t)stream); });
+}
+
+int shc_ckks_encode_workspace_size(const shc_context *ctx, size_t *bytes) {
+    return guarded([&] { *bytes = shc::ckks_encode_workspace(ctx_of(ctx)); });
+}
+
+int shc_ckks_encode(const shc_context *ctx, const double *values, uint32_t count, double scale, uint32_t limbs,
+                    uint64_t *plain, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ckks_encode(c, values, count, scale, limbs, plain, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_ckks_decode_workspace_size(const shc_context *ctx, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = shc::ckks_decode_workspace(c, limbs);
+    });
+}
+
+int shc_ckks_decode(const shc_context *ctx, const uint64_t *plain, uint32_t limbs, double scale, double *values,
+                    void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ckks_decode(c, plain, limbs, scale, values, ws, ws_bytes, (hipStream_t)stream);
+    });
 }
 
 int shc_rescale_workspace_size(const shc_context *ctx, uint32_t count, uint32_t polys, uint32_t limbs, size_t *bytes) {

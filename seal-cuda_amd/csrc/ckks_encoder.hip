@@ -1,0 +1,282 @@
+// ckks_encoder.hip -- CKKS canonical-embedding encode / decode in fp64 on gfx950.
+//
+// Same algorithm as CKKSEncoder (src/ckks.cpp:98-277 encode, :474-580 decode; GPU
+// twin src/ckks_cuda.cu:314-453, 900-971): conjugate-symmetric scatter through the
+// slot index map, inverse DWT over the complex 2N-th roots with scale/N folded into
+// the last layer, rounding, reduction into each limb, NTT; decode runs the inverse.
+// Decode composes the CRT value without multi-precision loops: the wrap count
+// k = round(sum_j y_j / q_j) comes from a double sum (exact while |x| << Q) and the
+// centred integer x = sum_j y_j Qhat_j - k Q is formed exactly modulo 2^128.
+#include "ckks_encoder.h"
+#include "modarith.h"
+#include "ntt.h"
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+namespace shc {
+namespace {
+
+constexpr int TPB = 256;
+
+inline uint32_t blocks_for(size_t n) {
+    const size_t b = (n + TPB - 1) / TPB;
+    return (uint32_t)(b < (size_t(1) << 20) ? b : (size_t(1) << 20));
+}
+
+// slot -> coefficient position (CKKSEncoder matrix_reps_index_map_, src/ckks.cpp:44-56)
+__device__ __forceinline__ uint32_t slot_pos(uint32_t i, int logn) {
+    const uint64_t n = 1ull << logn, slots = n >> 1, mask = 2 * n - 1;
+    const uint32_t e = i < slots ? i : i - (uint32_t)slots;
+    uint64_t pos = 1, b = 3;
+    for (uint32_t k = e; k; k >>= 1) {
+        if (k & 1) pos = (pos * b) & mask;
+        b = (b * b) & mask;
+    }
+    const uint64_t idx = i < slots ? (pos - 1) >> 1 : (2 * n - pos - 1) >> 1;
+    return logn ? __brev((uint32_t)idx) >> (32 - logn) : 0;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ void scatter_kernel(const double *__restrict__ values, uint32_t count, double2 *__restrict__ a, int logn) {
+    const uint32_t slots = 1u << (logn - 1);
+    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < slots; i += gridDim.x * TPB) {
+        double2 v = make_double2(0.0, 0.0);
+        if (i < count) v = make_double2(values[2 * i], values[2 * i + 1]);
+        a[slot_pos(i, logn)] = v;
+        a[slot_pos(i + slots, logn)] = make_double2(v.x, -v.y);
+    }
+}
+
+// one Gentleman-Sande layer of DWTHandler::transformFromRev with m groups of size 2*gap
+__global__ void fft_inv_layer(double2 *__restrict__ a, const double2 *__restrict__ r, uint32_t m, uint32_t gap,
+                              uint32_t ridx, double scalar, int last, uint32_t half) {
+    for (uint32_t b = blockIdx.x * TPB + threadIdx.x; b < half; b += gridDim.x * TPB) {
+        const uint32_t i = b / gap, j = b % gap;
+        double2 *x = a + 2 * gap * i + j, *y = x + gap;
+        const double2 u = *x, v = *y;
+        double2 w = r[ridx + i];
+        double2 s = make_double2(u.x + v.x, u.y + v.y), d = make_double2(u.x - v.x, u.y - v.y);
+        if (last) {
+            w = make_double2(w.x * scalar, w.y * scalar);
+            s = make_double2(s.x * scalar, s.y * scalar);
+        }
+        *x = s;
+        *y = cmul(d, w);
+    }
+}
+
+// one Cooley-Tukey layer of DWTHandler::transformToRev
+__global__ void fft_fwd_layer(double2 *__restrict__ a, const double2 *__restrict__ r, uint32_t gap, uint32_t ridx,
+                              uint32_t half) {
+    for (uint32_t b = blockIdx.x * TPB + threadIdx.x; b < half; b += gridDim.x * TPB) {
+        const uint32_t i = b / gap, j = b % gap;
+        double2 *x = a + 2 * gap * i + j, *y = x + gap;
+        const double2 u = *x, v = cmul(*y, r[ridx + i]);
+        *x = make_double2(u.x + v.x, u.y + v.y);
+        *y = make_double2(u.x - v.x, u.y - v.y);
+    }
+}
+
+// round the real parts, reduce into each limb; record max |coefficient| (positive
+// doubles order like their bit patterns, so an integer atomicMax works)
+__global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__restrict__ out,
+                                    const DevPrime *__restrict__ primes, uint32_t L, int logn,
+                                    unsigned long long *maxbits) {
+    const size_t n = (size_t)1 << logn;
+    const double two64 = 18446744073709551616.0;
+    for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < n; c += (size_t)gridDim.x * TPB) {
+        double d = rint(a[c].x);
+        const bool neg = signbit(d);
+        d = fabs(d);
+        atomicMax(maxbits, (unsigned long long)__double_as_longlong(d));
+        if (d >= two64 * two64) d = 0;  // rejected on the host by the range check
+        const uint64_t hi = (uint64_t)(d / two64);
+        const uint64_t lo = (uint64_t)fmod(d, two64);
+        for (uint32_t l = 0; l < L; l++) {
+            const DevPrime &P = primes[l];
+            const uint64_t r = barrett128(u128{lo, hi}, P.q, P.ratio_lo, P.ratio_hi);
+            out[l * n + c] = neg ? negmod(r, P.q) : r;
+        }
+    }
+}
+
+__global__ void compose_kernel(const uint64_t *__restrict__ c, double2 *__restrict__ a, CkksLevelDev T,
+                               const DevPrime *__restrict__ primes, int logn, double inv_scale) {
+    const size_t n = (size_t)1 << logn;
+    const double two64 = 18446744073709551616.0;
+    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
+        double f = 0.0;
+        uint64_t lo = 0, hi = 0;
+        for (uint32_t j = 0; j < T.L; j++) {
+            const uint64_t y = shoup(c[j * n + i], T.qhat_inv[2 * j], T.qhat_inv[2 * j + 1], primes[j].q);
+            f += (double)y * T.inv_q[j];
+            // (lo, hi) += y * qhat_j mod 2^128
+            const u128 p = mul64x64(y, T.qhat128[2 * j]);
+            const uint64_t nlo = lo + p.lo;
+            hi += p.hi + y * T.qhat128[2 * j + 1] + (nlo < lo ? 1ull : 0ull);
+            lo = nlo;
+        }
+        const uint64_t k = (uint64_t)llrint(f);
+        // x = acc - k Q (mod 2^128), read as a signed 128-bit integer
+        const u128 kq = mul64x64(k, T.Q_lo);
+        const uint64_t kq_hi = kq.hi + k * T.Q_hi;
+        const uint64_t xlo = lo - kq.lo;
+        const uint64_t xhi = hi - kq_hi - (lo < kq.lo ? 1ull : 0ull);
+        double v;
+        if (xhi >> 63) {
+            const uint64_t nlo = ~xlo + 1, nhi = ~xhi + (nlo == 0 ? 1ull : 0ull);
+            v = -((double)nhi * two64 + (double)nlo);
+        } else {
+            v = (double)xhi * two64 + (double)xlo;
+        }
+        a[i] = make_double2(v * inv_scale, 0.0);
+    }
+}
+
+__global__ void gather_kernel(const double2 *__restrict__ a, double *__restrict__ values, int logn) {
+    const uint32_t slots = 1u << (logn - 1);
+    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < slots; i += gridDim.x * TPB) {
+        const double2 v = a[slot_pos(i, logn)];
+        values[2 * i] = v.x;
+        values[2 * i + 1] = v.y;
+    }
+}
+
+// ComplexRoots::getRoot (src/utils/croots.cpp:31-60)
+std::complex<double> root_of(size_t m, size_t index) {
+    index &= m - 1;
+    if (index <= m / 8) return std::polar(1.0, 2 * M_PI * (double)index / (double)m);
+    if (index <= m / 4) {
+        auto r = root_of(m, m / 4 - index);
+        return {r.imag(), r.real()};
+    }
+    if (index <= m / 2) return -std::conj(root_of(m, m / 2 - index));
+    if (index <= 3 * m / 4) return -root_of(m, index - m / 2);
+    return std::conj(root_of(m, m - index));
+}
+
+}  // namespace
+
+void build_ckks_tables(Context &c) {
+    auto t = std::make_shared<CkksTables>();
+    const size_t n = c.n, m = 2 * n;
+    const int logn = c.logn;
+    std::vector<double> rt(4 * n, 0.0);  // [rp re, im]... then [irp]
+    auto put = [&](size_t base, size_t i, std::complex<double> v) {
+        rt[2 * (base + i)] = v.real();
+        rt[2 * (base + i) + 1] = v.imag();
+    };
+    put(0, 0, 1.0);
+    put(n, 0, 1.0);
+    if (m >= 8) {
+        for (size_t i = 1; i < n; i++) {
+            put(0, i, root_of(m, hm::reverse_bits((uint32_t)i, logn)));
+            put(n, i, std::conj(root_of(m, hm::reverse_bits((uint32_t)(i - 1), logn) + 1)));
+        }
+    } else if (m == 4) {
+        put(0, 1, {0, 1});
+        put(n, 1, {0, -1});
+    }
+    t->roots = DevBuf(rt.size() * 8);
+    SHC_HIP(hipMemcpy(t->roots.ptr, rt.data(), rt.size() * 8, hipMemcpyHostToDevice));
+    const size_t K = c.moduli.size();
+    t->levels.resize(K + 1);
+    for (size_t L = 1; L <= K; L++) {
+        std::vector<uint64_t> q(c.moduli.begin(), c.moduli.begin() + L);
+        std::vector<uint64_t> words;
+        std::vector<double> invq;
+        const hm::Big Q = hm::big_product(q);
+        for (size_t j = 0; j < L; j++) {
+            const hm::Big qh = hm::big_product_except(q, j);
+            const uint64_t inv = hm::invmod(hm::big_mod_word(qh, q[j]), q[j]);
+            words.push_back(inv);
+            words.push_back(hm::shoup(inv, q[j]));
+        }
+        for (size_t j = 0; j < L; j++) {
+            const hm::Big qh = hm::big_product_except(q, j);
+            words.push_back(qh.size() > 0 ? qh[0] : 0);
+            words.push_back(qh.size() > 1 ? qh[1] : 0);
+            invq.push_back(1.0 / (double)q[j]);
+        }
+        DevBuf b(words.size() * 8 + invq.size() * 8);
+        SHC_HIP(hipMemcpy(b.ptr, words.data(), words.size() * 8, hipMemcpyHostToDevice));
+        SHC_HIP(hipMemcpy(static_cast<char *>(b.ptr) + words.size() * 8, invq.data(), invq.size() * 8,
+                          hipMemcpyHostToDevice));
+        CkksLevelDev &d = t->levels[L];
+        d.L = (uint32_t)L;
+        d.Q_lo = Q.size() > 0 ? Q[0] : 0;
+        d.Q_hi = Q.size() > 1 ? Q[1] : 0;
+        d.qhat_inv = b.as<const uint64_t>();
+        d.qhat128 = b.as<const uint64_t>() + 2 * L;
+        d.inv_q = reinterpret_cast<const double *>(static_cast<char *>(b.ptr) + words.size() * 8);
+        d.total_bits = hm::big_bit_count(Q);
+        t->bufs.push_back(std::move(b));
+    }
+    c.ckks = t;
+}
+
+size_t ckks_encode_workspace(const Context &c) { return c.n * 16 + 256; }
+
+void ckks_encode(const Context &c, const double *values, uint32_t count, double scale, uint32_t L, uint64_t *plain,
+                 void *ws, size_t ws_bytes, hipStream_t st) {
+    if (c.scheme != SCHEME_CKKS || !c.ckks) fail(ERR_INVALID_ARGUMENT, "unsupported scheme");
+    if (c.logn < 1) fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree too small");
+    if (count > c.n / 2) fail(ERR_INVALID_ARGUMENT, "values_size is too large");
+    if (!ws || ws_bytes < ckks_encode_workspace(c)) fail(ERR_INVALID_ARGUMENT, "workspace too small for encoding");
+    const CkksLevelDev &T = c.ckks->levels[L];
+    // CKKSEncoder::encodeInternal scale check (src/ckks.cpp:120-123)
+    if (!(scale > 0) || (int)std::log2(scale) + 1 >= T.total_bits) fail(ERR_INVALID_ARGUMENT, "scale out of bounds");
+    const size_t n = c.n;
+    double2 *a = static_cast<double2 *>(ws);
+    auto *maxbits = reinterpret_cast<unsigned long long *>(static_cast<char *>(ws) + n * 16);
+    const double2 *irp = c.ckks->roots.as<const double2>() + n;
+    scatter_kernel<<<blocks_for(n / 2), TPB, 0, st>>>(values, count, a, c.logn);
+    uint32_t gap = 1, ridx = 1;
+    for (uint32_t m = (uint32_t)(n >> 1); m >= 1; m >>= 1, gap <<= 1) {
+        fft_inv_layer<<<blocks_for(n / 2), TPB, 0, st>>>(a, irp, m, gap, ridx, scale / (double)n, m == 1,
+                                                         (uint32_t)(n / 2));
+        ridx += m;
+    }
+    SHC_HIP(hipMemsetAsync(maxbits, 0, 8, st));
+    round_reduce_kernel<<<blocks_for(n), TPB, 0, st>>>(a, plain, c.primes(), L, c.logn, maxbits);
+    launch_check("ckks encode");
+    unsigned long long mb = 0;
+    SHC_HIP(hipMemcpyAsync(&mb, maxbits, 8, hipMemcpyDeviceToHost, st));
+    SHC_HIP(hipStreamSynchronize(st));
+    double maxc;
+    std::memcpy(&maxc, &mb, 8);
+    const int bits = (int)std::ceil(std::log2(std::max(maxc, 1.0))) + 1;
+    if (bits >= T.total_bits || bits > 128) fail(ERR_INVALID_ARGUMENT, "encoded values are too large");
+    ntt_forward(plain, L, L, c.primes(), c.logn, NTT_FULL, st);
+}
+
+size_t ckks_decode_workspace(const Context &c, uint32_t L) { return (size_t)L * c.n * 8 + c.n * 16 + 256; }
+
+void ckks_decode(const Context &c, const uint64_t *plain, uint32_t L, double scale, double *values, void *ws,
+                 size_t ws_bytes, hipStream_t st) {
+    if (c.scheme != SCHEME_CKKS || !c.ckks) fail(ERR_INVALID_ARGUMENT, "unsupported scheme");
+    if (!ws || ws_bytes < ckks_decode_workspace(c, L)) fail(ERR_INVALID_ARGUMENT, "workspace too small for decoding");
+    const CkksLevelDev &T = c.ckks->levels[L];
+    if (!(scale > 0) || (int)std::log2(scale) >= T.total_bits) fail(ERR_INVALID_ARGUMENT, "scale out of bounds");
+    const size_t n = c.n;
+    uint64_t *cp = static_cast<uint64_t *>(ws);
+    double2 *a = reinterpret_cast<double2 *>(static_cast<char *>(ws) + (((size_t)L * n * 8 + 255) & ~size_t(255)));
+    SHC_HIP(hipMemcpyAsync(cp, plain, (size_t)L * n * 8, hipMemcpyDeviceToDevice, st));
+    ntt_inverse(cp, L, L, c.primes(), c.logn, NTT_FULL, st);
+    compose_kernel<<<blocks_for(n), TPB, 0, st>>>(cp, a, T, c.primes(), c.logn, 1.0 / scale);
+    const double2 *rp = c.ckks->roots.as<const double2>();
+    uint32_t gap = (uint32_t)(n >> 1), ridx = 1;
+    for (uint32_t m = 1; m < n; m <<= 1, gap >>= 1) {
+        fft_fwd_layer<<<blocks_for(n / 2), TPB, 0, st>>>(a, rp, gap, ridx, (uint32_t)(n / 2));
+        ridx += m;
+    }
+    gather_kernel<<<blocks_for(n / 2), TPB, 0, st>>>(a, values, c.logn);
+    launch_check("ckks decode");
+}
+
+}  // namespace shc

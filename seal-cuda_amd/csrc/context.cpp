@@ -1,5 +1,6 @@
 // context.cpp -- builds and uploads the per-parameter-set tables.
 #include "context.h"
+#include "ckks_encoder.h"
 #include <cstring>
 
 namespace shc {
@@ -106,6 +107,7 @@ std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector
             c->d_plain_prime = upload(tp);
         }
     }
+    if (scheme == SCHEME_CKKS) build_ckks_tables(*c);
     return c;
 }
 

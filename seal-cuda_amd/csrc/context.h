@@ -19,6 +19,8 @@ namespace shc {
 
 enum Scheme { SCHEME_NONE = 0, SCHEME_BFV = 1, SCHEME_CKKS = 2, SCHEME_BGV = 3 };
 
+struct CkksTables;
+
 struct Context {
     Scheme scheme = SCHEME_NONE;
     int logn = 0;
@@ -36,6 +38,7 @@ struct Context {
     std::map<uint64_t, DevPrime> aux;  // NTT-ready auxiliary primes (Bsk, t) by value
     std::vector<std::unique_ptr<BehzLevel>> behz_levels;  // index L (BFV)
     DevBuf d_plain_prime;              // DevPrime[1] for t (BFV, batching)
+    std::shared_ptr<CkksTables> ckks;  // CKKS encoder tables
 
     int device = 0;
 

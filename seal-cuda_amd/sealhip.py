@@ -66,6 +66,10 @@ def lib():
         L.shc_decrypt.argtypes = [_vp, _vp, u32, _vp, u32, _vp, _vp, sz, _vp]
         L.shc_batch_encode.argtypes = [_vp, _vp, u32, _vp, _vp]
         L.shc_batch_decode.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.shc_ckks_encode_workspace_size.argtypes = [_vp, ctypes.POINTER(sz)]
+        L.shc_ckks_encode.argtypes = [_vp, _vp, u32, ctypes.c_double, u32, _vp, _vp, sz, _vp]
+        L.shc_ckks_decode_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
+        L.shc_ckks_decode.argtypes = [_vp, _vp, u32, ctypes.c_double, _vp, _vp, sz, _vp]
         L.shc_rescale_workspace_size.argtypes = [_vp, u32, u32, u32, ctypes.POINTER(sz)]
         L.shc_rescale_to_next.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
         _lib = L
@@ -203,6 +207,18 @@ class Context:
 
     def batch_decode(self, d_plain, d_values, d_scratch, stream=None):
         check(lib().shc_batch_decode(self.handle, d_plain, d_values, d_scratch, stream))
+
+    def ckks_encode_workspace_size(self):
+        return self._size(lib().shc_ckks_encode_workspace_size)
+
+    def ckks_encode(self, d_values, count, scale, limbs, d_plain, d_ws, ws_bytes, stream=None):
+        check(lib().shc_ckks_encode(self.handle, d_values, count, scale, limbs, d_plain, d_ws, ws_bytes, stream))
+
+    def ckks_decode_workspace_size(self, limbs):
+        return self._size(lib().shc_ckks_decode_workspace_size, limbs)
+
+    def ckks_decode(self, d_plain, limbs, scale, d_values, d_ws, ws_bytes, stream=None):
+        check(lib().shc_ckks_decode(self.handle, d_plain, limbs, scale, d_values, d_ws, ws_bytes, stream))
 
     def rescale_workspace_size(self, count, polys, limbs):
         b = ctypes.c_size_t()

@@ -70,6 +70,9 @@ def oracle():
         lib.or_batch_index_map.argtypes = [ctypes.c_int, u64p]
         lib.or_batch_encode.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, sz, u64p]
         lib.or_batch_decode.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, u64p]
+        dpp = ctypes.POINTER(ctypes.c_double)
+        lib.or_ckks_encode.argtypes = [ctypes.c_int, u64p, sz, dpp, sz, ctypes.c_double, u64p]
+        lib.or_ckks_decode.argtypes = [ctypes.c_int, u64p, sz, u64p, ctypes.c_double, dpp]
         lib.or_behz_bases.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p,
                                       ctypes.POINTER(sz)]
         lib.or_bfv_multiply.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p]
@@ -267,6 +270,22 @@ def batch_decode(plain, t, logn):
     out = np.zeros(1 << logn, dtype=np.uint64)
     assert oracle().or_batch_decode(logn, t, ptr(np.ascontiguousarray(plain, dtype=np.uint64)), ptr(out)) == 0
     return out
+
+
+def ckks_encode(values, moduli, scale, logn):
+    """complex values (<= N/2) -> NTT-form plaintext [L][N]"""
+    v = np.ascontiguousarray(np.stack([np.real(values), np.imag(values)], axis=1).reshape(-1), dtype=np.float64)
+    out = np.zeros((len(moduli), 1 << logn), dtype=np.uint64)
+    rc = oracle().or_ckks_encode(logn, ptr(_arr(moduli)), len(moduli), dptr(v), len(values), scale, ptr(out))
+    assert rc == 0, rc
+    return out
+
+
+def ckks_decode(plain, moduli, scale, logn):
+    out = np.zeros(1 << logn, dtype=np.float64)
+    assert oracle().or_ckks_decode(logn, ptr(_arr(moduli)), len(moduli), ptr(np.ascontiguousarray(plain)), scale,
+                                   dptr(out)) == 0
+    return out[0::2] + 1j * out[1::2]
 
 
 def rns_of_small(values, moduli):

@@ -1,0 +1,162 @@
+"""GPU CKKS encoder parity and the CKKS chain end to end.
+
+Encoder: plaintext words equal the oracle's except rare coefficients whose rounding
+differs by one (double-precision DWT, the reference's own tolerance case); decode
+within 2^-30 relative of the oracle on the same plaintext.
+Chain (the CKKS chain workload's parameters: N = 2^15, 60 + 14 x 40 + 60 bits, scale
+2^40): encode -> encrypt -> 3 x (multiply -> relinearize -> rescale) -> decrypt ->
+decode equals the oracle's decode of the same words within 2^-20 absolute, every
+intermediate ciphertext word equals the oracle's on the same inputs, and the result
+is the product of the inputs up to CKKS noise.
+"""
+import numpy as np
+import pytest
+import torch
+
+import checkers as ck
+from gpu_util import empty_dev, from_dev, hip, stream, to_dev
+
+pytestmark = pytest.mark.gpu
+CKKS = 2
+
+
+def ws(nbytes):
+    return empty_dev((max(nbytes, 8) + 7) // 8)
+
+
+def to_dev_f64(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def slots(rng, count):
+    return rng.uniform(-1, 1, count) + 1j * rng.uniform(-1, 1, count)
+
+
+def interleave(v):
+    return np.stack([np.real(v), np.imag(v)], axis=1).reshape(-1)
+
+
+@pytest.mark.parametrize("logn,bits,L,scale,count", [
+    (10, [60, 40, 40, 60], 3, 2.0 ** 40, 512),
+    (13, [60, 40, 40, 40, 60], 4, 2.0 ** 40, 3000),
+    (15, [60] + [40] * 14 + [60], 15, 2.0 ** 40, 1 << 14),
+    (12, [60, 60, 60, 60], 3, 2.0 ** 80, 2048),
+])
+def test_encode_decode_parity(logn, bits, L, scale, count):
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, bits)
+    ctx = hip().Context(CKKS, n, moduli)
+    rng = np.random.default_rng(logn)
+    v = slots(rng, count)
+    dv = to_dev_f64(interleave(v))
+    plain = empty_dev((L, n))
+    ew = ctx.ckks_encode_workspace_size()
+    eb = ws(ew)
+    ctx.ckks_encode(dv.data_ptr(), count, scale, L, plain.data_ptr(), eb.data_ptr(), ew, stream())
+    got = from_dev(plain)
+    want = ck.ckks_encode(v, moduli[:L], scale, logn)
+    cg = ck.ntt_rns(got, moduli[:L], logn, inverse=True).astype(object)
+    cw = ck.ntt_rns(want, moduli[:L], logn, inverse=True).astype(object)
+    diff = np.mod(cg - cw, moduli[0])
+    diff = np.minimum(diff, moduli[0] - diff)
+    assert diff.max() <= 1 and np.mean(diff != 0) < 1e-2
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    dw = ctx.ckks_decode_workspace_size(L)
+    db = ws(dw)
+    dwant = to_dev(want)
+    ctx.ckks_decode(dwant.data_ptr(), L, scale, out.data_ptr(), db.data_ptr(), dw, stream())
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    dec = o[0::2] + 1j * o[1::2]
+    ref = ck.ckks_decode(want, moduli[:L], scale, logn)
+    assert np.max(np.abs(dec - ref) / np.maximum(1, np.abs(ref))) < 2.0 ** -30
+    assert np.max(np.abs(dec[:count] - v)) < 2.0 ** -20
+
+
+def test_encode_rejects_too_large():
+    n = 1 << 10
+    moduli = ck.coeff_modulus_create(n, [30, 30])
+    ctx = hip().Context(CKKS, n, moduli)
+    dv = to_dev_f64(interleave(np.array([1e12 + 0j])))
+    plain = empty_dev((2, n))
+    ew = ctx.ckks_encode_workspace_size()
+    eb = ws(ew)
+    with pytest.raises(hip().SealHipError):
+        ctx.ckks_encode(dv.data_ptr(), 1, 2.0 ** 40, 2, plain.data_ptr(), eb.data_ptr(), ew, stream())
+
+
+def test_ckks_chain_end_to_end():
+    logn = 15
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [60] + [40] * 14 + [60])
+    ctx = hip().Context(CKKS, n, moduli)
+    K = len(moduli)
+    L = K - 1
+    scale = 2.0 ** 40
+    sk, pk, rk = empty_dev((K, n)), empty_dev((2, K, n)), empty_dev((K - 1, 2, K, n))
+    ctx.keygen_secret(1, sk.data_ptr(), stream())
+    ctx.keygen_public(sk.data_ptr(), 2, pk.data_ptr(), stream())
+    kw = ctx.keygen_workspace_size()
+    kb = ws(kw)
+    ctx.keygen_relin(sk.data_ptr(), 3, rk.data_ptr(), kb.data_ptr(), kw, stream())
+    rng = np.random.default_rng(4)
+    x, y = slots(rng, n // 2), slots(rng, n // 2)
+    ew = ctx.ckks_encode_workspace_size()
+    eb = ws(ew)
+    cts = []
+    for vec, seed in ((x, 10), (y, 11)):
+        dv = to_dev_f64(interleave(vec))
+        p = empty_dev((L, n))
+        ctx.ckks_encode(dv.data_ptr(), n // 2, scale, L, p.data_ptr(), eb.data_ptr(), ew, stream())
+        ct = empty_dev((1, 2, L, n))
+        cw = ctx.encrypt_workspace_size(L)
+        cb = ws(cw)
+        ctx.encrypt(pk.data_ptr(), p.data_ptr(), L, seed, ct.data_ptr(), cb.data_ptr(), cw, stream())
+        cts.append(ct)
+    acc, other = cts
+    want = x.copy()
+    cur_scale = scale
+    key_words = from_dev(rk)
+    for step in range(3):
+        prod = empty_dev((1, 3, L, n))
+        ctx.ckks_multiply(acc.data_ptr(), other.data_ptr(), prod.data_ptr(), 1, L, stream())
+        a_w, b_w = from_dev(acc)[0], from_dev(other)[0]
+        kw2 = ctx.keyswitch_workspace_size(1, L)
+        kb2 = ws(kw2)
+        ctx.relinearize(prod.data_ptr(), rk.data_ptr(), 1, L, kb2.data_ptr(), kw2, stream())
+        nxt = empty_dev((1, 2, L - 1, n))
+        rw = ctx.rescale_workspace_size(1, 2, L)
+        rb = ws(rw)
+        relin = prod[:, :2].contiguous()
+        ctx.rescale_to_next(relin.data_ptr(), nxt.data_ptr(), 1, 2, L, rb.data_ptr(), rw, stream())
+        # words equal the oracle's on the same inputs
+        o3 = ck.ckks_multiply(a_w, b_w, moduli[:L], logn)
+        o2 = ck.relinearize(CKKS, o3, key_words, moduli, L, logn)
+        np.testing.assert_array_equal(from_dev(nxt)[0], ck.rescale(CKKS, o2, moduli, logn))
+        cur_scale = cur_scale * scale / moduli[L - 1]
+        want = want * y
+        L -= 1
+        acc = nxt
+        # the multiplier drops to the new level by rescaling an encryption of y at the new scale
+        if step < 2:
+            p = empty_dev((L, n))
+            dv = to_dev_f64(interleave(y))
+            ctx.ckks_encode(dv.data_ptr(), n // 2, scale, L, p.data_ptr(), eb.data_ptr(), ew, stream())
+            other = empty_dev((1, 2, L, n))
+            cw = ctx.encrypt_workspace_size(L)
+            cb = ws(cw)
+            ctx.encrypt(pk.data_ptr(), p.data_ptr(), L, 20 + step, other.data_ptr(), cb.data_ptr(), cw, stream())
+    dec = empty_dev((L, n))
+    ctx.decrypt(acc.data_ptr(), 2, sk.data_ptr(), L, dec.data_ptr(), 0, 0, stream())
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    dw = ctx.ckks_decode_workspace_size(L)
+    db = ws(dw)
+    ctx.ckks_decode(dec.data_ptr(), L, cur_scale, out.data_ptr(), db.data_ptr(), dw, stream())
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    got = o[0::2] + 1j * o[1::2]
+    # the GPU decode equals the oracle's decode of the same plaintext words within 2^-20
+    ref = ck.ckks_decode(from_dev(dec), moduli[:L], cur_scale, logn)
+    assert np.max(np.abs(got - ref)) < 2.0 ** -20
+    # and the chain computed x * y^3 (CKKS noise at scale 2^40 after three rescales)
+    assert np.max(np.abs(got - want)) < 2.0 ** -10
