@@ -57,9 +57,16 @@ def test_encode_decode_parity(logn, bits, L, scale, count):
     want = ck.ckks_encode(v, moduli[:L], scale, logn)
     cg = ck.ntt_rns(got, moduli[:L], logn, inverse=True).astype(object)
     cw = ck.ntt_rns(want, moduli[:L], logn, inverse=True).astype(object)
-    diff = np.mod(cg - cw, moduli[0])
-    diff = np.minimum(diff, moduli[0] - diff)
-    assert diff.max() <= 1 and np.mean(diff != 0) < 1e-2
+    q0 = moduli[0]
+    diff = np.mod(cg - cw, q0)
+    diff = np.minimum(diff, q0 - diff)
+    # an integer rounded from a double is fixed only to the double's precision: at
+    # scale 2^40 that means +-1 in rare near-half cases; at 2^80 the coefficients'
+    # ulp is ~2^27, so allow 2^-45 of the coefficient magnitude (~ scale * |v|)
+    tol = max(1.0, scale * 4 * 2.0 ** -45)
+    assert float(diff.max()) <= tol
+    if tol == 1.0:
+        assert np.mean(diff != 0) < 1e-2
     out = torch.empty(n, dtype=torch.float64, device="cuda")
     dw = ctx.ckks_decode_workspace_size(L)
     db = ws(dw)
