@@ -212,6 +212,15 @@ int shc_rescale_to_next(const shc_context *ctx, const uint64_t *d_in, uint64_t *
                         uint32_t poly_count, uint32_t limb_count, void *d_workspace, size_t workspace_bytes,
                         void *stream);
 
+/* Instrumentation (no counterpart in the reference; for benchmarks and profilers):
+ * arms `ctx` so that its next key switch (shc_relinearize, shc_switch_key,
+ * shc_apply_galois_keyswitch) records the caller-created hipEvent_t `begin_event`
+ * right before and `end_event` right after its ModUp rows-pass kernel -- the dominant
+ * kernel of a key switch -- on that call's stream, then disarms.  For N < 2^13 the
+ * events bracket the whole ModUp forward NTT; a batch large enough to run in slices
+ * reports its first slice.  Not safe with concurrent key switches on one context. */
+int shc_probe_keyswitch(shc_context *ctx, void *begin_event, void *end_event);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,11 @@ struct alignas(32) DivRoundConst {
     uint64_t pad;
 };
 
+// Instrumentation: a pair of caller-created events recorded around one kernel launch.
+struct KernelProbe {
+    hipEvent_t begin, end;
+};
+
 // Error raised by host code; carries the exception family of the reference
 // (std::invalid_argument / logic_error / out_of_range / runtime_error).
 enum ErrKind { ERR_OK = 0, ERR_INVALID_ARGUMENT = -1, ERR_LOGIC = -2, ERR_OUT_OF_RANGE = -3, ERR_RUNTIME = -4,

@@ -6,6 +6,8 @@
 namespace shc {
 
 DevPrime upload_prime(Context &c, uint64_t q) {
+    // the NTT's lazy ranges reach 8q, which must fit 64 bits
+    if (q >> 61) fail(ERR_INVALID_ARGUMENT, "modulus above 61 bits");
     const hm::NttTables t = hm::make_ntt_tables(c.logn, q);
     DevPrime p{};
     p.q = t.q;

@@ -42,6 +42,13 @@ struct Context {
 
     int device = 0;
 
+    // Instrumentation: when armed, the next key switch records these events around its
+    // ModUp rows-pass launch (the whole ModUp forward NTT for N < 2^13; the first
+    // slice's, for a batch run in slices) and disarms.  Not safe with concurrent key
+    // switches on one context.
+    mutable KernelProbe probe{nullptr, nullptr};
+    mutable bool probe_armed = false;
+
     size_t key_count() const { return moduli.size(); }
     const DevPrime *primes() const { return d_primes.as<DevPrime>(); }
     const DevPrime *keyswitch_primes(size_t level) const {

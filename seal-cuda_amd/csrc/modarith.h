@@ -39,6 +39,27 @@ __device__ __forceinline__ uint64_t shoup_lazy(uint64_t x, uint64_t w, uint64_t 
     return x * w - hi * q;
 }
 
+// floor(x * y / 2^64) - e with e in {0, 1, 2}: the high word without the carries of
+// the two middle partial products' low halves and of x_lo * y_lo (3 multiplies).
+__host__ __device__ __forceinline__ uint64_t mulhi64_approx(uint64_t x, uint64_t y) {
+    const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
+    const uint64_t mid = ((uint64_t)xh * yl >> 32) + ((uint64_t)xl * yh >> 32);
+    return (uint64_t)xh * yh + mid;
+}
+
+// x * w mod q in [0, 4q), for any 64-bit x, w < q < 2^62, wq = floor(w 2^64 / q) and
+// neg_q = 2^64 - q.  The quotient estimate is Shoup's minus at most 2, so the remainder
+// x w - Q q lies in [0, 4q) and equals its low 64 bits, computed as x w + Q neg_q with
+// only the low halves of the cross products (9 multiplies in all).
+__host__ __device__ __forceinline__ uint64_t shoup_lazier(uint64_t x, uint64_t w, uint64_t wq, uint64_t neg_q) {
+    const uint64_t Q = mulhi64_approx(x, wq);
+    const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), wl = (uint32_t)w, wh = (uint32_t)(w >> 32);
+    const uint32_t ql = (uint32_t)Q, qh = (uint32_t)(Q >> 32), nl = (uint32_t)neg_q, nh = (uint32_t)(neg_q >> 32);
+    const uint64_t p = (uint64_t)xl * wl + (uint64_t)ql * nl;
+    const uint32_t cross = xl * wh + xh * wl + ql * nh + qh * nl;
+    return p + ((uint64_t)cross << 32);
+}
+
 __device__ __forceinline__ uint64_t shoup(uint64_t x, uint64_t w, uint64_t wq, uint64_t q) {
     uint64_t r = shoup_lazy(x, w, wq, q);
     return r >= q ? r - q : r;

@@ -8,21 +8,36 @@ namespace shc {
 enum NttOut { NTT_FULL = 0,   // fully reduced, [0, q)
               NTT_LAZY = 1 }; // forward: [0, 4q); inverse: [0, 2q)
 
+// Limb layout of a batch: limb (i, j) is limb number i * A + j * B of the buffer and is
+// reduced by primes[i], for i < I and j < J.  The [poly][limb][coeff] layout with m
+// limbs per polynomial is {A = 1, B = m, I = m, J = polys}; a target-major key-switch
+// buffer [prime][ct][digit] is {A = cts * digits, B = 1, I = primes, J = cts * digits}.
+struct LimbGrid {
+    uint32_t A, B, I, J;
+};
+
 // In-place transform of `count` limbs stored back to back, each of length 2^logn.
 // Limb k (0 <= k < count) is reduced modulo primes[k % limbs_per_poly], i.e. the data is
-// a [poly][limb][coeff] array with `limbs_per_poly` limbs per polynomial.
-//   forward: natural order in (values < 4q), bit-reversed order out
-//   inverse: bit-reversed order in (values < 2q), natural order out, times N^-1
+// a [poly][limb][coeff] array with `limbs_per_poly` limbs per polynomial (count must be
+// a multiple of limbs_per_poly).
+//   forward: natural order in (values < 8q), bit-reversed order out
+//   inverse: bit-reversed order in (values < 4q), natural order out, times N^-1
+// Every prime must be below 2^61.
 void ntt_forward(uint64_t *data, uint32_t count, uint32_t limbs_per_poly, const DevPrime *primes, int logn,
                  NttOut out, hipStream_t stream);
 void ntt_inverse(uint64_t *data, uint32_t count, uint32_t limbs_per_poly, const DevPrime *primes, int logn,
                  NttOut out, hipStream_t stream);
+// Forward transform of the I x J limbs of a LimbGrid.
+void ntt_forward_grid(uint64_t *data, LimbGrid grid, const DevPrime *primes, int logn, NttOut out,
+                      hipStream_t stream);
 
-// Key-switching ModUp fused with the forward NTT: T[ct][j][i] = NTT_i(d[ct][j] mod t_i)
-// for digits j < L and targets i <= L (targets[i]), d[ct] at d + ct * d_ct_stride with
-// L limbs.  Output fully reduced.  Returns false (and does nothing) for N < 2^13, where
-// the caller gathers first and runs the plain transform.
+// Key-switching ModUp fused with the forward NTT: T[i][ct][j] = NTT_i(d[ct][j] mod t_i)
+// for digits j < L (primes digits[j]) and targets i <= L (targets[i]), d[ct] at
+// d + ct * d_ct_stride with L limbs.  Output fully reduced.  Returns false (and does
+// nothing) for N < 2^13, where the caller gathers first and runs the mapped transform.
+// With a probe, its events are recorded right before and after the rows-pass launch.
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
-                       const DevPrime *targets, int logn, hipStream_t stream);
+                       const DevPrime *targets, const DevPrime *digits, int logn, hipStream_t stream,
+                       const KernelProbe *probe = nullptr);
 
 }  // namespace shc

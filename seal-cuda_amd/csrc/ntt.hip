@@ -11,33 +11,28 @@
 //   * A tile lives in LDS; each thread pulls 2^E values into VGPRs and runs E
 //     butterfly layers there before writing them back, so one LDS round trip
 //     covers E layers.
-//   * Harvey lazy butterflies with Shoup twiddles keep values in [0,4q) (forward)
-//     or [0,2q) (inverse); the full reduction and the N^-1 scaling are folded into
+//   * Harvey-style lazy butterflies with Shoup twiddles keep values in [0,8q)
+//     (forward) or [0,4q) (inverse); the twiddle product uses a quotient estimate
+//     that may be low by up to 3 (9 32-bit multiplies instead of 10), which those
+//     wider ranges absorb.  The full reduction and the N^-1 scaling are folded into
 //     the store of the last pass.
 //
 // Twiddle indexing: forward layer l (m = 2^l groups) uses root_powers[m + group];
 // inverse layer l uses inv_root_powers[N - 2m + 1 + group], group = x >> (logn - l).
 #include "ntt.h"
-#include "modarith.h"
+#include "butterfly.h"
 
 namespace shc {
 namespace {
 
 __host__ __device__ constexpr int lds_pad(int a) { return a + (a >> 4); }
 
-template <bool INV>
-__device__ __forceinline__ void butterfly(uint64_t &x, uint64_t &y, ulonglong2 w, uint64_t q, uint64_t two_q) {
-    if (!INV) {
-        uint64_t u = x >= two_q ? x - two_q : x;
-        uint64_t v = shoup_lazy(y, w.x, w.y, q);
-        x = u + v;
-        y = u + two_q - v;
-    } else {
-        uint64_t u = x, v = y;
-        uint64_t s = u + v;
-        x = s >= two_q ? s - two_q : s;
-        y = shoup_lazy(u + two_q - v, w.x, w.y, q);
-    }
+// Twiddle tables are global memory; saying so lets the compiler use global loads
+// (flat loads would also hold up the LDS waits).
+__device__ __forceinline__ ulonglong2 load_tw(const ulonglong2 *tw, int i) {
+    typedef uint64_t g2 __attribute__((ext_vector_type(2)));
+    const g2 v = ((const __attribute__((address_space(1))) g2 *)tw)[i];
+    return ulonglong2{v.x, v.y};
 }
 
 // Position inside a 2^K-point sub-transform of register j of thread t, for the phase
@@ -56,7 +51,7 @@ __device__ __forceinline__ int phase_index(int t, int j) {
 // 2^L0 sub-transforms the full transform splits into at global layer L0.
 template <int LOGN, int K, int E, int S0, int EP, bool INV>
 __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int g, int L0,
-                                              const ulonglong2 *__restrict__ tw, uint64_t q, uint64_t two_q) {
+                                              const ulonglong2 *__restrict__ tw, uint64_t four_q, uint64_t neg_q) {
 #pragma unroll
     for (int ss = 0; ss < EP; ss++) {
         const int s = INV ? (S0 + EP - 1 - ss) : (S0 + ss);
@@ -67,8 +62,8 @@ __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int 
             const int idx = phase_index<K, E, S0, EP>(t, j);
             const int grp = (g << s) + (idx >> (K - s));
             const int l = L0 + s;
-            const ulonglong2 w = INV ? tw[(1 << LOGN) - (2 << l) + 1 + grp] : tw[(1 << l) + grp];
-            butterfly<INV>(r[j], r[j | (1 << b)], w, q, two_q);
+            const ulonglong2 w = load_tw(tw, INV ? (1 << LOGN) - (2 << l) + 1 + grp : (1 << l) + grp);
+            butterfly<INV>(r[j], r[j | (1 << b)], w, four_q, neg_q);
         }
     }
 }
@@ -77,7 +72,7 @@ __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int 
 // index of this thread's sub-transform to its LDS slot.
 template <int LOGN, int K, int E, bool INV, int P = 0, class Addr>
 __device__ __forceinline__ void run_phases(uint64_t *tile, const Addr &addr, int t, int g, int L0,
-                                           const ulonglong2 *__restrict__ tw, uint64_t q, uint64_t two_q,
+                                           const ulonglong2 *__restrict__ tw, uint64_t four_q, uint64_t neg_q,
                                            bool active) {
     constexpr int NP = (K + E - 1) / E;
     if constexpr (P < NP) {
@@ -89,19 +84,22 @@ __device__ __forceinline__ void run_phases(uint64_t *tile, const Addr &addr, int
         if (active) {
 #pragma unroll
             for (int j = 0; j < (1 << E); j++) r[j] = tile[addr(phase_index<K, E, S0, EP>(t, j))];
-            phase_compute<LOGN, K, E, S0, EP, INV>(r, t, g, L0, tw, q, two_q);
+            phase_compute<LOGN, K, E, S0, EP, INV>(r, t, g, L0, tw, four_q, neg_q);
 #pragma unroll
             for (int j = 0; j < (1 << E); j++) tile[addr(phase_index<K, E, S0, EP>(t, j))] = r[j];
         }
         __syncthreads();
-        run_phases<LOGN, K, E, INV, P + 1>(tile, addr, t, g, L0, tw, q, two_q, active);
+        run_phases<LOGN, K, E, INV, P + 1>(tile, addr, t, g, L0, tw, four_q, neg_q, active);
     }
 }
 
-// Final touch of a value leaving the last pass.
+// Final touch of a value leaving the last pass: forward [0, 8q) -> [0, 4q) (lazy) or
+// [0, q); inverse [0, 4q) times N^-1 (exact Shoup) -> [0, 2q) (lazy) or [0, q).
 template <bool INV, bool FULL>
 __device__ __forceinline__ uint64_t finish(uint64_t x, const DevPrime &P) {
     if (!INV) {
+        const uint64_t four_q = P.q << 2;
+        x = x >= four_q ? x - four_q : x;
         if (FULL) {
             x = x >= P.two_q ? x - P.two_q : x;
             x = x >= P.q ? x - P.q : x;
@@ -113,65 +111,81 @@ __device__ __forceinline__ uint64_t finish(uint64_t x, const DevPrime &P) {
     return x;
 }
 
-// Rows pass: SUBS contiguous rows of 2^K values per block; the row is the last K
-// layers of the limb (or the whole limb when K == LOGN).
+// Rows pass: one block takes row g (the last K layers; the whole limb when K == LOGN)
+// of SUBS limbs that share prime i -- limbs (i, j) for SUBS consecutive j -- so every
+// row in the block uses the same twiddles (broadcast loads, L1/L2 hits).  Blocks run
+// with the limb group fastest, so consecutive blocks also share them.
 template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
-__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__restrict__ data,
-                                                                  const DevPrime *__restrict__ primes,
-                                                                  uint32_t limbs_per_poly, uint32_t total_rows) {
+__device__ __forceinline__ void rows_body(uint64_t *__restrict__ data, const DevPrime *__restrict__ primes,
+                                          LimbGrid lg, uint32_t j0, uint32_t jn) {
     constexpr int T = 1 << (K - E);
     constexpr int SZ = 1 << K;
     constexpr int RPL = LOGN - K;  // log2(rows per limb)
     __shared__ uint64_t tile[lds_pad(SUBS * SZ)];
+    const uint32_t jc_count = (jn + SUBS - 1) / SUBS;
+    const uint32_t jc = blockIdx.x % jc_count;
+    const uint32_t r = blockIdx.x / jc_count;
+    const int g = (int)(r & ((1u << RPL) - 1));
+    const uint32_t i = r >> RPL;
+    const uint32_t jfirst = jc * SUBS;
+    const int nsub = (int)min<uint32_t>(SUBS, jn - jfirst);
+    const uint32_t limb0 = i * lg.A + (j0 + jfirst) * lg.B;
+    auto row_ptr = [&](int sub) { return data + (((size_t)(limb0 + (uint32_t)sub * lg.B)) << LOGN) + ((size_t)g << K); };
     const int tid = threadIdx.x;
-    const int sub = tid / T, t = tid % T;
-    const uint32_t row0 = blockIdx.x * SUBS;
-    const int nvalid = (int)min<uint32_t>(SUBS, total_rows - row0) * SZ;
-    uint64_t *base = data + (size_t)row0 * SZ;
-    for (int e = tid; e < nvalid; e += SUBS * T) tile[lds_pad(e)] = base[e];
+    for (int e = tid; e < nsub * SZ; e += SUBS * T) tile[lds_pad(e)] = row_ptr(e / SZ)[e % SZ];
     __syncthreads();
 
-    const uint32_t row = row0 + sub;
-    const bool active = row < total_rows;
-    const uint32_t limb = active ? (row >> RPL) : 0;
-    const DevPrime &P = primes[limb % limbs_per_poly];
-    const int g = (int)(row & ((1u << RPL) - 1));
+    const int sub = tid / T, t = tid % T;
+    const DevPrime &P = primes[i];
     const ulonglong2 *tw = INV ? P.inv : P.fwd;
-    const uint64_t q = P.q, two_q = P.two_q;
     auto addr = [sub](int idx) { return lds_pad(sub * SZ + idx); };
-    run_phases<LOGN, K, E, INV>(tile, addr, t, g, RPL, tw, q, two_q, active);
+    run_phases<LOGN, K, E, INV>(tile, addr, t, g, RPL, tw, P.q << 2, 0 - P.q, sub < nsub);
 
-    for (int e = tid; e < nvalid; e += SUBS * T) {
+    for (int e = tid; e < nsub * SZ; e += SUBS * T) {
         uint64_t x = tile[lds_pad(e)];
-        if (LAST) {
-            const uint32_t lb = (row0 + (uint32_t)(e / SZ)) >> RPL;
-            x = finish<INV, FULL>(x, primes[lb % limbs_per_poly]);
-        }
-        base[e] = x;
+        if (LAST) x = finish<INV, FULL>(x, P);
+        row_ptr(e / SZ)[e % SZ] = x;
     }
+}
+
+template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__restrict__ data,
+                                                                  const DevPrime *__restrict__ primes, LimbGrid lg,
+                                                                  uint32_t j0, uint32_t jn) {
+    rows_body<LOGN, K, E, SUBS, INV, LAST, FULL>(data, primes, lg, j0, jn);
+}
+
+// The key switch's ModUp rows pass: the same code as a kernel of its own, so that
+// profiles tell it apart from the other forward transforms.
+template <int LOGN, int K, int E, int SUBS>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_modup_kernel(uint64_t *__restrict__ data,
+                                                                        const DevPrime *__restrict__ primes,
+                                                                        LimbGrid lg) {
+    rows_body<LOGN, K, E, SUBS, false, true, true>(data, primes, lg, 0, lg.J);
 }
 
 // Columns pass: SUBS adjacent columns of one limb per block; a column is the first
 // K layers of the limb (stride 2^(LOGN-K) between its values).
 template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__restrict__ data,
-                                                                  const DevPrime *__restrict__ primes,
-                                                                  uint32_t limbs_per_poly) {
+                                                                  const DevPrime *__restrict__ primes, LimbGrid lg,
+                                                                  uint32_t j0, uint32_t jn) {
     constexpr int SZ = 1 << K;
     constexpr int C = 1 << (LOGN - K);
     constexpr int TILES = C / SUBS;
     __shared__ uint64_t tile[lds_pad(SUBS * SZ)];
     const int tid = threadIdx.x;
     const int col = tid % SUBS, t = tid / SUBS;
-    const uint32_t limb = blockIdx.x / TILES;
+    const uint32_t r = blockIdx.x / TILES;
     const int c0 = (int)(blockIdx.x % TILES) * SUBS;
-    uint64_t *base = data + ((size_t)limb << LOGN) + c0;
-    const DevPrime &P = primes[limb % limbs_per_poly];
+    const uint32_t i = r / jn, j = j0 + r % jn;
+    uint64_t *base = data + ((size_t)(i * lg.A + j * lg.B) << LOGN) + c0;
+    const DevPrime &P = primes[i];
     for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E)) tile[lds_pad(e)] = base[(e / SUBS) * C + (e % SUBS)];
     __syncthreads();
     const ulonglong2 *tw = INV ? P.inv : P.fwd;
     auto addr = [col](int idx) { return lds_pad(idx * SUBS + col); };
-    run_phases<LOGN, K, E, INV>(tile, addr, t, 0, 0, tw, P.q, P.two_q, true);
+    run_phases<LOGN, K, E, INV>(tile, addr, t, 0, 0, tw, P.q << 2, 0 - P.q, true);
     for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E)) {
         uint64_t x = tile[lds_pad(e)];
         if (LAST) x = finish<INV, FULL>(x, P);
@@ -179,76 +193,84 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
     }
 }
 
-// ModUp columns pass: limb m = (ct, j, i) of the output T[ct][j][i] is digit j of
-// ciphertext ct, read from d[ct][j], reduced modulo target prime i on load, then run
-// through the first K layers (the key switch's gather + forward NTT pass 1 in one).
+// ModUp columns pass into T[i][ct][j] (target-major, so that the rows pass walks one
+// prime's limbs at a time): digit j of ciphertext ct, read from d[ct][j], reduced
+// modulo target prime i on load when the digit's prime exceeds 8 q_i (the forward
+// butterflies take inputs < 8q), then the first K layers.  Blocks run in (ct, j, i)
+// order, i fastest, so the L+1 blocks that read one tile of a digit run together and
+// on one XCD (block % 8 is the tile's % 8), and the digit is read from HBM once.
 template <int LOGN, int K, int E, int SUBS>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const uint64_t *__restrict__ d,
                                                                         size_t d_ct_stride,
                                                                         uint64_t *__restrict__ T,
                                                                         const DevPrime *__restrict__ targets,
-                                                                        uint32_t L) {
+                                                                        const DevPrime *__restrict__ digits,
+                                                                        uint32_t count, uint32_t L) {
     constexpr int SZ = 1 << K;
     constexpr int C = 1 << (LOGN - K);
     constexpr int TILES = C / SUBS;
     __shared__ uint64_t tile[lds_pad(SUBS * SZ)];
     const int tid = threadIdx.x;
     const int col = tid % SUBS, t = tid / SUBS;
-    const uint32_t limb = blockIdx.x / TILES;
+    const uint32_t task = blockIdx.x / TILES;  // (ct * L + j) * (L + 1) + i
     const int c0 = (int)(blockIdx.x % TILES) * SUBS;
-    const uint32_t i = limb % (L + 1);
-    const uint32_t r = limb / (L + 1);
+    const uint32_t i = task % (L + 1);
+    const uint32_t r = task / (L + 1);
     const uint32_t j = r % L;
     const size_t ct = r / L;
     const uint64_t *in = d + ct * d_ct_stride + ((size_t)j << LOGN) + c0;
-    uint64_t *out = T + ((size_t)limb << LOGN) + c0;
+    uint64_t *out = T + ((((size_t)i * count + ct) * L + j) << LOGN) + c0;
     const DevPrime &P = targets[i];
-    for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E))
-        tile[lds_pad(e)] = barrett64(in[(e / SUBS) * C + (e % SUBS)], P.q, P.ratio_hi);
+    if (digits[j].q <= 8 * P.q) {
+        for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E)) tile[lds_pad(e)] = in[(e / SUBS) * C + (e % SUBS)];
+    } else {
+        for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E))
+            tile[lds_pad(e)] = barrett64(in[(e / SUBS) * C + (e % SUBS)], P.q, P.ratio_hi);
+    }
     __syncthreads();
     auto addr = [col](int idx) { return lds_pad(idx * SUBS + col); };
-    run_phases<LOGN, K, E, false>(tile, addr, t, 0, 0, P.fwd, P.q, P.two_q, true);
+    run_phases<LOGN, K, E, false>(tile, addr, t, 0, 0, P.fwd, P.q << 2, 0 - P.q, true);
     for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E)) out[(e / SUBS) * C + (e % SUBS)] = tile[lds_pad(e)];
 }
 
+// Transform of limbs (i, j), i < lg.I, j0 <= j < j0 + jn.
 template <int LOGN, bool INV, bool FULL>
-void launch_ntt(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, hipStream_t st) {
+void launch_ntt(uint64_t *data, LimbGrid lg, uint32_t j0, uint32_t jn, const DevPrime *primes, hipStream_t st) {
     if constexpr (LOGN <= 12) {
         constexpr int K = LOGN;
         constexpr int E = K < 4 ? K : 4;
         constexpr int T = 1 << (K - E);
         constexpr int SUBS = (T >= 256) ? 1 : (256 / T);
-        const uint32_t blocks = (count + SUBS - 1) / SUBS;
-        ntt_rows_kernel<LOGN, K, E, SUBS, INV, true, FULL><<<blocks, SUBS * T, 0, st>>>(data, primes, lpp, count);
+        const uint32_t blocks = lg.I * ((jn + SUBS - 1) / SUBS);
+        ntt_rows_kernel<LOGN, K, E, SUBS, INV, true, FULL><<<blocks, SUBS * T, 0, st>>>(data, primes, lg, j0, jn);
     } else {
         constexpr int K1 = LOGN / 2;
         constexpr int K2 = LOGN - K1;
         constexpr int E = 4;
-        constexpr int CS = 16;                               // columns per block
-        constexpr int RS = (4096 >> K2) < 1 ? 1 : (4096 >> K2); // rows per block
-        const uint32_t col_blocks = count * ((1u << K2) / CS);
-        const uint32_t rows = count << K1;
-        const uint32_t row_blocks = (rows + RS - 1) / RS;
+        constexpr int CS = 16;                                    // columns per block
+        constexpr int RS = (4096 >> K2) < 1 ? 1 : (4096 >> K2);  // rows per block
+        const uint32_t col_blocks = lg.I * jn * ((1u << K2) / CS);
+        const uint32_t row_blocks = (lg.I << K1) * ((jn + RS - 1) / RS);
         if (!INV) {
             ntt_cols_kernel<LOGN, K1, E, CS, false, false, FULL>
-                <<<col_blocks, CS << (K1 - E), 0, st>>>(data, primes, lpp);
+                <<<col_blocks, CS << (K1 - E), 0, st>>>(data, primes, lg, j0, jn);
             ntt_rows_kernel<LOGN, K2, E, RS, false, true, FULL>
-                <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lpp, rows);
+                <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lg, j0, jn);
         } else {
             ntt_rows_kernel<LOGN, K2, E, RS, true, false, FULL>
-                <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lpp, rows);
+                <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lg, j0, jn);
             ntt_cols_kernel<LOGN, K1, E, CS, true, true, FULL>
-                <<<col_blocks, CS << (K1 - E), 0, st>>>(data, primes, lpp);
+                <<<col_blocks, CS << (K1 - E), 0, st>>>(data, primes, lg, j0, jn);
         }
     }
 }
 
 template <bool INV, bool FULL>
-void dispatch_chunk(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn,
+void dispatch_chunk(uint64_t *data, LimbGrid lg, uint32_t j0, uint32_t jn, const DevPrime *primes, int logn,
                     hipStream_t st) {
     switch (logn) {
 #define SHC_NTT_CASE(L) \
-    case L: launch_ntt<L, INV, FULL>(data, count, lpp, primes, st); break;
+    case L: launch_ntt<L, INV, FULL>(data, lg, j0, jn, primes, st); break;
         SHC_NTT_CASE(1) SHC_NTT_CASE(2) SHC_NTT_CASE(3) SHC_NTT_CASE(4) SHC_NTT_CASE(5) SHC_NTT_CASE(6)
         SHC_NTT_CASE(7) SHC_NTT_CASE(8) SHC_NTT_CASE(9) SHC_NTT_CASE(10) SHC_NTT_CASE(11) SHC_NTT_CASE(12)
         SHC_NTT_CASE(13) SHC_NTT_CASE(14) SHC_NTT_CASE(15) SHC_NTT_CASE(16) SHC_NTT_CASE(17)
@@ -258,23 +280,26 @@ void dispatch_chunk(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime
     launch_check("ntt launch");
 }
 
-// Splits very large batches so one launch stays well below 2^31 work-items (every
-// pass runs N/16 work-items per limb); chunks hold whole polynomials so that limb k
-// keeps prime k % lpp.
+// Splits very large batches over j so that one launch covers at most 2^34 / N limbs
+// (every pass runs N/16 work-items per limb, so a launch stays below 2^30).
 template <bool INV, bool FULL>
-void dispatch(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn, hipStream_t st) {
-    uint32_t max_limbs = (uint32_t)((uint64_t(1) << 34) >> logn);
-    max_limbs = max_limbs < lpp ? lpp : max_limbs - max_limbs % lpp;
-    for (uint32_t done = 0; done < count;) {
-        const uint32_t c = count - done < max_limbs ? count - done : max_limbs;
-        dispatch_chunk<INV, FULL>(data + ((size_t)done << logn), c, lpp, primes, logn, st);
-        done += c;
+void dispatch(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, hipStream_t st) {
+    if (logn < 1 || logn > 17) fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree must be 2^1 .. 2^17");
+    if ((uint64_t)lg.I * lg.J == 0) return;
+    if ((uint64_t)(lg.I - 1) * lg.A + (uint64_t)(lg.J - 1) * lg.B > UINT32_MAX)
+        fail(ERR_INVALID_ARGUMENT, "too many limbs for one transform call");
+    const uint64_t max_limbs = (uint64_t(1) << 34) >> logn;
+    if (lg.I > max_limbs) fail(ERR_INVALID_ARGUMENT, "too many primes for one transform call");
+    const uint32_t step = (uint32_t)(max_limbs / lg.I < lg.J ? max_limbs / lg.I : lg.J);
+    for (uint32_t j0 = 0; j0 < lg.J; j0 += step) {
+        const uint32_t jn = lg.J - j0 < step ? lg.J - j0 : step;
+        dispatch_chunk<INV, FULL>(data, lg, j0, jn, primes, logn, st);
     }
 }
 
 template <int LOGN>
 void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts, uint32_t L, const DevPrime *targets,
-                  hipStream_t st) {
+                  const DevPrime *digits, const KernelProbe *probe, hipStream_t st) {
     constexpr int K1 = LOGN / 2;
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
@@ -282,49 +307,57 @@ void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts,
     constexpr int RS = 4096 >> K2;
     const uint32_t limbs = cts * L * (L + 1);
     ntt_cols_modup_kernel<LOGN, K1, E, CS>
-        <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, L);
-    const uint32_t rows = limbs << K1;
-    ntt_rows_kernel<LOGN, K2, E, RS, false, true, true>
-        <<<(rows + RS - 1) / RS, RS << (K2 - E), 0, st>>>(T, targets, L + 1, rows);
+        <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L);
+    const LimbGrid lg{cts * L, 1, L + 1, cts * L};
+    if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
+    ntt_rows_modup_kernel<LOGN, K2, E, RS>
+        <<<((L + 1) << K1) * ((lg.J + RS - 1) / RS), RS << (K2 - E), 0, st>>>(T, targets, lg);
+    if (probe) SHC_HIP(hipEventRecord(probe->end, st));
 }
 
 }  // namespace
 
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
-                       const DevPrime *targets, int logn, hipStream_t st) {
+                       const DevPrime *targets, const DevPrime *digits, int logn, hipStream_t st,
+                       const KernelProbe *probe) {
     if (logn < 13 || logn > 17 || count == 0) return false;
-    const uint64_t per_ct = (uint64_t)L * (L + 1);
-    uint32_t max_cts = (uint32_t)(((uint64_t(1) << 34) >> logn) / per_ct);
-    if (max_cts == 0) max_cts = 1;
-    for (uint32_t done = 0; done < count;) {
-        const uint32_t c = count - done < max_cts ? count - done : max_cts;
-        const uint64_t *dd = d + (size_t)done * d_ct_stride;
-        uint64_t *TT = T + ((size_t)done * per_ct << logn);
-        switch (logn) {
-            case 13: launch_modup<13>(dd, d_ct_stride, TT, c, L, targets, st); break;
-            case 14: launch_modup<14>(dd, d_ct_stride, TT, c, L, targets, st); break;
-            case 15: launch_modup<15>(dd, d_ct_stride, TT, c, L, targets, st); break;
-            case 16: launch_modup<16>(dd, d_ct_stride, TT, c, L, targets, st); break;
-            case 17: launch_modup<17>(dd, d_ct_stride, TT, c, L, targets, st); break;
-        }
-        launch_check("ntt modup launch");
-        done += c;
+    if ((uint64_t)count * L * (L + 1) > ((uint64_t(1) << 34) >> logn))
+        fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
+    switch (logn) {
+        case 13: launch_modup<13>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 14: launch_modup<14>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 15: launch_modup<15>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 16: launch_modup<16>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 17: launch_modup<17>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
     }
+    launch_check("ntt modup launch");
     return true;
 }
+
+namespace {
+LimbGrid poly_grid(uint32_t count, uint32_t lpp) {
+    if (lpp == 0 || count % lpp) fail(ERR_INVALID_ARGUMENT, "limb count must be a multiple of limbs per polynomial");
+    return LimbGrid{1, lpp, lpp, count / lpp};
+}
+}  // namespace
 
 void ntt_forward(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn, NttOut out,
                  hipStream_t st) {
     if (count == 0) return;
-    if (out == NTT_FULL) dispatch<false, true>(data, count, lpp, primes, logn, st);
-    else dispatch<false, false>(data, count, lpp, primes, logn, st);
+    ntt_forward_grid(data, poly_grid(count, lpp), primes, logn, out, st);
+}
+
+void ntt_forward_grid(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, NttOut out, hipStream_t st) {
+    if (out == NTT_FULL) dispatch<false, true>(data, lg, primes, logn, st);
+    else dispatch<false, false>(data, lg, primes, logn, st);
 }
 
 void ntt_inverse(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn, NttOut out,
                  hipStream_t st) {
     if (count == 0) return;
-    if (out == NTT_FULL) dispatch<true, true>(data, count, lpp, primes, logn, st);
-    else dispatch<true, false>(data, count, lpp, primes, logn, st);
+    const LimbGrid lg = poly_grid(count, lpp);
+    if (out == NTT_FULL) dispatch<true, true>(data, lg, primes, logn, st);
+    else dispatch<true, false>(data, lg, primes, logn, st);
 }
 
 }  // namespace shc

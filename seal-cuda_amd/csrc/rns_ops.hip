@@ -5,9 +5,11 @@
 // reference's ~11k per-limb launches:
 //
 //   1. CKKS: inverse NTT of c2 (digits d_j in coefficient form)
-//   2. modup_gather: T[j][i] = d_j mod q_i for every digit j < L and target prime
-//      i in {q_0..q_{L-1}, special}
-//   3. one batched forward NTT over all L(L+1) limbs of T
+//   2. modup: T[i][ct][j] = d_j mod q_i for every digit j < L and target prime
+//      i in {q_0..q_{L-1}, special}, target-major so that consecutive limbs share a
+//      prime (its twiddles stay in L2)
+//   3. one batched forward NTT over all L(L+1) limbs of T (fused with step 2 for
+//      N >= 2^13)
 //   4. inner_product: P_k[i] = sum_j T[j][i] * key[j][k][i] with 128-bit accumulation,
 //      one Barrett reduction at the end; G ciphertexts per thread share each key load
 //   5. ModDown: inverse NTT of the special limbs, rounding correction, forward NTT
@@ -46,56 +48,66 @@ __global__ void tensor_kernel(const uint64_t *__restrict__ a, size_t a_stride, c
     }
 }
 
-// T[ct][j][i][c] = d[ct][j][c] mod q_i, i in [0, L] (target i == L is the special prime).
+// T[i][ct][j][c] = d[ct][j][c] mod q_i, i in [0, L] (target i == L is the special
+// prime); left unreduced when the digit's prime is at most 8 q_i (the forward NTT
+// takes inputs < 8q).
 __global__ void modup_gather_kernel(const uint64_t *__restrict__ d, size_t d_stride, uint64_t *__restrict__ T,
                                     const DevPrime *__restrict__ targets, const DevPrime *__restrict__ digits,
-                                    uint32_t L, int logn, size_t total) {
+                                    uint32_t count, uint32_t L, int logn, size_t total) {
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t n = (size_t)1 << logn;
     const size_t c = idx & (n - 1);
-    size_t r = idx >> logn;
-    const uint32_t i = (uint32_t)(r % (L + 1));
-    r /= (L + 1);
+    const size_t r = idx >> logn;
+    const size_t per_target = (size_t)count * L;
+    const uint32_t i = (uint32_t)(r / per_target);
     const uint32_t j = (uint32_t)(r % L);
-    const size_t ct = r / L;
+    const size_t ct = (r % per_target) / L;
     const uint64_t v = d[ct * d_stride + ((size_t)j << logn) + c];
     const DevPrime &Q = targets[i];
-    T[idx] = digits[j].q <= Q.q ? v : barrett64(v, Q.q, Q.ratio_hi);
+    T[idx] = digits[j].q <= 8 * Q.q ? v : barrett64(v, Q.q, Q.ratio_hi);
     }
 }
 
-// P[ct][k][i][c] = sum_j T[ct][j][i][c] * key[j][k][ki][c] mod q_i for i < L, and the
-// special target (i == L) goes to S[ct][k][c].  Each thread owns one (i, c) for G
-// consecutive ciphertexts, so every key word it loads is used G times.
+// P[ct][k][i][c] = sum_j T[i][ct][j][c] * key[j][k][ki][c] mod q_i for i < L, and the
+// special target (i == L) goes to S[ct][k][c].  A block owns a tile of 256 flat (i, c)
+// positions for one group of G ciphertexts; a thread owns one position and keeps
+// 128-bit sums for its G ciphertexts, so each key word it loads serves G products.
+// Blocks are dealt to the 8 XCDs round-robin (XCD = block % 8); the mapping below puts
+// every group of one tile on the same XCD, one after another, so the tile's key words
+// come from HBM once and then from that XCD's L2.
 template <int G>
-__global__ void inner_product_kernel(const uint64_t *__restrict__ T, const uint64_t *__restrict__ key,
-                                     uint64_t *__restrict__ P, uint64_t *__restrict__ S,
-                                     const DevPrime *__restrict__ targets, uint32_t L, uint32_t K, int logn,
-                                     uint32_t count) {
+__global__ __launch_bounds__(TPB) void inner_product_kernel(const uint64_t *__restrict__ T,
+                                                            const uint64_t *__restrict__ key,
+                                                            uint64_t *__restrict__ P, uint64_t *__restrict__ S,
+                                                            const DevPrime *__restrict__ targets, uint32_t L,
+                                                            uint32_t K, int logn, uint32_t count, uint32_t tiles,
+                                                            uint32_t groups) {
+    const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const uint32_t tile = (k / groups) * 8 + xcd;
+    if (tile >= tiles) return;
+    const uint32_t grp = k % groups;
     const size_t n = (size_t)1 << logn;
-    const size_t per_group = (size_t)(L + 1) << logn;
-    const size_t groups = (count + G - 1) / G;
-    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < groups * per_group;
-         idx += (size_t)gridDim.x * TPB) {
-    const size_t grp = idx / per_group;
-    const size_t rem = idx % per_group;
-    const uint32_t i = (uint32_t)(rem >> logn);
-    const size_t c = rem & (n - 1);
+    const size_t flat = (size_t)tile * TPB + threadIdx.x;
+    if (flat >= (size_t)(L + 1) << logn) return;
+    const uint32_t i = (uint32_t)(flat >> logn);
+    const size_t c = flat & (n - 1);
     const uint32_t ki = (i == L) ? K - 1 : i;
     const DevPrime &Q = targets[i];
-    const size_t ct0 = grp * G;
+    const size_t ct0 = (size_t)grp * G;
     const int g_valid = (int)min<size_t>(G, count - ct0);
-    const size_t t_ct_stride = (size_t)L * (L + 1) << logn;
+    const uint64_t *Ti = T + ((((size_t)i * count + ct0) * L) << logn) + c;
+    const uint64_t *kp = key + (size_t)ki * n + c;
+    const size_t kstride = (size_t)K << logn;  // key[j][k] -> key[j][k+1]
     u128 acc0[G], acc1[G];
 #pragma unroll
     for (int g = 0; g < G; g++) acc0[g] = acc1[g] = u128{0, 0};
     for (uint32_t j = 0; j < L; j++) {
-        const uint64_t k0 = key[(((size_t)j * 2 + 0) * K + ki) * n + c];
-        const uint64_t k1 = key[(((size_t)j * 2 + 1) * K + ki) * n + c];
+        const uint64_t k0 = kp[(2 * j) * kstride];
+        const uint64_t k1 = kp[(2 * j + 1) * kstride];
 #pragma unroll
         for (int g = 0; g < G; g++) {
             if (g < g_valid) {
-                const uint64_t t = T[(ct0 + g) * t_ct_stride + (((size_t)j * (L + 1) + i) << logn) + c];
+                const uint64_t t = Ti[((size_t)g * L + j) << logn];
                 add128(acc0[g], mul64x64(t, k0));
                 add128(acc1[g], mul64x64(t, k1));
             }
@@ -115,7 +127,6 @@ __global__ void inner_product_kernel(const uint64_t *__restrict__ T, const uint6
                 P[((ct * 2 + 1) * L + i) * n + c] = r1;
             }
         }
-    }
     }
 }
 
@@ -226,14 +237,15 @@ void tensor_strided(const DevPrime *primes, const uint64_t *a, size_t a_stride, 
 
 size_t keyswitch_workspace(const Context &c, uint32_t count, uint32_t L) { return ks_layout(c, count, L).total; }
 
-void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
-                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st) {
+namespace {
+
+// One launch sequence over `count` ciphertexts (few enough that every launch stays
+// within the NTT's per-launch limb bound).
+void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct,
+                      size_t ct_stride, const uint64_t *key, uint32_t count, uint32_t L, void *ws, hipStream_t st,
+                      const KernelProbe *probe) {
     const size_t K = c.key_count();
-    if (K < 2) fail(ERR_LOGIC, "keyswitching is not supported by the context");
-    if (L < 1 || L > K - 1) fail(ERR_INVALID_ARGUMENT, "level out of range for key switching");
-    if (!count) return;
     const KsLayout lay = ks_layout(c, count, L);
-    if (!ws || ws_bytes < lay.total) fail(ERR_INVALID_ARGUMENT, "workspace too small for key switching");
     char *base = static_cast<char *>(ws);
     uint64_t *d = reinterpret_cast<uint64_t *>(base + lay.d);
     uint64_t *T = reinterpret_cast<uint64_t *>(base + lay.T);
@@ -251,17 +263,23 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
     launch_check("gather");
     if (ckks) ntt_inverse(d, count * L, L, c.primes(), logn, NTT_FULL, st);
     // 2-3. ModUp
-    if (!ntt_forward_modup(d, (size_t)L * n, T, count, L, targets, logn, st)) {
+    if (!ntt_forward_modup(d, (size_t)L * n, T, count, L, targets, c.primes(), logn, st, probe)) {
         const size_t twords = (size_t)count * L * (L + 1) * n;
-        modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(d, (size_t)L * n, T, targets, c.primes(), L, logn,
-                                                               twords);
+        modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(d, (size_t)L * n, T, targets, c.primes(), count, L,
+                                                               logn, twords);
         launch_check("modup_gather");
-        ntt_forward(T, count * L * (L + 1), L + 1, targets, logn, NTT_FULL, st);
+        if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
+        ntt_forward_grid(T, LimbGrid{count * L, 1, L + 1, count * L}, targets, logn, NTT_FULL, st);
+        if (probe) SHC_HIP(hipEventRecord(probe->end, st));
     }
     // 4. inner product with the key
     constexpr int G = 4;
-    const size_t ip_threads = ((size_t)(count + G - 1) / G) * (L + 1) * n;
-    inner_product_kernel<G><<<blocks_for(ip_threads), TPB, 0, st>>>(T, key, P, S, targets, L, (uint32_t)K, logn, count);
+    const uint32_t groups = (count + G - 1) / G;
+    const uint32_t tiles = (uint32_t)((((size_t)(L + 1) << logn) + TPB - 1) / TPB);
+    const uint64_t ip_blocks = (uint64_t)((tiles + 7) / 8) * 8 * groups;
+    if (ip_blocks * TPB >= (uint64_t(1) << 32)) fail(ERR_INVALID_ARGUMENT, "key-switch batch too large");
+    inner_product_kernel<G><<<(uint32_t)ip_blocks, TPB, 0, st>>>(T, key, P, S, targets, L, (uint32_t)K, logn, count,
+                                                                tiles, groups);
     launch_check("inner_product");
     // 5. ModDown by the special prime
     ntt_inverse(S, count * 2, 1, targets + L, logn, NTT_FULL, st);
@@ -274,6 +292,32 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
     moddown_finish_kernel<<<blocks_for(uwords), TPB, 0, st>>>(P, U, ct, ct_stride, c.primes(), c.ks_consts(), L, logn,
                                                              uwords);
     launch_check("moddown_finish");
+}
+
+}  // namespace
+
+void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
+                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st) {
+    const size_t K = c.key_count();
+    if (K < 2) fail(ERR_LOGIC, "keyswitching is not supported by the context");
+    if (L < 1 || L > K - 1) fail(ERR_INVALID_ARGUMENT, "level out of range for key switching");
+    if (!count) return;
+    if (!ws || ws_bytes < ks_layout(c, count, L).total)
+        fail(ERR_INVALID_ARGUMENT, "workspace too small for key switching");
+    // The ModUp buffer holds L(L+1) limbs per ciphertext; one NTT launch covers at most
+    // 2^34 / N limbs, so large batches run in slices of whole ciphertexts.
+    const uint64_t per_ct = (uint64_t)L * (L + 1);
+    const uint64_t max_cts = ((uint64_t(1) << 34) >> c.logn) / per_ct;
+    if (max_cts == 0) fail(ERR_INVALID_ARGUMENT, "level too large for key switching at this degree");
+    const KernelProbe probe = c.probe;
+    const bool armed = c.probe_armed;
+    c.probe_armed = false;
+    for (uint32_t done = 0; done < count;) {
+        const uint32_t n = (uint32_t)(count - done < max_cts ? count - done : max_cts);
+        switch_key_chunk(c, target + (size_t)done * target_stride, target_stride, ct + (size_t)done * ct_stride,
+                         ct_stride, key, n, L, ws, st, armed && done == 0 ? &probe : nullptr);
+        done += n;
+    }
 }
 
 size_t rescale_workspace(const Context &c, uint32_t count, uint32_t polys, uint32_t L) {

@@ -72,6 +72,7 @@ def lib():
         L.shc_ckks_decode.argtypes = [_vp, _vp, u32, ctypes.c_double, _vp, _vp, sz, _vp]
         L.shc_rescale_workspace_size.argtypes = [_vp, u32, u32, u32, ctypes.POINTER(sz)]
         L.shc_rescale_to_next.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
+        L.shc_probe_keyswitch.argtypes = [_vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -79,6 +80,39 @@ def lib():
 def check(status):
     if status != 0:
         raise SealHipError(status, lib().shc_last_error().decode())
+
+
+class HipEvent:
+    """A timing hipEvent_t from the HIP runtime the library is linked against (for
+    shc_probe_keyswitch, which records on the library's launch stream)."""
+    _rt = None
+
+    def __init__(self):
+        if HipEvent._rt is None:
+            lib()
+            rt = ctypes.CDLL("libamdhip64.so")
+            rt.hipEventCreate.argtypes = [ctypes.POINTER(_vp)]
+            rt.hipEventDestroy.argtypes = [_vp]
+            rt.hipEventSynchronize.argtypes = [_vp]
+            rt.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), _vp, _vp]
+            HipEvent._rt = rt
+        h = _vp()
+        if HipEvent._rt.hipEventCreate(ctypes.byref(h)) != 0:
+            raise SealHipError(-5, "hipEventCreate failed")
+        self.handle = h.value
+
+    def elapsed_ms(self, end):
+        """Milliseconds from this event to `end` (waits for `end`)."""
+        if HipEvent._rt.hipEventSynchronize(end.handle) != 0:
+            raise SealHipError(-5, "hipEventSynchronize failed")
+        ms = ctypes.c_float()
+        if HipEvent._rt.hipEventElapsedTime(ctypes.byref(ms), self.handle, end.handle) != 0:
+            raise SealHipError(-5, "hipEventElapsedTime failed")
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) and HipEvent._rt is not None:
+            HipEvent._rt.hipEventDestroy(self.handle)
 
 
 def init(device=0):
@@ -227,3 +261,7 @@ class Context:
 
     def rescale_to_next(self, d_in, d_out, count, polys, limbs, d_ws, ws_bytes, stream=None):
         check(lib().shc_rescale_to_next(self.handle, d_in, d_out, count, polys, limbs, d_ws, ws_bytes, stream))
+
+    def probe_keyswitch(self, begin, end):
+        """Arm the next key switch to record HipEvents around its ModUp rows pass."""
+        check(lib().shc_probe_keyswitch(self.handle, begin.handle, end.handle))
