@@ -11,6 +11,8 @@ per-ciphertext path (one call per ciphertext).
 `--workload ntt`: batched negacyclic NTT, N = 2^16, 24 primes of 60 bits,
 128 ciphertexts x 2 polys x 24 limbs, forward then inverse.
 
+`--workload copy`: a 1 GiB device copy, the calibration run for HBM byte counters.
+
 Prints one JSON line.  Data are seeded uniform residues (synthetic) and injected
 uniform key words; every word is fully reduced.
 """
@@ -33,7 +35,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="mulrelin", choices=["mulrelin", "ntt"])
+    p.add_argument("--workload", default="mulrelin", choices=["mulrelin", "ntt", "copy"])
     p.add_argument("--batch", type=int, default=128)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-plain", action="store_true", help="skip the plain per-ciphertext leg")
@@ -271,13 +273,34 @@ def bench_ntt(args):
     return out
 
 
+def bench_copy(args):
+    """HBM copy of 1 GiB (torch's device copy kernel): the calibration run for the
+    FETCH_SIZE / WRITE_SIZE counters (known 1 GiB read + 1 GiB written per launch)."""
+    import torch
+    words = 1 << 27
+    src = torch.ones(words, dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    with Timer(torch) as tm:
+        for _ in range(args.steps):
+            dst.copy_(src)
+    ms = tm.ms / args.steps
+    gbs = 2 * words * 8 / (ms * 1e-3) / 1e9
+    return {"metric": "HBM copy GB/s (read + write), 1 GiB", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic (ones)",
+            "config": {"workload": "device copy of 2^27 int64 words", "parallelism": "single GPU"}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus N > 1 is not supported "
                                    "(independent ciphertexts shard across GPUs with no exchange)"}))
         sys.exit(2)
-    out = bench_mulrelin(args) if args.workload == "mulrelin" else bench_ntt(args)
+    out = {"mulrelin": bench_mulrelin, "ntt": bench_ntt, "copy": bench_copy}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -212,6 +212,42 @@ int shc_rescale_to_next(const shc_context *ctx, const uint64_t *d_in, uint64_t *
                         uint32_t poly_count, uint32_t limb_count, void *d_workspace, size_t workspace_bytes,
                         void *stream);
 
+/* ---- coefficient-wise evaluator operations ------------------------------------
+ * Polynomials are [polys][limb_count][N] (polys = ciphertexts x size); out may alias
+ * an input.  Both operands in the same form (coefficient or NTT). */
+
+/* out = a + b.  Replaces EvaluatorCuda::addInplace (src/evaluator_cuda.cu:156-208;
+ * CPU src/evaluator.cpp:167-266). */
+int shc_add(const shc_context *ctx, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out, uint32_t polys,
+            uint32_t limb_count, void *stream);
+/* out = a - b.  Replaces EvaluatorCuda::subInplace (src/evaluator_cuda.cu:210-281). */
+int shc_sub(const shc_context *ctx, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out, uint32_t polys,
+            uint32_t limb_count, void *stream);
+/* out = -a.  Replaces EvaluatorCuda::negateInplace (src/evaluator_cuda.cu:145-154). */
+int shc_negate(const shc_context *ctx, const uint64_t *d_a, uint64_t *d_out, uint32_t polys, uint32_t limb_count,
+               void *stream);
+/* out[p] = a[p] * plain (dyadic; a and plain in NTT form, plain [limb_count][N]).
+ * Replaces EvaluatorCuda::multiplyPlainNtt (src/evaluator_cuda.cu:1811-1867). */
+int shc_multiply_plain_ntt(const shc_context *ctx, const uint64_t *d_a, const uint64_t *d_plain, uint64_t *d_out,
+                           uint32_t polys, uint32_t limb_count, void *stream);
+/* Ciphertexts [count][size][limb_count][N]: out = ct with poly 0 +- plain ([limb_count][N],
+ * same form as ct; subtract != 0 subtracts).  CKKS add/subPlainInplace
+ * (src/evaluator_cuda.cu:1544-1690). */
+int shc_add_plain(const shc_context *ctx, const uint64_t *d_ct, const uint64_t *d_plain, uint64_t *d_out,
+                  uint32_t count, uint32_t size, uint32_t limb_count, int subtract, void *stream);
+/* BFV: out = ct with c0 +- Delta m, m = plain [N] mod t (coefficient form), SEAL's
+ * rounding of q m / t.  BFV add/subPlainInplace through multiplyAdd/SubPlainWith-
+ * ScalingVariant (src/utils/scalingvariant.cpp:53-131). */
+int shc_bfv_add_plain(const shc_context *ctx, const uint64_t *d_ct, const uint64_t *d_plain, uint64_t *d_out,
+                      uint32_t count, uint32_t size, uint32_t limb_count, int subtract, void *stream);
+/* BFV: out[p] = a[p] * m for coefficient-form a [polys][limb_count][N] and m = plain [N]
+ * mod t lifted as m - t for m >= ceil(t/2).  Replaces EvaluatorCuda::multiplyPlainNormal
+ * (src/evaluator_cuda.cu:1744-1807). */
+int shc_bfv_multiply_plain_workspace_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
+int shc_bfv_multiply_plain(const shc_context *ctx, const uint64_t *d_a, const uint64_t *d_plain, uint64_t *d_out,
+                           uint32_t polys, uint32_t limb_count, void *d_workspace, size_t workspace_bytes,
+                           void *stream);
+
 /* Instrumentation (no counterpart in the reference; for benchmarks and profilers):
  * arms `ctx` so that its next key switch (shc_relinearize, shc_switch_key,
  * shc_apply_galois_keyswitch) records the caller-created hipEvent_t `begin_event`

@@ -15,6 +15,7 @@
 typedef unsigned __int128 u128;
 
 static uint64_t addm(uint64_t a, uint64_t b, uint64_t q) { uint64_t s = a + b; return s >= q ? s - q : s; }
+static uint64_t subm(uint64_t a, uint64_t b, uint64_t q) { return a >= b ? a - b : a + q - b; }
 
 int or_encrypt_zero_asymmetric(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *pk,
                                size_t pk_limbs, const uint64_t *u, const uint64_t *e0, const uint64_t *e1,
@@ -168,10 +169,24 @@ int or_bfv_decrypt_scale_round(int logn, const uint64_t *q, size_t L, uint64_t t
     return 0;
 }
 
+static int bfv_plain_scaled(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *plain,
+                            size_t plain_count, uint64_t *c0, int subtract);
+
 int or_bfv_add_plain_scaled(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *plain,
                             size_t plain_count, uint64_t *c0) {
-    /* multiplyAddPlainWithScalingVariant (src/utils/scalingvariant.cpp:48-86):
-       c0 += m * floor(Q/t) + floor((m * (Q mod t) + floor((t+1)/2)) / t)  (mod q_j). */
+    return bfv_plain_scaled(logn, q, L, t, plain, plain_count, c0, 0);
+}
+
+int or_bfv_sub_plain_scaled(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *plain,
+                            size_t plain_count, uint64_t *c0) {
+    return bfv_plain_scaled(logn, q, L, t, plain, plain_count, c0, 1);
+}
+
+static int bfv_plain_scaled(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *plain,
+                            size_t plain_count, uint64_t *c0, int subtract) {
+    /* multiplyAdd / multiplySubPlainWithScalingVariant (src/utils/scalingvariant.cpp:
+       53-131): c0 +-= m * floor(Q/t) + floor((m * (Q mod t) + floor((t+1)/2)) / t)
+       (mod q_j). */
     size_t n = (size_t)1 << logn;
     /* Q mod t and floor(Q/t) mod q_j by long division of the product */
     uint64_t prod[70] = {1};
@@ -204,7 +219,7 @@ int or_bfv_add_plain_scaled(int logn, const uint64_t *q, size_t L, uint64_t t, c
         uint64_t fix = (uint64_t)(((u128)m * q_mod_t + half) / t);
         for (size_t j = 0; j < L; j++) {
             uint64_t v = (uint64_t)(((u128)m * delta[j] + fix) % q[j]);
-            c0[j * n + c] = addm(c0[j * n + c], v, q[j]);
+            c0[j * n + c] = subtract ? subm(c0[j * n + c], v, q[j]) : addm(c0[j * n + c], v, q[j]);
         }
     }
     return 0;

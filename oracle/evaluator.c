@@ -165,3 +165,45 @@ int or_rescale_to_next(int scheme, int logn, const uint64_t *moduli, size_t L, c
     free(last); free(tmp);
     return 0;
 }
+
+int or_multiply_plain_ntt(int logn, const uint64_t *moduli, size_t L, const uint64_t *a, size_t polys,
+                          const uint64_t *plain, uint64_t *out) {
+    /* Evaluator::multiplyPlainNtt (src/evaluator.cpp:1932-1990): dyadic product of every
+       NTT-form polynomial with the NTT-form plaintext, limb by limb. */
+    size_t n = (size_t)1 << logn;
+    for (size_t p = 0; p < polys; p++)
+        for (size_t i = 0; i < L; i++)
+            for (size_t c = 0; c < n; c++)
+                out[(p * L + i) * n + c] = or_mulmod(a[(p * L + i) * n + c], plain[i * n + c], moduli[i]);
+    return 0;
+}
+
+int or_bfv_multiply_plain(int logn, const uint64_t *moduli, size_t L, uint64_t t, const uint64_t *a, size_t polys,
+                          const uint64_t *plain, uint64_t *out) {
+    /* EvaluatorCuda::multiplyPlainNormal (src/evaluator_cuda.cu:1726-1807): lift m to
+       m - t when m >= ceil(t/2) (gMultiplyPlainNormalUtilA/B), NTT both operands, dyadic
+       product, inverse NTT; fully reduced.  Coefficient-form a: [polys][L][N]. */
+    size_t n = (size_t)1 << logn;
+    uint64_t *lift = malloc(n * 8);
+    const uint64_t thr = (t + 1) >> 1;
+    or_ntt_tables tb;
+    for (size_t i = 0; i < L; i++) {
+        uint64_t q = moduli[i];
+        if (or_ntt_tables_init(&tb, logn, q)) { free(lift); return -1; }
+        for (size_t c = 0; c < n; c++) {
+            uint64_t m = plain[c] % q;
+            lift[c] = plain[c] >= thr ? subm(m, t % q, q) : m;
+        }
+        or_ntt_forward(&tb, lift);
+        for (size_t p = 0; p < polys; p++) {
+            uint64_t *dst = out + (p * L + i) * n;
+            memcpy(dst, a + (p * L + i) * n, n * 8);
+            or_ntt_forward(&tb, dst);
+            for (size_t c = 0; c < n; c++) dst[c] = or_mulmod(dst[c], lift[c], q);
+            or_ntt_inverse(&tb, dst);
+        }
+        or_ntt_tables_free(&tb);
+    }
+    free(lift);
+    return 0;
+}

@@ -125,6 +125,10 @@ int or_bfv_decrypt_scale_round(int logn, const uint64_t *q, size_t L, uint64_t t
                                uint64_t *out);
 int or_bfv_add_plain_scaled(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *plain,
                             size_t plain_count, uint64_t *c0);
+/* c0 -= Delta m (multiplySubPlainWithScalingVariant, src/utils/scalingvariant.cpp:94-131).
+   Pinned by: the reference subPlainInplace on the same words (tests/test_oracle_evaluator.py). */
+int or_bfv_sub_plain_scaled(int logn, const uint64_t *q, size_t L, uint64_t t, const uint64_t *plain,
+                            size_t plain_count, uint64_t *c0);
 void or_batch_index_map(int logn, uint64_t *map);
 int or_batch_encode(int logn, uint64_t t, const uint64_t *values, size_t count, uint64_t *plain);
 int or_batch_decode(int logn, uint64_t t, const uint64_t *plain, uint64_t *values);
@@ -139,6 +143,16 @@ int or_ckks_decode(int logn, const uint64_t *moduli, size_t L, const uint64_t *p
 /* Evaluator::modSwitchScaleToNext (src/evaluator.cpp:1165-1240): in [polys][L][N] -> out [polys][L-1][N] */
 int or_rescale_to_next(int scheme, int logn, const uint64_t *moduli, size_t L, const uint64_t *in, size_t polys,
                        uint64_t *out);
+/* Evaluator::multiplyPlainNtt (src/evaluator.cpp:1932): [polys][L][N] x plain [L][N].
+   Pinned by: the reference multiplyPlainInplace (CKKS, NTT form) on the same words. */
+int or_multiply_plain_ntt(int logn, const uint64_t *moduli, size_t L, const uint64_t *a, size_t polys,
+                          const uint64_t *plain, uint64_t *out);
+/* GPU multiplyPlainNormal (src/evaluator_cuda.cu:1726-1807): the plaintext is lifted
+   (m >= ceil(t/2) -> m - t) for every plaintext, monomials included.  Pinned by: the
+   reference CPU multiplyPlainInplace on non-monomial plaintexts (equal words), and on a
+   monomial below ceil(t/2) (no lift on either path). */
+int or_bfv_multiply_plain(int logn, const uint64_t *moduli, size_t L, uint64_t t, const uint64_t *a, size_t polys,
+                          const uint64_t *plain, uint64_t *out);
 
 #ifdef __cplusplus
 }

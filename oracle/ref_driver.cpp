@@ -305,4 +305,52 @@ int ref_rescale(void *h, uint64_t L, uint64_t size, const uint64_t *in, uint64_t
     } catch (const std::exception &e) { fprintf(stderr, "ref_rescale: %s\n", e.what()); return -1; }
 }
 
+// Evaluator::addInplace / subInplace / negateInplace (src/evaluator.cpp:149-357) on
+// ciphertexts of `size` polys at the level with L primes: op 0 = a + b, 1 = a - b,
+// 2 = -a.
+int ref_ct_binary(void *h, uint64_t L, uint64_t size, int op, const uint64_t *a, const uint64_t *b, uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        Ciphertext x, y;
+        load_ct(s, x, L, size, a, 1024.0);
+        if (op == 2) {
+            s.evaluator->negateInplace(x);
+        } else {
+            load_ct(s, y, L, size, b, 1024.0);
+            if (op == 0) s.evaluator->addInplace(x, y);
+            else s.evaluator->subInplace(x, y);
+        }
+        std::memcpy(out, x.data(), size * L * s.n * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_ct_binary: %s\n", e.what()); return -1; }
+}
+
+// Evaluator::addPlainInplace / subPlainInplace / multiplyPlainInplace (src/evaluator.cpp:
+// 1603-1955): op 0 = add, 1 = sub, 2 = multiply.  BFV plain: [N] mod t, ciphertext in
+// coefficient form; CKKS plain: NTT form [L][N] at the ciphertext's level.
+int ref_plain_op(void *h, uint64_t L, uint64_t size, int op, const uint64_t *ct, const uint64_t *plain,
+                 uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        Ciphertext x;
+        load_ct(s, x, L, size, ct, 1024.0);
+        Plaintext p;
+        if (s.scheme == SchemeType::ckks) {
+            p.parmsID() = parmsIDZero;
+            p.resize(L * s.n);
+            std::memcpy(p.data(), plain, L * s.n * 8);
+            p.parmsID() = parms_for_level(s, L);
+            p.scale() = 1024.0;
+        } else {
+            p.resize(s.n);
+            std::memcpy(p.data(), plain, s.n * 8);
+        }
+        if (op == 0) s.evaluator->addPlainInplace(x, p);
+        else if (op == 1) s.evaluator->subPlainInplace(x, p);
+        else s.evaluator->multiplyPlainInplace(x, p);
+        std::memcpy(out, x.data(), size * L * s.n * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_plain_op: %s\n", e.what()); return -1; }
+}
+
 }  // extern "C"

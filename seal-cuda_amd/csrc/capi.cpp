@@ -4,6 +4,7 @@
 #include "ntt.h"
 #include "ckks_encoder.h"
 #include "crypto.h"
+#include "elementwise.h"
 #include "galois.h"
 #include "rns_ops.h"
 #include <string>
@@ -326,6 +327,54 @@ int shc_rescale_to_next(const shc_context *ctx, const uint64_t *in, uint64_t *ou
         const auto &c = ctx_of(ctx);
         check_limbs(c, limbs);
         shc::rescale_to_next(c, in, out, count, polys, limbs, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_add(const shc_context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t polys,
+            uint32_t limbs, void *stream) {
+    return guarded([&] { shc::poly_add(ctx_of(ctx), a, b, out, polys, limbs, (hipStream_t)stream); });
+}
+
+int shc_sub(const shc_context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t polys,
+            uint32_t limbs, void *stream) {
+    return guarded([&] { shc::poly_sub(ctx_of(ctx), a, b, out, polys, limbs, (hipStream_t)stream); });
+}
+
+int shc_negate(const shc_context *ctx, const uint64_t *a, uint64_t *out, uint32_t polys, uint32_t limbs,
+               void *stream) {
+    return guarded([&] { shc::poly_negate(ctx_of(ctx), a, out, polys, limbs, (hipStream_t)stream); });
+}
+
+int shc_multiply_plain_ntt(const shc_context *ctx, const uint64_t *a, const uint64_t *plain, uint64_t *out,
+                           uint32_t polys, uint32_t limbs, void *stream) {
+    return guarded([&] { shc::multiply_plain_ntt(ctx_of(ctx), a, plain, out, polys, limbs, (hipStream_t)stream); });
+}
+
+int shc_add_plain(const shc_context *ctx, const uint64_t *ct, const uint64_t *plain, uint64_t *out, uint32_t count,
+                  uint32_t size, uint32_t limbs, int subtract, void *stream) {
+    return guarded([&] {
+        shc::add_plain_poly0(ctx_of(ctx), ct, plain, out, count, size, limbs, subtract != 0, (hipStream_t)stream);
+    });
+}
+
+int shc_bfv_add_plain(const shc_context *ctx, const uint64_t *ct, const uint64_t *plain, uint64_t *out,
+                      uint32_t count, uint32_t size, uint32_t limbs, int subtract, void *stream) {
+    return guarded([&] {
+        shc::bfv_add_plain(ctx_of(ctx), ct, plain, out, count, size, limbs, subtract != 0, (hipStream_t)stream);
+    });
+}
+
+int shc_bfv_multiply_plain_workspace_size(const shc_context *ctx, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        if (!bytes) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        *bytes = shc::bfv_multiply_plain_workspace(ctx_of(ctx), limbs);
+    });
+}
+
+int shc_bfv_multiply_plain(const shc_context *ctx, const uint64_t *a, const uint64_t *plain, uint64_t *out,
+                           uint32_t polys, uint32_t limbs, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        shc::bfv_multiply_plain(ctx_of(ctx), a, plain, out, polys, limbs, ws, ws_bytes, (hipStream_t)stream);
     });
 }
 

@@ -46,6 +46,8 @@ void encrypt(const Context &c, const uint64_t *pk, const uint64_t *plain, uint32
              void *ws, size_t ws_bytes, hipStream_t st);
 // c0 += Delta m with SEAL's rounding fix, at level L (src/utils/scalingvariant.cpp:48-86).
 void bfv_add_plain_scaled(const Context &c, uint64_t *c0, const uint64_t *plain, uint32_t L, hipStream_t st);
+// c0 +- Delta m (multiplyAdd / multiplySubPlainWithScalingVariant, scalingvariant.cpp:53-131).
+void bfv_scale_add(const Context &c, uint64_t *c0, const uint64_t *plain, uint32_t L, bool subtract, hipStream_t st);
 
 // Decryption: phase = sum_i c_i s^i for ct [size][L][N]; BFV: out [N] mod t after
 // scale-and-round; CKKS: out [L][N] NTT-form plaintext (src/decryptor.cpp:79-330).

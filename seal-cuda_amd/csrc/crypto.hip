@@ -118,7 +118,7 @@ __device__ __forceinline__ uint64_t div128_small(u128 x, uint64_t t, uint64_t r_
 
 // c0[l][c] += m Delta_l + floor((m (Q mod t) + floor((t+1)/2)) / t)   (mod q_l)
 __global__ void bfv_scale_add_kernel(uint64_t *__restrict__ c0, const uint64_t *__restrict__ plain, BehzDev B,
-                                     const DevPrime *__restrict__ qp, int logn) {
+                                     const DevPrime *__restrict__ qp, int logn, bool subtract) {
     const size_t n = (size_t)1 << logn;
     const uint64_t t = B.t, half = (t + 1) >> 1;
     for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < n; c += (size_t)gridDim.x * TPB) {
@@ -130,7 +130,7 @@ __global__ void bfv_scale_add_kernel(uint64_t *__restrict__ c0, const uint64_t *
             const DevPrime &Q = qp[l];
             const uint64_t v = addmod(shoup(m, B.delta[2 * l], B.delta[2 * l + 1], Q.q), barrett64(fix, Q.q, Q.ratio_hi),
                                       Q.q);
-            c0[l * n + c] = addmod(c0[l * n + c], v, Q.q);
+            c0[l * n + c] = subtract ? submod(c0[l * n + c], v, Q.q) : addmod(c0[l * n + c], v, Q.q);
         }
     }
 }
@@ -329,10 +329,14 @@ void encrypt_zero_asymmetric(const Context &c, const uint64_t *pk, const uint64_
     if (L1 != L) rescale_to_next(c, z, out, 1, 2, L1, rws, rescale_workspace(c, 1, 2, L1), st);
 }
 
-void bfv_add_plain_scaled(const Context &c, uint64_t *c0, const uint64_t *plain, uint32_t L, hipStream_t st) {
+void bfv_scale_add(const Context &c, uint64_t *c0, const uint64_t *plain, uint32_t L, bool subtract, hipStream_t st) {
     const BehzDev &B = c.behz(L).dev;
-    bfv_scale_add_kernel<<<blocks_for(c.n), TPB, 0, st>>>(c0, plain, B, c.primes(), c.logn);
+    bfv_scale_add_kernel<<<blocks_for(c.n), TPB, 0, st>>>(c0, plain, B, c.primes(), c.logn, subtract);
     launch_check("bfv_scale_add");
+}
+
+void bfv_add_plain_scaled(const Context &c, uint64_t *c0, const uint64_t *plain, uint32_t L, hipStream_t st) {
+    bfv_scale_add(c, c0, plain, L, false, st);
 }
 
 void encrypt(const Context &c, const uint64_t *pk, const uint64_t *plain, uint32_t L, uint64_t seed, uint64_t *out,

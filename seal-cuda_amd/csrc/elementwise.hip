@@ -1,0 +1,164 @@
+// elementwise.hip -- coefficient-wise evaluator operations.
+//
+// Evaluator::negateInplace / addInplace / subInplace (src/evaluator.cpp:149-357; GPU
+// src/evaluator_cuda.cu:145-281), addPlainInplace / subPlainInplace (evaluator.cpp:
+// 1603-1764, GPU :1544-1690; BFV through src/utils/scalingvariant.cpp:53-131),
+// multiplyPlainNtt (evaluator.cpp:1932, GPU :1811) and the GPU multiplyPlainNormal
+// (src/evaluator_cuda.cu:1726-1807: the plaintext is always lifted, m >= ceil(t/2) ->
+// m + q_i - t, then NTT, dyadic product, inverse NTT).  The CPU multiplyPlainNormal
+// multiplies by a monomial plaintext without lifting it (evaluator.cpp:1818-1866), so
+// for a monomial with m >= ceil(t/2) the two reference paths give different words;
+// this build follows the GPU path it replaces.
+#include "elementwise.h"
+#include "behz.h"
+#include "crypto.h"
+#include "modarith.h"
+#include "ntt.h"
+
+namespace shc {
+namespace {
+
+constexpr int TPB = 256;
+
+inline uint32_t blocks_for(size_t n) {
+    const size_t b = (n + TPB - 1) / TPB;
+    return (uint32_t)(b < (size_t(1) << 20) ? b : (size_t(1) << 20));
+}
+
+enum BinOp { BIN_ADD, BIN_SUB, BIN_MUL_PLAIN };
+
+template <int OP>
+__global__ void binary_kernel(const uint64_t *__restrict__ a, const uint64_t *__restrict__ b, uint64_t *out,
+                              const DevPrime *__restrict__ primes, uint32_t L, int logn, size_t total) {
+    const size_t limb_words = (size_t)L << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t r = idx % limb_words;
+        const DevPrime &P = primes[r >> logn];
+        const uint64_t x = a[idx];
+        uint64_t o;
+        if (OP == BIN_ADD) o = addmod(x, b[idx], P.q);
+        else if (OP == BIN_SUB) o = submod(x, b[idx], P.q);
+        else o = mulmod(x, b[r], P.q, P.ratio_lo, P.ratio_hi);
+        out[idx] = o;
+    }
+}
+
+__global__ void negate_kernel(const uint64_t *__restrict__ a, uint64_t *out, const DevPrime *__restrict__ primes,
+                              uint32_t L, int logn, size_t total) {
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB)
+        out[idx] = negmod(a[idx], primes[(idx >> logn) % L].q);
+}
+
+__global__ void add_plain_poly0_kernel(const uint64_t *ct, const uint64_t *__restrict__ plain, uint64_t *out,
+                                       const DevPrime *__restrict__ primes, uint32_t size, uint32_t L, int logn,
+                                       bool subtract, size_t total) {
+    const size_t limb_words = (size_t)L << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t r = idx % limb_words;
+        const uint64_t x = ct[idx];
+        if ((idx / limb_words) % size == 0) {
+            const uint64_t q = primes[r >> logn].q;
+            out[idx] = subtract ? submod(x, plain[r], q) : addmod(x, plain[r], q);
+        } else {
+            out[idx] = x;
+        }
+    }
+}
+
+// lifted[l][c] = (m - t [m >= ceil(t/2)]) mod q_l for m = plain[c] < t.
+__global__ void plain_lift_kernel(const uint64_t *__restrict__ plain, uint64_t *__restrict__ lifted, uint64_t t,
+                                  const DevPrime *__restrict__ primes, uint32_t L, int logn, size_t total) {
+    const uint64_t thr = (t + 1) >> 1;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t c = idx & (((size_t)1 << logn) - 1);
+        const DevPrime &P = primes[idx >> logn];
+        const uint64_t m = plain[c];
+        const uint64_t mq = m >= P.q ? barrett64(m, P.q, P.ratio_hi) : m;
+        lifted[idx] = m >= thr ? submod(mq, t >= P.q ? barrett64(t, P.q, P.ratio_hi) : t, P.q) : mq;
+    }
+}
+
+void check_level(const Context &c, uint32_t L) {
+    if (L < 1 || L > c.key_count()) fail(ERR_INVALID_ARGUMENT, "limb count out of range");
+}
+
+}  // namespace
+
+void poly_add(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t polys, uint32_t L,
+              hipStream_t st) {
+    check_level(c, L);
+    const size_t total = ((size_t)polys * L) << c.logn;
+    if (!total) return;
+    binary_kernel<BIN_ADD><<<blocks_for(total), TPB, 0, st>>>(a, b, out, c.primes(), L, c.logn, total);
+    launch_check("add");
+}
+
+void poly_sub(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t polys, uint32_t L,
+              hipStream_t st) {
+    check_level(c, L);
+    const size_t total = ((size_t)polys * L) << c.logn;
+    if (!total) return;
+    binary_kernel<BIN_SUB><<<blocks_for(total), TPB, 0, st>>>(a, b, out, c.primes(), L, c.logn, total);
+    launch_check("sub");
+}
+
+void poly_negate(const Context &c, const uint64_t *a, uint64_t *out, uint32_t polys, uint32_t L, hipStream_t st) {
+    check_level(c, L);
+    const size_t total = ((size_t)polys * L) << c.logn;
+    if (!total) return;
+    negate_kernel<<<blocks_for(total), TPB, 0, st>>>(a, out, c.primes(), L, c.logn, total);
+    launch_check("negate");
+}
+
+void multiply_plain_ntt(const Context &c, const uint64_t *a, const uint64_t *plain, uint64_t *out, uint32_t polys,
+                        uint32_t L, hipStream_t st) {
+    check_level(c, L);
+    const size_t total = ((size_t)polys * L) << c.logn;
+    if (!total) return;
+    binary_kernel<BIN_MUL_PLAIN><<<blocks_for(total), TPB, 0, st>>>(a, plain, out, c.primes(), L, c.logn, total);
+    launch_check("multiply_plain");
+}
+
+void add_plain_poly0(const Context &c, const uint64_t *ct, const uint64_t *plain, uint64_t *out, uint32_t count,
+                     uint32_t size, uint32_t L, bool subtract, hipStream_t st) {
+    check_level(c, L);
+    if (size < 1) fail(ERR_INVALID_ARGUMENT, "ciphertext size must be at least 1");
+    const size_t total = (((size_t)count * size) * L) << c.logn;
+    if (!total) return;
+    add_plain_poly0_kernel<<<blocks_for(total), TPB, 0, st>>>(ct, plain, out, c.primes(), size, L, c.logn, subtract,
+                                                              total);
+    launch_check("add_plain");
+}
+
+void bfv_add_plain(const Context &c, const uint64_t *ct, const uint64_t *plain, uint64_t *out, uint32_t count,
+                   uint32_t size, uint32_t L, bool subtract, hipStream_t st) {
+    if (c.scheme != SCHEME_BFV) fail(ERR_LOGIC, "scaled plaintext addition is for BFV");
+    if (L < 1 || L >= c.key_count()) fail(ERR_INVALID_ARGUMENT, "level out of range");
+    if (size < 1) fail(ERR_INVALID_ARGUMENT, "ciphertext size must be at least 1");
+    const size_t ct_words = ((size_t)size * L) << c.logn;
+    if (out != ct) SHC_HIP(hipMemcpyAsync(out, ct, (size_t)count * ct_words * 8, hipMemcpyDeviceToDevice, st));
+    for (uint32_t i = 0; i < count; i++) bfv_scale_add(c, out + i * ct_words, plain, L, subtract, st);
+}
+
+size_t bfv_multiply_plain_workspace(const Context &c, uint32_t L) { return ((size_t)L << c.logn) * 8; }
+
+void bfv_multiply_plain(const Context &c, const uint64_t *a, const uint64_t *plain, uint64_t *out, uint32_t polys,
+                        uint32_t L, void *ws, size_t ws_bytes, hipStream_t st) {
+    if (c.scheme != SCHEME_BFV || !c.plain) fail(ERR_LOGIC, "coefficient-form plain multiply is for BFV");
+    check_level(c, L);
+    if (!ws || ws_bytes < bfv_multiply_plain_workspace(c, L))
+        fail(ERR_INVALID_ARGUMENT, "workspace too small for plain multiply");
+    const size_t words = ((size_t)polys * L) << c.logn;
+    if (!words) return;
+    uint64_t *lifted = static_cast<uint64_t *>(ws);
+    const size_t lw = (size_t)L << c.logn;
+    plain_lift_kernel<<<blocks_for(lw), TPB, 0, st>>>(plain, lifted, c.plain, c.primes(), L, c.logn, lw);
+    launch_check("plain_lift");
+    ntt_forward(lifted, L, L, c.primes(), c.logn, NTT_FULL, st);
+    if (out != a) SHC_HIP(hipMemcpyAsync(out, a, words * 8, hipMemcpyDeviceToDevice, st));
+    ntt_forward(out, polys * L, L, c.primes(), c.logn, NTT_FULL, st);
+    multiply_plain_ntt(c, out, lifted, out, polys, L, st);
+    ntt_inverse(out, polys * L, L, c.primes(), c.logn, NTT_FULL, st);
+}
+
+}  // namespace shc

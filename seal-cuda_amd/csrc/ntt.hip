@@ -25,8 +25,6 @@
 namespace shc {
 namespace {
 
-__host__ __device__ constexpr int lds_pad(int a) { return a + (a >> 4); }
-
 // Twiddle tables are global memory; saying so lets the compiler use global loads
 // (flat loads would also hold up the LDS waits).
 __device__ __forceinline__ ulonglong2 load_tw(const ulonglong2 *tw, int i) {
@@ -68,12 +66,14 @@ __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int 
     }
 }
 
-// Runs all phases of a sub-transform whose values sit in LDS; `addr(idx)` maps an
-// index of this thread's sub-transform to its LDS slot.
-template <int LOGN, int K, int E, bool INV, int P = 0, class Addr>
-__device__ __forceinline__ void run_phases(uint64_t *tile, const Addr &addr, int t, int g, int L0,
-                                           const ulonglong2 *__restrict__ tw, uint64_t four_q, uint64_t neg_q,
-                                           bool active) {
+// Runs all phases of a sub-transform.  The first phase reads its registers straight
+// from global memory through `load(idx)` and the last writes them back through
+// `store(idx, value)`; in between, values cross threads through LDS at `addr(idx)`,
+// so a pass makes one LDS round trip per phase boundary and one barrier each.
+template <int LOGN, int K, int E, bool INV, int P = 0, class Addr, class Load, class Store>
+__device__ __forceinline__ void run_phases(uint64_t *tile, const Addr &addr, const Load &load, const Store &store,
+                                           int t, int g, int L0, const ulonglong2 *__restrict__ tw,
+                                           uint64_t four_q, uint64_t neg_q, bool active) {
     constexpr int NP = (K + E - 1) / E;
     if constexpr (P < NP) {
         // forward runs phases from layer 0 upwards, inverse from the top layer down
@@ -83,15 +83,28 @@ __device__ __forceinline__ void run_phases(uint64_t *tile, const Addr &addr, int
         uint64_t r[1 << E];
         if (active) {
 #pragma unroll
-            for (int j = 0; j < (1 << E); j++) r[j] = tile[addr(phase_index<K, E, S0, EP>(t, j))];
+            for (int j = 0; j < (1 << E); j++) {
+                const int idx = phase_index<K, E, S0, EP>(t, j);
+                r[j] = P == 0 ? load(idx) : tile[addr(idx)];
+            }
             phase_compute<LOGN, K, E, S0, EP, INV>(r, t, g, L0, tw, four_q, neg_q);
 #pragma unroll
-            for (int j = 0; j < (1 << E); j++) tile[addr(phase_index<K, E, S0, EP>(t, j))] = r[j];
+            for (int j = 0; j < (1 << E); j++) {
+                const int idx = phase_index<K, E, S0, EP>(t, j);
+                if (P == NP - 1) store(idx, r[j]);
+                else tile[addr(idx)] = r[j];
+            }
         }
-        __syncthreads();
-        run_phases<LOGN, K, E, INV, P + 1>(tile, addr, t, g, L0, tw, four_q, neg_q, active);
+        if constexpr (P + 1 < NP) {
+            __syncthreads();
+            run_phases<LOGN, K, E, INV, P + 1>(tile, addr, load, store, t, g, L0, tw, four_q, neg_q, active);
+        }
     }
 }
+
+// LDS slot of element idx of a row: XOR-swizzled by idx's high nibble so that both
+// the "stride 16" and the "16 consecutive" phase patterns hit distinct banks.
+__host__ __device__ constexpr int swz(int idx) { return idx ^ ((idx >> 4) & 15); }
 
 // Final touch of a value leaving the last pass: forward [0, 8q) -> [0, 4q) (lazy) or
 // [0, q); inverse [0, 4q) times N^-1 (exact Shoup) -> [0, 2q) (lazy) or [0, q).
@@ -121,7 +134,8 @@ __device__ __forceinline__ void rows_body(uint64_t *__restrict__ data, const Dev
     constexpr int T = 1 << (K - E);
     constexpr int SZ = 1 << K;
     constexpr int RPL = LOGN - K;  // log2(rows per limb)
-    __shared__ uint64_t tile[lds_pad(SUBS * SZ)];
+    constexpr int NP = (K + E - 1) / E;
+    __shared__ uint64_t tile[NP > 1 ? SUBS * SZ : 1];
     const uint32_t jc_count = (jn + SUBS - 1) / SUBS;
     const uint32_t jc = blockIdx.x % jc_count;
     const uint32_t r = blockIdx.x / jc_count;
@@ -129,22 +143,31 @@ __device__ __forceinline__ void rows_body(uint64_t *__restrict__ data, const Dev
     const uint32_t i = r >> RPL;
     const uint32_t jfirst = jc * SUBS;
     const int nsub = (int)min<uint32_t>(SUBS, jn - jfirst);
-    const uint32_t limb0 = i * lg.A + (j0 + jfirst) * lg.B;
-    auto row_ptr = [&](int sub) { return data + (((size_t)(limb0 + (uint32_t)sub * lg.B)) << LOGN) + ((size_t)g << K); };
     const int tid = threadIdx.x;
-    for (int e = tid; e < nsub * SZ; e += SUBS * T) tile[lds_pad(e)] = row_ptr(e / SZ)[e % SZ];
-    __syncthreads();
-
     const int sub = tid / T, t = tid % T;
+    const bool active = sub < nsub;
+    uint64_t *row = data + (((size_t)(i * lg.A + (j0 + jfirst + (uint32_t)(active ? sub : 0)) * lg.B)) << LOGN) +
+                    ((size_t)g << K);
     const DevPrime &P = primes[i];
     const ulonglong2 *tw = INV ? P.inv : P.fwd;
-    auto addr = [sub](int idx) { return lds_pad(sub * SZ + idx); };
-    run_phases<LOGN, K, E, INV>(tile, addr, t, g, RPL, tw, P.q << 2, 0 - P.q, sub < nsub);
-
-    for (int e = tid; e < nsub * SZ; e += SUBS * T) {
-        uint64_t x = tile[lds_pad(e)];
-        if (LAST) x = finish<INV, FULL>(x, P);
-        row_ptr(e / SZ)[e % SZ] = x;
+    auto addr = [sub](int idx) { return sub * SZ + swz(idx); };
+    auto load = [row](int idx) { return row[idx]; };
+    if constexpr (NP == 1) {
+        auto store = [row, &P](int idx, uint64_t x) { row[idx] = LAST ? finish<INV, FULL>(x, P) : x; };
+        run_phases<LOGN, K, E, INV>(tile, addr, load, store, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
+    } else {
+        // the last phase leaves 16 consecutive values per thread; stage them in LDS so
+        // that the block writes whole rows with consecutive lanes on consecutive words
+        auto stage = [&](int idx, uint64_t x) { tile[addr(idx)] = x; };
+        run_phases<LOGN, K, E, INV>(tile, addr, load, stage, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
+        __syncthreads();
+        const uint32_t row0 = i * lg.A + (j0 + jfirst) * lg.B;
+        for (int e = tid; e < nsub * SZ; e += SUBS * T) {
+            const int sb = e / SZ, off = e % SZ;
+            const uint64_t x = tile[sb * SZ + swz(off)];
+            data[(((size_t)(row0 + (uint32_t)sb * lg.B)) << LOGN) + ((size_t)g << K) + off] =
+                LAST ? finish<INV, FULL>(x, P) : x;
+        }
     }
 }
 
@@ -173,24 +196,19 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
     constexpr int SZ = 1 << K;
     constexpr int C = 1 << (LOGN - K);
     constexpr int TILES = C / SUBS;
-    __shared__ uint64_t tile[lds_pad(SUBS * SZ)];
+    __shared__ uint64_t tile[SUBS * SZ];
     const int tid = threadIdx.x;
     const int col = tid % SUBS, t = tid / SUBS;
     const uint32_t r = blockIdx.x / TILES;
     const int c0 = (int)(blockIdx.x % TILES) * SUBS;
     const uint32_t i = r / jn, j = j0 + r % jn;
-    uint64_t *base = data + ((size_t)(i * lg.A + j * lg.B) << LOGN) + c0;
+    uint64_t *base = data + ((size_t)(i * lg.A + j * lg.B) << LOGN) + c0 + col;
     const DevPrime &P = primes[i];
-    for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E)) tile[lds_pad(e)] = base[(e / SUBS) * C + (e % SUBS)];
-    __syncthreads();
     const ulonglong2 *tw = INV ? P.inv : P.fwd;
-    auto addr = [col](int idx) { return lds_pad(idx * SUBS + col); };
-    run_phases<LOGN, K, E, INV>(tile, addr, t, 0, 0, tw, P.q << 2, 0 - P.q, true);
-    for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E)) {
-        uint64_t x = tile[lds_pad(e)];
-        if (LAST) x = finish<INV, FULL>(x, P);
-        base[(e / SUBS) * C + (e % SUBS)] = x;
-    }
+    auto addr = [col](int idx) { return idx * SUBS + col; };
+    auto load = [base](int idx) { return base[(size_t)idx * C]; };
+    auto store = [base, &P](int idx, uint64_t x) { base[(size_t)idx * C] = LAST ? finish<INV, FULL>(x, P) : x; };
+    run_phases<LOGN, K, E, INV>(tile, addr, load, store, t, 0, 0, tw, P.q << 2, 0 - P.q, true);
 }
 
 // ModUp columns pass into T[i][ct][j] (target-major, so that the rows pass walks one
@@ -209,7 +227,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
     constexpr int SZ = 1 << K;
     constexpr int C = 1 << (LOGN - K);
     constexpr int TILES = C / SUBS;
-    __shared__ uint64_t tile[lds_pad(SUBS * SZ)];
+    __shared__ uint64_t tile[SUBS * SZ];
     const int tid = threadIdx.x;
     const int col = tid % SUBS, t = tid / SUBS;
     const uint32_t task = blockIdx.x / TILES;  // (ct * L + j) * (L + 1) + i
@@ -218,22 +236,19 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
     const uint32_t r = task / (L + 1);
     const uint32_t j = r % L;
     const size_t ct = r / L;
-    const uint64_t *in = d + ct * d_ct_stride + ((size_t)j << LOGN) + c0;
-    uint64_t *out = T + ((((size_t)i * count + ct) * L + j) << LOGN) + c0;
+    const uint64_t *in = d + ct * d_ct_stride + ((size_t)j << LOGN) + c0 + col;
+    uint64_t *out = T + ((((size_t)i * count + ct) * L + j) << LOGN) + c0 + col;
     const DevPrime &P = targets[i];
-    if (digits[j].q <= 8 * P.q) {
-        for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E)) tile[lds_pad(e)] = in[(e / SUBS) * C + (e % SUBS)];
-    } else {
-        for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E))
-            tile[lds_pad(e)] = barrett64(in[(e / SUBS) * C + (e % SUBS)], P.q, P.ratio_hi);
-    }
-    __syncthreads();
-    auto addr = [col](int idx) { return lds_pad(idx * SUBS + col); };
-    run_phases<LOGN, K, E, false>(tile, addr, t, 0, 0, P.fwd, P.q << 2, 0 - P.q, true);
-    for (int e = tid; e < SUBS * SZ; e += SUBS << (K - E)) out[(e / SUBS) * C + (e % SUBS)] = tile[lds_pad(e)];
+    const bool reduce = digits[j].q > 8 * P.q;
+    auto addr = [col](int idx) { return idx * SUBS + col; };
+    auto load = [in, reduce, &P](int idx) {
+        const uint64_t x = in[(size_t)idx * C];
+        return reduce ? barrett64(x, P.q, P.ratio_hi) : x;
+    };
+    auto store = [out](int idx, uint64_t x) { out[(size_t)idx * C] = x; };
+    run_phases<LOGN, K, E, false>(tile, addr, load, store, t, 0, 0, P.fwd, P.q << 2, 0 - P.q, true);
 }
 
-// Transform of limbs (i, j), i < lg.I, j0 <= j < j0 + jn.
 template <int LOGN, bool INV, bool FULL>
 void launch_ntt(uint64_t *data, LimbGrid lg, uint32_t j0, uint32_t jn, const DevPrime *primes, hipStream_t st) {
     if constexpr (LOGN <= 12) {

@@ -73,6 +73,14 @@ def lib():
         L.shc_rescale_workspace_size.argtypes = [_vp, u32, u32, u32, ctypes.POINTER(sz)]
         L.shc_rescale_to_next.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
         L.shc_probe_keyswitch.argtypes = [_vp, _vp, _vp]
+        for name in ("shc_add", "shc_sub"):
+            getattr(L, name).argtypes = [_vp, _vp, _vp, _vp, u32, u32, _vp]
+        L.shc_negate.argtypes = [_vp, _vp, _vp, u32, u32, _vp]
+        L.shc_multiply_plain_ntt.argtypes = [_vp, _vp, _vp, _vp, u32, u32, _vp]
+        L.shc_add_plain.argtypes = [_vp, _vp, _vp, _vp, u32, u32, u32, ctypes.c_int, _vp]
+        L.shc_bfv_add_plain.argtypes = [_vp, _vp, _vp, _vp, u32, u32, u32, ctypes.c_int, _vp]
+        L.shc_bfv_multiply_plain_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
+        L.shc_bfv_multiply_plain.argtypes = [_vp, _vp, _vp, _vp, u32, u32, _vp, sz, _vp]
         _lib = L
     return _lib
 
@@ -261,6 +269,30 @@ class Context:
 
     def rescale_to_next(self, d_in, d_out, count, polys, limbs, d_ws, ws_bytes, stream=None):
         check(lib().shc_rescale_to_next(self.handle, d_in, d_out, count, polys, limbs, d_ws, ws_bytes, stream))
+
+    def add(self, a, b, out, polys, limbs, stream=None):
+        check(lib().shc_add(self.handle, a, b, out, polys, limbs, stream))
+
+    def sub(self, a, b, out, polys, limbs, stream=None):
+        check(lib().shc_sub(self.handle, a, b, out, polys, limbs, stream))
+
+    def negate(self, a, out, polys, limbs, stream=None):
+        check(lib().shc_negate(self.handle, a, out, polys, limbs, stream))
+
+    def multiply_plain_ntt(self, a, plain, out, polys, limbs, stream=None):
+        check(lib().shc_multiply_plain_ntt(self.handle, a, plain, out, polys, limbs, stream))
+
+    def add_plain(self, ct, plain, out, count, size, limbs, subtract=False, stream=None):
+        check(lib().shc_add_plain(self.handle, ct, plain, out, count, size, limbs, int(subtract), stream))
+
+    def bfv_add_plain(self, ct, plain, out, count, size, limbs, subtract=False, stream=None):
+        check(lib().shc_bfv_add_plain(self.handle, ct, plain, out, count, size, limbs, int(subtract), stream))
+
+    def bfv_multiply_plain_workspace_size(self, limbs):
+        return self._size(lib().shc_bfv_multiply_plain_workspace_size, limbs)
+
+    def bfv_multiply_plain(self, a, plain, out, polys, limbs, d_ws, ws_bytes, stream=None):
+        check(lib().shc_bfv_multiply_plain(self.handle, a, plain, out, polys, limbs, d_ws, ws_bytes, stream))
 
     def probe_keyswitch(self, begin, end):
         """Arm the next key switch to record HipEvents around its ModUp rows pass."""

@@ -67,6 +67,9 @@ def oracle():
         lib.or_decrypt_phase.argtypes = [ctypes.c_int, ctypes.c_int, u64p, sz, u64p, sz, u64p, sz, u64p]
         lib.or_bfv_decrypt_scale_round.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p]
         lib.or_bfv_add_plain_scaled.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, sz, u64p]
+        lib.or_bfv_sub_plain_scaled.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, sz, u64p]
+        lib.or_multiply_plain_ntt.argtypes = [ctypes.c_int, u64p, sz, u64p, sz, u64p, u64p]
+        lib.or_bfv_multiply_plain.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, sz, u64p, u64p]
         lib.or_batch_index_map.argtypes = [ctypes.c_int, u64p]
         lib.or_batch_encode.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, sz, u64p]
         lib.or_batch_decode.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, u64p]
@@ -108,6 +111,8 @@ def ref():
         lib.ref_batch_decode.argtypes = [ctypes.c_void_p, u64p, u64p]
         lib.ref_ckks_encode.argtypes = [ctypes.c_void_p, u64, dp, u64, ctypes.c_double, u64p]
         lib.ref_ckks_decode.argtypes = [ctypes.c_void_p, u64, u64p, ctypes.c_double, dp]
+        lib.ref_ct_binary.argtypes = [ctypes.c_void_p, u64, u64, ctypes.c_int, u64p, u64p, u64p]
+        lib.ref_plain_op.argtypes = [ctypes.c_void_p, u64, u64, ctypes.c_int, u64p, u64p, u64p]
         _ref = lib
     return _ref
 
@@ -259,6 +264,54 @@ def bfv_add_plain_scaled(c0, plain, moduli, t, logn):
     return c
 
 
+def bfv_sub_plain_scaled(c0, plain, moduli, t, logn):
+    c = np.ascontiguousarray(c0, dtype=np.uint64).copy()
+    L = c.shape[0]
+    p = np.ascontiguousarray(plain, dtype=np.uint64)
+    assert oracle().or_bfv_sub_plain_scaled(logn, ptr(_arr(moduli[:L])), L, t, ptr(p), len(p), ptr(c)) == 0
+    return c
+
+
+def multiply_plain_ntt(a, plain, moduli, logn):
+    """a: [polys][L][N] NTT form, plain [L][N] NTT form"""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    polys, L, n = a.shape
+    out = np.zeros_like(a)
+    assert oracle().or_multiply_plain_ntt(logn, ptr(_arr(moduli[:L])), L, ptr(a), polys,
+                                          ptr(np.ascontiguousarray(plain, dtype=np.uint64)), ptr(out)) == 0
+    return out
+
+
+def bfv_multiply_plain(a, plain, moduli, t, logn):
+    """a: [polys][L][N] coefficient form, plain [N] mod t"""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    polys, L, n = a.shape
+    out = np.zeros_like(a)
+    assert oracle().or_bfv_multiply_plain(logn, ptr(_arr(moduli[:L])), L, t, ptr(a), polys,
+                                          ptr(np.ascontiguousarray(plain, dtype=np.uint64)), ptr(out)) == 0
+    return out
+
+
+def mod_add(a, b, moduli):
+    """[..., L, N] + [..., L, N] mod moduli[l] (closed form)"""
+    q = np.array(moduli[:a.shape[-2]], dtype=np.uint64)[:, None]
+    s = a.astype(np.uint64) + b.astype(np.uint64)
+    return np.where(s >= q, s - q, s)
+
+
+def mod_sub(a, b, moduli):
+    q = np.array(moduli[:a.shape[-2]], dtype=np.uint64)[:, None]
+    a = a.astype(np.uint64)
+    b = b.astype(np.uint64)
+    return np.where(a >= b, a - b, a + q - b)
+
+
+def mod_neg(a, moduli):
+    q = np.array(moduli[:a.shape[-2]], dtype=np.uint64)[:, None]
+    a = a.astype(np.uint64)
+    return np.where(a == 0, a, q - a)
+
+
 def batch_encode(values, t, logn):
     v = np.ascontiguousarray(values, dtype=np.uint64)
     out = np.zeros(1 << logn, dtype=np.uint64)
@@ -367,6 +420,22 @@ class RefSession:
         assert ref().ref_ckks_decode(self.h, L, ptr(np.ascontiguousarray(plain, dtype=np.uint64)), scale,
                                      dptr(out)) == 0
         return out[0::2] + 1j * out[1::2]
+
+    def ct_binary(self, op, a, b, L):
+        """op 0 add, 1 sub, 2 negate (b ignored) on ciphertexts [size][L][N]"""
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        out = np.zeros_like(a)
+        bb = a if b is None else np.ascontiguousarray(b, dtype=np.uint64)
+        assert ref().ref_ct_binary(self.h, L, a.shape[0], op, ptr(a), ptr(bb), ptr(out)) == 0
+        return out
+
+    def plain_op(self, op, ct, plain, L):
+        """op 0 addPlain, 1 subPlain, 2 multiplyPlain on ct [size][L][N]"""
+        ct = np.ascontiguousarray(ct, dtype=np.uint64)
+        out = np.zeros_like(ct)
+        assert ref().ref_plain_op(self.h, L, ct.shape[0], op, ptr(ct), ptr(np.ascontiguousarray(plain, dtype=np.uint64)),
+                                  ptr(out)) == 0
+        return out
 
     def rescale(self, ct, L):
         size = ct.shape[0]

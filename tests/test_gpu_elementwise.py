@@ -1,0 +1,109 @@
+"""GPU parity of the coefficient-wise evaluator operations.
+
+add / sub / negate and the plaintext operations (CKKS add/sub/multiply by an NTT-form
+plaintext; BFV add/sub of Delta m and the lifted plain multiply) equal the oracle word
+for word on batches of ciphertexts, in place and out of place, including a plain
+modulus larger than every data prime (no fast plain lift).
+"""
+import numpy as np
+import pytest
+
+import checkers as ck
+from gpu_util import empty_dev, from_dev, hip, stream, to_dev
+
+pytestmark = pytest.mark.gpu
+BFV, CKKS = 1, 2
+
+
+def ws(nbytes):
+    return empty_dev((max(nbytes, 8) + 7) // 8)
+
+
+@pytest.mark.parametrize("scheme,logn,bits,L,count", [
+    (CKKS, 12, [60, 40, 40, 60], 3, 3),
+    (BFV, 13, [43, 43, 44, 44, 44], 4, 2),
+    (CKKS, 16, [60] * 5, 4, 2),
+])
+def test_add_sub_negate(scheme, logn, bits, L, count):
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, bits)
+    ctx = hip().Context(scheme, n, moduli, 65537 if scheme == BFV else 0)
+    rng = np.random.default_rng(logn)
+    a = ck.uniform_residues(rng, (count, 2, L, n), moduli[:L])
+    b = ck.uniform_residues(rng, (count, 2, L, n), moduli[:L])
+    da, db = to_dev(a), to_dev(b)
+    out = empty_dev((count, 2, L, n))
+    polys = count * 2
+    ctx.add(da.data_ptr(), db.data_ptr(), out.data_ptr(), polys, L, stream())
+    assert np.array_equal(from_dev(out), ck.mod_add(a, b, moduli))
+    ctx.sub(da.data_ptr(), db.data_ptr(), out.data_ptr(), polys, L, stream())
+    assert np.array_equal(from_dev(out), ck.mod_sub(a, b, moduli))
+    ctx.negate(da.data_ptr(), out.data_ptr(), polys, L, stream())
+    assert np.array_equal(from_dev(out), ck.mod_neg(a, moduli))
+    # in place: a += b
+    ctx.add(da.data_ptr(), db.data_ptr(), da.data_ptr(), polys, L, stream())
+    assert np.array_equal(from_dev(da), ck.mod_add(a, b, moduli))
+
+
+@pytest.mark.parametrize("logn,bits,L,count", [(12, [60, 40, 40, 60], 3, 3), (15, [60] + [40] * 6 + [60], 7, 2)])
+def test_ckks_plain_ops(logn, bits, L, count):
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, bits)
+    ctx = hip().Context(CKKS, n, moduli)
+    rng = np.random.default_rng(logn + 1)
+    a = ck.uniform_residues(rng, (count, 2, L, n), moduli[:L])
+    p = ck.uniform_residues(rng, (L, n), moduli[:L])
+    da, dp = to_dev(a), to_dev(p)
+    out = empty_dev((count, 2, L, n))
+    for subtract in (False, True):
+        ctx.add_plain(da.data_ptr(), dp.data_ptr(), out.data_ptr(), count, 2, L, subtract, stream())
+        want = a.copy()
+        for c in range(count):
+            want[c, 0] = (ck.mod_sub if subtract else ck.mod_add)(a[c, 0], p, moduli)
+        assert np.array_equal(from_dev(out), want)
+    ctx.multiply_plain_ntt(da.data_ptr(), dp.data_ptr(), out.data_ptr(), count * 2, L, stream())
+    want = ck.multiply_plain_ntt(a.reshape(count * 2, L, n), p, moduli, logn).reshape(count, 2, L, n)
+    assert np.array_equal(from_dev(out), want)
+
+
+@pytest.mark.parametrize("logn,bits,L,t,count", [
+    (13, [43, 43, 44, 44, 44], 4, 65537, 2),
+    (12, [30, 30, 40], 2, (1 << 35) - 31, 3),
+    (16, [60] * 5, 4, 65537, 1),
+])
+def test_bfv_plain_ops(logn, bits, L, t, count):
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, bits)
+    ctx = hip().Context(BFV, n, moduli, t)
+    rng = np.random.default_rng(logn + 2)
+    a = ck.uniform_residues(rng, (count, 2, L, n), moduli[:L])
+    m = rng.integers(0, t, size=n, dtype=np.uint64)
+    m[0], m[1] = (t + 1) // 2, (t - 1) // 2
+    da, dm = to_dev(a), to_dev(m)
+    out = empty_dev((count, 2, L, n))
+    for subtract in (False, True):
+        ctx.bfv_add_plain(da.data_ptr(), dm.data_ptr(), out.data_ptr(), count, 2, L, subtract, stream())
+        want = a.copy()
+        f = ck.bfv_sub_plain_scaled if subtract else ck.bfv_add_plain_scaled
+        for c in range(count):
+            want[c, 0] = f(a[c, 0], m, moduli, t, logn)
+        assert np.array_equal(from_dev(out), want)
+    wb = ctx.bfv_multiply_plain_workspace_size(L)
+    w = ws(wb)
+    ctx.bfv_multiply_plain(da.data_ptr(), dm.data_ptr(), out.data_ptr(), count * 2, L, w.data_ptr(), wb, stream())
+    want = ck.bfv_multiply_plain(a.reshape(count * 2, L, n), m, moduli, t, logn).reshape(count, 2, L, n)
+    assert np.array_equal(from_dev(out), want)
+    # in place
+    ctx.bfv_multiply_plain(da.data_ptr(), dm.data_ptr(), da.data_ptr(), count * 2, L, w.data_ptr(), wb, stream())
+    assert np.array_equal(from_dev(da), want)
+
+
+def test_plain_ops_reject_bad_arguments():
+    n = 1 << 10
+    moduli = ck.coeff_modulus_create(n, [30, 30, 30])
+    ctx = hip().Context(CKKS, n, moduli)
+    d = empty_dev((2, 2, n))
+    with pytest.raises(hip().SealHipError):
+        ctx.add(d.data_ptr(), d.data_ptr(), d.data_ptr(), 2, 4, stream())   # more limbs than primes
+    with pytest.raises(hip().SealHipError):
+        ctx.bfv_add_plain(d.data_ptr(), d.data_ptr(), d.data_ptr(), 1, 2, 2, False, stream())   # CKKS context

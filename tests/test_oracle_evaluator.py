@@ -203,3 +203,78 @@ def test_bfv_multiply_matches_reference(logn, bits, L, t):
     a = ck.uniform_residues(rng, (2, L, n), moduli[:L])
     b = ck.uniform_residues(rng, (2, L, n), moduli[:L])
     assert np.array_equal(ck.bfv_multiply(a, b, moduli[:L], t, logn), sess.multiply(a, b, L))
+
+
+@ref_only
+@pytest.mark.ref
+@pytest.mark.parametrize("scheme,logn,bits,L,t", [
+    (CKKS, 10, [60, 40, 40, 60], 3, 0),
+    (BFV, 10, [40, 40, 50], 2, 65537),
+    (BFV, 12, [43, 43, 44, 44, 44], 4, 65537),
+    (BFV, 11, [30, 30, 40], 2, (1 << 35) - 31),   # t above every q_i: no fast plain lift
+])
+def test_elementwise_matches_reference(scheme, logn, bits, L, t):
+    """add / sub / negate (closed form) and the plaintext operations (oracle) equal the
+    reference Evaluator's words on the same inputs."""
+    n, moduli = params(logn, bits)
+    if t and t > min(moduli[:L]):
+        # t must be prime-free relative to q only for batching; any t works for plain ops
+        pass
+    sess = ck.RefSession(scheme, n, moduli, t)
+    rng = np.random.default_rng(logn * 7 + L)
+    a = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    b = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    assert np.array_equal(ck.mod_add(a, b, moduli), sess.ct_binary(0, a, b, L))
+    assert np.array_equal(ck.mod_sub(a, b, moduli), sess.ct_binary(1, a, b, L))
+    assert np.array_equal(ck.mod_neg(a, moduli), sess.ct_binary(2, a, None, L))
+    if scheme == CKKS:
+        p = ck.uniform_residues(rng, (L, n), moduli[:L])
+        want = a.copy()
+        want[0] = ck.mod_add(a[0], p, moduli)
+        assert np.array_equal(want, sess.plain_op(0, a, p, L))
+        want[0] = ck.mod_sub(a[0], p, moduli)
+        assert np.array_equal(want, sess.plain_op(1, a, p, L))
+        assert np.array_equal(ck.multiply_plain_ntt(a, p, moduli, logn), sess.plain_op(2, a, p, L))
+    else:
+        m = rng.integers(0, t, size=n, dtype=np.uint64)
+        want = a.copy()
+        want[0] = ck.bfv_add_plain_scaled(a[0], m, moduli, t, logn)
+        assert np.array_equal(want, sess.plain_op(0, a, m, L))
+        want[0] = ck.bfv_sub_plain_scaled(a[0], m, moduli, t, logn)
+        assert np.array_equal(want, sess.plain_op(1, a, m, L))
+        assert np.array_equal(ck.bfv_multiply_plain(a, m, moduli, t, logn), sess.plain_op(2, a, m, L))
+        # a monomial below ceil(t/2) takes the CPU's monomial path without a lift: same words
+        mono = np.zeros(n, dtype=np.uint64)
+        mono[5] = (t - 1) // 2
+        assert np.array_equal(ck.bfv_multiply_plain(a, mono, moduli, t, logn), sess.plain_op(2, a, mono, L))
+
+
+@pytest.mark.parametrize("logn,bits,L,t", [(8, [40, 40, 50], 2, 65537), (9, [30, 30, 40], 2, (1 << 35) - 31)])
+def test_bfv_multiply_plain_is_negacyclic_product_with_lift(logn, bits, L, t):
+    """The oracle's BFV plain multiply equals the schoolbook negacyclic product of each
+    limb with the centred plaintext (m - t for m >= ceil(t/2)), computed with Python
+    integers -- a dropped lift, a wrong threshold or a cyclic wrap fails it."""
+    n, moduli = params(logn, bits)
+    rng = np.random.default_rng(11)
+    a = ck.uniform_residues(rng, (1, L, n), moduli[:L])
+    m = rng.integers(0, t, size=n, dtype=np.uint64)
+    m[0] = (t + 1) // 2   # exactly at the threshold: lifted
+    m[1] = (t - 1) // 2   # just below: not lifted
+    got = ck.bfv_multiply_plain(a, m, moduli, t, logn)
+    centred = [int(v) - t if int(v) >= (t + 1) // 2 else int(v) for v in m]
+    for i in range(L):
+        q = moduli[i]
+        x = [int(v) for v in a[0, i]]
+        acc = [0] * n
+        for u in range(n):
+            if x[u] == 0:
+                continue
+            for v in range(n):
+                if centred[v] == 0:
+                    continue
+                k = u + v
+                if k < n:
+                    acc[k] += x[u] * centred[v]
+                else:
+                    acc[k - n] -= x[u] * centred[v]
+        assert [c % q for c in acc] == [int(v) for v in got[0, i]]
