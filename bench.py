@@ -27,6 +27,10 @@ sys.path.insert(0, os.path.join(HERE, "seal-cuda_amd"))
 sys.path.insert(0, os.path.join(HERE, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the ModUp rows pass at the default workload (batch 128),
+# measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB device copy
+# (profiles/mulrelin_pmc_traffic.md); a profiler cannot run inside this process.
+ROWS_MODUP_TRAFFIC_B128 = 40.49e9 + 40.27e9
 CKKS = 2
 
 
@@ -40,6 +44,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-plain", action="store_true", help="skip the plain per-ciphertext leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--no-serve", action="store_true", help="skip the served-latency leg")
+    p.add_argument("--serve-seconds", type=float, default=1.5, help="arrival span of the served-latency leg")
     return p.parse_args()
 
 
@@ -92,6 +98,49 @@ def cpu_baseline_mulrelin(moduli, logn, budget_s):
     dt = time.perf_counter() - t0
     return {"value": ops / dt, "unit": "mul+relin ops/s", "cores": 1, "kind": kind,
             "sample": f"{ops} CKKS mul+relin at N=2^{logn}, L={L} (+1 special), one thread, {dt:.1f} s"}
+
+
+def served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, requests, max_batch, seed=7):
+    """Open-loop serving: `requests` mul+relin requests arrive as a Poisson process at
+    `rate` per second; each request's ciphertext pair sits in a ring of B resident
+    slots.  The server takes every request that has arrived (at most max_batch), runs
+    them as one batched multiply + relinearize and waits for it; a request's latency is
+    its batch's completion time minus its arrival time.  max_batch = 1 is the plain
+    per-ciphertext path under the same arrivals."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=requests))
+    s2, s3 = 2 * L * n * 8, 3 * L * n * 8
+
+    def run(slot, k):
+        ctx.ckks_multiply(a.data_ptr() + slot * s2, b.data_ptr() + slot * s2, out.data_ptr() + slot * s3, k, L, st)
+        ctx.relinearize(out.data_ptr() + slot * s3, key.data_ptr(), k, L, ws.data_ptr(), ws_bytes, st)
+
+    lat = np.empty(requests)
+    batches = 0
+    done = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while done < requests:
+        now = time.perf_counter() - t0
+        if arrivals[done] > now:
+            time.sleep(min(arrivals[done] - now, 1e-3))
+            continue
+        ready = int(np.searchsorted(arrivals, now, side="right")) - done
+        k = min(ready, max_batch)
+        slot = done % B
+        k1 = min(k, B - slot)
+        run(slot, k1)
+        if k > k1:
+            run(0, k - k1)
+        torch.cuda.synchronize()
+        fin = time.perf_counter() - t0
+        lat[done:done + k] = fin - arrivals[done:done + k]
+        done += k
+        batches += 1
+    span = time.perf_counter() - t0
+    return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_batch": requests / batches, "served_per_s": requests / span}
 
 
 def bench_mulrelin(args):
@@ -160,6 +209,15 @@ def bench_mulrelin(args):
         plain = {"ops_s": B / (tp.ms * 1e-3), "ms_per_step": tp.ms, "calls": 2 * B,
                  "self_check": f"batched == per-ciphertext words on ciphertexts {probe}"}
 
+    served = None
+    if not args.no_serve:
+        # arrivals at half the batched throughput, the batched server against the plain one
+        rate = 0.5 * ops_s
+        reqs = max(200, int(rate * args.serve_seconds))
+        served = {"arrivals": "poisson", "rate_per_s": rate, "requests": reqs,
+                  "batched": served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, reqs, B),
+                  "plain": served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, reqs, 1)}
+
     # ModUp rows pass: reads and writes every ModUp limb once (B x L x (L+1) limbs) and
     # reads the forward twiddle table of each of the L+1 target primes
     tlimbs = B * L * (L + 1)
@@ -181,7 +239,8 @@ def bench_mulrelin(args):
                                f"batch of {B} ciphertext pairs, NTT form, key resident",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved_rows, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_rows / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_rows / HBM_PEAK_GBS,
+                     "traffic": ROWS_MODUP_TRAFFIC_B128 if B == 128 else None,
                      "kernel": f"ntt_rows_modup_kernel<16,8,4,16>: key-switch ModUp NTT rows pass, {tlimbs} limbs "
                                f"x last {logn - logn // 2} layers",
                      "algorithmic_bytes": rows_bytes, "avg_ms": rows_ms,
@@ -190,7 +249,7 @@ def bench_mulrelin(args):
                           "algorithmic_bytes_per_op": unit_bytes},
         "plain": plain,
         "batched_over_plain": (ops_s / plain["ops_s"]) if plain else None,
-        "served_latency": None,
+        "served_latency": served,
         "wall_s": wall,
     }
     if not args.no_cpu:

@@ -105,7 +105,8 @@ int shc_switch_key(const shc_context *ctx, const uint64_t *d_target, size_t targ
                    size_t workspace_bytes, void *stream);
 
 /* Relinearize size-3 ciphertexts d_ct[count][3][L][N] in place: the size-2 result is
- * left in polys 0 and 1 of each ciphertext.
+ * left in polys 0 and 1 of each ciphertext; poly 2 is used as scratch (its words are
+ * unspecified afterwards).
  * Replaces EvaluatorCuda::relinearizeInplace (src/evaluator_cuda.cuh:101-105,
  * src/evaluator_cuda.cu:703-744). */
 int shc_relinearize(const shc_context *ctx, uint64_t *d_ct, const uint64_t *d_relin_key, uint32_t count,
@@ -247,6 +248,12 @@ int shc_bfv_multiply_plain_workspace_size(const shc_context *ctx, uint32_t limb_
 int shc_bfv_multiply_plain(const shc_context *ctx, const uint64_t *d_a, const uint64_t *d_plain, uint64_t *d_out,
                            uint32_t polys, uint32_t limb_count, void *d_workspace, size_t workspace_bytes,
                            void *stream);
+
+/* BFV: d_out [limb_count][N] = NTT of m lifted as in shc_bfv_multiply_plain, m = d_plain
+ * [N] mod t.  Replaces EvaluatorCuda::transformToNttInplace(PlaintextCuda&, ParmsID)
+ * (src/evaluator_cuda.cu:1869-1935). */
+int shc_bfv_plain_to_ntt(const shc_context *ctx, const uint64_t *d_plain, uint32_t limb_count, uint64_t *d_out,
+                         void *stream);
 
 /* Instrumentation (no counterpart in the reference; for benchmarks and profilers):
  * arms `ctx` so that its next key switch (shc_relinearize, shc_switch_key,

@@ -151,7 +151,8 @@ int shc_relinearize(const shc_context *ctx, uint64_t *ct, const uint64_t *key, u
         const auto &c = ctx_of(ctx);
         check_limbs(c, limbs);
         const size_t poly = (size_t)limbs * c.n;
-        shc::switch_key(c, ct + 2 * poly, 3 * poly, ct, 3 * poly, key, count, limbs, ws, ws_bytes, (hipStream_t)stream);
+        shc::switch_key(c, ct + 2 * poly, 3 * poly, ct, 3 * poly, key, count, limbs, ws, ws_bytes, (hipStream_t)stream,
+                        true);
     });
 }
 
@@ -376,6 +377,10 @@ int shc_bfv_multiply_plain(const shc_context *ctx, const uint64_t *a, const uint
     return guarded([&] {
         shc::bfv_multiply_plain(ctx_of(ctx), a, plain, out, polys, limbs, ws, ws_bytes, (hipStream_t)stream);
     });
+}
+
+int shc_bfv_plain_to_ntt(const shc_context *ctx, const uint64_t *plain, uint32_t limbs, uint64_t *out, void *stream) {
+    return guarded([&] { shc::bfv_plain_to_ntt(ctx_of(ctx), plain, limbs, out, (hipStream_t)stream); });
 }
 
 int shc_probe_keyswitch(shc_context *ctx, void *begin_event, void *end_event) {

@@ -23,6 +23,8 @@ void bfv_add_plain(const Context &c, const uint64_t *ct, const uint64_t *plain, 
 // BFV plain multiply of coefficient-form polynomials [polys][L][N] by m ([N] mod t),
 // lifted to [L][N] as m - t for m >= ceil(t/2), through the NTT.
 size_t bfv_multiply_plain_workspace(const Context &c, uint32_t L);
+// The lifted plaintext in NTT form at L limbs (transformToNttInplace(PlaintextCuda&)).
+void bfv_plain_to_ntt(const Context &c, const uint64_t *plain, uint32_t L, uint64_t *out, hipStream_t st);
 void bfv_multiply_plain(const Context &c, const uint64_t *a, const uint64_t *plain, uint64_t *out, uint32_t polys,
                         uint32_t L, void *ws, size_t ws_bytes, hipStream_t st);
 

@@ -140,6 +140,15 @@ void bfv_add_plain(const Context &c, const uint64_t *ct, const uint64_t *plain, 
     for (uint32_t i = 0; i < count; i++) bfv_scale_add(c, out + i * ct_words, plain, L, subtract, st);
 }
 
+void bfv_plain_to_ntt(const Context &c, const uint64_t *plain, uint32_t L, uint64_t *out, hipStream_t st) {
+    if (c.scheme != SCHEME_BFV || !c.plain) fail(ERR_LOGIC, "plaintext lift is for BFV");
+    check_level(c, L);
+    const size_t lw = (size_t)L << c.logn;
+    plain_lift_kernel<<<blocks_for(lw), TPB, 0, st>>>(plain, out, c.plain, c.primes(), L, c.logn, lw);
+    launch_check("plain_lift");
+    ntt_forward(out, L, L, c.primes(), c.logn, NTT_FULL, st);
+}
+
 size_t bfv_multiply_plain_workspace(const Context &c, uint32_t L) { return ((size_t)L << c.logn) * 8; }
 
 void bfv_multiply_plain(const Context &c, const uint64_t *a, const uint64_t *plain, uint64_t *out, uint32_t polys,
@@ -151,10 +160,7 @@ void bfv_multiply_plain(const Context &c, const uint64_t *a, const uint64_t *pla
     const size_t words = ((size_t)polys * L) << c.logn;
     if (!words) return;
     uint64_t *lifted = static_cast<uint64_t *>(ws);
-    const size_t lw = (size_t)L << c.logn;
-    plain_lift_kernel<<<blocks_for(lw), TPB, 0, st>>>(plain, lifted, c.plain, c.primes(), L, c.logn, lw);
-    launch_check("plain_lift");
-    ntt_forward(lifted, L, L, c.primes(), c.logn, NTT_FULL, st);
+    bfv_plain_to_ntt(c, plain, L, lifted, st);
     if (out != a) SHC_HIP(hipMemcpyAsync(out, a, words * 8, hipMemcpyDeviceToDevice, st));
     ntt_forward(out, polys * L, L, c.primes(), c.logn, NTT_FULL, st);
     multiply_plain_ntt(c, out, lifted, out, polys, L, st);

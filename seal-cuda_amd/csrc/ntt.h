@@ -27,8 +27,10 @@ void ntt_forward(uint64_t *data, uint32_t count, uint32_t limbs_per_poly, const 
                  NttOut out, hipStream_t stream);
 void ntt_inverse(uint64_t *data, uint32_t count, uint32_t limbs_per_poly, const DevPrime *primes, int logn,
                  NttOut out, hipStream_t stream);
-// Forward transform of the I x J limbs of a LimbGrid.
+// Forward / inverse transform of the I x J limbs of a LimbGrid.
 void ntt_forward_grid(uint64_t *data, LimbGrid grid, const DevPrime *primes, int logn, NttOut out,
+                      hipStream_t stream);
+void ntt_inverse_grid(uint64_t *data, LimbGrid grid, const DevPrime *primes, int logn, NttOut out,
                       hipStream_t stream);
 
 // Key-switching ModUp fused with the forward NTT: T[i][ct][j] = NTT_i(d[ct][j] mod t_i)

@@ -370,7 +370,10 @@ void ntt_forward_grid(uint64_t *data, LimbGrid lg, const DevPrime *primes, int l
 void ntt_inverse(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn, NttOut out,
                  hipStream_t st) {
     if (count == 0) return;
-    const LimbGrid lg = poly_grid(count, lpp);
+    ntt_inverse_grid(data, poly_grid(count, lpp), primes, logn, out, st);
+}
+
+void ntt_inverse_grid(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, NttOut out, hipStream_t st) {
     if (out == NTT_FULL) dispatch<true, true>(data, lg, primes, logn, st);
     else dispatch<true, false>(data, lg, primes, logn, st);
 }

@@ -23,8 +23,12 @@ size_t keyswitch_workspace(const Context &c, uint32_t count, uint32_t L);
 //           (CKKS: NTT form; BFV: coefficient form), L limbs each.
 //   ct:     c0,c1 of each ciphertext, stride `ct_stride` words between ciphertexts.
 //   key:    [K-1][2][K][N] key-switching key at key level (NTT form).
+//   target_is_scratch: the caller gives the target up (relinearization's c2, a Galois
+//           scratch row): a CKKS target is then inverse transformed in place instead of
+//           being copied first.
 void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
-                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st);
+                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st,
+                bool target_is_scratch = false);
 
 // Divide-and-round by the last prime of level L, dropping it (CKKS rescale in NTT form,
 // BFV modulus switch in coefficient form); in: [count][polys][L][N], out: [count][polys][L-1][N].

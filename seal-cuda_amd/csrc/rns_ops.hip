@@ -243,7 +243,7 @@ namespace {
 // within the NTT's per-launch limb bound).
 void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct,
                       size_t ct_stride, const uint64_t *key, uint32_t count, uint32_t L, void *ws, hipStream_t st,
-                      const KernelProbe *probe) {
+                      const KernelProbe *probe, bool target_is_scratch) {
     const size_t K = c.key_count();
     const KsLayout lay = ks_layout(c, count, L);
     char *base = static_cast<char *>(ws);
@@ -257,15 +257,27 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
     const bool ckks = c.scheme == SCHEME_CKKS;
     const DevPrime *targets = c.keyswitch_primes(L);
 
-    // 1. digits in coefficient form
-    const size_t dwords = (size_t)count * L * n;
-    gather_polys_kernel<<<blocks_for(dwords), TPB, 0, st>>>(target, target_stride, d, (size_t)L * n, dwords);
-    launch_check("gather");
-    if (ckks) ntt_inverse(d, count * L, L, c.primes(), logn, NTT_FULL, st);
+    // 1. digits in coefficient form: BFV targets already are; a CKKS target that the
+    // caller gives up is transformed where it lies, any other is copied first
+    const uint64_t *digits = target;
+    size_t dstride = target_stride;
+    if (ckks) {
+        if (target_is_scratch && target_stride % n == 0 && target_stride / n <= UINT32_MAX / count) {
+            ntt_inverse_grid(const_cast<uint64_t *>(target), LimbGrid{1, (uint32_t)(target_stride / n), L, count},
+                             c.primes(), logn, NTT_FULL, st);
+        } else {
+            const size_t dwords = (size_t)count * L * n;
+            gather_polys_kernel<<<blocks_for(dwords), TPB, 0, st>>>(target, target_stride, d, (size_t)L * n, dwords);
+            launch_check("gather");
+            ntt_inverse(d, count * L, L, c.primes(), logn, NTT_FULL, st);
+            digits = d;
+            dstride = (size_t)L * n;
+        }
+    }
     // 2-3. ModUp
-    if (!ntt_forward_modup(d, (size_t)L * n, T, count, L, targets, c.primes(), logn, st, probe)) {
+    if (!ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, st, probe)) {
         const size_t twords = (size_t)count * L * (L + 1) * n;
-        modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(d, (size_t)L * n, T, targets, c.primes(), count, L,
+        modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(digits, dstride, T, targets, c.primes(), count, L,
                                                                logn, twords);
         launch_check("modup_gather");
         if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
@@ -297,7 +309,8 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
 }  // namespace
 
 void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
-                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st) {
+                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st,
+                bool target_is_scratch) {
     const size_t K = c.key_count();
     if (K < 2) fail(ERR_LOGIC, "keyswitching is not supported by the context");
     if (L < 1 || L > K - 1) fail(ERR_INVALID_ARGUMENT, "level out of range for key switching");
@@ -315,7 +328,7 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
     for (uint32_t done = 0; done < count;) {
         const uint32_t n = (uint32_t)(count - done < max_cts ? count - done : max_cts);
         switch_key_chunk(c, target + (size_t)done * target_stride, target_stride, ct + (size_t)done * ct_stride,
-                         ct_stride, key, n, L, ws, st, armed && done == 0 ? &probe : nullptr);
+                         ct_stride, key, n, L, ws, st, armed && done == 0 ? &probe : nullptr, target_is_scratch);
         done += n;
     }
 }

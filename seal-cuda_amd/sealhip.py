@@ -81,6 +81,7 @@ def lib():
         L.shc_bfv_add_plain.argtypes = [_vp, _vp, _vp, _vp, u32, u32, u32, ctypes.c_int, _vp]
         L.shc_bfv_multiply_plain_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
         L.shc_bfv_multiply_plain.argtypes = [_vp, _vp, _vp, _vp, u32, u32, _vp, sz, _vp]
+        L.shc_bfv_plain_to_ntt.argtypes = [_vp, _vp, u32, _vp, _vp]
         _lib = L
     return _lib
 
@@ -293,6 +294,9 @@ class Context:
 
     def bfv_multiply_plain(self, a, plain, out, polys, limbs, d_ws, ws_bytes, stream=None):
         check(lib().shc_bfv_multiply_plain(self.handle, a, plain, out, polys, limbs, d_ws, ws_bytes, stream))
+
+    def bfv_plain_to_ntt(self, plain, limbs, out, stream=None):
+        check(lib().shc_bfv_plain_to_ntt(self.handle, plain, limbs, out, stream))
 
     def probe_keyswitch(self, begin, end):
         """Arm the next key switch to record HipEvents around its ModUp rows pass."""
