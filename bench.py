@@ -11,6 +11,9 @@ per-ciphertext path (one call per ciphertext).
 `--workload ntt`: batched negacyclic NTT, N = 2^16, 24 primes of 60 bits,
 128 ciphertexts x 2 polys x 24 limbs, forward then inverse.
 
+`--workload chain`: the CKKS chain (N=2^15, 64 pairs at uneven levels, 3 x mul -> relin ->
+rescale) captured in a HIP graph.
+
 `--workload copy`: a 1 GiB device copy, the calibration run for HBM byte counters.
 
 Prints one JSON line.  Data are seeded uniform residues (synthetic) and injected
@@ -27,7 +30,7 @@ sys.path.insert(0, os.path.join(HERE, "seal-cuda_amd"))
 sys.path.insert(0, os.path.join(HERE, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the ModUp rows pass at the default workload (batch 128),
+# HBM bytes per launch of the dominant kernel at the default workload (batch 128),
 # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB device copy
 # (profiles/mulrelin_pmc_traffic.md); a profiler cannot run inside this process.
 ROWS_MODUP_TRAFFIC_B128 = 40.49e9 + 40.27e9
@@ -39,7 +42,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="mulrelin", choices=["mulrelin", "ntt", "copy"])
+    p.add_argument("--workload", default="mulrelin", choices=["mulrelin", "ntt", "copy", "chain"])
+    p.add_argument("--no-bfv", action="store_true", help="skip the BFV leg of the mulrelin workload")
     p.add_argument("--batch", type=int, default=128)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-plain", action="store_true", help="skip the plain per-ciphertext leg")
@@ -241,6 +245,7 @@ def bench_mulrelin(args):
         "roofline": {"bound": "hbm", "achieved": achieved_rows, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_rows / HBM_PEAK_GBS,
                      "traffic": ROWS_MODUP_TRAFFIC_B128 if B == 128 else None,
+                     "butterflies_per_launch": rows_bfly,
                      "kernel": f"ntt_rows_modup_kernel<16,8,4,16>: key-switch ModUp NTT rows pass, {tlimbs} limbs "
                                f"x last {logn - logn // 2} layers",
                      "algorithmic_bytes": rows_bytes, "avg_ms": rows_ms,
@@ -252,6 +257,10 @@ def bench_mulrelin(args):
         "served_latency": served,
         "wall_s": wall,
     }
+    if not args.no_bfv:
+        del a, b, out, ws
+        torch.cuda.empty_cache()
+        out_json["bfv"] = bench_bfv_leg(torch, sealhip, B, args.steps, args.warmup)
     if not args.no_cpu:
         out_json["cpu_baseline"] = cpu_baseline_mulrelin(moduli, logn, args.cpu_seconds)
     return out_json
@@ -332,6 +341,140 @@ def bench_ntt(args):
     return out
 
 
+def bench_bfv_leg(torch, sealhip, B, steps, warmup):
+    """BFV multiply (BEHZ) + relinearize at the headline shape: N=2^16, 24 data + 1 special
+    60-bit primes, t = 65537, coefficient form, batch B, inputs and key resident."""
+    logn, n, K = 16, 1 << 16, 25
+    L = K - 1
+    moduli = sealhip.coeff_modulus_create(n, [60] * K)
+    t = 65537
+    ctx = sealhip.Context(sealhip.SCHEME_BFV, n, moduli, t)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    a = torch.empty((B, 2, L, n), dtype=torch.int64, device="cuda")
+    b = torch.empty((B, 2, L, n), dtype=torch.int64, device="cuda")
+    key = torch.empty((K - 1, 2, K, n), dtype=torch.int64, device="cuda")
+    fill_uniform(torch, a, moduli[:L], 2, gen)
+    fill_uniform(torch, b, moduli[:L], 2, gen)
+    fill_uniform(torch, key, moduli, 2, gen)
+    out = torch.empty((B, 3, L, n), dtype=torch.int64, device="cuda")
+    mw = ctx.multiply_workspace_size(B, L)
+    kw = ctx.keyswitch_workspace_size(B, L)
+    ws = torch.empty(((max(mw, kw) + 7) // 8,), dtype=torch.int64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        ctx.multiply(a.data_ptr(), b.data_ptr(), out.data_ptr(), B, L, ws.data_ptr(), mw, st)
+        ctx.relinearize(out.data_ptr(), key.data_ptr(), B, L, ws.data_ptr(), kw, st)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    with Timer(torch) as tm:
+        for _ in range(steps):
+            step()
+    ms = tm.ms / steps
+    del a, b, key, out, ws
+    torch.cuda.empty_cache()
+    return {"ops_s": B / (ms * 1e-3), "ms_per_step": ms, "unit": "mul+relin ops/s",
+            "config": f"BFV multiply (BEHZ) + relinearize, N=2^16, 24 data + 1 special 60-bit primes, t={t}, "
+                      f"batch of {B} ciphertext pairs, coefficient form, key resident"}
+
+
+def bench_chain(args):
+    """CKKS chain at N=2^15, CoeffModulus::Create(32768, {60, 40 x 14, 60}) (15 data primes
+    + special), scale 2^40 words: 64 ciphertext pairs at levels drawn from U[4, 15]; per
+    pair, 3 x (multiply by the partner at the current level -> relinearize -> rescale),
+    ciphertexts of one level batched together, the whole chain captured in one HIP graph
+    and replayed.  A graph replay is one step; value = mul+relin+rescale per second."""
+    import numpy as np
+    import torch
+    import sealhip
+    sealhip.init(0)
+    logn, n = 15, 1 << 15
+    moduli = sealhip.coeff_modulus_create(n, [60] + [40] * 14 + [60])
+    K = len(moduli)
+    ctx = sealhip.Context(sealhip.SCHEME_CKKS, n, moduli)
+    rng = np.random.default_rng(9)
+    pairs = 64
+    levels = rng.integers(4, K, size=pairs)           # data levels 4..15
+    groups = {int(l): int(c) for l, c in zip(*np.unique(levels, return_counts=True))}
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    key = torch.empty((K - 1, 2, K, n), dtype=torch.int64, device="cuda")
+    fill_uniform(torch, key, moduli, 2, gen)
+    bufs = {}
+    max_ws = 0
+    for l, c in groups.items():
+        x = torch.empty((c, 2, l, n), dtype=torch.int64, device="cuda")
+        fill_uniform(torch, x, moduli[:l], 2, gen)
+        ys = []
+        for s in range(3):   # the partner at each level of the chain
+            y = torch.empty((c, 2, l - s, n), dtype=torch.int64, device="cuda")
+            fill_uniform(torch, y, moduli[:l - s], 2, gen)
+            ys.append(y)
+        prods = [torch.empty((c, 3, l - s, n), dtype=torch.int64, device="cuda") for s in range(3)]
+        rels = [torch.empty((c, 2, l - s, n), dtype=torch.int64, device="cuda") for s in range(3)]
+        accs = [torch.empty((c, 2, l - s - 1, n), dtype=torch.int64, device="cuda") for s in range(3)]
+        bufs[l] = (c, x, ys, prods, accs, rels)
+        for s in range(3):
+            max_ws = max(max_ws, ctx.keyswitch_workspace_size(c, l - s), ctx.rescale_workspace_size(c, 2, l - s))
+    ws = torch.empty(((max_ws + 7) // 8,), dtype=torch.int64, device="cuda")
+
+    def chain(st):
+        for l, (c, x, ys, prods, accs, rels) in bufs.items():
+            cur = x
+            for s in range(3):
+                ll = l - s
+                ctx.ckks_multiply(cur.data_ptr(), ys[s].data_ptr(), prods[s].data_ptr(), c, ll, st)
+                ctx.relinearize(prods[s].data_ptr(), key.data_ptr(), c, ll, ws.data_ptr(), max_ws, st)
+                rels[s].copy_(prods[s][:, :2])   # the size-2 results, packed
+                ctx.rescale_to_next(rels[s].data_ptr(), accs[s].data_ptr(), c, 2, ll, ws.data_ptr(), max_ws, st)
+                cur = accs[s]
+
+    st = torch.cuda.current_stream().cuda_stream
+    chain(st)
+    torch.cuda.synchronize()
+    eager = {l: b[4][2].clone() for l, b in bufs.items()}
+    t0 = time.perf_counter()
+    with Timer(torch) as te:
+        for _ in range(args.steps):
+            chain(st)
+    eager_ms = te.ms / args.steps
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(graph, stream=cap):
+            chain(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.current_stream().wait_stream(cap)
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    # every replay must give the eager words (HIP graphs replayed with eager work between)
+    for _ in range(2):
+        graph.replay()
+        torch.cuda.synchronize()
+        for l, b in bufs.items():
+            if not torch.equal(b[4][2], eager[l]):
+                raise SystemExit("bench self-check failed: graph replay differs from the eager chain")
+        chain(st)
+        torch.cuda.synchronize()
+    with Timer(torch) as tg:
+        for _ in range(args.steps):
+            graph.replay()
+    wall = time.perf_counter() - t0
+    ms = tg.ms / args.steps
+    ops = pairs * 3
+    return {"metric": "CKKS mul+relin+rescale per second, N=2^15 chain of 3, uneven levels, HIP graph",
+            "value": ops / (ms * 1e-3), "unit": "mul+relin+rescale/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded uniform residues; injected key)",
+            "config": {"workload": f"CKKS chain: N=2^15, 15+1 primes (60, 40x14, 60), {pairs} pairs at levels "
+                                   f"U[4,15] ({len(groups)} level groups), 3 x (mul, relin, rescale)",
+                       "parallelism": "single GPU"},
+            "eager_ms_per_step": eager_ms, "graph_over_eager": eager_ms / ms,
+            "self_check": "every graph replay equals the eager chain's words", "wall_s": wall}
+
+
 def bench_copy(args):
     """HBM copy of 1 GiB (torch's device copy kernel): the calibration run for the
     FETCH_SIZE / WRITE_SIZE counters (known 1 GiB read + 1 GiB written per launch)."""
@@ -359,7 +502,7 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus N > 1 is not supported "
                                    "(independent ciphertexts shard across GPUs with no exchange)"}))
         sys.exit(2)
-    out = {"mulrelin": bench_mulrelin, "ntt": bench_ntt, "copy": bench_copy}[args.workload](args)
+    out = {"mulrelin": bench_mulrelin, "ntt": bench_ntt, "copy": bench_copy, "chain": bench_chain}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

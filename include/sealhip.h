@@ -4,7 +4,7 @@
  * The reference has no C ABI: its GPU operations are reached through the C++
  * classes of src/troy_cuda.cuh and, one level down, through the kernel_util::k*
  * launchers (src/kernelutils.cuh:448-706).  This header is the thin layer that the
- * C++ facade (seal-cuda_amd/troy/) and any FFI (ctypes, cgo, JNI) bind; each entry
+ * C++ class API (include/troyhip.hpp) and any FFI (ctypes, cgo, JNI) bind; each entry
  * point names the reference interface it replaces.
  *
  * Conventions
@@ -254,6 +254,28 @@ int shc_bfv_multiply_plain(const shc_context *ctx, const uint64_t *d_a, const ui
  * (src/evaluator_cuda.cu:1869-1935). */
 int shc_bfv_plain_to_ntt(const shc_context *ctx, const uint64_t *d_plain, uint32_t limb_count, uint64_t *d_out,
                          void *stream);
+
+/* ---- hoisted rotations --------------------------------------------------------
+ * Sums of rotations of one ciphertext (the diagonal matrix-vector pattern of
+ * app/matmul.cuh:98-121: rotateVectorInplace / rotateRowsInplace then addInplace) with
+ * c1's ModUp done once for all rotations.  The words equal the sum of separate
+ * shc_apply_galois_keyswitch results.  A Galois key is first prepared for a level
+ * (permuted copy plus a sign-pattern correction, shc_hoisted_key_size bytes). */
+int shc_hoisted_key_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
+int shc_hoisted_key_workspace_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
+int shc_hoisted_key_prepare(const shc_context *ctx, const uint64_t *d_galois_key, uint32_t galois_elt,
+                            uint32_t limb_count, uint64_t *d_hoisted_key, void *d_workspace, size_t workspace_bytes,
+                            void *stream);
+/* d_out[c] = sum_r rotate_{galois_elts[r]}(d_in[c]) for size-2 ciphertexts
+ * [count][2][L][N] (CKKS: NTT form; BFV: coefficient form); d_out must not overlap d_in.
+ * galois_elts, d_hoisted_keys and d_galois_keys are host arrays of `rotations` entries
+ * (device pointers for the keys; the Galois keys serve inputs with a zero digit, where
+ * hoisting is not exact).  Synchronises `stream` once, so it cannot be graph-captured. */
+int shc_rotate_sum_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limb_count, size_t *bytes);
+int shc_rotate_sum(const shc_context *ctx, const uint64_t *d_in, uint32_t count, uint32_t limb_count,
+                   const uint32_t *galois_elts, const uint64_t *const *d_hoisted_keys,
+                   const uint64_t *const *d_galois_keys, uint32_t rotations, uint64_t *d_out, void *d_workspace,
+                   size_t workspace_bytes, void *stream);
 
 /* Instrumentation (no counterpart in the reference; for benchmarks and profilers):
  * arms `ctx` so that its next key switch (shc_relinearize, shc_switch_key,

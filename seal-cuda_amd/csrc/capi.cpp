@@ -7,6 +7,7 @@
 #include "elementwise.h"
 #include "galois.h"
 #include "rns_ops.h"
+#include "rotations.h"
 #include <string>
 
 struct shc_context {
@@ -381,6 +382,42 @@ int shc_bfv_multiply_plain(const shc_context *ctx, const uint64_t *a, const uint
 
 int shc_bfv_plain_to_ntt(const shc_context *ctx, const uint64_t *plain, uint32_t limbs, uint64_t *out, void *stream) {
     return guarded([&] { shc::bfv_plain_to_ntt(ctx_of(ctx), plain, limbs, out, (hipStream_t)stream); });
+}
+
+int shc_hoisted_key_size(const shc_context *ctx, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        if (!bytes) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        *bytes = shc::hoisted_key_words(ctx_of(ctx), limbs) * 8;
+    });
+}
+
+int shc_hoisted_key_workspace_size(const shc_context *ctx, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        if (!bytes) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        *bytes = shc::hoisted_key_workspace(ctx_of(ctx), limbs);
+    });
+}
+
+int shc_hoisted_key_prepare(const shc_context *ctx, const uint64_t *key, uint32_t elt, uint32_t limbs, uint64_t *out,
+                            void *ws, size_t ws_bytes, void *stream) {
+    return guarded(
+        [&] { shc::hoisted_key_prepare(ctx_of(ctx), key, elt, limbs, out, ws, ws_bytes, (hipStream_t)stream); });
+}
+
+int shc_rotate_sum_workspace_size(const shc_context *ctx, uint32_t count, uint32_t limbs, size_t *bytes) {
+    return guarded([&] {
+        if (!bytes) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        *bytes = shc::rotate_sum_workspace(ctx_of(ctx), count, limbs);
+    });
+}
+
+int shc_rotate_sum(const shc_context *ctx, const uint64_t *in, uint32_t count, uint32_t limbs, const uint32_t *elts,
+                   const uint64_t *const *hoisted, const uint64_t *const *keys, uint32_t rotations, uint64_t *out,
+                   void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        shc::rotate_sum(ctx_of(ctx), in, count, limbs, elts, hoisted, keys, rotations, out, ws, ws_bytes,
+                        (hipStream_t)stream);
+    });
 }
 
 int shc_probe_keyswitch(shc_context *ctx, void *begin_event, void *end_event) {

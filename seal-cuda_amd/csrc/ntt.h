@@ -42,4 +42,22 @@ bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint3
                        const DevPrime *targets, const DevPrime *digits, int logn, hipStream_t stream,
                        const KernelProbe *probe = nullptr);
 
+// ModDown by the special prime p fused into the last transforms of a key switch, for
+// `count` ciphertexts at L limbs (N >= 2^13; returns false and does nothing otherwise):
+//   u_i = ([s + floor(p/2)]_p mod q_i) - (floor(p/2) mod q_i),  ct_k[i] += (P_k[i] - u_i) p^-1
+// with s = S[ct][k] (special limb, coefficient form) and P = [ct][k][L][N].
+//   CKKS: buf = U scratch [count][2][L][N]; P in NTT form (u is transformed on the way).
+//   BFV:  buf = P itself (coefficient form after the in-place inverse transform).
+struct ModDownArgs {
+    const uint64_t *S;
+    const uint64_t *P;
+    uint64_t *ct;
+    size_t ct_stride;
+    const DivRoundConst *dc;
+    uint64_t p;
+    uint32_t L;
+};
+bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &args, uint32_t count, const DevPrime *primes, int logn,
+                 hipStream_t stream);
+
 }  // namespace shc

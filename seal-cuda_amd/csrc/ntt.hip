@@ -128,9 +128,9 @@ __device__ __forceinline__ uint64_t finish(uint64_t x, const DevPrime &P) {
 // of SUBS limbs that share prime i -- limbs (i, j) for SUBS consecutive j -- so every
 // row in the block uses the same twiddles (broadcast loads, L1/L2 hits).  Blocks run
 // with the limb group fastest, so consecutive blocks also share them.
-template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
-__device__ __forceinline__ void rows_body(uint64_t *__restrict__ data, const DevPrime *__restrict__ primes,
-                                          LimbGrid lg, uint32_t j0, uint32_t jn) {
+template <int LOGN, int K, int E, int SUBS, bool INV, class Load, class Store>
+__device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, uint32_t I, uint32_t j0, uint32_t jn,
+                                          const Load &load_elem, const Store &store_elem) {
     constexpr int T = 1 << (K - E);
     constexpr int SZ = 1 << K;
     constexpr int RPL = LOGN - K;  // log2(rows per limb)
@@ -141,19 +141,19 @@ __device__ __forceinline__ void rows_body(uint64_t *__restrict__ data, const Dev
     const uint32_t r = blockIdx.x / jc_count;
     const int g = (int)(r & ((1u << RPL) - 1));
     const uint32_t i = r >> RPL;
-    const uint32_t jfirst = jc * SUBS;
-    const int nsub = (int)min<uint32_t>(SUBS, jn - jfirst);
+    const uint32_t jfirst = j0 + jc * SUBS;
+    const int nsub = (int)min<uint32_t>(SUBS, j0 + jn - jfirst);
     const int tid = threadIdx.x;
     const int sub = tid / T, t = tid % T;
     const bool active = sub < nsub;
-    uint64_t *row = data + (((size_t)(i * lg.A + (j0 + jfirst + (uint32_t)(active ? sub : 0)) * lg.B)) << LOGN) +
-                    ((size_t)g << K);
+    const uint32_t j = jfirst + (uint32_t)(active ? sub : 0);
+    const size_t row_off = (size_t)g << K;
     const DevPrime &P = primes[i];
     const ulonglong2 *tw = INV ? P.inv : P.fwd;
     auto addr = [sub](int idx) { return sub * SZ + swz(idx); };
-    auto load = [row](int idx) { return row[idx]; };
+    auto load = [&](int idx) { return load_elem(i, j, row_off + idx); };
     if constexpr (NP == 1) {
-        auto store = [row, &P](int idx, uint64_t x) { row[idx] = LAST ? finish<INV, FULL>(x, P) : x; };
+        auto store = [&](int idx, uint64_t x) { store_elem(i, j, row_off + idx, x); };
         run_phases<LOGN, K, E, INV>(tile, addr, load, store, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
     } else {
         // the last phase leaves 16 consecutive values per thread; stage them in LDS so
@@ -161,21 +161,55 @@ __device__ __forceinline__ void rows_body(uint64_t *__restrict__ data, const Dev
         auto stage = [&](int idx, uint64_t x) { tile[addr(idx)] = x; };
         run_phases<LOGN, K, E, INV>(tile, addr, load, stage, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
         __syncthreads();
-        const uint32_t row0 = i * lg.A + (j0 + jfirst) * lg.B;
         for (int e = tid; e < nsub * SZ; e += SUBS * T) {
             const int sb = e / SZ, off = e % SZ;
-            const uint64_t x = tile[sb * SZ + swz(off)];
-            data[(((size_t)(row0 + (uint32_t)sb * lg.B)) << LOGN) + ((size_t)g << K) + off] =
-                LAST ? finish<INV, FULL>(x, P) : x;
+            store_elem(i, jfirst + (uint32_t)sb, row_off + off, tile[sb * SZ + swz(off)]);
         }
     }
 }
+
+// Columns pass: SUBS adjacent columns of one limb per block; a column is the first
+// K layers of the limb (stride 2^(LOGN-K) between its values).
+template <int LOGN, int K, int E, int SUBS, bool INV, class Load, class Store>
+__device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, uint32_t j0, uint32_t jn,
+                                          const Load &load_elem, const Store &store_elem) {
+    constexpr int SZ = 1 << K;
+    constexpr int C = 1 << (LOGN - K);
+    constexpr int TILES = C / SUBS;
+    __shared__ uint64_t tile[SUBS * SZ];
+    const int tid = threadIdx.x;
+    const int col = tid % SUBS, t = tid / SUBS;
+    const uint32_t r = blockIdx.x / TILES;
+    const size_t c = (size_t)(blockIdx.x % TILES) * SUBS + col;
+    const uint32_t i = r / jn, j = j0 + r % jn;
+    const DevPrime &P = primes[i];
+    const ulonglong2 *tw = INV ? P.inv : P.fwd;
+    auto addr = [col](int idx) { return idx * SUBS + col; };
+    auto load = [&](int idx) { return load_elem(i, j, (size_t)idx * C + c); };
+    auto store = [&](int idx, uint64_t x) { store_elem(i, j, (size_t)idx * C + c, x); };
+    run_phases<LOGN, K, E, INV>(tile, addr, load, store, t, 0, 0, tw, P.q << 2, 0 - P.q, true);
+}
+
+// Plain in-place passes over a LimbGrid buffer.
+struct GridAccess {
+    uint64_t *data;
+    LimbGrid lg;
+    int logn;
+    __device__ uint64_t *at(uint32_t i, uint32_t j, size_t off) const {
+        return data + ((size_t)(i * lg.A + j * lg.B) << logn) + off;
+    }
+};
 
 template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__restrict__ data,
                                                                   const DevPrime *__restrict__ primes, LimbGrid lg,
                                                                   uint32_t j0, uint32_t jn) {
-    rows_body<LOGN, K, E, SUBS, INV, LAST, FULL>(data, primes, lg, j0, jn);
+    const GridAccess g{data, lg, LOGN};
+    rows_body<LOGN, K, E, SUBS, INV>(
+        primes, lg.I, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
+        });
 }
 
 // The key switch's ModUp rows pass: the same code as a kernel of its own, so that
@@ -184,31 +218,22 @@ template <int LOGN, int K, int E, int SUBS>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_modup_kernel(uint64_t *__restrict__ data,
                                                                         const DevPrime *__restrict__ primes,
                                                                         LimbGrid lg) {
-    rows_body<LOGN, K, E, SUBS, false, true, true>(data, primes, lg, 0, lg.J);
+    const GridAccess g{data, lg, LOGN};
+    rows_body<LOGN, K, E, SUBS, false>(
+        primes, lg.I, 0, lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { *g.at(i, j, off) = finish<false, true>(x, primes[i]); });
 }
 
-// Columns pass: SUBS adjacent columns of one limb per block; a column is the first
-// K layers of the limb (stride 2^(LOGN-K) between its values).
 template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__restrict__ data,
                                                                   const DevPrime *__restrict__ primes, LimbGrid lg,
                                                                   uint32_t j0, uint32_t jn) {
-    constexpr int SZ = 1 << K;
-    constexpr int C = 1 << (LOGN - K);
-    constexpr int TILES = C / SUBS;
-    __shared__ uint64_t tile[SUBS * SZ];
-    const int tid = threadIdx.x;
-    const int col = tid % SUBS, t = tid / SUBS;
-    const uint32_t r = blockIdx.x / TILES;
-    const int c0 = (int)(blockIdx.x % TILES) * SUBS;
-    const uint32_t i = r / jn, j = j0 + r % jn;
-    uint64_t *base = data + ((size_t)(i * lg.A + j * lg.B) << LOGN) + c0 + col;
-    const DevPrime &P = primes[i];
-    const ulonglong2 *tw = INV ? P.inv : P.fwd;
-    auto addr = [col](int idx) { return idx * SUBS + col; };
-    auto load = [base](int idx) { return base[(size_t)idx * C]; };
-    auto store = [base, &P](int idx, uint64_t x) { base[(size_t)idx * C] = LAST ? finish<INV, FULL>(x, P) : x; };
-    run_phases<LOGN, K, E, INV>(tile, addr, load, store, t, 0, 0, tw, P.q << 2, 0 - P.q, true);
+    const GridAccess g{data, lg, LOGN};
+    cols_body<LOGN, K, E, SUBS, INV>(
+        primes, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
+        });
 }
 
 // ModUp columns pass into T[i][ct][j] (target-major, so that the rows pass walks one
@@ -247,6 +272,84 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
     };
     auto store = [out](int idx, uint64_t x) { out[(size_t)idx * C] = x; };
     run_phases<LOGN, K, E, false>(tile, addr, load, store, t, 0, 0, P.fwd, P.q << 2, 0 - P.q, true);
+}
+
+// ---- ModDown by the special prime p, fused into the transforms (switch_key step 5):
+//   u_i = ([s + floor(p/2)]_p mod q_i) - (floor(p/2) mod q_i)     (s: special limb, coeff)
+//   ct_k[i] += (P_k[i] - u_i) * p^-1 mod q_i
+// CKKS runs it as the forward NTT of u (first pass computes u from s on load, last pass
+// adds into ct on store); BFV as the inverse NTT of P (last pass subtracts u and adds).
+// Limb (i, j): prime i < L, j = 2 ct + k.
+__device__ __forceinline__ uint64_t moddown_prep(uint64_t s, uint64_t p, const DevPrime &Q, const DivRoundConst &d) {
+    uint64_t v = s + (p >> 1);
+    v = v >= p ? v - p : v;
+    return submod(barrett64(v, Q.q, Q.ratio_hi), d.half_mod, Q.q);
+}
+
+template <int LOGN>
+__device__ __forceinline__ void moddown_add(const ModDownArgs &a, uint32_t i, uint32_t j, size_t off, uint64_t pv,
+                                            uint64_t u, const DevPrime &Q) {
+    uint64_t *dst = a.ct + (size_t)(j >> 1) * a.ct_stride + ((size_t)((j & 1) * a.L + i) << LOGN) + off;
+    const uint64_t v = shoup(submod(pv, u, Q.q), a.dc[i].inv, a.dc[i].inv_shoup, Q.q);
+    *dst = addmod(*dst, v, Q.q);
+}
+
+template <int LOGN, int K, int E, int SUBS>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_moddown_prep_kernel(uint64_t *__restrict__ U,
+                                                                               const DevPrime *__restrict__ primes,
+                                                                               ModDownArgs a, uint32_t jn) {
+    cols_body<LOGN, K, E, SUBS, false>(
+        primes, 0, jn,
+        [&](uint32_t i, uint32_t j, size_t off) {
+            return moddown_prep(a.S[((size_t)j << LOGN) + off], a.p, primes[i], a.dc[i]);
+        },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { U[((size_t)(j * a.L + i) << LOGN) + off] = x; });
+}
+
+template <int LOGN, int K, int E, int SUBS>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_moddown_kernel(const uint64_t *__restrict__ U,
+                                                                          const DevPrime *__restrict__ primes,
+                                                                          ModDownArgs a, uint32_t jn) {
+    rows_body<LOGN, K, E, SUBS, false>(
+        primes, a.L, 0, jn, [&](uint32_t i, uint32_t j, size_t off) { return U[((size_t)(j * a.L + i) << LOGN) + off]; },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            const DevPrime &Q = primes[i];
+            moddown_add<LOGN>(a, i, j, off, a.P[((size_t)(j * a.L + i) << LOGN) + off], finish<false, true>(x, Q), Q);
+        });
+}
+
+template <int LOGN, int K, int E, int SUBS>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_moddown_kernel(const uint64_t *__restrict__ Pt,
+                                                                          const DevPrime *__restrict__ primes,
+                                                                          ModDownArgs a, uint32_t jn) {
+    cols_body<LOGN, K, E, SUBS, true>(
+        primes, 0, jn, [&](uint32_t i, uint32_t j, size_t off) { return Pt[((size_t)(j * a.L + i) << LOGN) + off]; },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            const DevPrime &Q = primes[i];
+            const uint64_t u = moddown_prep(a.S[((size_t)j << LOGN) + off], a.p, Q, a.dc[i]);
+            moddown_add<LOGN>(a, i, j, off, finish<true, true>(x, Q), u, Q);
+        });
+}
+
+template <int LOGN>
+void launch_moddown(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t count, const DevPrime *primes,
+                    hipStream_t st) {
+    constexpr int K1 = LOGN / 2;
+    constexpr int K2 = LOGN - K1;
+    constexpr int E = 4;
+    constexpr int CS = 16;
+    constexpr int RS = 4096 >> K2;
+    const uint32_t jn = 2 * count;
+    const uint32_t col_blocks = a.L * jn * ((1u << K2) / CS);
+    const uint32_t row_blocks = (a.L << K1) * ((jn + RS - 1) / RS);
+    if (ckks) {
+        ntt_cols_moddown_prep_kernel<LOGN, K1, E, CS><<<col_blocks, CS << (K1 - E), 0, st>>>(buf, primes, a, jn);
+        ntt_rows_moddown_kernel<LOGN, K2, E, RS><<<row_blocks, RS << (K2 - E), 0, st>>>(buf, primes, a, jn);
+    } else {
+        ntt_rows_kernel<LOGN, K2, E, RS, true, false, true>
+            <<<row_blocks, RS << (K2 - E), 0, st>>>(buf, primes, LimbGrid{1, a.L, a.L, jn}, 0, jn);
+        ntt_cols_moddown_kernel<LOGN, K1, E, CS><<<col_blocks, CS << (K1 - E), 0, st>>>(buf, primes, a, jn);
+    }
 }
 
 template <int LOGN, bool INV, bool FULL>
@@ -355,6 +458,22 @@ LimbGrid poly_grid(uint32_t count, uint32_t lpp) {
     return LimbGrid{1, lpp, lpp, count / lpp};
 }
 }  // namespace
+
+bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t count, const DevPrime *primes, int logn,
+                 hipStream_t st) {
+    if (logn < 13 || logn > 17 || count == 0) return false;
+    if ((uint64_t)2 * count * a.L > ((uint64_t(1) << 34) >> logn))
+        fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
+    switch (logn) {
+        case 13: launch_moddown<13>(ckks, buf, a, count, primes, st); break;
+        case 14: launch_moddown<14>(ckks, buf, a, count, primes, st); break;
+        case 15: launch_moddown<15>(ckks, buf, a, count, primes, st); break;
+        case 16: launch_moddown<16>(ckks, buf, a, count, primes, st); break;
+        case 17: launch_moddown<17>(ckks, buf, a, count, primes, st); break;
+    }
+    launch_check("ntt moddown launch");
+    return true;
+}
 
 void ntt_forward(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn, NttOut out,
                  hipStream_t st) {

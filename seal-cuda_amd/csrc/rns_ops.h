@@ -30,6 +30,26 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
                 const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st,
                 bool target_is_scratch = false);
 
+// Stages of a key switch at level L over `count` ciphertexts, for callers that compose
+// them differently (hoisted rotations):
+//   modup_to_T: T[i][ct][j] = NTT_i(d_j mod q_i), digits d (coefficient form) at
+//               digits + ct * dstride + j * N; targets i <= L (special last).
+//   keyswitch_inner_product: P[ct][k][i] = sum_j T[i][ct][j] key[j][k][ki] (i < L),
+//               S[ct][k] the special target's.
+//   moddown_into: ct_k[i] += ModDown(P_k, S_k)[i] (S is consumed; U is scratch
+//               [count][2][L][N]; P in NTT form).
+// The regions of a key-switch workspace of `count` ciphertexts at level L.
+struct KsRegions {
+    uint64_t *d, *T, *P, *S, *U;
+};
+KsRegions keyswitch_regions(const Context &c, uint32_t count, uint32_t L, void *ws);
+void modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64_t *T, uint32_t count, uint32_t L,
+                hipStream_t st, const KernelProbe *probe = nullptr);
+void keyswitch_inner_product(const Context &c, const uint64_t *T, const uint64_t *key, uint64_t *P, uint64_t *S,
+                             uint32_t count, uint32_t L, hipStream_t st);
+void moddown_into(const Context &c, uint64_t *P, uint64_t *S, uint64_t *U, uint64_t *ct, size_t ct_stride,
+                  uint32_t count, uint32_t L, hipStream_t st);
+
 // Divide-and-round by the last prime of level L, dropping it (CKKS rescale in NTT form,
 // BFV modulus switch in coefficient form); in: [count][polys][L][N], out: [count][polys][L-1][N].
 size_t rescale_workspace(const Context &c, uint32_t count, uint32_t polys, uint32_t L);

@@ -237,6 +237,13 @@ void tensor_strided(const DevPrime *primes, const uint64_t *a, size_t a_stride, 
 
 size_t keyswitch_workspace(const Context &c, uint32_t count, uint32_t L) { return ks_layout(c, count, L).total; }
 
+KsRegions keyswitch_regions(const Context &c, uint32_t count, uint32_t L, void *ws) {
+    const KsLayout l = ks_layout(c, count, L);
+    char *b = static_cast<char *>(ws);
+    auto at = [b](size_t off) { return reinterpret_cast<uint64_t *>(b + off); };
+    return KsRegions{at(l.d), at(l.T), at(l.P), at(l.S), at(l.U)};
+}
+
 namespace {
 
 // One launch sequence over `count` ciphertexts (few enough that every launch stays
@@ -244,7 +251,6 @@ namespace {
 void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct,
                       size_t ct_stride, const uint64_t *key, uint32_t count, uint32_t L, void *ws, hipStream_t st,
                       const KernelProbe *probe, bool target_is_scratch) {
-    const size_t K = c.key_count();
     const KsLayout lay = ks_layout(c, count, L);
     char *base = static_cast<char *>(ws);
     uint64_t *d = reinterpret_cast<uint64_t *>(base + lay.d);
@@ -255,7 +261,6 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
     const int logn = c.logn;
     const size_t n = c.n;
     const bool ckks = c.scheme == SCHEME_CKKS;
-    const DevPrime *targets = c.keyswitch_primes(L);
 
     // 1. digits in coefficient form: BFV targets already are; a CKKS target that the
     // caller gives up is transformed where it lies, any other is copied first
@@ -274,28 +279,51 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
             dstride = (size_t)L * n;
         }
     }
-    // 2-3. ModUp
-    if (!ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, st, probe)) {
-        const size_t twords = (size_t)count * L * (L + 1) * n;
-        modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(digits, dstride, T, targets, c.primes(), count, L,
-                                                               logn, twords);
-        launch_check("modup_gather");
-        if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
-        ntt_forward_grid(T, LimbGrid{count * L, 1, L + 1, count * L}, targets, logn, NTT_FULL, st);
-        if (probe) SHC_HIP(hipEventRecord(probe->end, st));
-    }
-    // 4. inner product with the key
+    // 2-4. ModUp and the inner product with the key
+    modup_to_T(c, digits, dstride, T, count, L, st, probe);
+    keyswitch_inner_product(c, T, key, P, S, count, L, st);
+    // 5. ModDown by the special prime
+    moddown_into(c, P, S, U, ct, ct_stride, count, L, st);
+}
+
+}  // namespace
+
+void modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64_t *T, uint32_t count, uint32_t L,
+                hipStream_t st, const KernelProbe *probe) {
+    const DevPrime *targets = c.keyswitch_primes(L);
+    const int logn = c.logn;
+    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, st, probe)) return;
+    const size_t twords = ((size_t)count * L * (L + 1)) << logn;
+    modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(digits, dstride, T, targets, c.primes(), count, L, logn,
+                                                           twords);
+    launch_check("modup_gather");
+    if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
+    ntt_forward_grid(T, LimbGrid{count * L, 1, L + 1, count * L}, targets, logn, NTT_FULL, st);
+    if (probe) SHC_HIP(hipEventRecord(probe->end, st));
+}
+
+void keyswitch_inner_product(const Context &c, const uint64_t *T, const uint64_t *key, uint64_t *P, uint64_t *S,
+                             uint32_t count, uint32_t L, hipStream_t st) {
     constexpr int G = 4;
+    const int logn = c.logn;
     const uint32_t groups = (count + G - 1) / G;
     const uint32_t tiles = (uint32_t)((((size_t)(L + 1) << logn) + TPB - 1) / TPB);
     const uint64_t ip_blocks = (uint64_t)((tiles + 7) / 8) * 8 * groups;
     if (ip_blocks * TPB >= (uint64_t(1) << 32)) fail(ERR_INVALID_ARGUMENT, "key-switch batch too large");
-    inner_product_kernel<G><<<(uint32_t)ip_blocks, TPB, 0, st>>>(T, key, P, S, targets, L, (uint32_t)K, logn, count,
-                                                                tiles, groups);
+    inner_product_kernel<G><<<(uint32_t)ip_blocks, TPB, 0, st>>>(T, key, P, S, c.keyswitch_primes(L), L,
+                                                                (uint32_t)c.key_count(), logn, count, tiles, groups);
     launch_check("inner_product");
-    // 5. ModDown by the special prime
-    ntt_inverse(S, count * 2, 1, targets + L, logn, NTT_FULL, st);
-    const size_t uwords = (size_t)count * 2 * L * n;
+}
+
+void moddown_into(const Context &c, uint64_t *P, uint64_t *S, uint64_t *U, uint64_t *ct, size_t ct_stride,
+                  uint32_t count, uint32_t L, hipStream_t st) {
+    const size_t K = c.key_count();
+    const int logn = c.logn;
+    const bool ckks = c.scheme == SCHEME_CKKS;
+    ntt_inverse(S, count * 2, 1, c.keyswitch_primes(L) + L, logn, NTT_FULL, st);
+    const size_t uwords = ((size_t)count * 2 * L) << logn;
+    const ModDownArgs md{S, P, ct, ct_stride, c.ks_consts(), c.moduli[K - 1], L};
+    if (ntt_moddown(ckks, ckks ? U : P, md, count, c.primes(), logn, st)) return;
     divround_prep_kernel<<<blocks_for(uwords), TPB, 0, st>>>(S, U, c.primes(), c.ks_consts(), c.moduli[K - 1], L, logn,
                                                             uwords);
     launch_check("moddown_prep");
@@ -305,8 +333,6 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
                                                              uwords);
     launch_check("moddown_finish");
 }
-
-}  // namespace
 
 void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
                 const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st,

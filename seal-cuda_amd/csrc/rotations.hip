@@ -1,0 +1,255 @@
+// rotations.hip -- hoisted rotation and summation, exact against separate rotations.
+//
+// A rotation (Evaluator::applyGaloisInplace, src/evaluator.cpp:2210-2250; GPU
+// src/evaluator_cuda.cu:2011-2104) switches the key of sigma(c1), whose digits are the
+// canonical residues of sigma applied to c1's coefficients.  Hoisting transforms c1's
+// digits once: T_ij = NTT_i(d_j mod q_i), and sigma(T_ij) (an NTT-domain permutation)
+// is the transform of sigma(d_j) mod q_i -- except where sigma negates a coefficient.
+// There the reference's digit is q_j - d (for d != 0), which is q_j mod q_i above the
+// hoisted -d mod q_i.  So, per target prime i and with S_sigma the 0/1 polynomial of
+// negated positions,
+//   sum_j NTT_i(digit_j) key_j
+//     = sigma(sum_j T_ij sigma^-1(key_j)) + NTT_i(S_sigma) sum_j (q_j mod q_i) key_j
+//     = sigma(sum_j T_ij key'_j + corr')
+// with key' = sigma^-1(key) and corr' = sigma^-1(NTT_i(S_sigma) sum_j (q_j mod q_i) key_j),
+// prepared once per key and level.  A zero digit (d = 0) needs no correction, so calls
+// whose c1 has one fall back to separate rotations and the words always equal the
+// reference's.  Each rotation then costs one inner product, one permutation of the
+// 2(L+1) product limbs and one ModDown instead of a full ModUp (L(L+1) transforms).
+#include "rotations.h"
+#include "elementwise.h"
+#include "galois.h"
+#include "hostmath.h"
+#include "modarith.h"
+#include "ntt.h"
+#include "rns_ops.h"
+
+namespace shc {
+namespace {
+
+constexpr int TPB = 256;
+
+inline uint32_t blocks_for(size_t n) {
+    const size_t b = (n + TPB - 1) / TPB;
+    return (uint32_t)(b < (size_t(1) << 20) ? b : (size_t(1) << 20));
+}
+
+inline size_t align256(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+// neg[l][k] = 1 where sigma_elt negates the coefficient that lands at k (the source
+// i = k elt^-1 mod N moves to i elt mod 2N, negated when that is >= N), in each limb.
+__global__ void negated_positions_kernel(uint64_t *__restrict__ neg, uint32_t elt, uint32_t elt_inv, int logn,
+                                         size_t total) {
+    const uint32_t n = 1u << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const uint32_t k = (uint32_t)(idx & (n - 1));
+        const uint32_t i = (uint32_t)(((uint64_t)k * elt_inv) & (n - 1));
+        neg[idx] = (((uint64_t)i * elt) >> logn) & 1;
+    }
+}
+
+// C[k][i][c] = negntt[i][c] * sum_{j<L} (q_j mod q_i) key[j][k][ki][c] mod q_i, i <= L
+// (ki = i, or the special prime's key limb K-1 for i == L).
+__global__ void correction_kernel(const uint64_t *__restrict__ negntt, const uint64_t *__restrict__ key,
+                                  uint64_t *__restrict__ C, const DevPrime *__restrict__ targets,
+                                  const DevPrime *__restrict__ data, uint32_t L, uint32_t K, int logn, size_t total) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t c = idx & (n - 1);
+        const size_t r = idx >> logn;
+        const uint32_t i = (uint32_t)(r % (L + 1));
+        const uint32_t k = (uint32_t)(r / (L + 1));
+        const DevPrime &Q = targets[i];
+        const uint32_t ki = i == L ? K - 1 : i;
+        u128 acc{0, 0};
+        for (uint32_t j = 0; j < L; j++) {
+            const uint64_t cij = barrett64(data[j].q, Q.q, Q.ratio_hi);
+            add128(acc, mul64x64(cij, key[(((size_t)j * 2 + k) * K + ki) * n + c]));
+        }
+        const uint64_t s = barrett128(acc, Q.q, Q.ratio_lo, Q.ratio_hi);
+        C[idx] = mulmod(s, negntt[(size_t)i * n + c], Q.q, Q.ratio_lo, Q.ratio_hi);
+    }
+}
+
+// P[ct][k][i] += corr[k][i] (i < L), S[ct][k] += corr[k][L].
+__global__ void add_correction_kernel(uint64_t *__restrict__ P, uint64_t *__restrict__ S,
+                                      const uint64_t *__restrict__ corr, const DevPrime *__restrict__ targets,
+                                      uint32_t L, int logn, size_t total) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t c = idx & (n - 1);
+        const size_t r = idx >> logn;
+        const uint32_t i = (uint32_t)(r % (L + 1));
+        const size_t rk = r / (L + 1);  // ct * 2 + k
+        const uint32_t k = (uint32_t)(rk & 1);
+        const uint64_t q = targets[i].q;
+        const uint64_t v = corr[((size_t)k * (L + 1) + i) * n + c];
+        uint64_t *dst = i < L ? P + ((rk * L + i) << logn) + c : S + (rk << logn) + c;
+        *dst = addmod(*dst, v, q);
+    }
+}
+
+// out c0 = (first ? 0 : out c0) + s0, out c1 = first ? 0 : out c1; s0: [count][L][N].
+__global__ void accumulate_c0_kernel(uint64_t *__restrict__ out, const uint64_t *__restrict__ s0,
+                                     const DevPrime *__restrict__ primes, uint32_t L, int logn, bool first,
+                                     size_t total) {
+    const size_t lw = (size_t)L << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t ct = idx / lw, r = idx % lw;
+        uint64_t *c0 = out + ct * 2 * lw + r;
+        const uint64_t v = s0[idx];
+        if (first) {
+            *c0 = v;
+            c0[lw] = 0;
+        } else {
+            *c0 = addmod(*c0, v, primes[r >> logn].q);
+        }
+    }
+}
+
+__global__ void clear_flag_kernel(unsigned long long *flag) { *flag = 0; }
+
+// flag |= 1 if any digit word is zero; digits of ct at digits + ct * dstride.
+__global__ void zero_digit_kernel(const uint64_t *__restrict__ digits, size_t dstride, size_t per_ct,
+                                  unsigned long long *flag, size_t total) {
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        if (digits[(idx / per_ct) * dstride + idx % per_ct] == 0) atomicOr(flag, 1ull);
+    }
+}
+
+struct RotLayout {
+    size_t gal, tmpct, P2, S2, C0, flag, total;
+};
+
+RotLayout rot_layout(const Context &c, uint32_t count, uint32_t L) {
+    const size_t limb = c.n * 8;
+    RotLayout l;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
+    l.gal = take(galois_keyswitch_workspace(c, count, L));
+    l.tmpct = take((size_t)count * 2 * L * limb);
+    l.P2 = take((size_t)count * 2 * L * limb);
+    l.S2 = take((size_t)count * 2 * limb);
+    l.C0 = take((size_t)count * L * limb);
+    l.flag = take(8);
+    l.total = off;
+    return l;
+}
+
+void check_level(const Context &c, uint32_t L) {
+    if (c.key_count() < 2) fail(ERR_LOGIC, "keyswitching is not supported by the context");
+    if (L < 1 || L > c.key_count() - 1) fail(ERR_INVALID_ARGUMENT, "level out of range for key switching");
+}
+
+}  // namespace
+
+size_t hoisted_key_words(const Context &c, uint32_t L) {
+    const size_t K = c.key_count();
+    return ((K - 1) * 2 * K + 2 * ((size_t)L + 1)) * c.n;
+}
+
+size_t hoisted_key_workspace(const Context &c, uint32_t L) { return 3 * ((size_t)L + 1) * c.n * 8; }
+
+void hoisted_key_prepare(const Context &c, const uint64_t *galois_key, uint32_t elt, uint32_t L, uint64_t *out,
+                         void *ws, size_t ws_bytes, hipStream_t st) {
+    check_level(c, L);
+    const uint64_t m = 2 * c.n;
+    if (!(elt & 1) || elt >= m) fail(ERR_INVALID_ARGUMENT, "Galois element is not valid");
+    if (!ws || ws_bytes < hoisted_key_workspace(c, L)) fail(ERR_INVALID_ARGUMENT, "workspace too small");
+    const uint32_t K = (uint32_t)c.key_count();
+    const size_t n = c.n;
+    const uint32_t elt_inv = (uint32_t)hm::invmod(elt, m);
+    // key' = sigma^-1(key), limb by limb
+    apply_galois(c, galois_key, K * n, out, K * n, (K - 1) * 2, K, elt_inv, true, st);
+    // corr' = sigma^-1(NTT_i(S_sigma) * sum_j (q_j mod q_i) key_j[i])
+    uint64_t *neg = static_cast<uint64_t *>(ws);
+    uint64_t *C = neg + ((size_t)L + 1) * n;
+    const size_t nw = ((size_t)L + 1) * n;
+    negated_positions_kernel<<<blocks_for(nw), TPB, 0, st>>>(neg, elt, elt_inv, c.logn, nw);
+    launch_check("negated_positions");
+    const DevPrime *targets = c.keyswitch_primes(L);
+    ntt_forward_grid(neg, LimbGrid{1, L + 1, L + 1, 1}, targets, c.logn, NTT_FULL, st);
+    correction_kernel<<<blocks_for(2 * nw), TPB, 0, st>>>(neg, galois_key, C, targets, c.primes(), L, K, c.logn,
+                                                          2 * nw);
+    launch_check("correction");
+    apply_galois(c, C, nw, out + (size_t)(K - 1) * 2 * K * n, nw, 2, L + 1, elt_inv, true, st);
+}
+
+size_t rotate_sum_workspace(const Context &c, uint32_t count, uint32_t L) { return rot_layout(c, count, L).total; }
+
+void rotate_sum(const Context &c, const uint64_t *in, uint32_t count, uint32_t L, const uint32_t *elts,
+                const uint64_t *const *hoisted, const uint64_t *const *keys, uint32_t rotations, uint64_t *out,
+                void *ws, size_t ws_bytes, hipStream_t st) {
+    check_level(c, L);
+    if (!count) return;
+    if (!rotations || !elts || !hoisted || !keys) fail(ERR_INVALID_ARGUMENT, "no rotations given");
+    const RotLayout lay = rot_layout(c, count, L);
+    if (!ws || ws_bytes < lay.total) fail(ERR_INVALID_ARGUMENT, "workspace too small for rotation sums");
+    if (in == out) fail(ERR_INVALID_ARGUMENT, "rotate_sum cannot run in place");
+    const uint64_t max_cts = ((uint64_t(1) << 34) >> c.logn) / ((uint64_t)L * (L + 1));
+    if ((uint64_t)count > max_cts) fail(ERR_INVALID_ARGUMENT, "rotation batch too large for one call");
+    const bool ckks = c.scheme == SCHEME_CKKS;
+    const int logn = c.logn;
+    const size_t n = c.n, lw = (size_t)L * n;
+    char *base = static_cast<char *>(ws);
+    void *gal_ws = base + lay.gal;
+    const size_t gal_bytes = galois_keyswitch_workspace(c, count, L);
+    uint64_t *tmpct = reinterpret_cast<uint64_t *>(base + lay.tmpct);
+    uint64_t *P2 = reinterpret_cast<uint64_t *>(base + lay.P2);
+    uint64_t *S2 = reinterpret_cast<uint64_t *>(base + lay.S2);
+    uint64_t *C0 = reinterpret_cast<uint64_t *>(base + lay.C0);
+    auto *flag = reinterpret_cast<unsigned long long *>(base + lay.flag);
+    // the hoisted path uses the key-switch regions that follow the Galois scratch row
+    const size_t sigma_bytes = align256((size_t)count * lw * 8);
+    const KsRegions ks = keyswitch_regions(c, count, L, base + lay.gal + sigma_bytes);
+
+    // 1. digits of c1 in coefficient form
+    const uint64_t *digits = in + lw;
+    size_t dstride = 2 * lw;
+    if (ckks) {
+        SHC_HIP(hipMemcpy2DAsync(ks.d, lw * 8, in + lw, 2 * lw * 8, lw * 8, count, hipMemcpyDeviceToDevice, st));
+        ntt_inverse(ks.d, count * L, L, c.primes(), logn, NTT_FULL, st);
+        digits = ks.d;
+        dstride = lw;
+    }
+    // 2. a zero digit has no sign correction: separate rotations then
+    clear_flag_kernel<<<1, 1, 0, st>>>(flag);
+    zero_digit_kernel<<<blocks_for(count * lw), TPB, 0, st>>>(digits, dstride, lw, flag, count * lw);
+    launch_check("zero_digit");
+    unsigned long long has_zero = 0;
+    SHC_HIP(hipMemcpyAsync(&has_zero, flag, 8, hipMemcpyDeviceToHost, st));
+    SHC_HIP(hipStreamSynchronize(st));
+    if (has_zero) {
+        for (uint32_t r = 0; r < rotations; r++) {
+            SHC_HIP(hipMemcpyAsync(tmpct, in, (size_t)count * 2 * lw * 8, hipMemcpyDeviceToDevice, st));
+            apply_galois_keyswitch(c, tmpct, count, L, elts[r], keys[r], gal_ws, gal_bytes, st);
+            if (r == 0) SHC_HIP(hipMemcpyAsync(out, tmpct, (size_t)count * 2 * lw * 8, hipMemcpyDeviceToDevice, st));
+            else poly_add(c, out, tmpct, out, count * 2, L, st);
+        }
+        return;
+    }
+    // 3. one ModUp for every rotation
+    modup_to_T(c, digits, dstride, ks.T, count, L, st);
+    const DevPrime *targets = c.keyswitch_primes(L);
+    const size_t K = c.key_count();
+    const size_t corr_off = (K - 1) * 2 * K * n;
+    for (uint32_t r = 0; r < rotations; r++) {
+        // (sigma(c0), 0) accumulated into out
+        apply_galois(c, in, 2 * lw, C0, lw, count, L, elts[r], ckks, st);
+        accumulate_c0_kernel<<<blocks_for(count * lw), TPB, 0, st>>>(out, C0, c.primes(), L, logn, r == 0,
+                                                                     count * lw);
+        launch_check("accumulate_c0");
+        // products with the prepared key, corrected, then permuted by sigma
+        keyswitch_inner_product(c, ks.T, hoisted[r], ks.P, ks.S, count, L, st);
+        const size_t cw = ((size_t)count * 2 * (L + 1)) << logn;
+        add_correction_kernel<<<blocks_for(cw), TPB, 0, st>>>(ks.P, ks.S, hoisted[r] + corr_off, targets, L, logn,
+                                                              cw);
+        launch_check("add_correction");
+        apply_galois(c, ks.P, lw, P2, lw, count * 2, L, elts[r], true, st);
+        apply_galois(c, ks.S, n, S2, n, count * 2, 1, elts[r], true, st);
+        // ModDown of this rotation's key switch, added into out
+        moddown_into(c, P2, S2, ks.U, out, 2 * lw, count, L, st);
+    }
+}
+
+}  // namespace shc

@@ -82,6 +82,12 @@ def lib():
         L.shc_bfv_multiply_plain_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
         L.shc_bfv_multiply_plain.argtypes = [_vp, _vp, _vp, _vp, u32, u32, _vp, sz, _vp]
         L.shc_bfv_plain_to_ntt.argtypes = [_vp, _vp, u32, _vp, _vp]
+        L.shc_hoisted_key_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
+        L.shc_hoisted_key_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
+        L.shc_hoisted_key_prepare.argtypes = [_vp, _vp, u32, u32, _vp, _vp, sz, _vp]
+        L.shc_rotate_sum_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
+        L.shc_rotate_sum.argtypes = [_vp, _vp, u32, u32, ctypes.POINTER(u32), ctypes.POINTER(_vp),
+                                     ctypes.POINTER(_vp), u32, _vp, _vp, sz, _vp]
         _lib = L
     return _lib
 
@@ -297,6 +303,25 @@ class Context:
 
     def bfv_plain_to_ntt(self, plain, limbs, out, stream=None):
         check(lib().shc_bfv_plain_to_ntt(self.handle, plain, limbs, out, stream))
+
+    def hoisted_key_size(self, limbs):
+        return self._size(lib().shc_hoisted_key_size, limbs)
+
+    def hoisted_key_workspace_size(self, limbs):
+        return self._size(lib().shc_hoisted_key_workspace_size, limbs)
+
+    def hoisted_key_prepare(self, d_key, elt, limbs, d_out, d_ws, ws_bytes, stream=None):
+        check(lib().shc_hoisted_key_prepare(self.handle, d_key, elt, limbs, d_out, d_ws, ws_bytes, stream))
+
+    def rotate_sum_workspace_size(self, count, limbs):
+        return self._size(lib().shc_rotate_sum_workspace_size, count, limbs)
+
+    def rotate_sum(self, d_in, count, limbs, elts, d_hoisted, d_keys, d_out, d_ws, ws_bytes, stream=None):
+        r = len(elts)
+        e = (ctypes.c_uint32 * r)(*elts)
+        h = (_vp * r)(*d_hoisted)
+        k = (_vp * r)(*d_keys)
+        check(lib().shc_rotate_sum(self.handle, d_in, count, limbs, e, h, k, r, d_out, d_ws, ws_bytes, stream))
 
     def probe_keyswitch(self, begin, end):
         """Arm the next key switch to record HipEvents around its ModUp rows pass."""

@@ -1,0 +1,79 @@
+"""Hoisted rotation sums on the GPU.
+
+shc_rotate_sum (one ModUp of c1 for every rotation, keys prepared with the sign-pattern
+correction) must give the words of summing separate rotations -- the oracle's
+applyGaloisInplace restatement, itself pinned against the reference -- for CKKS (NTT
+form) and BFV (coefficient form), for positive, negative and conjugation elements, and
+when c1 has zero digits (where the call falls back to separate rotations).
+"""
+import numpy as np
+import pytest
+
+import checkers as ck
+from gpu_util import empty_dev, from_dev, hip, stream, to_dev
+
+pytestmark = pytest.mark.gpu
+BFV, CKKS = 1, 2
+
+
+def ws(nbytes):
+    return empty_dev((max(nbytes, 8) + 7) // 8)
+
+
+def run_case(scheme, logn, bits, L, count, steps, zero_digits=False, seed=0):
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, bits)
+    K = len(moduli)
+    ctx = hip().Context(scheme, n, moduli, 65537 if scheme == BFV else 0)
+    rng = np.random.default_rng(seed)
+    cts = ck.uniform_residues(rng, (count, 2, L, n), moduli[:L])
+    if zero_digits:
+        # zero some coefficient-form digits of c1 (CKKS holds c1 in NTT form)
+        c1 = cts[0, 1]
+        if scheme == CKKS:
+            c1 = ck.ntt_rns(c1, moduli[:L], logn, inverse=True)
+        c1[:, 5] = 0
+        c1[1, 17] = 0
+        cts[0, 1] = ck.ntt_rns(c1, moduli[:L], logn) if scheme == CKKS else c1
+    elts = [ck.galois_elt(logn, s) for s in steps]
+    keys = [ck.random_key(rng, moduli, logn) for _ in steps]
+    dkeys = [to_dev(k) for k in keys]
+    hb = ctx.hoisted_key_size(L)
+    hw = ctx.hoisted_key_workspace_size(L)
+    hws = ws(hw)
+    hkeys = []
+    for e, dk in zip(elts, dkeys):
+        h = empty_dev(((hb + 7) // 8,))
+        ctx.hoisted_key_prepare(dk.data_ptr(), e, L, h.data_ptr(), hws.data_ptr(), hw, stream())
+        hkeys.append(h)
+    din = to_dev(cts)
+    out = empty_dev((count, 2, L, n))
+    rw = ctx.rotate_sum_workspace_size(count, L)
+    rws = ws(rw)
+    ctx.rotate_sum(din.data_ptr(), count, L, elts, [h.data_ptr() for h in hkeys], [k.data_ptr() for k in dkeys],
+                   out.data_ptr(), rws.data_ptr(), rw, stream())
+    got = from_dev(out)
+    for c in range(count):
+        want = None
+        for e, k in zip(elts, keys):
+            r = ck.apply_galois_keyswitch(scheme, cts[c], e, k, moduli, L, logn)
+            want = r if want is None else ck.mod_add(want, r, moduli)
+        np.testing.assert_array_equal(got[c], want, err_msg=f"ciphertext {c}")
+    # the input is untouched
+    np.testing.assert_array_equal(from_dev(din), cts)
+
+
+@pytest.mark.parametrize("scheme,logn,bits,L,count,steps", [
+    (CKKS, 10, [60, 40, 40, 60], 3, 2, [1, 2, -1]),
+    (CKKS, 13, [60, 40, 40, 40, 60], 4, 3, [1, 4, -3, 0]),
+    (BFV, 12, [40, 40, 50], 2, 2, [1, -2, 0]),
+    (BFV, 13, [43, 43, 44, 44, 44], 4, 1, [3, 7]),
+    (CKKS, 15, [60] + [40] * 6 + [60], 5, 2, [1, 2, 4, 8]),   # a level below the first
+])
+def test_rotate_sum_equals_separate_rotations(scheme, logn, bits, L, count, steps):
+    run_case(scheme, logn, bits, L, count, steps, seed=logn + L)
+
+
+@pytest.mark.parametrize("scheme", [CKKS, BFV])
+def test_rotate_sum_zero_digit_falls_back(scheme):
+    run_case(scheme, 12, [40, 40, 50], 2, 2, [1, 2], zero_digits=True, seed=5)
