@@ -14,6 +14,9 @@ per-ciphertext path (one call per ciphertext).
 `--workload chain`: the CKKS chain (N=2^15, 64 pairs at uneven levels, 3 x mul -> relin ->
 rescale) captured in a HIP graph.
 
+`--workload rotsum`: hoisted rotations (N=2^16, 16 ciphertexts, 15 rotations each, summed)
+against the same sum through separate rotations.
+
 `--workload copy`: a 1 GiB device copy, the calibration run for HBM byte counters.
 
 Prints one JSON line.  Data are seeded uniform residues (synthetic) and injected
@@ -42,7 +45,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="mulrelin", choices=["mulrelin", "ntt", "copy", "chain"])
+    p.add_argument("--workload", default="mulrelin", choices=["mulrelin", "ntt", "copy", "chain", "rotsum"])
     p.add_argument("--no-bfv", action="store_true", help="skip the BFV leg of the mulrelin workload")
     p.add_argument("--batch", type=int, default=128)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -475,6 +478,85 @@ def bench_chain(args):
             "self_check": "every graph replay equals the eager chain's words", "wall_s": wall}
 
 
+def bench_rotsum(args):
+    """Hoisted rotations: N=2^16, CoeffModulus::Create(65536, {60, 50 x 23, 60}) (24 data
+    primes + special), 16 ciphertexts in NTT form, each replaced by the sum of its 15
+    rotations by 2^k (k = 0..14) -- one ModUp per ciphertext (shc_rotate_sum).  The same
+    sum through 15 separate rotations per ciphertext is timed beside it, and both must
+    give the same words.  value = rotations per second."""
+    import torch
+    import sealhip
+    sealhip.init(0)
+    logn, n = 16, 1 << 16
+    moduli = sealhip.coeff_modulus_create(n, [60] + [50] * 23 + [60])
+    K = len(moduli)
+    L = K - 1
+    cts, steps = 16, [1 << k for k in range(15)]
+    ctx = sealhip.Context(sealhip.SCHEME_CKKS, n, moduli)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.empty((cts, 2, L, n), dtype=torch.int64, device="cuda")
+    fill_uniform(torch, x, moduli[:L], 2, gen)
+    elts = []
+    keys, hkeys = [], []
+    hb, hw = ctx.hoisted_key_size(L), ctx.hoisted_key_workspace_size(L)
+    hws = torch.empty(((hw + 7) // 8,), dtype=torch.int64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    for s_ in steps:
+        elts.append(ctx.galois_elt_from_step(s_))
+        k = torch.empty((K - 1, 2, K, n), dtype=torch.int64, device="cuda")
+        fill_uniform(torch, k, moduli, 2, gen)
+        h = torch.empty(((hb + 7) // 8,), dtype=torch.int64, device="cuda")
+        ctx.hoisted_key_prepare(k.data_ptr(), elts[-1], L, h.data_ptr(), hws.data_ptr(), hw, st)
+        keys.append(k)
+        hkeys.append(h)
+    out = torch.empty_like(x)
+    rw = ctx.rotate_sum_workspace_size(cts, L)
+    gw = ctx.galois_workspace_size(cts, L)
+    ws = torch.empty(((max(rw, gw) + 7) // 8,), dtype=torch.int64, device="cuda")
+    hptrs, kptrs = [h.data_ptr() for h in hkeys], [k.data_ptr() for k in keys]
+
+    def hoisted():
+        ctx.rotate_sum(x.data_ptr(), cts, L, elts, hptrs, kptrs, out.data_ptr(), ws.data_ptr(), rw, st)
+
+    tmp = torch.empty_like(x)
+    acc = torch.empty_like(x)
+
+    def separate():
+        for r, e in enumerate(elts):
+            tmp.copy_(x)
+            ctx.apply_galois_keyswitch(tmp.data_ptr(), kptrs[r], e, cts, L, ws.data_ptr(), gw, st)
+            if r == 0:
+                acc.copy_(tmp)
+            else:
+                ctx.add(acc.data_ptr(), tmp.data_ptr(), acc.data_ptr(), 2 * cts, L, st)
+
+    for _ in range(args.warmup):
+        hoisted()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with Timer(torch) as th:
+        for _ in range(args.steps):
+            hoisted()
+    wall = time.perf_counter() - t0
+    separate()
+    torch.cuda.synchronize()
+    if not torch.equal(out, acc):
+        raise SystemExit("bench self-check failed: hoisted rotation sum differs from separate rotations")
+    with Timer(torch) as ts:
+        separate()
+    ms = th.ms / args.steps
+    rots = cts * len(steps)
+    return {"metric": "CKKS rotations/s, N=2^16, 16 ct x 15 rotations summed, hoisted",
+            "value": rots / (ms * 1e-3), "unit": "rotations/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded uniform residues; injected Galois keys)",
+            "config": {"workload": "hoisted rotations: N=2^16, 24+1 primes (60, 50x23, 60), 16 ciphertexts, "
+                                   "rotations by 2^k, k=0..14, summed",
+                       "parallelism": "single GPU"},
+            "separate_ms_per_step": ts.ms, "hoisted_over_separate": ts.ms / ms,
+            "self_check": "hoisted words == sum of 15 separate rotations", "wall_s": wall}
+
+
 def bench_copy(args):
     """HBM copy of 1 GiB (torch's device copy kernel): the calibration run for the
     FETCH_SIZE / WRITE_SIZE counters (known 1 GiB read + 1 GiB written per launch)."""
@@ -502,7 +584,8 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus N > 1 is not supported "
                                    "(independent ciphertexts shard across GPUs with no exchange)"}))
         sys.exit(2)
-    out = {"mulrelin": bench_mulrelin, "ntt": bench_ntt, "copy": bench_copy, "chain": bench_chain}[args.workload](args)
+    out = {"mulrelin": bench_mulrelin, "ntt": bench_ntt, "copy": bench_copy, "chain": bench_chain,
+           "rotsum": bench_rotsum}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 
