@@ -170,9 +170,11 @@ __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, u
 
 // Columns pass: SUBS adjacent columns of one limb per block; a column is the first
 // K layers of the limb (stride 2^(LOGN-K) between its values).
-template <int LOGN, int K, int E, int SUBS, bool INV, class Load, class Store>
+// Blocks run limb (i, j) by limb with j fastest, or with i fastest when I_FAST (then I
+// is the prime count) -- the order for loads shared by all primes of one j.
+template <int LOGN, int K, int E, int SUBS, bool INV, bool I_FAST = false, class Load, class Store>
 __device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, uint32_t j0, uint32_t jn,
-                                          const Load &load_elem, const Store &store_elem) {
+                                          const Load &load_elem, const Store &store_elem, uint32_t I = 1) {
     constexpr int SZ = 1 << K;
     constexpr int C = 1 << (LOGN - K);
     constexpr int TILES = C / SUBS;
@@ -181,7 +183,7 @@ __device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, u
     const int col = tid % SUBS, t = tid / SUBS;
     const uint32_t r = blockIdx.x / TILES;
     const size_t c = (size_t)(blockIdx.x % TILES) * SUBS + col;
-    const uint32_t i = r / jn, j = j0 + r % jn;
+    const uint32_t i = I_FAST ? r % I : r / jn, j = j0 + (I_FAST ? r / I : r % jn);
     const DevPrime &P = primes[i];
     const ulonglong2 *tw = INV ? P.inv : P.fwd;
     auto addr = [col](int idx) { return idx * SUBS + col; };
@@ -298,12 +300,13 @@ template <int LOGN, int K, int E, int SUBS>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_moddown_prep_kernel(uint64_t *__restrict__ U,
                                                                                const DevPrime *__restrict__ primes,
                                                                                ModDownArgs a, uint32_t jn) {
-    cols_body<LOGN, K, E, SUBS, false>(
+    // every prime reads the same special limb: i fastest keeps it in L2
+    cols_body<LOGN, K, E, SUBS, false, true>(
         primes, 0, jn,
         [&](uint32_t i, uint32_t j, size_t off) {
             return moddown_prep(a.S[((size_t)j << LOGN) + off], a.p, primes[i], a.dc[i]);
         },
-        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { U[((size_t)(j * a.L + i) << LOGN) + off] = x; });
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { U[((size_t)(j * a.L + i) << LOGN) + off] = x; }, a.L);
 }
 
 template <int LOGN, int K, int E, int SUBS>

@@ -71,21 +71,77 @@ __global__ void correction_kernel(const uint64_t *__restrict__ negntt, const uin
     }
 }
 
-// P[ct][k][i] += corr[k][i] (i < L), S[ct][k] += corr[k][L].
-__global__ void add_correction_kernel(uint64_t *__restrict__ P, uint64_t *__restrict__ S,
-                                      const uint64_t *__restrict__ corr, const DevPrime *__restrict__ targets,
-                                      uint32_t L, int logn, size_t total) {
+// Products of R rotations at once, each T word read once for all of them:
+//   P_r[ct][k][i] = sum_j T[i][ct][j] key'_r[j][k][ki] + corr'_r[k][i]   (S_r for i == L)
+// One thread per (i, c) position and G ciphertexts, so each key word it loads serves G
+// products; blocks of one tile run on one XCD (XCD = block % 8) so that the tile's key
+// words are served from its L2.
+constexpr int RMAX = 4;
+constexpr int RG = 2;  // ciphertexts per thread
+struct RotBatch {
+    const uint64_t *key[RMAX];   // prepared keys; corr' follows at corr_off words
+    uint64_t *P[RMAX], *S[RMAX];
+};
+
+template <int R>
+__global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *__restrict__ T, RotBatch rb,
+                                                                size_t corr_off, const DevPrime *__restrict__ targets,
+                                                                uint32_t L, uint32_t K, int logn, uint32_t count,
+                                                                uint32_t tiles, uint32_t groups) {
+    const uint32_t xcd = blockIdx.x & 7, q8 = blockIdx.x >> 3;
+    const uint32_t tile = (q8 / groups) * 8 + xcd;
+    if (tile >= tiles) return;
+    const uint32_t ct0 = (q8 % groups) * RG;
+    const int gv = (int)min<uint32_t>(RG, count - ct0);
     const size_t n = (size_t)1 << logn;
-    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t c = idx & (n - 1);
-        const size_t r = idx >> logn;
-        const uint32_t i = (uint32_t)(r % (L + 1));
-        const size_t rk = r / (L + 1);  // ct * 2 + k
-        const uint32_t k = (uint32_t)(rk & 1);
-        const uint64_t q = targets[i].q;
-        const uint64_t v = corr[((size_t)k * (L + 1) + i) * n + c];
-        uint64_t *dst = i < L ? P + ((rk * L + i) << logn) + c : S + (rk << logn) + c;
-        *dst = addmod(*dst, v, q);
+    const size_t flat = (size_t)tile * TPB + threadIdx.x;
+    if (flat >= (size_t)(L + 1) << logn) return;
+    const uint32_t i = (uint32_t)(flat >> logn);
+    const size_t c = flat & (n - 1);
+    const uint32_t ki = (i == L) ? K - 1 : i;
+    const DevPrime &Q = targets[i];
+    const uint64_t *Ti = T + ((((size_t)i * count + ct0) * L) << logn) + c;
+    const size_t kstride = (size_t)K << logn;
+    u128 a0[R][RG], a1[R][RG];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int g = 0; g < RG; g++) a0[r][g] = a1[r][g] = u128{0, 0};
+    for (uint32_t j = 0; j < L; j++) {
+        uint64_t t[RG];
+#pragma unroll
+        for (int g = 0; g < RG; g++) t[g] = g < gv ? Ti[((size_t)g * L + j) << logn] : 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint64_t *kp = rb.key[r] + (size_t)ki * n + c;
+            const uint64_t k0 = kp[(2 * j) * kstride], k1 = kp[(2 * j + 1) * kstride];
+#pragma unroll
+            for (int g = 0; g < RG; g++) {
+                add128(a0[r][g], mul64x64(t[g], k0));
+                add128(a1[r][g], mul64x64(t[g], k1));
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint64_t *corr = rb.key[r] + corr_off;
+        const uint64_t c0 = corr[(size_t)i * n + c], c1 = corr[((size_t)(L + 1) + i) * n + c];
+#pragma unroll
+        for (int g = 0; g < RG; g++) {
+            if (g >= gv) continue;
+            const size_t ct = ct0 + g;
+            add128(a0[r][g], u128{c0, 0});
+            add128(a1[r][g], u128{c1, 0});
+            const uint64_t r0 = barrett128(a0[r][g], Q.q, Q.ratio_lo, Q.ratio_hi);
+            const uint64_t r1 = barrett128(a1[r][g], Q.q, Q.ratio_lo, Q.ratio_hi);
+            if (i == L) {
+                rb.S[r][(ct * 2 + 0) * n + c] = r0;
+                rb.S[r][(ct * 2 + 1) * n + c] = r1;
+            } else {
+                rb.P[r][((ct * 2 + 0) * L + i) * n + c] = r0;
+                rb.P[r][((ct * 2 + 1) * L + i) * n + c] = r1;
+            }
+        }
     }
 }
 
@@ -118,7 +174,7 @@ __global__ void zero_digit_kernel(const uint64_t *__restrict__ digits, size_t ds
 }
 
 struct RotLayout {
-    size_t gal, tmpct, P2, S2, C0, flag, total;
+    size_t gal, tmpct, P2, S2, C0, Pm, Sm, flag, total;
 };
 
 RotLayout rot_layout(const Context &c, uint32_t count, uint32_t L) {
@@ -131,6 +187,8 @@ RotLayout rot_layout(const Context &c, uint32_t count, uint32_t L) {
     l.P2 = take((size_t)count * 2 * L * limb);
     l.S2 = take((size_t)count * 2 * limb);
     l.C0 = take((size_t)count * L * limb);
+    l.Pm = take((size_t)RMAX * count * 2 * L * limb);
+    l.Sm = take((size_t)RMAX * count * 2 * limb);
     l.flag = take(8);
     l.total = off;
     return l;
@@ -233,22 +291,40 @@ void rotate_sum(const Context &c, const uint64_t *in, uint32_t count, uint32_t L
     const DevPrime *targets = c.keyswitch_primes(L);
     const size_t K = c.key_count();
     const size_t corr_off = (K - 1) * 2 * K * n;
-    for (uint32_t r = 0; r < rotations; r++) {
-        // (sigma(c0), 0) accumulated into out
-        apply_galois(c, in, 2 * lw, C0, lw, count, L, elts[r], ckks, st);
-        accumulate_c0_kernel<<<blocks_for(count * lw), TPB, 0, st>>>(out, C0, c.primes(), L, logn, r == 0,
-                                                                     count * lw);
-        launch_check("accumulate_c0");
-        // products with the prepared key, corrected, then permuted by sigma
-        keyswitch_inner_product(c, ks.T, hoisted[r], ks.P, ks.S, count, L, st);
-        const size_t cw = ((size_t)count * 2 * (L + 1)) << logn;
-        add_correction_kernel<<<blocks_for(cw), TPB, 0, st>>>(ks.P, ks.S, hoisted[r] + corr_off, targets, L, logn,
-                                                              cw);
-        launch_check("add_correction");
-        apply_galois(c, ks.P, lw, P2, lw, count * 2, L, elts[r], true, st);
-        apply_galois(c, ks.S, n, S2, n, count * 2, 1, elts[r], true, st);
-        // ModDown of this rotation's key switch, added into out
-        moddown_into(c, P2, S2, ks.U, out, 2 * lw, count, L, st);
+    uint64_t *Pm = reinterpret_cast<uint64_t *>(base + lay.Pm);
+    uint64_t *Sm = reinterpret_cast<uint64_t *>(base + lay.Sm);
+    const uint32_t tiles = (uint32_t)((((size_t)(L + 1) << logn) + TPB - 1) / TPB);
+    const uint32_t groups = (count + RG - 1) / RG;
+    const uint64_t blocks = (uint64_t)((tiles + 7) / 8) * 8 * groups;
+    if (blocks * TPB >= (uint64_t(1) << 32)) fail(ERR_INVALID_ARGUMENT, "rotation batch too large for one call");
+    for (uint32_t r0 = 0; r0 < rotations; r0 += RMAX) {
+        const uint32_t nr = rotations - r0 < (uint32_t)RMAX ? rotations - r0 : (uint32_t)RMAX;
+        // products with the prepared keys (corrections included), RMAX rotations per pass over T
+        RotBatch rb{};
+        for (uint32_t q = 0; q < nr; q++) {
+            rb.key[q] = hoisted[r0 + q];
+            rb.P[q] = Pm + (size_t)q * count * 2 * lw;
+            rb.S[q] = Sm + (size_t)q * count * 2 * n;
+        }
+        switch (nr) {
+            case 1: rotation_products_kernel<1><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, groups); break;
+            case 2: rotation_products_kernel<2><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, groups); break;
+            case 3: rotation_products_kernel<3><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, groups); break;
+            default: rotation_products_kernel<4><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, groups); break;
+        }
+        launch_check("rotation_products");
+        for (uint32_t q = 0; q < nr; q++) {
+            const uint32_t r = r0 + q;
+            // (sigma(c0), 0) accumulated into out
+            apply_galois(c, in, 2 * lw, C0, lw, count, L, elts[r], ckks, st);
+            accumulate_c0_kernel<<<blocks_for(count * lw), TPB, 0, st>>>(out, C0, c.primes(), L, logn, r == 0,
+                                                                         count * lw);
+            launch_check("accumulate_c0");
+            // the products permuted by sigma, then this rotation's ModDown added into out
+            apply_galois(c, rb.P[q], lw, P2, lw, count * 2, L, elts[r], true, st);
+            apply_galois(c, rb.S[q], n, S2, n, count * 2, 1, elts[r], true, st);
+            moddown_into(c, P2, S2, ks.U, out, 2 * lw, count, L, st);
+        }
     }
 }
 
