@@ -17,6 +17,9 @@ rescale) captured in a HIP graph.
 `--workload rotsum`: hoisted rotations (N=2^16, 16 ciphertexts, 15 rotations each, summed)
 against the same sum through separate rotations.
 
+`--workload matvec`: CKKS 1024 x 1024 matrix-vector products by baby-step giant-step
+rotations (N=2^15, batches of 16 requests).
+
 `--workload copy`: a 1 GiB device copy, the calibration run for HBM byte counters.
 
 Prints one JSON line.  Data are seeded uniform residues (synthetic) and injected
@@ -45,7 +48,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="mulrelin", choices=["mulrelin", "ntt", "copy", "chain", "rotsum"])
+    p.add_argument("--workload", default="mulrelin",
+                   choices=["mulrelin", "ntt", "copy", "chain", "rotsum", "matvec"])
     p.add_argument("--no-bfv", action="store_true", help="skip the BFV leg of the mulrelin workload")
     p.add_argument("--batch", type=int, default=128)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -557,6 +561,83 @@ def bench_rotsum(args):
             "self_check": "hoisted words == sum of 15 separate rotations", "wall_s": wall}
 
 
+def bench_matvec(args):
+    """CKKS matrix-vector products (the CKKS request type of the serving workload): N=2^15,
+    CoeffModulus::Create(32768, {60, 40 x 14, 60}), ciphertexts at the first level (15
+    primes), a 1024 x 1024 matrix as 1024 NTT-form diagonals, baby-step giant-step 32 x 32
+    (seal-cuda_amd/matvec.py).  A step is one batch of 16 requests; value = matvec/s; a
+    single request's latency is timed beside it.  Self-check: the batched product of one
+    ciphertext equals its product computed alone."""
+    import torch
+    import sealhip
+    import matvec
+    sealhip.init(0)
+    logn, n = 15, 1 << 15
+    moduli = sealhip.coeff_modulus_create(n, [60] + [40] * 14 + [60])
+    K = len(moduli)
+    L = K - 1
+    b = g = 32
+    batch = 16
+    ctx = sealhip.Context(sealhip.SCHEME_CKKS, n, moduli)
+    gen = torch.Generator(device="cuda").manual_seed(6)
+    st = torch.cuda.current_stream().cuda_stream
+    plains = torch.empty((g, b, L, n), dtype=torch.int64, device="cuda")
+    fill_uniform(torch, plains, moduli[:L], 2, gen)
+
+    def key():
+        k = torch.empty((K - 1, 2, K, n), dtype=torch.int64, device="cuda")
+        fill_uniform(torch, k, moduli, 2, gen)
+        return k
+
+    baby = [key() for _ in range(1, b)]
+    giant = [key() for _ in range(1, g)]
+    hb, hw = ctx.hoisted_key_size(L), ctx.hoisted_key_workspace_size(L)
+    hws = torch.empty(((hw + 7) // 8,), dtype=torch.int64, device="cuda")
+    hoisted = []
+    for j, k in zip(range(1, b), baby):
+        h = torch.empty(((hb + 7) // 8,), dtype=torch.int64, device="cuda")
+        ctx.hoisted_key_prepare(k.data_ptr(), ctx.galois_elt_from_step(j), L, h.data_ptr(), hws.data_ptr(), hw, st)
+        hoisted.append(h)
+    mv = matvec.BsgsMatvec(ctx, n, L, b, g, plains.data_ptr(), [k.data_ptr() for k in baby],
+                           [h.data_ptr() for h in hoisted], [k.data_ptr() for k in giant])
+    x = torch.empty((batch, 2, L, n), dtype=torch.int64, device="cuda")
+    fill_uniform(torch, x, moduli[:L], 2, gen)
+    out = torch.empty_like(x)
+    baby_buf = torch.empty((b, batch, 2, L, n), dtype=torch.int64, device="cuda")
+    inner = torch.empty_like(x)
+    wsz = mv.workspace_sizes(batch)
+    ws = torch.empty(((wsz + 7) // 8,), dtype=torch.int64, device="cuda")
+
+    def run(count):
+        mv.run(x.data_ptr(), count, out.data_ptr(), baby_buf.data_ptr(), inner.data_ptr(), ws.data_ptr(), wsz, st)
+
+    for _ in range(args.warmup):
+        run(batch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with Timer(torch) as tb:
+        for _ in range(args.steps):
+            run(batch)
+    wall = time.perf_counter() - t0
+    first = out[0].clone()
+    run(1)
+    torch.cuda.synchronize()
+    if not torch.equal(out[0], first):
+        raise SystemExit("bench self-check failed: batched matvec differs from the single-request product")
+    with Timer(torch) as t1:
+        for _ in range(args.steps):
+            run(1)
+    ms = tb.ms / args.steps
+    return {"metric": "CKKS BSGS matvec/s, N=2^15, 1024x1024, batch 16", "value": batch / (ms * 1e-3),
+            "unit": "matvec/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+            "data": "synthetic (seeded uniform residues for ciphertexts, diagonals and keys)",
+            "config": {"workload": "CKKS matvec: N=2^15, 15+1 primes (60, 40x14, 60), 1024x1024 diagonals, "
+                                   "BSGS 32x32, 16 requests per batch", "parallelism": "single GPU"},
+            "single_request_ms": t1.ms / args.steps,
+            "self_check": "batched product of request 0 == its product alone", "wall_s": wall}
+
+
 def bench_copy(args):
     """HBM copy of 1 GiB (torch's device copy kernel): the calibration run for the
     FETCH_SIZE / WRITE_SIZE counters (known 1 GiB read + 1 GiB written per launch)."""
@@ -585,7 +666,7 @@ def main():
                                    "(independent ciphertexts shard across GPUs with no exchange)"}))
         sys.exit(2)
     out = {"mulrelin": bench_mulrelin, "ntt": bench_ntt, "copy": bench_copy, "chain": bench_chain,
-           "rotsum": bench_rotsum}[args.workload](args)
+           "rotsum": bench_rotsum, "matvec": bench_matvec}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

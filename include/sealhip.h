@@ -231,6 +231,13 @@ int shc_negate(const shc_context *ctx, const uint64_t *d_a, uint64_t *d_out, uin
  * Replaces EvaluatorCuda::multiplyPlainNtt (src/evaluator_cuda.cu:1811-1867). */
 int shc_multiply_plain_ntt(const shc_context *ctx, const uint64_t *d_a, const uint64_t *d_plain, uint64_t *d_out,
                            uint32_t polys, uint32_t limb_count, void *stream);
+/* d_out[p] = sum_t d_plains[t] * d_cts[t][p] for NTT-form polynomials: d_cts
+ * [terms][polys][limb_count][N], d_plains [terms][limb_count][N], d_out [polys][L][N]
+ * (not overlapping d_cts); 128-bit sums, one reduction per 64 terms.  The inner sums of
+ * a diagonal matrix-vector product (app/matmul.cuh:98-121: multiplyPlain then add per
+ * diagonal) in one pass. */
+int shc_multiply_plain_accumulate(const shc_context *ctx, const uint64_t *d_cts, const uint64_t *d_plains,
+                                  uint32_t terms, uint32_t polys, uint32_t limb_count, uint64_t *d_out, void *stream);
 /* Ciphertexts [count][size][limb_count][N]: out = ct with poly 0 +- plain ([limb_count][N],
  * same form as ct; subtract != 0 subtracts).  CKKS add/subPlainInplace
  * (src/evaluator_cuda.cu:1544-1690). */
@@ -276,6 +283,15 @@ int shc_rotate_sum(const shc_context *ctx, const uint64_t *d_in, uint32_t count,
                    const uint32_t *galois_elts, const uint64_t *const *d_hoisted_keys,
                    const uint64_t *const *d_galois_keys, uint32_t rotations, uint64_t *d_out, void *d_workspace,
                    size_t workspace_bytes, void *stream);
+
+/* d_out[r][c] = rotate_{galois_elts[r]}(d_in[c]) (d_out: [rotations][count][2][L][N], not
+ * overlapping d_in), the words of separate shc_apply_galois_keyswitch calls, with one
+ * ModUp of c1 for all rotations -- the baby steps of a BSGS matrix-vector product.  Same
+ * arguments and workspace as shc_rotate_sum. */
+int shc_rotate_many(const shc_context *ctx, const uint64_t *d_in, uint32_t count, uint32_t limb_count,
+                    const uint32_t *galois_elts, const uint64_t *const *d_hoisted_keys,
+                    const uint64_t *const *d_galois_keys, uint32_t rotations, uint64_t *d_out, void *d_workspace,
+                    size_t workspace_bytes, void *stream);
 
 /* Instrumentation (no counterpart in the reference; for benchmarks and profilers):
  * arms `ctx` so that its next key switch (shc_relinearize, shc_switch_key,

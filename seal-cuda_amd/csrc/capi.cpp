@@ -420,6 +420,22 @@ int shc_rotate_sum(const shc_context *ctx, const uint64_t *in, uint32_t count, u
     });
 }
 
+int shc_rotate_many(const shc_context *ctx, const uint64_t *in, uint32_t count, uint32_t limbs, const uint32_t *elts,
+                    const uint64_t *const *hoisted, const uint64_t *const *keys, uint32_t rotations, uint64_t *out,
+                    void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        shc::rotate_many(ctx_of(ctx), in, count, limbs, elts, hoisted, keys, rotations, out, ws, ws_bytes,
+                         (hipStream_t)stream);
+    });
+}
+
+int shc_multiply_plain_accumulate(const shc_context *ctx, const uint64_t *cts, const uint64_t *plains, uint32_t terms,
+                                  uint32_t polys, uint32_t limbs, uint64_t *out, void *stream) {
+    return guarded([&] {
+        shc::multiply_plain_accumulate(ctx_of(ctx), cts, plains, terms, polys, limbs, out, (hipStream_t)stream);
+    });
+}
+
 int shc_probe_keyswitch(shc_context *ctx, void *begin_event, void *end_event) {
     return guarded([&] {
         const auto &c = ctx_of(ctx);

@@ -17,6 +17,10 @@ void multiply_plain_ntt(const Context &c, const uint64_t *a, const uint64_t *pla
 // Ciphertexts [count][size][L][N]: out = ct with poly 0 +- plain ([L][N], same form).
 void add_plain_poly0(const Context &c, const uint64_t *ct, const uint64_t *plain, uint64_t *out, uint32_t count,
                      uint32_t size, uint32_t L, bool subtract, hipStream_t st);
+// out[p] = sum_j plains[j] * cts[j][p] (dyadic, all NTT form): cts [terms][polys][L][N],
+// plains [terms][L][N], out [polys][L][N] (not overlapping cts).
+void multiply_plain_accumulate(const Context &c, const uint64_t *cts, const uint64_t *plains, uint32_t terms,
+                               uint32_t polys, uint32_t L, uint64_t *out, hipStream_t st);
 // BFV: out = ct with c0 +- Delta m (SEAL's scaling variant), plain m: [N] mod t.
 void bfv_add_plain(const Context &c, const uint64_t *ct, const uint64_t *plain, uint64_t *out, uint32_t count,
                    uint32_t size, uint32_t L, bool subtract, hipStream_t st);

@@ -78,6 +78,24 @@ __global__ void plain_lift_kernel(const uint64_t *__restrict__ plain, uint64_t *
     }
 }
 
+// out[ct][p][l][c] = sum_j plains[j][l][c] * cts[j][ct][p][l][c] mod q_l, 128-bit sums
+// reduced every 64 terms.
+__global__ void plain_dot_kernel(const uint64_t *__restrict__ cts, const uint64_t *__restrict__ plains,
+                                 uint64_t *__restrict__ out, const DevPrime *__restrict__ primes, uint32_t terms,
+                                 uint32_t L, int logn, size_t total) {
+    const size_t lw = (size_t)L << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t r = idx % lw;
+        const DevPrime &P = primes[r >> logn];
+        u128 acc{0, 0};
+        for (uint32_t j = 0; j < terms; j++) {
+            add128(acc, mul64x64(cts[(size_t)j * total + idx], plains[(size_t)j * lw + r]));
+            if ((j & 63) == 63) acc = u128{barrett128(acc, P.q, P.ratio_lo, P.ratio_hi), 0};
+        }
+        out[idx] = barrett128(acc, P.q, P.ratio_lo, P.ratio_hi);
+    }
+}
+
 void check_level(const Context &c, uint32_t L) {
     if (L < 1 || L > c.key_count()) fail(ERR_INVALID_ARGUMENT, "limb count out of range");
 }
@@ -147,6 +165,16 @@ void bfv_plain_to_ntt(const Context &c, const uint64_t *plain, uint32_t L, uint6
     plain_lift_kernel<<<blocks_for(lw), TPB, 0, st>>>(plain, out, c.plain, c.primes(), L, c.logn, lw);
     launch_check("plain_lift");
     ntt_forward(out, L, L, c.primes(), c.logn, NTT_FULL, st);
+}
+
+void multiply_plain_accumulate(const Context &c, const uint64_t *cts, const uint64_t *plains, uint32_t terms,
+                               uint32_t polys, uint32_t L, uint64_t *out, hipStream_t st) {
+    check_level(c, L);
+    if (!terms) fail(ERR_INVALID_ARGUMENT, "no terms to accumulate");
+    const size_t total = ((size_t)polys * L) << c.logn;
+    if (!total) return;
+    plain_dot_kernel<<<blocks_for(total), TPB, 0, st>>>(cts, plains, out, c.primes(), terms, L, c.logn, total);
+    launch_check("plain_dot");
 }
 
 size_t bfv_multiply_plain_workspace(const Context &c, uint32_t L) { return ((size_t)L << c.logn) * 8; }

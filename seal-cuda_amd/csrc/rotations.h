@@ -21,4 +21,10 @@ void rotate_sum(const Context &c, const uint64_t *in, uint32_t count, uint32_t L
                 const uint64_t *const *hoisted, const uint64_t *const *keys, uint32_t rotations, uint64_t *out,
                 void *ws, size_t ws_bytes, hipStream_t st);
 
+// out[r][ct] = rotate_{elt_r}(in[ct]) (out: [rotations][count][2][L][N]), the words of
+// separate apply_galois_keyswitch calls, with one ModUp for all rotations.
+void rotate_many(const Context &c, const uint64_t *in, uint32_t count, uint32_t L, const uint32_t *elts,
+                 const uint64_t *const *hoisted, const uint64_t *const *keys, uint32_t rotations, uint64_t *out,
+                 void *ws, size_t ws_bytes, hipStream_t st);
+
 }  // namespace shc

@@ -235,9 +235,13 @@ void hoisted_key_prepare(const Context &c, const uint64_t *galois_key, uint32_t 
 
 size_t rotate_sum_workspace(const Context &c, uint32_t count, uint32_t L) { return rot_layout(c, count, L).total; }
 
-void rotate_sum(const Context &c, const uint64_t *in, uint32_t count, uint32_t L, const uint32_t *elts,
-                const uint64_t *const *hoisted, const uint64_t *const *keys, uint32_t rotations, uint64_t *out,
-                void *ws, size_t ws_bytes, hipStream_t st) {
+namespace {
+
+// The hoisted rotations of `in`, summed into out (sum) or each into its own
+// out + r * count * 2L N (!sum).
+void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32_t L, const uint32_t *elts,
+                    const uint64_t *const *hoisted, const uint64_t *const *keys, uint32_t rotations, uint64_t *out,
+                    void *ws, size_t ws_bytes, hipStream_t st, bool sum) {
     check_level(c, L);
     if (!count) return;
     if (!rotations || !elts || !hoisted || !keys) fail(ERR_INVALID_ARGUMENT, "no rotations given");
@@ -277,11 +281,14 @@ void rotate_sum(const Context &c, const uint64_t *in, uint32_t count, uint32_t L
     unsigned long long has_zero = 0;
     SHC_HIP(hipMemcpyAsync(&has_zero, flag, 8, hipMemcpyDeviceToHost, st));
     SHC_HIP(hipStreamSynchronize(st));
+    const size_t ct_words = (size_t)count * 2 * lw;
     if (has_zero) {
         for (uint32_t r = 0; r < rotations; r++) {
-            SHC_HIP(hipMemcpyAsync(tmpct, in, (size_t)count * 2 * lw * 8, hipMemcpyDeviceToDevice, st));
-            apply_galois_keyswitch(c, tmpct, count, L, elts[r], keys[r], gal_ws, gal_bytes, st);
-            if (r == 0) SHC_HIP(hipMemcpyAsync(out, tmpct, (size_t)count * 2 * lw * 8, hipMemcpyDeviceToDevice, st));
+            uint64_t *dst = sum ? tmpct : out + r * ct_words;
+            SHC_HIP(hipMemcpyAsync(dst, in, ct_words * 8, hipMemcpyDeviceToDevice, st));
+            apply_galois_keyswitch(c, dst, count, L, elts[r], keys[r], gal_ws, gal_bytes, st);
+            if (!sum) continue;
+            if (r == 0) SHC_HIP(hipMemcpyAsync(out, tmpct, ct_words * 8, hipMemcpyDeviceToDevice, st));
             else poly_add(c, out, tmpct, out, count * 2, L, st);
         }
         return;
@@ -315,17 +322,32 @@ void rotate_sum(const Context &c, const uint64_t *in, uint32_t count, uint32_t L
         launch_check("rotation_products");
         for (uint32_t q = 0; q < nr; q++) {
             const uint32_t r = r0 + q;
-            // (sigma(c0), 0) accumulated into out
+            uint64_t *dst = sum ? out : out + r * ct_words;
+            // (sigma(c0), 0) into dst (accumulated when summing)
             apply_galois(c, in, 2 * lw, C0, lw, count, L, elts[r], ckks, st);
-            accumulate_c0_kernel<<<blocks_for(count * lw), TPB, 0, st>>>(out, C0, c.primes(), L, logn, r == 0,
-                                                                         count * lw);
+            accumulate_c0_kernel<<<blocks_for(count * lw), TPB, 0, st>>>(dst, C0, c.primes(), L, logn,
+                                                                         !sum || r == 0, count * lw);
             launch_check("accumulate_c0");
-            // the products permuted by sigma, then this rotation's ModDown added into out
+            // the products permuted by sigma, then this rotation's ModDown added into dst
             apply_galois(c, rb.P[q], lw, P2, lw, count * 2, L, elts[r], true, st);
             apply_galois(c, rb.S[q], n, S2, n, count * 2, 1, elts[r], true, st);
-            moddown_into(c, P2, S2, ks.U, out, 2 * lw, count, L, st);
+            moddown_into(c, P2, S2, ks.U, dst, 2 * lw, count, L, st);
         }
     }
+}
+
+}  // namespace
+
+void rotate_sum(const Context &c, const uint64_t *in, uint32_t count, uint32_t L, const uint32_t *elts,
+                const uint64_t *const *hoisted, const uint64_t *const *keys, uint32_t rotations, uint64_t *out,
+                void *ws, size_t ws_bytes, hipStream_t st) {
+    rotate_hoisted(c, in, count, L, elts, hoisted, keys, rotations, out, ws, ws_bytes, st, true);
+}
+
+void rotate_many(const Context &c, const uint64_t *in, uint32_t count, uint32_t L, const uint32_t *elts,
+                 const uint64_t *const *hoisted, const uint64_t *const *keys, uint32_t rotations, uint64_t *out,
+                 void *ws, size_t ws_bytes, hipStream_t st) {
+    rotate_hoisted(c, in, count, L, elts, hoisted, keys, rotations, out, ws, ws_bytes, st, false);
 }
 
 }  // namespace shc

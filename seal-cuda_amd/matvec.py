@@ -1,0 +1,86 @@
+"""Diagonal matrix-vector products on CKKS ciphertexts with baby-step giant-step
+rotations (Halevi-Shoup), over the C ABI.
+
+The reference's caller of this pattern is app/matmul.cuh:98-121 (one rotateVector and
+one multiplyPlain per diagonal, summed).  Here, for a d x d matrix with d = b * g and
+the vector replicated across the slots:
+
+    M v = sum_{h<g} rot_{h b}( sum_{j<b} diag'_{h,j} * rot_j(v) ),
+    diag'_{h,j} = rot_{-h b}(diag_{h b + j}),  diag_k[i] = M[i mod d][(i + k) mod d],
+
+where rot_k(x)[i] = x[i + k] (rotateVector(k)).  The b - 1 baby rotations share one
+ModUp (shc_rotate_many), each inner sum is one shc_multiply_plain_accumulate pass, and
+the g - 1 giant rotations are ordinary key switches.  All ciphertexts of a batch share
+the diagonals and keys.
+"""
+import numpy as np
+
+import sealhip
+
+
+def diagonals(M, slots, b, g):
+    """Pre-rotated diagonals diag'_{h,j} as complex slot vectors, [g][b][slots]."""
+    d = M.shape[0]
+    assert M.shape == (d, d) and d == b * g and slots % d == 0
+    i = np.arange(slots)
+    out = np.empty((g, b, slots), dtype=np.complex128)
+    for h in range(g):
+        for j in range(b):
+            k = h * b + j
+            diag = M[i % d, (i + k) % d]
+            out[h, j] = np.roll(diag, h * b)  # rot_{-hb}: x[i] <- diag[i - hb]
+    return out
+
+
+class BsgsMatvec:
+    """One matrix, encoded once: `plains` is a device buffer [g][b][L][N] of the
+    pre-rotated diagonals in NTT form (CKKS-encoded at the ciphertexts' level L).
+    baby_keys / baby_hoisted: Galois keys of steps 1..b-1 and their prepared forms at
+    level L; giant_keys: Galois keys of steps b, 2b, ..., (g-1) b."""
+
+    def __init__(self, ctx, n, L, b, g, plains, baby_keys, baby_hoisted, giant_keys):
+        self.ctx, self.n, self.L, self.b, self.g = ctx, n, L, b, g
+        self.plains = plains
+        self.baby_elts = [ctx.galois_elt_from_step(j) for j in range(1, b)]
+        self.giant_elts = [ctx.galois_elt_from_step(h * b) for h in range(1, g)]
+        self.baby_keys, self.baby_hoisted, self.giant_keys = baby_keys, baby_hoisted, giant_keys
+
+    def workspace_sizes(self, count):
+        ctx, L = self.ctx, self.L
+        return max(ctx.rotate_sum_workspace_size(count, L), ctx.galois_workspace_size(count, L))
+
+    def run(self, d_in, count, d_out, d_baby, d_inner, d_ws, ws_bytes, stream=None):
+        """d_in, d_out: [count][2][L][N]; d_baby: scratch [b][count][2][L][N];
+        d_inner: scratch [count][2][L][N]; d_ws: workspace_sizes(count) bytes."""
+        import ctypes
+        ctx, n, L, b, g = self.ctx, self.n, self.L, self.b, self.g
+        ct_bytes = count * 2 * L * n * 8
+        plain_words = L * n
+        # baby steps: rot_0 = v, rot_1..rot_{b-1} from one ModUp
+        _memcpy_d2d(d_baby, d_in, ct_bytes, stream)
+        if b > 1:
+            ctx.rotate_many(d_in, count, L, self.baby_elts, self.baby_hoisted, self.baby_keys,
+                            d_baby + ct_bytes, d_ws, ws_bytes, stream)
+        for h in range(g):
+            dst = d_out if h == 0 else d_inner
+            ctx.multiply_plain_accumulate(d_baby, self.plains + h * b * plain_words * 8, b, count * 2, L, dst,
+                                          stream)
+            if h > 0:
+                ctx.apply_galois_keyswitch(d_inner, self.giant_keys[h - 1], self.giant_elts[h - 1], count, L,
+                                           d_ws, ws_bytes, stream)
+                ctx.add(d_out, d_inner, d_out, count * 2, L, stream)
+
+
+_rt = None
+
+
+def _memcpy_d2d(dst, src, nbytes, stream):
+    import ctypes
+    global _rt
+    if _rt is None:
+        sealhip.lib()
+        _rt = ctypes.CDLL("libamdhip64.so")
+        _rt.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.c_void_p]
+    if _rt.hipMemcpyAsync(dst, src, nbytes, 3, stream) != 0:  # hipMemcpyDeviceToDevice
+        raise sealhip.SealHipError(-5, "hipMemcpyAsync failed")

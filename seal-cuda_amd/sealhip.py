@@ -88,6 +88,8 @@ def lib():
         L.shc_rotate_sum_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
         L.shc_rotate_sum.argtypes = [_vp, _vp, u32, u32, ctypes.POINTER(u32), ctypes.POINTER(_vp),
                                      ctypes.POINTER(_vp), u32, _vp, _vp, sz, _vp]
+        L.shc_rotate_many.argtypes = L.shc_rotate_sum.argtypes
+        L.shc_multiply_plain_accumulate.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, _vp]
         _lib = L
     return _lib
 
@@ -322,6 +324,16 @@ class Context:
         h = (_vp * r)(*d_hoisted)
         k = (_vp * r)(*d_keys)
         check(lib().shc_rotate_sum(self.handle, d_in, count, limbs, e, h, k, r, d_out, d_ws, ws_bytes, stream))
+
+    def rotate_many(self, d_in, count, limbs, elts, d_hoisted, d_keys, d_out, d_ws, ws_bytes, stream=None):
+        r = len(elts)
+        e = (ctypes.c_uint32 * r)(*elts)
+        h = (_vp * r)(*d_hoisted)
+        k = (_vp * r)(*d_keys)
+        check(lib().shc_rotate_many(self.handle, d_in, count, limbs, e, h, k, r, d_out, d_ws, ws_bytes, stream))
+
+    def multiply_plain_accumulate(self, d_cts, d_plains, terms, polys, limbs, d_out, stream=None):
+        check(lib().shc_multiply_plain_accumulate(self.handle, d_cts, d_plains, terms, polys, limbs, d_out, stream))
 
     def probe_keyswitch(self, begin, end):
         """Arm the next key switch to record HipEvents around its ModUp rows pass."""

@@ -1,0 +1,120 @@
+"""BSGS diagonal matrix-vector product (seal-cuda_amd/matvec.py) on the GPU.
+
+A CKKS encryption of a replicated vector v times a d x d matrix M (d = 64 = 8 x 8 baby
+and giant steps, N = 2^12): the ciphertext words equal the oracle's composition of the
+same operations (rotations as applyGaloisInplace, plain products, sums), and the
+decryption decodes to M v within CKKS noise.
+"""
+import numpy as np
+import pytest
+import torch
+
+import checkers as ck
+from gpu_util import empty_dev, from_dev, hip, stream, to_dev
+
+pytestmark = pytest.mark.gpu
+CKKS = 2
+
+
+def ws(nbytes):
+    return empty_dev((max(nbytes, 8) + 7) // 8)
+
+
+def interleave(v):
+    return np.stack([np.real(v), np.imag(v)], axis=1).reshape(-1)
+
+
+def test_bsgs_matvec_words_and_values():
+    import matvec
+    logn, n = 12, 1 << 12
+    slots = n // 2
+    moduli = ck.coeff_modulus_create(n, [60, 40, 40, 60])
+    K = len(moduli)
+    L = K - 1
+    ctx = hip().Context(CKKS, n, moduli)
+    d, b, g = 64, 8, 8
+    scale = 2.0 ** 40
+    rng = np.random.default_rng(31)
+    M = rng.uniform(-1, 1, (d, d))
+    v = rng.uniform(-1, 1, d)
+    # keys
+    sk, pk = empty_dev((K, n)), empty_dev((2, K, n))
+    ctx.keygen_secret(7, sk.data_ptr(), stream())
+    ctx.keygen_public(sk.data_ptr(), 8, pk.data_ptr(), stream())
+    kw = ctx.keygen_workspace_size()
+    kb = ws(kw)
+    hb, hw = ctx.hoisted_key_size(L), ctx.hoisted_key_workspace_size(L)
+    hws = ws(hw)
+
+    def galois_key(step, seed):
+        k = empty_dev((K - 1, 2, K, n))
+        ctx.keygen_galois(sk.data_ptr(), ctx.galois_elt_from_step(step), seed, k.data_ptr(), kb.data_ptr(), kw,
+                          stream())
+        return k
+
+    baby = [galois_key(j, 100 + j) for j in range(1, b)]
+    hoisted = []
+    for j, k in zip(range(1, b), baby):
+        h = empty_dev(((hb + 7) // 8,))
+        ctx.hoisted_key_prepare(k.data_ptr(), ctx.galois_elt_from_step(j), L, h.data_ptr(), hws.data_ptr(), hw,
+                                stream())
+        hoisted.append(h)
+    giant = [galois_key(h * b, 200 + h) for h in range(1, g)]
+    # pre-rotated diagonals, encoded at the ciphertext level
+    diags = matvec.diagonals(M, slots, b, g)
+    plains = empty_dev((g, b, L, n))
+    ew = ctx.ckks_encode_workspace_size()
+    eb = ws(ew)
+    keep = []
+    for h in range(g):
+        for j in range(b):
+            dv = torch.from_numpy(interleave(diags[h, j])).cuda()
+            keep.append(dv)
+            ctx.ckks_encode(dv.data_ptr(), slots, scale, L, plains[h, j].data_ptr(), eb.data_ptr(), ew, stream())
+    # the vector, replicated over the slots, encrypted
+    vs = np.tile(v, slots // d).astype(np.complex128)
+    dvv = torch.from_numpy(interleave(vs)).cuda()
+    pv = empty_dev((L, n))
+    ctx.ckks_encode(dvv.data_ptr(), slots, scale, L, pv.data_ptr(), eb.data_ptr(), ew, stream())
+    ct = empty_dev((1, 2, L, n))
+    cw = ctx.encrypt_workspace_size(L)
+    cb = ws(cw)
+    ctx.encrypt(pk.data_ptr(), pv.data_ptr(), L, 9, ct.data_ptr(), cb.data_ptr(), cw, stream())
+    # the product on the GPU
+    mv = matvec.BsgsMatvec(ctx, n, L, b, g, plains.data_ptr(), [k.data_ptr() for k in baby],
+                           [h.data_ptr() for h in hoisted], [k.data_ptr() for k in giant])
+    out = empty_dev((1, 2, L, n))
+    baby_buf = empty_dev((b, 1, 2, L, n))
+    inner = empty_dev((1, 2, L, n))
+    wsz = mv.workspace_sizes(1)
+    wb = ws(wsz)
+    mv.run(ct.data_ptr(), 1, out.data_ptr(), baby_buf.data_ptr(), inner.data_ptr(), wb.data_ptr(), wsz, stream())
+    got = from_dev(out)[0]
+    # the same operations through the oracle, word for word
+    c0 = from_dev(ct)[0]
+    P = from_dev(plains)
+    rots = [c0] + [ck.apply_galois_keyswitch(CKKS, c0, ctx.galois_elt_from_step(j), from_dev(k), moduli, L, logn)
+                   for j, k in zip(range(1, b), baby)]
+    want = None
+    for h in range(g):
+        acc = None
+        for j in range(b):
+            t = ck.multiply_plain_ntt(rots[j], P[h, j], moduli, logn)
+            acc = t if acc is None else ck.mod_add(acc, t, moduli)
+        if h > 0:
+            acc = ck.apply_galois_keyswitch(CKKS, acc, ctx.galois_elt_from_step(h * b), from_dev(giant[h - 1]),
+                                            moduli, L, logn)
+        want = acc if want is None else ck.mod_add(want, acc, moduli)
+    np.testing.assert_array_equal(got, want)
+    # and it computes M v
+    dec = empty_dev((L, n))
+    ctx.decrypt(out.data_ptr(), 2, sk.data_ptr(), L, dec.data_ptr(), 0, 0, stream())
+    res = torch.empty(n, dtype=torch.float64, device="cuda")
+    dw = ctx.ckks_decode_workspace_size(L)
+    db = ws(dw)
+    ctx.ckks_decode(dec.data_ptr(), L, scale * scale, res.data_ptr(), db.data_ptr(), dw, stream())
+    torch.cuda.synchronize()
+    r = res.cpu().numpy()
+    vals = r[0::2] + 1j * r[1::2]
+    np.testing.assert_allclose(vals[:d].real, M @ v, atol=1e-3)
+    np.testing.assert_allclose(vals[d:2 * d].real, M @ v, atol=1e-3)

@@ -1,7 +1,7 @@
-"""Hoisted rotation sums on the GPU.
+"""Hoisted rotations on the GPU (sums and separate outputs).
 
-shc_rotate_sum (one ModUp of c1 for every rotation, keys prepared with the sign-pattern
-correction) must give the words of summing separate rotations -- the oracle's
+shc_rotate_sum and shc_rotate_many (one ModUp of c1 for every rotation, keys prepared with
+the sign-pattern correction) must give the words of summing (or listing) separate rotations -- the oracle's
 applyGaloisInplace restatement, itself pinned against the reference -- for CKKS (NTT
 form) and BFV (coefficient form), for positive, negative and conjugation elements, and
 when c1 has zero digits (where the call falls back to separate rotations).
@@ -48,16 +48,19 @@ def run_case(scheme, logn, bits, L, count, steps, zero_digits=False, seed=0):
         hkeys.append(h)
     din = to_dev(cts)
     out = empty_dev((count, 2, L, n))
+    many = empty_dev((len(elts), count, 2, L, n))
     rw = ctx.rotate_sum_workspace_size(count, L)
     rws = ws(rw)
-    ctx.rotate_sum(din.data_ptr(), count, L, elts, [h.data_ptr() for h in hkeys], [k.data_ptr() for k in dkeys],
-                   out.data_ptr(), rws.data_ptr(), rw, stream())
-    got = from_dev(out)
+    hp, kp = [h.data_ptr() for h in hkeys], [k.data_ptr() for k in dkeys]
+    ctx.rotate_sum(din.data_ptr(), count, L, elts, hp, kp, out.data_ptr(), rws.data_ptr(), rw, stream())
+    ctx.rotate_many(din.data_ptr(), count, L, elts, hp, kp, many.data_ptr(), rws.data_ptr(), rw, stream())
+    got, got_many = from_dev(out), from_dev(many)
     for c in range(count):
         want = None
-        for e, k in zip(elts, keys):
-            r = ck.apply_galois_keyswitch(scheme, cts[c], e, k, moduli, L, logn)
-            want = r if want is None else ck.mod_add(want, r, moduli)
+        for r, (e, k) in enumerate(zip(elts, keys)):
+            rot = ck.apply_galois_keyswitch(scheme, cts[c], e, k, moduli, L, logn)
+            np.testing.assert_array_equal(got_many[r, c], rot, err_msg=f"rotation {r} of ciphertext {c}")
+            want = rot if want is None else ck.mod_add(want, rot, moduli)
         np.testing.assert_array_equal(got[c], want, err_msg=f"ciphertext {c}")
     # the input is untouched
     np.testing.assert_array_equal(from_dev(din), cts)
