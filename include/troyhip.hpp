@@ -14,6 +14,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <istream>
+#include <ostream>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -45,6 +48,90 @@ inline void check(int status) {
 inline void hip_check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
+
+namespace detail {
+// BLAKE2b (RFC 7693), unkeyed, outlen <= 64 bytes: the hash the reference names
+// parameter sets with (src/utils/hash.h:25-32 over src/encryptionparams.cpp:118-156).
+inline void blake2b(const uint8_t *in, size_t inlen, uint8_t *out, size_t outlen) {
+    static const uint64_t IV[8] = {0x6a09e667f3bcc908ull, 0xbb67ae8584caa73bull, 0x3c6ef372fe94f82bull,
+                                   0xa54ff53a5f1d36f1ull, 0x510e527fade682d1ull, 0x9b05688c2b3e6c1full,
+                                   0x1f83d9abfb41bd6bull, 0x5be0cd19137e2179ull};
+    static const uint8_t SIGMA[12][16] = {
+        {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15}, {14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3},
+        {11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4}, {7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8},
+        {9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13}, {2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9},
+        {12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11}, {13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10},
+        {6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5}, {10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0},
+        {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15}, {14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3}};
+    auto rotr = [](uint64_t x, int r) { return (x >> r) | (x << (64 - r)); };
+    uint64_t h[8];
+    for (int i = 0; i < 8; i++) h[i] = IV[i];
+    h[0] ^= 0x01010000ull ^ (uint64_t)outlen;
+    uint64_t t = 0;
+    auto compress = [&](const uint8_t *block, bool last) {
+        uint64_t m[16], v[16];
+        for (int i = 0; i < 16; i++) {
+            m[i] = 0;
+            for (int b = 0; b < 8; b++) m[i] |= (uint64_t)block[8 * i + b] << (8 * b);
+        }
+        for (int i = 0; i < 8; i++) v[i] = h[i], v[i + 8] = IV[i];
+        v[12] ^= t;
+        if (last) v[14] = ~v[14];
+        auto G = [&](int a, int b, int c, int d, uint64_t x, uint64_t y) {
+            v[a] = v[a] + v[b] + x;
+            v[d] = rotr(v[d] ^ v[a], 32);
+            v[c] = v[c] + v[d];
+            v[b] = rotr(v[b] ^ v[c], 24);
+            v[a] = v[a] + v[b] + y;
+            v[d] = rotr(v[d] ^ v[a], 16);
+            v[c] = v[c] + v[d];
+            v[b] = rotr(v[b] ^ v[c], 63);
+        };
+        for (int r = 0; r < 12; r++) {
+            const uint8_t *s = SIGMA[r];
+            G(0, 4, 8, 12, m[s[0]], m[s[1]]);
+            G(1, 5, 9, 13, m[s[2]], m[s[3]]);
+            G(2, 6, 10, 14, m[s[4]], m[s[5]]);
+            G(3, 7, 11, 15, m[s[6]], m[s[7]]);
+            G(0, 5, 10, 15, m[s[8]], m[s[9]]);
+            G(1, 6, 11, 12, m[s[10]], m[s[11]]);
+            G(2, 7, 8, 13, m[s[12]], m[s[13]]);
+            G(3, 4, 9, 14, m[s[14]], m[s[15]]);
+        }
+        for (int i = 0; i < 8; i++) h[i] ^= v[i] ^ v[i + 8];
+    };
+    uint8_t block[128];
+    size_t off = 0;
+    while (inlen - off > 128) {
+        t += 128;
+        compress(in + off, false);
+        off += 128;
+    }
+    const size_t rest = inlen - off;
+    std::memset(block, 0, 128);
+    if (rest) std::memcpy(block, in + off, rest);
+    t += rest;
+    compress(block, true);
+    for (size_t i = 0; i < outlen; i++) out[i] = (uint8_t)(h[i / 8] >> (8 * (i % 8)));
+}
+
+
+// parms_id of a parameter set: BLAKE2b-256 of the words (scheme, N, q_0 .. q_{L-1},
+// plain modulus), little endian (src/encryptionparams.cpp:118-156).
+inline std::array<uint64_t, 4> parms_hash(uint64_t scheme, uint64_t n, const std::vector<uint64_t> &q, uint64_t t) {
+    std::vector<uint64_t> words{scheme, n};
+    words.insert(words.end(), q.begin(), q.end());
+    words.push_back(t);
+    std::vector<uint8_t> bytes(words.size() * 8);
+    for (size_t i = 0; i < words.size(); i++)
+        for (int b = 0; b < 8; b++) bytes[8 * i + b] = (uint8_t)(words[i] >> (8 * b));
+    uint8_t out[32];
+    blake2b(bytes.data(), bytes.size(), out, 32);
+    std::array<uint64_t, 4> h{};
+    for (int i = 0; i < 32; i++) h[i / 8] |= (uint64_t)out[i] << (8 * (i % 8));
+    return h;
+}
+}  // namespace detail
 
 // KernelProvider::initialize (src/kernelprovider.cuh:18-21): must run before any object
 // that touches the device is created.
@@ -326,6 +413,20 @@ public:
         return s_->ws.get();
     }
     uint64_t prime(size_t i) const { return s_->parms.coeffModulus()[i].value(); }
+    // The reference's parms_id of a level: BLAKE2b-256 of (scheme, N, the level's primes,
+    // plain modulus) as little-endian words (EncryptionParameters::computeParmsID,
+    // src/encryptionparams.cpp:118-156).
+    std::array<uint64_t, 4> parmsHash(ParmsID id) const {
+        std::vector<uint64_t> q;
+        for (uint32_t i = 0; i < id.level; i++) q.push_back(prime(i));
+        return detail::parms_hash((uint64_t)scheme(), polyModulusDegree(), q, s_->parms.plainModulus().value());
+    }
+    // The level whose parms_id is `h`, or parmsIDZero.
+    ParmsID levelOfHash(const std::array<uint64_t, 4> &h) const {
+        for (uint32_t l = 1; l <= keyCount(); l++)
+            if (parmsHash(ParmsID{l}) == h) return ParmsID{l};
+        return parmsIDZero;
+    }
     int levelBits(ParmsID id) const {
         int b = 0;
         for (uint32_t i = 0; i < id.level; i++) b += s_->parms.coeffModulus()[i].bitCount();
@@ -369,16 +470,59 @@ private:
     double scale_ = 1.0;
 };
 
+class Evaluator;
+
+namespace detail {
+template <class T>
+inline void put(std::ostream &os, const T &v) {
+    os.write(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+template <class T>
+inline T get(std::istream &is) {
+    T v{};
+    is.read(reinterpret_cast<char *>(&v), sizeof(T));
+    if (!is) throw std::invalid_argument("stream ended before the ciphertext did");
+    return v;
+}
+}  // namespace detail
+
 // CiphertextCuda (src/ciphertext_cuda.cuh:12-246): [size][L][N] words on the device.
 class Ciphertext {
 public:
     Ciphertext() = default;
     void resize(const SEALContext &ctx, ParmsID id, size_t size) {
         n_ = ctx.polyModulusDegree();
+        if (id != parms_id_ || parms_hash_ == std::array<uint64_t, 4>{}) parms_hash_ = ctx.parmsHash(id);
         parms_id_ = id;
         size_ = size;
         data_.resize(size * id.level * n_);
     }
+
+    // CiphertextCuda::save / load (src/ciphertext_cuda.cu:8-35, 138-183): parms_id
+    // (32 B), is_ntt_form (1 B), size, N, L (8 B each), scale (double), correction
+    // factor (1), seed (0), terms flag (false), word count, words -- the reference's raw
+    // layout, so files move between the two.  Seeded ciphertexts (c1 expanded from a
+    // curand stream) are not loadable here.
+    void save(std::ostream &os) const {
+        saveHeader(os, false);
+        const std::vector<uint64_t> h = data_.toHost();
+        const uint64_t words = (uint64_t)(size_ * polyWords());
+        detail::put(os, words);
+        os.write(reinterpret_cast<const char *>(h.data()), words * 8);
+    }
+    void load(std::istream &is, const SEALContext &ctx) {
+        loadHeader(is, ctx, false);
+        const uint64_t words = detail::get<uint64_t>(is);
+        if (words != size_ * polyWords()) throw std::invalid_argument("ciphertext word count does not match its shape");
+        std::vector<uint64_t> h(words);
+        is.read(reinterpret_cast<char *>(h.data()), words * 8);
+        if (!is) throw std::invalid_argument("stream ended before the ciphertext did");
+        data_.fromHost(h.data(), h.size());
+    }
+    // saveTerms / loadTerms (src/ciphertext_cuda.cu:37-136): only the listed coefficients
+    // of c0 (coefficient form), then every other polynomial in full.
+    void saveTerms(std::ostream &os, const Evaluator &ev, const std::vector<size_t> &term_ids) const;
+    void loadTerms(std::istream &is, const Evaluator &ev, const SEALContext &ctx, const std::vector<size_t> &term_ids);
     ParmsID parmsID() const { return parms_id_; }
     ParmsID &parmsID() { return parms_id_; }
     size_t size() const { return size_; }
@@ -397,9 +541,42 @@ public:
     // Keeps the first `size` polynomials (relinearization drops the third).
     void shrink(size_t size) { size_ = size; }
     DeviceArray<uint64_t> &dynArray() { return data_; }
+    const std::array<uint64_t, 4> &parmsHash() const { return parms_hash_; }
 
 private:
+    void saveHeader(std::ostream &os, bool terms) const {
+        detail::put(os, parms_hash_);
+        detail::put(os, ntt_form_);
+        detail::put(os, (uint64_t)size_);
+        detail::put(os, (uint64_t)n_);
+        detail::put(os, (uint64_t)coeffModulusSize());
+        detail::put(os, scale_);
+        detail::put(os, (uint64_t)1);  // correction factor
+        detail::put(os, (uint64_t)0);  // seed
+        detail::put(os, terms);
+    }
+    void loadHeader(std::istream &is, const SEALContext &ctx, bool terms) {
+        const auto hash = detail::get<std::array<uint64_t, 4>>(is);
+        const bool ntt = detail::get<bool>(is);
+        const uint64_t size = detail::get<uint64_t>(is), n = detail::get<uint64_t>(is), L = detail::get<uint64_t>(is);
+        const double scale = detail::get<double>(is);
+        (void)detail::get<uint64_t>(is);  // correction factor
+        const uint64_t seed = detail::get<uint64_t>(is);
+        const bool is_terms = detail::get<bool>(is);
+        if (seed != 0) throw std::invalid_argument("seeded ciphertexts are not supported");
+        if (is_terms != terms)
+            throw std::invalid_argument(terms ? "Trying to load a normal ciphertext, but term indices is specified"
+                                              : "Trying to load a termed ciphertext, but indices is not specified");
+        const ParmsID id = ctx.levelOfHash(hash);
+        if (id == parmsIDZero || id.level != L || n != ctx.polyModulusDegree())
+            throw std::invalid_argument("ciphertext is not valid for encryption parameters");
+        resize(ctx, id, size);
+        ntt_form_ = ntt;
+        scale_ = scale;
+    }
+
     DeviceArray<uint64_t> data_;
+    std::array<uint64_t, 4> parms_hash_{};
     ParmsID parms_id_ = parmsIDZero;
     size_t size_ = 0, n_ = 0;
     bool ntt_form_ = false;
@@ -1141,5 +1318,40 @@ private:
 
     SEALContext ctx_;
 };
+
+inline void Ciphertext::saveTerms(std::ostream &os, const Evaluator &ev, const std::vector<size_t> &term_ids) const {
+    Ciphertext coeff = *this;
+    if (coeff.isNttForm()) ev.transformFromNttInplace(coeff);
+    const std::vector<uint64_t> h = coeff.toHost();
+    saveHeader(os, true);
+    const size_t L = coeffModulusSize();
+    for (size_t id : term_ids) {
+        if (id >= n_) throw std::invalid_argument("term index out of range");
+        for (size_t j = 0; j < L; j++) detail::put(os, h[j * n_ + id]);
+    }
+    const uint64_t rest = (uint64_t)((size_ - 1) * polyWords());
+    detail::put(os, rest);
+    os.write(reinterpret_cast<const char *>(h.data() + polyWords()), rest * 8);
+}
+
+inline void Ciphertext::loadTerms(std::istream &is, const Evaluator &ev, const SEALContext &ctx,
+                                  const std::vector<size_t> &term_ids) {
+    loadHeader(is, ctx, true);
+    const size_t L = coeffModulusSize();
+    std::vector<uint64_t> h(size_ * polyWords(), 0);
+    for (size_t id : term_ids) {
+        if (id >= n_) throw std::invalid_argument("term index out of range");
+        for (size_t j = 0; j < L; j++) h[j * n_ + id] = detail::get<uint64_t>(is);
+    }
+    const uint64_t rest = detail::get<uint64_t>(is);
+    if (rest != (size_ - 1) * polyWords()) throw std::invalid_argument("ciphertext word count does not match its shape");
+    is.read(reinterpret_cast<char *>(h.data() + polyWords()), rest * 8);
+    if (!is) throw std::invalid_argument("stream ended before the ciphertext did");
+    data_.fromHost(h.data(), h.size());
+    if (ntt_form_) {
+        ntt_form_ = false;
+        ev.transformToNttInplace(*this);
+    }
+}
 
 }  // namespace troyhip

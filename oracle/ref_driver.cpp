@@ -353,4 +353,15 @@ int ref_plain_op(void *h, uint64_t L, uint64_t size, int op, const uint64_t *ct,
     } catch (const std::exception &e) { fprintf(stderr, "ref_plain_op: %s\n", e.what()); return -1; }
 }
 
+// EncryptionParameters::parmsID (src/encryptionparams.cpp:118-156) of the chain level
+// with L primes; out: 4 words.
+int ref_parms_id(void *h, uint64_t L, uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        ParmsID id = L == s.K ? s.context->keyParmsID() : parms_for_level(s, L);
+        for (int i = 0; i < 4; i++) out[i] = id[i];
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_parms_id: %s\n", e.what()); return -1; }
+}
+
 }  // extern "C"

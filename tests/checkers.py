@@ -113,6 +113,7 @@ def ref():
         lib.ref_ckks_decode.argtypes = [ctypes.c_void_p, u64, u64p, ctypes.c_double, dp]
         lib.ref_ct_binary.argtypes = [ctypes.c_void_p, u64, u64, ctypes.c_int, u64p, u64p, u64p]
         lib.ref_plain_op.argtypes = [ctypes.c_void_p, u64, u64, ctypes.c_int, u64p, u64p, u64p]
+        lib.ref_parms_id.argtypes = [ctypes.c_void_p, u64, u64p]
         _ref = lib
     return _ref
 
@@ -436,6 +437,11 @@ class RefSession:
         assert ref().ref_plain_op(self.h, L, ct.shape[0], op, ptr(ct), ptr(np.ascontiguousarray(plain, dtype=np.uint64)),
                                   ptr(out)) == 0
         return out
+
+    def parms_id(self, L):
+        out = np.zeros(4, dtype=np.uint64)
+        assert ref().ref_parms_id(self.h, L, ptr(out)) == 0
+        return [int(x) for x in out]
 
     def rescale(self, ct, L):
         size = ct.shape[0]

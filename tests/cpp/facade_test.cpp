@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <sstream>
 
 using namespace troyhip;
 
@@ -202,6 +203,39 @@ static void ckks() {
     Ciphertext sq;
     ev.multiply(cx, cx, sq);
     EXPECT(throws_invalid([&] { Ciphertext t2; ev.multiply(sq, cx, t2); }), "ckks size-3 multiply throws");
+
+    // serialization in the reference's layout: round trip, header fields, terms
+    {
+        std::stringstream ss;
+        cx.save(ss);
+        const std::string bytes = ss.str();
+        uint64_t size_field = 0, n_field = 0, l_field = 0;
+        std::memcpy(&size_field, bytes.data() + 33, 8);
+        std::memcpy(&n_field, bytes.data() + 41, 8);
+        std::memcpy(&l_field, bytes.data() + 49, 8);
+        EXPECT(bytes.size() == 32 + 1 + 8 * 3 + 8 * 3 + 1 + 8 + cx.size() * cx.polyWords() * 8 && size_field == 2 &&
+                   n_field == n && l_field == 3 && bytes[32] == 1,
+               "ckks save layout");
+        Ciphertext back;
+        back.load(ss, ctx);
+        EXPECT(back.toHost() == cx.toHost() && back.scale() == cx.scale() && back.isNttForm() &&
+                   back.parmsID() == cx.parmsID(),
+               "ckks save/load round trip");
+        std::vector<size_t> terms{0, 1, 7, 100};
+        std::stringstream st;
+        cx.saveTerms(st, ev, terms);
+        Ciphertext part;
+        part.loadTerms(st, ev, ctx, terms);
+        Ciphertext full_coeff = cx, part_coeff = part;
+        ev.transformFromNttInplace(full_coeff);
+        ev.transformFromNttInplace(part_coeff);
+        const auto f = full_coeff.toHost(), g = part_coeff.toHost();
+        bool ok = true;
+        for (size_t j = 0; j < 3; j++)
+            for (size_t c : terms) ok = ok && f[j * n + c] == g[j * n + c];
+        for (size_t i = 3 * n; i < f.size(); i++) ok = ok && f[i] == g[i];
+        EXPECT(ok, "ckks saveTerms/loadTerms keep the listed terms and c1");
+    }
 
     EncryptionParameters bad(SchemeType::ckks);
     bad.setPolyModulusDegree(8192);
