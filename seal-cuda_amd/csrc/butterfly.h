@@ -12,15 +12,21 @@ namespace shc {
 template <bool INV>
 __host__ __device__ __forceinline__ void butterfly(uint64_t &x, uint64_t &y, ulonglong2 w, uint64_t four_q,
                                                    uint64_t neg_q) {
+    // v >= 4q ? v - 4q : v for v < 8q: with 4q < 2^63, v - 4q wraps above 2^63 exactly
+    // when v < 4q, so the sign of the high word selects
+    auto sub_if_ge = [four_q](uint64_t v) {
+        const uint64_t t = v - four_q;
+        return (int32_t)(t >> 32) < 0 ? v : t;
+    };
     if (!INV) {
-        const uint64_t u = x >= four_q ? x - four_q : x;
+        const uint64_t u = sub_if_ge(x);
         const uint64_t v = shoup_lazier(y, w.x, w.y, neg_q);
         x = u + v;
         y = u + four_q - v;
     } else {
         const uint64_t s = x + y;
         const uint64_t d = x + four_q - y;
-        x = s >= four_q ? s - four_q : s;
+        x = sub_if_ge(s);
         y = shoup_lazier(d, w.x, w.y, neg_q);
     }
 }

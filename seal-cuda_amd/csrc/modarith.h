@@ -56,8 +56,12 @@ __host__ __device__ __forceinline__ uint64_t shoup_lazier(uint64_t x, uint64_t w
     const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), wl = (uint32_t)w, wh = (uint32_t)(w >> 32);
     const uint32_t ql = (uint32_t)Q, qh = (uint32_t)(Q >> 32), nl = (uint32_t)neg_q, nh = (uint32_t)(neg_q >> 32);
     const uint64_t p = (uint64_t)xl * wl + (uint64_t)ql * nl;
-    const uint32_t cross = xl * wh + xh * wl + ql * nh + qh * nl;
-    return p + ((uint64_t)cross << 32);
+    // the cross products only reach the high word: 32-bit arithmetic there
+    uint32_t hi = (uint32_t)(p >> 32) + xl * wh + xh * wl + ql * nh + qh * nl;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(hi));  // keeps the 32-bit add from being widened back to 64 bits
+#endif
+    return ((uint64_t)hi << 32) | (uint32_t)p;
 }
 
 __device__ __forceinline__ uint64_t shoup(uint64_t x, uint64_t w, uint64_t wq, uint64_t q) {

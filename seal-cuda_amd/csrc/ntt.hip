@@ -161,9 +161,21 @@ __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, u
         auto stage = [&](int idx, uint64_t x) { tile[addr(idx)] = x; };
         run_phases<LOGN, K, E, INV>(tile, addr, load, stage, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
         __syncthreads();
-        for (int e = tid; e < nsub * SZ; e += SUBS * T) {
-            const int sb = e / SZ, off = e % SZ;
-            store_elem(i, jfirst + (uint32_t)sb, row_off + off, tile[sb * SZ + swz(off)]);
+        if (nsub == SUBS) {
+            // a full block: a fixed number of elements per thread, unrolled so that every
+            // row's base address is uniform (scalar) and only the offset varies by lane
+            constexpr int PER = SUBS * SZ / (SUBS * T);
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const int e = tid + k * SUBS * T;
+                const int sb = e / SZ, off = e % SZ;
+                store_elem(i, jfirst + (uint32_t)sb, row_off + off, tile[sb * SZ + swz(off)]);
+            }
+        } else {
+            for (int e = tid; e < nsub * SZ; e += SUBS * T) {
+                const int sb = e / SZ, off = e % SZ;
+                store_elem(i, jfirst + (uint32_t)sb, row_off + off, tile[sb * SZ + swz(off)]);
+            }
         }
     }
 }
@@ -215,15 +227,16 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__r
 }
 
 // The key switch's ModUp rows pass: the same code as a kernel of its own, so that
-// profiles tell it apart from the other forward transforms.
-template <int LOGN, int K, int E, int SUBS>
+// profiles tell it apart from the other forward transforms.  LAZY leaves outputs in
+// [0, 4q) for consumers whose 128-bit sums absorb that range.
+template <int LOGN, int K, int E, int SUBS, bool LAZY>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_modup_kernel(uint64_t *__restrict__ data,
                                                                         const DevPrime *__restrict__ primes,
                                                                         LimbGrid lg) {
     const GridAccess g{data, lg, LOGN};
     rows_body<LOGN, K, E, SUBS, false>(
         primes, lg.I, 0, lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
-        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { *g.at(i, j, off) = finish<false, true>(x, primes[i]); });
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { *g.at(i, j, off) = finish<false, !LAZY>(x, primes[i]); });
 }
 
 template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
@@ -420,7 +433,7 @@ void dispatch(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, hip
 
 template <int LOGN>
 void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts, uint32_t L, const DevPrime *targets,
-                  const DevPrime *digits, const KernelProbe *probe, hipStream_t st) {
+                  const DevPrime *digits, bool lazy_out, const KernelProbe *probe, hipStream_t st) {
     constexpr int K1 = LOGN / 2;
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
@@ -430,26 +443,27 @@ void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts,
     ntt_cols_modup_kernel<LOGN, K1, E, CS>
         <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L);
     const LimbGrid lg{cts * L, 1, L + 1, cts * L};
+    const uint32_t row_blocks = ((L + 1) << K1) * ((lg.J + RS - 1) / RS);
     if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
-    ntt_rows_modup_kernel<LOGN, K2, E, RS>
-        <<<((L + 1) << K1) * ((lg.J + RS - 1) / RS), RS << (K2 - E), 0, st>>>(T, targets, lg);
+    if (lazy_out) ntt_rows_modup_kernel<LOGN, K2, E, RS, true><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
+    else ntt_rows_modup_kernel<LOGN, K2, E, RS, false><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
     if (probe) SHC_HIP(hipEventRecord(probe->end, st));
 }
 
 }  // namespace
 
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
-                       const DevPrime *targets, const DevPrime *digits, int logn, hipStream_t st,
+                       const DevPrime *targets, const DevPrime *digits, int logn, bool lazy_out, hipStream_t st,
                        const KernelProbe *probe) {
     if (logn < 13 || logn > 17 || count == 0) return false;
     if ((uint64_t)count * L * (L + 1) > ((uint64_t(1) << 34) >> logn))
         fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
     switch (logn) {
-        case 13: launch_modup<13>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
-        case 14: launch_modup<14>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
-        case 15: launch_modup<15>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
-        case 16: launch_modup<16>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
-        case 17: launch_modup<17>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 13: launch_modup<13>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
+        case 14: launch_modup<14>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
+        case 15: launch_modup<15>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
+        case 16: launch_modup<16>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
+        case 17: launch_modup<17>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
     }
     launch_check("ntt modup launch");
     return true;

@@ -292,7 +292,14 @@ void modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64
                 hipStream_t st, const KernelProbe *probe) {
     const DevPrime *targets = c.keyswitch_primes(L);
     const int logn = c.logn;
-    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, st, probe)) return;
+    // The consumers (inner products with the key, hoisted rotation products) add L
+    // products T * key (key < q) and one more term < q into 128 bits before reducing,
+    // so T may stay in [0, 4q) when (4L + 1) q^2 < 2^128 for every target prime.
+    unsigned __int128 qmax = 0;
+    for (uint32_t i = 0; i < L; i++) qmax = c.moduli[i] > qmax ? c.moduli[i] : qmax;
+    qmax = c.moduli[c.key_count() - 1] > qmax ? c.moduli[c.key_count() - 1] : qmax;
+    const bool lazy = qmax * qmax <= ~(unsigned __int128)0 / (4 * (unsigned __int128)L + 1);
+    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, lazy, st, probe)) return;
     const size_t twords = ((size_t)count * L * (L + 1)) << logn;
     modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(digits, dstride, T, targets, c.primes(), count, L, logn,
                                                            twords);
