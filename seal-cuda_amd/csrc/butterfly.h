@@ -4,31 +4,43 @@
 
 namespace shc {
 
-// Forward (Cooley-Tukey, x + w y, x - w y): x, y in [0, 8q) -> [0, 8q).
+// v >= c ? v - c : v, for c < 2^63 and v < c + 2^63: v - c wraps above 2^63 exactly
+// when v < c, so the sign of the high word selects.
+__host__ __device__ __forceinline__ uint64_t sub_if_ge(uint64_t v, uint64_t c) {
+    const uint64_t t = v - c;
+    return (int32_t)(t >> 32) < 0 ? v : t;
+}
+
+// Forward (Cooley-Tukey, x + w y, x - w y): u = x, less red_q when red and x >= red_q;
+// outputs u + v and u + 4q - v with the twiddle product v = shoup_lazier(y) in [0, 4q)
+// (y may be any 64-bit word).  Schedules:
+//   standard, q < 2^61: red with red_q = 4q on every layer, [0, 8q) -> [0, 8q);
+//   wide, q < 2^60: red with red_q = 8q takes x < 16q to outputs < 12q, and no
+//     reduction takes x < 12q to outputs < 16q (16q <= 2^64).
 // Inverse (Gentleman-Sande, x + y, (x - y) w): x, y in [0, 4q) -> [0, 4q).
-// The twiddle product is shoup_lazier's, in [0, 4q); 8q fits 64 bits for q < 2^61.
 // Same congruences as the reference's Harvey butterflies (src/utils/dwthandler.h:88-372),
 // with wider lazy ranges.
 template <bool INV>
 __host__ __device__ __forceinline__ void butterfly(uint64_t &x, uint64_t &y, ulonglong2 w, uint64_t four_q,
-                                                   uint64_t neg_q) {
-    // v >= 4q ? v - 4q : v for v < 8q: with 4q < 2^63, v - 4q wraps above 2^63 exactly
-    // when v < 4q, so the sign of the high word selects
-    auto sub_if_ge = [four_q](uint64_t v) {
-        const uint64_t t = v - four_q;
-        return (int32_t)(t >> 32) < 0 ? v : t;
-    };
+                                                   uint64_t neg_q, bool red, uint64_t red_q) {
     if (!INV) {
-        const uint64_t u = sub_if_ge(x);
+        const uint64_t u = red ? sub_if_ge(x, red_q) : x;
         const uint64_t v = shoup_lazier(y, w.x, w.y, neg_q);
         x = u + v;
         y = u + four_q - v;
     } else {
         const uint64_t s = x + y;
         const uint64_t d = x + four_q - y;
-        x = sub_if_ge(s);
+        x = sub_if_ge(s, four_q);
         y = shoup_lazier(d, w.x, w.y, neg_q);
     }
+}
+
+// The standard schedule.
+template <bool INV>
+__host__ __device__ __forceinline__ void butterfly(uint64_t &x, uint64_t &y, ulonglong2 w, uint64_t four_q,
+                                                   uint64_t neg_q) {
+    butterfly<INV>(x, y, w, four_q, neg_q, true, four_q);
 }
 
 }  // namespace shc

@@ -16,6 +16,11 @@
 //     that may be low by up to 3 (9 32-bit multiplies instead of 10), which those
 //     wider ranges absorb.  The full reduction and the N^-1 scaling are folded into
 //     the store of the last pass.
+//   * The key switch's ModUp transforms take the wide schedule when every target
+//     prime is below 2^60: values may grow to 16q inside a pass (local layers 0, 1
+//     and odd layers skip the conditional subtraction, even layers >= 2 subtract 8q)
+//     and one subtraction of 8q at the end of the pass brings them back below 8q, so
+//     passes still hand each other [0, 8q).
 //
 // Twiddle indexing: forward layer l (m = 2^l groups) uses root_powers[m + group];
 // inverse layer l uses inv_root_powers[N - 2m + 1 + group], group = x >> (logn - l).
@@ -46,8 +51,9 @@ __device__ __forceinline__ int phase_index(int t, int j) {
 }
 
 // One phase on the thread's registers.  g = index of this sub-transform among the
-// 2^L0 sub-transforms the full transform splits into at global layer L0.
-template <int LOGN, int K, int E, int S0, int EP, bool INV>
+// 2^L0 sub-transforms the full transform splits into at global layer L0.  W selects
+// the wide forward schedule (q < 2^60).
+template <int LOGN, int K, int E, int S0, int EP, bool INV, bool W>
 __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int g, int L0,
                                               const ulonglong2 *__restrict__ tw, uint64_t four_q, uint64_t neg_q) {
 #pragma unroll
@@ -57,11 +63,15 @@ __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int 
 #pragma unroll
         for (int j = 0; j < (1 << E); j++) {
             if (j & (1 << b)) continue;
-            const int idx = phase_index<K, E, S0, EP>(t, j);
+            // in the first phase a register's group does not depend on the thread (its
+            // index bits above K - s all come from j): with t = 0 it is a constant, so
+            // the twiddle address is uniform and the load scalar
+            const int idx = phase_index<K, E, S0, EP>(S0 == 0 ? 0 : t, j);
             const int grp = (g << s) + (idx >> (K - s));
             const int l = L0 + s;
             const ulonglong2 w = load_tw(tw, INV ? (1 << LOGN) - (2 << l) + 1 + grp : (1 << l) + grp);
-            butterfly<INV>(r[j], r[j | (1 << b)], w, four_q, neg_q);
+            const bool red = !W || (s >= 2 && !(s & 1));
+            butterfly<INV>(r[j], r[j | (1 << b)], w, four_q, neg_q, red, W ? four_q << 1 : four_q);
         }
     }
 }
@@ -70,7 +80,7 @@ __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int 
 // from global memory through `load(idx)` and the last writes them back through
 // `store(idx, value)`; in between, values cross threads through LDS at `addr(idx)`,
 // so a pass makes one LDS round trip per phase boundary and one barrier each.
-template <int LOGN, int K, int E, bool INV, int P = 0, class Addr, class Load, class Store>
+template <int LOGN, int K, int E, bool INV, bool W, int P = 0, class Addr, class Load, class Store>
 __device__ __forceinline__ void run_phases(uint64_t *tile, const Addr &addr, const Load &load, const Store &store,
                                            int t, int g, int L0, const ulonglong2 *__restrict__ tw,
                                            uint64_t four_q, uint64_t neg_q, bool active) {
@@ -87,17 +97,17 @@ __device__ __forceinline__ void run_phases(uint64_t *tile, const Addr &addr, con
                 const int idx = phase_index<K, E, S0, EP>(t, j);
                 r[j] = P == 0 ? load(idx) : tile[addr(idx)];
             }
-            phase_compute<LOGN, K, E, S0, EP, INV>(r, t, g, L0, tw, four_q, neg_q);
+            phase_compute<LOGN, K, E, S0, EP, INV, W>(r, t, g, L0, tw, four_q, neg_q);
 #pragma unroll
             for (int j = 0; j < (1 << E); j++) {
                 const int idx = phase_index<K, E, S0, EP>(t, j);
-                if (P == NP - 1) store(idx, r[j]);
+                if (P == NP - 1) store(idx, W ? sub_if_ge(r[j], four_q << 1) : r[j]);  // wide: [0,16q) -> [0,8q)
                 else tile[addr(idx)] = r[j];
             }
         }
         if constexpr (P + 1 < NP) {
             __syncthreads();
-            run_phases<LOGN, K, E, INV, P + 1>(tile, addr, load, store, t, g, L0, tw, four_q, neg_q, active);
+            run_phases<LOGN, K, E, INV, W, P + 1>(tile, addr, load, store, t, g, L0, tw, four_q, neg_q, active);
         }
     }
 }
@@ -128,7 +138,7 @@ __device__ __forceinline__ uint64_t finish(uint64_t x, const DevPrime &P) {
 // of SUBS limbs that share prime i -- limbs (i, j) for SUBS consecutive j -- so every
 // row in the block uses the same twiddles (broadcast loads, L1/L2 hits).  Blocks run
 // with the limb group fastest, so consecutive blocks also share them.
-template <int LOGN, int K, int E, int SUBS, bool INV, class Load, class Store>
+template <int LOGN, int K, int E, int SUBS, bool INV, bool W = false, class Load, class Store>
 __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, uint32_t I, uint32_t j0, uint32_t jn,
                                           const Load &load_elem, const Store &store_elem) {
     constexpr int T = 1 << (K - E);
@@ -154,12 +164,12 @@ __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, u
     auto load = [&](int idx) { return load_elem(i, j, row_off + idx); };
     if constexpr (NP == 1) {
         auto store = [&](int idx, uint64_t x) { store_elem(i, j, row_off + idx, x); };
-        run_phases<LOGN, K, E, INV>(tile, addr, load, store, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
+        run_phases<LOGN, K, E, INV, W>(tile, addr, load, store, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
     } else {
         // the last phase leaves 16 consecutive values per thread; stage them in LDS so
         // that the block writes whole rows with consecutive lanes on consecutive words
         auto stage = [&](int idx, uint64_t x) { tile[addr(idx)] = x; };
-        run_phases<LOGN, K, E, INV>(tile, addr, load, stage, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
+        run_phases<LOGN, K, E, INV, W>(tile, addr, load, stage, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
         __syncthreads();
         if (nsub == SUBS) {
             // a full block: a fixed number of elements per thread, unrolled so that every
@@ -201,7 +211,7 @@ __device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, u
     auto addr = [col](int idx) { return idx * SUBS + col; };
     auto load = [&](int idx) { return load_elem(i, j, (size_t)idx * C + c); };
     auto store = [&](int idx, uint64_t x) { store_elem(i, j, (size_t)idx * C + c, x); };
-    run_phases<LOGN, K, E, INV>(tile, addr, load, store, t, 0, 0, tw, P.q << 2, 0 - P.q, true);
+    run_phases<LOGN, K, E, INV, false>(tile, addr, load, store, t, 0, 0, tw, P.q << 2, 0 - P.q, true);
 }
 
 // Plain in-place passes over a LimbGrid buffer.
@@ -228,13 +238,14 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__r
 
 // The key switch's ModUp rows pass: the same code as a kernel of its own, so that
 // profiles tell it apart from the other forward transforms.  LAZY leaves outputs in
-// [0, 4q) for consumers whose 128-bit sums absorb that range.
-template <int LOGN, int K, int E, int SUBS, bool LAZY>
+// [0, 4q) for consumers whose 128-bit sums absorb that range; W: the wide schedule
+// (every prime below 2^60).
+template <int LOGN, int K, int E, int SUBS, bool LAZY, bool W>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_modup_kernel(uint64_t *__restrict__ data,
                                                                         const DevPrime *__restrict__ primes,
                                                                         LimbGrid lg) {
     const GridAccess g{data, lg, LOGN};
-    rows_body<LOGN, K, E, SUBS, false>(
+    rows_body<LOGN, K, E, SUBS, false, W>(
         primes, lg.I, 0, lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { *g.at(i, j, off) = finish<false, !LAZY>(x, primes[i]); });
 }
@@ -257,7 +268,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
 // butterflies take inputs < 8q), then the first K layers.  Blocks run in (ct, j, i)
 // order, i fastest, so the L+1 blocks that read one tile of a digit run together and
 // on one XCD (block % 8 is the tile's % 8), and the digit is read from HBM once.
-template <int LOGN, int K, int E, int SUBS>
+template <int LOGN, int K, int E, int SUBS, bool W>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const uint64_t *__restrict__ d,
                                                                         size_t d_ct_stride,
                                                                         uint64_t *__restrict__ T,
@@ -286,7 +297,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
         return reduce ? barrett64(x, P.q, P.ratio_hi) : x;
     };
     auto store = [out](int idx, uint64_t x) { out[(size_t)idx * C] = x; };
-    run_phases<LOGN, K, E, false>(tile, addr, load, store, t, 0, 0, P.fwd, P.q << 2, 0 - P.q, true);
+    run_phases<LOGN, K, E, false, W>(tile, addr, load, store, t, 0, 0, P.fwd, P.q << 2, 0 - P.q, true);
 }
 
 // ---- ModDown by the special prime p, fused into the transforms (switch_key step 5):
@@ -431,7 +442,7 @@ void dispatch(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, hip
     }
 }
 
-template <int LOGN>
+template <int LOGN, bool W>
 void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts, uint32_t L, const DevPrime *targets,
                   const DevPrime *digits, bool lazy_out, const KernelProbe *probe, hipStream_t st) {
     constexpr int K1 = LOGN / 2;
@@ -440,30 +451,45 @@ void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts,
     constexpr int CS = 16;
     constexpr int RS = 4096 >> K2;
     const uint32_t limbs = cts * L * (L + 1);
-    ntt_cols_modup_kernel<LOGN, K1, E, CS>
+    ntt_cols_modup_kernel<LOGN, K1, E, CS, W>
         <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L);
     const LimbGrid lg{cts * L, 1, L + 1, cts * L};
     const uint32_t row_blocks = ((L + 1) << K1) * ((lg.J + RS - 1) / RS);
     if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
-    if (lazy_out) ntt_rows_modup_kernel<LOGN, K2, E, RS, true><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
-    else ntt_rows_modup_kernel<LOGN, K2, E, RS, false><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
+    if (lazy_out) ntt_rows_modup_kernel<LOGN, K2, E, RS, true, W><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
+    else ntt_rows_modup_kernel<LOGN, K2, E, RS, false, W><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
     if (probe) SHC_HIP(hipEventRecord(probe->end, st));
 }
 
 }  // namespace
 
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
-                       const DevPrime *targets, const DevPrime *digits, int logn, bool lazy_out, hipStream_t st,
-                       const KernelProbe *probe) {
+                       const DevPrime *targets, const DevPrime *digits, int logn, bool lazy_out, bool wide,
+                       hipStream_t st, const KernelProbe *probe) {
     if (logn < 13 || logn > 17 || count == 0) return false;
     if ((uint64_t)count * L * (L + 1) > ((uint64_t(1) << 34) >> logn))
         fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
     switch (logn) {
-        case 13: launch_modup<13>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
-        case 14: launch_modup<14>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
-        case 15: launch_modup<15>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
-        case 16: launch_modup<16>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
-        case 17: launch_modup<17>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st); break;
+        case 13:
+            if (wide) launch_modup<13, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            else launch_modup<13, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            break;
+        case 14:
+            if (wide) launch_modup<14, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            else launch_modup<14, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            break;
+        case 15:
+            if (wide) launch_modup<15, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            else launch_modup<15, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            break;
+        case 16:
+            if (wide) launch_modup<16, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            else launch_modup<16, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            break;
+        case 17:
+            if (wide) launch_modup<17, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            else launch_modup<17, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
+            break;
     }
     launch_check("ntt modup launch");
     return true;

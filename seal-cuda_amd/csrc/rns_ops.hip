@@ -299,7 +299,8 @@ void modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64
     for (uint32_t i = 0; i < L; i++) qmax = c.moduli[i] > qmax ? c.moduli[i] : qmax;
     qmax = c.moduli[c.key_count() - 1] > qmax ? c.moduli[c.key_count() - 1] : qmax;
     const bool lazy = qmax * qmax <= ~(unsigned __int128)0 / (4 * (unsigned __int128)L + 1);
-    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, lazy, st, probe)) return;
+    const bool wide = qmax < ((unsigned __int128)1 << 60);
+    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, lazy, wide, st, probe)) return;
     const size_t twords = ((size_t)count * L * (L + 1)) << logn;
     modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(digits, dstride, T, targets, c.primes(), count, L, logn,
                                                            twords);

@@ -3,6 +3,9 @@
 //   * shoup_lazier(x, w, wq, 2^64 - q) is congruent to x w mod q and below 4q;
 //   * the forward butterfly maps [0, 8q)^2 into [0, 8q)^2 with x + w y, x - w y;
 //   * the inverse butterfly maps [0, 4q)^2 into [0, 4q)^2 with x + y, (x - y) w;
+//   * for q < 2^60, the wide forward schedule: reducing by 8q maps x in [0, 16q) to
+//     outputs in [0, 12q), no reduction maps x in [0, 12q) to outputs in [0, 16q), for
+//     any 64-bit y; sub_if_ge(v, 8q) takes [0, 16q) to [0, 8q);
 // for primes from 17 to 61 bits, random and extreme operands (0, range ends,
 // multiples of q +- 1, all-ones words).  Exact products come from unsigned __int128.
 // Prints "ok <cases>" and exits 0, or prints the first failing case and exits 1.
@@ -16,6 +19,7 @@ using wide = unsigned __int128;
 using shc::butterfly;
 using shc::mulhi64_approx;
 using shc::shoup_lazier;
+using shc::sub_if_ge;
 
 static u64 state = 0x9e3779b97f4a7c15ull;
 static u64 next() {  // splitmix64
@@ -113,6 +117,32 @@ int main() {
                 if (ix >= 4 * q || iy >= 4 * q || ix % q != (ax + ay) % q || iy % q != mulmod((ax + q - ay) % q, w, q)) {
                     printf("inverse butterfly q=%llu w=%llu case %d\n", (unsigned long long)q, (unsigned long long)w, k);
                     return 1;
+                }
+                if (q < (u64(1) << 60)) {
+                    // wide schedule, reducing layer: x < 16q, y any word
+                    u64 rx = operand(16 * q, q, k % 8), ry = k % 3 == 0 ? ~0ull - k : next();
+                    const u64 rsx = rx % q, rwy = mulmod(ry % q, w, q);
+                    butterfly<false>(rx, ry, ulonglong2{w, wq}, four_q, neg_q, true, 8 * q);
+                    if (rx >= 12 * q || ry >= 12 * q || rx % q != (rsx + rwy) % q || ry % q != (rsx + q - rwy) % q) {
+                        printf("wide reducing butterfly q=%llu w=%llu case %d\n", (unsigned long long)q,
+                               (unsigned long long)w, k);
+                        return 1;
+                    }
+                    // wide schedule, layer without reduction: x < 12q
+                    u64 nx = operand(12 * q, q, k % 8), ny = operand(16 * q, q, (k / 8) % 8);
+                    const u64 nsx = nx % q, nwy = mulmod(ny % q, w, q);
+                    butterfly<false>(nx, ny, ulonglong2{w, wq}, four_q, neg_q, false, 8 * q);
+                    if (nx >= 16 * q || ny >= 16 * q || nx % q != (nsx + nwy) % q || ny % q != (nsx + q - nwy) % q) {
+                        printf("wide butterfly q=%llu w=%llu case %d\n", (unsigned long long)q, (unsigned long long)w, k);
+                        return 1;
+                    }
+                    // end of a wide pass
+                    const u64 v = operand(16 * q, q, k % 8), e = sub_if_ge(v, 8 * q);
+                    if (e >= 8 * q || e % q != v % q) {
+                        printf("sub_if_ge q=%llu v=%llu\n", (unsigned long long)q, (unsigned long long)v);
+                        return 1;
+                    }
+                    cases += 3;
                 }
                 cases += 4;
             }

@@ -3,8 +3,9 @@
 Builds tests/host/butterfly_check.cpp with hipcc (host code only; no GPU needed)
 against the same header the gfx950 kernels include, and runs it: congruences against
 exact 128-bit arithmetic and the lazy ranges ([0, 8q) forward, [0, 4q) inverse,
-[0, 4q) for the twiddle product) for primes of 17 to 61 bits with random and extreme
-operands.
+[0, 4q) for the twiddle product, and for q < 2^60 the wide forward schedule's
+[0, 16q) -> [0, 12q) and [0, 12q) -> [0, 16q)) for primes of 17 to 61 bits with
+random and extreme operands.
 """
 import os
 import shutil
