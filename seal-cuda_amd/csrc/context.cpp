@@ -48,6 +48,8 @@ static DevBuf upload(const std::vector<T> &v) {
 
 std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector<uint64_t> &moduli, uint64_t plain) {
     if (logn < 1 || logn > 17) fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree must be a power of two in [2, 131072]");
+    // At most 64 primes below 2^60 (the reference's bounds, src/utils/defines.h:19-27):
+    // the key switch relies on them (16q fits a word in its ModUp transforms; 63 products below 4q^2 fit 128 bits).
     if (moduli.empty() || moduli.size() > 64) fail(ERR_INVALID_ARGUMENT, "coeff_modulus size is invalid");
     for (size_t i = 0; i < moduli.size(); i++) {
         if (moduli[i] >> 60 || moduli[i] < 2) fail(ERR_INVALID_ARGUMENT, "coeff_modulus bit count is invalid");

@@ -35,14 +35,13 @@ void ntt_inverse_grid(uint64_t *data, LimbGrid grid, const DevPrime *primes, int
 
 // Key-switching ModUp fused with the forward NTT: T[i][ct][j] = NTT_i(d[ct][j] mod t_i)
 // for digits j < L (primes digits[j]) and targets i <= L (targets[i]), d[ct] at
-// d + ct * d_ct_stride with L limbs.  Output in [0, 4 t_i) when lazy_out, else fully
-// reduced.  wide: every target prime is below 2^60 (the transforms may then let values
-// grow to 16 t_i inside a pass).  Returns false (and does nothing) for N < 2^13, where
-// the caller gathers first and runs the mapped transform.
+// d + ct * d_ct_stride with L limbs.  Target primes must be below 2^60; output in
+// [0, 4 t_i).  Returns false (and does nothing) for N < 2^13, where the caller gathers
+// first and runs the mapped transform.
 // With a probe, its events are recorded right before and after the rows-pass launch.
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
-                       const DevPrime *targets, const DevPrime *digits, int logn, bool lazy_out, bool wide,
-                       hipStream_t stream, const KernelProbe *probe = nullptr);
+                       const DevPrime *targets, const DevPrime *digits, int logn, hipStream_t stream,
+                       const KernelProbe *probe = nullptr);
 
 // ModDown by the special prime p fused into the last transforms of a key switch, for
 // `count` ciphertexts at L limbs (N >= 2^13; returns false and does nothing otherwise):

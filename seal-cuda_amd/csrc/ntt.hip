@@ -16,11 +16,11 @@
 //     that may be low by up to 3 (9 32-bit multiplies instead of 10), which those
 //     wider ranges absorb.  The full reduction and the N^-1 scaling are folded into
 //     the store of the last pass.
-//   * The key switch's ModUp transforms take the wide schedule when every target
-//     prime is below 2^60: values may grow to 16q inside a pass (local layers 0, 1
-//     and odd layers skip the conditional subtraction, even layers >= 2 subtract 8q)
-//     and one subtraction of 8q at the end of the pass brings them back below 8q, so
-//     passes still hand each other [0, 8q).
+//   * The key switch's ModUp transforms take the wide schedule, which needs q < 2^60
+//     (contexts hold only such primes): values may grow to 16q inside a pass (local
+//     layers 0, 1 and odd layers skip the conditional subtraction, even layers >= 2
+//     subtract 8q) and one subtraction of 8q at the end of the pass brings them back
+//     below 8q, so passes still hand each other [0, 8q).
 //
 // Twiddle indexing: forward layer l (m = 2^l groups) uses root_powers[m + group];
 // inverse layer l uses inv_root_powers[N - 2m + 1 + group], group = x >> (logn - l).
@@ -237,17 +237,16 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__r
 }
 
 // The key switch's ModUp rows pass: the same code as a kernel of its own, so that
-// profiles tell it apart from the other forward transforms.  LAZY leaves outputs in
-// [0, 4q) for consumers whose 128-bit sums absorb that range; W: the wide schedule
-// (every prime below 2^60).
-template <int LOGN, int K, int E, int SUBS, bool LAZY, bool W>
+// profiles tell it apart from the other forward transforms.  Wide schedule; outputs
+// left in [0, 4q) (the key products' 128-bit sums absorb that range).
+template <int LOGN, int K, int E, int SUBS>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_modup_kernel(uint64_t *__restrict__ data,
                                                                         const DevPrime *__restrict__ primes,
                                                                         LimbGrid lg) {
     const GridAccess g{data, lg, LOGN};
-    rows_body<LOGN, K, E, SUBS, false, W>(
+    rows_body<LOGN, K, E, SUBS, false, true>(
         primes, lg.I, 0, lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
-        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { *g.at(i, j, off) = finish<false, !LAZY>(x, primes[i]); });
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { *g.at(i, j, off) = finish<false, false>(x, primes[i]); });
 }
 
 template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
@@ -268,7 +267,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
 // butterflies take inputs < 8q), then the first K layers.  Blocks run in (ct, j, i)
 // order, i fastest, so the L+1 blocks that read one tile of a digit run together and
 // on one XCD (block % 8 is the tile's % 8), and the digit is read from HBM once.
-template <int LOGN, int K, int E, int SUBS, bool W>
+template <int LOGN, int K, int E, int SUBS>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const uint64_t *__restrict__ d,
                                                                         size_t d_ct_stride,
                                                                         uint64_t *__restrict__ T,
@@ -297,7 +296,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
         return reduce ? barrett64(x, P.q, P.ratio_hi) : x;
     };
     auto store = [out](int idx, uint64_t x) { out[(size_t)idx * C] = x; };
-    run_phases<LOGN, K, E, false, W>(tile, addr, load, store, t, 0, 0, P.fwd, P.q << 2, 0 - P.q, true);
+    run_phases<LOGN, K, E, false, true>(tile, addr, load, store, t, 0, 0, P.fwd, P.q << 2, 0 - P.q, true);
 }
 
 // ---- ModDown by the special prime p, fused into the transforms (switch_key step 5):
@@ -442,54 +441,38 @@ void dispatch(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, hip
     }
 }
 
-template <int LOGN, bool W>
+template <int LOGN>
 void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts, uint32_t L, const DevPrime *targets,
-                  const DevPrime *digits, bool lazy_out, const KernelProbe *probe, hipStream_t st) {
+                  const DevPrime *digits, const KernelProbe *probe, hipStream_t st) {
     constexpr int K1 = LOGN / 2;
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
     constexpr int CS = 16;
     constexpr int RS = 4096 >> K2;
     const uint32_t limbs = cts * L * (L + 1);
-    ntt_cols_modup_kernel<LOGN, K1, E, CS, W>
+    ntt_cols_modup_kernel<LOGN, K1, E, CS>
         <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L);
     const LimbGrid lg{cts * L, 1, L + 1, cts * L};
     const uint32_t row_blocks = ((L + 1) << K1) * ((lg.J + RS - 1) / RS);
     if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
-    if (lazy_out) ntt_rows_modup_kernel<LOGN, K2, E, RS, true, W><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
-    else ntt_rows_modup_kernel<LOGN, K2, E, RS, false, W><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
+    ntt_rows_modup_kernel<LOGN, K2, E, RS><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
     if (probe) SHC_HIP(hipEventRecord(probe->end, st));
 }
 
 }  // namespace
 
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
-                       const DevPrime *targets, const DevPrime *digits, int logn, bool lazy_out, bool wide,
-                       hipStream_t st, const KernelProbe *probe) {
+                       const DevPrime *targets, const DevPrime *digits, int logn, hipStream_t st,
+                       const KernelProbe *probe) {
     if (logn < 13 || logn > 17 || count == 0) return false;
     if ((uint64_t)count * L * (L + 1) > ((uint64_t(1) << 34) >> logn))
         fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
     switch (logn) {
-        case 13:
-            if (wide) launch_modup<13, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            else launch_modup<13, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            break;
-        case 14:
-            if (wide) launch_modup<14, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            else launch_modup<14, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            break;
-        case 15:
-            if (wide) launch_modup<15, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            else launch_modup<15, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            break;
-        case 16:
-            if (wide) launch_modup<16, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            else launch_modup<16, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            break;
-        case 17:
-            if (wide) launch_modup<17, true>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            else launch_modup<17, false>(d, d_ct_stride, T, count, L, targets, digits, lazy_out, probe, st);
-            break;
+        case 13: launch_modup<13>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 14: launch_modup<14>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 15: launch_modup<15>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 16: launch_modup<16>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 17: launch_modup<17>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
     }
     launch_check("ntt modup launch");
     return true;

@@ -33,8 +33,8 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
 // Stages of a key switch at level L over `count` ciphertexts, for callers that compose
 // them differently (hoisted rotations):
 //   modup_to_T: T[i][ct][j] = NTT_i(d_j mod q_i), digits d (coefficient form) at
-//               digits + ct * dstride + j * N; targets i <= L (special last).  Values
-//               may be left in [0, 4 q_i) when the consumers' 128-bit sums allow it.
+//               digits + ct * dstride + j * N; targets i <= L (special last); values
+//               in [0, 4 q_i).
 //   keyswitch_inner_product: P[ct][k][i] = sum_j T[i][ct][j] key[j][k][ki] (i < L),
 //               S[ct][k] the special target's.
 //   moddown_into: ct_k[i] += ModDown(P_k, S_k)[i] (S is consumed; U is scratch

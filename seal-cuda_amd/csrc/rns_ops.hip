@@ -14,6 +14,7 @@
 //      one Barrett reduction at the end; G ciphertexts per thread share each key load
 //   5. ModDown: inverse NTT of the special limbs, rounding correction, forward NTT
 //      (CKKS) or inverse NTT of P (BFV), then ct_k[i] += (P_k[i] - t) * p^-1.
+#include "accum.h"
 #include "modarith.h"
 #include "ntt.h"
 #include "rns_ops.h"
@@ -71,7 +72,10 @@ __global__ void modup_gather_kernel(const uint64_t *__restrict__ d, size_t d_str
 // P[ct][k][i][c] = sum_j T[i][ct][j][c] * key[j][k][ki][c] mod q_i for i < L, and the
 // special target (i == L) goes to S[ct][k][c].  A block owns a tile of 256 flat (i, c)
 // positions for one group of G ciphertexts; a thread owns one position and keeps
-// 128-bit sums for its G ciphertexts, so each key word it loads serves G products.
+// three-word sums (Acc3) for its G ciphertexts, so each key word it loads serves G
+// products.  The loads of digit j + 2 are issued before the products of digit j (past
+// the end, the last digit again).  In a partial last group the missing ciphertexts
+// repeat the last one (loads stay in bounds; their sums are not stored).
 // Blocks are dealt to the 8 XCDs round-robin (XCD = block % 8); the mapping below puts
 // every group of one tile on the same XCD, one after another, so the tile's key words
 // come from HBM once and then from that XCD's L2.
@@ -98,27 +102,48 @@ __global__ __launch_bounds__(TPB) void inner_product_kernel(const uint64_t *__re
     const uint64_t *Ti = T + ((((size_t)i * count + ct0) * L) << logn) + c;
     const uint64_t *kp = key + (size_t)ki * n + c;
     const size_t kstride = (size_t)K << logn;  // key[j][k] -> key[j][k+1]
-    u128 acc0[G], acc1[G];
+    const uint32_t one = opaque_one();
+    size_t goff[G];
 #pragma unroll
-    for (int g = 0; g < G; g++) acc0[g] = acc1[g] = u128{0, 0};
-    for (uint32_t j = 0; j < L; j++) {
-        const uint64_t k0 = kp[(2 * j) * kstride];
-        const uint64_t k1 = kp[(2 * j + 1) * kstride];
+    for (int g = 0; g < G; g++) goff[g] = ((size_t)min(g, g_valid - 1) * L) << logn;
+    Acc3 acc0[G], acc1[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) acc0[g] = acc1[g] = Acc3{0, 0, 0};
+    // two digits in flight: buffer b holds digit j while the loads of digit j + 2 go out
+    uint64_t tb[2][G], kb[2][2];
+    auto fetch = [&](int b, uint32_t j) {
+        j = j < L ? j : L - 1;  // past the end: the last digit again (in bounds, unused)
+        kb[b][0] = kp[(2 * j) * kstride];
+        kb[b][1] = kp[(2 * j + 1) * kstride];
+#pragma unroll
+        for (int g = 0; g < G; g++) tb[b][g] = Ti[goff[g] + ((size_t)j << logn)];
+    };
+    auto step = [&](int b, uint32_t jnext) {
+        uint64_t t[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) t[g] = tb[b][g];
+        const uint64_t k0 = kb[b][0], k1 = kb[b][1];
+        fetch(b, jnext);
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            if (g < g_valid) {
-                const uint64_t t = Ti[((size_t)g * L + j) << logn];
-                add128(acc0[g], mul64x64(t, k0));
-                add128(acc1[g], mul64x64(t, k1));
-            }
+            mac(acc0[g], t[g], k0, one);
+            mac(acc1[g], t[g], k1, one);
         }
+    };
+    fetch(0, 0);
+    fetch(1, 1);
+    uint32_t j = 0;
+    for (; j + 2 <= L; j += 2) {
+        step(0, j + 2);
+        step(1, j + 3);
     }
+    if (j < L) step(0, L);
 #pragma unroll
     for (int g = 0; g < G; g++) {
         if (g < g_valid) {
             const size_t ct = ct0 + g;
-            const uint64_t r0 = barrett128(acc0[g], Q.q, Q.ratio_lo, Q.ratio_hi);
-            const uint64_t r1 = barrett128(acc1[g], Q.q, Q.ratio_lo, Q.ratio_hi);
+            const uint64_t r0 = acc_reduce(acc0[g], Q);
+            const uint64_t r1 = acc_reduce(acc1[g], Q);
             if (i == L) {
                 S[(ct * 2 + 0) * n + c] = r0;
                 S[(ct * 2 + 1) * n + c] = r1;
@@ -292,15 +317,9 @@ void modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64
                 hipStream_t st, const KernelProbe *probe) {
     const DevPrime *targets = c.keyswitch_primes(L);
     const int logn = c.logn;
-    // The consumers (inner products with the key, hoisted rotation products) add L
-    // products T * key (key < q) and one more term < q into 128 bits before reducing,
-    // so T may stay in [0, 4q) when (4L + 1) q^2 < 2^128 for every target prime.
-    unsigned __int128 qmax = 0;
-    for (uint32_t i = 0; i < L; i++) qmax = c.moduli[i] > qmax ? c.moduli[i] : qmax;
-    qmax = c.moduli[c.key_count() - 1] > qmax ? c.moduli[c.key_count() - 1] : qmax;
-    const bool lazy = qmax * qmax <= ~(unsigned __int128)0 / (4 * (unsigned __int128)L + 1);
-    const bool wide = qmax < ((unsigned __int128)1 << 60);
-    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, lazy, wide, st, probe)) return;
+    // T is left in [0, 4q): its consumers (inner products with the key, hoisted rotation
+    // products) sum at most 63 products below 4q * q < 2^122 in 128 bits.
+    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, st, probe)) return;
     const size_t twords = ((size_t)count * L * (L + 1)) << logn;
     modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(digits, dstride, T, targets, c.primes(), count, L, logn,
                                                            twords);

@@ -55,6 +55,8 @@ def test_ckks_multiply(logn, bits, L, count):
     (BFV, 10, [40, 40, 50], 2, 2),
     (BFV, 13, [43, 43, 44, 44, 44], 4, 2),
     (CKKS, 15, [60] + [40] * 14 + [60], 15, 1),
+    # 60-bit primes at 20 digits, and 5 ciphertexts: a partial last group of 4
+    (CKKS, 13, [60] * 21, 20, 5),
 ])
 def test_relinearize(scheme, logn, bits, L, count):
     ctx, moduli = make_ctx(scheme, logn, bits, 65537 if scheme == BFV else 0)
