@@ -75,10 +75,13 @@ __global__ void correction_kernel(const uint64_t *__restrict__ negntt, const uin
 
 // Products of R rotations at once, each T word read once for all of them:
 //   P_r[ct][k][i] = sum_j T[i][ct][j] key'_r[j][k][ki] + corr'_r[k][i]   (S_r for i == L)
-// One thread per (i, c) position and ciphertext, with three-word sums (Acc3); the loads
-// of digit j + 1 go out before the products of digit j.  Blocks of one tile run on one XCD (XCD = block % 8) so that the tile's key
-// words are served from its L2.
+// One thread per (i, c) position and RG ciphertexts (each key word it loads serves RG
+// products), with three-word sums (Acc3); the loads of digit j + 1 go out before the
+// products of digit j, and in a partial last group the missing ciphertext repeats the
+// last one (in bounds, not stored).  Blocks of one tile run on one XCD (XCD = block % 8)
+// so that the tile's key words are served from its L2.
 constexpr int RMAX = 4;
+constexpr int RG = 2;
 struct RotBatch {
     const uint64_t *key[RMAX];   // prepared keys; corr' follows at corr_off words
     uint64_t *P[RMAX], *S[RMAX];
@@ -89,10 +92,12 @@ __global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *
                                                                 size_t corr_off, const DevPrime *__restrict__ targets,
                                                                 uint32_t L, uint32_t K, int logn, uint32_t count,
                                                                 uint32_t tiles) {
+    const uint32_t groups = (count + RG - 1) / RG;
     const uint32_t xcd = blockIdx.x & 7, q8 = blockIdx.x >> 3;
-    const uint32_t tile = (q8 / count) * 8 + xcd;
+    const uint32_t tile = (q8 / groups) * 8 + xcd;
     if (tile >= tiles) return;
-    const uint32_t ct = q8 % count;
+    const uint32_t ct0 = (q8 % groups) * RG;
+    const int gv = (int)min<uint32_t>(RG, count - ct0);
     const size_t n = (size_t)1 << logn;
     const size_t flat = (size_t)tile * TPB + threadIdx.x;
     if (flat >= (size_t)(L + 1) << logn) return;
@@ -100,15 +105,21 @@ __global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *
     const size_t c = flat & (n - 1);
     const uint32_t ki = (i == L) ? K - 1 : i;
     const DevPrime &Q = targets[i];
-    const uint64_t *Ti = T + ((((size_t)i * count + ct) * L) << logn) + c;
+    const uint64_t *Ti = T + ((((size_t)i * count + ct0) * L) << logn) + c;
     const size_t kstride = (size_t)K << logn;
     const uint32_t one = opaque_one();
-    Acc3 a0[R], a1[R];
+    size_t goff[RG];
 #pragma unroll
-    for (int r = 0; r < R; r++) a0[r] = a1[r] = Acc3{0, 0, 0};
-    uint64_t tn, k0n[R], k1n[R];
+    for (int g = 0; g < RG; g++) goff[g] = ((size_t)min(g, gv - 1) * L) << logn;
+    Acc3 a0[R][RG], a1[R][RG];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int g = 0; g < RG; g++) a0[r][g] = a1[r][g] = Acc3{0, 0, 0};
+    uint64_t tn[RG], k0n[R], k1n[R];
     auto fetch = [&](uint32_t j) {
-        tn = Ti[(size_t)j << logn];
+#pragma unroll
+        for (int g = 0; g < RG; g++) tn[g] = Ti[goff[g] + ((size_t)j << logn)];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const uint64_t *kp = rb.key[r] + (size_t)ki * n + c;
@@ -118,30 +129,38 @@ __global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *
     };
     fetch(0);
     for (uint32_t j = 0; j < L; j++) {
-        const uint64_t t = tn;
-        uint64_t k0[R], k1[R];
+        uint64_t t[RG], k0[R], k1[R];
+#pragma unroll
+        for (int g = 0; g < RG; g++) t[g] = tn[g];
 #pragma unroll
         for (int r = 0; r < R; r++) k0[r] = k0n[r], k1[r] = k1n[r];
         fetch(j + 1 < L ? j + 1 : j);
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            mac(a0[r], t, k0[r], one);
-            mac(a1[r], t, k1[r], one);
-        }
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int g = 0; g < RG; g++) {
+                mac(a0[r][g], t[g], k0[r], one);
+                mac(a1[r][g], t[g], k1[r], one);
+            }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const uint64_t *corr = rb.key[r] + corr_off;
-        acc_add(a0[r], corr[(size_t)i * n + c], one);
-        acc_add(a1[r], corr[((size_t)(L + 1) + i) * n + c], one);
-        const uint64_t r0 = acc_reduce(a0[r], Q);
-        const uint64_t r1 = acc_reduce(a1[r], Q);
-        if (i == L) {
-            rb.S[r][(ct * 2 + 0) * n + c] = r0;
-            rb.S[r][(ct * 2 + 1) * n + c] = r1;
-        } else {
-            rb.P[r][((ct * 2 + 0) * L + i) * n + c] = r0;
-            rb.P[r][((ct * 2 + 1) * L + i) * n + c] = r1;
+        const uint64_t c0 = corr[(size_t)i * n + c], c1 = corr[((size_t)(L + 1) + i) * n + c];
+#pragma unroll
+        for (int g = 0; g < RG; g++) {
+            if (g >= gv) continue;
+            const size_t ct = ct0 + g;
+            acc_add(a0[r][g], c0, one);
+            acc_add(a1[r][g], c1, one);
+            const uint64_t r0 = acc_reduce(a0[r][g], Q), r1 = acc_reduce(a1[r][g], Q);
+            if (i == L) {
+                rb.S[r][(ct * 2 + 0) * n + c] = r0;
+                rb.S[r][(ct * 2 + 1) * n + c] = r1;
+            } else {
+                rb.P[r][((ct * 2 + 0) * L + i) * n + c] = r0;
+                rb.P[r][((ct * 2 + 1) * L + i) * n + c] = r1;
+            }
         }
     }
 }
@@ -302,7 +321,7 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
     uint64_t *Pm = reinterpret_cast<uint64_t *>(base + lay.Pm);
     uint64_t *Sm = reinterpret_cast<uint64_t *>(base + lay.Sm);
     const uint32_t tiles = (uint32_t)((((size_t)(L + 1) << logn) + TPB - 1) / TPB);
-    const uint64_t blocks = (uint64_t)((tiles + 7) / 8) * 8 * count;
+    const uint64_t blocks = (uint64_t)((tiles + 7) / 8) * 8 * ((count + RG - 1) / RG);
     if (blocks * TPB >= (uint64_t(1) << 32)) fail(ERR_INVALID_ARGUMENT, "rotation batch too large for one call");
     for (uint32_t r0 = 0; r0 < rotations; r0 += RMAX) {
         const uint32_t nr = rotations - r0 < (uint32_t)RMAX ? rotations - r0 : (uint32_t)RMAX;
