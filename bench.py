@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="mulrelin",
-                   choices=["mulrelin", "ntt", "copy", "chain", "rotsum", "matvec"])
+                   choices=["mulrelin", "ntt", "copy", "chain", "rotsum", "matvec", "serve"])
     p.add_argument("--no-bfv", action="store_true", help="skip the BFV leg of the mulrelin workload")
     p.add_argument("--batch", type=int, default=128)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -638,6 +638,164 @@ def bench_matvec(args):
             "self_check": "batched product of request 0 == its product alone", "wall_s": wall}
 
 
+class MatvecServer:
+    """One request type of the serving workload: a context, one matrix's diagonals (NTT
+    form, synthetic residues), resident Galois keys (synthetic residues; the baby steps
+    also prepared for hoisting), a pool of resident input ciphertexts, and batch buffers
+    for up to max_batch requests."""
+
+    def __init__(self, torch, sealhip, matvec, scheme, n, moduli, plain, b, g, pool, max_batch, gen, st):
+        K = len(moduli)
+        L = K - 1
+        self.torch, self.n, self.L, self.st, self.max_batch = torch, n, L, st, max_batch
+        self.ctx = sealhip.Context(scheme, n, moduli, plain)
+        ctx = self.ctx
+        self.plains = torch.empty((g, b, L, n), dtype=torch.int64, device="cuda")
+        fill_uniform(torch, self.plains, moduli[:L], 2, gen)
+
+        def key():
+            k = torch.empty((K - 1, 2, K, n), dtype=torch.int64, device="cuda")
+            fill_uniform(torch, k, moduli, 2, gen)
+            return k
+
+        self.baby = [key() for _ in range(1, b)]
+        self.giant = [key() for _ in range(1, g)]
+        hb, hw = ctx.hoisted_key_size(L), ctx.hoisted_key_workspace_size(L)
+        hws = torch.empty(((hw + 7) // 8,), dtype=torch.int64, device="cuda")
+        self.hoisted = []
+        for j, k in zip(range(1, b), self.baby):
+            h = torch.empty(((hb + 7) // 8,), dtype=torch.int64, device="cuda")
+            ctx.hoisted_key_prepare(k.data_ptr(), ctx.galois_elt_from_step(j), L, h.data_ptr(), hws.data_ptr(), hw,
+                                    st)
+            self.hoisted.append(h)
+        self.mv = matvec.BsgsMatvec(ctx, n, L, b, g, self.plains.data_ptr(), [k.data_ptr() for k in self.baby],
+                                    [h.data_ptr() for h in self.hoisted], [k.data_ptr() for k in self.giant],
+                                    bfv=scheme == sealhip.SCHEME_BFV)
+        self.pool = torch.empty((pool, 2, L, n), dtype=torch.int64, device="cuda")
+        fill_uniform(torch, self.pool, moduli[:L], 2, gen)
+        self.x = torch.empty((max_batch, 2, L, n), dtype=torch.int64, device="cuda")
+        self.out = torch.empty_like(self.x)
+        self.babybuf = torch.empty((b, max_batch, 2, L, n), dtype=torch.int64, device="cuda")
+        self.inner = torch.empty_like(self.x)
+        self.wsz = self.mv.workspace_sizes(max_batch)
+        self.ws = torch.empty(((self.wsz + 7) // 8,), dtype=torch.int64, device="cuda")
+
+    def run(self, inputs):
+        """One batch: the pool entries `inputs` gathered into the batch buffer, then one
+        batched matvec into self.out[:len(inputs)]."""
+        for k, i in enumerate(inputs):
+            self.x[k].copy_(self.pool[i])
+        self.mv.run(self.x.data_ptr(), len(inputs), self.out.data_ptr(), self.babybuf.data_ptr(),
+                    self.inner.data_ptr(), self.ws.data_ptr(), self.wsz, self.st)
+
+
+def serve_mixed(torch, servers, rate, requests, max_batch, seed):
+    """Open-loop serving of a mixed stream: Poisson arrivals at `rate` per second, each
+    request of type 0 or 1 with probability 1/2 and an input drawn from that type's pool.
+    The server picks the type whose oldest waiting request arrived first, runs every
+    waiting request of that type (at most max_batch) as one batch, and waits for it; a
+    request's latency is its batch's completion minus its arrival.  max_batch = 1 is the
+    plain one-request-at-a-time path under the same arrivals."""
+    import collections
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=requests))
+    types = (rng.random(requests) < 0.5).astype(int)
+    inputs = rng.integers(0, servers[0].pool.shape[0], size=requests)
+    pending = [collections.deque(), collections.deque()]
+    lat = np.empty(requests)
+    batches = [0, 0]
+    nxt = done = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while done < requests:
+        now = time.perf_counter() - t0
+        while nxt < requests and arrivals[nxt] <= now:
+            pending[types[nxt]].append(nxt)
+            nxt += 1
+        if not pending[0] and not pending[1]:
+            time.sleep(min(arrivals[nxt] - now, 1e-3))
+            continue
+        ty = 0 if not pending[1] or (pending[0] and pending[0][0] < pending[1][0]) else 1
+        batch = [pending[ty].popleft() for _ in range(min(max_batch, len(pending[ty])))]
+        servers[ty].run([int(inputs[r]) for r in batch])
+        torch.cuda.synchronize()
+        fin = time.perf_counter() - t0
+        lat[batch] = fin - arrivals[batch]
+        done += len(batch)
+        batches[ty] += 1
+    span = time.perf_counter() - t0
+    res = {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+           "served_per_s": requests / span}
+    for ty, name in enumerate(("bfv", "ckks")):
+        sel = lat[types == ty]
+        res[name] = {"requests": int(sel.size), "p50_ms": float(np.percentile(sel, 50) * 1e3),
+                     "p99_ms": float(np.percentile(sel, 99) * 1e3),
+                     "mean_batch": float(sel.size / max(batches[ty], 1))}
+    return res
+
+
+def bench_serve(args):
+    """The serving workload: a Poisson stream of matrix-vector requests, half BFV (N=8192,
+    BFVDefault(8192) primes, t = 65537, a 4096 x 4096 matrix mod t, BSGS 64 x 64 in each
+    row of the slot matrix) and half CKKS (N=2^15, Create(32768, {60, 40 x 14, 60}), a
+    1024 x 1024 matrix, BSGS 32 x 32), keys and diagonals resident (synthetic residues).
+    Arrivals at half the batched server's capacity for the mix (measured first with full
+    batches); value = requests/s served by the batched server, with p50/p99 latency, and
+    the plain one-at-a-time server under the same arrivals beside it.  Self-check: for
+    each type, a request's words in a full batch equal its words computed alone."""
+    import torch
+    import sealhip
+    import matvec
+    sealhip.init(0)
+    st = torch.cuda.current_stream().cuda_stream
+    gen = torch.Generator(device="cuda").manual_seed(8)
+    B = 16
+    bfv_moduli = [0x7fffffd8001, 0x7fffffc8001, 0xfffffffc001, 0xffffff6c001, 0xfffffebc001]
+    ckks_moduli = sealhip.coeff_modulus_create(1 << 15, [60] + [40] * 14 + [60])
+    servers = [MatvecServer(torch, sealhip, matvec, sealhip.SCHEME_BFV, 8192, bfv_moduli, 65537, 64, 64, 32, B,
+                            gen, st),
+               MatvecServer(torch, sealhip, matvec, sealhip.SCHEME_CKKS, 1 << 15, ckks_moduli, 0, 32, 32, 32, B,
+                            gen, st)]
+    # capacity with full batches, per type
+    full = list(range(B))
+    cap = []
+    for sv in servers:
+        for _ in range(max(args.warmup, 1)):
+            sv.run(full)
+        with Timer(torch) as t:
+            for _ in range(args.steps):
+                sv.run(full)
+        cap.append(B * args.steps / (t.ms * 1e-3))
+    mix_cap = 1.0 / (0.5 / cap[0] + 0.5 / cap[1])
+    # determinism: a request's words in a full batch equal its words alone
+    for sv, name in zip(servers, ("bfv", "ckks")):
+        sv.run(full)
+        torch.cuda.synchronize()
+        batched = [sv.out[0].clone(), sv.out[B - 1].clone()]
+        for k, i in ((0, 0), (1, B - 1)):
+            sv.run([i])
+            torch.cuda.synchronize()
+            if not torch.equal(sv.out[0], batched[k]):
+                raise SystemExit(f"bench self-check failed: {name} request {i} differs alone and in a batch")
+    rate = 0.5 * mix_cap
+    reqs = max(200, int(rate * args.serve_seconds))
+    batched = serve_mixed(torch, servers, rate, reqs, B, seed=11)
+    plain = serve_mixed(torch, servers, rate, reqs, 1, seed=11)
+    return {"metric": "mixed BFV/CKKS matvec requests/s served; p50/p99 latency", "value": batched["served_per_s"],
+            "unit": "requests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 / batched["served_per_s"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64",
+            "data": "synthetic (seeded uniform residues for ciphertexts, diagonals and keys; Poisson arrivals)",
+            "config": {"workload": "serving: 50% BFV matvec (N=8192, BFVDefault(8192), t=65537, 4096x4096, "
+                                   "BSGS 64x64), 50% CKKS matvec (N=2^15, 60+40x14+60, 1024x1024, BSGS 32x32), "
+                                   f"batches of at most {B}", "parallelism": "single GPU"},
+            "capacity_per_s": {"bfv": cap[0], "ckks": cap[1], "mix": mix_cap},
+            "arrivals": {"kind": "poisson", "rate_per_s": rate, "requests": reqs},
+            "batched": batched, "plain": plain, "batched_p99_over_plain_p99": batched["p99_ms"] / plain["p99_ms"],
+            "self_check": "per type, requests 0 and 15 give the same words alone and in a full batch"}
+
+
 def bench_copy(args):
     """HBM copy of 1 GiB (torch's device copy kernel): the calibration run for the
     FETCH_SIZE / WRITE_SIZE counters (known 1 GiB read + 1 GiB written per launch)."""
@@ -666,7 +824,7 @@ def main():
                                    "(independent ciphertexts shard across GPUs with no exchange)"}))
         sys.exit(2)
     out = {"mulrelin": bench_mulrelin, "ntt": bench_ntt, "copy": bench_copy, "chain": bench_chain,
-           "rotsum": bench_rotsum, "matvec": bench_matvec}[args.workload](args)
+           "rotsum": bench_rotsum, "matvec": bench_matvec, "serve": bench_serve}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

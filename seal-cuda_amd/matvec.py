@@ -1,45 +1,52 @@
-"""Diagonal matrix-vector products on CKKS ciphertexts with baby-step giant-step
-rotations (Halevi-Shoup), over the C ABI.
+"""Diagonal matrix-vector products with baby-step giant-step rotations (Halevi-Shoup),
+over the C ABI, for CKKS and BFV ciphertexts.
 
 The reference's caller of this pattern is app/matmul.cuh:98-121 (one rotateVector and
 one multiplyPlain per diagonal, summed).  Here, for a d x d matrix with d = b * g and
-the vector replicated across the slots:
+the vector replicated across each rotation cycle of the slots (CKKS: all N/2 slots;
+BFV: each of the two rows of N/2 slots, which rotateRows turns independently):
 
     M v = sum_{h<g} rot_{h b}( sum_{j<b} diag'_{h,j} * rot_j(v) ),
     diag'_{h,j} = rot_{-h b}(diag_{h b + j}),  diag_k[i] = M[i mod d][(i + k) mod d],
 
-where rot_k(x)[i] = x[i + k] (rotateVector(k)).  The b - 1 baby rotations share one
-ModUp (shc_rotate_many), each inner sum is one shc_multiply_plain_accumulate pass, and
-the g - 1 giant rotations are ordinary key switches.  All ciphertexts of a batch share
-the diagonals and keys.
+where rot_k(x)[i] = x[i + k] (rotateVector / rotateRows(k)).  The b - 1 baby rotations
+share one ModUp (shc_rotate_many), each inner sum is one shc_multiply_plain_accumulate
+pass over NTT-form operands, and the g - 1 giant rotations are ordinary key switches.
+BFV ciphertexts live in coefficient form: the b baby rotations go to NTT form once, and
+each inner sum comes back by one inverse transform before its giant rotation.  All
+ciphertexts of a batch share the diagonals and keys.
 """
 import numpy as np
 
 import sealhip
 
 
-def diagonals(M, slots, b, g):
-    """Pre-rotated diagonals diag'_{h,j} as complex slot vectors, [g][b][slots]."""
+def diagonals(M, slots, b, g, rows=1):
+    """Pre-rotated diagonals diag'_{h,j} as slot vectors, [g][b][slots]: complex for CKKS
+    (rows = 1), or M's dtype for BFV (rows = 2: each row of slots / 2 rotates on its own)."""
     d = M.shape[0]
-    assert M.shape == (d, d) and d == b * g and slots % d == 0
-    i = np.arange(slots)
-    out = np.empty((g, b, slots), dtype=np.complex128)
+    cycle = slots // rows
+    assert M.shape == (d, d) and d == b * g and cycle % d == 0
+    i = np.arange(cycle)
+    out = np.empty((g, b, rows, cycle), dtype=np.complex128 if rows == 1 else M.dtype)
     for h in range(g):
         for j in range(b):
             k = h * b + j
             diag = M[i % d, (i + k) % d]
-            out[h, j] = np.roll(diag, h * b)  # rot_{-hb}: x[i] <- diag[i - hb]
-    return out
+            out[h, j, :] = np.roll(diag, h * b)  # rot_{-hb}: x[i] <- diag[i - hb]
+    return out.reshape(g, b, slots)
 
 
 class BsgsMatvec:
     """One matrix, encoded once: `plains` is a device buffer [g][b][L][N] of the
-    pre-rotated diagonals in NTT form (CKKS-encoded at the ciphertexts' level L).
-    baby_keys / baby_hoisted: Galois keys of steps 1..b-1 and their prepared forms at
-    level L; giant_keys: Galois keys of steps b, 2b, ..., (g-1) b."""
+    pre-rotated diagonals in NTT form (CKKS: encoded at the ciphertexts' level L; BFV:
+    batch-encoded, then shc_bfv_plain_to_ntt).  baby_keys / baby_hoisted: Galois keys of
+    steps 1..b-1 and their prepared forms at level L; giant_keys: Galois keys of steps
+    b, 2b, ..., (g-1) b.  bfv: the ciphertexts are BFV (coefficient form)."""
 
-    def __init__(self, ctx, n, L, b, g, plains, baby_keys, baby_hoisted, giant_keys):
+    def __init__(self, ctx, n, L, b, g, plains, baby_keys, baby_hoisted, giant_keys, bfv=False):
         self.ctx, self.n, self.L, self.b, self.g = ctx, n, L, b, g
+        self.bfv = bfv
         self.plains = plains
         self.baby_elts = [ctx.galois_elt_from_step(j) for j in range(1, b)]
         self.giant_elts = [ctx.galois_elt_from_step(h * b) for h in range(1, g)]
@@ -61,10 +68,14 @@ class BsgsMatvec:
         if b > 1:
             ctx.rotate_many(d_in, count, L, self.baby_elts, self.baby_hoisted, self.baby_keys,
                             d_baby + ct_bytes, d_ws, ws_bytes, stream)
+        if self.bfv:
+            ctx.ntt_forward(d_baby, b * count * 2, L, stream)
         for h in range(g):
             dst = d_out if h == 0 else d_inner
             ctx.multiply_plain_accumulate(d_baby, self.plains + h * b * plain_words * 8, b, count * 2, L, dst,
                                           stream)
+            if self.bfv:
+                ctx.ntt_inverse(dst, count * 2, L, stream)
             if h > 0:
                 ctx.apply_galois_keyswitch(d_inner, self.giant_keys[h - 1], self.giant_elts[h - 1], count, L,
                                            d_ws, ws_bytes, stream)
