@@ -152,8 +152,7 @@ int shc_relinearize(const shc_context *ctx, uint64_t *ct, const uint64_t *key, u
         const auto &c = ctx_of(ctx);
         check_limbs(c, limbs);
         const size_t poly = (size_t)limbs * c.n;
-        shc::switch_key(c, ct + 2 * poly, 3 * poly, ct, 3 * poly, key, count, limbs, ws, ws_bytes, (hipStream_t)stream,
-                        true);
+        shc::switch_key(c, ct + 2 * poly, 3 * poly, ct, 3 * poly, key, count, limbs, ws, ws_bytes, (hipStream_t)stream);
     });
 }
 

@@ -107,7 +107,7 @@ void apply_galois_keyswitch(const Context &c, uint64_t *ct, uint32_t count, uint
     const size_t words = (size_t)count * poly;
     shift_zero_kernel<<<blocks_for(words), TPB, 0, st>>>(ct, poly, words);
     launch_check("shift_zero");
-    switch_key(c, sigma, poly, ct, 2 * poly, key, count, L, ks_ws, ws_bytes - sigma_bytes, st, true);
+    switch_key(c, sigma, poly, ct, 2 * poly, key, count, L, ks_ws, ws_bytes - sigma_bytes, st);
 }
 
 }  // namespace shc

@@ -32,16 +32,23 @@ void ntt_forward_grid(uint64_t *data, LimbGrid grid, const DevPrime *primes, int
                       hipStream_t stream);
 void ntt_inverse_grid(uint64_t *data, LimbGrid grid, const DevPrime *primes, int logn, NttOut out,
                       hipStream_t stream);
+// Inverse transform of src's limbs (grid src_grid) into data (grid grid, same I x J);
+// src is left as it was.
+void ntt_inverse_from(const uint64_t *src, LimbGrid src_grid, uint64_t *data, LimbGrid grid, const DevPrime *primes,
+                      int logn, NttOut out, hipStream_t stream);
 
 // Key-switching ModUp fused with the forward NTT: T[i][ct][j] = NTT_i(d[ct][j] mod t_i)
 // for digits j < L (primes digits[j]) and targets i <= L (targets[i]), d[ct] at
 // d + ct * d_ct_stride with L limbs.  Target primes must be below 2^60; output in
-// [0, 4 t_i).  Returns false (and does nothing) for N < 2^13, where the caller gathers
-// first and runs the mapped transform.
+// [0, 4 t_i) when reduce_4q, else [0, 8 t_i).  skip_diag: limbs T[i][ct][i] are not
+// produced and targets i < L hold L - 1 limbs per ciphertext, T[i][ct][j - (j > i)]
+// at limb i * count * L + ct * (L - 1) + ...; the special target keeps all L.  Returns
+// false (and does nothing) for N < 2^13, where the caller gathers first and runs the
+// mapped transform.
 // With a probe, its events are recorded right before and after the rows-pass launch.
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
-                       const DevPrime *targets, const DevPrime *digits, int logn, hipStream_t stream,
-                       const KernelProbe *probe = nullptr);
+                       const DevPrime *targets, const DevPrime *digits, int logn, bool skip_diag, bool reduce_4q,
+                       hipStream_t stream, const KernelProbe *probe = nullptr);
 
 // ModDown by the special prime p fused into the last transforms of a key switch, for
 // `count` ciphertexts at L limbs (N >= 2^13; returns false and does nothing otherwise):

@@ -137,10 +137,11 @@ __device__ __forceinline__ uint64_t finish(uint64_t x, const DevPrime &P) {
 // Rows pass: one block takes row g (the last K layers; the whole limb when K == LOGN)
 // of SUBS limbs that share prime i -- limbs (i, j) for SUBS consecutive j -- so every
 // row in the block uses the same twiddles (broadcast loads, L1/L2 hits).  Blocks run
-// with the limb group fastest, so consecutive blocks also share them.
+// with the limb group fastest, so consecutive blocks also share them.  jn_short != 0:
+// primes i < I - 1 have only jn_short limbs (the grid is sized for jn).
 template <int LOGN, int K, int E, int SUBS, bool INV, bool W = false, class Load, class Store>
 __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, uint32_t I, uint32_t j0, uint32_t jn,
-                                          const Load &load_elem, const Store &store_elem) {
+                                          const Load &load_elem, const Store &store_elem, uint32_t jn_short = 0) {
     constexpr int T = 1 << (K - E);
     constexpr int SZ = 1 << K;
     constexpr int RPL = LOGN - K;  // log2(rows per limb)
@@ -152,7 +153,9 @@ __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, u
     const int g = (int)(r & ((1u << RPL) - 1));
     const uint32_t i = r >> RPL;
     const uint32_t jfirst = j0 + jc * SUBS;
-    const int nsub = (int)min<uint32_t>(SUBS, j0 + jn - jfirst);
+    const uint32_t jend = j0 + (jn_short && i + 1 < I ? jn_short : jn);
+    if (jfirst >= jend) return;  // uniform: a whole block past a short row
+    const int nsub = (int)min<uint32_t>(SUBS, jend - jfirst);
     const int tid = threadIdx.x;
     const int sub = tid / T, t = tid % T;
     const bool active = sub < nsub;
@@ -236,17 +239,37 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__r
         });
 }
 
+// Rows pass of a transform whose input lies elsewhere: reads limb (i, j) from `src`
+// (grid slg) and writes it to `data` (grid lg), same shape.
+template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_from_kernel(const uint64_t *__restrict__ src,
+                                                                       LimbGrid slg, uint64_t *__restrict__ data,
+                                                                       const DevPrime *__restrict__ primes,
+                                                                       LimbGrid lg, uint32_t j0, uint32_t jn) {
+    const GridAccess g{data, lg, LOGN};
+    const GridAccess s{const_cast<uint64_t *>(src), slg, LOGN};
+    rows_body<LOGN, K, E, SUBS, INV>(
+        primes, lg.I, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *s.at(i, j, off); },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
+        });
+}
+
 // The key switch's ModUp rows pass: the same code as a kernel of its own, so that
 // profiles tell it apart from the other forward transforms.  Wide schedule; outputs
-// left in [0, 4q) (the key products' 128-bit sums absorb that range).
-template <int LOGN, int K, int E, int SUBS>
+// left in [0, 4q) (FIN) or [0, 8q), ranges the key products' 128-bit sums absorb.
+// jn_short: see rows_body (targets without their own digit's limb).
+template <int LOGN, int K, int E, int SUBS, bool FIN>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_modup_kernel(uint64_t *__restrict__ data,
                                                                         const DevPrime *__restrict__ primes,
-                                                                        LimbGrid lg) {
+                                                                        LimbGrid lg, uint32_t jn_short) {
     const GridAccess g{data, lg, LOGN};
     rows_body<LOGN, K, E, SUBS, false, true>(
         primes, lg.I, 0, lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
-        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { *g.at(i, j, off) = finish<false, false>(x, primes[i]); });
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            *g.at(i, j, off) = FIN ? finish<false, false>(x, primes[i]) : x;
+        },
+        jn_short);
 }
 
 template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
@@ -267,13 +290,15 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
 // butterflies take inputs < 8q), then the first K layers.  Blocks run in (ct, j, i)
 // order, i fastest, so the L+1 blocks that read one tile of a digit run together and
 // on one XCD (block % 8 is the tile's % 8), and the digit is read from HBM once.
+// skip_diag: limb (i, i) is not produced (its NTT is the NTT-form input limb) and the
+// targets i < L keep L - 1 limbs per ciphertext, T[i][ct][j - (j > i)].
 template <int LOGN, int K, int E, int SUBS>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const uint64_t *__restrict__ d,
                                                                         size_t d_ct_stride,
                                                                         uint64_t *__restrict__ T,
                                                                         const DevPrime *__restrict__ targets,
                                                                         const DevPrime *__restrict__ digits,
-                                                                        uint32_t count, uint32_t L) {
+                                                                        uint32_t count, uint32_t L, bool skip_diag) {
     constexpr int SZ = 1 << K;
     constexpr int C = 1 << (LOGN - K);
     constexpr int TILES = C / SUBS;
@@ -286,8 +311,11 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
     const uint32_t r = task / (L + 1);
     const uint32_t j = r % L;
     const size_t ct = r / L;
+    if (skip_diag && i == j) return;  // uniform: the whole block
+    const bool short_row = skip_diag && i < L;
+    const size_t limb = (size_t)i * count * L + ct * (short_row ? L - 1 : L) + (short_row && j > i ? j - 1 : j);
     const uint64_t *in = d + ct * d_ct_stride + ((size_t)j << LOGN) + c0 + col;
-    uint64_t *out = T + ((((size_t)i * count + ct) * L + j) << LOGN) + c0 + col;
+    uint64_t *out = T + (limb << LOGN) + c0 + col;
     const DevPrime &P = targets[i];
     const bool reduce = digits[j].q > 8 * P.q;
     auto addr = [col](int idx) { return idx * SUBS + col; };
@@ -409,6 +437,59 @@ void launch_ntt(uint64_t *data, LimbGrid lg, uint32_t j0, uint32_t jn, const Dev
     }
 }
 
+// Inverse transform of src's limbs into data (rows pass from src, columns pass in place).
+template <int LOGN, bool FULL>
+void launch_inverse_from(const uint64_t *src, LimbGrid slg, uint64_t *data, LimbGrid lg, uint32_t j0, uint32_t jn,
+                         const DevPrime *primes, hipStream_t st) {
+    if constexpr (LOGN <= 12) {
+        constexpr int K = LOGN;
+        constexpr int E = K < 4 ? K : 4;
+        constexpr int T = 1 << (K - E);
+        constexpr int SUBS = (T >= 256) ? 1 : (256 / T);
+        const uint32_t blocks = lg.I * ((jn + SUBS - 1) / SUBS);
+        ntt_rows_from_kernel<LOGN, K, E, SUBS, true, true, FULL>
+            <<<blocks, SUBS * T, 0, st>>>(src, slg, data, primes, lg, j0, jn);
+    } else {
+        constexpr int K1 = LOGN / 2;
+        constexpr int K2 = LOGN - K1;
+        constexpr int E = 4;
+        constexpr int CS = 16;
+        constexpr int RS = (4096 >> K2) < 1 ? 1 : (4096 >> K2);
+        const uint32_t col_blocks = lg.I * jn * ((1u << K2) / CS);
+        const uint32_t row_blocks = (lg.I << K1) * ((jn + RS - 1) / RS);
+        ntt_rows_from_kernel<LOGN, K2, E, RS, true, false, FULL>
+            <<<row_blocks, RS << (K2 - E), 0, st>>>(src, slg, data, primes, lg, j0, jn);
+        ntt_cols_kernel<LOGN, K1, E, CS, true, true, FULL>
+            <<<col_blocks, CS << (K1 - E), 0, st>>>(data, primes, lg, j0, jn);
+    }
+}
+
+template <bool FULL>
+void dispatch_inverse_from(const uint64_t *src, LimbGrid slg, uint64_t *data, LimbGrid lg, const DevPrime *primes,
+                           int logn, hipStream_t st) {
+    if (logn < 1 || logn > 17) fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree must be 2^1 .. 2^17");
+    if (slg.I != lg.I || slg.J != lg.J) fail(ERR_LOGIC, "transform source and destination shapes differ");
+    if ((uint64_t)lg.I * lg.J == 0) return;
+    for (const LimbGrid &g : {slg, lg})
+        if ((uint64_t)(g.I - 1) * g.A + (uint64_t)(g.J - 1) * g.B > UINT32_MAX)
+            fail(ERR_INVALID_ARGUMENT, "too many limbs for one transform call");
+    const uint64_t max_limbs = (uint64_t(1) << 34) >> logn;
+    if (lg.I > max_limbs) fail(ERR_INVALID_ARGUMENT, "too many primes for one transform call");
+    const uint32_t step = (uint32_t)(max_limbs / lg.I < lg.J ? max_limbs / lg.I : lg.J);
+    for (uint32_t j0 = 0; j0 < lg.J; j0 += step) {
+        const uint32_t jn = lg.J - j0 < step ? lg.J - j0 : step;
+        switch (logn) {
+#define SHC_NTT_CASE(L) \
+    case L: launch_inverse_from<L, FULL>(src, slg, data, lg, j0, jn, primes, st); break;
+            SHC_NTT_CASE(1) SHC_NTT_CASE(2) SHC_NTT_CASE(3) SHC_NTT_CASE(4) SHC_NTT_CASE(5) SHC_NTT_CASE(6)
+            SHC_NTT_CASE(7) SHC_NTT_CASE(8) SHC_NTT_CASE(9) SHC_NTT_CASE(10) SHC_NTT_CASE(11) SHC_NTT_CASE(12)
+            SHC_NTT_CASE(13) SHC_NTT_CASE(14) SHC_NTT_CASE(15) SHC_NTT_CASE(16) SHC_NTT_CASE(17)
+#undef SHC_NTT_CASE
+        }
+        launch_check("ntt launch");
+    }
+}
+
 template <bool INV, bool FULL>
 void dispatch_chunk(uint64_t *data, LimbGrid lg, uint32_t j0, uint32_t jn, const DevPrime *primes, int logn,
                     hipStream_t st) {
@@ -443,7 +524,7 @@ void dispatch(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, hip
 
 template <int LOGN>
 void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts, uint32_t L, const DevPrime *targets,
-                  const DevPrime *digits, const KernelProbe *probe, hipStream_t st) {
+                  const DevPrime *digits, bool skip_diag, bool reduce_4q, const KernelProbe *probe, hipStream_t st) {
     constexpr int K1 = LOGN / 2;
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
@@ -451,28 +532,32 @@ void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts,
     constexpr int RS = 4096 >> K2;
     const uint32_t limbs = cts * L * (L + 1);
     ntt_cols_modup_kernel<LOGN, K1, E, CS>
-        <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L);
+        <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L, skip_diag);
     const LimbGrid lg{cts * L, 1, L + 1, cts * L};
     const uint32_t row_blocks = ((L + 1) << K1) * ((lg.J + RS - 1) / RS);
     if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
-    ntt_rows_modup_kernel<LOGN, K2, E, RS><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg);
+    const uint32_t jn_short = skip_diag ? cts * (L - 1) : 0;
+    if (reduce_4q)
+        ntt_rows_modup_kernel<LOGN, K2, E, RS, true><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg, jn_short);
+    else
+        ntt_rows_modup_kernel<LOGN, K2, E, RS, false><<<row_blocks, RS << (K2 - E), 0, st>>>(T, targets, lg, jn_short);
     if (probe) SHC_HIP(hipEventRecord(probe->end, st));
 }
 
 }  // namespace
 
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
-                       const DevPrime *targets, const DevPrime *digits, int logn, hipStream_t st,
-                       const KernelProbe *probe) {
+                       const DevPrime *targets, const DevPrime *digits, int logn, bool skip_diag, bool reduce_4q,
+                       hipStream_t st, const KernelProbe *probe) {
     if (logn < 13 || logn > 17 || count == 0) return false;
     if ((uint64_t)count * L * (L + 1) > ((uint64_t(1) << 34) >> logn))
         fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
     switch (logn) {
-        case 13: launch_modup<13>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
-        case 14: launch_modup<14>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
-        case 15: launch_modup<15>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
-        case 16: launch_modup<16>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
-        case 17: launch_modup<17>(d, d_ct_stride, T, count, L, targets, digits, probe, st); break;
+        case 13: launch_modup<13>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
+        case 14: launch_modup<14>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
+        case 15: launch_modup<15>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
+        case 16: launch_modup<16>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
+        case 17: launch_modup<17>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
     }
     launch_check("ntt modup launch");
     return true;
@@ -521,6 +606,12 @@ void ntt_inverse(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *p
 void ntt_inverse_grid(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, NttOut out, hipStream_t st) {
     if (out == NTT_FULL) dispatch<true, true>(data, lg, primes, logn, st);
     else dispatch<true, false>(data, lg, primes, logn, st);
+}
+
+void ntt_inverse_from(const uint64_t *src, LimbGrid src_grid, uint64_t *data, LimbGrid grid, const DevPrime *primes,
+                      int logn, NttOut out, hipStream_t st) {
+    if (out == NTT_FULL) dispatch_inverse_from<true>(src, src_grid, data, grid, primes, logn, st);
+    else dispatch_inverse_from<false>(src, src_grid, data, grid, primes, logn, st);
 }
 
 }  // namespace shc

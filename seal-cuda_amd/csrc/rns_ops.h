@@ -23,20 +23,21 @@ size_t keyswitch_workspace(const Context &c, uint32_t count, uint32_t L);
 //           (CKKS: NTT form; BFV: coefficient form), L limbs each.
 //   ct:     c0,c1 of each ciphertext, stride `ct_stride` words between ciphertexts.
 //   key:    [K-1][2][K][N] key-switching key at key level (NTT form).
-//   target_is_scratch: the caller gives the target up (relinearization's c2, a Galois
-//           scratch row): a CKKS target is then inverse transformed in place instead of
-//           being copied first.
+// The target is read only.
 void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
-                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st,
-                bool target_is_scratch = false);
+                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st);
 
 // Stages of a key switch at level L over `count` ciphertexts, for callers that compose
 // them differently (hoisted rotations):
 //   modup_to_T: T[i][ct][j] = NTT_i(d_j mod q_i), digits d (coefficient form) at
 //               digits + ct * dstride + j * N; targets i <= L (special last); values
-//               in [0, 4 q_i).
+//               in [0, 8 q_i) (L <= 31) or [0, 4 q_i).  skip_diag (CKKS, whose inputs
+//               are NTT-form): the limbs (i, i) are left out where N >= 2^13 and the
+//               targets i < L hold L - 1 limbs per ciphertext; returns whether it did.
 //   keyswitch_inner_product: P[ct][k][i] = sum_j T[i][ct][j] key[j][k][ki] (i < L),
-//               S[ct][k] the special target's.
+//               S[ct][k] the special target's; with diag (after a skipped diagonal)
+//               the term j == i reads the NTT-form input limb diag + ct * diag_stride
+//               + i * N.
 //   moddown_into: ct_k[i] += ModDown(P_k, S_k)[i] (S is consumed; U is scratch
 //               [count][2][L][N]; P in NTT form).
 // The regions of a key-switch workspace of `count` ciphertexts at level L.
@@ -44,10 +45,11 @@ struct KsRegions {
     uint64_t *d, *T, *P, *S, *U;
 };
 KsRegions keyswitch_regions(const Context &c, uint32_t count, uint32_t L, void *ws);
-void modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64_t *T, uint32_t count, uint32_t L,
-                hipStream_t st, const KernelProbe *probe = nullptr);
+bool modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64_t *T, uint32_t count, uint32_t L,
+                hipStream_t st, const KernelProbe *probe = nullptr, bool skip_diag = false);
 void keyswitch_inner_product(const Context &c, const uint64_t *T, const uint64_t *key, uint64_t *P, uint64_t *S,
-                             uint32_t count, uint32_t L, hipStream_t st);
+                             uint32_t count, uint32_t L, hipStream_t st, const uint64_t *diag = nullptr,
+                             size_t diag_stride = 0);
 void moddown_into(const Context &c, uint64_t *P, uint64_t *S, uint64_t *U, uint64_t *ct, size_t ct_stride,
                   uint32_t count, uint32_t L, hipStream_t st);
 

@@ -75,7 +75,9 @@ __global__ void modup_gather_kernel(const uint64_t *__restrict__ d, size_t d_str
 // three-word sums (Acc3) for its G ciphertexts, so each key word it loads serves G
 // products.  The loads of digit j + 2 are issued before the products of digit j (past
 // the end, the last digit again).  In a partial last group the missing ciphertexts
-// repeat the last one (loads stay in bounds; their sums are not stored).
+// repeat the last one (loads stay in bounds; their sums are not stored).  With diag,
+// T holds no limb (i, i) for i < L (ModUp's skip_diag layout) and that term is the
+// NTT-form input limb diag[ct * diag_stride + i * N] itself (CKKS: NTT(INTT(x)) = x).
 // Blocks are dealt to the 8 XCDs round-robin (XCD = block % 8); the mapping below puts
 // every group of one tile on the same XCD, one after another, so the tile's key words
 // come from HBM once and then from that XCD's L2.
@@ -85,7 +87,8 @@ __global__ __launch_bounds__(TPB) void inner_product_kernel(const uint64_t *__re
                                                             uint64_t *__restrict__ P, uint64_t *__restrict__ S,
                                                             const DevPrime *__restrict__ targets, uint32_t L,
                                                             uint32_t K, int logn, uint32_t count, uint32_t tiles,
-                                                            uint32_t groups) {
+                                                            uint32_t groups, const uint64_t *__restrict__ diag,
+                                                            size_t diag_stride) {
     const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
     const uint32_t tile = (k / groups) * 8 + xcd;
     if (tile >= tiles) return;
@@ -99,13 +102,19 @@ __global__ __launch_bounds__(TPB) void inner_product_kernel(const uint64_t *__re
     const DevPrime &Q = targets[i];
     const size_t ct0 = (size_t)grp * G;
     const int g_valid = (int)min<size_t>(G, count - ct0);
-    const uint64_t *Ti = T + ((((size_t)i * count + ct0) * L) << logn) + c;
+    const bool short_row = diag && i < L;  // uniform over the block
+    const uint32_t Lt = short_row ? L - 1 : L;
+    const uint64_t *Ti = T + (((size_t)i * count * L + ct0 * Lt) << logn) + c;
     const uint64_t *kp = key + (size_t)ki * n + c;
     const size_t kstride = (size_t)K << logn;  // key[j][k] -> key[j][k+1]
     const uint32_t one = opaque_one();
     size_t goff[G];
+    const uint64_t *dp[G];
 #pragma unroll
-    for (int g = 0; g < G; g++) goff[g] = ((size_t)min(g, g_valid - 1) * L) << logn;
+    for (int g = 0; g < G; g++) {
+        goff[g] = ((size_t)min(g, g_valid - 1) * Lt) << logn;
+        dp[g] = short_row ? diag + (ct0 + min(g, g_valid - 1)) * diag_stride + ((size_t)i << logn) + c : nullptr;
+    }
     Acc3 acc0[G], acc1[G];
 #pragma unroll
     for (int g = 0; g < G; g++) acc0[g] = acc1[g] = Acc3{0, 0, 0};
@@ -115,8 +124,10 @@ __global__ __launch_bounds__(TPB) void inner_product_kernel(const uint64_t *__re
         j = j < L ? j : L - 1;  // past the end: the last digit again (in bounds, unused)
         kb[b][0] = kp[(2 * j) * kstride];
         kb[b][1] = kp[(2 * j + 1) * kstride];
+        const bool dj = short_row && j == i;
+        const size_t jt = short_row && j > i ? j - 1 : j;
 #pragma unroll
-        for (int g = 0; g < G; g++) tb[b][g] = Ti[goff[g] + ((size_t)j << logn)];
+        for (int g = 0; g < G; g++) tb[b][g] = *(dj ? dp[g] : Ti + goff[g] + (jt << logn));
     };
     auto step = [&](int b, uint32_t jnext) {
         uint64_t t[G];
@@ -275,7 +286,7 @@ namespace {
 // within the NTT's per-launch limb bound).
 void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct,
                       size_t ct_stride, const uint64_t *key, uint32_t count, uint32_t L, void *ws, hipStream_t st,
-                      const KernelProbe *probe, bool target_is_scratch) {
+                      const KernelProbe *probe) {
     const KsLayout lay = ks_layout(c, count, L);
     char *base = static_cast<char *>(ws);
     uint64_t *d = reinterpret_cast<uint64_t *>(base + lay.d);
@@ -287,39 +298,44 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
     const size_t n = c.n;
     const bool ckks = c.scheme == SCHEME_CKKS;
 
-    // 1. digits in coefficient form: BFV targets already are; a CKKS target that the
-    // caller gives up is transformed where it lies, any other is copied first
+    // 1. digits in coefficient form: BFV targets already are; a CKKS target is
+    // transformed into d and stays in NTT form, where it supplies the key products'
+    // own-prime terms (the ModUp of digit i to prime i is the input limb itself)
     const uint64_t *digits = target;
     size_t dstride = target_stride;
+    const uint64_t *diag = nullptr;
     if (ckks) {
-        if (target_is_scratch && target_stride % n == 0 && target_stride / n <= UINT32_MAX / count) {
-            ntt_inverse_grid(const_cast<uint64_t *>(target), LimbGrid{1, (uint32_t)(target_stride / n), L, count},
-                             c.primes(), logn, NTT_FULL, st);
+        if (target_stride % n == 0 && target_stride / n <= UINT32_MAX / count) {
+            ntt_inverse_from(target, LimbGrid{1, (uint32_t)(target_stride / n), L, count}, d,
+                             LimbGrid{1, L, L, count}, c.primes(), logn, NTT_FULL, st);
+            diag = target;
         } else {
             const size_t dwords = (size_t)count * L * n;
             gather_polys_kernel<<<blocks_for(dwords), TPB, 0, st>>>(target, target_stride, d, (size_t)L * n, dwords);
             launch_check("gather");
             ntt_inverse(d, count * L, L, c.primes(), logn, NTT_FULL, st);
-            digits = d;
-            dstride = (size_t)L * n;
         }
+        digits = d;
+        dstride = (size_t)L * n;
     }
     // 2-4. ModUp and the inner product with the key
-    modup_to_T(c, digits, dstride, T, count, L, st, probe);
-    keyswitch_inner_product(c, T, key, P, S, count, L, st);
+    const bool skipped = modup_to_T(c, digits, dstride, T, count, L, st, probe, diag != nullptr);
+    keyswitch_inner_product(c, T, key, P, S, count, L, st, skipped ? diag : nullptr, target_stride);
     // 5. ModDown by the special prime
     moddown_into(c, P, S, U, ct, ct_stride, count, L, st);
 }
 
 }  // namespace
 
-void modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64_t *T, uint32_t count, uint32_t L,
-                hipStream_t st, const KernelProbe *probe) {
+bool modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64_t *T, uint32_t count, uint32_t L,
+                hipStream_t st, const KernelProbe *probe, bool skip_diag) {
     const DevPrime *targets = c.keyswitch_primes(L);
     const int logn = c.logn;
-    // T is left in [0, 4q): its consumers (inner products with the key, hoisted rotation
-    // products) sum at most 63 products below 4q * q < 2^122 in 128 bits.
-    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, st, probe)) return;
+    // T's consumers (inner products with the key, hoisted rotation products) sum L
+    // products T * key (key < q < 2^60) and one more term < q in 128 bits: T may stay
+    // in [0, 8q) up to L = 31 (31 * 8q^2 < 2^127.95), in [0, 4q) beyond.
+    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, skip_diag, L > 31, st, probe))
+        return skip_diag;
     const size_t twords = ((size_t)count * L * (L + 1)) << logn;
     modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(digits, dstride, T, targets, c.primes(), count, L, logn,
                                                            twords);
@@ -327,10 +343,11 @@ void modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64
     if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
     ntt_forward_grid(T, LimbGrid{count * L, 1, L + 1, count * L}, targets, logn, NTT_FULL, st);
     if (probe) SHC_HIP(hipEventRecord(probe->end, st));
+    return false;
 }
 
 void keyswitch_inner_product(const Context &c, const uint64_t *T, const uint64_t *key, uint64_t *P, uint64_t *S,
-                             uint32_t count, uint32_t L, hipStream_t st) {
+                             uint32_t count, uint32_t L, hipStream_t st, const uint64_t *diag, size_t diag_stride) {
     constexpr int G = 4;
     const int logn = c.logn;
     const uint32_t groups = (count + G - 1) / G;
@@ -338,7 +355,8 @@ void keyswitch_inner_product(const Context &c, const uint64_t *T, const uint64_t
     const uint64_t ip_blocks = (uint64_t)((tiles + 7) / 8) * 8 * groups;
     if (ip_blocks * TPB >= (uint64_t(1) << 32)) fail(ERR_INVALID_ARGUMENT, "key-switch batch too large");
     inner_product_kernel<G><<<(uint32_t)ip_blocks, TPB, 0, st>>>(T, key, P, S, c.keyswitch_primes(L), L,
-                                                                (uint32_t)c.key_count(), logn, count, tiles, groups);
+                                                                (uint32_t)c.key_count(), logn, count, tiles, groups,
+                                                                diag, diag_stride);
     launch_check("inner_product");
 }
 
@@ -362,8 +380,7 @@ void moddown_into(const Context &c, uint64_t *P, uint64_t *S, uint64_t *U, uint6
 }
 
 void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
-                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st,
-                bool target_is_scratch) {
+                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st) {
     const size_t K = c.key_count();
     if (K < 2) fail(ERR_LOGIC, "keyswitching is not supported by the context");
     if (L < 1 || L > K - 1) fail(ERR_INVALID_ARGUMENT, "level out of range for key switching");
@@ -381,7 +398,7 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
     for (uint32_t done = 0; done < count;) {
         const uint32_t n = (uint32_t)(count - done < max_cts ? count - done : max_cts);
         switch_key_chunk(c, target + (size_t)done * target_stride, target_stride, ct + (size_t)done * ct_stride,
-                         ct_stride, key, n, L, ws, st, armed && done == 0 ? &probe : nullptr, target_is_scratch);
+                         ct_stride, key, n, L, ws, st, armed && done == 0 ? &probe : nullptr);
         done += n;
     }
 }

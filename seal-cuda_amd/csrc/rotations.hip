@@ -78,8 +78,10 @@ __global__ void correction_kernel(const uint64_t *__restrict__ negntt, const uin
 // One thread per (i, c) position and RG ciphertexts (each key word it loads serves RG
 // products), with three-word sums (Acc3); the loads of digit j + 1 go out before the
 // products of digit j, and in a partial last group the missing ciphertext repeats the
-// last one (in bounds, not stored).  Blocks of one tile run on one XCD (XCD = block % 8)
-// so that the tile's key words are served from its L2.
+// last one (in bounds, not stored).  With diag, T holds no limb (i, i) for i < L and
+// that term is the NTT-form input limb diag[ct * diag_stride + i * N].  Blocks of one
+// tile run on one XCD (XCD = block % 8) so that the tile's key words are served from
+// its L2.
 constexpr int RMAX = 4;
 constexpr int RG = 2;
 struct RotBatch {
@@ -91,7 +93,8 @@ template <int R>
 __global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *__restrict__ T, RotBatch rb,
                                                                 size_t corr_off, const DevPrime *__restrict__ targets,
                                                                 uint32_t L, uint32_t K, int logn, uint32_t count,
-                                                                uint32_t tiles) {
+                                                                uint32_t tiles, const uint64_t *__restrict__ diag,
+                                                                size_t diag_stride) {
     const uint32_t groups = (count + RG - 1) / RG;
     const uint32_t xcd = blockIdx.x & 7, q8 = blockIdx.x >> 3;
     const uint32_t tile = (q8 / groups) * 8 + xcd;
@@ -105,12 +108,18 @@ __global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *
     const size_t c = flat & (n - 1);
     const uint32_t ki = (i == L) ? K - 1 : i;
     const DevPrime &Q = targets[i];
-    const uint64_t *Ti = T + ((((size_t)i * count + ct0) * L) << logn) + c;
+    const bool short_row = diag && i < L;  // uniform over the block
+    const uint32_t Lt = short_row ? L - 1 : L;
+    const uint64_t *Ti = T + (((size_t)i * count * L + (size_t)ct0 * Lt) << logn) + c;
     const size_t kstride = (size_t)K << logn;
     const uint32_t one = opaque_one();
     size_t goff[RG];
+    const uint64_t *dp[RG];
 #pragma unroll
-    for (int g = 0; g < RG; g++) goff[g] = ((size_t)min(g, gv - 1) * L) << logn;
+    for (int g = 0; g < RG; g++) {
+        goff[g] = ((size_t)min(g, gv - 1) * Lt) << logn;
+        dp[g] = short_row ? diag + (ct0 + min(g, gv - 1)) * diag_stride + ((size_t)i << logn) + c : nullptr;
+    }
     Acc3 a0[R][RG], a1[R][RG];
 #pragma unroll
     for (int r = 0; r < R; r++)
@@ -118,8 +127,10 @@ __global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *
         for (int g = 0; g < RG; g++) a0[r][g] = a1[r][g] = Acc3{0, 0, 0};
     uint64_t tn[RG], k0n[R], k1n[R];
     auto fetch = [&](uint32_t j) {
+        const bool dj = short_row && j == i;
+        const size_t jt = short_row && j > i ? j - 1 : j;
 #pragma unroll
-        for (int g = 0; g < RG; g++) tn[g] = Ti[goff[g] + ((size_t)j << logn)];
+        for (int g = 0; g < RG; g++) tn[g] = *(dj ? dp[g] : Ti + goff[g] + (jt << logn));
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const uint64_t *kp = rb.key[r] + (size_t)ki * n + c;
@@ -285,12 +296,13 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
     const size_t sigma_bytes = align256((size_t)count * lw * 8);
     const KsRegions ks = keyswitch_regions(c, count, L, base + lay.gal + sigma_bytes);
 
-    // 1. digits of c1 in coefficient form
+    // 1. digits of c1 in coefficient form (CKKS: transformed into ks.d; c1 itself, in NTT
+    // form, supplies the own-prime terms of the products)
     const uint64_t *digits = in + lw;
     size_t dstride = 2 * lw;
     if (ckks) {
-        SHC_HIP(hipMemcpy2DAsync(ks.d, lw * 8, in + lw, 2 * lw * 8, lw * 8, count, hipMemcpyDeviceToDevice, st));
-        ntt_inverse(ks.d, count * L, L, c.primes(), logn, NTT_FULL, st);
+        ntt_inverse_from(in + lw, LimbGrid{1, 2 * L, L, count}, ks.d, LimbGrid{1, L, L, count}, c.primes(), logn,
+                         NTT_FULL, st);
         digits = ks.d;
         dstride = lw;
     }
@@ -314,7 +326,8 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
         return;
     }
     // 3. one ModUp for every rotation
-    modup_to_T(c, digits, dstride, ks.T, count, L, st);
+    const bool skipped = modup_to_T(c, digits, dstride, ks.T, count, L, st, nullptr, ckks);
+    const uint64_t *diag = skipped ? in + lw : nullptr;
     const DevPrime *targets = c.keyswitch_primes(L);
     const size_t K = c.key_count();
     const size_t corr_off = (K - 1) * 2 * K * n;
@@ -333,10 +346,10 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
             rb.S[q] = Sm + (size_t)q * count * 2 * n;
         }
         switch (nr) {
-            case 1: rotation_products_kernel<1><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles); break;
-            case 2: rotation_products_kernel<2><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles); break;
-            case 3: rotation_products_kernel<3><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles); break;
-            default: rotation_products_kernel<4><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles); break;
+            case 1: rotation_products_kernel<1><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, diag, 2 * lw); break;
+            case 2: rotation_products_kernel<2><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, diag, 2 * lw); break;
+            case 3: rotation_products_kernel<3><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, diag, 2 * lw); break;
+            default: rotation_products_kernel<4><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, diag, 2 * lw); break;
         }
         launch_check("rotation_products");
         for (uint32_t q = 0; q < nr; q++) {
