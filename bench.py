@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel at the default workload (batch 128),
 # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB device copy
 # (profiles/mulrelin_pmc_traffic.md); a profiler cannot run inside this process.
-ROWS_MODUP_TRAFFIC_B128 = 40.49e9 + 40.27e9
+ROWS_MODUP_TRAFFIC_B128 = 38.88e9 + 38.65e9
 CKKS = 2
 
 
@@ -229,9 +229,11 @@ def bench_mulrelin(args):
                   "batched": served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, reqs, B),
                   "plain": served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, reqs, 1)}
 
-    # ModUp rows pass: reads and writes every ModUp limb once (B x L x (L+1) limbs) and
-    # reads the forward twiddle table of each of the L+1 target primes
-    tlimbs = B * L * (L + 1)
+    # ModUp rows pass: reads and writes every ModUp limb once and reads the forward
+    # twiddle table of each of the L+1 target primes; B x L x L limbs (CKKS: each data
+    # prime's own digit is the NTT-form input limb, so L - 1 digits go to each of the L
+    # data primes and L to the special prime)
+    tlimbs = B * L * L
     rows_bytes = 2 * tlimbs * n * 8 + K * n * 16
     rows_bfly = tlimbs * (n // 2) * (logn - logn // 2)
     limb_bytes = n * 8
@@ -253,7 +255,7 @@ def bench_mulrelin(args):
                      "frac": achieved_rows / HBM_PEAK_GBS,
                      "traffic": ROWS_MODUP_TRAFFIC_B128 if B == 128 else None,
                      "butterflies_per_launch": rows_bfly,
-                     "kernel": f"ntt_rows_modup_kernel<16,8,4,16>: key-switch ModUp NTT rows pass, {tlimbs} limbs "
+                     "kernel": f"ntt_rows_modup_kernel<16,8,4,16,false>: key-switch ModUp NTT rows pass, {tlimbs} limbs "
                                f"x last {logn - logn // 2} layers",
                      "algorithmic_bytes": rows_bytes, "avg_ms": rows_ms,
                      "butterflies_per_s": rows_bfly / (rows_ms * 1e-3)},
