@@ -606,7 +606,7 @@ def bench_matvec(args):
     fill_uniform(torch, x, moduli[:L], 2, gen)
     out = torch.empty_like(x)
     baby_buf = torch.empty((b, batch, 2, L, n), dtype=torch.int64, device="cuda")
-    inner = torch.empty_like(x)
+    inner = torch.empty((g, batch, 2, L, n), dtype=torch.int64, device="cuda")
     wsz = mv.workspace_sizes(batch)
     ws = torch.empty(((wsz + 7) // 8,), dtype=torch.int64, device="cuda")
 
@@ -678,7 +678,7 @@ class MatvecServer:
         self.x = torch.empty((max_batch, 2, L, n), dtype=torch.int64, device="cuda")
         self.out = torch.empty_like(self.x)
         self.babybuf = torch.empty((b, max_batch, 2, L, n), dtype=torch.int64, device="cuda")
-        self.inner = torch.empty_like(self.x)
+        self.inner = torch.empty((g, max_batch, 2, L, n), dtype=torch.int64, device="cuda")
         self.wsz = self.mv.workspace_sizes(max_batch)
         self.ws = torch.empty(((self.wsz + 7) // 8,), dtype=torch.int64, device="cuda")
 

@@ -238,6 +238,15 @@ int shc_multiply_plain_ntt(const shc_context *ctx, const uint64_t *d_a, const ui
  * diagonal) in one pass. */
 int shc_multiply_plain_accumulate(const shc_context *ctx, const uint64_t *d_cts, const uint64_t *d_plains,
                                   uint32_t terms, uint32_t polys, uint32_t limb_count, uint64_t *d_out, void *stream);
+/* `outputs` such sums over the same d_cts at once: d_plains [outputs][terms][limb_count][N],
+ * d_out [outputs][polys][limb_count][N] (not overlapping d_cts); output o equals
+ * shc_multiply_plain_accumulate with d_plains + o * terms * limb_count * N.  Each ciphertext
+ * word is read once for up to 8 outputs (all the giant steps' inner sums of a
+ * baby-step giant-step matrix-vector product in a few passes).  No counterpart in the
+ * reference (its diagonal products read each rotation once per diagonal). */
+int shc_multiply_plain_accumulate_many(const shc_context *ctx, const uint64_t *d_cts, const uint64_t *d_plains,
+                                       uint32_t terms, uint32_t outputs, uint32_t polys, uint32_t limb_count,
+                                       uint64_t *d_out, void *stream);
 /* Ciphertexts [count][size][limb_count][N]: out = ct with poly 0 +- plain ([limb_count][N],
  * same form as ct; subtract != 0 subtracts).  CKKS add/subPlainInplace
  * (src/evaluator_cuda.cu:1544-1690). */

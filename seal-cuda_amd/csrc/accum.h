@@ -57,4 +57,10 @@ __device__ __forceinline__ uint64_t acc_reduce(const Acc3 &a, const DevPrime &Q)
     return barrett128(acc_value(a), Q.q, Q.ratio_lo, Q.ratio_hi);
 }
 
+// Replaces the value by value mod q (for sums longer than the 128-bit bound allows).
+__device__ __forceinline__ void acc_fold(Acc3 &a, const DevPrime &Q) {
+    const uint64_t r = acc_reduce(a, Q);
+    a = Acc3{(uint32_t)r, r >> 32, 0};
+}
+
 }  // namespace shc

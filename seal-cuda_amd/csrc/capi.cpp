@@ -435,6 +435,15 @@ int shc_multiply_plain_accumulate(const shc_context *ctx, const uint64_t *cts, c
     });
 }
 
+int shc_multiply_plain_accumulate_many(const shc_context *ctx, const uint64_t *cts, const uint64_t *plains,
+                                       uint32_t terms, uint32_t outputs, uint32_t polys, uint32_t limbs, uint64_t *out,
+                                       void *stream) {
+    return guarded([&] {
+        shc::multiply_plain_accumulate_many(ctx_of(ctx), cts, plains, terms, outputs, polys, limbs, out,
+                                            (hipStream_t)stream);
+    });
+}
+
 int shc_probe_keyswitch(shc_context *ctx, void *begin_event, void *end_event) {
     return guarded([&] {
         const auto &c = ctx_of(ctx);

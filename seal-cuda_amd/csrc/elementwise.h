@@ -21,6 +21,10 @@ void add_plain_poly0(const Context &c, const uint64_t *ct, const uint64_t *plain
 // plains [terms][L][N], out [polys][L][N] (not overlapping cts).
 void multiply_plain_accumulate(const Context &c, const uint64_t *cts, const uint64_t *plains, uint32_t terms,
                                uint32_t polys, uint32_t L, uint64_t *out, hipStream_t st);
+// `outputs` such sums over the same cts: plains [outputs][terms][L][N], out
+// [outputs][polys][L][N]; each ct word is read once for up to 8 outputs.
+void multiply_plain_accumulate_many(const Context &c, const uint64_t *cts, const uint64_t *plains, uint32_t terms,
+                                    uint32_t outputs, uint32_t polys, uint32_t L, uint64_t *out, hipStream_t st);
 // BFV: out = ct with c0 +- Delta m (SEAL's scaling variant), plain m: [N] mod t.
 void bfv_add_plain(const Context &c, const uint64_t *ct, const uint64_t *plain, uint64_t *out, uint32_t count,
                    uint32_t size, uint32_t L, bool subtract, hipStream_t st);

@@ -10,6 +10,7 @@
 // for a monomial with m >= ceil(t/2) the two reference paths give different words;
 // this build follows the GPU path it replaces.
 #include "elementwise.h"
+#include "accum.h"
 #include "behz.h"
 #include "crypto.h"
 #include "modarith.h"
@@ -78,21 +79,37 @@ __global__ void plain_lift_kernel(const uint64_t *__restrict__ plain, uint64_t *
     }
 }
 
-// out[ct][p][l][c] = sum_j plains[j][l][c] * cts[j][ct][p][l][c] mod q_l, 128-bit sums
-// reduced every 64 terms.
-__global__ void plain_dot_kernel(const uint64_t *__restrict__ cts, const uint64_t *__restrict__ plains,
-                                 uint64_t *__restrict__ out, const DevPrime *__restrict__ primes, uint32_t terms,
-                                 uint32_t L, int logn, size_t total) {
+// out[h][ct][p][l][c] = sum_j plains[h][j][l][c] * cts[j][ct][p][l][c] mod q_l for the
+// H outputs h0 .. h0 + H - 1 (hn of them valid; the rest repeat the last, unstored):
+// each ciphertext word is read once for H outputs (the diagonal matrix-vector products'
+// giant steps), sums in three words (Acc3), folded mod q every 64 terms.
+template <int H>
+__global__ void plain_dot_many_kernel(const uint64_t *__restrict__ cts, const uint64_t *__restrict__ plains,
+                                      uint64_t *__restrict__ out, const DevPrime *__restrict__ primes, uint32_t terms,
+                                      uint32_t h0, uint32_t hn, uint32_t L, int logn, size_t total) {
     const size_t lw = (size_t)L << logn;
+    const uint32_t one = opaque_one();
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
         const size_t r = idx % lw;
         const DevPrime &P = primes[r >> logn];
-        u128 acc{0, 0};
+        const uint64_t *pp[H];
+#pragma unroll
+        for (int h = 0; h < H; h++) pp[h] = plains + (size_t)(h0 + min<uint32_t>(h, hn - 1)) * terms * lw + r;
+        Acc3 acc[H];
+#pragma unroll
+        for (int h = 0; h < H; h++) acc[h] = Acc3{0, 0, 0};
         for (uint32_t j = 0; j < terms; j++) {
-            add128(acc, mul64x64(cts[(size_t)j * total + idx], plains[(size_t)j * lw + r]));
-            if ((j & 63) == 63) acc = u128{barrett128(acc, P.q, P.ratio_lo, P.ratio_hi), 0};
+            const uint64_t t = cts[(size_t)j * total + idx];
+#pragma unroll
+            for (int h = 0; h < H; h++) mac(acc[h], t, pp[h][(size_t)j * lw], one);
+            if ((j & 63) == 63) {
+#pragma unroll
+                for (int h = 0; h < H; h++) acc_fold(acc[h], P);
+            }
         }
-        out[idx] = barrett128(acc, P.q, P.ratio_lo, P.ratio_hi);
+#pragma unroll
+        for (int h = 0; h < H; h++)
+            if ((uint32_t)h < hn) out[(size_t)(h0 + h) * total + idx] = acc_reduce(acc[h], P);
     }
 }
 
@@ -169,12 +186,27 @@ void bfv_plain_to_ntt(const Context &c, const uint64_t *plain, uint32_t L, uint6
 
 void multiply_plain_accumulate(const Context &c, const uint64_t *cts, const uint64_t *plains, uint32_t terms,
                                uint32_t polys, uint32_t L, uint64_t *out, hipStream_t st) {
+    multiply_plain_accumulate_many(c, cts, plains, terms, 1, polys, L, out, st);
+}
+
+void multiply_plain_accumulate_many(const Context &c, const uint64_t *cts, const uint64_t *plains, uint32_t terms,
+                                    uint32_t outputs, uint32_t polys, uint32_t L, uint64_t *out, hipStream_t st) {
     check_level(c, L);
     if (!terms) fail(ERR_INVALID_ARGUMENT, "no terms to accumulate");
+    if (!outputs) fail(ERR_INVALID_ARGUMENT, "no outputs to accumulate");
     const size_t total = ((size_t)polys * L) << c.logn;
     if (!total) return;
-    plain_dot_kernel<<<blocks_for(total), TPB, 0, st>>>(cts, plains, out, c.primes(), terms, L, c.logn, total);
-    launch_check("plain_dot");
+    const uint32_t blocks = blocks_for(total);
+    for (uint32_t h0 = 0; h0 < outputs;) {
+        const uint32_t left = outputs - h0;
+        const uint32_t hn = left >= 8 ? 8 : left;
+        if (hn > 4) plain_dot_many_kernel<8><<<blocks, TPB, 0, st>>>(cts, plains, out, c.primes(), terms, h0, hn, L, c.logn, total);
+        else if (hn > 2) plain_dot_many_kernel<4><<<blocks, TPB, 0, st>>>(cts, plains, out, c.primes(), terms, h0, hn, L, c.logn, total);
+        else if (hn == 2) plain_dot_many_kernel<2><<<blocks, TPB, 0, st>>>(cts, plains, out, c.primes(), terms, h0, hn, L, c.logn, total);
+        else plain_dot_many_kernel<1><<<blocks, TPB, 0, st>>>(cts, plains, out, c.primes(), terms, h0, hn, L, c.logn, total);
+        launch_check("plain_dot");
+        h0 += hn;
+    }
 }
 
 size_t bfv_multiply_plain_workspace(const Context &c, uint32_t L) { return ((size_t)L << c.logn) * 8; }

@@ -10,10 +10,11 @@ BFV: each of the two rows of N/2 slots, which rotateRows turns independently):
     diag'_{h,j} = rot_{-h b}(diag_{h b + j}),  diag_k[i] = M[i mod d][(i + k) mod d],
 
 where rot_k(x)[i] = x[i + k] (rotateVector / rotateRows(k)).  The b - 1 baby rotations
-share one ModUp (shc_rotate_many), each inner sum is one shc_multiply_plain_accumulate
-pass over NTT-form operands, and the g - 1 giant rotations are ordinary key switches.
-BFV ciphertexts live in coefficient form: the b baby rotations go to NTT form once, and
-each inner sum comes back by one inverse transform before its giant rotation.  All
+share one ModUp (shc_rotate_many), the g inner sums come from one
+shc_multiply_plain_accumulate_many call over NTT-form operands (each baby rotation read
+once per 8 giant steps), and the g - 1 giant rotations are ordinary key switches.  BFV
+ciphertexts live in coefficient form: the b baby rotations go to NTT form once, and the
+inner sums come back by one inverse transform before the giant rotations.  All
 ciphertexts of a batch share the diagonals and keys.
 """
 import numpy as np
@@ -58,7 +59,7 @@ class BsgsMatvec:
 
     def run(self, d_in, count, d_out, d_baby, d_inner, d_ws, ws_bytes, stream=None):
         """d_in, d_out: [count][2][L][N]; d_baby: scratch [b][count][2][L][N];
-        d_inner: scratch [count][2][L][N]; d_ws: workspace_sizes(count) bytes."""
+        d_inner: scratch [g][count][2][L][N]; d_ws: workspace_sizes(count) bytes."""
         import ctypes
         ctx, n, L, b, g = self.ctx, self.n, self.L, self.b, self.g
         ct_bytes = count * 2 * L * n * 8
@@ -70,16 +71,20 @@ class BsgsMatvec:
                             d_baby + ct_bytes, d_ws, ws_bytes, stream)
         if self.bfv:
             ctx.ntt_forward(d_baby, b * count * 2, L, stream)
-        for h in range(g):
-            dst = d_out if h == 0 else d_inner
-            ctx.multiply_plain_accumulate(d_baby, self.plains + h * b * plain_words * 8, b, count * 2, L, dst,
-                                          stream)
-            if self.bfv:
-                ctx.ntt_inverse(dst, count * 2, L, stream)
-            if h > 0:
-                ctx.apply_galois_keyswitch(d_inner, self.giant_keys[h - 1], self.giant_elts[h - 1], count, L,
-                                           d_ws, ws_bytes, stream)
-                ctx.add(d_out, d_inner, d_out, count * 2, L, stream)
+        # every giant step's inner sum, inner[h] = sum_j diag'_{h,j} * rot_j(v)
+        ctx.multiply_plain_accumulate_many(d_baby, self.plains, b, g, count * 2, L, d_inner, stream)
+        if self.bfv:
+            ctx.ntt_inverse(d_inner, g * count * 2, L, stream)
+        # giant steps: rot_{h b}(inner[h]), summed
+        for h in range(1, g):
+            ctx.apply_galois_keyswitch(d_inner + h * ct_bytes, self.giant_keys[h - 1], self.giant_elts[h - 1], count,
+                                       L, d_ws, ws_bytes, stream)
+        if g == 1:
+            _memcpy_d2d(d_out, d_inner, ct_bytes, stream)
+        else:
+            ctx.add(d_inner, d_inner + ct_bytes, d_out, count * 2, L, stream)
+            for h in range(2, g):
+                ctx.add(d_out, d_inner + h * ct_bytes, d_out, count * 2, L, stream)
 
 
 _rt = None

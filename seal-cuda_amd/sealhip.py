@@ -90,6 +90,7 @@ def lib():
                                      ctypes.POINTER(_vp), u32, _vp, _vp, sz, _vp]
         L.shc_rotate_many.argtypes = L.shc_rotate_sum.argtypes
         L.shc_multiply_plain_accumulate.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, _vp]
+        L.shc_multiply_plain_accumulate_many.argtypes = [_vp, _vp, _vp, u32, u32, u32, u32, _vp, _vp]
         _lib = L
     return _lib
 
@@ -334,6 +335,10 @@ class Context:
 
     def multiply_plain_accumulate(self, d_cts, d_plains, terms, polys, limbs, d_out, stream=None):
         check(lib().shc_multiply_plain_accumulate(self.handle, d_cts, d_plains, terms, polys, limbs, d_out, stream))
+
+    def multiply_plain_accumulate_many(self, d_cts, d_plains, terms, outputs, polys, limbs, d_out, stream=None):
+        check(lib().shc_multiply_plain_accumulate_many(self.handle, d_cts, d_plains, terms, outputs, polys, limbs,
+                                                       d_out, stream))
 
     def probe_keyswitch(self, begin, end):
         """Arm the next key switch to record HipEvents around its ModUp rows pass."""

@@ -1,9 +1,10 @@
 """GPU parity of the coefficient-wise evaluator operations.
 
 add / sub / negate and the plaintext operations (CKKS add/sub/multiply by an NTT-form
-plaintext; BFV add/sub of Delta m and the lifted plain multiply) equal the oracle word
-for word on batches of ciphertexts, in place and out of place, including a plain
-modulus larger than every data prime (no fast plain lift).
+plaintext; BFV add/sub of Delta m and the lifted plain multiply; sums of NTT-form plain
+products, one or several outputs at once) equal the oracle word for word on batches of
+ciphertexts, in place and out of place, including a plain modulus larger than every
+data prime (no fast plain lift).
 """
 import numpy as np
 import pytest
@@ -107,3 +108,30 @@ def test_plain_ops_reject_bad_arguments():
         ctx.add(d.data_ptr(), d.data_ptr(), d.data_ptr(), 2, 4, stream())   # more limbs than primes
     with pytest.raises(hip().SealHipError):
         ctx.bfv_add_plain(d.data_ptr(), d.data_ptr(), d.data_ptr(), 1, 2, 2, False, stream())   # CKKS context
+
+
+@pytest.mark.parametrize("scheme,terms,outputs", [(CKKS, 70, 11), (BFV, 5, 2)])
+def test_multiply_plain_accumulate_many(scheme, terms, outputs):
+    """sum_j plains[o][j] * cts[j] for several outputs o at once equals the oracle's plain
+    products and sums (70 terms cross the 64-term fold; 11 outputs take an 8-wide and a
+    partial 4-wide pass), and each output equals the single-output call."""
+    logn, L, polys = 10, 3, 4
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [60, 50, 40, 60])
+    ctx = hip().Context(scheme, n, moduli, 65537 if scheme == BFV else 0)
+    rng = np.random.default_rng(terms + outputs)
+    cts = ck.uniform_residues(rng, (terms, polys, L, n), moduli[:L])
+    plains = ck.uniform_residues(rng, (outputs, terms, L, n), moduli[:L])
+    dc, dp = to_dev(cts), to_dev(plains)
+    out = empty_dev((outputs, polys, L, n))
+    ctx.multiply_plain_accumulate_many(dc.data_ptr(), dp.data_ptr(), terms, outputs, polys, L, out.data_ptr(), stream())
+    got = from_dev(out)
+    one = empty_dev((polys, L, n))
+    for o in range(outputs):
+        want = None
+        for j in range(terms):
+            t = ck.multiply_plain_ntt(cts[j], plains[o, j], moduli[:L], logn)
+            want = t if want is None else ck.mod_add(want, t, moduli)
+        np.testing.assert_array_equal(got[o], want)
+        ctx.multiply_plain_accumulate(dc.data_ptr(), dp[o].data_ptr(), terms, polys, L, one.data_ptr(), stream())
+        np.testing.assert_array_equal(from_dev(one), want)

@@ -88,7 +88,7 @@ def test_bsgs_matvec_words_and_values():
                            [h.data_ptr() for h in hoisted], [k.data_ptr() for k in giant])
     out = empty_dev((1, 2, L, n))
     baby_buf = empty_dev((b, 1, 2, L, n))
-    inner = empty_dev((1, 2, L, n))
+    inner = empty_dev((g, 1, 2, L, n))
     wsz = mv.workspace_sizes(1)
     wb = ws(wsz)
     mv.run(ct.data_ptr(), 1, out.data_ptr(), baby_buf.data_ptr(), inner.data_ptr(), wb.data_ptr(), wsz, stream())
@@ -177,7 +177,7 @@ def test_bsgs_matvec_bfv_words_and_values():
                            [h.data_ptr() for h in hoisted], [k.data_ptr() for k in giant], bfv=True)
     out = empty_dev((1, 2, L, n))
     baby_buf = empty_dev((b, 1, 2, L, n))
-    inner = empty_dev((1, 2, L, n))
+    inner = empty_dev((g, 1, 2, L, n))
     wsz = mv.workspace_sizes(1)
     wb = ws(wsz)
     mv.run(ct.data_ptr(), 1, out.data_ptr(), baby_buf.data_ptr(), inner.data_ptr(), wb.data_ptr(), wsz, stream())
@@ -200,7 +200,7 @@ def test_bsgs_matvec_bfv_words_and_values():
     np.testing.assert_array_equal(got, want)
     # the same request in a batch of two gives the same words
     ct2 = torch.cat([ct, to_dev(ck.uniform_residues(rng, (1, 2, L, n), moduli[:L]))])
-    out2, baby2, inner2 = empty_dev((2, 2, L, n)), empty_dev((b, 2, 2, L, n)), empty_dev((2, 2, L, n))
+    out2, baby2, inner2 = empty_dev((2, 2, L, n)), empty_dev((b, 2, 2, L, n)), empty_dev((g, 2, 2, L, n))
     wsz2 = mv.workspace_sizes(2)
     wb2 = ws(wsz2)
     mv.run(ct2.data_ptr(), 2, out2.data_ptr(), baby2.data_ptr(), inner2.data_ptr(), wb2.data_ptr(), wsz2, stream())
