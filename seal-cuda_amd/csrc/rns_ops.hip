@@ -4,12 +4,14 @@
 // restructured as a handful of wide launches over a whole batch instead of the
 // reference's ~11k per-limb launches:
 //
-//   1. CKKS: inverse NTT of c2 (digits d_j in coefficient form)
+//   1. CKKS: inverse NTT of c2 into the workspace (digits d_j in coefficient form;
+//      c2 stays in NTT form)
 //   2. modup: T[i][ct][j] = d_j mod q_i for every digit j < L and target prime
 //      i in {q_0..q_{L-1}, special}, target-major so that consecutive limbs share a
 //      prime (its twiddles stay in L2)
-//   3. one batched forward NTT over all L(L+1) limbs of T (fused with step 2 for
-//      N >= 2^13)
+//   3. one batched forward NTT over the limbs of T (fused with step 2 for N >= 2^13):
+//      L(L+1) of them per ciphertext, L x L for CKKS, whose own-prime limbs (j == i)
+//      are c2's NTT-form limbs themselves (the reference's i == j shortcut)
 //   4. inner_product: P_k[i] = sum_j T[j][i] * key[j][k][i] with 128-bit accumulation,
 //      one Barrett reduction at the end; G ciphertexts per thread share each key load
 //   5. ModDown: inverse NTT of the special limbs, rounding correction, forward NTT
@@ -387,7 +389,7 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
     if (!count) return;
     if (!ws || ws_bytes < ks_layout(c, count, L).total)
         fail(ERR_INVALID_ARGUMENT, "workspace too small for key switching");
-    // The ModUp buffer holds L(L+1) limbs per ciphertext; one NTT launch covers at most
+    // The ModUp buffer has room for L(L+1) limbs per ciphertext; one NTT launch covers at most
     // 2^34 / N limbs, so large batches run in slices of whole ciphertexts.
     const uint64_t per_ct = (uint64_t)L * (L + 1);
     const uint64_t max_cts = ((uint64_t(1) << 34) >> c.logn) / per_ct;

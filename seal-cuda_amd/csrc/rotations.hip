@@ -15,7 +15,8 @@
 // prepared once per key and level.  A zero digit (d = 0) needs no correction, so calls
 // whose c1 has one fall back to separate rotations and the words always equal the
 // reference's.  Each rotation then costs one inner product, one permutation of the
-// 2(L+1) product limbs and one ModDown instead of a full ModUp (L(L+1) transforms).
+// 2(L+1) product limbs and one ModDown instead of a full ModUp (L x L transforms for
+// CKKS, L(L+1) for BFV).
 #include "rotations.h"
 #include "elementwise.h"
 #include "galois.h"
