@@ -4,10 +4,13 @@
 // RNSTool steps (src/utils/rns.cpp:849-960); GPU twins src/evaluator_cuda.cu:283-382,
 // src/utils/rns_cuda.cu:300-443.  Per coefficient, each base conversion is a short
 // dot product over the input base: the thread stages the scaled inputs in LDS and
-// accumulates every output prime with 128-bit sums, so intermediate residues
-// never leave the kernel.  Scalar multiplies that precede a conversion in the
-// reference (x m_tilde, x t) are folded into the conversion's first constant.
+// accumulates the output primes four at a time (each staged word read once per four)
+// with three-word product sums (accum.h; inputs < 2^61, at most 64 terms, so every sum
+// stays below 2^128), so intermediate residues never leave the kernel.  Scalar
+// multiplies that precede a conversion in the reference (x m_tilde, x t) are folded into
+// the conversion's first constant.
 #include "behz.h"
+#include "accum.h"
 #include "context.h"
 #include "modarith.h"
 #include "ntt.h"
@@ -28,6 +31,28 @@ __device__ __forceinline__ uint64_t shoup_pair(uint64_t x, const uint64_t *pair,
     return shoup(x, pair[0], pair[1], q);
 }
 
+constexpr int OB = 4;  // output primes per pass over the staged inputs
+
+// acc[u] = sum_j v[j * BT + tid] * rows[(o0 + u) * terms + j] for u < OB (rows past
+// `outputs` repeat the last one; the caller does not use those sums).  The rows are
+// read-only tables with block-uniform addresses: read through the constant address
+// space, so that they become scalar loads.
+typedef const __attribute__((address_space(4))) uint64_t const_u64;
+__device__ __forceinline__ void dot_rows(Acc3 (&acc)[OB], const uint64_t *v, const uint64_t *__restrict__ rows,
+                                         uint32_t terms, uint32_t o0, uint32_t outputs, int tid, uint32_t one) {
+    const_u64 *r[OB];
+#pragma unroll
+    for (int u = 0; u < OB; u++) {
+        r[u] = (const_u64 *)(rows + (size_t)min<uint32_t>(o0 + u, outputs - 1) * terms);
+        acc[u] = Acc3{0, 0, 0};
+    }
+    for (uint32_t j = 0; j < terms; j++) {
+        const uint64_t y = v[j * BT + tid];
+#pragma unroll
+        for (int u = 0; u < OB; u++) mac(acc[u], y, r[u][j], one);
+    }
+}
+
 // (1)-(2): fastbconvmTilde + smMrq.  x: [P][L][N] base q -> out: [P][bsk][N] base Bsk.
 __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restrict__ x, uint64_t *__restrict__ out,
                                                          BehzDev B, const DevPrime *__restrict__ qp, int logn,
@@ -35,6 +60,7 @@ __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restr
     extern __shared__ uint64_t ys[];  // [rows][BT], rows = max(|q|, |B|)
     const size_t n = (size_t)1 << logn;
     const int tid = threadIdx.x;
+    const uint32_t one = opaque_one();
     for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
         const size_t p = idx >> logn, c = idx & (n - 1);
         for (uint32_t j = 0; j < B.L; j++)
@@ -44,18 +70,22 @@ __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restr
         const uint64_t *row_mt = B.qhat_mod_bskm + (size_t)B.bsk * B.L;
         for (uint32_t j = 0; j < B.L; j++) s_mt += ys[j * BT + tid] * row_mt[j];
         const uint64_t r = ((s_mt & 0xffffffffull) * B.m_tilde_neg_inv_q) & 0xffffffffull;
-        for (uint32_t i = 0; i < B.bsk; i++) {
-            const DevPrime &P = B.bsk_primes[i];
-            const uint64_t *row = B.qhat_mod_bskm + (size_t)i * B.L;
-            u128 acc{0, 0};
-            for (uint32_t j = 0; j < B.L; j++) add128(acc, mul64x64(ys[j * BT + tid], row[j]));
-            const uint64_t s = barrett128(acc, P.q, P.ratio_lo, P.ratio_hi);
-            // centred r (m_tilde is a power of two, so r >= m_tilde/2 is negative)
-            const uint64_t rc = r >= 0x80000000ull ? r + P.q - 0x100000000ull : r;
-            u128 v = mul64x64(rc, B.q_mod_bsk[i]);
-            add128(v, u128{s, 0});
-            const uint64_t w = barrett128(v, P.q, P.ratio_lo, P.ratio_hi);
-            out[(p * B.bsk + i) * n + c] = shoup_pair(w, B.inv_mt_bsk + 2 * i, P.q);
+        for (uint32_t i0 = 0; i0 < B.bsk; i0 += OB) {
+            Acc3 acc[OB];
+            dot_rows(acc, ys, B.qhat_mod_bskm, B.L, i0, B.bsk, tid, one);
+#pragma unroll
+            for (int u = 0; u < OB; u++) {
+                const uint32_t i = i0 + u;
+                if (i >= B.bsk) break;
+                const DevPrime &P = B.bsk_primes[i];
+                const uint64_t s = acc_reduce(acc[u], P);
+                // centred r (m_tilde is a power of two, so r >= m_tilde/2 is negative)
+                const uint64_t rc = r >= 0x80000000ull ? r + P.q - 0x100000000ull : r;
+                u128 v = mul64x64(rc, B.q_mod_bsk[i]);
+                add128(v, u128{s, 0});
+                const uint64_t w = barrett128(v, P.q, P.ratio_lo, P.ratio_hi);
+                out[(p * B.bsk + i) * n + c] = shoup_pair(w, B.inv_mt_bsk + 2 * i, P.q);
+            }
         }
     }
 }
@@ -67,18 +97,23 @@ __global__ __launch_bounds__(BT) void behz_floor_kernel(const uint64_t *__restri
     extern __shared__ uint64_t ws[];  // [rows][BT], rows = max(|q|, |B|)
     const size_t n = (size_t)1 << logn;
     const int tid = threadIdx.x;
+    const uint32_t one = opaque_one();
     for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
         const size_t p = idx >> logn, c = idx & (n - 1);
         for (uint32_t j = 0; j < B.L; j++)
             ws[j * BT + tid] = shoup_pair(Tq[(p * B.L + j) * n + c], B.t_qhat_inv + 2 * j, qp[j].q);
-        for (uint32_t i = 0; i < B.bsk; i++) {
-            const DevPrime &P = B.bsk_primes[i];
-            const uint64_t *row = B.qhat_mod_bskm + (size_t)i * B.L;
-            u128 acc{0, 0};
-            for (uint32_t j = 0; j < B.L; j++) add128(acc, mul64x64(ws[j * BT + tid], row[j]));
-            const uint64_t conv = barrett128(acc, P.q, P.ratio_lo, P.ratio_hi);
-            const uint64_t tb = shoup_pair(Tb[(p * B.bsk + i) * n + c], B.t_bsk + 2 * i, P.q);
-            Z[(p * B.bsk + i) * n + c] = shoup_pair(tb + P.q - conv, B.inv_q_bsk + 2 * i, P.q);
+        for (uint32_t i0 = 0; i0 < B.bsk; i0 += OB) {
+            Acc3 acc[OB];
+            dot_rows(acc, ws, B.qhat_mod_bskm, B.L, i0, B.bsk, tid, one);
+#pragma unroll
+            for (int u = 0; u < OB; u++) {
+                const uint32_t i = i0 + u;
+                if (i >= B.bsk) break;
+                const DevPrime &P = B.bsk_primes[i];
+                const uint64_t conv = acc_reduce(acc[u], P);
+                const uint64_t tb = shoup_pair(Tb[(p * B.bsk + i) * n + c], B.t_bsk + 2 * i, P.q);
+                Z[(p * B.bsk + i) * n + c] = shoup_pair(tb + P.q - conv, B.inv_q_bsk + 2 * i, P.q);
+            }
         }
     }
 }
@@ -90,28 +125,34 @@ __global__ __launch_bounds__(BT) void behz_sk_kernel(const uint64_t *__restrict_
     extern __shared__ uint64_t zs[];  // [rows][BT], rows = max(|q|, |B|)
     const size_t n = (size_t)1 << logn;
     const int tid = threadIdx.x;
+    const uint32_t one = opaque_one();
     const DevPrime &Msk = B.bsk_primes[B.bsz];
     for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
         const size_t p = idx >> logn, c = idx & (n - 1);
         for (uint32_t j = 0; j < B.bsz; j++)
             zs[j * BT + tid] = shoup_pair(Z[(p * B.bsk + j) * n + c], B.bhat_inv + 2 * j, B.bsk_primes[j].q);
-        u128 acc{0, 0};
-        for (uint32_t j = 0; j < B.bsz; j++) add128(acc, mul64x64(zs[j * BT + tid], B.bhat_mod_msk[j]));
-        const uint64_t conv = barrett128(acc, Msk.q, Msk.ratio_lo, Msk.ratio_hi);
+        Acc3 acc{0, 0, 0};
+        const_u64 *bm = (const_u64 *)B.bhat_mod_msk;
+        for (uint32_t j = 0; j < B.bsz; j++) mac(acc, zs[j * BT + tid], bm[j], one);
+        const uint64_t conv = acc_reduce(acc, Msk);
         const uint64_t zmsk = Z[(p * B.bsk + B.bsz) * n + c];
         const uint64_t alpha = shoup(conv + B.m_sk - zmsk, B.inv_prod_B_msk, B.inv_prod_B_msk_shoup, B.m_sk);
         const bool neg = alpha > B.m_sk_half;
-        for (uint32_t i = 0; i < B.L; i++) {
-            const DevPrime &Q = qp[i];
-            const uint64_t *row = B.bhat_mod_q + (size_t)i * B.bsz;
-            u128 s{0, 0};
-            for (uint32_t j = 0; j < B.bsz; j++) add128(s, mul64x64(zs[j * BT + tid], row[j]));
-            const uint64_t o = barrett128(s, Q.q, Q.ratio_lo, Q.ratio_hi);
-            // Shenoy-Kumaresan correction with the centred alpha_sk
-            const uint64_t pb = B.prod_B_mod_q[i];
-            u128 v = neg ? mul64x64(B.m_sk - alpha, pb) : mul64x64(alpha, Q.q - pb);
-            add128(v, u128{o, 0});
-            out[(p * B.L + i) * n + c] = barrett128(v, Q.q, Q.ratio_lo, Q.ratio_hi);
+        for (uint32_t i0 = 0; i0 < B.L; i0 += OB) {
+            Acc3 s[OB];
+            dot_rows(s, zs, B.bhat_mod_q, B.bsz, i0, B.L, tid, one);
+#pragma unroll
+            for (int u = 0; u < OB; u++) {
+                const uint32_t i = i0 + u;
+                if (i >= B.L) break;
+                const DevPrime &Q = qp[i];
+                const uint64_t o = acc_reduce(s[u], Q);
+                // Shenoy-Kumaresan correction with the centred alpha_sk
+                const uint64_t pb = B.prod_B_mod_q[i];
+                u128 v = neg ? mul64x64(B.m_sk - alpha, pb) : mul64x64(alpha, Q.q - pb);
+                add128(v, u128{o, 0});
+                out[(p * B.L + i) * n + c] = barrett128(v, Q.q, Q.ratio_lo, Q.ratio_hi);
+            }
         }
     }
 }
