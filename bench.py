@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel at the default workload (batch 128),
 # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB device copy
 # (profiles/mulrelin_pmc_traffic.md); a profiler cannot run inside this process.
-ROWS_MODUP_TRAFFIC_B128 = 38.88e9 + 38.65e9
+KEYPROD_TRAFFIC_B128 = 40.93e9 + 3.36e9
 CKKS = 2
 
 
@@ -229,13 +229,16 @@ def bench_mulrelin(args):
                   "batched": served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, reqs, B),
                   "plain": served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, reqs, 1)}
 
-    # ModUp rows pass: reads and writes every ModUp limb once and reads the forward
-    # twiddle table of each of the L+1 target primes; B x L x L limbs (CKKS: each data
-    # prime's own digit is the NTT-form input limb, so L - 1 digits go to each of the L
-    # data primes and L to the special prime)
+    # The dominant kernel: the ModUp rows pass fused with the products by the key.  It
+    # reads every ModUp limb once (B x L x L: each data prime's own digit is the NTT-form
+    # input limb, so L - 1 digits go to each of the L data primes and L to the special
+    # prime), those B x L input limbs, the key's 2 L (L+1) limbs and the L+1 twiddle
+    # tables once, and writes the 2 B (L+1) product limbs; it runs the last 8 butterfly
+    # layers of every ModUp limb and 2 B L (L+1) N multiply-adds.
     tlimbs = B * L * L
-    rows_bytes = 2 * tlimbs * n * 8 + K * n * 16
+    rows_bytes = (tlimbs + B * L + 2 * L * (L + 1) + 2 * B * (L + 1)) * n * 8 + K * n * 16
     rows_bfly = tlimbs * (n // 2) * (logn - logn // 2)
+    rows_macs = 2 * B * L * (L + 1) * n
     limb_bytes = n * 8
     unit_bytes = 6 * L * limb_bytes + (2 * (K - 1) * K * limb_bytes) / B  # inputs 4L + output 2L + key share
     step_bytes = unit_bytes * B
@@ -253,12 +256,13 @@ def bench_mulrelin(args):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved_rows, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_rows / HBM_PEAK_GBS,
-                     "traffic": ROWS_MODUP_TRAFFIC_B128 if B == 128 else None,
-                     "butterflies_per_launch": rows_bfly,
-                     "kernel": f"ntt_rows_modup_kernel<16,8,4,16,false>: key-switch ModUp NTT rows pass, {tlimbs} limbs "
-                               f"x last {logn - logn // 2} layers",
+                     "traffic": KEYPROD_TRAFFIC_B128 if B == 128 else None,
+                     "butterflies_per_launch": rows_bfly, "macs_per_launch": rows_macs,
+                     "kernel": f"ntt_rows_keyprod_kernel<16,8,4,16,4>: key-switch ModUp NTT rows pass over {tlimbs} "
+                               f"limbs (last {logn - logn // 2} layers) fused with the key products",
                      "algorithmic_bytes": rows_bytes, "avg_ms": rows_ms,
-                     "butterflies_per_s": rows_bfly / (rows_ms * 1e-3)},
+                     "butterflies_per_s": rows_bfly / (rows_ms * 1e-3),
+                     "macs_per_s": rows_macs / (rows_ms * 1e-3)},
         "step_roofline": {"achieved": achieved_step, "unit": "GB/s", "frac": achieved_step / HBM_PEAK_GBS,
                           "algorithmic_bytes_per_op": unit_bytes},
         "plain": plain,

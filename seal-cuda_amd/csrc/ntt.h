@@ -50,6 +50,17 @@ bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint3
                        const DevPrime *targets, const DevPrime *digits, int logn, bool skip_diag, bool reduce_4q,
                        hipStream_t stream, const KernelProbe *probe = nullptr);
 
+// The same ModUp with its rows pass fused into the products by the key (N = 2^15, 2^16;
+// returns false and does nothing otherwise): the columns pass fills T as above (diag !=
+// nullptr: skip_diag), then P[ct][k][i] = sum_j NTT_i(d_j mod t_i) key[j][k][ki] for
+// i < L and S[ct][k] for the special target (ki = key_count - 1), the own-prime terms
+// read from diag + ct * diag_stride + i * N.  key: [L.. ][2][key_count][N] rows of the
+// key-switching key.  With a probe, its events bracket the fused launch.
+bool ntt_modup_keyprod(const uint64_t *d, size_t d_ct_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
+                       uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t count, uint32_t L,
+                       uint32_t key_count, const DevPrime *targets, const DevPrime *digits, int logn,
+                       hipStream_t stream, const KernelProbe *probe = nullptr);
+
 // ModDown by the special prime p fused into the last transforms of a key switch, for
 // `count` ciphertexts at L limbs (N >= 2^13; returns false and does nothing otherwise):
 //   u_i = ([s + floor(p/2)]_p mod q_i) - (floor(p/2) mod q_i),  ct_k[i] += (P_k[i] - u_i) p^-1

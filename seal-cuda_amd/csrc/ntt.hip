@@ -25,6 +25,7 @@
 // Twiddle indexing: forward layer l (m = 2^l groups) uses root_powers[m + group];
 // inverse layer l uses inv_root_powers[N - 2m + 1 + group], group = x >> (logn - l).
 #include "ntt.h"
+#include "accum.h"
 #include "butterfly.h"
 
 namespace shc {
@@ -215,6 +216,111 @@ __device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, u
     auto load = [&](int idx) { return load_elem(i, j, (size_t)idx * C + c); };
     auto store = [&](int idx, uint64_t x) { store_elem(i, j, (size_t)idx * C + c, x); };
     run_phases<LOGN, K, E, INV, false>(tile, addr, load, store, t, 0, 0, tw, P.q << 2, 0 - P.q, true);
+}
+
+// The key switch's ModUp rows pass fused with the products by the key (N = 2^15, 2^16):
+// a block owns row g of target prime i for CG ciphertexts.  Each step transforms SUBS
+// rows of T -- CG ciphertexts x SUBS/CG digits, the last K layers with the wide
+// schedule -- stages them in LDS, and then every thread multiplies its coefficient of
+// those rows by the key words of their digits and adds into three-word sums (one per
+// ciphertext and key component), so the transformed limbs never return to HBM:
+//   P[ct][k][i][c] = sum_j NTT_i(T[i][ct][j])[c] * key[j][k][ki][c]   (S for i == L).
+// With diag (CKKS, T without its own-prime limbs), the term j == i is the NTT-form input
+// limb diag[ct * diag_stride + i * N].  The groups start at ciphertext ct_base.  A step
+// past the last digit, or a ciphertext past the batch, transforms a repeated row and
+// does not add or store it.  Blocks of one
+// (i, g) run one after another on one XCD, so its key slice comes from that XCD's L2.
+// Sum bound: up to 31 digits the transformed words (< 8q) enter the sums directly
+// (31 * 8q^2 + q^2 < 2^128 for q < 2^60); beyond, reduce_4q takes them below 4q first.
+template <int LOGN, int K, int E, int SUBS, int CG>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
+    const uint64_t *__restrict__ T, const uint64_t *__restrict__ key, uint64_t *__restrict__ Pout,
+    uint64_t *__restrict__ Sout, const DevPrime *__restrict__ primes, const uint64_t *__restrict__ diag,
+    size_t diag_stride, uint32_t count, uint32_t L, uint32_t K_key, uint32_t ct_base, uint32_t groups, uint32_t tiles,
+    bool reduce_4q) {
+    constexpr int TT = 1 << (K - E);  // threads per row
+    constexpr int SZ = 1 << K;        // row length
+    static_assert(SZ == SUBS * TT, "one coefficient per thread in the product phase");
+    static_assert(SUBS % CG == 0, "whole digits per step");
+    constexpr int RPL = LOGN - K;
+    constexpr int DJ = SUBS / CG;  // digits per step
+    __shared__ uint64_t tile[SUBS * SZ];
+    const uint32_t xcd = blockIdx.x & 7, kk = blockIdx.x >> 3;
+    const uint32_t tl = (kk / groups) * 8 + xcd;
+    if (tl >= tiles) return;  // uniform: the whole block
+    const uint32_t grp = kk % groups;
+    const uint32_t i = tl >> RPL;
+    const int g = (int)(tl & ((1u << RPL) - 1));
+    const uint32_t ct0 = ct_base + grp * CG;
+    const uint32_t gv = min<uint32_t>(CG, count - ct0);
+    const bool short_row = diag && i < L;
+    const uint32_t Lt = short_row ? L - 1 : L;
+    const int tid = threadIdx.x;
+    const int sub = tid / TT, t = tid % TT;
+    const int gg = sub % CG, dj = sub / CG;  // this thread's transform row: ciphertext gg, digit dj
+    const size_t n = (size_t)1 << LOGN;
+    const size_t row_off = (size_t)g << K;
+    const DevPrime &P = primes[i];
+    const uint64_t four_q = P.q << 2;
+    const uint32_t ki = i == L ? K_key - 1 : i;
+    const size_t kstride = (size_t)K_key << LOGN;  // key[j][k] -> key[j][k+1]
+    const uint64_t *kp = key + ((size_t)ki << LOGN) + row_off + tid;
+    const uint32_t one = opaque_one();
+    Acc3 a0[CG], a1[CG];
+#pragma unroll
+    for (int c = 0; c < CG; c++) a0[c] = a1[c] = Acc3{0, 0, 0};
+    const uint64_t *trow =
+        T + (((size_t)i * count * L + (size_t)(ct0 + min<uint32_t>(gg, gv - 1)) * Lt) << LOGN) + row_off;
+    auto addr = [sub](int idx) { return sub * SZ + swz(idx); };
+    auto stage = [&](int idx, uint64_t x) { tile[addr(idx)] = x; };
+    const uint32_t steps = (Lt + DJ - 1) / DJ;
+    for (uint32_t s = 0; s < steps; s++) {
+        const uint64_t *row = trow + ((size_t)min<uint32_t>(s * DJ + dj, Lt - 1) << LOGN);
+        auto load = [row](int idx) { return row[idx]; };
+        if (s) __syncthreads();  // the previous step's products have read the tile
+        // the twiddles are the same in every step; reloading them (L1/L2 hits) keeps them
+        // from being held in registers across steps
+        const ulonglong2 *tw = P.fwd;
+        asm volatile("" : "+s"(tw));
+        run_phases<LOGN, K, E, false, true>(tile, addr, load, stage, t, g, RPL, tw, four_q, 0 - P.q, true);
+        __syncthreads();
+#pragma unroll 1
+        for (int d = 0; d < DJ; d++) {
+            const uint32_t jt = s * DJ + d;
+            if (jt >= Lt) break;  // uniform
+            const uint32_t j = short_row && jt >= i ? jt + 1 : jt;
+            const uint64_t k0 = kp[(2 * j) * kstride], k1 = kp[(2 * j + 1) * kstride];
+#pragma unroll
+            for (int c = 0; c < CG; c++) {
+                uint64_t x = tile[(d * CG + c) * SZ + swz(tid)];
+                if (reduce_4q) x = sub_if_ge(x, four_q);
+                mac(a0[c], x, k0, one);
+                mac(a1[c], x, k1, one);
+            }
+        }
+    }
+    if (short_row) {
+        const uint64_t k0 = kp[(2 * i) * kstride], k1 = kp[(2 * i + 1) * kstride];
+#pragma unroll
+        for (int c = 0; c < CG; c++) {
+            const uint64_t x = diag[(ct0 + min<uint32_t>(c, gv - 1)) * diag_stride + ((size_t)i << LOGN) + row_off + tid];
+            mac(a0[c], x, k0, one);
+            mac(a1[c], x, k1, one);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CG; c++) {
+        if ((uint32_t)c >= gv) break;
+        const size_t ct = ct0 + c;
+        const uint64_t r0 = acc_reduce(a0[c], P), r1 = acc_reduce(a1[c], P);
+        if (i == L) {
+            Sout[(ct * 2 + 0) * n + row_off + tid] = r0;
+            Sout[(ct * 2 + 1) * n + row_off + tid] = r1;
+        } else {
+            Pout[((ct * 2 + 0) * L + i) * n + row_off + tid] = r0;
+            Pout[((ct * 2 + 1) * L + i) * n + row_off + tid] = r1;
+        }
+    }
 }
 
 // Plain in-place passes over a LimbGrid buffer.
@@ -564,11 +670,61 @@ bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint3
 }
 
 namespace {
+template <int LOGN>
+void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
+                          uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t cts, uint32_t L,
+                          uint32_t K_key, const DevPrime *targets, const DevPrime *digits, const KernelProbe *probe,
+                          hipStream_t st) {
+    constexpr int K1 = LOGN / 2;
+    constexpr int K2 = LOGN - K1;
+    constexpr int E = 4;
+    constexpr int CS = 16;
+    constexpr int RS = 4096 >> K2;
+    const uint32_t limbs = cts * L * (L + 1);
+    ntt_cols_modup_kernel<LOGN, K1, E, CS>
+        <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L, diag != nullptr);
+    const uint32_t tiles = (L + 1) << K1;
+    const uint64_t tile_slots = (uint64_t)((tiles + 7) / 8) * 8;
+    if (tile_slots * ((cts + 3) / 4) >= (uint64_t(1) << 32))
+        fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
+    // groups of 4 ciphertexts, then the rest as a group of 2 and/or 1, so that no row is
+    // transformed for a ciphertext that is not there
+    if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
+    const uint32_t g4 = cts / 4, rest = cts % 4;
+    if (g4)
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 4><<<(uint32_t)(tile_slots * g4), RS << (K2 - E), 0, st>>>(
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, 0, g4, tiles, L > 31);
+    if (rest >= 2)
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 2><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, g4 * 4, 1, tiles, L > 31);
+    if (rest & 1)
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 1><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, cts - 1, 1, tiles, L > 31);
+    if (probe) SHC_HIP(hipEventRecord(probe->end, st));
+}
+
 LimbGrid poly_grid(uint32_t count, uint32_t lpp) {
     if (lpp == 0 || count % lpp) fail(ERR_INVALID_ARGUMENT, "limb count must be a multiple of limbs per polynomial");
     return LimbGrid{1, lpp, lpp, count / lpp};
 }
 }  // namespace
+
+bool ntt_modup_keyprod(const uint64_t *d, size_t d_ct_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
+                       uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t count, uint32_t L,
+                       uint32_t key_count, const DevPrime *targets, const DevPrime *digits, int logn,
+                       hipStream_t st, const KernelProbe *probe) {
+    if ((logn != 15 && logn != 16) || count == 0) return false;
+    if ((uint64_t)count * L * (L + 1) > ((uint64_t(1) << 34) >> logn))
+        fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
+    if (logn == 15)
+        launch_modup_keyprod<15>(d, d_ct_stride, T, key, P, S, diag, diag_stride, count, L, key_count, targets,
+                                 digits, probe, st);
+    else
+        launch_modup_keyprod<16>(d, d_ct_stride, T, key, P, S, diag, diag_stride, count, L, key_count, targets,
+                                 digits, probe, st);
+    launch_check("ntt modup keyprod launch");
+    return true;
+}
 
 bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t count, const DevPrime *primes, int logn,
                  hipStream_t st) {

@@ -320,9 +320,13 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
         digits = d;
         dstride = (size_t)L * n;
     }
-    // 2-4. ModUp and the inner product with the key
-    const bool skipped = modup_to_T(c, digits, dstride, T, count, L, st, probe, diag != nullptr);
-    keyswitch_inner_product(c, T, key, P, S, count, L, st, skipped ? diag : nullptr, target_stride);
+    // 2-4. ModUp and the inner product with the key: fused past the columns pass where
+    // the transform shape allows (N = 2^15, 2^16), else T goes through HBM once more
+    if (!ntt_modup_keyprod(digits, dstride, T, key, P, S, diag, target_stride, count, L, (uint32_t)c.key_count(),
+                           c.keyswitch_primes(L), c.primes(), logn, st, probe)) {
+        const bool skipped = modup_to_T(c, digits, dstride, T, count, L, st, probe, diag != nullptr);
+        keyswitch_inner_product(c, T, key, P, S, count, L, st, skipped ? diag : nullptr, target_stride);
+    }
     // 5. ModDown by the special prime
     moddown_into(c, P, S, U, ct, ct_stride, count, L, st);
 }

@@ -57,6 +57,11 @@ def test_ckks_multiply(logn, bits, L, count):
     (CKKS, 15, [60] + [40] * 14 + [60], 15, 1),
     # 60-bit primes at 20 digits, and 5 ciphertexts: a partial last group of 4
     (CKKS, 13, [60] * 21, 20, 5),
+    # the ModUp rows pass fused with the key products (N = 2^15, 2^16): groups of 4
+    # ciphertexts with a partial last one, CKKS (own-prime limbs from the input) and BFV
+    (CKKS, 15, [60] * 9, 8, 6),
+    (BFV, 15, [60] * 7, 6, 3),
+    (CKKS, 16, [60] * 4, 3, 5),
 ])
 def test_relinearize(scheme, logn, bits, L, count):
     ctx, moduli = make_ctx(scheme, logn, bits, 65537 if scheme == BFV else 0)
