@@ -271,8 +271,6 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
     for (int c = 0; c < CG; c++) a0[c] = a1[c] = Acc3{0, 0, 0};
     const uint64_t *trow =
         T + (((size_t)i * count * L + (size_t)(ct0 + min<uint32_t>(gg, gv - 1)) * Lt) << LOGN) + row_off;
-    auto addr = [sub](int idx) { return sub * SZ + swz(idx); };
-    auto stage = [&](int idx, uint64_t x) { tile[addr(idx)] = x; };
     const uint32_t steps = (Lt + DJ - 1) / DJ;
     for (uint32_t s = 0; s < steps; s++) {
         const uint64_t *row = trow + ((size_t)min<uint32_t>(s * DJ + dj, Lt - 1) << LOGN);
@@ -282,7 +280,14 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
         // from being held in registers across steps
         const ulonglong2 *tw = P.fwd;
         asm volatile("" : "+s"(tw));
-        run_phases<LOGN, K, E, false, true>(tile, addr, load, stage, t, g, RPL, tw, four_q, 0 - P.q, true);
+        // small groups (short, latency-bound launches) also recompute their LDS addresses
+        // each step, for fewer registers and more waves; a group of 4 keeps them (fewer
+        // instructions; measured faster)
+        int sub_s = sub, t_s = t;
+        if constexpr (CG < 4) asm volatile("" : "+v"(sub_s), "+v"(t_s));
+        auto addr_s = [sub_s](int idx) { return sub_s * SZ + swz(idx); };
+        auto stage_s = [&](int idx, uint64_t x) { tile[addr_s(idx)] = x; };
+        run_phases<LOGN, K, E, false, true>(tile, addr_s, load, stage_s, t_s, g, RPL, tw, four_q, 0 - P.q, true);
         __syncthreads();
 #pragma unroll 1
         for (int d = 0; d < DJ; d++) {
