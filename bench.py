@@ -647,13 +647,16 @@ def bench_matvec(args):
 class MatvecServer:
     """One request type of the serving workload: a context, one matrix's diagonals (NTT
     form, synthetic residues), resident Galois keys (synthetic residues; the baby steps
-    also prepared for hoisting), a pool of resident input ciphertexts, and batch buffers
-    for up to max_batch requests."""
+    also prepared for hoisting), a pool of resident input ciphertexts, and batch buffers.
+    Admission by HBM footprint: the batch buffers of m requests take m ciphertexts each
+    for the input, the output, the b baby steps and the g inner sums, plus the key-switch
+    workspace of m ciphertexts; max_batch is the largest m <= cap whose buffers fit in
+    `budget` bytes (keys and diagonals are resident and counted apart)."""
 
-    def __init__(self, torch, sealhip, matvec, scheme, n, moduli, plain, b, g, pool, max_batch, gen, st):
+    def __init__(self, torch, sealhip, matvec, scheme, n, moduli, plain, b, g, pool, cap, budget, gen, st):
         K = len(moduli)
         L = K - 1
-        self.torch, self.n, self.L, self.st, self.max_batch = torch, n, L, st, max_batch
+        self.torch, self.n, self.L, self.st = torch, n, L, st
         self.ctx = sealhip.Context(scheme, n, moduli, plain)
         ctx = self.ctx
         self.plains = torch.empty((g, b, L, n), dtype=torch.int64, device="cuda")
@@ -679,6 +682,16 @@ class MatvecServer:
                                     bfv=scheme == sealhip.SCHEME_BFV)
         self.pool = torch.empty((pool, 2, L, n), dtype=torch.int64, device="cuda")
         fill_uniform(torch, self.pool, moduli[:L], 2, gen)
+        ct_bytes = 2 * L * n * 8
+        footprint = lambda m: m * ct_bytes * (2 + b + g) + self.mv.workspace_sizes(m)
+        max_batch = cap
+        while max_batch > 1 and footprint(max_batch) > budget:
+            max_batch -= 1
+        self.max_batch = max_batch
+        self.batch_bytes = footprint(max_batch)
+        self.per_request_bytes = footprint(2) - footprint(1)
+        self.resident_bytes = (self.plains.numel() + sum(k.numel() for k in self.baby + self.giant)
+                               + sum(h.numel() for h in self.hoisted)) * 8
         self.x = torch.empty((max_batch, 2, L, n), dtype=torch.int64, device="cuda")
         self.out = torch.empty_like(self.x)
         self.babybuf = torch.empty((b, max_batch, 2, L, n), dtype=torch.int64, device="cuda")
@@ -699,9 +712,10 @@ def serve_mixed(torch, servers, rate, requests, max_batch, seed):
     """Open-loop serving of a mixed stream: Poisson arrivals at `rate` per second, each
     request of type 0 or 1 with probability 1/2 and an input drawn from that type's pool.
     The server picks the type whose oldest waiting request arrived first, runs every
-    waiting request of that type (at most max_batch) as one batch, and waits for it; a
-    request's latency is its batch's completion minus its arrival.  max_batch = 1 is the
-    plain one-request-at-a-time path under the same arrivals."""
+    waiting request of that type (at most max_batch, and at most what that type's
+    admission allows) as one batch, and waits for it; a request's latency is its batch's
+    completion minus its arrival.  max_batch = 1 is the plain one-request-at-a-time path
+    under the same arrivals."""
     import collections
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -723,7 +737,7 @@ def serve_mixed(torch, servers, rate, requests, max_batch, seed):
             time.sleep(min(arrivals[nxt] - now, 1e-3))
             continue
         ty = 0 if not pending[1] or (pending[0] and pending[0][0] < pending[1][0]) else 1
-        batch = [pending[ty].popleft() for _ in range(min(max_batch, len(pending[ty])))]
+        batch = [pending[ty].popleft() for _ in range(min(max_batch, servers[ty].max_batch, len(pending[ty])))]
         servers[ty].run([int(inputs[r]) for r in batch])
         torch.cuda.synchronize()
         fin = time.perf_counter() - t0
@@ -757,29 +771,31 @@ def bench_serve(args):
     st = torch.cuda.current_stream().cuda_stream
     gen = torch.Generator(device="cuda").manual_seed(8)
     B = 16
+    budget = 8 << 30  # HBM for each request type's batch buffers
     bfv_moduli = [0x7fffffd8001, 0x7fffffc8001, 0xfffffffc001, 0xffffff6c001, 0xfffffebc001]
     ckks_moduli = sealhip.coeff_modulus_create(1 << 15, [60] + [40] * 14 + [60])
     servers = [MatvecServer(torch, sealhip, matvec, sealhip.SCHEME_BFV, 8192, bfv_moduli, 65537, 64, 64, 32, B,
-                            gen, st),
+                            budget, gen, st),
                MatvecServer(torch, sealhip, matvec, sealhip.SCHEME_CKKS, 1 << 15, ckks_moduli, 0, 32, 32, 32, B,
-                            gen, st)]
+                            budget, gen, st)]
     # capacity with full batches, per type
-    full = list(range(B))
     cap = []
     for sv in servers:
+        full = list(range(sv.max_batch))
         for _ in range(max(args.warmup, 1)):
             sv.run(full)
         with Timer(torch) as t:
             for _ in range(args.steps):
                 sv.run(full)
-        cap.append(B * args.steps / (t.ms * 1e-3))
+        cap.append(sv.max_batch * args.steps / (t.ms * 1e-3))
     mix_cap = 1.0 / (0.5 / cap[0] + 0.5 / cap[1])
     # determinism: a request's words in a full batch equal its words alone
     for sv, name in zip(servers, ("bfv", "ckks")):
-        sv.run(full)
+        last = sv.max_batch - 1
+        sv.run(list(range(sv.max_batch)))
         torch.cuda.synchronize()
-        batched = [sv.out[0].clone(), sv.out[B - 1].clone()]
-        for k, i in ((0, 0), (1, B - 1)):
+        batched = [sv.out[0].clone(), sv.out[last].clone()]
+        for k, i in ((0, 0), (1, last)):
             sv.run([i])
             torch.cuda.synchronize()
             if not torch.equal(sv.out[0], batched[k]):
@@ -795,11 +811,19 @@ def bench_serve(args):
             "data": "synthetic (seeded uniform residues for ciphertexts, diagonals and keys; Poisson arrivals)",
             "config": {"workload": "serving: 50% BFV matvec (N=8192, BFVDefault(8192), t=65537, 4096x4096, "
                                    "BSGS 64x64), 50% CKKS matvec (N=2^15, 60+40x14+60, 1024x1024, BSGS 32x32), "
-                                   f"batches of at most {B}", "parallelism": "single GPU"},
+                                   f"batches of at most {B} within {budget >> 30} GiB of batch buffers per type",
+                       "parallelism": "single GPU"},
             "capacity_per_s": {"bfv": cap[0], "ckks": cap[1], "mix": mix_cap},
+            "admission": {"budget_bytes_per_type": budget,
+                          "max_batch": {"bfv": servers[0].max_batch, "ckks": servers[1].max_batch},
+                          "per_request_bytes": {"bfv": servers[0].per_request_bytes,
+                                                "ckks": servers[1].per_request_bytes},
+                          "batch_bytes": {"bfv": servers[0].batch_bytes, "ckks": servers[1].batch_bytes},
+                          "resident_key_and_diagonal_bytes": {"bfv": servers[0].resident_bytes,
+                                                              "ckks": servers[1].resident_bytes}},
             "arrivals": {"kind": "poisson", "rate_per_s": rate, "requests": reqs},
             "batched": batched, "plain": plain, "batched_p99_over_plain_p99": batched["p99_ms"] / plain["p99_ms"],
-            "self_check": "per type, requests 0 and 15 give the same words alone and in a full batch"}
+            "self_check": "per type, the first and last request of a full batch give the same words alone"}
 
 
 def bench_copy(args):

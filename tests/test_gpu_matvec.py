@@ -213,3 +213,41 @@ def test_bsgs_matvec_bfv_words_and_values():
     ctx.batch_decode(dec.data_ptr(), vals.data_ptr(), scratch.data_ptr(), stream())
     mv_want = (M.astype(object).dot(v.astype(object)) % T).astype(np.uint64)
     np.testing.assert_array_equal(from_dev(vals), np.tile(mv_want, n // d))
+
+
+def test_bsgs_matvec_ckks_batch_equals_alone():
+    """A request's words are the same in a batch of three as computed alone (uniform
+    words stand in for encryptions, diagonals and keys)."""
+    import matvec
+    logn, n = 12, 1 << 12
+    moduli = ck.coeff_modulus_create(n, [60, 40, 40, 60])
+    K = len(moduli)
+    L = K - 1
+    ctx = hip().Context(CKKS, n, moduli)
+    b = g = 4
+    rng = np.random.default_rng(41)
+    plains = to_dev(ck.uniform_residues(rng, (g, b, L, n), moduli[:L]))
+    baby = [to_dev(ck.random_key(rng, moduli, logn)) for _ in range(1, b)]
+    giant = [to_dev(ck.random_key(rng, moduli, logn)) for _ in range(1, g)]
+    hb, hw = ctx.hoisted_key_size(L), ctx.hoisted_key_workspace_size(L)
+    hws = ws(hw)
+    hoisted = []
+    for j, k in zip(range(1, b), baby):
+        h = empty_dev(((hb + 7) // 8,))
+        ctx.hoisted_key_prepare(k.data_ptr(), ctx.galois_elt_from_step(j), L, h.data_ptr(), hws.data_ptr(), hw,
+                                stream())
+        hoisted.append(h)
+    mv = matvec.BsgsMatvec(ctx, n, L, b, g, plains.data_ptr(), [k.data_ptr() for k in baby],
+                           [h.data_ptr() for h in hoisted], [k.data_ptr() for k in giant])
+    cts = to_dev(ck.uniform_residues(rng, (3, 2, L, n), moduli[:L]))
+
+    def run(x, count):
+        out, bb, inner = empty_dev((count, 2, L, n)), empty_dev((b, count, 2, L, n)), empty_dev((g, count, 2, L, n))
+        wsz = mv.workspace_sizes(count)
+        wb = ws(wsz)
+        mv.run(x.data_ptr(), count, out.data_ptr(), bb.data_ptr(), inner.data_ptr(), wb.data_ptr(), wsz, stream())
+        return from_dev(out)
+
+    batched = run(cts, 3)
+    for k in range(3):
+        np.testing.assert_array_equal(run(cts[k:k + 1].contiguous(), 1)[0], batched[k])
