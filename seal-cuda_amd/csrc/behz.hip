@@ -39,7 +39,7 @@ constexpr int OB = 4;  // output primes per pass over the staged inputs
 // space, so that they become scalar loads.
 typedef const __attribute__((address_space(4))) uint64_t const_u64;
 __device__ __forceinline__ void dot_rows(Acc3 (&acc)[OB], const uint64_t *v, const uint64_t *__restrict__ rows,
-                                         uint32_t terms, uint32_t o0, uint32_t outputs, int tid, uint32_t one) {
+                                         uint32_t terms, uint32_t o0, uint32_t outputs, int tid) {
     const_u64 *r[OB];
 #pragma unroll
     for (int u = 0; u < OB; u++) {
@@ -49,7 +49,7 @@ __device__ __forceinline__ void dot_rows(Acc3 (&acc)[OB], const uint64_t *v, con
     for (uint32_t j = 0; j < terms; j++) {
         const uint64_t y = v[j * BT + tid];
 #pragma unroll
-        for (int u = 0; u < OB; u++) mac(acc[u], y, r[u][j], one);
+        for (int u = 0; u < OB; u++) mac<true>(acc[u], y, r[u][j]);
     }
 }
 
@@ -60,7 +60,6 @@ __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restr
     extern __shared__ uint64_t ys[];  // [rows][BT], rows = max(|q|, |B|)
     const size_t n = (size_t)1 << logn;
     const int tid = threadIdx.x;
-    const uint32_t one = opaque_one();
     for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
         const size_t p = idx >> logn, c = idx & (n - 1);
         for (uint32_t j = 0; j < B.L; j++)
@@ -72,7 +71,7 @@ __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restr
         const uint64_t r = ((s_mt & 0xffffffffull) * B.m_tilde_neg_inv_q) & 0xffffffffull;
         for (uint32_t i0 = 0; i0 < B.bsk; i0 += OB) {
             Acc3 acc[OB];
-            dot_rows(acc, ys, B.qhat_mod_bskm, B.L, i0, B.bsk, tid, one);
+            dot_rows(acc, ys, B.qhat_mod_bskm, B.L, i0, B.bsk, tid);
 #pragma unroll
             for (int u = 0; u < OB; u++) {
                 const uint32_t i = i0 + u;
@@ -97,14 +96,13 @@ __global__ __launch_bounds__(BT) void behz_floor_kernel(const uint64_t *__restri
     extern __shared__ uint64_t ws[];  // [rows][BT], rows = max(|q|, |B|)
     const size_t n = (size_t)1 << logn;
     const int tid = threadIdx.x;
-    const uint32_t one = opaque_one();
     for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
         const size_t p = idx >> logn, c = idx & (n - 1);
         for (uint32_t j = 0; j < B.L; j++)
             ws[j * BT + tid] = shoup_pair(Tq[(p * B.L + j) * n + c], B.t_qhat_inv + 2 * j, qp[j].q);
         for (uint32_t i0 = 0; i0 < B.bsk; i0 += OB) {
             Acc3 acc[OB];
-            dot_rows(acc, ws, B.qhat_mod_bskm, B.L, i0, B.bsk, tid, one);
+            dot_rows(acc, ws, B.qhat_mod_bskm, B.L, i0, B.bsk, tid);
 #pragma unroll
             for (int u = 0; u < OB; u++) {
                 const uint32_t i = i0 + u;
@@ -125,7 +123,6 @@ __global__ __launch_bounds__(BT) void behz_sk_kernel(const uint64_t *__restrict_
     extern __shared__ uint64_t zs[];  // [rows][BT], rows = max(|q|, |B|)
     const size_t n = (size_t)1 << logn;
     const int tid = threadIdx.x;
-    const uint32_t one = opaque_one();
     const DevPrime &Msk = B.bsk_primes[B.bsz];
     for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
         const size_t p = idx >> logn, c = idx & (n - 1);
@@ -133,14 +130,14 @@ __global__ __launch_bounds__(BT) void behz_sk_kernel(const uint64_t *__restrict_
             zs[j * BT + tid] = shoup_pair(Z[(p * B.bsk + j) * n + c], B.bhat_inv + 2 * j, B.bsk_primes[j].q);
         Acc3 acc{0, 0, 0};
         const_u64 *bm = (const_u64 *)B.bhat_mod_msk;
-        for (uint32_t j = 0; j < B.bsz; j++) mac(acc, zs[j * BT + tid], bm[j], one);
+        for (uint32_t j = 0; j < B.bsz; j++) mac<true>(acc, zs[j * BT + tid], bm[j]);
         const uint64_t conv = acc_reduce(acc, Msk);
         const uint64_t zmsk = Z[(p * B.bsk + B.bsz) * n + c];
         const uint64_t alpha = shoup(conv + B.m_sk - zmsk, B.inv_prod_B_msk, B.inv_prod_B_msk_shoup, B.m_sk);
         const bool neg = alpha > B.m_sk_half;
         for (uint32_t i0 = 0; i0 < B.L; i0 += OB) {
             Acc3 s[OB];
-            dot_rows(s, zs, B.bhat_mod_q, B.bsz, i0, B.L, tid, one);
+            dot_rows(s, zs, B.bhat_mod_q, B.bsz, i0, B.L, tid);
 #pragma unroll
             for (int u = 0; u < OB; u++) {
                 const uint32_t i = i0 + u;

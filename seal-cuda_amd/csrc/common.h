@@ -18,6 +18,9 @@ struct alignas(64) DevPrime {
     uint64_t inv_n, inv_n_shoup;     // N^-1 mod q as a Shoup operand
     const ulonglong2 *fwd;           // [N]  (w, floor(w 2^64 / q))
     const ulonglong2 *inv;           // [N]
+    // 2^64 - 4q and 2^64 - 8q: the NTT's conditional subtractions as one 64-bit add
+    // (a constant the compiler cannot see as a negation stays an add)
+    uint64_t neg_four_q, neg_eight_q;
 };
 
 // Constants of a divide-and-round by a prime p into prime q_i (p = the special prime

@@ -12,6 +12,8 @@ DevPrime upload_prime(Context &c, uint64_t q) {
     DevPrime p{};
     p.q = t.q;
     p.two_q = t.q << 1;
+    p.neg_four_q = 0 - (t.q << 2);
+    p.neg_eight_q = 0 - (t.q << 3);
     hm::barrett_ratio(t.q, p.ratio_lo, p.ratio_hi);
     p.inv_n = t.inv_n;
     p.inv_n_shoup = t.inv_n_shoup;

@@ -88,7 +88,6 @@ __global__ void plain_dot_many_kernel(const uint64_t *__restrict__ cts, const ui
                                       uint64_t *__restrict__ out, const DevPrime *__restrict__ primes, uint32_t terms,
                                       uint32_t h0, uint32_t hn, uint32_t L, int logn, size_t total) {
     const size_t lw = (size_t)L << logn;
-    const uint32_t one = opaque_one();
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
         const size_t r = idx % lw;
         const DevPrime &P = primes[r >> logn];
@@ -101,7 +100,7 @@ __global__ void plain_dot_many_kernel(const uint64_t *__restrict__ cts, const ui
         for (uint32_t j = 0; j < terms; j++) {
             const uint64_t t = cts[(size_t)j * total + idx];
 #pragma unroll
-            for (int h = 0; h < H; h++) mac(acc[h], t, pp[h][(size_t)j * lw], one);
+            for (int h = 0; h < H; h++) mac(acc[h], t, pp[h][(size_t)j * lw]);
             if ((j & 63) == 63) {
 #pragma unroll
                 for (int h = 0; h < H; h++) acc_fold(acc[h], P);

@@ -51,12 +51,20 @@ __device__ __forceinline__ int phase_index(int t, int j) {
     return ((other >> LOWB) << (K - S0)) | (jl << LOWB) | (other & ((1 << LOWB) - 1));
 }
 
+// A prime's butterfly constants: 4q, 2^64 - q, 2^64 - 4q, 2^64 - 8q.
+struct BfConst {
+    uint64_t four_q, neg_q, neg_four_q, neg_eight_q;
+};
+__device__ __forceinline__ BfConst bf_const(const DevPrime &P) {
+    return BfConst{P.q << 2, 0 - P.q, P.neg_four_q, P.neg_eight_q};
+}
+
 // One phase on the thread's registers.  g = index of this sub-transform among the
 // 2^L0 sub-transforms the full transform splits into at global layer L0.  W selects
 // the wide forward schedule (q < 2^60).
 template <int LOGN, int K, int E, int S0, int EP, bool INV, bool W>
 __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int g, int L0,
-                                              const ulonglong2 *__restrict__ tw, uint64_t four_q, uint64_t neg_q) {
+                                              const ulonglong2 *__restrict__ tw, const BfConst &bc) {
 #pragma unroll
     for (int ss = 0; ss < EP; ss++) {
         const int s = INV ? (S0 + EP - 1 - ss) : (S0 + ss);
@@ -72,7 +80,7 @@ __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int 
             const int l = L0 + s;
             const ulonglong2 w = load_tw(tw, INV ? (1 << LOGN) - (2 << l) + 1 + grp : (1 << l) + grp);
             const bool red = !W || (s >= 2 && !(s & 1));
-            butterfly<INV>(r[j], r[j | (1 << b)], w, four_q, neg_q, red, W ? four_q << 1 : four_q);
+            butterfly<INV>(r[j], r[j | (1 << b)], w, bc.four_q, bc.neg_q, red, W ? bc.neg_eight_q : bc.neg_four_q);
         }
     }
 }
@@ -84,7 +92,7 @@ __device__ __forceinline__ void phase_compute(uint64_t (&r)[1 << E], int t, int 
 template <int LOGN, int K, int E, bool INV, bool W, int P = 0, class Addr, class Load, class Store>
 __device__ __forceinline__ void run_phases(uint64_t *tile, const Addr &addr, const Load &load, const Store &store,
                                            int t, int g, int L0, const ulonglong2 *__restrict__ tw,
-                                           uint64_t four_q, uint64_t neg_q, bool active) {
+                                           const BfConst &bc, bool active) {
     constexpr int NP = (K + E - 1) / E;
     if constexpr (P < NP) {
         // forward runs phases from layer 0 upwards, inverse from the top layer down
@@ -98,17 +106,17 @@ __device__ __forceinline__ void run_phases(uint64_t *tile, const Addr &addr, con
                 const int idx = phase_index<K, E, S0, EP>(t, j);
                 r[j] = P == 0 ? load(idx) : tile[addr(idx)];
             }
-            phase_compute<LOGN, K, E, S0, EP, INV, W>(r, t, g, L0, tw, four_q, neg_q);
+            phase_compute<LOGN, K, E, S0, EP, INV, W>(r, t, g, L0, tw, bc);
 #pragma unroll
             for (int j = 0; j < (1 << E); j++) {
                 const int idx = phase_index<K, E, S0, EP>(t, j);
-                if (P == NP - 1) store(idx, W ? sub_if_ge(r[j], four_q << 1) : r[j]);  // wide: [0,16q) -> [0,8q)
+                if (P == NP - 1) store(idx, W ? sub_if_ge_neg(r[j], bc.neg_eight_q) : r[j]);  // wide: [0,16q) -> [0,8q)
                 else tile[addr(idx)] = r[j];
             }
         }
         if constexpr (P + 1 < NP) {
             __syncthreads();
-            run_phases<LOGN, K, E, INV, W, P + 1>(tile, addr, load, store, t, g, L0, tw, four_q, neg_q, active);
+            run_phases<LOGN, K, E, INV, W, P + 1>(tile, addr, load, store, t, g, L0, tw, bc, active);
         }
     }
 }
@@ -168,12 +176,12 @@ __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, u
     auto load = [&](int idx) { return load_elem(i, j, row_off + idx); };
     if constexpr (NP == 1) {
         auto store = [&](int idx, uint64_t x) { store_elem(i, j, row_off + idx, x); };
-        run_phases<LOGN, K, E, INV, W>(tile, addr, load, store, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
+        run_phases<LOGN, K, E, INV, W>(tile, addr, load, store, t, g, RPL, tw, bf_const(P), active);
     } else {
         // the last phase leaves 16 consecutive values per thread; stage them in LDS so
         // that the block writes whole rows with consecutive lanes on consecutive words
         auto stage = [&](int idx, uint64_t x) { tile[addr(idx)] = x; };
-        run_phases<LOGN, K, E, INV, W>(tile, addr, load, stage, t, g, RPL, tw, P.q << 2, 0 - P.q, active);
+        run_phases<LOGN, K, E, INV, W>(tile, addr, load, stage, t, g, RPL, tw, bf_const(P), active);
         __syncthreads();
         if (nsub == SUBS) {
             // a full block: a fixed number of elements per thread, unrolled so that every
@@ -215,7 +223,7 @@ __device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, u
     auto addr = [col](int idx) { return idx * SUBS + col; };
     auto load = [&](int idx) { return load_elem(i, j, (size_t)idx * C + c); };
     auto store = [&](int idx, uint64_t x) { store_elem(i, j, (size_t)idx * C + c, x); };
-    run_phases<LOGN, K, E, INV, false>(tile, addr, load, store, t, 0, 0, tw, P.q << 2, 0 - P.q, true);
+    run_phases<LOGN, K, E, INV, false>(tile, addr, load, store, t, 0, 0, tw, bf_const(P), true);
 }
 
 // The key switch's ModUp rows pass fused with the products by the key (N = 2^15, 2^16):
@@ -231,13 +239,13 @@ __device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, u
 // does not add or store it.  Blocks of one
 // (i, g) run one after another on one XCD, so its key slice comes from that XCD's L2.
 // Sum bound: up to 31 digits the transformed words (< 8q) enter the sums directly
-// (31 * 8q^2 + q^2 < 2^128 for q < 2^60); beyond, reduce_4q takes them below 4q first.
-template <int LOGN, int K, int E, int SUBS, int CG>
+// (31 * 8q^2 + q^2 < 2^128 for q < 2^60); beyond, R4 takes them below 4q first.
+template <int LOGN, int K, int E, int SUBS, int CG, bool R4>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
     const uint64_t *__restrict__ T, const uint64_t *__restrict__ key, uint64_t *__restrict__ Pout,
     uint64_t *__restrict__ Sout, const DevPrime *__restrict__ primes, const uint64_t *__restrict__ diag,
-    size_t diag_stride, uint32_t count, uint32_t L, uint32_t K_key, uint32_t ct_base, uint32_t groups, uint32_t tiles,
-    bool reduce_4q) {
+    size_t diag_stride, uint32_t count, uint32_t L, uint32_t K_key, uint32_t ct_base, uint32_t groups,
+    uint32_t tiles) {
     constexpr int TT = 1 << (K - E);  // threads per row
     constexpr int SZ = 1 << K;        // row length
     static_assert(SZ == SUBS * TT, "one coefficient per thread in the product phase");
@@ -261,11 +269,10 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
     const size_t n = (size_t)1 << LOGN;
     const size_t row_off = (size_t)g << K;
     const DevPrime &P = primes[i];
-    const uint64_t four_q = P.q << 2;
+    const BfConst bc = bf_const(P);
     const uint32_t ki = i == L ? K_key - 1 : i;
     const size_t kstride = (size_t)K_key << LOGN;  // key[j][k] -> key[j][k+1]
     const uint64_t *kp = key + ((size_t)ki << LOGN) + row_off + tid;
-    const uint32_t one = opaque_one();
     Acc3 a0[CG], a1[CG];
 #pragma unroll
     for (int c = 0; c < CG; c++) a0[c] = a1[c] = Acc3{0, 0, 0};
@@ -287,7 +294,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
         if constexpr (CG < 4) asm volatile("" : "+v"(sub_s), "+v"(t_s));
         auto addr_s = [sub_s](int idx) { return sub_s * SZ + swz(idx); };
         auto stage_s = [&](int idx, uint64_t x) { tile[addr_s(idx)] = x; };
-        run_phases<LOGN, K, E, false, true>(tile, addr_s, load, stage_s, t_s, g, RPL, tw, four_q, 0 - P.q, true);
+        run_phases<LOGN, K, E, false, true>(tile, addr_s, load, stage_s, t_s, g, RPL, tw, bc, true);
         __syncthreads();
 #pragma unroll 1
         for (int d = 0; d < DJ; d++) {
@@ -298,9 +305,9 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
 #pragma unroll
             for (int c = 0; c < CG; c++) {
                 uint64_t x = tile[(d * CG + c) * SZ + swz(tid)];
-                if (reduce_4q) x = sub_if_ge(x, four_q);
-                mac(a0[c], x, k0, one);
-                mac(a1[c], x, k1, one);
+                if (R4) x = sub_if_ge_neg(x, bc.neg_four_q);
+                mac(a0[c], x, k0);
+                mac(a1[c], x, k1);
             }
         }
     }
@@ -309,8 +316,8 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
 #pragma unroll
         for (int c = 0; c < CG; c++) {
             const uint64_t x = diag[(ct0 + min<uint32_t>(c, gv - 1)) * diag_stride + ((size_t)i << LOGN) + row_off + tid];
-            mac(a0[c], x, k0, one);
-            mac(a1[c], x, k1, one);
+            mac(a0[c], x, k0);
+            mac(a1[c], x, k1);
         }
     }
 #pragma unroll
@@ -435,7 +442,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
         return reduce ? barrett64(x, P.q, P.ratio_hi) : x;
     };
     auto store = [out](int idx, uint64_t x) { out[(size_t)idx * C] = x; };
-    run_phases<LOGN, K, E, false, true>(tile, addr, load, store, t, 0, 0, P.fwd, P.q << 2, 0 - P.q, true);
+    run_phases<LOGN, K, E, false, true>(tile, addr, load, store, t, 0, 0, P.fwd, bf_const(P), true);
 }
 
 // ---- ModDown by the special prime p, fused into the transforms (switch_key step 5):
@@ -675,6 +682,28 @@ bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint3
 }
 
 namespace {
+// The fused rows pass + key products over a batch: groups of 4 ciphertexts, then the
+// rest as a group of 2 and/or 1, so that no row is transformed for a ciphertext that is
+// not there.
+template <int LOGN, bool R4>
+void launch_keyprod_groups(const uint64_t *T, const uint64_t *key, uint64_t *P, uint64_t *S, const uint64_t *diag,
+                           size_t diag_stride, uint32_t cts, uint32_t L, uint32_t K_key, const DevPrime *targets,
+                           uint32_t tiles, uint64_t tile_slots, hipStream_t st) {
+    constexpr int K2 = LOGN - LOGN / 2;
+    constexpr int E = 4;
+    constexpr int RS = 4096 >> K2;
+    const uint32_t g4 = cts / 4, rest = cts % 4;
+    if (g4)
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 4, R4><<<(uint32_t)(tile_slots * g4), RS << (K2 - E), 0, st>>>(
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, 0, g4, tiles);
+    if (rest >= 2)
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 2, R4><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, g4 * 4, 1, tiles);
+    if (rest & 1)
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 1, R4><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, cts - 1, 1, tiles);
+}
+
 template <int LOGN>
 void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
                           uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t cts, uint32_t L,
@@ -684,7 +713,6 @@ void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
     constexpr int CS = 16;
-    constexpr int RS = 4096 >> K2;
     const uint32_t limbs = cts * L * (L + 1);
     ntt_cols_modup_kernel<LOGN, K1, E, CS>
         <<<limbs * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L, diag != nullptr);
@@ -692,19 +720,14 @@ void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const
     const uint64_t tile_slots = (uint64_t)((tiles + 7) / 8) * 8;
     if (tile_slots * ((cts + 3) / 4) >= (uint64_t(1) << 32))
         fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
-    // groups of 4 ciphertexts, then the rest as a group of 2 and/or 1, so that no row is
-    // transformed for a ciphertext that is not there
     if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
-    const uint32_t g4 = cts / 4, rest = cts % 4;
-    if (g4)
-        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 4><<<(uint32_t)(tile_slots * g4), RS << (K2 - E), 0, st>>>(
-            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, 0, g4, tiles, L > 31);
-    if (rest >= 2)
-        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 2><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
-            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, g4 * 4, 1, tiles, L > 31);
-    if (rest & 1)
-        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 1><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
-            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, cts - 1, 1, tiles, L > 31);
+    // up to 31 digits the transformed words (< 8q) enter the sums unreduced
+    if (L > 31)
+        launch_keyprod_groups<LOGN, true>(T, key, P, S, diag, diag_stride, cts, L, K_key, targets, tiles, tile_slots,
+                                          st);
+    else
+        launch_keyprod_groups<LOGN, false>(T, key, P, S, diag, diag_stride, cts, L, K_key, targets, tiles,
+                                           tile_slots, st);
     if (probe) SHC_HIP(hipEventRecord(probe->end, st));
 }
 

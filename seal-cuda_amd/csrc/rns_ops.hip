@@ -109,7 +109,6 @@ __global__ __launch_bounds__(TPB) void inner_product_kernel(const uint64_t *__re
     const uint64_t *Ti = T + (((size_t)i * count * L + ct0 * Lt) << logn) + c;
     const uint64_t *kp = key + (size_t)ki * n + c;
     const size_t kstride = (size_t)K << logn;  // key[j][k] -> key[j][k+1]
-    const uint32_t one = opaque_one();
     size_t goff[G];
     const uint64_t *dp[G];
 #pragma unroll
@@ -139,8 +138,8 @@ __global__ __launch_bounds__(TPB) void inner_product_kernel(const uint64_t *__re
         fetch(b, jnext);
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            mac(acc0[g], t[g], k0, one);
-            mac(acc1[g], t[g], k1, one);
+            mac(acc0[g], t[g], k0);
+            mac(acc1[g], t[g], k1);
         }
     };
     fetch(0, 0);

@@ -63,11 +63,10 @@ __global__ void correction_kernel(const uint64_t *__restrict__ negntt, const uin
         const uint32_t k = (uint32_t)(r / (L + 1));
         const DevPrime &Q = targets[i];
         const uint32_t ki = i == L ? K - 1 : i;
-        const uint32_t one = opaque_one();
         Acc3 acc{0, 0, 0};
         for (uint32_t j = 0; j < L; j++) {
             const uint64_t cij = barrett64(data[j].q, Q.q, Q.ratio_hi);
-            mac(acc, cij, key[(((size_t)j * 2 + k) * K + ki) * n + c], one);
+            mac(acc, cij, key[(((size_t)j * 2 + k) * K + ki) * n + c]);
         }
         const uint64_t s = acc_reduce(acc, Q);
         C[idx] = mulmod(s, negntt[(size_t)i * n + c], Q.q, Q.ratio_lo, Q.ratio_hi);
@@ -113,7 +112,6 @@ __global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *
     const uint32_t Lt = short_row ? L - 1 : L;
     const uint64_t *Ti = T + (((size_t)i * count * L + (size_t)ct0 * Lt) << logn) + c;
     const size_t kstride = (size_t)K << logn;
-    const uint32_t one = opaque_one();
     size_t goff[RG];
     const uint64_t *dp[RG];
 #pragma unroll
@@ -151,8 +149,8 @@ __global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *
         for (int r = 0; r < R; r++)
 #pragma unroll
             for (int g = 0; g < RG; g++) {
-                mac(a0[r][g], t[g], k0[r], one);
-                mac(a1[r][g], t[g], k1[r], one);
+                mac(a0[r][g], t[g], k0[r]);
+                mac(a1[r][g], t[g], k1[r]);
             }
     }
 #pragma unroll
@@ -163,8 +161,8 @@ __global__ __launch_bounds__(TPB) void rotation_products_kernel(const uint64_t *
         for (int g = 0; g < RG; g++) {
             if (g >= gv) continue;
             const size_t ct = ct0 + g;
-            acc_add(a0[r][g], c0, one);
-            acc_add(a1[r][g], c1, one);
+            acc_add(a0[r][g], c0);
+            acc_add(a1[r][g], c1);
             const uint64_t r0 = acc_reduce(a0[r][g], Q), r1 = acc_reduce(a1[r][g], Q);
             if (i == L) {
                 rb.S[r][(ct * 2 + 0) * n + c] = r0;

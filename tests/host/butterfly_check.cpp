@@ -5,13 +5,17 @@
 //   * the inverse butterfly maps [0, 4q)^2 into [0, 4q)^2 with x + y, (x - y) w;
 //   * for q < 2^60, the wide forward schedule: reducing by 8q maps x in [0, 16q) to
 //     outputs in [0, 12q), no reduction maps x in [0, 12q) to outputs in [0, 16q), for
-//     any 64-bit y; sub_if_ge(v, 8q) takes [0, 16q) to [0, 8q);
+//     any 64-bit y; sub_if_ge(v, 8q) and sub_if_ge_neg(v, 2^64 - 8q) take [0, 16q) to
+//     [0, 8q);
+//   * the key-switch product sums (accum.h): 31 products t k with t < 8q or 63 with
+//     t < 4q (k < q), plus one correction word, equal the exact 128-bit sum;
 // for primes from 17 to 61 bits, random and extreme operands (0, range ends,
 // multiples of q +- 1, all-ones words).  Exact products come from unsigned __int128.
 // Prints "ok <cases>" and exits 0, or prints the first failing case and exits 1.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
+#include "accum.h"
 #include "butterfly.h"
 
 using u64 = uint64_t;
@@ -20,6 +24,7 @@ using shc::butterfly;
 using shc::mulhi64_approx;
 using shc::shoup_lazier;
 using shc::sub_if_ge;
+using shc::sub_if_ge_neg;
 
 static u64 state = 0x9e3779b97f4a7c15ull;
 static u64 next() {  // splitmix64
@@ -122,7 +127,7 @@ int main() {
                     // wide schedule, reducing layer: x < 16q, y any word
                     u64 rx = operand(16 * q, q, k % 8), ry = k % 3 == 0 ? ~0ull - k : next();
                     const u64 rsx = rx % q, rwy = mulmod(ry % q, w, q);
-                    butterfly<false>(rx, ry, ulonglong2{w, wq}, four_q, neg_q, true, 8 * q);
+                    butterfly<false>(rx, ry, ulonglong2{w, wq}, four_q, neg_q, true, 0 - 8 * q);
                     if (rx >= 12 * q || ry >= 12 * q || rx % q != (rsx + rwy) % q || ry % q != (rsx + q - rwy) % q) {
                         printf("wide reducing butterfly q=%llu w=%llu case %d\n", (unsigned long long)q,
                                (unsigned long long)w, k);
@@ -131,18 +136,41 @@ int main() {
                     // wide schedule, layer without reduction: x < 12q
                     u64 nx = operand(12 * q, q, k % 8), ny = operand(16 * q, q, (k / 8) % 8);
                     const u64 nsx = nx % q, nwy = mulmod(ny % q, w, q);
-                    butterfly<false>(nx, ny, ulonglong2{w, wq}, four_q, neg_q, false, 8 * q);
+                    butterfly<false>(nx, ny, ulonglong2{w, wq}, four_q, neg_q, false, 0 - 8 * q);
                     if (nx >= 16 * q || ny >= 16 * q || nx % q != (nsx + nwy) % q || ny % q != (nsx + q - nwy) % q) {
                         printf("wide butterfly q=%llu w=%llu case %d\n", (unsigned long long)q, (unsigned long long)w, k);
                         return 1;
                     }
                     // end of a wide pass
                     const u64 v = operand(16 * q, q, k % 8), e = sub_if_ge(v, 8 * q);
-                    if (e >= 8 * q || e % q != v % q) {
+                    const u64 en = sub_if_ge_neg(v, 0 - 8 * q);
+                    if (e >= 8 * q || e % q != v % q || en != e) {
                         printf("sub_if_ge q=%llu v=%llu\n", (unsigned long long)q, (unsigned long long)v);
                         return 1;
                     }
-                    cases += 3;
+                    // product sums: 31 terms below 8q or 63 below 4q, then a correction
+                    for (int bound = 8; bound >= 4; bound -= 4) {
+                        shc::Acc3 acc{0, 0, 0};
+                        wide exact = 0;
+                        const int terms = bound == 8 ? 31 : 63;
+                        for (int m = 0; m < terms; m++) {
+                            // case 0: every term at its maximum
+                            const int kc = (k * 3 + m) % 8;
+                            const u64 t = k == 0 ? bound * q - 1 : operand(bound * q, q, (k + m) % 8);
+                            const u64 kw = k == 0 || kc == 3 ? q - 1 : operand(q, q, kc);
+                            shc::mac(acc, t, kw);
+                            exact += (wide)t * kw;
+                        }
+                        const u64 corr = k % 2 ? ~0ull : next();
+                        shc::acc_add(acc, corr);
+                        exact += corr;
+                        const shc::u128 got = shc::acc_value(acc);
+                        if (got.lo != (u64)exact || got.hi != (u64)(exact >> 64)) {
+                            printf("product sum q=%llu bound %dq case %d\n", (unsigned long long)q, bound, k);
+                            return 1;
+                        }
+                    }
+                    cases += 5;
                 }
                 cases += 4;
             }
