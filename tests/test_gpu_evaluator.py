@@ -62,6 +62,9 @@ def test_ckks_multiply(logn, bits, L, count):
     (CKKS, 15, [60] * 9, 8, 6),
     (BFV, 15, [60] * 7, 6, 3),
     (CKKS, 16, [60] * 4, 3, 5),
+    # 33 digits: the fused kernel reduces the transformed words below 4q before the
+    # products (the 128-bit sum bound); groups of 4, 2 and 1
+    (CKKS, 15, [60] * 34, 33, 7),
 ])
 def test_relinearize(scheme, logn, bits, L, count):
     ctx, moduli = make_ctx(scheme, logn, bits, 65537 if scheme == BFV else 0)
