@@ -227,20 +227,25 @@ __device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, u
 }
 
 // The key switch's ModUp rows pass fused with the products by the key (N = 2^15, 2^16):
-// a block owns row g of target prime i for CG ciphertexts.  Each step transforms SUBS
-// rows of T -- CG ciphertexts x SUBS/CG digits, the last K layers with the wide
-// schedule -- stages them in LDS, and then every thread multiplies its coefficient of
-// those rows by the key words of their digits and adds into three-word sums (one per
-// ciphertext and key component), so the transformed limbs never return to HBM:
+// a block owns RW adjacent rows of target prime i for CG ciphertexts (NS = CG x RW
+// slots).  Each step transforms SUBS rows of T -- NS slots x SUBS/NS digits, the last K
+// layers with the wide schedule -- stages them in LDS, and then every thread multiplies
+// its coefficient of those rows by the key words of their digits and adds into
+// three-word sums (one per slot and key component), so the transformed limbs never
+// return to HBM:
 //   P[ct][k][i][c] = sum_j NTT_i(T[i][ct][j])[c] * key[j][k][ki][c]   (S for i == L).
 // With diag (CKKS, T without its own-prime limbs), the term j == i is the NTT-form input
 // limb diag[ct * diag_stride + i * N].  The groups start at ciphertext ct_base.  A step
 // past the last digit, or a ciphertext past the batch, transforms a repeated row and
-// does not add or store it.  Blocks of one
-// (i, g) run one after another on one XCD, so its key slice comes from that XCD's L2.
+// does not add or store it.  Groups of 4 and 2 ciphertexts take one row (RW = 1); a
+// single ciphertext takes two rows (RW = 2), so that a step holds 8 of its digits
+// instead of 16 (24 digits in 3 steps, not 2 with a third of the rows wasted).  With
+// RW = 2 the slots run slot-major (each wave's rows share one row index, so its twiddle
+// loads stay scalar).  Blocks of one tile run one after another on one XCD, so its key
+// slice comes from that XCD's L2.
 // Sum bound: up to 31 digits the transformed words (< 8q) enter the sums directly
 // (31 * 8q^2 + q^2 < 2^128 for q < 2^60); beyond, R4 takes them below 4q first.
-template <int LOGN, int K, int E, int SUBS, int CG, bool R4>
+template <int LOGN, int K, int E, int SUBS, int CG, int RW, bool R4>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
     const uint64_t *__restrict__ T, const uint64_t *__restrict__ key, uint64_t *__restrict__ Pout,
     uint64_t *__restrict__ Sout, const DevPrime *__restrict__ primes, const uint64_t *__restrict__ diag,
@@ -249,35 +254,44 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
     constexpr int TT = 1 << (K - E);  // threads per row
     constexpr int SZ = 1 << K;        // row length
     static_assert(SZ == SUBS * TT, "one coefficient per thread in the product phase");
-    static_assert(SUBS % CG == 0, "whole digits per step");
+    constexpr int NS = CG * RW;
+    static_assert(SUBS % NS == 0, "whole digits per step");
     constexpr int RPL = LOGN - K;
-    constexpr int DJ = SUBS / CG;  // digits per step
+    constexpr int GPL = (1 << RPL) / RW;  // row groups per limb
+    static_assert(GPL * RW == (1 << RPL), "whole row groups");
+    constexpr int DJ = SUBS / NS;  // digits per step
+    static_assert(RW == 1 || DJ % (64 / TT) == 0, "slot-major: one slot per wave");
     __shared__ uint64_t tile[SUBS * SZ];
     const uint32_t xcd = blockIdx.x & 7, kk = blockIdx.x >> 3;
     const uint32_t tl = (kk / groups) * 8 + xcd;
     if (tl >= tiles) return;  // uniform: the whole block
     const uint32_t grp = kk % groups;
-    const uint32_t i = tl >> RPL;
-    const int g = (int)(tl & ((1u << RPL) - 1));
+    const uint32_t i = tl / GPL;
+    const int g0 = (int)(tl % GPL) * RW;  // first row of the group
     const uint32_t ct0 = ct_base + grp * CG;
     const uint32_t gv = min<uint32_t>(CG, count - ct0);
     const bool short_row = diag && i < L;
     const uint32_t Lt = short_row ? L - 1 : L;
     const int tid = threadIdx.x;
     const int sub = tid / TT, t = tid % TT;
-    const int gg = sub % CG, dj = sub / CG;  // this thread's transform row: ciphertext gg, digit dj
+    // this thread's transform row: slot gg (ciphertext gg % CG, row gg / CG), digit dj;
+    // tile row of (slot c, digit d): slot(c, d)
+    const int gg = RW > 1 ? sub / DJ : sub % NS, dj = RW > 1 ? sub % DJ : sub / NS;
+    auto slot = [](int c, int d) { return RW > 1 ? c * DJ + d : d * NS + c; };
     const size_t n = (size_t)1 << LOGN;
-    const size_t row_off = (size_t)g << K;
+    auto row_off = [g0](int r) { return (size_t)(g0 + r) << K; };
+    // the row index is uniform across a wave (one slot per wave when RW > 1)
+    const int g_t = RW > 1 ? __builtin_amdgcn_readfirstlane(g0 + gg / CG) : g0;
     const DevPrime &P = primes[i];
     const BfConst bc = bf_const(P);
     const uint32_t ki = i == L ? K_key - 1 : i;
     const size_t kstride = (size_t)K_key << LOGN;  // key[j][k] -> key[j][k+1]
-    const uint64_t *kp = key + ((size_t)ki << LOGN) + row_off + tid;
-    Acc3 a0[CG], a1[CG];
+    const uint64_t *kp = key + ((size_t)ki << LOGN) + row_off(0) + tid;  // row r: + r * SZ
+    Acc3 a0[NS], a1[NS];
 #pragma unroll
-    for (int c = 0; c < CG; c++) a0[c] = a1[c] = Acc3{0, 0, 0};
-    const uint64_t *trow =
-        T + (((size_t)i * count * L + (size_t)(ct0 + min<uint32_t>(gg, gv - 1)) * Lt) << LOGN) + row_off;
+    for (int c = 0; c < NS; c++) a0[c] = a1[c] = Acc3{0, 0, 0};
+    const uint64_t *trow = T + (((size_t)i * count * L + (size_t)(ct0 + min<uint32_t>(gg % CG, gv - 1)) * Lt) << LOGN) +
+                           row_off(gg / CG);
     const uint32_t steps = (Lt + DJ - 1) / DJ;
     for (uint32_t s = 0; s < steps; s++) {
         const uint64_t *row = trow + ((size_t)min<uint32_t>(s * DJ + dj, Lt - 1) << LOGN);
@@ -294,43 +308,50 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
         if constexpr (CG < 4) asm volatile("" : "+v"(sub_s), "+v"(t_s));
         auto addr_s = [sub_s](int idx) { return sub_s * SZ + swz(idx); };
         auto stage_s = [&](int idx, uint64_t x) { tile[addr_s(idx)] = x; };
-        run_phases<LOGN, K, E, false, true>(tile, addr_s, load, stage_s, t_s, g, RPL, tw, bc, true);
+        run_phases<LOGN, K, E, false, true>(tile, addr_s, load, stage_s, t_s, g_t, RPL, tw, bc, true);
         __syncthreads();
 #pragma unroll 1
         for (int d = 0; d < DJ; d++) {
             const uint32_t jt = s * DJ + d;
             if (jt >= Lt) break;  // uniform
             const uint32_t j = short_row && jt >= i ? jt + 1 : jt;
-            const uint64_t k0 = kp[(2 * j) * kstride], k1 = kp[(2 * j + 1) * kstride];
+            uint64_t k0[RW], k1[RW];
 #pragma unroll
-            for (int c = 0; c < CG; c++) {
-                uint64_t x = tile[(d * CG + c) * SZ + swz(tid)];
+            for (int r = 0; r < RW; r++) {
+                k0[r] = kp[(2 * j) * kstride + r * SZ];
+                k1[r] = kp[(2 * j + 1) * kstride + r * SZ];
+            }
+#pragma unroll
+            for (int c = 0; c < NS; c++) {
+                uint64_t x = tile[slot(c, d) * SZ + swz(tid)];
                 if (R4) x = sub_if_ge_neg(x, bc.neg_four_q);
-                mac(a0[c], x, k0);
-                mac(a1[c], x, k1);
+                mac(a0[c], x, k0[c / CG]);
+                mac(a1[c], x, k1[c / CG]);
             }
         }
     }
     if (short_row) {
-        const uint64_t k0 = kp[(2 * i) * kstride], k1 = kp[(2 * i + 1) * kstride];
 #pragma unroll
-        for (int c = 0; c < CG; c++) {
-            const uint64_t x = diag[(ct0 + min<uint32_t>(c, gv - 1)) * diag_stride + ((size_t)i << LOGN) + row_off + tid];
+        for (int c = 0; c < NS; c++) {
+            const uint64_t k0 = kp[(2 * i) * kstride + (c / CG) * SZ], k1 = kp[(2 * i + 1) * kstride + (c / CG) * SZ];
+            const uint64_t x = diag[(ct0 + min<uint32_t>(c % CG, gv - 1)) * diag_stride + ((size_t)i << LOGN) +
+                                    row_off(c / CG) + tid];
             mac(a0[c], x, k0);
             mac(a1[c], x, k1);
         }
     }
 #pragma unroll
-    for (int c = 0; c < CG; c++) {
-        if ((uint32_t)c >= gv) break;
-        const size_t ct = ct0 + c;
+    for (int c = 0; c < NS; c++) {
+        if ((uint32_t)(c % CG) >= gv) continue;
+        const size_t ct = ct0 + c % CG;
+        const size_t off = row_off(c / CG) + tid;
         const uint64_t r0 = acc_reduce(a0[c], P), r1 = acc_reduce(a1[c], P);
         if (i == L) {
-            Sout[(ct * 2 + 0) * n + row_off + tid] = r0;
-            Sout[(ct * 2 + 1) * n + row_off + tid] = r1;
+            Sout[(ct * 2 + 0) * n + off] = r0;
+            Sout[(ct * 2 + 1) * n + off] = r1;
         } else {
-            Pout[((ct * 2 + 0) * L + i) * n + row_off + tid] = r0;
-            Pout[((ct * 2 + 1) * L + i) * n + row_off + tid] = r1;
+            Pout[((ct * 2 + 0) * L + i) * n + off] = r0;
+            Pout[((ct * 2 + 1) * L + i) * n + off] = r1;
         }
     }
 }
@@ -694,14 +715,18 @@ void launch_keyprod_groups(const uint64_t *T, const uint64_t *key, uint64_t *P, 
     constexpr int RS = 4096 >> K2;
     const uint32_t g4 = cts / 4, rest = cts % 4;
     if (g4)
-        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 4, R4><<<(uint32_t)(tile_slots * g4), RS << (K2 - E), 0, st>>>(
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 4, 1, R4><<<(uint32_t)(tile_slots * g4), RS << (K2 - E), 0, st>>>(
             T, key, P, S, targets, diag, diag_stride, cts, L, K_key, 0, g4, tiles);
     if (rest >= 2)
-        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 2, R4><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 2, 1, R4><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
             T, key, P, S, targets, diag, diag_stride, cts, L, K_key, g4 * 4, 1, tiles);
-    if (rest & 1)
-        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 1, R4><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
-            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, cts - 1, 1, tiles);
+    if (rest & 1) {
+        // the single ciphertext: two rows per block
+        const uint32_t tiles2 = tiles / 2;
+        const uint32_t slots2 = (tiles2 + 7) / 8 * 8;
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 1, 2, R4><<<slots2, RS << (K2 - E), 0, st>>>(
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, cts - 1, 1, tiles2);
+    }
 }
 
 template <int LOGN>
