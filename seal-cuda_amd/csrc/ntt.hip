@@ -148,27 +148,55 @@ __device__ __forceinline__ uint64_t finish(uint64_t x, const DevPrime &P) {
 // row in the block uses the same twiddles (broadcast loads, L1/L2 hits).  Blocks run
 // with the limb group fastest, so consecutive blocks also share them.  jn_short != 0:
 // primes i < I - 1 have only jn_short limbs (the grid is sized for jn).
-template <int LOGN, int K, int E, int SUBS, bool INV, bool W = false, class Load, class Store>
+// MX (fewer than SUBS limbs per prime, e.g. one ciphertext): a block takes SUBS
+// (limb, row) pairs of prime i instead, limb fastest, so that its threads are not idle;
+// the rows of a block then differ and so do their twiddles.
+template <int LOGN, int K, int E, int SUBS>
+__host__ __device__ constexpr uint32_t rows_blocks_per_prime(uint32_t jn, bool mx) {
+    return mx ? ((jn << (LOGN - K)) + SUBS - 1) / SUBS : (1u << (LOGN - K)) * ((jn + SUBS - 1) / SUBS);
+}
+template <int LOGN, int K, int SUBS>
+__host__ __device__ constexpr bool rows_mixed(uint32_t jn) {
+    return LOGN > K && jn < (uint32_t)SUBS;
+}
+
+template <int LOGN, int K, int E, int SUBS, bool INV, bool W = false, bool MX = false, class Load, class Store>
 __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, uint32_t I, uint32_t j0, uint32_t jn,
                                           const Load &load_elem, const Store &store_elem, uint32_t jn_short = 0) {
     constexpr int T = 1 << (K - E);
     constexpr int SZ = 1 << K;
     constexpr int RPL = LOGN - K;  // log2(rows per limb)
     constexpr int NP = (K + E - 1) / E;
+    static_assert(!MX || (NP > 1 && RPL > 0), "pairs of (limb, row) need rows and a staged store");
     __shared__ uint64_t tile[NP > 1 ? SUBS * SZ : 1];
-    const uint32_t jc_count = (jn + SUBS - 1) / SUBS;
-    const uint32_t jc = blockIdx.x % jc_count;
-    const uint32_t r = blockIdx.x / jc_count;
-    const int g = (int)(r & ((1u << RPL) - 1));
-    const uint32_t i = r >> RPL;
-    const uint32_t jfirst = j0 + jc * SUBS;
-    const uint32_t jend = j0 + (jn_short && i + 1 < I ? jn_short : jn);
-    if (jfirst >= jend) return;  // uniform: a whole block past a short row
-    const int nsub = (int)min<uint32_t>(SUBS, jend - jfirst);
     const int tid = threadIdx.x;
     const int sub = tid / T, t = tid % T;
-    const bool active = sub < nsub;
-    const uint32_t j = jfirst + (uint32_t)(active ? sub : 0);
+    uint32_t i, jfirst = 0, q0 = 0;
+    int g, nsub;
+    bool active;
+    uint32_t j;
+    if constexpr (MX) {
+        const uint32_t per = rows_blocks_per_prime<LOGN, K, E, SUBS>(jn, true);
+        i = blockIdx.x / per;
+        q0 = (blockIdx.x % per) * SUBS;
+        nsub = (int)min<uint32_t>(SUBS, (jn << RPL) - q0);
+        active = sub < nsub;
+        const uint32_t q = q0 + (uint32_t)(active ? sub : 0);
+        j = j0 + q % jn;
+        g = (int)(q / jn);
+    } else {
+        const uint32_t jc_count = (jn + SUBS - 1) / SUBS;
+        const uint32_t jc = blockIdx.x % jc_count;
+        const uint32_t r = blockIdx.x / jc_count;
+        g = (int)(r & ((1u << RPL) - 1));
+        i = r >> RPL;
+        jfirst = j0 + jc * SUBS;
+        const uint32_t jend = j0 + (jn_short && i + 1 < I ? jn_short : jn);
+        if (jfirst >= jend) return;  // uniform: a whole block past a short row
+        nsub = (int)min<uint32_t>(SUBS, jend - jfirst);
+        active = sub < nsub;
+        j = jfirst + (uint32_t)(active ? sub : 0);
+    }
     const size_t row_off = (size_t)g << K;
     const DevPrime &P = primes[i];
     const ulonglong2 *tw = INV ? P.inv : P.fwd;
@@ -183,7 +211,13 @@ __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, u
         auto stage = [&](int idx, uint64_t x) { tile[addr(idx)] = x; };
         run_phases<LOGN, K, E, INV, W>(tile, addr, load, stage, t, g, RPL, tw, bf_const(P), active);
         __syncthreads();
-        if (nsub == SUBS) {
+        if constexpr (MX) {
+            for (int e = tid; e < nsub * SZ; e += SUBS * T) {
+                const int sb = e / SZ, off = e % SZ;
+                const uint32_t q = q0 + (uint32_t)sb;
+                store_elem(i, j0 + q % jn, ((size_t)(q / jn) << K) + off, tile[sb * SZ + swz(off)]);
+            }
+        } else if (nsub == SUBS) {
             // a full block: a fixed number of elements per thread, unrolled so that every
             // row's base address is uniform (scalar) and only the offset varies by lane
             constexpr int PER = SUBS * SZ / (SUBS * T);
@@ -366,12 +400,12 @@ struct GridAccess {
     }
 };
 
-template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
+template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL, bool MX = false>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__restrict__ data,
                                                                   const DevPrime *__restrict__ primes, LimbGrid lg,
                                                                   uint32_t j0, uint32_t jn) {
     const GridAccess g{data, lg, LOGN};
-    rows_body<LOGN, K, E, SUBS, INV>(
+    rows_body<LOGN, K, E, SUBS, INV, false, MX>(
         primes, lg.I, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
@@ -380,14 +414,14 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__r
 
 // Rows pass of a transform whose input lies elsewhere: reads limb (i, j) from `src`
 // (grid slg) and writes it to `data` (grid lg), same shape.
-template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
+template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL, bool MX = false>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_from_kernel(const uint64_t *__restrict__ src,
                                                                        LimbGrid slg, uint64_t *__restrict__ data,
                                                                        const DevPrime *__restrict__ primes,
                                                                        LimbGrid lg, uint32_t j0, uint32_t jn) {
     const GridAccess g{data, lg, LOGN};
     const GridAccess s{const_cast<uint64_t *>(src), slg, LOGN};
-    rows_body<LOGN, K, E, SUBS, INV>(
+    rows_body<LOGN, K, E, SUBS, INV, false, MX>(
         primes, lg.I, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *s.at(i, j, off); },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
@@ -499,11 +533,11 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_moddown_prep_kernel(
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { U[((size_t)(j * a.L + i) << LOGN) + off] = x; }, a.L);
 }
 
-template <int LOGN, int K, int E, int SUBS>
+template <int LOGN, int K, int E, int SUBS, bool MX>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_moddown_kernel(const uint64_t *__restrict__ U,
                                                                           const DevPrime *__restrict__ primes,
                                                                           ModDownArgs a, uint32_t jn) {
-    rows_body<LOGN, K, E, SUBS, false>(
+    rows_body<LOGN, K, E, SUBS, false, false, MX>(
         primes, a.L, 0, jn, [&](uint32_t i, uint32_t j, size_t off) { return U[((size_t)(j * a.L + i) << LOGN) + off]; },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             const DevPrime &Q = primes[i];
@@ -534,13 +568,22 @@ void launch_moddown(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t cou
     constexpr int RS = 4096 >> K2;
     const uint32_t jn = 2 * count;
     const uint32_t col_blocks = a.L * jn * ((1u << K2) / CS);
-    const uint32_t row_blocks = (a.L << K1) * ((jn + RS - 1) / RS);
+    const bool mx = rows_mixed<LOGN, K2, RS>(jn);
+    const uint32_t row_blocks = a.L * rows_blocks_per_prime<LOGN, K2, E, RS>(jn, mx);
     if (ckks) {
         ntt_cols_moddown_prep_kernel<LOGN, K1, E, CS><<<col_blocks, CS << (K1 - E), 0, st>>>(buf, primes, a, jn);
-        ntt_rows_moddown_kernel<LOGN, K2, E, RS><<<row_blocks, RS << (K2 - E), 0, st>>>(buf, primes, a, jn);
+        if (mx)
+            ntt_rows_moddown_kernel<LOGN, K2, E, RS, true><<<row_blocks, RS << (K2 - E), 0, st>>>(buf, primes, a, jn);
+        else
+            ntt_rows_moddown_kernel<LOGN, K2, E, RS, false><<<row_blocks, RS << (K2 - E), 0, st>>>(buf, primes, a, jn);
     } else {
-        ntt_rows_kernel<LOGN, K2, E, RS, true, false, true>
-            <<<row_blocks, RS << (K2 - E), 0, st>>>(buf, primes, LimbGrid{1, a.L, a.L, jn}, 0, jn);
+        const LimbGrid lg{1, a.L, a.L, jn};
+        if (mx)
+            ntt_rows_kernel<LOGN, K2, E, RS, true, false, true, true>
+                <<<row_blocks, RS << (K2 - E), 0, st>>>(buf, primes, lg, 0, jn);
+        else
+            ntt_rows_kernel<LOGN, K2, E, RS, true, false, true, false>
+                <<<row_blocks, RS << (K2 - E), 0, st>>>(buf, primes, lg, 0, jn);
         ntt_cols_moddown_kernel<LOGN, K1, E, CS><<<col_blocks, CS << (K1 - E), 0, st>>>(buf, primes, a, jn);
     }
 }
@@ -561,15 +604,24 @@ void launch_ntt(uint64_t *data, LimbGrid lg, uint32_t j0, uint32_t jn, const Dev
         constexpr int CS = 16;                                    // columns per block
         constexpr int RS = (4096 >> K2) < 1 ? 1 : (4096 >> K2);  // rows per block
         const uint32_t col_blocks = lg.I * jn * ((1u << K2) / CS);
-        const uint32_t row_blocks = (lg.I << K1) * ((jn + RS - 1) / RS);
+        const bool mx = rows_mixed<LOGN, K2, RS>(jn);
+        const uint32_t row_blocks = lg.I * rows_blocks_per_prime<LOGN, K2, E, RS>(jn, mx);
         if (!INV) {
             ntt_cols_kernel<LOGN, K1, E, CS, false, false, FULL>
                 <<<col_blocks, CS << (K1 - E), 0, st>>>(data, primes, lg, j0, jn);
-            ntt_rows_kernel<LOGN, K2, E, RS, false, true, FULL>
-                <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lg, j0, jn);
+            if (mx)
+                ntt_rows_kernel<LOGN, K2, E, RS, false, true, FULL, true>
+                    <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lg, j0, jn);
+            else
+                ntt_rows_kernel<LOGN, K2, E, RS, false, true, FULL, false>
+                    <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lg, j0, jn);
         } else {
-            ntt_rows_kernel<LOGN, K2, E, RS, true, false, FULL>
-                <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lg, j0, jn);
+            if (mx)
+                ntt_rows_kernel<LOGN, K2, E, RS, true, false, FULL, true>
+                    <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lg, j0, jn);
+            else
+                ntt_rows_kernel<LOGN, K2, E, RS, true, false, FULL, false>
+                    <<<row_blocks, RS << (K2 - E), 0, st>>>(data, primes, lg, j0, jn);
             ntt_cols_kernel<LOGN, K1, E, CS, true, true, FULL>
                 <<<col_blocks, CS << (K1 - E), 0, st>>>(data, primes, lg, j0, jn);
         }
@@ -595,9 +647,14 @@ void launch_inverse_from(const uint64_t *src, LimbGrid slg, uint64_t *data, Limb
         constexpr int CS = 16;
         constexpr int RS = (4096 >> K2) < 1 ? 1 : (4096 >> K2);
         const uint32_t col_blocks = lg.I * jn * ((1u << K2) / CS);
-        const uint32_t row_blocks = (lg.I << K1) * ((jn + RS - 1) / RS);
-        ntt_rows_from_kernel<LOGN, K2, E, RS, true, false, FULL>
-            <<<row_blocks, RS << (K2 - E), 0, st>>>(src, slg, data, primes, lg, j0, jn);
+        const bool mx = rows_mixed<LOGN, K2, RS>(jn);
+        const uint32_t row_blocks = lg.I * rows_blocks_per_prime<LOGN, K2, E, RS>(jn, mx);
+        if (mx)
+            ntt_rows_from_kernel<LOGN, K2, E, RS, true, false, FULL, true>
+                <<<row_blocks, RS << (K2 - E), 0, st>>>(src, slg, data, primes, lg, j0, jn);
+        else
+            ntt_rows_from_kernel<LOGN, K2, E, RS, true, false, FULL, false>
+                <<<row_blocks, RS << (K2 - E), 0, st>>>(src, slg, data, primes, lg, j0, jn);
         ntt_cols_kernel<LOGN, K1, E, CS, true, true, FULL>
             <<<col_blocks, CS << (K1 - E), 0, st>>>(data, primes, lg, j0, jn);
     }
