@@ -342,7 +342,9 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
         if constexpr (CG < 4) asm volatile("" : "+v"(sub_s), "+v"(t_s));
         auto addr_s = [sub_s](int idx) { return sub_s * SZ + swz(idx); };
         auto stage_s = [&](int idx, uint64_t x) { tile[addr_s(idx)] = x; };
-        run_phases<LOGN, K, E, false, true>(tile, addr_s, load, stage_s, t_s, g_t, RPL, tw, bc, true);
+        // rows past the last digit (the last step of a short target) are not transformed:
+        // with a group of 4 that is a whole wave, which leaves the SIMD to other waves
+        run_phases<LOGN, K, E, false, true>(tile, addr_s, load, stage_s, t_s, g_t, RPL, tw, bc, s * DJ + dj < Lt);
         __syncthreads();
 #pragma unroll 1
         for (int d = 0; d < DJ; d++) {
