@@ -46,7 +46,19 @@ __device__ __forceinline__ void dot_rows(Acc3 (&acc)[OB], const uint64_t *v, con
         r[u] = (const_u64 *)(rows + (size_t)min<uint32_t>(o0 + u, outputs - 1) * terms);
         acc[u] = Acc3{0, 0, 0};
     }
-    for (uint32_t j = 0; j < terms; j++) {
+    // four terms at a time: their LDS words and table entries are requested together, so
+    // one wait covers four terms
+    uint32_t j = 0;
+    for (; j + 4 <= terms; j += 4) {
+        uint64_t y[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) y[k] = v[(j + k) * BT + tid];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int u = 0; u < OB; u++) mac<true>(acc[u], y[k], r[u][j + k]);
+    }
+    for (; j < terms; j++) {
         const uint64_t y = v[j * BT + tid];
 #pragma unroll
         for (int u = 0; u < OB; u++) mac<true>(acc[u], y, r[u][j]);
