@@ -462,12 +462,13 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
 
 // ModUp columns pass into T[i][ct][j] (target-major, so that the rows pass walks one
 // prime's limbs at a time): a block takes one tile of 16 columns of digit j of
-// ciphertext ct (read from d[ct][j]), keeps it in registers, and for each target prime i
-// in turn reduces it modulo q_i when the digit's prime exceeds 8 q_i (the forward
-// butterflies take inputs < 8q) and runs the first K layers -- one LDS round trip and
-// two barriers per target -- so the digit is read once and the loads' latency is paid
-// once per tile, not once per target (measured 0.6 ms faster per 128 ciphertexts than
-// one block per target).
+// ciphertext ct (read from d[ct][j]), keeps it in registers, and for each of tc target
+// primes i in turn reduces it modulo q_i when the digit's prime exceeds 8 q_i (the
+// forward butterflies take inputs < 8q) and runs the first K layers -- one LDS round
+// trip and two barriers per target -- so the loads' latency is paid once per tile and
+// chunk of targets, not once per target.  Large batches take all L + 1 targets per block
+// (measured 0.6 ms faster per 128 ciphertexts than one block per target); small ones
+// split the targets into chunks so that the grid still fills the GPU.
 // skip_diag: limb (i, i) is not produced (its NTT is the NTT-form input limb) and the
 // targets i < L keep L - 1 limbs per ciphertext, T[i][ct][j - (j > i)].
 template <int LOGN, int K, int E, int SUBS>
@@ -476,14 +477,18 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
                                                                         uint64_t *__restrict__ T,
                                                                         const DevPrime *__restrict__ targets,
                                                                         const DevPrime *__restrict__ digits,
-                                                                        uint32_t count, uint32_t L, bool skip_diag) {
+                                                                        uint32_t count, uint32_t L, bool skip_diag,
+                                                                        uint32_t tc) {
     static_assert(K >= E, "the first phase fills all 2^E registers");
     constexpr int C = 1 << (LOGN - K);
     constexpr int TILES = C / SUBS;
     __shared__ uint64_t tile[SUBS * (1 << K)];
     const int tid = threadIdx.x;
     const int col = tid % SUBS, t = tid / SUBS;
-    const uint32_t r = blockIdx.x / TILES;  // ct * L + j
+    const uint32_t chunks = (L + tc) / tc;  // ceil((L + 1) / tc)
+    const uint32_t rc = blockIdx.x / TILES;  // (ct * L + j) * chunks + chunk
+    const uint32_t r = rc / chunks;          // ct * L + j
+    const uint32_t i0 = (rc % chunks) * tc, i1 = min<uint32_t>(i0 + tc, L + 1);
     const int c0 = (int)(blockIdx.x % TILES) * SUBS;
     const uint32_t j = r % L;
     const size_t ct = r / L;
@@ -493,7 +498,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
     for (int k = 0; k < (1 << E); k++) dv[k] = in[(size_t)phase_index<K, E, 0, E>(t, k) * C];
     const uint64_t qj = digits[j].q;
     auto addr = [col](int idx) { return idx * SUBS + col; };
-    for (uint32_t i = 0; i <= L; i++) {
+    for (uint32_t i = i0; i < i1; i++) {
         if (skip_diag && i == j) continue;  // uniform
         const bool short_row = skip_diag && i < L;
         const size_t limb = (size_t)i * count * L + ct * (short_row ? L - 1 : L) + (short_row && j > i ? j - 1 : j);
@@ -725,6 +730,15 @@ void dispatch(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, hip
     }
 }
 
+// Targets per block of the ModUp columns pass: all L + 1 when the (digit, tile) blocks
+// alone fill the GPU, fewer (chunks of them) so that a small batch still has about
+// 16384 blocks.
+inline uint32_t modup_targets_per_block(uint32_t tile_blocks, uint32_t L) {
+    const uint32_t want = (16384 + tile_blocks - 1) / tile_blocks;
+    const uint32_t chunks = want < 1 ? 1 : (want > L + 1 ? L + 1 : want);
+    return (L + chunks) / chunks;  // ceil((L + 1) / chunks)
+}
+
 template <int LOGN>
 void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts, uint32_t L, const DevPrime *targets,
                   const DevPrime *digits, bool skip_diag, bool reduce_4q, const KernelProbe *probe, hipStream_t st) {
@@ -733,8 +747,9 @@ void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts,
     constexpr int E = 4;
     constexpr int CS = 16;
     constexpr int RS = 4096 >> K2;
-    ntt_cols_modup_kernel<LOGN, K1, E, CS>
-        <<<cts * L * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L, skip_diag);
+    const uint32_t tc = modup_targets_per_block(cts * L * ((1u << K2) / CS), L);
+    ntt_cols_modup_kernel<LOGN, K1, E, CS><<<cts * L * ((L + tc) / tc) * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(
+        d, d_stride, T, targets, digits, cts, L, skip_diag, tc);
     const LimbGrid lg{cts * L, 1, L + 1, cts * L};
     const uint32_t row_blocks = ((L + 1) << K1) * ((lg.J + RS - 1) / RS);
     if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
@@ -801,8 +816,9 @@ void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
     constexpr int CS = 16;
-    ntt_cols_modup_kernel<LOGN, K1, E, CS>
-        <<<cts * L * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits, cts, L, diag != nullptr);
+    const uint32_t tc = modup_targets_per_block(cts * L * ((1u << K2) / CS), L);
+    ntt_cols_modup_kernel<LOGN, K1, E, CS><<<cts * L * ((L + tc) / tc) * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(
+        d, d_stride, T, targets, digits, cts, L, diag != nullptr, tc);
     const uint32_t tiles = (L + 1) << K1;
     const uint64_t tile_slots = (uint64_t)((tiles + 7) / 8) * 8;
     if (tile_slots * ((cts + 3) / 4) >= (uint64_t(1) << 32))
