@@ -468,10 +468,11 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
 // trip and two barriers per target -- so the loads' latency is paid once per tile and
 // chunk of targets, not once per target.  Large batches take all L + 1 targets per block
 // (measured 0.6 ms faster per 128 ciphertexts than one block per target); small ones
-// split the targets into chunks so that the grid still fills the GPU.
+// split the targets into chunks so that the grid still fills the GPU, down to one target
+// per block (ONE: nothing held in registers, 60 VGPRs instead of 132).
 // skip_diag: limb (i, i) is not produced (its NTT is the NTT-form input limb) and the
 // targets i < L keep L - 1 limbs per ciphertext, T[i][ct][j - (j > i)].
-template <int LOGN, int K, int E, int SUBS>
+template <int LOGN, int K, int E, int SUBS, bool ONE>
 __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const uint64_t *__restrict__ d,
                                                                         size_t d_ct_stride,
                                                                         uint64_t *__restrict__ T,
@@ -493,11 +494,29 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
     const uint32_t j = r % L;
     const size_t ct = r / L;
     const uint64_t *in = d + ct * d_ct_stride + ((size_t)j << LOGN) + c0 + col;
+    const uint64_t qj = digits[j].q;
+    auto addr = [col](int idx) { return idx * SUBS + col; };
+    if constexpr (ONE) {
+        // one target per block (small batches): the first phase loads straight from the
+        // digit, nothing held across targets (fewer registers, more waves)
+        const uint32_t i = i0;
+        if (skip_diag && i == j) return;  // uniform: the whole block
+        const bool short_row = skip_diag && i < L;
+        const size_t limb = (size_t)i * count * L + ct * (short_row ? L - 1 : L) + (short_row && j > i ? j - 1 : j);
+        uint64_t *out = T + (limb << LOGN) + c0 + col;
+        const DevPrime &P = targets[i];
+        const bool reduce = qj > 8 * P.q;
+        auto load = [&](int idx, int) {
+            const uint64_t x = in[(size_t)idx * C];
+            return reduce ? barrett64(x, P.q, P.ratio_hi) : x;
+        };
+        auto store = [out](int idx, uint64_t x) { out[(size_t)idx * C] = x; };
+        run_phases<LOGN, K, E, false, true>(tile, addr, load, store, t, 0, 0, P.fwd, bf_const(P), true);
+        return;
+    }
     uint64_t dv[1 << E];
 #pragma unroll
     for (int k = 0; k < (1 << E); k++) dv[k] = in[(size_t)phase_index<K, E, 0, E>(t, k) * C];
-    const uint64_t qj = digits[j].q;
-    auto addr = [col](int idx) { return idx * SUBS + col; };
     for (uint32_t i = i0; i < i1; i++) {
         if (skip_diag && i == j) continue;  // uniform
         const bool short_row = skip_diag && i < L;
@@ -748,8 +767,13 @@ void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts,
     constexpr int CS = 16;
     constexpr int RS = 4096 >> K2;
     const uint32_t tc = modup_targets_per_block(cts * L * ((1u << K2) / CS), L);
-    ntt_cols_modup_kernel<LOGN, K1, E, CS><<<cts * L * ((L + tc) / tc) * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(
-        d, d_stride, T, targets, digits, cts, L, skip_diag, tc);
+    const uint32_t col_grid = cts * L * ((L + tc) / tc) * ((1u << K2) / CS);
+    if (tc == 1)
+        ntt_cols_modup_kernel<LOGN, K1, E, CS, true><<<col_grid, CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits,
+                                                                                       cts, L, skip_diag, tc);
+    else
+        ntt_cols_modup_kernel<LOGN, K1, E, CS, false><<<col_grid, CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits,
+                                                                                        cts, L, skip_diag, tc);
     const LimbGrid lg{cts * L, 1, L + 1, cts * L};
     const uint32_t row_blocks = ((L + 1) << K1) * ((lg.J + RS - 1) / RS);
     if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
@@ -817,8 +841,13 @@ void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const
     constexpr int E = 4;
     constexpr int CS = 16;
     const uint32_t tc = modup_targets_per_block(cts * L * ((1u << K2) / CS), L);
-    ntt_cols_modup_kernel<LOGN, K1, E, CS><<<cts * L * ((L + tc) / tc) * ((1u << K2) / CS), CS << (K1 - E), 0, st>>>(
-        d, d_stride, T, targets, digits, cts, L, diag != nullptr, tc);
+    const uint32_t col_grid = cts * L * ((L + tc) / tc) * ((1u << K2) / CS);
+    if (tc == 1)
+        ntt_cols_modup_kernel<LOGN, K1, E, CS, true><<<col_grid, CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits,
+                                                                                       cts, L, diag != nullptr, tc);
+    else
+        ntt_cols_modup_kernel<LOGN, K1, E, CS, false><<<col_grid, CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits,
+                                                                                        cts, L, diag != nullptr, tc);
     const uint32_t tiles = (L + 1) << K1;
     const uint64_t tile_slots = (uint64_t)((tiles + 7) / 8) * 8;
     if (tile_slots * ((cts + 3) / 4) >= (uint64_t(1) << 32))
