@@ -40,6 +40,12 @@ HBM_PEAK_GBS = 8000.0
 # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB device copy
 # (profiles/mulrelin_pmc_traffic.md); a profiler cannot run inside this process.
 KEYPROD_TRAFFIC_B128 = 40.93e9 + 3.36e9
+# VALU wave instructions per launch of the same kernel (rocprofv3 --pmc SQ_INSTS_VALU, same
+# file) and the VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles
+# per SIMD at the 2.4 GHz peak engine clock.  The kernel is bound by VALU issue (exact
+# 64-bit modular arithmetic from 32-bit multiplies), so this is its figure of merit.
+KEYPROD_VALU_B128 = 8027439104
+VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 CKKS = 2
 
 
@@ -258,11 +264,16 @@ def bench_mulrelin(args):
                      "frac": achieved_rows / HBM_PEAK_GBS,
                      "traffic": KEYPROD_TRAFFIC_B128 if B == 128 else None,
                      "butterflies_per_launch": rows_bfly, "macs_per_launch": rows_macs,
-                     "kernel": f"ntt_rows_keyprod_kernel<16,8,4,16,4>: key-switch ModUp NTT rows pass over {tlimbs} "
-                               f"limbs (last {logn - logn // 2} layers) fused with the key products",
+                     "kernel": f"ntt_rows_keyprod_kernel<16,8,4,16,4,1,false>: key-switch ModUp NTT rows pass over "
+                               f"{tlimbs} limbs (last {logn - logn // 2} layers) fused with the key products",
                      "algorithmic_bytes": rows_bytes, "avg_ms": rows_ms,
                      "butterflies_per_s": rows_bfly / (rows_ms * 1e-3),
-                     "macs_per_s": rows_macs / (rows_ms * 1e-3)},
+                     "macs_per_s": rows_macs / (rows_ms * 1e-3),
+                     "valu": ({"wave_instructions": KEYPROD_VALU_B128,
+                               "achieved_per_s": KEYPROD_VALU_B128 / (rows_ms * 1e-3),
+                               "peak_per_s": VALU_PEAK_WAVE_INSTR_S,
+                               "frac": KEYPROD_VALU_B128 / (rows_ms * 1e-3) / VALU_PEAK_WAVE_INSTR_S}
+                              if B == 128 else None)},
         "step_roofline": {"achieved": achieved_step, "unit": "GB/s", "frac": achieved_step / HBM_PEAK_GBS,
                           "algorithmic_bytes_per_op": unit_bytes},
         "plain": plain,
