@@ -346,18 +346,33 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
         // rows past the last digit (the last step of a short target) are not transformed:
         // with a group of 4 that is a whole wave, which leaves the SIMD to other waves
         run_phases<LOGN, K, E, false, true>(tile, addr_s, load, stage_s, t_s, g_t, RPL, tw, bc, s * DJ + dj < Lt);
+        // the key words of a digit are requested one digit ahead (the first before the
+        // barrier), at clamped indices rather than behind a branch
+        auto key_j = [&](uint32_t jt) {
+            jt = min<uint32_t>(jt, Lt - 1);
+            return short_row && jt >= i ? jt + 1 : jt;
+        };
+        uint64_t kn0[RW], kn1[RW];
+        auto fetch_keys = [&](uint32_t j) {
+#pragma unroll
+            for (int r = 0; r < RW; r++) {
+                kn0[r] = kp[(2 * j) * kstride + r * SZ];
+                kn1[r] = kp[(2 * j + 1) * kstride + r * SZ];
+            }
+        };
+        fetch_keys(key_j(s * DJ));
         __syncthreads();
 #pragma unroll 1
         for (int d = 0; d < DJ; d++) {
             const uint32_t jt = s * DJ + d;
             if (jt >= Lt) break;  // uniform
-            const uint32_t j = short_row && jt >= i ? jt + 1 : jt;
             uint64_t k0[RW], k1[RW];
 #pragma unroll
             for (int r = 0; r < RW; r++) {
-                k0[r] = kp[(2 * j) * kstride + r * SZ];
-                k1[r] = kp[(2 * j + 1) * kstride + r * SZ];
+                k0[r] = kn0[r];
+                k1[r] = kn1[r];
             }
+            fetch_keys(key_j(jt + 1));
 #pragma unroll
             for (int c = 0; c < NS; c++) {
                 uint64_t x = tile[slot(c, d) * SZ + swz(tid)];
