@@ -13,7 +13,7 @@
 //     covers E layers.
 //   * Harvey-style lazy butterflies with Shoup twiddles keep values in [0,8q)
 //     (forward) or [0,4q) (inverse); the twiddle product uses a quotient estimate
-//     that may be low by up to 3 (9 32-bit multiplies instead of 10), which those
+//     that may be low by up to 2 (9 32-bit multiplies instead of 10), which those
 //     wider ranges absorb.  The full reduction and the N^-1 scaling are folded into
 //     the store of the last pass.
 //   * The key switch's ModUp transforms take the wide schedule, which needs q < 2^60

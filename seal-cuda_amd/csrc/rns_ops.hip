@@ -13,7 +13,9 @@
 //      L(L+1) of them per ciphertext, L x L for CKKS, whose own-prime limbs (j == i)
 //      are c2's NTT-form limbs themselves (the reference's i == j shortcut)
 //   4. inner_product: P_k[i] = sum_j T[j][i] * key[j][k][i] with 128-bit accumulation,
-//      one Barrett reduction at the end; G ciphertexts per thread share each key load
+//      one Barrett reduction at the end; G ciphertexts per thread share each key load.
+//      At N = 2^15 and 2^16 the NTT's rows pass and this product are one kernel
+//      (ntt_modup_keyprod): the transformed limbs never return to HBM
 //   5. ModDown: inverse NTT of the special limbs, rounding correction, forward NTT
 //      (CKKS) or inverse NTT of P (BFV), then ct_k[i] += (P_k[i] - t) * p^-1.
 #include "accum.h"
