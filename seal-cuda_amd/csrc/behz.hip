@@ -27,17 +27,25 @@ inline uint32_t grid_for(size_t n) {
     return (uint32_t)(b < (size_t(1) << 20) ? b : (size_t(1) << 20));
 }
 
+// Output primes per conversion pass: 5 when that leaves fewer unused sums than 4.
+inline int out_group(uint32_t outputs) {
+    const uint32_t w4 = (outputs + 3) / 4 * 4 - outputs, w5 = (outputs + 4) / 5 * 5 - outputs;
+    return w5 < w4 ? 5 : 4;
+}
+
 __device__ __forceinline__ uint64_t shoup_pair(uint64_t x, const uint64_t *pair, uint64_t q) {
     return shoup(x, pair[0], pair[1], q);
 }
 
-constexpr int OB = 4;  // output primes per pass over the staged inputs
+// OB: output primes per pass over the staged inputs, 4 or 5, whichever wastes fewer
+// sums for the conversion's output count (25 Bsk primes at 24 data primes: 5 x 5)
 
 // acc[u] = sum_j v[j * BT + tid] * rows[(o0 + u) * terms + j] for u < OB (rows past
 // `outputs` repeat the last one; the caller does not use those sums).  The rows are
 // read-only tables with block-uniform addresses: read through the constant address
 // space, so that they become scalar loads.
 typedef const __attribute__((address_space(4))) uint64_t const_u64;
+template <int OB>
 __device__ __forceinline__ void dot_rows(Acc3 (&acc)[OB], const uint64_t *v, const uint64_t *__restrict__ rows,
                                          uint32_t terms, uint32_t o0, uint32_t outputs, int tid) {
     const_u64 *r[OB];
@@ -66,6 +74,7 @@ __device__ __forceinline__ void dot_rows(Acc3 (&acc)[OB], const uint64_t *v, con
 }
 
 // (1)-(2): fastbconvmTilde + smMrq.  x: [P][L][N] base q -> out: [P][bsk][N] base Bsk.
+template <int OB>
 __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restrict__ x, uint64_t *__restrict__ out,
                                                          BehzDev B, const DevPrime *__restrict__ qp, int logn,
                                                          size_t total) {
@@ -102,6 +111,7 @@ __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restr
 }
 
 // (6)-(7): multiply by t and fastFloor.  Tq: [P][L][N], Tb: [P][bsk][N] -> Z: [P][bsk][N].
+template <int OB>
 __global__ __launch_bounds__(BT) void behz_floor_kernel(const uint64_t *__restrict__ Tq, const uint64_t *__restrict__ Tb,
                                                         uint64_t *__restrict__ Z, BehzDev B,
                                                         const DevPrime *__restrict__ qp, int logn, size_t total) {
@@ -129,6 +139,7 @@ __global__ __launch_bounds__(BT) void behz_floor_kernel(const uint64_t *__restri
 }
 
 // (8): fastbconvSk.  Z: [P][bsk][N] -> out: [P][L][N] base q.
+template <int OB>
 __global__ __launch_bounds__(BT) void behz_sk_kernel(const uint64_t *__restrict__ Z, uint64_t *__restrict__ out,
                                                      BehzDev B, const DevPrime *__restrict__ qp, int logn,
                                                      size_t total) {
@@ -353,8 +364,13 @@ void bfv_multiply(const Context &c, const uint64_t *a, const uint64_t *b, uint64
     SHC_HIP(hipMemcpyAsync(Bq, b, qwords * 8, hipMemcpyDeviceToDevice, st));
     const size_t ext = (size_t)count * 2 * n;
     const size_t lds = (size_t)(L > B.bsz ? L : B.bsz) * BT * 8;
-    behz_extend_kernel<<<grid_for(ext), BT, lds, st>>>(a, Ab, B, qp, logn, ext);
-    behz_extend_kernel<<<grid_for(ext), BT, lds, st>>>(b, Bb, B, qp, logn, ext);
+    if (out_group(B.bsk) == 5) {
+        behz_extend_kernel<5><<<grid_for(ext), BT, lds, st>>>(a, Ab, B, qp, logn, ext);
+        behz_extend_kernel<5><<<grid_for(ext), BT, lds, st>>>(b, Bb, B, qp, logn, ext);
+    } else {
+        behz_extend_kernel<4><<<grid_for(ext), BT, lds, st>>>(a, Ab, B, qp, logn, ext);
+        behz_extend_kernel<4><<<grid_for(ext), BT, lds, st>>>(b, Bb, B, qp, logn, ext);
+    }
     launch_check("behz_extend");
     ntt_forward(Aq, count * 2 * L, L, qp, logn, NTT_FULL, st);
     ntt_forward(Bq, count * 2 * L, L, qp, logn, NTT_FULL, st);
@@ -365,9 +381,15 @@ void bfv_multiply(const Context &c, const uint64_t *a, const uint64_t *b, uint64
     ntt_inverse(Tq, count * 3 * L, L, qp, logn, NTT_FULL, st);
     ntt_inverse(Tb, count * 3 * bsk, bsk, bp, logn, NTT_FULL, st);
     const size_t fl = (size_t)count * 3 * n;
-    behz_floor_kernel<<<grid_for(fl), BT, lds, st>>>(Tq, Tb, Z, B, qp, logn, fl);
+    if (out_group(B.bsk) == 5)
+        behz_floor_kernel<5><<<grid_for(fl), BT, lds, st>>>(Tq, Tb, Z, B, qp, logn, fl);
+    else
+        behz_floor_kernel<4><<<grid_for(fl), BT, lds, st>>>(Tq, Tb, Z, B, qp, logn, fl);
     launch_check("behz_floor");
-    behz_sk_kernel<<<grid_for(fl), BT, lds, st>>>(Z, out, B, qp, logn, fl);
+    if (out_group(B.L) == 5)
+        behz_sk_kernel<5><<<grid_for(fl), BT, lds, st>>>(Z, out, B, qp, logn, fl);
+    else
+        behz_sk_kernel<4><<<grid_for(fl), BT, lds, st>>>(Z, out, B, qp, logn, fl);
     launch_check("behz_sk");
 }
 
