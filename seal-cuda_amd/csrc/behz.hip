@@ -4,7 +4,7 @@
 // RNSTool steps (src/utils/rns.cpp:849-960); GPU twins src/evaluator_cuda.cu:283-382,
 // src/utils/rns_cuda.cu:300-443.  Per coefficient, each base conversion is a short
 // dot product over the input base: the thread stages the scaled inputs in LDS and
-// accumulates the output primes four at a time (each staged word read once per four)
+// accumulates the output primes four or five at a time (each staged word read once per group)
 // with three-word product sums (accum.h; inputs < 2^61, at most 64 terms, so every sum
 // stays below 2^128), so intermediate residues never leave the kernel.  Scalar
 // multiplies that precede a conversion in the reference (x m_tilde, x t) are folded into
