@@ -77,6 +77,20 @@ def test_relinearize(scheme, logn, bits, L, count):
         np.testing.assert_array_equal(got[c], ck.relinearize(scheme, ct3[c], key, moduli, L, logn))
 
 
+def test_relinearize_columns_all_targets():
+    """A batch large enough that each ModUp columns block transforms its digit tile for
+    all L + 1 targets (104 x 20 x 8 tiles >= 16384 blocks); three ciphertexts against the
+    oracle, spread over the batch."""
+    logn, L, count = 13, 20, 104
+    ctx, moduli = make_ctx(CKKS, logn, [60] * (L + 1))
+    rng = np.random.default_rng(13)
+    ct3 = ck.uniform_residues(rng, (count, 3, L, 1 << logn), moduli[:L])
+    key = ck.random_key(rng, moduli, logn)
+    got = run_relin(ctx, ct3, key, L)
+    for c in (0, count // 2 - 1, count - 1):
+        np.testing.assert_array_equal(got[c], ck.relinearize(CKKS, ct3[c], key, moduli, L, logn))
+
+
 def test_relinearize_target_shape():
     """N = 2^16, 24 data primes + 1 special prime (60 bits each), one ciphertext."""
     logn, L = 16, 24
