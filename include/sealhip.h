@@ -276,7 +276,10 @@ int shc_bfv_plain_to_ntt(const shc_context *ctx, const uint64_t *d_plain, uint32
  * app/matmul.cuh:98-121: rotateVectorInplace / rotateRowsInplace then addInplace) with
  * c1's ModUp done once for all rotations.  The words equal the sum of separate
  * shc_apply_galois_keyswitch results.  A Galois key is first prepared for a level
- * (permuted copy plus a sign-pattern correction, shc_hoisted_key_size bytes). */
+ * (permuted copy plus a sign-pattern correction, shc_hoisted_key_size bytes).  A prepared
+ * key is bound to its level and Galois element: its last two words name them, and
+ * shc_rotate_sum / shc_rotate_many reject (SHC_INVALID_ARGUMENT) a key prepared for
+ * another limb_count or element. */
 int shc_hoisted_key_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
 int shc_hoisted_key_workspace_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
 int shc_hoisted_key_prepare(const shc_context *ctx, const uint64_t *d_galois_key, uint32_t galois_elt,
@@ -308,7 +311,11 @@ int shc_rotate_many(const shc_context *ctx, const uint64_t *d_in, uint32_t count
  * right before and `end_event` right after its ModUp rows-pass kernel -- the dominant
  * kernel of a key switch -- on that call's stream, then disarms.  For N < 2^13 the
  * events bracket the whole ModUp forward NTT; a batch large enough to run in slices
- * reports its first slice.  Not safe with concurrent key switches on one context. */
+ * reports its first slice.  shc_rotate_sum / shc_rotate_many (hoisted key switches)
+ * record them around their first pass of products with the prepared keys (up to four
+ * rotations over the shared ModUp limbs), their dominant kernel.  Arming is
+ * thread-safe: of key switches that run concurrently on one context (on different
+ * streams), at most one takes the armed probe. */
 int shc_probe_keyswitch(shc_context *ctx, void *begin_event, void *end_event);
 
 #ifdef __cplusplus

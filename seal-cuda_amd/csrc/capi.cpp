@@ -448,8 +448,7 @@ int shc_probe_keyswitch(shc_context *ctx, void *begin_event, void *end_event) {
     return guarded([&] {
         const auto &c = ctx_of(ctx);
         if (!begin_event || !end_event) shc::fail(shc::ERR_INVALID_ARGUMENT, "null event");
-        c.probe = shc::KernelProbe{(hipEvent_t)begin_event, (hipEvent_t)end_event};
-        c.probe_armed = true;
+        c.arm_probe(shc::KernelProbe{(hipEvent_t)begin_event, (hipEvent_t)end_event});
     });
 }
 

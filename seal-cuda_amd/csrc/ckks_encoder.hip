@@ -94,7 +94,7 @@ __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__r
         const bool neg = signbit(d);
         d = fabs(d);
         atomicMax(maxbits, (unsigned long long)__double_as_longlong(d));
-        if (d >= two64 * two64) d = 0;  // rejected on the host by the range check
+        if (!(d < two64 * two64)) d = 0;  // too large, inf or NaN: rejected on the host after the range check
         const uint64_t hi = (uint64_t)(d / two64);
         const uint64_t lo = (uint64_t)fmod(d, two64);
         for (uint32_t l = 0; l < L; l++) {
@@ -250,6 +250,9 @@ void ckks_encode(const Context &c, const double *values, uint32_t count, double 
     SHC_HIP(hipStreamSynchronize(st));
     double maxc;
     std::memcpy(&maxc, &mb, 8);
+    // an infinite or NaN input makes the maximum non-finite (NaN orders above +inf as
+    // an unsigned bit pattern), which no coefficient modulus can hold
+    if (!std::isfinite(maxc)) fail(ERR_INVALID_ARGUMENT, "encoded values are not finite");
     const int bits = (int)std::ceil(std::log2(std::max(maxc, 1.0))) + 1;
     if (bits >= T.total_bits || bits > 128) fail(ERR_INVALID_ARGUMENT, "encoded values are too large");
     ntt_forward(plain, L, L, c.primes(), c.logn, NTT_FULL, st);

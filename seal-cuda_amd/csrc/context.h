@@ -10,6 +10,7 @@
 #include "behz.h"
 #include "common.h"
 #include "hostmath.h"
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -44,10 +45,27 @@ struct Context {
 
     // Instrumentation: when armed, the next key switch records these events around its
     // ModUp rows-pass launch (the whole ModUp forward NTT for N < 2^13; the first
-    // slice's, for a batch run in slices) and disarms.  Not safe with concurrent key
-    // switches on one context.
+    // slice's, for a batch run in slices) and disarms.  Key switches that run while it
+    // is not armed only read probe_armed; arming and taking the probe hold probe_mu, so
+    // concurrent key switches on one context (different streams) take it at most once.
+    mutable std::mutex probe_mu;
     mutable KernelProbe probe{nullptr, nullptr};
-    mutable bool probe_armed = false;
+    mutable std::atomic<bool> probe_armed{false};
+
+    void arm_probe(KernelProbe p) const {
+        std::lock_guard<std::mutex> g(probe_mu);
+        probe = p;
+        probe_armed.store(true, std::memory_order_release);
+    }
+    // The armed probe, disarming it; false when none is armed.
+    bool take_probe(KernelProbe &out) const {
+        if (!probe_armed.load(std::memory_order_acquire)) return false;
+        std::lock_guard<std::mutex> g(probe_mu);
+        if (!probe_armed.load(std::memory_order_relaxed)) return false;
+        out = probe;
+        probe_armed.store(false, std::memory_order_relaxed);
+        return true;
+    }
 
     size_t key_count() const { return moduli.size(); }
     const DevPrime *primes() const { return d_primes.as<DevPrime>(); }

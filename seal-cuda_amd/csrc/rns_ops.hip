@@ -399,9 +399,8 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
     const uint64_t per_ct = (uint64_t)L * (L + 1);
     const uint64_t max_cts = ((uint64_t(1) << 34) >> c.logn) / per_ct;
     if (max_cts == 0) fail(ERR_INVALID_ARGUMENT, "level too large for key switching at this degree");
-    const KernelProbe probe = c.probe;
-    const bool armed = c.probe_armed;
-    c.probe_armed = false;
+    KernelProbe probe{nullptr, nullptr};
+    const bool armed = c.take_probe(probe);
     for (uint32_t done = 0; done < count;) {
         const uint32_t n = (uint32_t)(count - done < max_cts ? count - done : max_cts);
         switch_key_chunk(c, target + (size_t)done * target_stride, target_stride, ct + (size_t)done * ct_stride,

@@ -5,7 +5,9 @@
 namespace shc {
 
 // A Galois key prepared for hoisting at level L: the key permuted by sigma^-1 in the
-// NTT domain ([K-1][2][K][N]) followed by the sign-pattern correction corr'[2][L+1][N].
+// NTT domain ([K-1][2][K][N]) followed by the sign-pattern correction corr'[2][L+1][N]
+// and two header words naming L and the Galois element, which rotate_sum / rotate_many
+// check (a key prepared for another level or element is rejected).
 size_t hoisted_key_words(const Context &c, uint32_t L);
 size_t hoisted_key_workspace(const Context &c, uint32_t L);
 void hoisted_key_prepare(const Context &c, const uint64_t *galois_key, uint32_t elt, uint32_t L, uint64_t *out,

@@ -195,6 +195,25 @@ __global__ void accumulate_c0_kernel(uint64_t *__restrict__ out, const uint64_t 
 
 __global__ void clear_flag_kernel(unsigned long long *flag) { *flag = 0; }
 
+// A prepared key ends with two header words: HOISTED_TAG | L, and the Galois element.
+constexpr uint64_t HOISTED_TAG = 0x484f4953ull << 32;  // "HOIS"
+__global__ void write_hoisted_header_kernel(uint64_t *hdr, uint32_t L, uint32_t elt) {
+    hdr[0] = HOISTED_TAG | L;
+    hdr[1] = elt;
+}
+
+// flag |= 2 if a prepared key's header does not name level L and element elts[r].
+constexpr int HDR_CHUNK = 16;
+struct HeaderBatch {
+    const uint64_t *hdr[HDR_CHUNK];
+    uint32_t elt[HDR_CHUNK];
+};
+__global__ void check_hoisted_headers_kernel(HeaderBatch hb, uint32_t nr, uint32_t L, unsigned long long *flag) {
+    const uint32_t r = threadIdx.x;
+    if (r >= nr) return;
+    if (hb.hdr[r][0] != (HOISTED_TAG | L) || hb.hdr[r][1] != hb.elt[r]) atomicOr(flag, 2ull);
+}
+
 // flag |= 1 if any digit word is zero; digits of ct at digits + ct * dstride.
 __global__ void zero_digit_kernel(const uint64_t *__restrict__ digits, size_t dstride, size_t per_ct,
                                   unsigned long long *flag, size_t total) {
@@ -233,7 +252,7 @@ void check_level(const Context &c, uint32_t L) {
 
 size_t hoisted_key_words(const Context &c, uint32_t L) {
     const size_t K = c.key_count();
-    return ((K - 1) * 2 * K + 2 * ((size_t)L + 1)) * c.n;
+    return ((K - 1) * 2 * K + 2 * ((size_t)L + 1)) * c.n + 2;
 }
 
 size_t hoisted_key_workspace(const Context &c, uint32_t L) { return 3 * ((size_t)L + 1) * c.n * 8; }
@@ -261,6 +280,8 @@ void hoisted_key_prepare(const Context &c, const uint64_t *galois_key, uint32_t 
                                                           2 * nw);
     launch_check("correction");
     apply_galois(c, C, nw, out + (size_t)(K - 1) * 2 * K * n, nw, 2, L + 1, elt_inv, true, st);
+    write_hoisted_header_kernel<<<1, 1, 0, st>>>(out + hoisted_key_words(c, L) - 2, L, elt);
+    launch_check("hoisted key header");
 }
 
 size_t rotate_sum_workspace(const Context &c, uint32_t count, uint32_t L) { return rot_layout(c, count, L).total; }
@@ -275,6 +296,8 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
     check_level(c, L);
     if (!count) return;
     if (!rotations || !elts || !hoisted || !keys) fail(ERR_INVALID_ARGUMENT, "no rotations given");
+    KernelProbe probe{nullptr, nullptr};
+    const bool armed = c.take_probe(probe);  // brackets the first pass of hoisted products
     const RotLayout lay = rot_layout(c, count, L);
     if (!ws || ws_bytes < lay.total) fail(ERR_INVALID_ARGUMENT, "workspace too small for rotation sums");
     if (in == out) fail(ERR_INVALID_ARGUMENT, "rotate_sum cannot run in place");
@@ -309,11 +332,26 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
     clear_flag_kernel<<<1, 1, 0, st>>>(flag);
     zero_digit_kernel<<<blocks_for(count * lw), TPB, 0, st>>>(digits, dstride, lw, flag, count * lw);
     launch_check("zero_digit");
-    unsigned long long has_zero = 0;
-    SHC_HIP(hipMemcpyAsync(&has_zero, flag, 8, hipMemcpyDeviceToHost, st));
+    // each prepared key must be the one of this level and element (its header words)
+    const size_t hdr_off = hoisted_key_words(c, L) - 2;
+    for (uint32_t r0 = 0; r0 < rotations; r0 += HDR_CHUNK) {
+        HeaderBatch hb{};
+        const uint32_t nr = rotations - r0 < (uint32_t)HDR_CHUNK ? rotations - r0 : (uint32_t)HDR_CHUNK;
+        for (uint32_t q = 0; q < nr; q++) {
+            if (!hoisted[r0 + q] || !keys[r0 + q]) fail(ERR_INVALID_ARGUMENT, "null key pointer");
+            hb.hdr[q] = hoisted[r0 + q] + hdr_off;
+            hb.elt[q] = elts[r0 + q];
+        }
+        check_hoisted_headers_kernel<<<1, 64, 0, st>>>(hb, nr, L, flag);
+        launch_check("hoisted key headers");
+    }
+    unsigned long long flags = 0;
+    SHC_HIP(hipMemcpyAsync(&flags, flag, 8, hipMemcpyDeviceToHost, st));
     SHC_HIP(hipStreamSynchronize(st));
+    if (flags & 2)
+        fail(ERR_INVALID_ARGUMENT, "a hoisted key was prepared for another level or Galois element");
     const size_t ct_words = (size_t)count * 2 * lw;
-    if (has_zero) {
+    if (flags & 1) {
         for (uint32_t r = 0; r < rotations; r++) {
             uint64_t *dst = sum ? tmpct : out + r * ct_words;
             SHC_HIP(hipMemcpyAsync(dst, in, ct_words * 8, hipMemcpyDeviceToDevice, st));
@@ -344,6 +382,7 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
             rb.P[q] = Pm + (size_t)q * count * 2 * lw;
             rb.S[q] = Sm + (size_t)q * count * 2 * n;
         }
+        if (armed && r0 == 0) SHC_HIP(hipEventRecord(probe.begin, st));
         switch (nr) {
             case 1: rotation_products_kernel<1><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, diag, 2 * lw); break;
             case 2: rotation_products_kernel<2><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, diag, 2 * lw); break;
@@ -351,6 +390,7 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
             default: rotation_products_kernel<4><<<(uint32_t)blocks, TPB, 0, st>>>(ks.T, rb, corr_off, targets, L, (uint32_t)K, logn, count, tiles, diag, 2 * lw); break;
         }
         launch_check("rotation_products");
+        if (armed && r0 == 0) SHC_HIP(hipEventRecord(probe.end, st));
         for (uint32_t q = 0; q < nr; q++) {
             const uint32_t r = r0 + q;
             uint64_t *dst = sum ? out : out + r * ct_words;

@@ -450,6 +450,17 @@ class RefSession:
         return out
 
 
+def parallel_map(fn, items, workers=None):
+    """[fn(x) for x in items] on a thread pool.  The oracle keeps no mutable global
+    state and ctypes releases the GIL during a call, so oracle calls on large shapes
+    run side by side; at most 16 threads (one GPU box's CPU share)."""
+    from concurrent.futures import ThreadPoolExecutor
+    items = list(items)
+    workers = workers or max(1, min(16, os.cpu_count() or 1, len(items)))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        return list(pool.map(fn, items))
+
+
 def random_key(rng, moduli, logn):
     """Uniform key-switching key words [K-1][2][K][N] at key level (injected keys)."""
     K = len(moduli)

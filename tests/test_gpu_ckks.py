@@ -92,6 +92,19 @@ def test_encode_rejects_too_large():
         ctx.ckks_encode(dv.data_ptr(), 1, 2.0 ** 40, 2, plain.data_ptr(), eb.data_ptr(), ew, stream())
 
 
+@pytest.mark.parametrize("bad", [np.inf, -np.inf, np.nan])
+def test_encode_rejects_non_finite(bad):
+    n = 1 << 10
+    moduli = ck.coeff_modulus_create(n, [60, 40, 60])
+    ctx = hip().Context(CKKS, n, moduli)
+    dv = to_dev_f64(interleave(np.array([0.5 + 0j, complex(bad, 0.0), 0.25j])))
+    plain = empty_dev((2, n))
+    ew = ctx.ckks_encode_workspace_size()
+    eb = ws(ew)
+    with pytest.raises(hip().SealHipError, match="not finite"):
+        ctx.ckks_encode(dv.data_ptr(), 3, 2.0 ** 40, 2, plain.data_ptr(), eb.data_ptr(), ew, stream())
+
+
 def test_ckks_chain_end_to_end():
     logn = 15
     n = 1 << logn

@@ -80,3 +80,40 @@ def test_rotate_sum_equals_separate_rotations(scheme, logn, bits, L, count, step
 @pytest.mark.parametrize("scheme", [CKKS, BFV])
 def test_rotate_sum_zero_digit_falls_back(scheme):
     run_case(scheme, 12, [40, 40, 50], 2, 2, [1, 2], zero_digits=True, seed=5)
+
+
+def test_hoisted_key_bound_to_level_and_element():
+    """A prepared key names its level and Galois element; using it at another level or
+    for another element is an invalid argument, not wrong words."""
+    logn, n = 12, 1 << 12
+    moduli = ck.coeff_modulus_create(n, [60, 40, 40, 60])
+    K = len(moduli)
+    ctx = hip().Context(CKKS, n, moduli)
+    rng = np.random.default_rng(3)
+    key = to_dev(ck.random_key(rng, moduli, logn))
+    e1, e2 = ck.galois_elt(logn, 1), ck.galois_elt(logn, 2)
+    L = 3
+    hb, hw = ctx.hoisted_key_size(L), ctx.hoisted_key_workspace_size(L)
+    h = empty_dev(((hb + 7) // 8,))
+    ctx.hoisted_key_prepare(key.data_ptr(), e1, L, h.data_ptr(), ws(hw).data_ptr(), hw, stream())
+    for lvl, elt in ((L, e2), (L - 1, e1)):
+        din = to_dev(ck.uniform_residues(rng, (1, 2, lvl, n), moduli[:lvl]))
+        out = empty_dev((1, 2, lvl, n))
+        rw = ctx.rotate_sum_workspace_size(1, lvl)
+        with pytest.raises(hip().SealHipError, match="prepared for another level"):
+            ctx.rotate_sum(din.data_ptr(), 1, lvl, [elt], [h.data_ptr()], [key.data_ptr()], out.data_ptr(),
+                           ws(rw).data_ptr(), rw, stream())
+    # the matching level and element still work
+    din = to_dev(ck.uniform_residues(rng, (1, 2, L, n), moduli[:L]))
+    out = empty_dev((1, 2, L, n))
+    rw = ctx.rotate_sum_workspace_size(1, L)
+    ctx.rotate_sum(din.data_ptr(), 1, L, [e1], [h.data_ptr()], [key.data_ptr()], out.data_ptr(), ws(rw).data_ptr(),
+                   rw, stream())
+    want = ck.apply_galois_keyswitch(CKKS, from_dev(din)[0], e1, from_dev(key), moduli, L, logn)
+    np.testing.assert_array_equal(from_dev(out)[0], want)
+
+
+def test_rotate_sum_seven_rotations_two_passes():
+    """Seven rotations: two passes of the hoisted products (4 + 3) accumulate into the
+    output and reuse the product buffers."""
+    run_case(CKKS, 12, [60, 40, 40, 60], 3, 3, [1, 2, 3, 5, -1, -4, 7], seed=77)
