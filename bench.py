@@ -47,6 +47,7 @@ KEYPROD_TRAFFIC_B128 = 40.93e9 + 3.36e9
 KEYPROD_VALU_B128 = 8084848640
 VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 CKKS = 2
+T_BFV = 65537
 
 
 def parse():
@@ -55,14 +56,18 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="mulrelin",
-                   choices=["mulrelin", "ntt", "copy", "chain", "rotsum", "matvec", "serve"])
+                   choices=["mulrelin", "ntt", "copy", "chain", "rotsum", "matvec", "serve", "bfv8192"])
     p.add_argument("--no-bfv", action="store_true", help="skip the BFV leg of the mulrelin workload")
     p.add_argument("--batch", type=int, default=128)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-plain", action="store_true", help="skip the plain per-ciphertext leg")
-    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--cpu-seconds", type=float, default=6.0,
+                   help="CPU-baseline budget per workload (half one thread, half all threads)")
     p.add_argument("--no-serve", action="store_true", help="skip the served-latency leg")
     p.add_argument("--serve-seconds", type=float, default=1.5, help="arrival span of the served-latency leg")
+    p.add_argument("--headline-only", action="store_true",
+                   help="mulrelin: skip the other workloads' sub-objects in the JSON line")
+    p.add_argument("--sub-steps", type=int, default=5, help="timed steps of each sub-workload in the default line")
     return p.parse_args()
 
 
@@ -89,9 +94,64 @@ class Timer:
         self.ms = self.a.elapsed_time(self.b)
 
 
+def cpu_threads():
+    """Host threads for the all-core CPU baseline: every core the process may use, at
+    most 16 (one GPU box's CPU share)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline(make_worker, unit, units_per_call, budget_s, sample):
+    """Times the CPU path on this host: make_worker() returns a callable that does one
+    call of independent work (its own session and buffers; `units_per_call` units of the
+    metric).  Half the budget runs one worker on one thread (latency per call, rate);
+    the other half runs one worker per thread on cpu_threads() threads at once
+    (all-core throughput; the reference CPU path and the oracle hold no global mutable
+    state, and ctypes releases the GIL during their calls).  The reference CPU library
+    (oracle/_ref) is used where it was shipped with this tree, else the oracle."""
+    import threading
+    import checkers as ck
+    kind = "reference" if ck.ref() is not None else "port"
+    half = budget_s / 2
+    w = make_worker(kind)
+    calls = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < half or calls == 0:
+        w()
+        calls += 1
+    dt1 = time.perf_counter() - t0
+    nt = cpu_threads()
+    workers = ck.parallel_map(lambda _: make_worker(kind), range(nt), workers=nt)  # contexts built side by side
+    counts = [0] * nt
+    start = time.perf_counter()
+    stop = start + half
+
+    def loop(i):
+        while time.perf_counter() < stop or counts[i] == 0:
+            workers[i]()
+            counts[i] += 1
+
+    th = [threading.Thread(target=loop, args=(i,)) for i in range(nt)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dtn = time.perf_counter() - start
+    return {"value": calls * units_per_call / dt1, "unit": unit, "cores": 1, "kind": kind,
+            "latency_ms": dt1 / calls * 1e3,
+            "sample": f"{sample}; one thread: {calls} call(s) in {dt1:.1f} s",
+            "all_core": {"value": sum(counts) * units_per_call / dtn, "unit": unit, "threads": nt,
+                         "nproc": os.cpu_count(),
+                         "sample": f"{nt} threads, one independent call each at a time: {sum(counts)} call(s) "
+                                   f"in {dtn:.1f} s"}}
+
+
 def cpu_baseline_mulrelin(moduli, logn, budget_s):
-    """The reference CPU path (oracle/_ref, src/evaluator.cpp multiply + relinearize) when
-    it was shipped with this tree, else the oracle restatement, on one host thread."""
+    """CKKS multiply + relinearize at the headline shape through the reference CPU path
+    (src/evaluator.cpp multiply + relinearize; oracle/_ref), else the oracle."""
     import numpy as np
     import checkers as ck
     n, K = 1 << logn, len(moduli)
@@ -100,21 +160,15 @@ def cpu_baseline_mulrelin(moduli, logn, budget_s):
     a = ck.uniform_residues(rng, (2, L, n), moduli[:L])
     b = ck.uniform_residues(rng, (2, L, n), moduli[:L])
     key = ck.random_key(rng, moduli, logn)
-    kind = "reference" if ck.ref() is not None else "port"
-    sess = ck.RefSession(CKKS, n, moduli) if kind == "reference" else None
-    ops = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s or ops == 0:
-        if sess is not None:
-            ct3 = sess.multiply(a, b, L)
-            sess.relinearize(ct3, key, L)
-        else:
-            ct3 = ck.ckks_multiply(a, b, moduli[:L], logn)
-            ck.relinearize(CKKS, ct3, key, moduli, L, logn)
-        ops += 1
-    dt = time.perf_counter() - t0
-    return {"value": ops / dt, "unit": "mul+relin ops/s", "cores": 1, "kind": kind,
-            "sample": f"{ops} CKKS mul+relin at N=2^{logn}, L={L} (+1 special), one thread, {dt:.1f} s"}
+
+    def make(kind):
+        if kind == "reference":
+            sess = ck.RefSession(CKKS, n, moduli)
+            return lambda: sess.relinearize(sess.multiply(a, b, L), key, L)
+        return lambda: ck.relinearize(CKKS, ck.ckks_multiply(a, b, moduli[:L], logn), key, moduli, L, logn)
+
+    return cpu_baseline(make, "mul+relin ops/s", 1, budget_s,
+                        f"CKKS mul+relin at N=2^{logn}, L={L} (+1 special), injected uniform words")
 
 
 def served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, requests, max_batch, seed=7):
@@ -291,27 +345,36 @@ def bench_mulrelin(args):
 
 
 def cpu_baseline_ntt(moduli, logn, budget_s):
+    """Forward + inverse negacyclic NTT of limbs at N = 2^logn through the reference CPU
+    path (nttNegacyclicHarvey / inverseNttNegacyclicHarvey, src/utils/ntt.cpp:158-213),
+    64 limbs of one prime per call (the tables are built once per call), cycling
+    through the primes."""
     import numpy as np
     import checkers as ck
     n = 1 << logn
+    per_call = 64
     rng = np.random.default_rng(3)
-    a = rng.integers(0, moduli[0], size=n, dtype=np.uint64)
-    ref = ck.ref()
-    kind = "reference" if ref is not None else "port"
-    t0 = time.perf_counter()
-    limbs = 0
-    while time.perf_counter() - t0 < budget_s:
-        q = moduli[limbs % len(moduli)]
-        b = a % np.uint64(q)
-        if ref is not None:
-            ref.ref_ntt(logn, q, ck.ptr(b), 1, 0)
-            ref.ref_ntt(logn, q, ck.ptr(b), 1, 1)
-        else:
-            b = ck.ntt(ck.ntt(b, q, logn), q, logn, inverse=True)
-        limbs += 1
-    dt = time.perf_counter() - t0
-    return {"value": limbs / dt, "unit": "limb fwd+inv NTT/s", "cores": 1, "kind": kind,
-            "sample": f"{limbs} limbs of N=2^{logn} (forward+inverse), one thread"}
+    base = [rng.integers(0, q, size=per_call * n, dtype=np.uint64) for q in moduli[:4]]
+
+    def make(kind):
+        state = {"k": 0}
+        buf = np.empty(per_call * n, dtype=np.uint64)
+
+        def run():
+            k = state["k"] % len(base)
+            state["k"] += 1
+            q = moduli[k]
+            buf[:] = base[k]
+            if kind == "reference":
+                ck.ref().ref_ntt(logn, q, ck.ptr(buf), per_call, 0)
+                ck.ref().ref_ntt(logn, q, ck.ptr(buf), per_call, 1)
+            else:
+                ck.oracle().or_ntt_batch(logn, q, ck.ptr(buf), per_call, 0)
+                ck.oracle().or_ntt_batch(logn, q, ck.ptr(buf), per_call, 1)
+        return run
+
+    return cpu_baseline(make, "limb fwd+inv/s", per_call, budget_s,
+                        f"forward + inverse NTT, N=2^{logn}, 60-bit primes, {per_call} limbs per call")
 
 
 def bench_ntt(args):
@@ -397,9 +460,22 @@ def bench_bfv_leg(torch, sealhip, B, steps, warmup):
         for _ in range(steps):
             step()
     ms = tm.ms / steps
+    # the batch must give each pair the same words as the per-ciphertext path
+    probe = [0, B // 2 + 1, B - 1]
+    batched_words = out[probe, :2].clone()
+    mw1, kw1 = ctx.multiply_workspace_size(1, L), ctx.keyswitch_workspace_size(1, L)
+    stride2, stride3 = 2 * L * n * 8, 3 * L * n * 8
+    for i in probe:
+        ctx.multiply(a.data_ptr() + i * stride2, b.data_ptr() + i * stride2, out.data_ptr() + i * stride3, 1, L,
+                     ws.data_ptr(), mw1, st)
+        ctx.relinearize(out.data_ptr() + i * stride3, key.data_ptr(), 1, L, ws.data_ptr(), kw1, st)
+    torch.cuda.synchronize()
+    if not torch.equal(out[probe, :2], batched_words):
+        raise SystemExit("bench self-check failed: BFV batched and per-ciphertext results differ")
     del a, b, key, out, ws
     torch.cuda.empty_cache()
     return {"ops_s": B / (ms * 1e-3), "ms_per_step": ms, "unit": "mul+relin ops/s",
+            "self_check": f"batched == per-ciphertext words on pairs {probe}",
             "config": f"BFV multiply (BEHZ) + relinearize, N=2^16, 24 data + 1 special 60-bit primes, t={t}, "
                       f"batch of {B} ciphertext pairs, coefficient form, key resident"}
 
@@ -443,12 +519,14 @@ def bench_chain(args):
             max_ws = max(max_ws, ctx.keyswitch_workspace_size(c, l - s), ctx.rescale_workspace_size(c, 2, l - s))
     ws = torch.empty(((max_ws + 7) // 8,), dtype=torch.int64, device="cuda")
 
-    def chain(st):
+    def chain(st, probe_at=None, probe=None):
         for l, (c, x, ys, prods, accs, rels) in bufs.items():
             cur = x
             for s in range(3):
                 ll = l - s
                 ctx.ckks_multiply(cur.data_ptr(), ys[s].data_ptr(), prods[s].data_ptr(), c, ll, st)
+                if probe_at == (l, s):
+                    ctx.probe_keyswitch(*probe)
                 ctx.relinearize(prods[s].data_ptr(), key.data_ptr(), c, ll, ws.data_ptr(), max_ws, st)
                 rels[s].copy_(prods[s][:, :2])   # the size-2 results, packed
                 ctx.rescale_to_next(rels[s].data_ptr(), accs[s].data_ptr(), c, 2, ll, ws.data_ptr(), max_ws, st)
@@ -458,6 +536,13 @@ def bench_chain(args):
     chain(st)
     torch.cuda.synchronize()
     eager = {l: b[4][2].clone() for l, b in bufs.items()}
+    # dominant kernel: the fused ModUp rows pass + key products of the largest level's
+    # first key switch, bracketed by HIP events on the library's launch stream (eager)
+    top = max(groups)
+    probe = (sealhip.HipEvent(), sealhip.HipEvent())
+    chain(st, probe_at=(top, 0), probe=probe)
+    torch.cuda.synchronize()
+    kp_ms = probe[0].elapsed_ms(probe[1])
     t0 = time.perf_counter()
     with Timer(torch) as te:
         for _ in range(args.steps):
@@ -488,15 +573,74 @@ def bench_chain(args):
     wall = time.perf_counter() - t0
     ms = tg.ms / args.steps
     ops = pairs * 3
-    return {"metric": "CKKS mul+relin+rescale per second, N=2^15 chain of 3, uneven levels, HIP graph",
-            "value": ops / (ms * 1e-3), "unit": "mul+relin+rescale/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded uniform residues; injected key)",
-            "config": {"workload": f"CKKS chain: N=2^15, 15+1 primes (60, 40x14, 60), {pairs} pairs at levels "
-                                   f"U[4,15] ({len(groups)} level groups), 3 x (mul, relin, rescale)",
-                       "parallelism": "single GPU"},
-            "eager_ms_per_step": eager_ms, "graph_over_eager": eager_ms / ms,
-            "self_check": "every graph replay equals the eager chain's words", "wall_s": wall}
+    limb = n * 8
+    # step: per (group, round) at level m, inputs 4m limbs and output 2(m-1) limbs per
+    # pair, the key's 2m(m+1) limbs once per key-switch call
+    step_bytes = sum((c * (4 * (l - s) + 2 * (l - s - 1)) + 2 * (l - s) * (l - s + 1)) * limb
+                     for l, c in groups.items() for s in range(3))
+    # the probed launch at level L = top for c pairs: L x L ModUp limbs and L input limbs
+    # per pair, the key's 2L(L+1) limbs, 2(L+1) product limbs per pair, L+1 twiddle tables
+    c_top, L_top = groups[top], top
+    kp_bytes = ((c_top * (L_top * L_top + L_top + 2 * (L_top + 1)) + 2 * L_top * (L_top + 1)) * limb
+                + (L_top + 1) * n * 16)
+    out = {"metric": "CKKS mul+relin+rescale per second, N=2^15 chain of 3, uneven levels, HIP graph",
+           "value": ops / (ms * 1e-3), "unit": "mul+relin+rescale/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded uniform residues; injected key)",
+           "config": {"workload": f"CKKS chain: N=2^15, 15+1 primes (60, 40x14, 60), {pairs} pairs at levels "
+                                  f"U[4,15] ({len(groups)} level groups), 3 x (mul, relin, rescale)",
+                      "parallelism": "single GPU"},
+           "roofline": {"bound": "hbm", "achieved": kp_bytes / (kp_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": kp_bytes / (kp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": f"ntt_rows_keyprod_kernel<15,...>: fused ModUp rows pass + key products of the "
+                                  f"level-{L_top} group ({c_top} pairs), eager pass",
+                        "algorithmic_bytes": kp_bytes, "avg_ms": kp_ms},
+           "step_roofline": {"achieved": step_bytes / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                             "frac": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "algorithmic_bytes_per_step": step_bytes},
+           "eager_ms_per_step": eager_ms, "graph_over_eager": eager_ms / ms,
+           "self_check": "every graph replay equals the eager chain's words", "wall_s": wall}
+    del bufs, ws, key
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_chain(moduli, logn, [int(x) for x in levels], args.cpu_seconds)
+    return out
+
+
+def cpu_baseline_chain(moduli, logn, levels, budget_s):
+    """The chain's pairs in order (their levels as drawn for the GPU batch), one pair per
+    call: 3 x (multiply, relinearize, rescale) through the reference CPU path
+    (src/evaluator.cpp), injected uniform words."""
+    import numpy as np
+    import checkers as ck
+    n, K = 1 << logn, len(moduli)
+    rng = np.random.default_rng(19)
+    key = ck.random_key(rng, moduli, logn)
+    top = max(levels)
+    xs = ck.uniform_residues(rng, (2, top, n), moduli[:top])
+    ys = ck.uniform_residues(rng, (2, top, n), moduli[:top])
+
+    def make(kind):
+        state = {"k": 0}
+        sess = ck.RefSession(CKKS, n, moduli) if kind == "reference" else None
+
+        def run():
+            l = levels[state["k"] % len(levels)]
+            state["k"] += 1
+            cur = np.ascontiguousarray(xs[:, :l])
+            for s in range(3):
+                ll = l - s
+                y = np.ascontiguousarray(ys[:, :ll])
+                if sess is not None:
+                    cur = sess.rescale(sess.relinearize(sess.multiply(cur, y, ll), key, ll), ll)
+                else:
+                    o2 = ck.relinearize(CKKS, ck.ckks_multiply(cur, y, moduli[:ll], logn), key, moduli, ll, logn)
+                    cur = ck.rescale(CKKS, o2, moduli, logn)
+        return run
+
+    return cpu_baseline(make, "mul+relin+rescale/s", 3, budget_s,
+                        "one pair per call (levels as drawn for the GPU batch, U[4,15]): "
+                        "3 x (multiply, relinearize, rescale), N=2^15")
 
 
 def bench_rotsum(args):
@@ -554,11 +698,16 @@ def bench_rotsum(args):
     for _ in range(args.warmup):
         hoisted()
     torch.cuda.synchronize()
+    # dominant kernel: the first pass of hoisted products (4 rotations over the shared
+    # ModUp limbs), bracketed by HIP events on the library's launch stream
+    probes = [(sealhip.HipEvent(), sealhip.HipEvent()) for _ in range(args.steps)]
     t0 = time.perf_counter()
     with Timer(torch) as th:
-        for _ in range(args.steps):
+        for s_ in range(args.steps):
+            ctx.probe_keyswitch(*probes[s_])
             hoisted()
     wall = time.perf_counter() - t0
+    prod_ms = sum(b.elapsed_ms(e) for b, e in probes) / args.steps
     separate()
     torch.cuda.synchronize()
     if not torch.equal(out, acc):
@@ -567,15 +716,70 @@ def bench_rotsum(args):
         separate()
     ms = th.ms / args.steps
     rots = cts * len(steps)
-    return {"metric": "CKKS rotations/s, N=2^16, 16 ct x 15 rotations summed, hoisted",
-            "value": rots / (ms * 1e-3), "unit": "rotations/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded uniform residues; injected Galois keys)",
-            "config": {"workload": "hoisted rotations: N=2^16, 24+1 primes (60, 50x23, 60), 16 ciphertexts, "
-                                   "rotations by 2^k, k=0..14, summed",
-                       "parallelism": "single GPU"},
-            "separate_ms_per_step": ts.ms, "hoisted_over_separate": ts.ms / ms,
-            "self_check": "hoisted words == sum of 15 separate rotations", "wall_s": wall}
+    limb = n * 8
+    # step (per ciphertext: 2L limbs in, 2L out; each prepared key once: 2L(L+1) key' limbs
+    # + 2(L+1) correction limbs)
+    key_limbs = 2 * L * (L + 1) + 2 * (L + 1)
+    step_bytes = (cts * 4 * L + len(steps) * key_limbs) * limb
+    # products pass, R = 4 rotations: the L x L ModUp limbs and L own-prime input limbs per
+    # ciphertext, R prepared keys, R x 2(L+1) product limbs per ciphertext
+    R = 4
+    prod_bytes = (cts * (L * L + L + R * 2 * (L + 1)) + R * key_limbs) * limb
+    out_json = {"metric": "CKKS rotations/s, N=2^16, 16 ct x 15 rotations summed, hoisted",
+                "value": rots / (ms * 1e-3), "unit": "rotations/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "u64",
+                "data": "synthetic (seeded uniform residues; injected Galois keys)",
+                "config": {"workload": "hoisted rotations: N=2^16, 24+1 primes (60, 50x23, 60), 16 ciphertexts, "
+                                       "rotations by 2^k, k=0..14, summed",
+                           "parallelism": "single GPU"},
+                "roofline": {"bound": "hbm", "achieved": prod_bytes / (prod_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": prod_bytes / (prod_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "traffic": None,
+                             "kernel": "rotation_products_kernel<4>: products of 4 rotations' prepared keys over "
+                                       "the shared ModUp limbs (first pass of each step)",
+                             "algorithmic_bytes": prod_bytes, "avg_ms": prod_ms},
+                "step_roofline": {"achieved": step_bytes / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                                  "frac": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "algorithmic_bytes_per_step": step_bytes},
+                "separate_ms_per_step": ts.ms, "hoisted_over_separate": ts.ms / ms,
+                "self_check": "hoisted words == sum of 15 separate rotations", "wall_s": wall}
+    del keys, hkeys, ws, x, out, tmp, acc
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        out_json["cpu_baseline"] = cpu_baseline_rotation(moduli, logn, steps, args.cpu_seconds)
+    return out_json
+
+
+def cpu_baseline_rotation(moduli, logn, steps, budget_s):
+    """Rotations at N=2^16 with 24 + 1 primes through the reference CPU path
+    (Evaluator::applyGaloisInplace, src/evaluator.cpp:2153-2307: one full key switch per
+    rotation; the reference has no hoisting), one rotation per call, cycling through the
+    15 steps, injected uniform words (one key shared by the steps)."""
+    import numpy as np
+    import checkers as ck
+    n, K = 1 << logn, len(moduli)
+    L = K - 1
+    rng = np.random.default_rng(29)
+    ct = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    key = ck.random_key(rng, moduli, logn)
+    elts = [ck.galois_elt(logn, s) for s in steps]
+
+    def make(kind):
+        state = {"k": 0}
+        sess = ck.RefSession(CKKS, n, moduli) if kind == "reference" else None
+
+        def run():
+            e = elts[state["k"] % len(elts)]
+            state["k"] += 1
+            if sess is not None:
+                sess.apply_galois(ct, e, key, L)
+            else:
+                ck.apply_galois_keyswitch(CKKS, ct, e, key, moduli, L, logn)
+        return run
+
+    return cpu_baseline(make, "rotations/s", 1, budget_s,
+                        f"one rotation (applyGaloisInplace) per call at N=2^{logn}, L={L} (+1 special)")
 
 
 def bench_matvec(args):
@@ -710,13 +914,13 @@ class MatvecServer:
         self.wsz = self.mv.workspace_sizes(max_batch)
         self.ws = torch.empty(((self.wsz + 7) // 8,), dtype=torch.int64, device="cuda")
 
-    def run(self, inputs):
+    def run(self, inputs, around_sums=None):
         """One batch: the pool entries `inputs` gathered into the batch buffer, then one
         batched matvec into self.out[:len(inputs)]."""
         for k, i in enumerate(inputs):
             self.x[k].copy_(self.pool[i])
         self.mv.run(self.x.data_ptr(), len(inputs), self.out.data_ptr(), self.babybuf.data_ptr(),
-                    self.inner.data_ptr(), self.ws.data_ptr(), self.wsz, self.st)
+                    self.inner.data_ptr(), self.ws.data_ptr(), self.wsz, self.st, around_sums=around_sums)
 
 
 def serve_mixed(torch, servers, rate, requests, max_batch, seed):
@@ -811,11 +1015,25 @@ def bench_serve(args):
             torch.cuda.synchronize()
             if not torch.equal(sv.out[0], batched[k]):
                 raise SystemExit(f"bench self-check failed: {name} request {i} differs alone and in a batch")
+    # dominant kernel of the mix: the inner sums of a full CKKS batch (all g giant steps'
+    # plain-product sums over the b baby rotations, shc_multiply_plain_accumulate_many)
+    ck_sv = servers[1]
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    sums_ms = []
+    for _ in range(max(args.steps, 1)):
+        ck_sv.run(list(range(ck_sv.max_batch)), around_sums=(ev[0].record, ev[1].record))
+        torch.cuda.synchronize()
+        sums_ms.append(ev[0].elapsed_time(ev[1]))
+    sums_ms = sum(sums_ms) / len(sums_ms)
+    m, bb, gg, Lc = ck_sv.max_batch, 32, 32, ck_sv.L
+    # reads the b baby rotations of m requests (2L limbs each) and the g x b diagonals
+    # (L limbs each), writes g inner sums per request (2L limbs)
+    sums_bytes = (bb * m * 2 * Lc + gg * bb * Lc + gg * m * 2 * Lc) * ck_sv.n * 8
     rate = 0.5 * mix_cap
     reqs = max(200, int(rate * args.serve_seconds))
     batched = serve_mixed(torch, servers, rate, reqs, B, seed=11)
     plain = serve_mixed(torch, servers, rate, reqs, 1, seed=11)
-    return {"metric": "mixed BFV/CKKS matvec requests/s served; p50/p99 latency", "value": batched["served_per_s"],
+    out = {"metric": "mixed BFV/CKKS matvec requests/s served; p50/p99 latency", "value": batched["served_per_s"],
             "unit": "requests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": 1e3 / batched["served_per_s"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u64",
@@ -834,7 +1052,223 @@ def bench_serve(args):
                                                               "ckks": servers[1].resident_bytes}},
             "arrivals": {"kind": "poisson", "rate_per_s": rate, "requests": reqs},
             "batched": batched, "plain": plain, "batched_p99_over_plain_p99": batched["p99_ms"] / plain["p99_ms"],
+            "roofline": {"bound": "hbm", "achieved": sums_bytes / (sums_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": sums_bytes / (sums_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": f"plain_dot_many_kernel: the {gg} inner sums of a full CKKS batch ({m} requests, "
+                                   f"{bb} baby rotations, L={Lc})",
+                         "algorithmic_bytes": sums_bytes, "avg_ms": sums_ms},
             "self_check": "per type, the first and last request of a full batch give the same words alone"}
+    del servers
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_serve(bfv_moduli, ckks_moduli, args.cpu_seconds)
+    return out
+
+
+def cpu_baseline_serve(bfv_moduli, ckks_moduli, budget_s):
+    """The mixed matvec request stream through the reference CPU path, by sampling: one
+    call does 1/32 of a CKKS request (N=2^15, L=15: 2 of its 62 rotations, 32 of its
+    1024 NTT-form plain products and additions) and 1/32 of a BFV request (N=8192, L=4:
+    4 of its 126 rotations, 128 of its 4096 plain products and additions), i.e. 1/16 of a
+    request of the half-and-half mix; the reference evaluates a matvec as rotations,
+    plain products and additions (app/matmul.cuh:98-121), with no hoisting."""
+    import numpy as np
+    import checkers as ck
+    rng = np.random.default_rng(31)
+    nb, nc = 8192, 1 << 15
+    Lb, Lc = len(bfv_moduli) - 1, len(ckks_moduli) - 1
+    ct_b = ck.uniform_residues(rng, (2, Lb, nb), bfv_moduli[:Lb])
+    key_b = ck.random_key(rng, bfv_moduli, 13)
+    pl_b = rng.integers(0, T_BFV, nb, dtype=np.uint64)
+    ct_c = ck.uniform_residues(rng, (2, Lc, nc), ckks_moduli[:Lc])
+    key_c = ck.random_key(rng, ckks_moduli, 15)
+    pl_c = ck.uniform_residues(rng, (Lc, nc), ckks_moduli[:Lc])
+    e_b, e_c = ck.galois_elt(13, 1), ck.galois_elt(15, 1)
+
+    def make(kind):
+        if kind == "reference":
+            sb = ck.RefSession(1, nb, bfv_moduli, T_BFV)
+            sc = ck.RefSession(CKKS, nc, ckks_moduli)
+            rot_b = lambda: sb.apply_galois(ct_b, e_b, key_b, Lb)
+            rot_c = lambda: sc.apply_galois(ct_c, e_c, key_c, Lc)
+            mul_b = lambda: sb.plain_op(2, ct_b, pl_b, Lb)
+            mul_c = lambda: sc.plain_op(2, ct_c, pl_c, Lc)
+            add_b = lambda: sb.ct_binary(0, ct_b, ct_b, Lb)
+            add_c = lambda: sc.ct_binary(0, ct_c, ct_c, Lc)
+        else:
+            rot_b = lambda: ck.apply_galois_keyswitch(1, ct_b, e_b, key_b, bfv_moduli, Lb, 13)
+            rot_c = lambda: ck.apply_galois_keyswitch(CKKS, ct_c, e_c, key_c, ckks_moduli, Lc, 15)
+            mul_b = lambda: ck.bfv_multiply_plain(ct_b, pl_b, bfv_moduli[:Lb], T_BFV, 13)
+            mul_c = lambda: ck.multiply_plain_ntt(ct_c, pl_c, ckks_moduli, 15)
+            add_b = lambda: ck.mod_add(ct_b, ct_b, bfv_moduli)
+            add_c = lambda: ck.mod_add(ct_c, ct_c, ckks_moduli)
+
+        def run():
+            for _ in range(2):
+                rot_c()
+            for _ in range(32):
+                mul_c()
+                add_c()
+            for _ in range(4):
+                rot_b()
+            for _ in range(128):
+                mul_b()
+                add_b()
+        return run
+
+    return cpu_baseline(make, "requests/s", 1.0 / 16, budget_s,
+                        "1/32 of a CKKS request + 1/32 of a BFV request per call (rotations, plain products and "
+                        "additions in the request's proportions)")
+
+
+def bench_bfv8192(args):
+    """The CPU-runnable BFV path: N=8192, BFVDefault(8192) primes {0x7fffffd8001,
+    0x7fffffc8001, 0xfffffffc001, 0xffffff6c001, 0xfffffebc001} (4 data + special),
+    t = 65537, real keys (device key generation).  Two vectors of 8192 slots uniform mod t
+    are batch-encoded and encrypted once; a step is multiply -> relinearize -> decrypt ->
+    batch-decode of that pair (one request, latency-bound).  Self-checks: the decoded
+    slots are the slot-wise products mod t, and the multiply + relinearize words equal the
+    oracle's on the same ciphertexts and key."""
+    import numpy as np
+    import torch
+    import sealhip
+    import checkers as ck
+    sealhip.init(0)
+    logn, n, t = 13, 8192, T_BFV
+    moduli = [0x7fffffd8001, 0x7fffffc8001, 0xfffffffc001, 0xffffff6c001, 0xfffffebc001]
+    K = len(moduli)
+    L = K - 1
+    ctx = sealhip.Context(sealhip.SCHEME_BFV, n, moduli, t)
+    st = torch.cuda.current_stream().cuda_stream
+    dev = lambda shape: torch.empty(shape, dtype=torch.int64, device="cuda")
+    wsb = lambda nbytes: dev(((max(nbytes, 8) + 7) // 8,))
+    sk, pk, rk = dev((K, n)), dev((2, K, n)), dev((K - 1, 2, K, n))
+    ctx.keygen_secret(1, sk.data_ptr(), st)
+    ctx.keygen_public(sk.data_ptr(), 2, pk.data_ptr(), st)
+    kw = ctx.keygen_workspace_size()
+    ctx.keygen_relin(sk.data_ptr(), 3, rk.data_ptr(), wsb(kw).data_ptr(), kw, st)
+    rng = np.random.default_rng(1)
+    x = rng.integers(0, t, n, dtype=np.uint64)
+    y = rng.integers(0, t, n, dtype=np.uint64)
+    px, py = dev((n,)), dev((n,))
+    for v, p_ in ((x, px), (y, py)):
+        dv = torch.from_numpy(v.view(np.int64).copy()).cuda()
+        ctx.batch_encode(dv.data_ptr(), n, p_.data_ptr(), st)
+    cts = dev((2, 2, L, n))
+    ew = ctx.encrypt_workspace_size(L)
+    eb = wsb(ew)
+    t0 = time.perf_counter()
+    ctx.encrypt(pk.data_ptr(), px.data_ptr(), L, 7, cts[0].data_ptr(), eb.data_ptr(), ew, st)
+    ctx.encrypt(pk.data_ptr(), py.data_ptr(), L, 8, cts[1].data_ptr(), eb.data_ptr(), ew, st)
+    torch.cuda.synchronize()
+    enc_ms = (time.perf_counter() - t0) * 1e3
+    prod = dev((1, 3, L, n))
+    mw, kw1, dw = ctx.multiply_workspace_size(1, L), ctx.keyswitch_workspace_size(1, L), ctx.decrypt_workspace_size(L)
+    mb, kb, db = wsb(mw), wsb(kw1), wsb(dw)
+    out, vals, scratch = dev((n,)), dev((n,)), dev((n,))
+
+    def step(probe=None):
+        ctx.multiply(cts[0].data_ptr(), cts[1].data_ptr(), prod.data_ptr(), 1, L, mb.data_ptr(), mw, st)
+        if probe:
+            ctx.probe_keyswitch(*probe)
+        ctx.relinearize(prod.data_ptr(), rk.data_ptr(), 1, L, kb.data_ptr(), kw1, st)
+        ctx.decrypt(prod.data_ptr(), 2, sk.data_ptr(), L, out.data_ptr(), db.data_ptr(), dw, st)
+        ctx.batch_decode(out.data_ptr(), vals.data_ptr(), scratch.data_ptr(), st)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    steps = max(args.steps, 20)
+    probes = [(sealhip.HipEvent(), sealhip.HipEvent()) for _ in range(steps)]
+    with Timer(torch) as tm:
+        for i in range(steps):
+            step(probes[i])
+    ms = tm.ms / steps
+    rows_ms = sum(b.elapsed_ms(e) for b, e in probes) / steps
+    want = (x.astype(object) * y.astype(object) % t).astype(np.uint64)
+    if not np.array_equal(vals.cpu().numpy().view(np.uint64), want):
+        raise SystemExit("bench self-check failed: BFV N=8192 pipeline does not decrypt to the slot products")
+    c = cts.cpu().numpy().view(np.uint64)
+    o2 = ck.relinearize(1, ck.bfv_multiply(c[0], c[1], moduli[:L], t, logn), rk.cpu().numpy().view(np.uint64),
+                        moduli, L, logn)
+    if not np.array_equal(prod.cpu().numpy().view(np.uint64)[0, :2], o2):
+        raise SystemExit("bench self-check failed: BFV N=8192 multiply+relinearize words differ from the oracle")
+    limb = n * 8
+    # the probed launch: the ModUp rows pass over the L(L+1) limbs of one ciphertext (read
+    # and written once) and the L+1 twiddle tables
+    rows_bytes = 2 * L * (L + 1) * limb + (L + 1) * n * 16
+    out_json = {"metric": "BFV multiply+relinearize+decrypt pipelines/s, N=8192 (latency)",
+                "value": 1e3 / ms, "unit": "pipelines/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+                "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "u64", "data": "synthetic (seeded slots mod t; device-generated keys)",
+                "config": {"workload": "BFV N=8192, BFVDefault(8192) primes (4 data + special), t=65537: one "
+                                       "multiply -> relinearize -> decrypt -> batch decode per step",
+                           "parallelism": "single GPU"},
+                "latency_ms": ms, "encrypt_two_ms": enc_ms,
+                "roofline": {"bound": "hbm", "achieved": rows_bytes / (rows_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": rows_bytes / (rows_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "traffic": None,
+                             "kernel": "ntt_rows_modup_kernel<13,...>: the key switch's ModUp rows pass "
+                                       f"({L * (L + 1)} limbs; a latency-bound launch)",
+                             "algorithmic_bytes": rows_bytes, "avg_ms": rows_ms},
+                "self_check": "decrypts to the slot products mod t; multiply+relinearize words == oracle"}
+    if not args.no_cpu:
+        ct2 = [np.ascontiguousarray(c[0]), np.ascontiguousarray(c[1])]
+        key_h = rk.cpu().numpy().view(np.uint64).copy()
+        sk_h = sk.cpu().numpy().view(np.uint64).copy()
+
+        def make(kind):
+            if kind == "reference":
+                sess = ck.RefSession(1, n, moduli, t)
+                return lambda: sess.batch_decode(sess.decrypt(sess.relinearize(sess.multiply(ct2[0], ct2[1], L),
+                                                                               key_h, L), sk_h, L))
+            return lambda: ck.batch_decode(ck.bfv_decrypt(ck.relinearize(1, ck.bfv_multiply(ct2[0], ct2[1],
+                                                                                             moduli[:L], t, logn),
+                                                                         key_h, moduli, L, logn),
+                                                          sk_h, moduli, t, logn), t, logn)
+
+        out_json["cpu_baseline"] = cpu_baseline(make, "pipelines/s", 1, args.cpu_seconds,
+                                                "multiply -> relinearize -> decrypt -> batch decode of the same "
+                                                "ciphertexts and keys, N=8192")
+    return out_json
+
+
+SUB_KEYS = ("metric", "value", "unit", "ms_per_step", "steps", "config", "roofline", "step_roofline", "cpu_baseline")
+
+
+def bench_default(args):
+    """The headline line (CKKS mul+relin at N=2^16, L=24, batch 128) with every other
+    workload as a sub-object under "configs": the CPU-runnable BFV path (N=8192), the
+    batched NTT, the CKKS chain in a HIP graph, the hoisted rotation sums and the served
+    matvec mix -- each with its value, step time, dominant-kernel roofline and CPU
+    baseline.  A sub-workload whose self-check fails stops the run; one that fails
+    otherwise is reported with its error."""
+    import copy
+    import torch
+    line = bench_mulrelin(args)
+    if args.headline_only:
+        return line
+    sub = copy.copy(args)
+    sub.steps, sub.warmup = args.sub_steps, min(args.warmup, 2)
+    configs = {}
+    for name, fn in (("bfv_n8192", bench_bfv8192), ("ntt", bench_ntt), ("chain", bench_chain),
+                     ("rotsum", bench_rotsum), ("serve", bench_serve)):
+        torch.cuda.empty_cache()
+        t0 = time.perf_counter()
+        try:
+            r = fn(sub)
+            d = {k: r[k] for k in SUB_KEYS if k in r}
+            for k in ("batched", "plain", "capacity_per_s", "admission", "hoisted_over_separate", "graph_over_eager",
+                      "latency_ms", "self_check"):
+                if k in r:
+                    d[k] = r[k]
+        except (RuntimeError, MemoryError, OSError) as e:
+            d = {"error": f"{type(e).__name__}: {e}"}
+        d["wall_s"] = time.perf_counter() - t0
+        configs[name] = d
+        print(f"[bench] {name} done in {d['wall_s']:.1f} s", file=sys.stderr, flush=True)
+    line["configs"] = configs
+    return line
 
 
 def bench_copy(args):
@@ -864,8 +1298,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus N > 1 is not supported "
                                    "(independent ciphertexts shard across GPUs with no exchange)"}))
         sys.exit(2)
-    out = {"mulrelin": bench_mulrelin, "ntt": bench_ntt, "copy": bench_copy, "chain": bench_chain,
-           "rotsum": bench_rotsum, "matvec": bench_matvec, "serve": bench_serve}[args.workload](args)
+    out = {"mulrelin": bench_default, "ntt": bench_ntt, "copy": bench_copy, "chain": bench_chain,
+           "rotsum": bench_rotsum, "matvec": bench_matvec, "serve": bench_serve,
+           "bfv8192": bench_bfv8192}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -57,9 +57,11 @@ class BsgsMatvec:
         ctx, L = self.ctx, self.L
         return max(ctx.rotate_sum_workspace_size(count, L), ctx.galois_workspace_size(count, L))
 
-    def run(self, d_in, count, d_out, d_baby, d_inner, d_ws, ws_bytes, stream=None):
+    def run(self, d_in, count, d_out, d_baby, d_inner, d_ws, ws_bytes, stream=None, around_sums=None):
         """d_in, d_out: [count][2][L][N]; d_baby: scratch [b][count][2][L][N];
-        d_inner: scratch [g][count][2][L][N]; d_ws: workspace_sizes(count) bytes."""
+        d_inner: scratch [g][count][2][L][N]; d_ws: workspace_sizes(count) bytes.
+        around_sums: optional (before, after) callables invoked right before and after the
+        inner-sum launch (e.g. recording timing events on `stream`)."""
         import ctypes
         ctx, n, L, b, g = self.ctx, self.n, self.L, self.b, self.g
         ct_bytes = count * 2 * L * n * 8
@@ -72,7 +74,11 @@ class BsgsMatvec:
         if self.bfv:
             ctx.ntt_forward(d_baby, b * count * 2, L, stream)
         # every giant step's inner sum, inner[h] = sum_j diag'_{h,j} * rot_j(v)
+        if around_sums:
+            around_sums[0]()
         ctx.multiply_plain_accumulate_many(d_baby, self.plains, b, g, count * 2, L, d_inner, stream)
+        if around_sums:
+            around_sums[1]()
         if self.bfv:
             ctx.ntt_inverse(d_inner, g * count * 2, L, stream)
         # giant steps: rot_{h b}(inner[h]), summed
