@@ -135,12 +135,23 @@ int shc_apply_galois_keyswitch(const shc_context *ctx, uint64_t *d_ct, const uin
                                size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------ keys, encryption
- * Sampling is counter based: word i of seed s is splitmix64's i-th output, so a seed
- * reproduces a polynomial exactly.  The reference samples with Blake2xb (CPU) or
- * curand (GPU); distributions match (ternary; centred binomial, sigma ~ 3.24;
- * uniform mod q_i), streams do not.  Every random input can also be passed
- * explicitly (shc_encrypt_zero_asymmetric). */
+ * Sampling is counter based, so a seed reproduces a polynomial exactly.  A context with
+ * a sampler key draws every stream from ChaCha20 under that 256-bit key, the seed being
+ * the 64-bit nonce (word i of seed s = words 2(i % 8), 2(i % 8) + 1 of keystream block
+ * i / 8, little endian): a cryptographic generator, which is what keys and encryptions
+ * need.  A context without one (the default of shc_context_create) draws word i of
+ * seed s as splitmix64's i-th output -- fast and reproducible, for explicitly seeded
+ * tests and benchmarks only: its streams have 64 bits of entropy.  The reference
+ * samples with Blake2xb (CPU) or curand (GPU); distributions match (ternary; centred
+ * binomial, sigma ~ 3.24; uniform mod q_i), streams do not.  Every random input can
+ * also be passed explicitly (shc_encrypt_zero_asymmetric). */
 enum { SHC_SAMPLE_TERNARY = 0, SHC_SAMPLE_CBD = 1, SHC_SAMPLE_UNIFORM = 2 };
+
+/* Sets the context's sampler key (32 bytes, read as 8 little-endian words), or clears
+ * it (key = NULL: splitmix64 streams).  Not synchronised with calls that sample on
+ * this context: set it before them.  No counterpart in the reference (its KeyGenerator
+ * seeds Blake2xb from std::random_device, src/randomgen.cpp). */
+int shc_context_set_sampler_key(shc_context *ctx, const uint8_t *key);
 
 /* d_out[limb][N] (coefficient form) for the first limb_count key-level primes.
  * Replaces samplePolyTernary / samplePolyCbd / samplePolyUniform (src/utils/rlwe_cuda.cu:25-150). */

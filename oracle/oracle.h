@@ -154,6 +154,15 @@ int or_multiply_plain_ntt(int logn, const uint64_t *moduli, size_t L, const uint
 int or_bfv_multiply_plain(int logn, const uint64_t *moduli, size_t L, uint64_t t, const uint64_t *a, size_t polys,
                           const uint64_t *plain, uint64_t *out);
 
+/* ---------------------------------------------------------------- samplers
+ * The build's documented streams (the reference's Blake2xb / curand streams cannot
+ * be reproduced).  Pinned by: RFC 8439 section 2.3.2 and OpenSSL's ChaCha20.       */
+void or_chacha20_block(const uint32_t key[8], uint64_t counter, uint64_t nonce, uint32_t out[16]);
+/* kind 0 ternary, 1 centred binomial, 2 uniform; key NULL = splitmix64 stream.
+ * out: [L][N] coefficient form. */
+int or_sample_poly(int kind, const uint32_t *key, uint64_t seed, int logn, const uint64_t *moduli, size_t L,
+                   uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

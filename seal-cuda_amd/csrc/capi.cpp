@@ -8,6 +8,7 @@
 #include "galois.h"
 #include "rns_ops.h"
 #include "rotations.h"
+#include <algorithm>
 #include <string>
 
 struct shc_context {
@@ -187,6 +188,22 @@ int shc_apply_galois_keyswitch(const shc_context *ctx, uint64_t *ct, const uint6
         const auto &c = ctx_of(ctx);
         check_limbs(c, limbs);
         shc::apply_galois_keyswitch(c, ct, count, limbs, elt, key, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_context_set_sampler_key(shc_context *ctx, const uint8_t *key) {
+    return guarded([&] {
+        if (!ctx || !ctx->impl) shc::fail(shc::ERR_INVALID_ARGUMENT, "context is null");
+        shc::Context &c = *ctx->impl;
+        if (!key) {
+            c.sampler_keyed = false;
+            std::fill(c.sampler_key, c.sampler_key + 8, 0u);
+            return;
+        }
+        for (int i = 0; i < 8; i++)
+            c.sampler_key[i] = (uint32_t)key[4 * i] | ((uint32_t)key[4 * i + 1] << 8) |
+                               ((uint32_t)key[4 * i + 2] << 16) | ((uint32_t)key[4 * i + 3] << 24);
+        c.sampler_keyed = true;
     });
 }
 

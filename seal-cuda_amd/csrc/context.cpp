@@ -95,6 +95,15 @@ std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector
         for (size_t i = 0; i + 1 < L; i++) rsc[L * K + i] = divconst(moduli[L - 1], moduli[i]);
     c->d_ks_consts = upload(ksc);
     c->d_rescale_consts = upload(rsc);
+    if (K >= 2) {
+        std::vector<uint64_t> f;
+        for (size_t j = 0; j + 1 < K; j++) {
+            const uint64_t w = moduli[K - 1] % moduli[j];
+            f.push_back(w);
+            f.push_back(hm::shoup(w, moduli[j]));
+        }
+        c->d_kswitch_factors = upload(f);
+    }
 
     if (scheme == SCHEME_BFV) {
         for (uint64_t q : moduli)

@@ -43,6 +43,16 @@ struct Context {
 
     int device = 0;
 
+    // Sampler: with sampler_keyed, word i of the stream of seed s is taken from the
+    // ChaCha20 keystream under this 256-bit key (nonce s, block counter i / 8);
+    // otherwise from splitmix64 (deterministic streams for explicitly seeded calls).
+    bool sampler_keyed = false;
+    uint32_t sampler_key[8] = {};
+
+    // (P mod q_j, its Shoup quotient) for j < K-1, P the special prime: the factors a
+    // key-switching key folds the new key in with (key generation).
+    DevBuf d_kswitch_factors;
+
     // Instrumentation: when armed, the next key switch records these events around its
     // ModUp rows-pass launch (the whole ModUp forward NTT for N < 2^13; the first
     // slice's, for a batch run in slices) and disarms.  Key switches that run while it

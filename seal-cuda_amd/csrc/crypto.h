@@ -1,8 +1,12 @@
 // crypto.h -- sampling, key generation, encryption, decryption and batch encoding.
 //
-// Randomness: every sampler is counter based.  Word i of the stream with seed s is
-// splitmix64's i-th output, mix(s + (i + 1) * 0x9E3779B97F4A7C15), so any coefficient
-// can be drawn independently by any thread and a seed reproduces a polynomial exactly.
+// Randomness: every sampler is counter based, so any coefficient can be drawn
+// independently by any thread and a seed reproduces a polynomial exactly.  A context
+// with a sampler key (Context::sampler_keyed) takes word i of the stream with seed s
+// from the ChaCha20 keystream under that 256-bit key (nonce s, block i / 8, words
+// 2(i % 8) and 2(i % 8) + 1 little endian) -- a cryptographic generator, for keys and
+// encryptions; without one, word i is splitmix64's i-th output,
+// mix(s + (i + 1) * 0x9E3779B97F4A7C15), for explicitly seeded tests and benchmarks.
 // The reference samples with Blake2xb (CPU, src/utils/rlwe.cpp:21-160) or curand (GPU,
 // src/utils/rlwe_cuda.cu:25-150); the distributions are the same, the streams differ.
 #pragma once

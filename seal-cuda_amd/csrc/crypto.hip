@@ -29,18 +29,98 @@ __host__ __device__ __forceinline__ uint64_t stream_word(uint64_t seed, uint64_t
 // Independent sub-stream seeds (u, e0, e1, per-digit keys, ...)
 inline uint64_t sub_seed(uint64_t seed, uint64_t k) { return mix64(seed ^ mix64(k + 0x5EA15EEDull)); }
 
+// ---- keyed streams: ChaCha20 (20 rounds, 256-bit key, 64-bit block counter, 64-bit
+// nonce; state words 12-15 = counter lo, hi, nonce lo, hi).  Word i of the stream of
+// seed s is words 2k, 2k+1 (little endian) of block i / 8 under nonce s, k = i % 8.
+struct SamplerKey {
+    uint32_t k[8];
+};
+
+__host__ __device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+#define SHC_QR(a, b, c, d)                 \
+    a += b; d ^= a; d = rotl32(d, 16);     \
+    c += d; b ^= c; b = rotl32(b, 12);     \
+    a += b; d ^= a; d = rotl32(d, 8);      \
+    c += d; b ^= c; b = rotl32(b, 7);
+
+__device__ __forceinline__ void chacha20_block(const SamplerKey &key, uint64_t counter, uint64_t nonce,
+                                               uint32_t (&out)[16]) {
+    uint32_t x[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u,
+                      key.k[0], key.k[1], key.k[2], key.k[3], key.k[4], key.k[5], key.k[6], key.k[7],
+                      (uint32_t)counter, (uint32_t)(counter >> 32), (uint32_t)nonce, (uint32_t)(nonce >> 32)};
+    uint32_t y[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) y[i] = x[i];
+#pragma unroll 1
+    for (int r = 0; r < 10; r++) {
+        SHC_QR(y[0], y[4], y[8], y[12]) SHC_QR(y[1], y[5], y[9], y[13])
+        SHC_QR(y[2], y[6], y[10], y[14]) SHC_QR(y[3], y[7], y[11], y[15])
+        SHC_QR(y[0], y[5], y[10], y[15]) SHC_QR(y[1], y[6], y[11], y[12])
+        SHC_QR(y[2], y[7], y[8], y[13]) SHC_QR(y[3], y[4], y[9], y[14])
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) out[i] = y[i] + x[i];
+}
+#undef SHC_QR
+
+__device__ __forceinline__ uint64_t block_word(const uint32_t (&b)[16], int k) {
+    return (uint64_t)b[2 * k] | ((uint64_t)b[2 * k + 1] << 32);
+}
+
+// A ternary or centred-binomial integer from one stream word (shared by both streams).
+__device__ __forceinline__ int64_t small_from_word(int kind, uint64_t w) {
+    if (kind == SAMPLE_TERNARY) return (int64_t)mulhi64(w, 3) - 1;  // floor(3 w / 2^64) in {0, 1, 2}
+    // centred binomial with 21 + 21 bits: variance 21/2 (sigma ~ 3.24), |s| <= 21
+    return (int64_t)__popcll(w & 0x1FFFFFull) - (int64_t)__popcll((w >> 21) & 0x1FFFFFull);
+}
+
+// One thread per keystream block: coefficients 8b .. 8b+7.
+__global__ void sample_small_keyed_kernel(int kind, SamplerKey key, uint64_t seed,
+                                          const DevPrime *__restrict__ primes, uint32_t L, int logn,
+                                          uint64_t *__restrict__ out) {
+    const size_t n = (size_t)1 << logn;
+    const size_t blocks = (n + 7) / 8;
+    for (size_t b = (size_t)blockIdx.x * TPB + threadIdx.x; b < blocks; b += (size_t)gridDim.x * TPB) {
+        uint32_t ks[16];
+        chacha20_block(key, b, seed, ks);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const size_t c = b * 8 + k;
+            if (c >= n) break;
+            const int64_t s = small_from_word(kind, block_word(ks, k));
+            for (uint32_t l = 0; l < L; l++) {
+                const uint64_t q = primes[l].q;
+                out[(size_t)l * n + c] = s < 0 ? q - (uint64_t)(-s) : (uint64_t)s;
+            }
+        }
+    }
+}
+
+// One thread per keystream block: outputs 4b .. 4b+3, output i from the 128-bit draw
+// (word 2i high, word 2i+1 low) reduced modulo its limb's prime.
+__global__ void sample_uniform_keyed_kernel(SamplerKey key, uint64_t seed, const DevPrime *__restrict__ primes,
+                                            int logn, size_t total, uint64_t *__restrict__ out) {
+    const size_t blocks = (total + 3) / 4;
+    for (size_t b = (size_t)blockIdx.x * TPB + threadIdx.x; b < blocks; b += (size_t)gridDim.x * TPB) {
+        uint32_t ks[16];
+        chacha20_block(key, b, seed, ks);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const size_t i = b * 4 + k;
+            if (i >= total) break;
+            const DevPrime &P = primes[i >> logn];
+            const u128 x{block_word(ks, 2 * k + 1), block_word(ks, 2 * k)};
+            out[i] = barrett128(x, P.q, P.ratio_lo, P.ratio_hi);
+        }
+    }
+}
+
 __global__ void sample_small_kernel(int kind, uint64_t seed, const DevPrime *__restrict__ primes, uint32_t L,
                                     int logn, uint64_t *__restrict__ out) {
     const size_t n = (size_t)1 << logn;
     for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < n; c += (size_t)gridDim.x * TPB) {
-        const uint64_t w = stream_word(seed, c);
-        int64_t s;
-        if (kind == SAMPLE_TERNARY) {
-            s = (int64_t)mulhi64(w, 3) - 1;  // floor(3 w / 2^64) in {0, 1, 2}
-        } else {
-            // centred binomial with 21 + 21 bits: variance 21/2 (sigma ~ 3.24), |s| <= 21
-            s = (int64_t)__popcll(w & 0x1FFFFFull) - (int64_t)__popcll((w >> 21) & 0x1FFFFFull);
-        }
+        const int64_t s = small_from_word(kind, stream_word(seed, c));
         for (uint32_t l = 0; l < L; l++) {
             const uint64_t q = primes[l].q;
             out[(size_t)l * n + c] = s < 0 ? q - (uint64_t)(-s) : (uint64_t)s;
@@ -204,6 +284,20 @@ __global__ void batch_gather_kernel(const uint64_t *__restrict__ ntt, uint64_t *
 
 void sample_poly(const Context &c, SampleKind kind, uint64_t seed, const DevPrime *primes, uint32_t L, uint64_t *out,
                  hipStream_t st) {
+    if (c.sampler_keyed) {
+        SamplerKey key;
+        for (int i = 0; i < 8; i++) key.k[i] = c.sampler_key[i];
+        if (kind == SAMPLE_UNIFORM) {
+            const size_t total = (size_t)L << c.logn;
+            sample_uniform_keyed_kernel<<<blocks_for((total + 3) / 4), TPB, 0, st>>>(key, seed, primes, c.logn, total,
+                                                                                    out);
+        } else {
+            sample_small_keyed_kernel<<<blocks_for((c.n + 7) / 8), TPB, 0, st>>>((int)kind, key, seed, primes, L,
+                                                                               c.logn, out);
+        }
+        launch_check("sample");
+        return;
+    }
     if (kind == SAMPLE_UNIFORM) {
         const size_t total = (size_t)L << c.logn;
         sample_uniform_kernel<<<blocks_for(total), TPB, 0, st>>>(seed, primes, c.logn, total, out);
@@ -253,19 +347,10 @@ void keygen_kswitch(const Context &c, const uint64_t *sk, const uint64_t *new_ke
     const size_t n = c.n;
     for (uint32_t j = 0; j + 1 < K; j++)
         encrypt_zero_symmetric(c, sk, c.primes(), K, true, sub_seed(seed, 100 + j), key + (size_t)j * 2 * K * n, st);
-    std::vector<uint64_t> f;
-    for (uint32_t j = 0; j + 1 < K; j++) {
-        const uint64_t w = c.moduli[K - 1] % c.moduli[j];
-        f.push_back(w);
-        f.push_back(hm::shoup(w, c.moduli[j]));
-    }
-    DevBuf fac(f.size() * 8);
-    SHC_HIP(hipMemcpyAsync(fac.ptr, f.data(), f.size() * 8, hipMemcpyHostToDevice, st));
     const size_t total = (size_t)(K - 1) << c.logn;
-    kswitch_fold_kernel<<<blocks_for(total), TPB, 0, st>>>(key, new_key, fac.as<uint64_t>(), c.primes(), K, c.logn,
-                                                           total);
+    kswitch_fold_kernel<<<blocks_for(total), TPB, 0, st>>>(key, new_key, c.d_kswitch_factors.as<uint64_t>(),
+                                                           c.primes(), K, c.logn, total);
     launch_check("kswitch_fold");
-    SHC_HIP(hipStreamSynchronize(st));  // fac is freed on return
 }
 
 size_t keygen_workspace(const Context &c) { return c.key_count() * c.n * 8; }

@@ -53,6 +53,7 @@ def lib():
         L.shc_apply_galois_keyswitch.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
         u64 = ctypes.c_uint64
         L.shc_sample_poly.argtypes = [_vp, ctypes.c_int, u64, u32, _vp, _vp]
+        L.shc_context_set_sampler_key.argtypes = [_vp, ctypes.c_char_p]
         L.shc_keygen_secret.argtypes = [_vp, u64, _vp, _vp]
         L.shc_keygen_public.argtypes = [_vp, _vp, u64, _vp, _vp]
         L.shc_keygen_workspace_size.argtypes = [_vp, ctypes.POINTER(sz)]
@@ -216,6 +217,12 @@ class Context:
         b = ctypes.c_size_t()
         check(fn(self.handle, *args, ctypes.byref(b)))
         return b.value
+
+    def set_sampler_key(self, key):
+        """32-byte ChaCha20 key for every sampler stream of this context (None: splitmix64)."""
+        if key is not None and len(key) != 32:
+            raise ValueError("sampler key must be 32 bytes")
+        check(lib().shc_context_set_sampler_key(self.handle, None if key is None else bytes(key)))
 
     def sample_poly(self, kind, seed, limbs, d_out, stream=None):
         check(lib().shc_sample_poly(self.handle, kind, seed, limbs, d_out, stream))

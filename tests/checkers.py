@@ -79,6 +79,10 @@ def oracle():
         lib.or_behz_bases.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p,
                                       ctypes.POINTER(sz)]
         lib.or_bfv_multiply.argtypes = [ctypes.c_int, u64p, sz, ctypes.c_uint64, u64p, u64p, u64p]
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        lib.or_chacha20_block.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint64, u32p]
+        lib.or_chacha20_block.restype = None
+        lib.or_sample_poly.argtypes = [ctypes.c_int, u32p, ctypes.c_uint64, ctypes.c_int, u64p, sz, u64p]
         _oracle = lib
     return _oracle
 
@@ -114,6 +118,7 @@ def ref():
         lib.ref_ct_binary.argtypes = [ctypes.c_void_p, u64, u64, ctypes.c_int, u64p, u64p, u64p]
         lib.ref_plain_op.argtypes = [ctypes.c_void_p, u64, u64, ctypes.c_int, u64p, u64p, u64p]
         lib.ref_parms_id.argtypes = [ctypes.c_void_p, u64, u64p]
+        lib.ref_plaintext_hex.argtypes = [ctypes.c_char_p, u64p, ctypes.POINTER(u64), ctypes.c_char_p, u64]
         _ref = lib
     return _ref
 
@@ -346,6 +351,26 @@ def rns_of_small(values, moduli):
     """[N] signed integers -> [L][N] residues (one integer reduced into every limb)."""
     v = np.asarray(values, dtype=object)
     return np.array([[int(x) % q for x in v] for q in moduli], dtype=np.uint64)
+
+
+def chacha20_block(key, counter, nonce):
+    """ChaCha20 block (16 words) for key = 32 bytes, 64-bit counter and nonce."""
+    k = np.frombuffer(bytes(key), dtype="<u4").astype(np.uint32).copy()
+    out = np.zeros(16, dtype=np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    oracle().or_chacha20_block(k.ctypes.data_as(u32p), counter, nonce, out.ctypes.data_as(u32p))
+    return out
+
+
+def sample_poly(kind, key, seed, logn, moduli):
+    """The library's sampler stream: key = 32 bytes (ChaCha20) or None (splitmix64)."""
+    L = len(moduli)
+    out = np.zeros((L, 1 << logn), dtype=np.uint64)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    k = None if key is None else np.frombuffer(bytes(key), dtype="<u4").astype(np.uint32).copy()
+    kp = None if k is None else k.ctypes.data_as(u32p)
+    assert oracle().or_sample_poly(kind, kp, seed, logn, ptr(_arr(moduli)), L, ptr(out)) == 0
+    return out
 
 
 def galois_elt(logn, step):

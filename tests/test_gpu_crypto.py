@@ -230,3 +230,53 @@ def test_bfv_rotate_rows_end_to_end():
     # rotate_rows(1): each row of the 2 x N/2 matrix shifts left by one
     want = np.concatenate([np.roll(x[:row], -1), np.roll(x[row:], -1)])
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("keyed", [False, True])
+def test_sampler_words_match_oracle(keyed):
+    """Every sampler kind gives the oracle's words: the splitmix64 stream of an unkeyed
+    context and the ChaCha20 stream of a keyed one (also at a degree whose coefficient
+    count is not a multiple of a keystream block's eight words)."""
+    for logn, bits in ((13, (43, 43, 44, 44, 44)), (2, (30, 30))):
+        n = 1 << logn
+        moduli = ck.coeff_modulus_create(n, list(bits))
+        ctx = hip().Context(BFV, n, moduli, T) if logn > 2 else hip().Context(CKKS, n, moduli)
+        key = bytes(range(7, 39)) if keyed else None
+        if keyed:
+            ctx.set_sampler_key(key)
+        K = len(moduli)
+        d = empty_dev((K, n))
+        for kind, seed in ((0, 11), (1, 12), (2, 13), (2, 1 << 63)):
+            ctx.sample_poly(kind, seed, K, d.data_ptr(), stream())
+            np.testing.assert_array_equal(from_dev(d), ck.sample_poly(kind, key, seed, logn, moduli),
+                                          err_msg=f"kind {kind}, seed {seed}, N {n}")
+
+
+def test_keyed_context_pipeline():
+    """Keys and encryptions drawn from the ChaCha20 stream decrypt as before, and the
+    sampler key changes every random word (the secret keys of the same seed differ)."""
+    ctx, moduli = bfv_ctx()
+    n, K = ctx.n, len(moduli)
+    L = K - 1
+    sk_plain = empty_dev((K, n))
+    ctx.keygen_secret(5, sk_plain.data_ptr(), stream())
+    ctx.set_sampler_key(bytes(range(32)))
+    sk, pk = empty_dev((K, n)), empty_dev((2, K, n))
+    ctx.keygen_secret(5, sk.data_ptr(), stream())
+    assert not np.array_equal(from_dev(sk), from_dev(sk_plain))
+    ctx.keygen_public(sk.data_ptr(), 6, pk.data_ptr(), stream())
+    rng = np.random.default_rng(4)
+    x = rng.integers(0, T, size=n, dtype=np.uint64)
+    px = empty_dev((n,))
+    ctx.batch_encode(to_dev(x).data_ptr(), n, px.data_ptr(), stream())
+    ct = empty_dev((1, 2, L, n))
+    ew = ctx.encrypt_workspace_size(L)
+    ctx.encrypt(pk.data_ptr(), px.data_ptr(), L, 7, ct.data_ptr(), ws(ew).data_ptr(), ew, stream())
+    dw = ctx.decrypt_workspace_size(L)
+    out, vals, scratch = empty_dev((n,)), empty_dev((n,)), empty_dev((n,))
+    ctx.decrypt(ct.data_ptr(), 2, sk.data_ptr(), L, out.data_ptr(), ws(dw).data_ptr(), dw, stream())
+    ctx.batch_decode(out.data_ptr(), vals.data_ptr(), scratch.data_ptr(), stream())
+    np.testing.assert_array_equal(from_dev(vals), x)
+    ctx.set_sampler_key(None)   # back to the splitmix64 stream
+    ctx.keygen_secret(5, sk.data_ptr(), stream())
+    np.testing.assert_array_equal(from_dev(sk), from_dev(sk_plain))
