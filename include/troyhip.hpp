@@ -3,9 +3,18 @@
 // Header only.  A program written against the reference's troyn:: aliases
 // (src/troy_cuda.cuh:20-42) switches by using namespace troyhip instead: the same class
 // names, method names, argument meanings and exception types.  Device buffers are
-// owned by RAII arrays (the reference's DeviceArray, src/utils/devicearray.cuh:106-200);
-// every operation runs on the default stream, as the reference's do, and returns when
-// its work is queued (host-visible results -- decode, toHost -- synchronise).
+// owned by RAII arrays (the reference's DeviceArray, src/utils/devicearray.cuh:106-200)
+// whose blocks come from a device memory pool (the reference's MemoryPoolCuda,
+// src/utils/memorypool_cuda.cuh:8-73), so steady-state operations do not call hipMalloc
+// or hipFree.  Every operation runs on the default stream, as the reference's do, and
+// returns when its work is queued (host-visible results -- decode, toHost --
+// synchronise).
+//
+// A ParmsID is the reference's: the 4-word BLAKE2b hash of a level's parameters
+// (src/encryptionparams.h:36).  Keys and encryptions are drawn from ChaCha20 under a
+// 256-bit key that every SEALContext takes from std::random_device.  The CPU-side
+// containers the reference converts to and from (cpu(), toHost(), the conversion
+// constructors) are troyhip::host::Plaintext and troyhip::host::Ciphertext.
 //
 // Link with libsealhip.so (and the HIP runtime).
 #pragma once
@@ -15,15 +24,19 @@
 
 #include <algorithm>
 #include <array>
-#include <istream>
-#include <ostream>
+#include <cctype>
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <istream>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <ostream>
 #include <random>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -152,6 +165,108 @@ private:
     }
 };
 
+// ----------------------------------------------------------------------- memory pool
+
+// MemoryPoolCuda (src/utils/memorypool_cuda.cuh:8-73): device blocks that arrays free
+// are kept per device and size and handed to the next request for at least that size
+// and at most twice it, so repeated operations stop calling hipMalloc / hipFree.  A
+// freed block is reused by later work on the default stream, which the stream orders
+// after every earlier use of the block.  When free device memory runs below a reserve,
+// or hipMalloc fails, the cached blocks are released and the allocation retried.
+class MemoryPool {
+public:
+    // A block of at least `bytes` (granted: its size); bytes > 0.
+    static void *get(size_t bytes, size_t &granted) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        const size_t want = round(bytes);
+        {
+            State &s = state();
+            std::lock_guard<std::mutex> g(s.mu);
+            auto &m = s.free[dev];
+            auto it = m.lower_bound(want);
+            if (it != m.end() && it->first <= 2 * want) {
+                void *p = it->second;
+                granted = it->first;
+                s.cached -= it->first;
+                m.erase(it);
+                return p;
+            }
+        }
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b < want + RESERVE) release();
+        void *p = nullptr;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            release();
+            hip_check(hipMalloc(&p, want), "hipMalloc");
+        }
+        {
+            State &s = state();
+            std::lock_guard<std::mutex> g(s.mu);
+            s.mallocs++;
+        }
+        granted = want;
+        return p;
+    }
+    static void put(void *p, size_t granted) {
+        if (!p) return;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return;
+        State &s = state();
+        std::lock_guard<std::mutex> g(s.mu);
+        s.free[dev].emplace(granted, p);
+        s.cached += granted;
+    }
+    // Frees every cached block (all devices); returns the bytes freed.
+    static size_t release() {
+        State &s = state();
+        std::lock_guard<std::mutex> g(s.mu);
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        size_t freed = 0;
+        for (auto &dm : s.free) {
+            if (dm.second.empty()) continue;
+            (void)hipSetDevice(dm.first);
+            (void)hipDeviceSynchronize();  // blocks may still be read by queued work
+            for (auto &e : dm.second) {
+                (void)hipFree(e.second);
+                freed += e.first;
+            }
+            dm.second.clear();
+        }
+        (void)hipSetDevice(cur);
+        s.cached = 0;
+        return freed;
+    }
+    static size_t cachedBytes() {
+        State &s = state();
+        std::lock_guard<std::mutex> g(s.mu);
+        return s.cached;
+    }
+    // hipMalloc calls the pool has made so far.
+    static size_t deviceAllocations() {
+        State &s = state();
+        std::lock_guard<std::mutex> g(s.mu);
+        return s.mallocs;
+    }
+
+private:
+    static constexpr size_t RESERVE = size_t(32) << 20;
+    static size_t round(size_t b) { return (b + 4095) & ~size_t(4095); }
+    struct State {
+        std::mutex mu;
+        std::map<int, std::multimap<size_t, void *>> free;
+        size_t cached = 0;
+        size_t mallocs = 0;
+    };
+    // Never destroyed: arrays with static storage may return blocks during exit.
+    static State &state() {
+        static State *s = new State;
+        return *s;
+    }
+};
+
 // ---------------------------------------------------------------------- device array
 
 template <class T>
@@ -160,9 +275,9 @@ public:
     DeviceArray() = default;
     explicit DeviceArray(size_t n) { resize(n); }
     DeviceArray(const DeviceArray &o) { *this = o; }
-    DeviceArray(DeviceArray &&o) noexcept : ptr_(o.ptr_), size_(o.size_) {
+    DeviceArray(DeviceArray &&o) noexcept : ptr_(o.ptr_), size_(o.size_), cap_(o.cap_) {
         o.ptr_ = nullptr;
-        o.size_ = 0;
+        o.size_ = o.cap_ = 0;
     }
     DeviceArray &operator=(const DeviceArray &o) {
         if (this != &o) {
@@ -176,18 +291,25 @@ public:
             release();
             ptr_ = o.ptr_;
             size_ = o.size_;
+            cap_ = o.cap_;
             o.ptr_ = nullptr;
-            o.size_ = 0;
+            o.size_ = o.cap_ = 0;
         }
         return *this;
     }
     ~DeviceArray() { release(); }
 
-    // Reallocates (contents are not kept) when the size changes.
+    // Contents are not kept when the block changes: a block between half and all of
+    // its capacity in use is kept, else it goes back to the pool for one of the new size.
     void resize(size_t n) {
         if (n == size_) return;
+        const size_t bytes = n * sizeof(T);
+        if (ptr_ && bytes <= cap_ && 2 * bytes >= cap_) {
+            size_ = n;
+            return;
+        }
         release();
-        if (n) hip_check(hipMalloc(reinterpret_cast<void **>(&ptr_), n * sizeof(T)), "hipMalloc");
+        if (n) ptr_ = static_cast<T *>(MemoryPool::get(bytes, cap_));
         size_ = n;
     }
     // Grows to at least n elements, keeping nothing.
@@ -203,18 +325,23 @@ public:
         if (size_) hip_check(hipMemcpy(v.data(), ptr_, size_ * sizeof(T), hipMemcpyDeviceToHost), "copy to host");
         return v;
     }
+    void setZero() {
+        if (size_) hip_check(hipMemset(ptr_, 0, size_ * sizeof(T)), "memset");
+    }
+    void release() {
+        if (ptr_) MemoryPool::put(ptr_, cap_);
+        ptr_ = nullptr;
+        size_ = cap_ = 0;
+    }
     T *get() { return ptr_; }
     const T *get() const { return ptr_; }
     size_t size() const { return size_; }
+    size_t capacity() const { return cap_ / sizeof(T); }
 
 private:
-    void release() {
-        if (ptr_) (void)hipFree(ptr_);
-        ptr_ = nullptr;
-        size_ = 0;
-    }
     T *ptr_ = nullptr;
     size_t size_ = 0;
+    size_t cap_ = 0;  // bytes of the pool block
 };
 
 // ------------------------------------------------------------------ parameter types
@@ -299,24 +426,23 @@ private:
     Modulus t_;
 };
 
-// A level of the modulus chain, named by its number of primes (the key level has all
-// K, the first data level K-1, the last 1).
-struct ParmsID {
-    uint32_t level = 0;
-    bool operator==(const ParmsID &o) const { return level == o.level; }
-    bool operator!=(const ParmsID &o) const { return level != o.level; }
-};
-static const ParmsID parmsIDZero{0};
+// ParmsID (src/encryptionparams.h:36): the BLAKE2b-256 hash block of a level's
+// parameters -- scheme, N, that level's primes, plain modulus -- as 4 little-endian
+// words (EncryptionParameters::computeParmsID, src/encryptionparams.cpp:118-156).
+using ParmsID = std::array<uint64_t, 4>;
+static const ParmsID parmsIDZero{0, 0, 0, 0};
 
 class SEALContext;
 
 // ContextDataCuda (src/context_cuda.cuh:11-170): one level of the chain.
 class ContextData {
 public:
-    ContextData(ParmsID id, std::vector<Modulus> q, SchemeType scheme, size_t n, Modulus t, bool has_next)
-        : id_(id), q_(std::move(q)), scheme_(scheme), n_(n), t_(t), has_next_(has_next) {}
+    ContextData(ParmsID id, uint32_t level, std::vector<Modulus> q, SchemeType scheme, size_t n, Modulus t,
+                ParmsID next)
+        : id_(id), level_(level), q_(std::move(q)), scheme_(scheme), n_(n), t_(t), next_(next) {}
     ParmsID parmsID() const { return id_; }
-    size_t chainIndex() const { return id_.level - 1; }
+    // Position in the chain: 0 for the last level, growing towards the key level.
+    size_t chainIndex() const { return level_ - 1; }
     const std::vector<Modulus> &coeffModulus() const { return q_; }
     SchemeType scheme() const { return scheme_; }
     size_t polyModulusDegree() const { return n_; }
@@ -326,18 +452,31 @@ public:
         for (auto &m : q_) b += m.bitCount();
         return b;
     }
-    ParmsID nextParmsID() const { return has_next_ ? ParmsID{id_.level - 1} : parmsIDZero; }
+    ParmsID nextParmsID() const { return next_; }
 
 private:
     ParmsID id_;
+    uint32_t level_;
     std::vector<Modulus> q_;
     SchemeType scheme_;
     size_t n_;
     Modulus t_;
-    bool has_next_;
+    ParmsID next_;
 };
 
+namespace detail {
+// 64 bits from the operating system's generator (std::random_device).
+inline uint64_t os_random64() {
+    std::random_device rd;
+    return ((uint64_t)rd() << 32) ^ rd();
+}
+}  // namespace detail
+
 // SEALContextCuda (src/context_cuda.cuh:172-238).  Copies share the device tables.
+// Levels are named by their parms_id; the chain holds the key level (all K primes)
+// and the data levels K-1 .. 1 (only K-1 when expand_mod_chain is false).  Every
+// context samples keys and encryptions from ChaCha20 under a fresh 256-bit key from
+// std::random_device (shc_context_set_sampler_key).
 class SEALContext {
 public:
     explicit SEALContext(const EncryptionParameters &parms, bool expand_mod_chain = true,
@@ -371,6 +510,23 @@ public:
             return;
         }
         s_->handle = h;
+        uint8_t key[32];
+        {
+            std::random_device rd;
+            for (int i = 0; i < 8; i++) {
+                const uint32_t w = rd();
+                for (int b = 0; b < 4; b++) key[4 * i + b] = (uint8_t)(w >> (8 * b));
+            }
+        }
+        check(shc_context_set_sampler_key(h, key));
+        std::memset(key, 0, sizeof key);
+        s_->ids.resize(q.size() + 1, parmsIDZero);
+        for (uint32_t l = 1; l <= q.size(); l++) {
+            std::vector<uint64_t> ql(v.begin(), v.begin() + l);
+            s_->ids[l] = detail::parms_hash((uint64_t)parms.scheme(), parms.polyModulusDegree(), ql,
+                                            parms.plainModulus().value());
+            s_->level_of[s_->ids[l]] = l;
+        }
     }
 
     bool parametersSet() const { return s_->handle != nullptr; }
@@ -384,18 +540,20 @@ public:
     const EncryptionParameters &parms() const { return s_->parms; }
     size_t keyCount() const { return s_->parms.coeffModulus().size(); }
 
-    ParmsID keyParmsID() const { return ParmsID{(uint32_t)keyCount()}; }
-    ParmsID firstParmsID() const { return ParmsID{(uint32_t)(keyCount() > 1 ? keyCount() - 1 : 1)}; }
-    ParmsID lastParmsID() const { return s_->expand ? ParmsID{1} : firstParmsID(); }
+    ParmsID keyParmsID() const { return idOf((uint32_t)keyCount()); }
+    ParmsID firstParmsID() const { return idOf(firstLevel()); }
+    ParmsID lastParmsID() const { return idOf(s_->expand ? 1 : firstLevel()); }
     bool usingKeyswitching() const { return keyCount() > 1; }
 
     std::shared_ptr<const ContextData> getContextData(ParmsID id) const {
-        if (id.level == 0 || id.level > keyCount()) return nullptr;
-        if (id.level != keyParmsID().level && id.level < lastParmsID().level) return nullptr;
+        const uint32_t l = levelOf(id);
+        if (!l || !inChain(l)) return nullptr;
         const auto &q = s_->parms.coeffModulus();
-        const bool has_next = id.level > lastParmsID().level && id.level != keyParmsID().level;
-        return std::make_shared<ContextData>(id, std::vector<Modulus>(q.begin(), q.begin() + id.level),
-                                             scheme(), polyModulusDegree(), s_->parms.plainModulus(), has_next);
+        const uint32_t last = s_->expand ? 1 : firstLevel();
+        const bool has_next = l > last && l != keyCount();
+        return std::make_shared<ContextData>(id, l, std::vector<Modulus>(q.begin(), q.begin() + l), scheme(),
+                                             polyModulusDegree(), s_->parms.plainModulus(),
+                                             has_next ? idOf(l - 1) : parmsIDZero);
     }
     std::shared_ptr<const ContextData> firstContextData() const { return getContextData(firstParmsID()); }
     std::shared_ptr<const ContextData> keyContextData() const { return getContextData(keyParmsID()); }
@@ -406,6 +564,24 @@ public:
         checkSet();
         return s_->handle;
     }
+    // The number of primes of the level named `id` (0 when no level of this context has
+    // that parms_id).
+    uint32_t levelOf(const ParmsID &id) const {
+        auto it = s_->level_of.find(id);
+        return it == s_->level_of.end() ? 0 : it->second;
+    }
+    // The same, throwing std::invalid_argument for an unknown parms_id.
+    uint32_t levelOrThrow(const ParmsID &id, const char *what) const {
+        const uint32_t l = levelOf(id);
+        if (!l || !inChain(l)) throw std::invalid_argument(std::string(what) + " is not valid for encryption parameters");
+        return l;
+    }
+    // The parms_id of the level with `level` primes.
+    ParmsID idOf(uint32_t level) const {
+        checkSet();
+        if (level == 0 || level > keyCount()) throw std::invalid_argument("no such level");
+        return s_->ids[level];
+    }
     // Scratch device memory shared by the operations of this context (one stream; the
     // reference's evaluators are not thread safe either).
     void *workspace(size_t bytes) const {
@@ -413,32 +589,24 @@ public:
         return s_->ws.get();
     }
     uint64_t prime(size_t i) const { return s_->parms.coeffModulus()[i].value(); }
-    // The reference's parms_id of a level: BLAKE2b-256 of (scheme, N, the level's primes,
-    // plain modulus) as little-endian words (EncryptionParameters::computeParmsID,
-    // src/encryptionparams.cpp:118-156).
-    std::array<uint64_t, 4> parmsHash(ParmsID id) const {
-        std::vector<uint64_t> q;
-        for (uint32_t i = 0; i < id.level; i++) q.push_back(prime(i));
-        return detail::parms_hash((uint64_t)scheme(), polyModulusDegree(), q, s_->parms.plainModulus().value());
-    }
-    // The level whose parms_id is `h`, or parmsIDZero.
-    ParmsID levelOfHash(const std::array<uint64_t, 4> &h) const {
-        for (uint32_t l = 1; l <= keyCount(); l++)
-            if (parmsHash(ParmsID{l}) == h) return ParmsID{l};
-        return parmsIDZero;
-    }
-    int levelBits(ParmsID id) const {
+    int levelBits(uint32_t level) const {
         int b = 0;
-        for (uint32_t i = 0; i < id.level; i++) b += s_->parms.coeffModulus()[i].bitCount();
+        for (uint32_t i = 0; i < level; i++) b += s_->parms.coeffModulus()[i].bitCount();
         return b;
     }
 
 private:
+    uint32_t firstLevel() const { return (uint32_t)(keyCount() > 1 ? keyCount() - 1 : 1); }
+    bool inChain(uint32_t l) const {
+        return l == keyCount() || (l <= firstLevel() && l >= (s_->expand ? 1u : firstLevel()));
+    }
     struct State {
         EncryptionParameters parms;
         bool expand = true;
         std::string error;
         shc_context *handle = nullptr;
+        std::vector<ParmsID> ids;               // by level (prime count)
+        std::map<ParmsID, uint32_t> level_of;   // parms_id -> level
         DeviceArray<uint64_t> ws;
         ~State() {
             if (handle) shc_context_destroy(handle);
@@ -447,22 +615,294 @@ private:
     std::shared_ptr<State> s_;
 };
 
+namespace detail {
+template <class T>
+inline void put(std::ostream &os, const T &v) {
+    os.write(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+template <class T>
+inline T get(std::istream &is, const char *what = "stream ended early") {
+    T v{};
+    is.read(reinterpret_cast<char *>(&v), sizeof(T));
+    if (!is) throw std::invalid_argument(what);
+    return v;
+}
+
+// Plaintext::operator=(const std::string &) (src/plaintext.cpp:91-200): "7FFx^3 + 1x^1
+// + 3" -- hexadecimal coefficients, decimal exponents strictly decreasing, "x^1" for
+// the linear term, a bare constant term, terms joined by exactly " + ".
+inline std::vector<uint64_t> parse_hex_poly(const std::string &s) {
+    auto is_hex = [](char c) { return std::isxdigit((unsigned char)c) != 0; };
+    auto hex_val = [](char c) { return c <= '9' ? c - '0' : (c | 0x20) - 'a' + 10; };
+    std::vector<std::pair<uint64_t, uint64_t>> terms;  // (power, coefficient)
+    size_t pos = 0;
+    uint64_t last = UINT64_MAX;
+    while (pos < s.size()) {
+        size_t len = 0;
+        uint64_t coeff = 0;
+        int bits = 0;
+        while (pos + len < s.size() && is_hex(s[pos + len])) {
+            const int v = hex_val(s[pos + len]);
+            if (bits || v) bits = bits ? bits + 4 : (v >= 8 ? 4 : v >= 4 ? 3 : v >= 2 ? 2 : 1);
+            if (bits > 64) throw std::invalid_argument("hex_poly has too large coefficients");
+            coeff = (coeff << 4) | (uint64_t)v;
+            len++;
+        }
+        if (!len) throw std::invalid_argument("unable to parse hex_poly");
+        pos += len;
+        uint64_t power = 0;
+        if (pos < s.size() && s[pos] != ' ') {
+            if (s[pos] != 'x' || pos + 1 >= s.size() || s[pos + 1] != '^')
+                throw std::invalid_argument("unable to parse hex_poly");
+            pos += 2;
+            size_t digits = 0;
+            while (pos < s.size() && s[pos] >= '0' && s[pos] <= '9') {
+                power = power * 10 + (uint64_t)(s[pos] - '0');
+                if (power > (uint64_t)0x7fffffff) throw std::invalid_argument("unable to parse hex_poly");
+                pos++;
+                digits++;
+            }
+            if (!digits) throw std::invalid_argument("unable to parse hex_poly");
+        }
+        if (power >= last) throw std::invalid_argument("unable to parse hex_poly");
+        last = power;
+        terms.emplace_back(power, coeff);
+        if (pos < s.size()) {
+            if (s.compare(pos, 3, " + ") != 0) throw std::invalid_argument("unable to parse hex_poly");
+            pos += 3;  // a trailing " + " ends the string, as the reference allows
+        }
+    }
+    std::vector<uint64_t> out(terms.empty() ? 0 : (size_t)terms.front().first + 1, 0);
+    bool any = false;
+    for (auto &t : terms) {
+        out[t.first] = t.second;
+        any = any || t.second;
+    }
+    if (!any) out.clear();  // all-zero: the empty polynomial (setZero of nothing)
+    return out;
+}
+
+// Plaintext::to_string (src/plaintext.h:491-498 over util::polyToHexString): nonzero
+// terms from the highest power down, upper-case hexadecimal, "0" for the zero polynomial.
+inline std::string hex_poly_string(const uint64_t *c, size_t count) {
+    std::ostringstream os;
+    bool empty = true;
+    for (size_t i = count; i-- > 0;) {
+        if (!c[i]) continue;
+        if (!empty) os << " + ";
+        char buf[24];
+        std::snprintf(buf, sizeof buf, "%llX", (unsigned long long)c[i]);
+        os << buf;
+        if (i) os << "x^" << i;
+        empty = false;
+    }
+    if (empty) os << "0";
+    return os.str();
+}
+}  // namespace detail
+
 // ------------------------------------------------------------------ data containers
 
-// PlaintextCuda: BFV plaintexts are N coefficients mod t (coefficient form) or, after
-// transformToNtt, [L][N] in NTT form; CKKS plaintexts are [L][N] in NTT form.
+// The CPU-side containers that the reference's cpu() / toHost() return and its
+// conversion constructors take (troy::Plaintext, troy::Ciphertext, src/plaintext.h,
+// src/ciphertext.h): the same fields in host memory.
+namespace host {
+
 class Plaintext {
 public:
     Plaintext() = default;
+    explicit Plaintext(size_t coeff_count) : data_(coeff_count, 0) {}
+    Plaintext(const std::string &hex_poly) { *this = hex_poly; }
+    Plaintext &operator=(const std::string &hex_poly) {
+        if (isNttForm()) throw std::logic_error("cannot set an NTT transformed Plaintext");
+        data_ = detail::parse_hex_poly(hex_poly);
+        return *this;
+    }
+    Plaintext &operator=(uint64_t const_coeff) {
+        data_.assign(1, const_coeff);
+        parms_id_ = parmsIDZero;
+        return *this;
+    }
+    void resize(size_t coeff_count) {
+        if (isNttForm()) throw std::logic_error("cannot resize an NTT transformed Plaintext");
+        data_.resize(coeff_count, 0);
+    }
+    size_t coeffCount() const { return data_.size(); }
+    size_t significantCoeffCount() const {
+        size_t k = data_.size();
+        while (k && !data_[k - 1]) k--;
+        return k;
+    }
+    size_t nonzeroCoeffCount() const { return (size_t)std::count_if(data_.begin(), data_.end(), [](uint64_t x) { return x; }); }
+    bool isZero() const { return significantCoeffCount() == 0; }
+    bool isNttForm() const { return parms_id_ != parmsIDZero; }
+    std::string to_string() const {
+        if (isNttForm()) throw std::invalid_argument("cannot convert NTT transformed plaintext to string");
+        return detail::hex_poly_string(data_.data(), data_.size());
+    }
+    uint64_t operator[](size_t i) const { return data_.at(i); }
+    uint64_t &operator[](size_t i) { return data_.at(i); }
+    // Plaintext::operator== (src/plaintext.h:396-410): the same significant
+    // coefficients, the same NTT level (or neither in NTT form), close scales.
+    bool operator==(const Plaintext &o) const {
+        const size_t a = significantCoeffCount(), b = o.significantCoeffCount();
+        const bool ids = (isNttForm() && o.isNttForm() && parms_id_ == o.parms_id_) || (!isNttForm() && !o.isNttForm());
+        const double d = std::fabs(scale_ - o.scale_);
+        return ids && a == b && std::equal(data_.begin(), data_.begin() + a, o.data_.begin()) &&
+               d <= std::max(std::fabs(scale_), std::fabs(o.scale_)) * 1e-9;
+    }
+    bool operator!=(const Plaintext &o) const { return !(*this == o); }
     ParmsID &parmsID() { return parms_id_; }
-    ParmsID parmsID() const { return parms_id_; }
+    const ParmsID &parmsID() const { return parms_id_; }
+    double &scale() { return scale_; }
+    double scale() const { return scale_; }
+    std::vector<uint64_t> &data() { return data_; }
+    const std::vector<uint64_t> &data() const { return data_; }
+
+private:
+    ParmsID parms_id_ = parmsIDZero;
+    double scale_ = 1.0;
+    std::vector<uint64_t> data_;
+};
+
+class Ciphertext {
+public:
+    ParmsID &parmsID() { return parms_id_; }
+    const ParmsID &parmsID() const { return parms_id_; }
+    bool &isNttForm() { return ntt_form_; }
+    bool isNttForm() const { return ntt_form_; }
+    size_t &size() { return size_; }
+    size_t size() const { return size_; }
+    size_t &polyModulusDegree() { return n_; }
+    size_t polyModulusDegree() const { return n_; }
+    size_t &coeffModulusSize() { return L_; }
+    size_t coeffModulusSize() const { return L_; }
+    double &scale() { return scale_; }
+    double scale() const { return scale_; }
+    uint64_t &correctionFactor() { return correction_; }
+    uint64_t correctionFactor() const { return correction_; }
+    std::vector<uint64_t> &data() { return data_; }
+    const std::vector<uint64_t> &data() const { return data_; }
+    // All-zero c1.. (an unencrypted value, Ciphertext::isTransparent, src/ciphertext.h).
+    bool isTransparent() const {
+        if (size_ < 2) return true;
+        return std::all_of(data_.begin() + (std::ptrdiff_t)(n_ * L_), data_.end(), [](uint64_t x) { return !x; });
+    }
+    bool operator==(const Ciphertext &o) const {
+        return parms_id_ == o.parms_id_ && ntt_form_ == o.ntt_form_ && size_ == o.size_ && n_ == o.n_ &&
+               L_ == o.L_ && scale_ == o.scale_ && correction_ == o.correction_ && data_ == o.data_;
+    }
+    bool operator!=(const Ciphertext &o) const { return !(*this == o); }
+
+private:
+    ParmsID parms_id_ = parmsIDZero;
+    bool ntt_form_ = false;
+    size_t size_ = 0, n_ = 0, L_ = 0;
+    double scale_ = 1.0;
+    uint64_t correction_ = 1;
+    std::vector<uint64_t> data_;
+};
+
+}  // namespace host
+
+// PlaintextCuda (src/plaintext_cuda.cuh): BFV plaintexts are coefficients mod t
+// (coefficient form, parms_id zero) or, after transformToNtt, [L][N] in NTT form; CKKS
+// plaintexts are [L][N] in NTT form with that level's parms_id.
+class Plaintext {
+public:
+    using pt_coeff_type = uint64_t;
+    Plaintext() = default;
+    explicit Plaintext(size_t coeff_count) { resize(coeff_count); }
+    Plaintext(const host::Plaintext &h) : parms_id_(h.parmsID()), scale_(h.scale()) {
+        data_.fromHost(h.data().data(), h.data().size());
+    }
+    Plaintext(const std::string &hex_poly) { *this = hex_poly; }
+    Plaintext(const Plaintext &) = default;
+    Plaintext(Plaintext &&) = default;
+    Plaintext &operator=(const Plaintext &) = default;
+    Plaintext &operator=(Plaintext &&) = default;
+    Plaintext &operator=(const std::string &hex_poly) {
+        if (isNttForm()) throw std::logic_error("cannot set an NTT transformed Plaintext");
+        host::Plaintext h(hex_poly);
+        data_.fromHost(h.data().data(), h.data().size());
+        return *this;
+    }
+    Plaintext &operator=(uint64_t const_coeff) {
+        data_.fromHost(&const_coeff, 1);
+        parms_id_ = parmsIDZero;
+        return *this;
+    }
+
+    host::Plaintext cpu() const {
+        host::Plaintext h;
+        h.parmsID() = parms_id_;
+        h.scale() = scale_;
+        h.data() = data_.toHost();
+        return h;
+    }
+    host::Plaintext toHost() const { return cpu(); }
+    std::string to_string() const { return cpu().to_string(); }
+    bool isZero() const { return cpu().isZero(); }
+    bool operator==(const Plaintext &o) const { return cpu() == o.cpu(); }
+    bool operator!=(const Plaintext &o) const { return !(*this == o); }
+    uint64_t operator[](size_t i) const {
+        if (i >= coeffCount()) throw std::out_of_range("coeff_index must be within [0, coeff_count)");
+        uint64_t v = 0;
+        hip_check(hipMemcpy(&v, data_.get() + i, 8, hipMemcpyDeviceToHost), "copy to host");
+        return v;
+    }
+
+    void resize(size_t coeff_count) {
+        if (isNttForm()) throw std::logic_error("cannot resize an NTT transformed Plaintext");
+        data_.resize(coeff_count);
+    }
+    void release() {
+        parms_id_ = parmsIDZero;
+        scale_ = 1.0;
+        data_.release();
+    }
+    void setZero() { data_.setZero(); }
+    void setZero(size_t start_coeff) {
+        if (start_coeff >= coeffCount()) throw std::out_of_range("start_coeff must be within [0, coeff_count)");
+        hip_check(hipMemset(data_.get() + start_coeff, 0, (coeffCount() - start_coeff) * 8), "memset");
+    }
+
+    // PlaintextCuda::save / load (src/plaintext_cuda.cu:6-27): parms_id (32 B), coeff
+    // count, scale, word count, words.
+    void save(std::ostream &os) const {
+        detail::put(os, parms_id_);
+        detail::put(os, (uint64_t)coeffCount());
+        detail::put(os, scale_);
+        const std::vector<uint64_t> h = data_.toHost();
+        detail::put(os, (uint64_t)h.size());
+        os.write(reinterpret_cast<const char *>(h.data()), (std::streamsize)(h.size() * 8));
+    }
+    void load(std::istream &is) {
+        const char *what = "stream ended before the plaintext did";
+        const ParmsID id = detail::get<ParmsID>(is, what);
+        const uint64_t count = detail::get<uint64_t>(is, what);
+        const double scale = detail::get<double>(is, what);
+        const uint64_t words = detail::get<uint64_t>(is, what);
+        if (words != count) throw std::invalid_argument("plaintext word count does not match its coefficient count");
+        std::vector<uint64_t> h(words);
+        is.read(reinterpret_cast<char *>(h.data()), (std::streamsize)(words * 8));
+        if (!is) throw std::invalid_argument(what);
+        data_.fromHost(h.data(), h.size());
+        parms_id_ = id;
+        scale_ = scale;
+    }
+
+    ParmsID &parmsID() { return parms_id_; }
+    const ParmsID &parmsID() const { return parms_id_; }
     double &scale() { return scale_; }
     double scale() const { return scale_; }
     bool isNttForm() const { return parms_id_ != parmsIDZero; }
     size_t coeffCount() const { return data_.size(); }
-    DeviceArray<uint64_t> &data() { return data_; }
-    const DeviceArray<uint64_t> &data() const { return data_; }
-    void resize(size_t coeff_count) { data_.resize(coeff_count); }
+    size_t capacity() const { return data_.capacity(); }
+    uint64_t *data() { return data_.get(); }
+    const uint64_t *data() const { return data_.get(); }
+    DeviceArray<uint64_t> &dynArray() { return data_; }
+    const DeviceArray<uint64_t> &dynArray() const { return data_; }
 
 private:
     DeviceArray<uint64_t> data_;
@@ -472,50 +912,76 @@ private:
 
 class Evaluator;
 
-namespace detail {
-template <class T>
-inline void put(std::ostream &os, const T &v) {
-    os.write(reinterpret_cast<const char *>(&v), sizeof(T));
-}
-template <class T>
-inline T get(std::istream &is) {
-    T v{};
-    is.read(reinterpret_cast<char *>(&v), sizeof(T));
-    if (!is) throw std::invalid_argument("stream ended before the ciphertext did");
-    return v;
-}
-}  // namespace detail
-
 // CiphertextCuda (src/ciphertext_cuda.cuh:12-246): [size][L][N] words on the device.
 class Ciphertext {
 public:
     Ciphertext() = default;
-    void resize(const SEALContext &ctx, ParmsID id, size_t size) {
-        n_ = ctx.polyModulusDegree();
-        if (id != parms_id_ || parms_hash_ == std::array<uint64_t, 4>{}) parms_hash_ = ctx.parmsHash(id);
-        parms_id_ = id;
-        size_ = size;
-        data_.resize(size * id.level * n_);
+    // Empty ciphertexts of the first level (or of `id`), as CiphertextCuda(context[, id]).
+    explicit Ciphertext(const SEALContext &ctx) { setLevel(ctx, ctx.firstParmsID()); }
+    Ciphertext(const SEALContext &ctx, ParmsID id) { setLevel(ctx, id); }
+    Ciphertext(const SEALContext &ctx, ParmsID id, size_t size) { resize(ctx, id, size); }
+    Ciphertext(const host::Ciphertext &h)
+        : parms_id_(h.parmsID()), size_(h.size()), n_(h.polyModulusDegree()), L_(h.coeffModulusSize()),
+          ntt_form_(h.isNttForm()), scale_(h.scale()), correction_(h.correctionFactor()) {
+        data_.fromHost(h.data().data(), h.data().size());
     }
+    Ciphertext(const Ciphertext &) = default;
+    Ciphertext(Ciphertext &&) = default;
+    Ciphertext &operator=(const Ciphertext &) = default;
+    Ciphertext &operator=(Ciphertext &&) = default;
+
+    void resize(const SEALContext &ctx, ParmsID id, size_t size) {
+        setLevel(ctx, id);
+        resize(size);
+    }
+    void resize(const SEALContext &ctx, size_t size) { resize(ctx, ctx.firstParmsID(), size); }
+    void resize(size_t size) {
+        size_ = size;
+        data_.resize(size * L_ * n_);
+    }
+    void release() {
+        parms_id_ = parmsIDZero;
+        ntt_form_ = false;
+        size_ = n_ = L_ = 0;
+        scale_ = 1.0;
+        correction_ = 1;
+        data_.release();
+    }
+
+    host::Ciphertext cpu() const {
+        host::Ciphertext h;
+        h.parmsID() = parms_id_;
+        h.isNttForm() = ntt_form_;
+        h.size() = size_;
+        h.polyModulusDegree() = n_;
+        h.coeffModulusSize() = L_;
+        h.scale() = scale_;
+        h.correctionFactor() = correction_;
+        h.data() = data_.toHost();
+        h.data().resize(size_ * L_ * n_);
+        return h;
+    }
+    host::Ciphertext toHost() const { return cpu(); }
+    bool isTransparent() const { return cpu().isTransparent(); }
 
     // CiphertextCuda::save / load (src/ciphertext_cuda.cu:8-35, 138-183): parms_id
     // (32 B), is_ntt_form (1 B), size, N, L (8 B each), scale (double), correction
-    // factor (1), seed (0), terms flag (false), word count, words -- the reference's raw
+    // factor, seed (0), terms flag (false), word count, words -- the reference's raw
     // layout, so files move between the two.  Seeded ciphertexts (c1 expanded from a
     // curand stream) are not loadable here.
     void save(std::ostream &os) const {
         saveHeader(os, false);
-        const std::vector<uint64_t> h = data_.toHost();
+        std::vector<uint64_t> h = data_.toHost();
         const uint64_t words = (uint64_t)(size_ * polyWords());
         detail::put(os, words);
-        os.write(reinterpret_cast<const char *>(h.data()), words * 8);
+        os.write(reinterpret_cast<const char *>(h.data()), (std::streamsize)(words * 8));
     }
     void load(std::istream &is, const SEALContext &ctx) {
         loadHeader(is, ctx, false);
-        const uint64_t words = detail::get<uint64_t>(is);
+        const uint64_t words = detail::get<uint64_t>(is, "stream ended before the ciphertext did");
         if (words != size_ * polyWords()) throw std::invalid_argument("ciphertext word count does not match its shape");
         std::vector<uint64_t> h(words);
-        is.read(reinterpret_cast<char *>(h.data()), words * 8);
+        is.read(reinterpret_cast<char *>(h.data()), (std::streamsize)(words * 8));
         if (!is) throw std::invalid_argument("stream ended before the ciphertext did");
         data_.fromHost(h.data(), h.size());
     }
@@ -523,77 +989,86 @@ public:
     // of c0 (coefficient form), then every other polynomial in full.
     void saveTerms(std::ostream &os, const Evaluator &ev, const std::vector<size_t> &term_ids) const;
     void loadTerms(std::istream &is, const Evaluator &ev, const SEALContext &ctx, const std::vector<size_t> &term_ids);
-    ParmsID parmsID() const { return parms_id_; }
+
     ParmsID &parmsID() { return parms_id_; }
+    const ParmsID &parmsID() const { return parms_id_; }
     size_t size() const { return size_; }
     size_t polyModulusDegree() const { return n_; }
-    size_t coeffModulusSize() const { return parms_id_.level; }
+    size_t coeffModulusSize() const { return L_; }
     bool isNttForm() const { return ntt_form_; }
     bool &isNttForm() { return ntt_form_; }
     double scale() const { return scale_; }
     double &scale() { return scale_; }
+    uint64_t correctionFactor() const { return correction_; }
+    uint64_t &correctionFactor() { return correction_; }
     uint64_t *data() { return data_.get(); }
     const uint64_t *data() const { return data_.get(); }
-    uint64_t *data(size_t poly) { return data_.get() + poly * coeffModulusSize() * n_; }
-    const uint64_t *data(size_t poly) const { return data_.get() + poly * coeffModulusSize() * n_; }
-    size_t polyWords() const { return coeffModulusSize() * n_; }
-    std::vector<uint64_t> toHost() const { return data_.toHost(); }
+    uint64_t *data(size_t poly) { return data_.get() + poly * L_ * n_; }
+    const uint64_t *data(size_t poly) const { return data_.get() + poly * L_ * n_; }
+    size_t polyWords() const { return L_ * n_; }
+    size_t polyCoeffSize() const { return L_ * n_; }
     // Keeps the first `size` polynomials (relinearization drops the third).
     void shrink(size_t size) { size_ = size; }
     DeviceArray<uint64_t> &dynArray() { return data_; }
-    const std::array<uint64_t, 4> &parmsHash() const { return parms_hash_; }
+    const DeviceArray<uint64_t> &dynArray() const { return data_; }
 
 private:
+    void setLevel(const SEALContext &ctx, ParmsID id) {
+        L_ = ctx.levelOrThrow(id, "parms_id");
+        parms_id_ = id;
+        n_ = ctx.polyModulusDegree();
+    }
     void saveHeader(std::ostream &os, bool terms) const {
-        detail::put(os, parms_hash_);
+        detail::put(os, parms_id_);
         detail::put(os, ntt_form_);
         detail::put(os, (uint64_t)size_);
         detail::put(os, (uint64_t)n_);
-        detail::put(os, (uint64_t)coeffModulusSize());
+        detail::put(os, (uint64_t)L_);
         detail::put(os, scale_);
-        detail::put(os, (uint64_t)1);  // correction factor
+        detail::put(os, (uint64_t)correction_);
         detail::put(os, (uint64_t)0);  // seed
         detail::put(os, terms);
     }
     void loadHeader(std::istream &is, const SEALContext &ctx, bool terms) {
-        const auto hash = detail::get<std::array<uint64_t, 4>>(is);
-        const bool ntt = detail::get<bool>(is);
-        const uint64_t size = detail::get<uint64_t>(is), n = detail::get<uint64_t>(is), L = detail::get<uint64_t>(is);
-        const double scale = detail::get<double>(is);
-        (void)detail::get<uint64_t>(is);  // correction factor
-        const uint64_t seed = detail::get<uint64_t>(is);
-        const bool is_terms = detail::get<bool>(is);
+        const char *what = "stream ended before the ciphertext did";
+        const ParmsID id = detail::get<ParmsID>(is, what);
+        const bool ntt = detail::get<bool>(is, what);
+        const uint64_t size = detail::get<uint64_t>(is, what), n = detail::get<uint64_t>(is, what),
+                       L = detail::get<uint64_t>(is, what);
+        const double scale = detail::get<double>(is, what);
+        const uint64_t correction = detail::get<uint64_t>(is, what);
+        const uint64_t seed = detail::get<uint64_t>(is, what);
+        const bool is_terms = detail::get<bool>(is, what);
         if (seed != 0) throw std::invalid_argument("seeded ciphertexts are not supported");
         if (is_terms != terms)
             throw std::invalid_argument(terms ? "Trying to load a normal ciphertext, but term indices is specified"
                                               : "Trying to load a termed ciphertext, but indices is not specified");
-        const ParmsID id = ctx.levelOfHash(hash);
-        if (id == parmsIDZero || id.level != L || n != ctx.polyModulusDegree())
+        if (ctx.levelOf(id) == 0 || ctx.levelOf(id) != L || n != ctx.polyModulusDegree())
             throw std::invalid_argument("ciphertext is not valid for encryption parameters");
         resize(ctx, id, size);
         ntt_form_ = ntt;
         scale_ = scale;
+        correction_ = correction;
     }
 
     DeviceArray<uint64_t> data_;
-    std::array<uint64_t, 4> parms_hash_{};
     ParmsID parms_id_ = parmsIDZero;
-    size_t size_ = 0, n_ = 0;
+    size_t size_ = 0, n_ = 0, L_ = 0;
     bool ntt_form_ = false;
     double scale_ = 1.0;
+    uint64_t correction_ = 1;
 };
 
-// SecretKeyCuda: [K][N] in NTT form.
+// SecretKeyCuda: a key-level plaintext [K][N] in NTT form.
 class SecretKey {
 public:
-    DeviceArray<uint64_t> &data() { return data_; }
-    const DeviceArray<uint64_t> &data() const { return data_; }
-    ParmsID parmsID() const { return parms_id_; }
-    ParmsID &parmsID() { return parms_id_; }
+    Plaintext &data() { return data_; }
+    const Plaintext &data() const { return data_; }
+    ParmsID &parmsID() { return data_.parmsID(); }
+    const ParmsID &parmsID() const { return data_.parmsID(); }
 
 private:
-    DeviceArray<uint64_t> data_;
-    ParmsID parms_id_ = parmsIDZero;
+    Plaintext data_;
 };
 
 // PublicKeyCuda: a size-2 key-level ciphertext in NTT form.
@@ -601,50 +1076,72 @@ class PublicKey {
 public:
     Ciphertext &data() { return data_; }
     const Ciphertext &data() const { return data_; }
-    ParmsID parmsID() const { return data_.parmsID(); }
+    ParmsID &parmsID() { return data_.parmsID(); }
+    const ParmsID &parmsID() const { return data_.parmsID(); }
 
 private:
     Ciphertext data_;
 };
 
-// KSwitchKeysCuda: [K-1][2][K][N] per key.  RelinKeys hold one (for s^2); GaloisKeys
-// hold one per Galois element.
-class RelinKeys {
+// KSwitchKeysCuda (src/kswitchkeys_cuda.cuh:31-170): a list of key-switching keys, each
+// [K-1][2][K][N] in NTT form (the words of the reference's K-1 public keys, back to
+// back); empty entries are absent keys.  The base of RelinKeys and GaloisKeys.
+class KSwitchKeys {
 public:
-    DeviceArray<uint64_t> &data() { return data_; }
-    const DeviceArray<uint64_t> &data() const { return data_; }
-    ParmsID parmsID() const { return parms_id_; }
+    // The number of keys present.
+    size_t size() const {
+        return (size_t)std::count_if(keys_.begin(), keys_.end(), [](const DeviceArray<uint64_t> &k) { return k.size(); });
+    }
+    std::vector<DeviceArray<uint64_t>> &data() { return keys_; }
+    const std::vector<DeviceArray<uint64_t>> &data() const { return keys_; }
+    DeviceArray<uint64_t> &data(size_t index) {
+        if (index >= keys_.size() || !keys_[index].size()) throw std::invalid_argument("keyswitching key does not exist");
+        return keys_[index];
+    }
+    const DeviceArray<uint64_t> &data(size_t index) const {
+        if (index >= keys_.size() || !keys_[index].size()) throw std::invalid_argument("keyswitching key does not exist");
+        return keys_[index];
+    }
     ParmsID &parmsID() { return parms_id_; }
-    size_t size() const { return data_.size() ? 1 : 0; }
+    const ParmsID &parmsID() const { return parms_id_; }
 
-private:
-    DeviceArray<uint64_t> data_;
+protected:
+    std::vector<DeviceArray<uint64_t>> keys_;
     ParmsID parms_id_ = parmsIDZero;
 };
 
-class GaloisKeys {
+// RelinKeysCuda (src/relinkeys_cuda.cuh:11-60): key for s^p at index p - 2.
+class RelinKeys : public KSwitchKeys {
 public:
-    bool hasKey(uint32_t elt) const { return keys_.count(elt) != 0; }
-    const DeviceArray<uint64_t> &key(uint32_t elt) const {
-        auto it = keys_.find(elt);
-        if (it == keys_.end()) throw std::invalid_argument("Galois key not present");
-        return it->second;
+    static size_t getIndex(size_t key_power) {
+        if (key_power < 2) throw std::invalid_argument("key_power cannot be less than 2");
+        return key_power - 2;
     }
-    DeviceArray<uint64_t> &operator[](uint32_t elt) { return keys_[elt]; }
-    size_t size() const { return keys_.size(); }
-    ParmsID parmsID() const { return parms_id_; }
-    ParmsID &parmsID() { return parms_id_; }
+    bool hasKey(size_t key_power) const {
+        const size_t i = getIndex(key_power);
+        return keys_.size() > i && keys_[i].size();
+    }
+    const DeviceArray<uint64_t> &key(size_t key_power) const { return data(getIndex(key_power)); }
+};
 
-private:
-    std::map<uint32_t, DeviceArray<uint64_t>> keys_;
-    ParmsID parms_id_ = parmsIDZero;
+// GaloisKeysCuda (src/galoiskeys_cuda.cuh:34-80): key of Galois element e at index (e - 1) / 2.
+class GaloisKeys : public KSwitchKeys {
+public:
+    static size_t getIndex(uint32_t galois_elt) {
+        if (!(galois_elt & 1)) throw std::invalid_argument("galois_elt is not valid");
+        return (galois_elt - 1) >> 1;
+    }
+    bool hasKey(uint32_t galois_elt) const {
+        const size_t i = getIndex(galois_elt);
+        return keys_.size() > i && keys_[i].size();
+    }
+    const DeviceArray<uint64_t> &key(uint32_t galois_elt) const {
+        if (!hasKey(galois_elt)) throw std::invalid_argument("Galois key not present");
+        return keys_[getIndex(galois_elt)];
+    }
 };
 
 namespace detail {
-inline uint64_t fresh_seed() {
-    std::random_device rd;
-    return ((uint64_t)rd() << 32) ^ rd();
-}
 inline uint32_t elt_from_step(size_t n, int step) {
     uint32_t elt = 0;
     check(shc_galois_elt_from_step((uint32_t)n, step, &elt));
@@ -655,24 +1152,27 @@ inline uint32_t elt_from_step(size_t n, int step) {
 // ---------------------------------------------------------------------- key generator
 
 // KeyGeneratorCuda (src/keygenerator_cuda.cuh:30-300).  Keys are generated on the
-// device with the library's counter-based sampler (seeded from std::random_device).
+// device from the context's ChaCha20 sampler stream (a 256-bit key from
+// std::random_device); every key takes a fresh 64-bit nonce from std::random_device.
 class KeyGenerator {
 public:
-    explicit KeyGenerator(const SEALContext &ctx) : ctx_(ctx), seed_(detail::fresh_seed()) {
+    explicit KeyGenerator(const SEALContext &ctx) : ctx_(ctx) {
         ctx_.checkSet();
-        sk_.data().resize(ctx_.keyCount() * ctx_.polyModulusDegree());
-        check(shc_keygen_secret(ctx_.handle(), next(), sk_.data().get(), nullptr));
-        sk_.parmsID() = ctx_.keyParmsID();
+        Plaintext &sk = sk_.data();
+        sk.dynArray().resize(ctx_.keyCount() * ctx_.polyModulusDegree());
+        check(shc_keygen_secret(ctx_.handle(), detail::os_random64(), sk.data(), nullptr));
+        sk.parmsID() = ctx_.keyParmsID();
     }
-    KeyGenerator(const SEALContext &ctx, const SecretKey &sk) : ctx_(ctx), sk_(sk), seed_(detail::fresh_seed()) {
+    KeyGenerator(const SEALContext &ctx, const SecretKey &sk) : ctx_(ctx), sk_(sk) {
         ctx_.checkSet();
+        if (sk.parmsID() != ctx_.keyParmsID()) throw std::invalid_argument("secret key is not valid for encryption parameters");
     }
     const SecretKey &secretKey() const { return sk_; }
 
     void createPublicKey(PublicKey &pk) const {
         pk.data().resize(ctx_, ctx_.keyParmsID(), 2);
         pk.data().isNttForm() = true;
-        check(shc_keygen_public(ctx_.handle(), sk_.data().get(), next(), pk.data().data(), nullptr));
+        check(shc_keygen_public(ctx_.handle(), sk_.data().data(), detail::os_random64(), pk.data().data(), nullptr));
     }
     PublicKey createPublicKey() const {
         PublicKey pk;
@@ -681,11 +1181,12 @@ public:
     }
     void createRelinKeys(RelinKeys &rk) const {
         requireKeyswitching();
-        rk.data().resize(keyWords());
+        rk.data().resize(1);
+        rk.data()[0].resize(keyWords());
         size_t ws = 0;
         check(shc_keygen_workspace_size(ctx_.handle(), &ws));
-        check(shc_keygen_relin(ctx_.handle(), sk_.data().get(), next(), rk.data().get(), ctx_.workspace(ws), ws,
-                               nullptr));
+        check(shc_keygen_relin(ctx_.handle(), sk_.data().data(), detail::os_random64(), rk.data()[0].get(),
+                               ctx_.workspace(ws), ws, nullptr));
         rk.parmsID() = ctx_.keyParmsID();
     }
     RelinKeys createRelinKeys() const {
@@ -697,11 +1198,12 @@ public:
         requireKeyswitching();
         size_t ws = 0;
         check(shc_keygen_workspace_size(ctx_.handle(), &ws));
+        if (gk.data().size() < ctx_.polyModulusDegree()) gk.data().resize(ctx_.polyModulusDegree());
         for (uint32_t e : elts) {
-            auto &k = gk[e];
+            auto &k = gk.data()[GaloisKeys::getIndex(e)];
             k.resize(keyWords());
-            check(shc_keygen_galois(ctx_.handle(), sk_.data().get(), e, next(), k.get(), ctx_.workspace(ws), ws,
-                                    nullptr));
+            check(shc_keygen_galois(ctx_.handle(), sk_.data().data(), e, detail::os_random64(), k.get(),
+                                    ctx_.workspace(ws), ws, nullptr));
         }
         gk.parmsID() = ctx_.keyParmsID();
     }
@@ -747,7 +1249,6 @@ public:
     }
 
 private:
-    uint64_t next() const { return seed_ + (counter_++) * 0x9E3779B97F4A7C15ull; }
     size_t keyWords() const {
         return (ctx_.keyCount() - 1) * 2 * ctx_.keyCount() * ctx_.polyModulusDegree();
     }
@@ -756,8 +1257,6 @@ private:
     }
     SEALContext ctx_;
     SecretKey sk_;
-    uint64_t seed_;
-    mutable uint64_t counter_ = 1;
 };
 
 // ------------------------------------------------------------------------- encoders
@@ -778,9 +1277,9 @@ public:
             if (v >= t) throw std::invalid_argument("input value is larger than plain_modulus");
         DeviceArray<uint64_t> dv;
         dv.fromHost(values.data(), values.size());
-        plain.resize(n);
         plain.parmsID() = parmsIDZero;
-        check(shc_batch_encode(ctx_.handle(), dv.get(), (uint32_t)values.size(), plain.data().get(), nullptr));
+        plain.resize(n);
+        check(shc_batch_encode(ctx_.handle(), dv.get(), (uint32_t)values.size(), plain.data(), nullptr));
     }
     void encode(const std::vector<int64_t> &values, Plaintext &plain) const {
         const int64_t t = (int64_t)ctx_.parms().plainModulus().value();
@@ -795,8 +1294,17 @@ public:
     void decode(const Plaintext &plain, std::vector<uint64_t> &out) const {
         if (plain.isNttForm()) throw std::invalid_argument("plain cannot be in NTT form");
         const size_t n = slotCount();
-        DeviceArray<uint64_t> dv(n), scratch(n);
-        check(shc_batch_decode(ctx_.handle(), plain.data().get(), dv.get(), scratch.get(), nullptr));
+        DeviceArray<uint64_t> full, dv(n), scratch(n);
+        const uint64_t *p = plain.data();
+        if (plain.coeffCount() != n) {
+            if (plain.coeffCount() > n) throw std::invalid_argument("plain is not valid for encryption parameters");
+            full.resize(n);
+            full.setZero();
+            if (plain.coeffCount())
+                hip_check(hipMemcpy(full.get(), plain.data(), plain.coeffCount() * 8, hipMemcpyDeviceToDevice), "copy");
+            p = full.get();
+        }
+        check(shc_batch_decode(ctx_.handle(), p, dv.get(), scratch.get(), nullptr));
         out = dv.toHost();
     }
     void decode(const Plaintext &plain, std::vector<int64_t> &out) const {
@@ -822,9 +1330,8 @@ public:
     size_t slotCount() const { return ctx_.polyModulusDegree() / 2; }
     void encode(const std::vector<std::complex<double>> &values, ParmsID id, double scale, Plaintext &plain) const {
         if (values.size() > slotCount()) throw std::invalid_argument("values has invalid size");
-        if (!ctx_.getContextData(id)) throw std::invalid_argument("parms_id is not valid for encryption parameters");
-        if (scale <= 0 || (int)std::log2(scale) >= ctx_.levelBits(id))
-            throw std::invalid_argument("scale out of bounds");
+        const uint32_t L = ctx_.levelOrThrow(id, "parms_id");
+        if (scale <= 0 || (int)std::log2(scale) >= ctx_.levelBits(L)) throw std::invalid_argument("scale out of bounds");
         std::vector<double> flat(2 * values.size());
         for (size_t i = 0; i < values.size(); i++) {
             flat[2 * i] = values[i].real();
@@ -832,11 +1339,12 @@ public:
         }
         DeviceArray<double> dv;
         dv.fromHost(flat.data(), flat.size());
-        plain.resize(id.level * ctx_.polyModulusDegree());
+        plain.parmsID() = parmsIDZero;
+        plain.resize(L * ctx_.polyModulusDegree());
         size_t ws = 0;
         check(shc_ckks_encode_workspace_size(ctx_.handle(), &ws));
-        check(shc_ckks_encode(ctx_.handle(), dv.get(), (uint32_t)values.size(), scale, id.level,
-                              plain.data().get(), ctx_.workspace(ws), ws, nullptr));
+        check(shc_ckks_encode(ctx_.handle(), dv.get(), (uint32_t)values.size(), scale, L, plain.data(),
+                              ctx_.workspace(ws), ws, nullptr));
         plain.parmsID() = id;
         plain.scale() = scale;
     }
@@ -862,12 +1370,12 @@ public:
     }
     void decode(const Plaintext &plain, std::vector<std::complex<double>> &out) const {
         if (!plain.isNttForm()) throw std::invalid_argument("plain is not in NTT form");
-        const uint32_t L = plain.parmsID().level;
+        const uint32_t L = ctx_.levelOrThrow(plain.parmsID(), "plain");
         const size_t n = ctx_.polyModulusDegree();
         DeviceArray<double> dv(n);
         size_t ws = 0;
         check(shc_ckks_decode_workspace_size(ctx_.handle(), L, &ws));
-        check(shc_ckks_decode(ctx_.handle(), plain.data().get(), L, plain.scale(), dv.get(), ctx_.workspace(ws), ws,
+        check(shc_ckks_decode(ctx_.handle(), plain.data(), L, plain.scale(), dv.get(), ctx_.workspace(ws), ws,
                               nullptr));
         const std::vector<double> h = dv.toHost();
         out.resize(slotCount());
@@ -878,22 +1386,36 @@ private:
     SEALContext ctx_;
 };
 
+namespace detail {
+// A BFV plaintext of fewer than N coefficients, zero-padded to N into `tmp` (the C ABI
+// takes N coefficients); the plaintext's own words when it already has N.
+inline const uint64_t *padded_plain(const SEALContext &ctx, const Plaintext &p, DeviceArray<uint64_t> &tmp) {
+    const size_t n = ctx.polyModulusDegree();
+    if (p.coeffCount() == n) return p.data();
+    if (p.coeffCount() > n) throw std::invalid_argument("plain is not valid for encryption parameters");
+    tmp.resize(n);
+    tmp.setZero();
+    if (p.coeffCount()) hip_check(hipMemcpy(tmp.get(), p.data(), p.coeffCount() * 8, hipMemcpyDeviceToDevice), "copy");
+    return tmp.get();
+}
+}  // namespace detail
+
 // ------------------------------------------------------------------- encrypt/decrypt
 
-// EncryptorCuda (src/encryptor_cuda.cuh:50-400).  Randomness: the library's counter-
-// based sampler, a fresh seed per call (the reference uses curand).
+// EncryptorCuda (src/encryptor_cuda.cuh:50-400).  Randomness: the context's ChaCha20
+// sampler stream, a fresh 64-bit nonce from std::random_device per encryption (the
+// reference uses curand).
 class Encryptor {
 public:
-    Encryptor(const SEALContext &ctx, const PublicKey &pk) : ctx_(ctx), pk_(pk), seed_(detail::fresh_seed()) {
+    Encryptor(const SEALContext &ctx, const PublicKey &pk) : ctx_(ctx), pk_(pk) {
         ctx_.checkSet();
         has_pk_ = true;
     }
-    Encryptor(const SEALContext &ctx, const SecretKey &sk) : ctx_(ctx), sk_(sk), seed_(detail::fresh_seed()) {
+    Encryptor(const SEALContext &ctx, const SecretKey &sk) : ctx_(ctx), sk_(sk) {
         ctx_.checkSet();
         has_sk_ = true;
     }
-    Encryptor(const SEALContext &ctx, const PublicKey &pk, const SecretKey &sk)
-        : ctx_(ctx), pk_(pk), sk_(sk), seed_(detail::fresh_seed()) {
+    Encryptor(const SEALContext &ctx, const PublicKey &pk, const SecretKey &sk) : ctx_(ctx), pk_(pk), sk_(sk) {
         ctx_.checkSet();
         has_pk_ = has_sk_ = true;
     }
@@ -901,13 +1423,16 @@ public:
     void encrypt(const Plaintext &plain, Ciphertext &ct) const {
         if (!has_pk_) throw std::logic_error("public key is not set");
         const ParmsID id = levelOf(plain);
+        const uint32_t L = ctx_.levelOf(id);
+        DeviceArray<uint64_t> tmp;
+        const uint64_t *p = ctx_.scheme() == SchemeType::ckks ? plain.data() : detail::padded_plain(ctx_, plain, tmp);
         ct.resize(ctx_, id, 2);
         ct.isNttForm() = ctx_.scheme() == SchemeType::ckks;
         ct.scale() = ctx_.scheme() == SchemeType::ckks ? plain.scale() : 1.0;
         size_t ws = 0;
-        check(shc_encrypt_workspace_size(ctx_.handle(), id.level, &ws));
-        check(shc_encrypt(ctx_.handle(), pk_.data().data(), plain.data().get(), id.level, next(), ct.data(),
-                          ctx_.workspace(ws), ws, nullptr));
+        check(shc_encrypt_workspace_size(ctx_.handle(), L, &ws));
+        check(shc_encrypt(ctx_.handle(), pk_.data().data(), p, L, detail::os_random64(), ct.data(), ctx_.workspace(ws),
+                          ws, nullptr));
     }
     Ciphertext encrypt(const Plaintext &plain) const {
         Ciphertext ct;
@@ -918,27 +1443,31 @@ public:
     void encryptSymmetric(const Plaintext &plain, Ciphertext &ct) const {
         if (!has_sk_) throw std::logic_error("secret key is not set");
         const ParmsID id = levelOf(plain);
-        ct.resize(ctx_, id, 2);
+        const uint32_t L = ctx_.levelOf(id);
         const bool ckks = ctx_.scheme() == SchemeType::ckks;
+        DeviceArray<uint64_t> tmp;
+        const uint64_t *p = ckks ? plain.data() : detail::padded_plain(ctx_, plain, tmp);
+        ct.resize(ctx_, id, 2);
         ct.isNttForm() = ckks;
         ct.scale() = ckks ? plain.scale() : 1.0;
-        check(shc_encrypt_zero_symmetric(ctx_.handle(), sk_.data().get(), id.level, next(), ct.data(), nullptr));
+        check(shc_encrypt_zero_symmetric(ctx_.handle(), sk_.data().data(), L, detail::os_random64(), ct.data(),
+                                         nullptr));
         if (ckks)
-            check(shc_add_plain(ctx_.handle(), ct.data(), plain.data().get(), ct.data(), 1, 2, id.level, 0, nullptr));
+            check(shc_add_plain(ctx_.handle(), ct.data(), p, ct.data(), 1, 2, L, 0, nullptr));
         else
-            check(shc_bfv_add_plain(ctx_.handle(), ct.data(), plain.data().get(), ct.data(), 1, 2, id.level, 0,
-                                    nullptr));
+            check(shc_bfv_add_plain(ctx_.handle(), ct.data(), p, ct.data(), 1, 2, L, 0, nullptr));
     }
     void encryptZero(ParmsID id, Ciphertext &ct) const {
         if (!has_pk_) throw std::logic_error("public key is not set");
+        const uint32_t L = ctx_.levelOrThrow(id, "parms_id");
         Plaintext zero;
         if (ctx_.scheme() == SchemeType::ckks) {
-            zero.resize(id.level * ctx_.polyModulusDegree());
-            hip_check(hipMemset(zero.data().get(), 0, zero.data().size() * 8), "memset");
+            zero.resize(L * ctx_.polyModulusDegree());
+            zero.setZero();
             zero.parmsID() = id;
         } else {
             zero.resize(ctx_.polyModulusDegree());
-            hip_check(hipMemset(zero.data().get(), 0, zero.data().size() * 8), "memset");
+            zero.setZero();
         }
         encrypt(zero, ct);
     }
@@ -948,18 +1477,16 @@ private:
     ParmsID levelOf(const Plaintext &plain) const {
         if (ctx_.scheme() == SchemeType::ckks) {
             if (!plain.isNttForm()) throw std::invalid_argument("plain must be in NTT form");
+            ctx_.levelOrThrow(plain.parmsID(), "plain");
             return plain.parmsID();
         }
         if (plain.isNttForm()) throw std::invalid_argument("plain cannot be in NTT form");
         return ctx_.firstParmsID();
     }
-    uint64_t next() const { return seed_ + (counter_++) * 0x9E3779B97F4A7C15ull; }
     SEALContext ctx_;
     PublicKey pk_;
     SecretKey sk_;
     bool has_pk_ = false, has_sk_ = false;
-    uint64_t seed_;
-    mutable uint64_t counter_ = 1;
 };
 
 // DecryptorCuda (src/decryptor_cuda.cuh:40-90).
@@ -968,13 +1495,14 @@ public:
     Decryptor(const SEALContext &ctx, const SecretKey &sk) : ctx_(ctx), sk_(sk) { ctx_.checkSet(); }
     void decrypt(const Ciphertext &ct, Plaintext &plain) {
         if (ct.size() < 2 || ct.size() > 3) throw std::invalid_argument("ciphertext size must be 2 or 3");
-        const uint32_t L = ct.parmsID().level;
+        const uint32_t L = ctx_.levelOrThrow(ct.parmsID(), "encrypted");
         const bool ckks = ctx_.scheme() == SchemeType::ckks;
         if (ckks != ct.isNttForm()) throw std::invalid_argument("ciphertext is in the wrong form");
         size_t ws = 0;
         check(shc_decrypt_workspace_size(ctx_.handle(), L, &ws));
+        plain.parmsID() = parmsIDZero;
         plain.resize(ckks ? (size_t)L * ctx_.polyModulusDegree() : ctx_.polyModulusDegree());
-        check(shc_decrypt(ctx_.handle(), ct.data(), (uint32_t)ct.size(), sk_.data().get(), L, plain.data().get(),
+        check(shc_decrypt(ctx_.handle(), ct.data(), (uint32_t)ct.size(), sk_.data().data(), L, plain.data(),
                           ctx_.workspace(ws), ws, nullptr));
         plain.parmsID() = ckks ? ct.parmsID() : parmsIDZero;
         plain.scale() = ct.scale();
@@ -994,7 +1522,7 @@ public:
 
     // ---- additive
     void negateInplace(Ciphertext &a) const {
-        check(shc_negate(ctx_.handle(), a.data(), a.data(), (uint32_t)a.size(), a.parmsID().level, nullptr));
+        check(shc_negate(ctx_.handle(), a.data(), a.data(), (uint32_t)a.size(), level(a), nullptr));
     }
     void negate(const Ciphertext &a, Ciphertext &out) const {
         out = a;
@@ -1022,14 +1550,14 @@ public:
     void multiplyInplace(Ciphertext &a, const Ciphertext &b) const {
         sameParms(a, b);
         if (a.size() != 2 || b.size() != 2) throw std::invalid_argument("only size-2 ciphertexts can be multiplied");
-        const uint32_t L = a.parmsID().level;
+        const uint32_t L = level(a);
         Ciphertext out;
         out.resize(ctx_, a.parmsID(), 3);
         out.isNttForm() = a.isNttForm();
         if (ctx_.scheme() == SchemeType::ckks) {
             if (!a.isNttForm() || !b.isNttForm()) throw std::invalid_argument("encrypted must be in NTT form");
             out.scale() = a.scale() * b.scale();
-            checkScale(out.scale(), a.parmsID());
+            checkScale(out.scale(), L);
             check(shc_ckks_multiply(ctx_.handle(), a.data(), b.data(), out.data(), 1, L, nullptr));
         } else {
             if (a.isNttForm() || b.isNttForm()) throw std::invalid_argument("encrypted cannot be in NTT form");
@@ -1056,10 +1584,10 @@ public:
         if (rk.parmsID() != ctx_.keyParmsID()) throw std::invalid_argument("relin_keys is not valid for encryption parameters");
         if (a.size() == 2) return;
         if (a.size() != 3) throw std::invalid_argument("destination_size must be at least 2");
-        const uint32_t L = a.parmsID().level;
+        const uint32_t L = level(a);
         size_t ws = 0;
         check(shc_keyswitch_workspace_size(ctx_.handle(), 1, L, &ws));
-        check(shc_relinearize(ctx_.handle(), a.data(), rk.data().get(), 1, L, ctx_.workspace(ws), ws, nullptr));
+        check(shc_relinearize(ctx_.handle(), a.data(), rk.key(2).get(), 1, L, ctx_.workspace(ws), ws, nullptr));
         a.shrink(2);
     }
     void relinearize(const Ciphertext &a, const RelinKeys &rk, Ciphertext &out) const {
@@ -1070,8 +1598,9 @@ public:
     // ---- modulus switching
     void rescaleToNext(const Ciphertext &a, Ciphertext &out) const {
         if (ctx_.scheme() != SchemeType::ckks) throw std::invalid_argument("unsupported operation for scheme type");
+        const double scale = a.scale() / (double)ctx_.prime(level(a) - 1);
         divideRound(a, out);
-        out.scale() = a.scale() / (double)ctx_.prime(a.parmsID().level - 1);
+        out.scale() = scale;
     }
     void rescaleToNextInplace(Ciphertext &a) const {
         Ciphertext out;
@@ -1079,23 +1608,27 @@ public:
         a = std::move(out);
     }
     void rescaleToInplace(Ciphertext &a, ParmsID id) const {
-        while (a.parmsID().level > id.level) rescaleToNextInplace(a);
+        const uint32_t target = targetLevel(a, id);
+        while (level(a) > target) rescaleToNextInplace(a);
     }
     void modSwitchToNext(const Ciphertext &a, Ciphertext &out) const {
         if (ctx_.scheme() == SchemeType::bfv) {
+            const double scale = a.scale();
             divideRound(a, out);
-            out.scale() = a.scale();
+            out.scale() = scale;
             return;
         }
         // CKKS: drop the last limb of every polynomial (modSwitchDropToNext)
         const uint32_t L = nextLevel(a);
-        out.resize(ctx_, ParmsID{L}, a.size());
-        out.isNttForm() = a.isNttForm();
-        out.scale() = a.scale();
+        Ciphertext r;
+        r.resize(ctx_, ctx_.idOf(L), a.size());
+        r.isNttForm() = a.isNttForm();
+        r.scale() = a.scale();
         const size_t n = ctx_.polyModulusDegree();
-        hip_check(hipMemcpy2D(out.data(), L * n * 8, a.data(), (L + 1) * n * 8, L * n * 8, a.size(),
+        hip_check(hipMemcpy2D(r.data(), L * n * 8, a.data(), (L + 1) * n * 8, L * n * 8, a.size(),
                               hipMemcpyDeviceToDevice),
                   "modswitch copy");
+        out = std::move(r);
     }
     void modSwitchToNextInplace(Ciphertext &a) const {
         Ciphertext out;
@@ -1103,20 +1636,25 @@ public:
         a = std::move(out);
     }
     void modSwitchToInplace(Ciphertext &a, ParmsID id) const {
-        while (a.parmsID().level > id.level) modSwitchToNextInplace(a);
+        const uint32_t target = targetLevel(a, id);
+        while (level(a) > target) modSwitchToNextInplace(a);
     }
     // Plaintexts in NTT form (CKKS) drop their last limb.
     void modSwitchToNextInplace(Plaintext &p) const {
         if (!p.isNttForm()) throw std::invalid_argument("plain is not in NTT form");
-        if (p.parmsID().level < 2) throw std::invalid_argument("end of modulus switching chain reached");
-        const uint32_t L = p.parmsID().level - 1;
+        const uint32_t L0 = ctx_.levelOrThrow(p.parmsID(), "plain");
+        if (L0 < 2 || L0 == ctx_.keyCount()) throw std::invalid_argument("end of modulus switching chain reached");
+        const uint32_t L = L0 - 1;
         DeviceArray<uint64_t> d(L * ctx_.polyModulusDegree());
-        hip_check(hipMemcpy(d.get(), p.data().get(), d.size() * 8, hipMemcpyDeviceToDevice), "copy");
-        p.data() = std::move(d);
-        p.parmsID() = ParmsID{L};
+        hip_check(hipMemcpy(d.get(), p.data(), d.size() * 8, hipMemcpyDeviceToDevice), "copy");
+        p.dynArray() = std::move(d);
+        p.parmsID() = ctx_.idOf(L);
     }
     void modSwitchToInplace(Plaintext &p, ParmsID id) const {
-        while (p.parmsID().level > id.level) modSwitchToNextInplace(p);
+        const uint32_t target = ctx_.levelOrThrow(id, "parms_id");
+        if (ctx_.levelOrThrow(p.parmsID(), "plain") < target)
+            throw std::invalid_argument("cannot switch to higher level modulus");
+        while (ctx_.levelOf(p.parmsID()) > target) modSwitchToNextInplace(p);
     }
 
     // ---- plaintext operations
@@ -1131,21 +1669,22 @@ public:
         subPlainInplace(out, p);
     }
     void multiplyPlainInplace(Ciphertext &a, const Plaintext &p) const {
-        const uint32_t L = a.parmsID().level;
+        const uint32_t L = level(a);
         if (a.isNttForm() != p.isNttForm()) throw std::invalid_argument("NTT form mismatch");
         if (a.isNttForm()) {
             if (a.parmsID() != p.parmsID()) throw std::invalid_argument("encrypted_ntt and plain_ntt parameter mismatch");
-            check(shc_multiply_plain_ntt(ctx_.handle(), a.data(), p.data().get(), a.data(), (uint32_t)a.size(), L,
-                                         nullptr));
+            check(shc_multiply_plain_ntt(ctx_.handle(), a.data(), p.data(), a.data(), (uint32_t)a.size(), L, nullptr));
         } else {
+            DeviceArray<uint64_t> tmp;
+            const uint64_t *pp = detail::padded_plain(ctx_, p, tmp);
             size_t ws = 0;
             check(shc_bfv_multiply_plain_workspace_size(ctx_.handle(), L, &ws));
-            check(shc_bfv_multiply_plain(ctx_.handle(), a.data(), p.data().get(), a.data(), (uint32_t)a.size(), L,
+            check(shc_bfv_multiply_plain(ctx_.handle(), a.data(), pp, a.data(), (uint32_t)a.size(), L,
                                          ctx_.workspace(ws), ws, nullptr));
         }
         if (ctx_.scheme() == SchemeType::ckks) {
             a.scale() *= p.scale();
-            checkScale(a.scale(), a.parmsID());
+            checkScale(a.scale(), L);
         }
     }
     void multiplyPlain(const Ciphertext &a, const Plaintext &p, Ciphertext &out) const {
@@ -1156,7 +1695,7 @@ public:
     // ---- NTT form
     void transformToNttInplace(Ciphertext &a) const {
         if (a.isNttForm()) throw std::invalid_argument("encrypted is already in NTT form");
-        check(shc_ntt_forward(ctx_.handle(), a.data(), (uint32_t)a.size(), a.parmsID().level, nullptr));
+        check(shc_ntt_forward(ctx_.handle(), a.data(), (uint32_t)a.size(), level(a), nullptr));
         a.isNttForm() = true;
     }
     void transformToNtt(const Ciphertext &a, Ciphertext &out) const {
@@ -1165,7 +1704,7 @@ public:
     }
     void transformFromNttInplace(Ciphertext &a) const {
         if (!a.isNttForm()) throw std::invalid_argument("encrypted is not in NTT form");
-        check(shc_ntt_inverse(ctx_.handle(), a.data(), (uint32_t)a.size(), a.parmsID().level, nullptr));
+        check(shc_ntt_inverse(ctx_.handle(), a.data(), (uint32_t)a.size(), level(a), nullptr));
         a.isNttForm() = false;
     }
     void transformFromNtt(const Ciphertext &a, Ciphertext &out) const {
@@ -1175,9 +1714,12 @@ public:
     // BFV plaintext -> lifted NTT form at level `id`.
     void transformToNttInplace(Plaintext &p, ParmsID id) const {
         if (p.isNttForm()) throw std::invalid_argument("plain is already in NTT form");
-        DeviceArray<uint64_t> d(id.level * ctx_.polyModulusDegree());
-        check(shc_bfv_plain_to_ntt(ctx_.handle(), p.data().get(), id.level, d.get(), nullptr));
-        p.data() = std::move(d);
+        const uint32_t L = ctx_.levelOrThrow(id, "parms_id");
+        DeviceArray<uint64_t> tmp;
+        const uint64_t *pp = detail::padded_plain(ctx_, p, tmp);
+        DeviceArray<uint64_t> d(L * ctx_.polyModulusDegree());
+        check(shc_bfv_plain_to_ntt(ctx_.handle(), pp, L, d.get(), nullptr));
+        p.dynArray() = std::move(d);
         p.parmsID() = id;
     }
 
@@ -1185,7 +1727,7 @@ public:
     void applyGaloisInplace(Ciphertext &a, uint32_t elt, const GaloisKeys &gk) const {
         if (gk.parmsID() != ctx_.keyParmsID()) throw std::invalid_argument("galois_keys is not valid for encryption parameters");
         if (a.size() != 2) throw std::invalid_argument("encrypted size must be 2");
-        const uint32_t L = a.parmsID().level;
+        const uint32_t L = level(a);
         size_t ws = 0;
         check(shc_galois_workspace_size(ctx_.handle(), 1, L, &ws));
         check(shc_apply_galois_keyswitch(ctx_.handle(), a.data(), gk.key(elt).get(), elt, 1, L, ctx_.workspace(ws), ws,
@@ -1229,12 +1771,18 @@ public:
     }
 
 private:
+    uint32_t level(const Ciphertext &a) const { return ctx_.levelOrThrow(a.parmsID(), "encrypted"); }
+    uint32_t targetLevel(const Ciphertext &a, ParmsID id) const {
+        const uint32_t target = ctx_.levelOrThrow(id, "parms_id");
+        if (level(a) < target) throw std::invalid_argument("cannot switch to higher level modulus");
+        return target;
+    }
     void sameParms(const Ciphertext &a, const Ciphertext &b) const {
         if (a.parmsID() != b.parmsID()) throw std::invalid_argument("encrypted1 and encrypted2 parameter mismatch");
         if (a.isNttForm() != b.isNttForm()) throw std::invalid_argument("NTT form mismatch");
     }
-    void checkScale(double scale, ParmsID id) const {
-        if (!(scale > 0) || (int)std::log2(scale) >= ctx_.levelBits(id)) throw std::invalid_argument("scale out of bounds");
+    void checkScale(double scale, uint32_t L) const {
+        if (!(scale > 0) || (int)std::log2(scale) >= ctx_.levelBits(L)) throw std::invalid_argument("scale out of bounds");
     }
     static bool scalesClose(double a, double b) {
         const double d = std::fabs(a - b);
@@ -1244,7 +1792,7 @@ private:
         sameParms(a, b);
         if (ctx_.scheme() == SchemeType::ckks && !scalesClose(a.scale(), b.scale()))
             throw std::invalid_argument("scale mismatch");
-        const uint32_t L = a.parmsID().level;
+        const uint32_t L = level(a);
         const size_t common = std::min(a.size(), b.size());
         if (b.size() > a.size()) {
             Ciphertext grown;
@@ -1265,33 +1813,35 @@ private:
         else check(shc_add(ctx_.handle(), a.data(), b.data(), a.data(), (uint32_t)common, L, nullptr));
     }
     void addSubPlain(Ciphertext &a, const Plaintext &p, bool sub) const {
-        const uint32_t L = a.parmsID().level;
+        const uint32_t L = level(a);
         if (ctx_.scheme() == SchemeType::ckks) {
             if (!p.isNttForm() || !a.isNttForm() || p.parmsID() != a.parmsID())
                 throw std::invalid_argument("encrypted and plain parameter mismatch");
             if (!scalesClose(a.scale(), p.scale())) throw std::invalid_argument("scale mismatch");
-            check(shc_add_plain(ctx_.handle(), a.data(), p.data().get(), a.data(), 1, (uint32_t)a.size(), L, sub,
-                                nullptr));
+            check(shc_add_plain(ctx_.handle(), a.data(), p.data(), a.data(), 1, (uint32_t)a.size(), L, sub, nullptr));
         } else {
             if (p.isNttForm() || a.isNttForm()) throw std::invalid_argument("NTT form mismatch");
-            check(shc_bfv_add_plain(ctx_.handle(), a.data(), p.data().get(), a.data(), 1, (uint32_t)a.size(), L, sub,
-                                    nullptr));
+            DeviceArray<uint64_t> tmp;
+            const uint64_t *pp = detail::padded_plain(ctx_, p, tmp);
+            check(shc_bfv_add_plain(ctx_.handle(), a.data(), pp, a.data(), 1, (uint32_t)a.size(), L, sub, nullptr));
         }
     }
     uint32_t nextLevel(const Ciphertext &a) const {
         auto cd = ctx_.getContextData(a.parmsID());
         if (!cd) throw std::invalid_argument("encrypted is not valid for encryption parameters");
         if (cd->nextParmsID() == parmsIDZero) throw std::invalid_argument("end of modulus switching chain reached");
-        return a.parmsID().level - 1;
+        return level(a) - 1;
     }
     void divideRound(const Ciphertext &a, Ciphertext &out) const {
         const uint32_t L = nextLevel(a) + 1;
-        out.resize(ctx_, ParmsID{L - 1}, a.size());
-        out.isNttForm() = a.isNttForm();
+        Ciphertext r;   // out may alias a
+        r.resize(ctx_, ctx_.idOf(L - 1), a.size());
+        r.isNttForm() = a.isNttForm();
         size_t ws = 0;
         check(shc_rescale_workspace_size(ctx_.handle(), 1, (uint32_t)a.size(), L, &ws));
-        check(shc_rescale_to_next(ctx_.handle(), a.data(), out.data(), 1, (uint32_t)a.size(), L, ctx_.workspace(ws),
+        check(shc_rescale_to_next(ctx_.handle(), a.data(), r.data(), 1, (uint32_t)a.size(), L, ctx_.workspace(ws),
                                   ws, nullptr));
+        out = std::move(r);
     }
     // EvaluatorCuda::rotateInternal (src/evaluator_cuda.cu:2106-2163): the key of the
     // step itself, else the NAF terms one by one.
@@ -1322,31 +1872,30 @@ private:
 inline void Ciphertext::saveTerms(std::ostream &os, const Evaluator &ev, const std::vector<size_t> &term_ids) const {
     Ciphertext coeff = *this;
     if (coeff.isNttForm()) ev.transformFromNttInplace(coeff);
-    const std::vector<uint64_t> h = coeff.toHost();
+    const std::vector<uint64_t> h = coeff.dynArray().toHost();
     saveHeader(os, true);
-    const size_t L = coeffModulusSize();
     for (size_t id : term_ids) {
         if (id >= n_) throw std::invalid_argument("term index out of range");
-        for (size_t j = 0; j < L; j++) detail::put(os, h[j * n_ + id]);
+        for (size_t j = 0; j < L_; j++) detail::put(os, h[j * n_ + id]);
     }
     const uint64_t rest = (uint64_t)((size_ - 1) * polyWords());
     detail::put(os, rest);
-    os.write(reinterpret_cast<const char *>(h.data() + polyWords()), rest * 8);
+    os.write(reinterpret_cast<const char *>(h.data() + polyWords()), (std::streamsize)(rest * 8));
 }
 
 inline void Ciphertext::loadTerms(std::istream &is, const Evaluator &ev, const SEALContext &ctx,
                                   const std::vector<size_t> &term_ids) {
     loadHeader(is, ctx, true);
-    const size_t L = coeffModulusSize();
+    const char *what = "stream ended before the ciphertext did";
     std::vector<uint64_t> h(size_ * polyWords(), 0);
     for (size_t id : term_ids) {
         if (id >= n_) throw std::invalid_argument("term index out of range");
-        for (size_t j = 0; j < L; j++) h[j * n_ + id] = detail::get<uint64_t>(is);
+        for (size_t j = 0; j < L_; j++) h[j * n_ + id] = detail::get<uint64_t>(is, what);
     }
-    const uint64_t rest = detail::get<uint64_t>(is);
+    const uint64_t rest = detail::get<uint64_t>(is, what);
     if (rest != (size_ - 1) * polyWords()) throw std::invalid_argument("ciphertext word count does not match its shape");
-    is.read(reinterpret_cast<char *>(h.data() + polyWords()), rest * 8);
-    if (!is) throw std::invalid_argument("stream ended before the ciphertext did");
+    is.read(reinterpret_cast<char *>(h.data() + polyWords()), (std::streamsize)(rest * 8));
+    if (!is) throw std::invalid_argument(what);
     data_.fromHost(h.data(), h.size());
     if (ntt_form_) {
         ntt_form_ = false;

@@ -10,6 +10,7 @@
 #include "utils/numth.h"
 #include "utils/galois.h"
 #include "utils/rns.h"
+#include <cstdio>
 #include <cstring>
 #include <vector>
 #include <memory>
@@ -351,6 +352,23 @@ int ref_plain_op(void *h, uint64_t L, uint64_t size, int op, const uint64_t *ct,
         std::memcpy(out, x.data(), size * L * s.n * 8);
         return 0;
     } catch (const std::exception &e) { fprintf(stderr, "ref_plain_op: %s\n", e.what()); return -1; }
+}
+
+// Plaintext(const std::string &) and Plaintext::to_string (src/plaintext.cpp:91-200,
+// src/plaintext.h:491-498): parses `hex`, writes its coefficients (at most *count) and
+// their number to *count, and its to_string() into str (at most str_cap bytes, NUL
+// terminated).  Returns -1 when the reference rejects the string (std::invalid_argument).
+int ref_plaintext_hex(const char *hex, uint64_t *coeffs, uint64_t *count, char *str, uint64_t str_cap) {
+    try {
+        Plaintext p(hex);
+        const uint64_t cap = *count;
+        *count = p.coeffCount();
+        for (uint64_t i = 0; i < *count && i < cap; i++) coeffs[i] = p[i];
+        const std::string s = p.to_string();
+        std::snprintf(str, str_cap, "%s", s.c_str());
+        return 0;
+    } catch (const std::invalid_argument &) { return -1; }
+    catch (...) { return -2; }
 }
 
 // EncryptionParameters::parmsID (src/encryptionparams.cpp:118-156) of the chain level

@@ -1,0 +1,246 @@
+// The reference's own C++ tests, compiled against this build through the troyn:: alias.
+//
+// A program written for the reference includes troy_cuda.cuh and uses namespace troyn;
+// here the same code includes troyhip.hpp and aliases troyn to troyhip.  The cases
+// follow test/evaluator_cuda.cu:2459-2534 (BFVRelinearize: decrypt KATs of hexadecimal
+// plaintexts through square, relinearize and modulus switching at N = 128, t = 64),
+// plus the reference containers' behaviour: hexadecimal plaintext strings
+// (src/plaintext.h:99-130, 470-498), PlaintextCuda::save / load
+// (src/plaintext_cuda.cu:6-27), parms_id hash blocks (src/encryptionparams.h:36),
+// cpu() / conversion constructors (src/ciphertext_cuda.cuh:20-49,
+// src/plaintext_cuda.cuh:27-42), KSwitchKeys (src/kswitchkeys_cuda.cuh:31-170), fresh
+// key material per KeyGenerator, and a memory pool that serves steady-state operations
+// without new device allocations.  Prints "ok <name>" per check, "troyn kat ok" at the
+// end; exits 1 on the first failure.
+#include "troyhip.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <sstream>
+
+namespace troyn = troyhip;
+using namespace troyn;
+
+#define EXPECT(cond, name)                                               \
+    do {                                                                 \
+        if (cond) {                                                      \
+            std::printf("ok %s\n", name);                                \
+        } else {                                                         \
+            std::printf("FAIL %s (%s:%d)\n", name, __FILE__, __LINE__); \
+            std::fflush(stdout);                                         \
+            std::exit(1);                                                \
+        }                                                                \
+    } while (0)
+
+template <class E, class F>
+static bool throws(F f) {
+    try {
+        f();
+    } catch (const E &) {
+        return true;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+// EvaluatorCudaTest.BFVRelinearize (test/evaluator_cuda.cu:2459-2534), statement for
+// statement.
+static void bfv_relinearize() {
+    EncryptionParameters parms(SchemeType::bfv);
+    Modulus plain_modulus(1 << 6);
+    parms.setPolyModulusDegree(128);
+    parms.setPlainModulus(plain_modulus);
+    parms.setCoeffModulus(CoeffModulus::Create(128, {40, 40, 40, 40}));
+
+    SEALContext context(parms, true, SecurityLevel::none);
+    KeyGenerator keygen(context);
+    PublicKey pk;
+    keygen.createPublicKey(pk);
+    RelinKeys rlk;
+    keygen.createRelinKeys(rlk);
+
+    Encryptor encryptor(context, pk);
+    Evaluator evaluator(context);
+    Decryptor decryptor(context, keygen.secretKey());
+
+    Ciphertext encrypted(context);
+    Ciphertext encrypted2(context);
+
+    Plaintext plain;
+    Plaintext plain2;
+
+    plain = 0;
+    encryptor.encrypt(plain, encrypted);
+    evaluator.squareInplace(encrypted);
+    evaluator.relinearizeInplace(encrypted, rlk);
+    decryptor.decrypt(encrypted, plain2);
+    EXPECT(plain == plain2, "BFVRelinearize: 0 squared");
+
+    encryptor.encrypt(plain, encrypted);
+    evaluator.squareInplace(encrypted);
+    evaluator.relinearizeInplace(encrypted, rlk);
+    evaluator.squareInplace(encrypted);
+    evaluator.relinearizeInplace(encrypted, rlk);
+    decryptor.decrypt(encrypted, plain2);
+    EXPECT(plain == plain2, "BFVRelinearize: 0 to the fourth");
+
+    plain = "1x^10 + 2";
+    encryptor.encrypt(plain, encrypted);
+    evaluator.squareInplace(encrypted);
+    evaluator.relinearizeInplace(encrypted, rlk);
+    decryptor.decrypt(encrypted, plain2);
+    EXPECT(plain2.to_string() == "1x^20 + 4x^10 + 4", "BFVRelinearize: (1x^10 + 2)^2");
+
+    encryptor.encrypt(plain, encrypted);
+    evaluator.squareInplace(encrypted);
+    evaluator.relinearizeInplace(encrypted, rlk);
+    evaluator.squareInplace(encrypted);
+    evaluator.relinearizeInplace(encrypted, rlk);
+    decryptor.decrypt(encrypted, plain2);
+    EXPECT(plain2.to_string() == "1x^40 + 8x^30 + 18x^20 + 20x^10 + 10", "BFVRelinearize: (1x^10 + 2)^4");
+
+    // Relinearization with modulus switching
+    plain = "1x^10 + 2";
+    encryptor.encrypt(plain, encrypted);
+    evaluator.squareInplace(encrypted);
+    evaluator.relinearizeInplace(encrypted, rlk);
+    evaluator.modSwitchToNextInplace(encrypted);
+    decryptor.decrypt(encrypted, plain2);
+    EXPECT(plain2.to_string() == "1x^20 + 4x^10 + 4", "BFVRelinearize: square, modswitch");
+
+    encryptor.encrypt(plain, encrypted);
+    evaluator.squareInplace(encrypted);
+    evaluator.relinearizeInplace(encrypted, rlk);
+    evaluator.modSwitchToNextInplace(encrypted);
+    evaluator.squareInplace(encrypted);
+    evaluator.relinearizeInplace(encrypted, rlk);
+    evaluator.modSwitchToNextInplace(encrypted);
+    decryptor.decrypt(encrypted, plain2);
+    EXPECT(plain2.to_string() == "1x^40 + 8x^30 + 18x^20 + 20x^10 + 10", "BFVRelinearize: twice with modswitch");
+}
+
+static void plaintext_strings_and_files() {
+    Plaintext p("7FFx^3 + 1x^1 + 3");
+    EXPECT(p.coeffCount() == 4 && p[3] == 0x7FF && p[2] == 0 && p[1] == 1 && p[0] == 3, "hex plaintext parse");
+    EXPECT(p.to_string() == "7FFx^3 + 1x^1 + 3", "hex plaintext round trip");
+    Plaintext lower("abcx^2 + f");
+    EXPECT(lower.to_string() == "ABCx^2 + F", "lower-case hex, upper-case output");
+    Plaintext zero("0");
+    EXPECT(zero.to_string() == "0" && zero.isZero(), "zero polynomial");
+    EXPECT(Plaintext("0x^5 + 1").to_string() == "1", "zero terms are omitted");
+    for (const char *bad : {"1x^2 + 1x^3", "1x^1 +1", "x^2", "1x2", "1 + 1", "-1",
+                            "10000000000000000x^1"})
+        EXPECT(throws<std::invalid_argument>([&] { Plaintext q(bad); }), bad);
+    Plaintext c;
+    c = 0x1234;
+    EXPECT(c.coeffCount() == 1 && c.to_string() == "1234", "constant assignment");
+    // the same polynomial with extra zero coefficients is equal
+    Plaintext padded("2x^1 + 1");
+    padded.resize(100);
+    EXPECT(padded == Plaintext("2x^1 + 1"), "equality ignores leading zero coefficients");
+    EXPECT(!(padded == Plaintext("2x^1 + 2")), "equality sees a different coefficient");
+    // PlaintextCuda::save / load layout: parms_id, coeff count, scale, word count, words
+    std::stringstream ss;
+    p.save(ss);
+    const std::string bytes = ss.str();
+    uint64_t count = 0, words = 0;
+    double scale = 0;
+    std::memcpy(&count, bytes.data() + 32, 8);
+    std::memcpy(&scale, bytes.data() + 40, 8);
+    std::memcpy(&words, bytes.data() + 48, 8);
+    EXPECT(bytes.size() == 56 + 4 * 8 && count == 4 && words == 4 && scale == 1.0, "plaintext save layout");
+    Plaintext back;
+    back.load(ss);
+    EXPECT(back == p && back.to_string() == p.to_string(), "plaintext save/load round trip");
+    std::stringstream cut(bytes.substr(0, bytes.size() - 3));
+    EXPECT(throws<std::invalid_argument>([&] { Plaintext q; q.load(cut); }), "truncated plaintext throws");
+}
+
+static void containers_and_keys() {
+    const size_t n = 4096;
+    EncryptionParameters parms(SchemeType::ckks);
+    parms.setPolyModulusDegree(n);
+    parms.setCoeffModulus(CoeffModulus::Create(n, {60, 40, 40, 60}));
+    SEALContext context(parms);
+    // parms_id is the 4-word hash block of each level, distinct per level
+    const ParmsID key = context.keyParmsID(), first = context.firstParmsID(), last = context.lastParmsID();
+    EXPECT(key != first && first != last && key != parmsIDZero && first != parmsIDZero, "parms ids are hash blocks");
+    EXPECT(context.getContextData(first)->nextParmsID() == context.idOf(2) &&
+               context.getContextData(last)->nextParmsID() == parmsIDZero,
+           "chain links by parms_id");
+    ParmsID bogus = first;
+    bogus[0] ^= 1;
+    EXPECT(!context.getContextData(bogus), "unknown parms_id has no context data");
+
+    KeyGenerator keygen(context), keygen2(context);
+    EXPECT(keygen.secretKey().data().toHost().data() != keygen2.secretKey().data().toHost().data(),
+           "two key generators draw different secret keys");
+    RelinKeys rlk = keygen.createRelinKeys();
+    KSwitchKeys &as_base = rlk;
+    EXPECT(rlk.size() == 1 && as_base.size() == 1 && rlk.hasKey(2) && !rlk.hasKey(3) && rlk.parmsID() == key,
+           "relin keys as KSwitchKeys");
+    GaloisKeys gk = keygen.createGaloisKeys(std::vector<int>{1, -1});
+    const uint32_t e1 = 3;
+    EXPECT(gk.size() == 2 && gk.hasKey(e1) && GaloisKeys::getIndex(e1) == 1 && !gk.hasKey(5), "galois keys index");
+    EXPECT(throws<std::invalid_argument>([&] { (void)gk.data(2); }), "absent key throws");
+
+    CKKSEncoder encoder(context);
+    Encryptor encryptor(context, keygen.createPublicKey());
+    Decryptor decryptor(context, keygen.secretKey());
+    Evaluator evaluator(context);
+    std::vector<std::complex<double>> v(encoder.slotCount());
+    for (size_t i = 0; i < v.size(); i++) v[i] = {std::sin(0.1 * i), std::cos(0.3 * i)};
+    Plaintext p;
+    encoder.encode(v, std::pow(2.0, 40), p);
+    Ciphertext ct = encryptor.encrypt(p);
+    // cpu() and the conversion constructor carry every field and word
+    troyhip::host::Ciphertext h = ct.cpu();
+    Ciphertext again(h);
+    EXPECT(h.size() == 2 && h.coeffModulusSize() == 3 && h.isNttForm() && h.parmsID() == first &&
+               again.toHost() == h && again.parmsID() == ct.parmsID() && again.scale() == ct.scale(),
+           "ciphertext cpu() round trip");
+    troyhip::host::Plaintext hp = p.cpu();
+    Plaintext p2(hp);
+    EXPECT(p2 == p && p2.isNttForm() && p2.parmsID() == first, "plaintext cpu() round trip");
+    // a ciphertext loaded from a stream names its level by the hash block
+    std::stringstream ss;
+    ct.save(ss);
+    Ciphertext loaded;
+    loaded.load(ss, context);
+    EXPECT(loaded.parmsID() == first && loaded.coeffModulusSize() == 3, "loaded ciphertext level");
+
+    // steady state: a mul + relin + rescale loop reuses pooled blocks
+    RelinKeys rk = keygen.createRelinKeys();
+    auto step = [&] {
+        Ciphertext a = ct;
+        evaluator.multiplyInplace(a, ct);
+        evaluator.relinearizeInplace(a, rk);
+        evaluator.rescaleToNextInplace(a);
+        return a;
+    };
+    step();
+    step();
+    const size_t allocs = MemoryPool::deviceAllocations();
+    Ciphertext r;
+    for (int i = 0; i < 10; i++) r = step();
+    EXPECT(MemoryPool::deviceAllocations() == allocs, "pooled steady state: no device allocation");
+    Plaintext out;
+    decryptor.decrypt(r, out);
+    std::vector<std::complex<double>> got;
+    encoder.decode(out, got);
+    double err = 0;
+    for (size_t i = 0; i < v.size(); i++) err = std::max(err, std::abs(got[i] - v[i] * v[i]));
+    EXPECT(err < 1e-4, "pooled result decrypts to the square");
+    const size_t cached = MemoryPool::cachedBytes();
+    EXPECT(MemoryPool::release() == cached && MemoryPool::cachedBytes() == 0, "pool release");
+}
+
+int main() {
+    KernelProvider::initialize();
+    bfv_relinearize();
+    plaintext_strings_and_files();
+    containers_and_keys();
+    std::printf("troyn kat ok\n");
+    return 0;
+}

@@ -48,3 +48,50 @@ def test_parms_id_matches_reference(exe, scheme, logn, bits, t):
         args = [exe, "parms", str(scheme), str(n), str(t)] + [str(q) for q in moduli[:L]]
         got = [int(x) for x in subprocess.run(args, capture_output=True, text=True, check=True).stdout.split()]
         assert got == sess.parms_id(L), f"level with {L} primes"
+
+
+HEX_CASES = ["7FFx^3 + 1x^1 + 3", "1x^10 + 2", "abcx^2 + f", "0", "0x^5 + 1", "1x^40 + 8x^30 + 18x^20 + 20x^10 + 10",
+             "FFFFFFFFFFFFFFFFx^1", "10000000000000000x^1", "000000000000000001x^2", "1x^2 + 1x^3", "1x^1 +1",
+             "x^2", "1x^2 + ", "1x2", "1 + 1", "1x^ + 1", "-1", "1x^1", "1x^0", "", "1x^2 + 0x^1 + 0",
+             "3x^2147483646", "1 ", " 1"]
+
+
+def hexpoly(exe, s):
+    out = subprocess.run([exe, "hexpoly", s], capture_output=True, text=True, check=True).stdout.split("\n")
+    if out[0] == "invalid":
+        return None
+    k = int(out[0])
+    return [int(x) for x in out[1:1 + k]], out[1 + k]
+
+
+@pytest.mark.parametrize("s", ["7FFx^3 + 1x^1 + 3", "abcx^2 + f", "1x^10 + 2"])
+def test_hex_plaintext_format(exe, s):
+    """Hexadecimal plaintext strings: coefficients by power, upper-case output."""
+    coeffs, text = hexpoly(exe, s)
+    want = {"7FFx^3 + 1x^1 + 3": [3, 1, 0, 0x7FF], "abcx^2 + f": [15, 0, 0xABC],
+            "1x^10 + 2": [2] + [0] * 9 + [1]}[s]
+    assert coeffs == want and text == s.upper().replace("X", "x")
+
+
+@pytest.mark.skipif(ck.ref() is None, reason="reference CPU library oracle/_ref not built here")
+@pytest.mark.ref
+@pytest.mark.parametrize("s", [c for c in HEX_CASES if "2147483646" not in c])
+def test_hex_plaintext_matches_reference(exe, s):
+    """Plaintext(hex) and to_string() against the reference's own parser (src/plaintext.cpp:91-200):
+    the same coefficients and string for valid input, a rejection for invalid input."""
+    import ctypes
+    import numpy as np
+    cap = 64
+    coeffs = np.zeros(cap, dtype=np.uint64)
+    count = ctypes.c_uint64(cap)
+    buf = ctypes.create_string_buffer(4096)
+    rc = ck.ref().ref_plaintext_hex(s.encode(), ck.ptr(coeffs), ctypes.byref(count), buf, 4096)
+    got = hexpoly(exe, s)
+    if rc == -1:
+        assert got is None, f"{s!r}: the reference rejects it"
+        return
+    assert rc == 0 and got is not None, f"{s!r}: the reference accepts it"
+    want = [int(x) for x in coeffs[:count.value]]
+    # an all-zero string leaves the reference plaintext at its previous (empty) size
+    assert got[0] == want, s
+    assert got[1] == buf.value.decode(), s
