@@ -171,50 +171,50 @@ def cpu_baseline_mulrelin(moduli, logn, budget_s):
                         f"CKKS mul+relin at N=2^{logn}, L={L} (+1 special), injected uniform words")
 
 
-def served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, requests, max_batch, seed=7):
-    """Open-loop serving: `requests` mul+relin requests arrive as a Poisson process at
-    `rate` per second; each request's ciphertext pair sits in a ring of B resident
-    slots.  The server takes every request that has arrived (at most max_batch), runs
-    them as one batched multiply + relinearize and waits for it; a request's latency is
-    its batch's completion time minus its arrival time.  max_batch = 1 is the plain
-    per-ciphertext path under the same arrivals."""
+def served_mulrelin(torch, sealhip, ctx, a, b, key, L, n, seconds):
+    """The served leg of the headline: multiply + relinearize requests through the
+    library's continuous-batching server, each request a pair of host ciphertexts
+    (N=2^16, L=24: 50 MB in, 25 MB out, over PCIe inside the latency).  Capacity is
+    measured with a burst; Poisson arrivals at half of it then go to the batched server
+    (2 lanes, batches of at most 32, a 32 GiB device arena) and to the plain server (one
+    lane, one request at a time).  Self-check: a request's words served alone equal its
+    words in the burst and the resident batch's words for the same pair."""
     import numpy as np
-    rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=requests))
-    s2, s3 = 2 * L * n * 8, 3 * L * n * 8
-
-    def run(slot, k):
-        ctx.ckks_multiply(a.data_ptr() + slot * s2, b.data_ptr() + slot * s2, out.data_ptr() + slot * s3, k, L, st)
-        ctx.relinearize(out.data_ptr() + slot * s3, key.data_ptr(), k, L, ws.data_ptr(), ws_bytes, st)
-
-    lat = np.empty(requests)
-    batches = 0
-    done = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    while done < requests:
-        now = time.perf_counter() - t0
-        if arrivals[done] > now:
-            time.sleep(min(arrivals[done] - now, 1e-3))
-            continue
-        ready = int(np.searchsorted(arrivals, now, side="right")) - done
-        k = min(ready, max_batch)
-        slot = done % B
-        k1 = min(k, B - slot)
-        run(slot, k1)
-        if k > k1:
-            run(0, k - k1)
-        torch.cuda.synchronize()
-        fin = time.perf_counter() - t0
-        lat[done:done + k] = fin - arrivals[done:done + k]
-        done += k
-        batches += 1
-    span = time.perf_counter() - t0
-    return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-            "mean_batch": requests / batches, "served_per_s": requests / span}
+    pool = []
+    for i in range(4):
+        pair = torch.stack([a[i], b[i]]).cpu()
+        pool.append(pair.reshape(-1).numpy().view(np.uint64).copy())
+    ring, B = 24, 32
+    staging = ring * 3 * pool[0].nbytes // 2 + (64 << 20)
+    batched_sv = sealhip.Server(32 << 30, staging, lanes=2, max_batch=B)
+    plain_sv = sealhip.Server(2 << 30, staging, lanes=1, max_batch=1)
+    tb, tp = batched_sv.add_mulrelin(ctx, L, key.data_ptr()), plain_sv.add_mulrelin(ctx, L, key.data_ptr())
+    serve_open_loop(batched_sv, [(tb, pool)], 0, 8, seed=1, ring=ring, burst=True)
+    cap = serve_open_loop(batched_sv, [(tb, pool)], 0, 48, seed=2, ring=ring, burst=True)
+    words = batched_sv.request_size(tb)[1]
+    outs = [np.empty(words, dtype=np.uint64) for _ in range(8)]
+    ids = [batched_sv.submit(tb, pool[i % 4], outs[i]) for i in range(8)]
+    for i in ids:
+        batched_sv.wait(i)
+    alone = np.empty(words, dtype=np.uint64)
+    plain_sv.wait(plain_sv.submit(tp, pool[1], alone))
+    if not (np.array_equal(alone, outs[1]) and np.array_equal(alone, outs[5])):
+        raise SystemExit("bench self-check failed: a served mul+relin request differs alone and in a batch")
+    rate = 0.5 * cap["served_per_s"]
+    reqs = max(100, int(rate * seconds))
+    res = {"server": "library (shc_server_*): host ciphertexts in and out, 2 lanes, batches of at most "
+                     f"{B}, 32 GiB device arena; plain = 1 lane, one request at a time",
+           "capacity_per_s": cap["served_per_s"], "arrivals": "poisson", "rate_per_s": rate, "requests": reqs,
+           "batched": serve_open_loop(batched_sv, [(tb, pool)], rate, reqs, seed=7, ring=ring),
+           "plain": serve_open_loop(plain_sv, [(tp, pool)], rate, reqs, seed=7, ring=ring),
+           "max_batches_in_flight": batched_sv.stats()["max_batches_in_flight"]}
+    batched_sv.close()
+    plain_sv.close()
+    return res, alone
 
 
 def bench_mulrelin(args):
+    import numpy as np
     import torch
     import sealhip
     sealhip.init(0)
@@ -282,12 +282,9 @@ def bench_mulrelin(args):
 
     served = None
     if not args.no_serve:
-        # arrivals at half the batched throughput, the batched server against the plain one
-        rate = 0.5 * ops_s
-        reqs = max(200, int(rate * args.serve_seconds))
-        served = {"arrivals": "poisson", "rate_per_s": rate, "requests": reqs,
-                  "batched": served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, reqs, B),
-                  "plain": served_latency(torch, ctx, a, b, out, key, ws, ws_bytes, st, B, L, n, rate, reqs, 1)}
+        served, served_words = served_mulrelin(torch, sealhip, ctx, a, b, key, L, n, args.serve_seconds)
+        if not np.array_equal(served_words.reshape(2, L, n), out[1, :2].cpu().numpy().view(np.uint64)):
+            raise SystemExit("bench self-check failed: a served request differs from the resident batch's words")
 
     # The dominant kernel: the ModUp rows pass fused with the products by the key.  It
     # reads every ModUp limb once (B x L x L: each data prime's own digit is the NTT-form
@@ -859,19 +856,16 @@ def bench_matvec(args):
             "self_check": "batched product of request 0 == its product alone", "wall_s": wall}
 
 
-class MatvecServer:
-    """One request type of the serving workload: a context, one matrix's diagonals (NTT
-    form, synthetic residues), resident Galois keys (synthetic residues; the baby steps
-    also prepared for hoisting), a pool of resident input ciphertexts, and batch buffers.
-    Admission by HBM footprint: the batch buffers of m requests take m ciphertexts each
-    for the input, the output, the b baby steps and the g inner sums, plus the key-switch
-    workspace of m ciphertexts; max_batch is the largest m <= cap whose buffers fit in
-    `budget` bytes (keys and diagonals are resident and counted apart)."""
+class MatvecResident:
+    """One request type of the serving workload, its device-resident part: a context, one
+    matrix's diagonals (NTT form, synthetic residues), Galois keys (synthetic residues;
+    the baby steps also prepared for hoisting at the ciphertexts' level) and a host pool
+    of input ciphertexts.  Requests run through the library's server (shc_server_*)."""
 
-    def __init__(self, torch, sealhip, matvec, scheme, n, moduli, plain, b, g, pool, cap, budget, gen, st):
+    def __init__(self, torch, sealhip, scheme, n, moduli, plain, b, g, pool, gen, st):
         K = len(moduli)
         L = K - 1
-        self.torch, self.n, self.L, self.st = torch, n, L, st
+        self.n, self.L, self.b, self.g = n, L, b, g
         self.ctx = sealhip.Context(scheme, n, moduli, plain)
         ctx = self.ctx
         self.plains = torch.empty((g, b, L, n), dtype=torch.int64, device="cuda")
@@ -892,81 +886,76 @@ class MatvecServer:
             ctx.hoisted_key_prepare(k.data_ptr(), ctx.galois_elt_from_step(j), L, h.data_ptr(), hws.data_ptr(), hw,
                                     st)
             self.hoisted.append(h)
-        self.mv = matvec.BsgsMatvec(ctx, n, L, b, g, self.plains.data_ptr(), [k.data_ptr() for k in self.baby],
-                                    [h.data_ptr() for h in self.hoisted], [k.data_ptr() for k in self.giant],
-                                    bfv=scheme == sealhip.SCHEME_BFV)
-        self.pool = torch.empty((pool, 2, L, n), dtype=torch.int64, device="cuda")
-        fill_uniform(torch, self.pool, moduli[:L], 2, gen)
-        ct_bytes = 2 * L * n * 8
-        footprint = lambda m: m * ct_bytes * (2 + b + g) + self.mv.workspace_sizes(m)
-        max_batch = cap
-        while max_batch > 1 and footprint(max_batch) > budget:
-            max_batch -= 1
-        self.max_batch = max_batch
-        self.batch_bytes = footprint(max_batch)
-        self.per_request_bytes = footprint(2) - footprint(1)
+        x = torch.empty((pool, 2, L, n), dtype=torch.int64, device="cuda")
+        fill_uniform(torch, x, moduli[:L], 2, gen)
+        torch.cuda.synchronize()
+        self.inputs = [r.reshape(-1).numpy().view("uint64").copy() for r in x.cpu()]
         self.resident_bytes = (self.plains.numel() + sum(k.numel() for k in self.baby + self.giant)
                                + sum(h.numel() for h in self.hoisted)) * 8
-        self.x = torch.empty((max_batch, 2, L, n), dtype=torch.int64, device="cuda")
-        self.out = torch.empty_like(self.x)
-        self.babybuf = torch.empty((b, max_batch, 2, L, n), dtype=torch.int64, device="cuda")
-        self.inner = torch.empty((g, max_batch, 2, L, n), dtype=torch.int64, device="cuda")
-        self.wsz = self.mv.workspace_sizes(max_batch)
-        self.ws = torch.empty(((self.wsz + 7) // 8,), dtype=torch.int64, device="cuda")
 
-    def run(self, inputs, around_sums=None):
-        """One batch: the pool entries `inputs` gathered into the batch buffer, then one
-        batched matvec into self.out[:len(inputs)]."""
-        for k, i in enumerate(inputs):
-            self.x[k].copy_(self.pool[i])
-        self.mv.run(self.x.data_ptr(), len(inputs), self.out.data_ptr(), self.babybuf.data_ptr(),
-                    self.inner.data_ptr(), self.ws.data_ptr(), self.wsz, self.st, around_sums=around_sums)
+    def register(self, server):
+        return server.add_matvec(self.ctx, self.L, self.b, self.g, self.plains.data_ptr(),
+                                 [k.data_ptr() for k in self.baby], [h.data_ptr() for h in self.hoisted],
+                                 [k.data_ptr() for k in self.giant])
 
 
-def serve_mixed(torch, servers, rate, requests, max_batch, seed):
-    """Open-loop serving of a mixed stream: Poisson arrivals at `rate` per second, each
-    request of type 0 or 1 with probability 1/2 and an input drawn from that type's pool.
-    The server picks the type whose oldest waiting request arrived first, runs every
-    waiting request of that type (at most max_batch, and at most what that type's
-    admission allows) as one batch, and waits for it; a request's latency is its batch's
-    completion minus its arrival.  max_batch = 1 is the plain one-request-at-a-time path
-    under the same arrivals."""
+def serve_open_loop(server, kinds, rate, requests, seed, ring=64, burst=False):
+    """Open-loop serving through the library's server.  kinds: [(type id, [host input
+    arrays])]; request r picks a kind uniformly at random and an input from its list, and
+    arrives at the r-th time of a Poisson process of `rate` per second (burst: all at
+    once).  It is submitted at its arrival time (submit copies the input into the server's
+    pinned staging), and its latency runs from arrival to the server's completion, when
+    the result has been copied back into host memory -- upload, queueing, batching,
+    device work and download included.  At most `ring` requests are outstanding (their
+    output buffers are reused)."""
     import collections
     import numpy as np
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=requests))
-    types = (rng.random(requests) < 0.5).astype(int)
-    inputs = rng.integers(0, servers[0].pool.shape[0], size=requests)
-    pending = [collections.deque(), collections.deque()]
+    arrivals = np.zeros(requests) if burst else np.cumsum(rng.exponential(1.0 / rate, size=requests))
+    kind = rng.integers(0, len(kinds), size=requests)
+    pick = rng.integers(0, 1 << 30, size=requests)
+    outs = [[np.empty(server.request_size(t)[1], dtype=np.uint64) for _ in range(ring)] for t, _ in kinds]
     lat = np.empty(requests)
-    batches = [0, 0]
-    nxt = done = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    while done < requests:
-        now = time.perf_counter() - t0
-        while nxt < requests and arrivals[nxt] <= now:
-            pending[types[nxt]].append(nxt)
-            nxt += 1
-        if not pending[0] and not pending[1]:
-            time.sleep(min(arrivals[nxt] - now, 1e-3))
-            continue
-        ty = 0 if not pending[1] or (pending[0] and pending[0][0] < pending[1][0]) else 1
-        batch = [pending[ty].popleft() for _ in range(min(max_batch, servers[ty].max_batch, len(pending[ty])))]
-        servers[ty].run([int(inputs[r]) for r in batch])
-        torch.cuda.synchronize()
-        fin = time.perf_counter() - t0
-        lat[batch] = fin - arrivals[batch]
-        done += len(batch)
-        batches[ty] += 1
-    span = time.perf_counter() - t0
+    bsz = np.empty(requests)
+    done_at = np.empty(requests)
+    pending = collections.deque()
+
+    def reap(block):
+        while pending:
+            r, rid = pending[0]
+            info = server.wait(rid, -1.0 if block else 0.0)
+            if info is None:
+                return
+            pending.popleft()
+            lat[r] = info.done_s - (t0 + arrivals[r])
+            done_at[r] = info.done_s
+            bsz[r] = info.batch_size
+            if block:
+                return
+
+    t0 = server.clock() + 0.005
+    for r in range(requests):
+        while len(pending) >= ring:
+            reap(True)
+        target = t0 + arrivals[r]
+        while True:
+            now = server.clock()
+            if now >= target:
+                break
+            reap(False)
+            time.sleep(min(target - now, 5e-4))
+        t, pool = kinds[kind[r]]
+        pending.append((r, server.submit(t, pool[pick[r] % len(pool)], outs[kind[r]][r % ring])))
+    while pending:
+        reap(True)
+    span = float(done_at.max() - t0)
     res = {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-           "served_per_s": requests / span}
-    for ty, name in enumerate(("bfv", "ckks")):
-        sel = lat[types == ty]
-        res[name] = {"requests": int(sel.size), "p50_ms": float(np.percentile(sel, 50) * 1e3),
-                     "p99_ms": float(np.percentile(sel, 99) * 1e3),
-                     "mean_batch": float(sel.size / max(batches[ty], 1))}
+           "served_per_s": requests / span, "mean_batch": float(bsz.mean()), "requests": requests}
+    if len(kinds) > 1:
+        res["per_type"] = [{"requests": int((kind == k).sum()),
+                            "p50_ms": float(np.percentile(lat[kind == k], 50) * 1e3),
+                            "p99_ms": float(np.percentile(lat[kind == k], 99) * 1e3),
+                            "mean_batch": float(bsz[kind == k].mean())} for k in range(len(kinds))]
     return res
 
 
@@ -974,91 +963,103 @@ def bench_serve(args):
     """The serving workload: a Poisson stream of matrix-vector requests, half BFV (N=8192,
     BFVDefault(8192) primes, t = 65537, a 4096 x 4096 matrix mod t, BSGS 64 x 64 in each
     row of the slot matrix) and half CKKS (N=2^15, Create(32768, {60, 40 x 14, 60}), a
-    1024 x 1024 matrix, BSGS 32 x 32), keys and diagonals resident (synthetic residues).
-    Arrivals at half the batched server's capacity for the mix (measured first with full
-    batches); value = requests/s served by the batched server, with p50/p99 latency, and
-    the plain one-at-a-time server under the same arrivals beside it.  Self-check: for
-    each type, a request's words in a full batch equal its words computed alone."""
+    1024 x 1024 matrix, BSGS 32 x 32), keys and diagonals resident (synthetic residues),
+    requests arriving as host ciphertexts.  The library's continuous-batching server
+    (2 lanes, batches of at most 16, a 24 GiB device arena) against the plain server (one
+    lane, one request at a time) under the same arrivals, at half the batched server's
+    capacity for the mix (measured first with a burst); value = requests/s served by the
+    batched server, with p50/p99 latency from arrival to the result in host memory.
+    Self-check: for each type, a request's words in a burst equal its words served alone."""
+    import numpy as np
     import torch
     import sealhip
-    import matvec
     sealhip.init(0)
     st = torch.cuda.current_stream().cuda_stream
     gen = torch.Generator(device="cuda").manual_seed(8)
-    B = 16
-    budget = 8 << 30  # HBM for each request type's batch buffers
+    B, lanes, arena = 16, 2, 24 << 30
     bfv_moduli = [0x7fffffd8001, 0x7fffffc8001, 0xfffffffc001, 0xffffff6c001, 0xfffffebc001]
     ckks_moduli = sealhip.coeff_modulus_create(1 << 15, [60] + [40] * 14 + [60])
-    servers = [MatvecServer(torch, sealhip, matvec, sealhip.SCHEME_BFV, 8192, bfv_moduli, 65537, 64, 64, 32, B,
-                            budget, gen, st),
-               MatvecServer(torch, sealhip, matvec, sealhip.SCHEME_CKKS, 1 << 15, ckks_moduli, 0, 32, 32, 32, B,
-                            budget, gen, st)]
-    # capacity with full batches, per type
-    cap = []
-    for sv in servers:
-        full = list(range(sv.max_batch))
-        for _ in range(max(args.warmup, 1)):
-            sv.run(full)
-        with Timer(torch) as t:
-            for _ in range(args.steps):
-                sv.run(full)
-        cap.append(sv.max_batch * args.steps / (t.ms * 1e-3))
+    res = [MatvecResident(torch, sealhip, sealhip.SCHEME_BFV, 8192, bfv_moduli, 65537, 64, 64, 8, gen, st),
+           MatvecResident(torch, sealhip, sealhip.SCHEME_CKKS, 1 << 15, ckks_moduli, 0, 32, 32, 8, gen, st)]
+    ring = 64
+    staging = ring * max(2 * r.inputs[0].size * 8 for r in res) + (64 << 20)
+    batched_sv = sealhip.Server(arena, staging, lanes=lanes, max_batch=B)
+    plain_sv = sealhip.Server(arena // 4, staging, lanes=1, max_batch=1)
+    kinds_b = [(r.register(batched_sv), r.inputs) for r in res]
+    kinds_p = [(r.register(plain_sv), r.inputs) for r in res]
+    # warm up, then capacity per type with a burst of 4 full batches
+    for t, pool in kinds_b:
+        serve_open_loop(batched_sv, [(t, pool)], 0, 2 * B, seed=1, burst=True)
+    cap = [serve_open_loop(batched_sv, [k], 0, 4 * B, seed=2, burst=True)["served_per_s"] for k in kinds_b]
     mix_cap = 1.0 / (0.5 / cap[0] + 0.5 / cap[1])
-    # determinism: a request's words in a full batch equal its words alone
-    for sv, name in zip(servers, ("bfv", "ckks")):
-        last = sv.max_batch - 1
-        sv.run(list(range(sv.max_batch)))
-        torch.cuda.synchronize()
-        batched = [sv.out[0].clone(), sv.out[last].clone()]
-        for k, i in ((0, 0), (1, last)):
-            sv.run([i])
-            torch.cuda.synchronize()
-            if not torch.equal(sv.out[0], batched[k]):
+    # determinism: a request's words in a burst equal its words alone
+    for (t, pool), (tp, _), name in zip(kinds_b, kinds_p, ("bfv", "ckks")):
+        words = batched_sv.request_size(t)[1]
+        outs = [np.empty(words, dtype=np.uint64) for _ in range(B)]
+        ids = [batched_sv.submit(t, pool[i % len(pool)], outs[i]) for i in range(B)]
+        infos = [batched_sv.wait(i) for i in ids]
+        alone = np.empty(words, dtype=np.uint64)
+        for i in (0, B - 1):
+            plain_sv.wait(plain_sv.submit(tp, pool[i % len(pool)], alone))
+            if not np.array_equal(alone, outs[i]):
                 raise SystemExit(f"bench self-check failed: {name} request {i} differs alone and in a batch")
-    # dominant kernel of the mix: the inner sums of a full CKKS batch (all g giant steps'
-    # plain-product sums over the b baby rotations, shc_multiply_plain_accumulate_many)
-    ck_sv = servers[1]
-    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-    sums_ms = []
-    for _ in range(max(args.steps, 1)):
-        ck_sv.run(list(range(ck_sv.max_batch)), around_sums=(ev[0].record, ev[1].record))
-        torch.cuda.synchronize()
-        sums_ms.append(ev[0].elapsed_time(ev[1]))
-    sums_ms = sum(sums_ms) / len(sums_ms)
-    m, bb, gg, Lc = ck_sv.max_batch, 32, 32, ck_sv.L
-    # reads the b baby rotations of m requests (2L limbs each) and the g x b diagonals
-    # (L limbs each), writes g inner sums per request (2L limbs)
-    sums_bytes = (bb * m * 2 * Lc + gg * bb * Lc + gg * m * 2 * Lc) * ck_sv.n * 8
+        if max(inf.batch_size for inf in infos) < 2:
+            raise SystemExit(f"bench self-check failed: the {name} burst was never batched")
     rate = 0.5 * mix_cap
     reqs = max(200, int(rate * args.serve_seconds))
-    batched = serve_mixed(torch, servers, rate, reqs, B, seed=11)
-    plain = serve_mixed(torch, servers, rate, reqs, 1, seed=11)
+    # the dominant kernel of the mix: the inner sums of a full CKKS batch (every giant
+    # step's plain-product sum over the b baby rotations, shc_multiply_plain_accumulate_many)
+    ck_res = res[1]
+    Lc, nc, bb, gg = ck_res.L, ck_res.n, ck_res.b, ck_res.g
+    babies = torch.empty((bb, B, 2, Lc, nc), dtype=torch.int64, device="cuda")
+    fill_uniform(torch, babies, ckks_moduli[:Lc], 3, gen)
+    inner = torch.empty((gg, B, 2, Lc, nc), dtype=torch.int64, device="cuda")
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    sums_ms = []
+    for _ in range(max(args.steps, 1) + 1):
+        ev[0].record()
+        ck_res.ctx.multiply_plain_accumulate_many(babies.data_ptr(), ck_res.plains.data_ptr(), bb, gg, B * 2, Lc,
+                                                  inner.data_ptr(), st)
+        ev[1].record()
+        torch.cuda.synchronize()
+        sums_ms.append(ev[0].elapsed_time(ev[1]))
+    sums_ms = sum(sums_ms[1:]) / len(sums_ms[1:])
+    del babies, inner
+    # reads the b baby rotations of B requests (2L limbs each) and the g x b diagonals
+    # (L limbs each), writes g inner sums per request (2L limbs)
+    sums_bytes = (bb * B * 2 * Lc + gg * bb * Lc + gg * B * 2 * Lc) * nc * 8
+    batched = serve_open_loop(batched_sv, kinds_b, rate, reqs, seed=11)
+    plain = serve_open_loop(plain_sv, kinds_p, rate, reqs, seed=11)
+    stats = batched_sv.stats()
     out = {"metric": "mixed BFV/CKKS matvec requests/s served; p50/p99 latency", "value": batched["served_per_s"],
-            "unit": "requests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": 1e3 / batched["served_per_s"], "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "u64",
-            "data": "synthetic (seeded uniform residues for ciphertexts, diagonals and keys; Poisson arrivals)",
-            "config": {"workload": "serving: 50% BFV matvec (N=8192, BFVDefault(8192), t=65537, 4096x4096, "
-                                   "BSGS 64x64), 50% CKKS matvec (N=2^15, 60+40x14+60, 1024x1024, BSGS 32x32), "
-                                   f"batches of at most {B} within {budget >> 30} GiB of batch buffers per type",
-                       "parallelism": "single GPU"},
-            "capacity_per_s": {"bfv": cap[0], "ckks": cap[1], "mix": mix_cap},
-            "admission": {"budget_bytes_per_type": budget,
-                          "max_batch": {"bfv": servers[0].max_batch, "ckks": servers[1].max_batch},
-                          "per_request_bytes": {"bfv": servers[0].per_request_bytes,
-                                                "ckks": servers[1].per_request_bytes},
-                          "batch_bytes": {"bfv": servers[0].batch_bytes, "ckks": servers[1].batch_bytes},
-                          "resident_key_and_diagonal_bytes": {"bfv": servers[0].resident_bytes,
-                                                              "ckks": servers[1].resident_bytes}},
-            "arrivals": {"kind": "poisson", "rate_per_s": rate, "requests": reqs},
-            "batched": batched, "plain": plain, "batched_p99_over_plain_p99": batched["p99_ms"] / plain["p99_ms"],
-            "roofline": {"bound": "hbm", "achieved": sums_bytes / (sums_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": sums_bytes / (sums_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": f"plain_dot_many_kernel: the {gg} inner sums of a full CKKS batch ({m} requests, "
-                                   f"{bb} baby rotations, L={Lc})",
-                         "algorithmic_bytes": sums_bytes, "avg_ms": sums_ms},
-            "self_check": "per type, the first and last request of a full batch give the same words alone"}
-    del servers
+           "unit": "requests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 / batched["served_per_s"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic (seeded uniform residues for ciphertexts, diagonals and keys; Poisson arrivals)",
+           "config": {"workload": "serving: 50% BFV matvec (N=8192, BFVDefault(8192), t=65537, 4096x4096, "
+                                  "BSGS 64x64), 50% CKKS matvec (N=2^15, 60+40x14+60, 1024x1024, BSGS 32x32); "
+                                  f"library server, {lanes} lanes, batches of at most {B}, "
+                                  f"{arena >> 30} GiB device arena; host ciphertexts in and out",
+                      "parallelism": "single GPU"},
+           "capacity_per_s": {"bfv": cap[0], "ckks": cap[1], "mix": mix_cap},
+           "admission": {"device_arena_bytes": arena,
+                         "device_bytes_of_one_request": {"bfv": batched_sv.request_size(kinds_b[0][0])[2],
+                                                         "ckks": batched_sv.request_size(kinds_b[1][0])[2]},
+                         "resident_key_and_diagonal_bytes": {"bfv": res[0].resident_bytes,
+                                                             "ckks": res[1].resident_bytes},
+                         "max_batches_in_flight": stats["max_batches_in_flight"],
+                         "peak_device_bytes": stats["peak_device_bytes"]},
+           "arrivals": {"kind": "poisson", "rate_per_s": rate, "requests": reqs},
+           "batched": batched, "plain": plain, "batched_p99_over_plain_p99": batched["p99_ms"] / plain["p99_ms"],
+           "roofline": {"bound": "hbm", "achieved": sums_bytes / (sums_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": sums_bytes / (sums_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": f"plain_dot_many_kernel: the {gg} inner sums of a full CKKS batch ({B} requests, "
+                                  f"{bb} baby rotations, L={Lc})",
+                        "algorithmic_bytes": sums_bytes, "avg_ms": sums_ms},
+           "self_check": "per type, the first and last request of a burst give the same words served alone"}
+    batched_sv.close()
+    plain_sv.close()
+    del res
     torch.cuda.empty_cache()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_serve(bfv_moduli, ckks_moduli, args.cpu_seconds)

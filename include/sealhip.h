@@ -33,7 +33,8 @@ enum {
     SHC_LOGIC_ERROR = -2,      /* std::logic_error      */
     SHC_OUT_OF_RANGE = -3,     /* std::out_of_range     */
     SHC_RUNTIME_ERROR = -4,    /* std::runtime_error    */
-    SHC_HIP_ERROR = -5         /* HIP runtime failure (reported as std::runtime_error) */
+    SHC_HIP_ERROR = -5,        /* HIP runtime failure (reported as std::runtime_error) */
+    SHC_TIMEOUT = 1            /* shc_server_wait: the request is not done yet (not an error) */
 };
 
 enum { SHC_SCHEME_NONE = 0, SHC_SCHEME_BFV = 1, SHC_SCHEME_CKKS = 2, SHC_SCHEME_BGV = 3 };
@@ -209,6 +210,16 @@ int shc_batch_decode(const shc_context *ctx, const uint64_t *d_plain, uint64_t *
 int shc_ckks_encode_workspace_size(const shc_context *ctx, size_t *bytes);
 int shc_ckks_encode(const shc_context *ctx, const double *d_values, uint32_t count, double scale, uint32_t limb_count,
                     uint64_t *d_plain, void *d_workspace, size_t workspace_bytes, void *stream);
+/* shc_ckks_encode without waiting for the device: the range check that shc_ckks_encode
+ * raises (encoded values not finite or too large for the coefficient modulus,
+ * CKKSEncoder::encodeInternal, src/ckks.cpp:167-171) is written to the device word
+ * *d_status instead -- 0 when encoded, SHC_INVALID_ARGUMENT when rejected, the plaintext
+ * then all zero -- for the caller to read when it needs to.  Argument errors known on
+ * the host are still returned.  The call launches kernels only, so it can be captured
+ * in a HIP graph.  No counterpart in the reference. */
+int shc_ckks_encode_deferred(const shc_context *ctx, const double *d_values, uint32_t count, double scale,
+                             uint32_t limb_count, uint64_t *d_plain, int32_t *d_status, void *d_workspace,
+                             size_t workspace_bytes, void *stream);
 int shc_ckks_decode_workspace_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
 int shc_ckks_decode(const shc_context *ctx, const uint64_t *d_plain, uint32_t limb_count, double scale,
                     double *d_values, void *d_workspace, size_t workspace_bytes, void *stream);
@@ -328,6 +339,70 @@ int shc_rotate_many(const shc_context *ctx, const uint64_t *d_in, uint32_t count
  * thread-safe: of key switches that run concurrently on one context (on different
  * streams), at most one takes the armed probe. */
 int shc_probe_keyswitch(shc_context *ctx, void *begin_event, void *end_event);
+
+/* ---- serving ------------------------------------------------------------------
+ * A continuous-batching request server (no counterpart in the reference, whose
+ * serving-shaped caller, app/matmul.cuh:98-121, runs one request at a time on the
+ * default stream).  The server allocates one device arena of `device_bytes` and one
+ * pinned host staging arena of `staging_bytes` at creation, and nothing afterwards.
+ * It drives `lanes` (1-4) HIP streams of default priority, one host thread each.
+ * Requests are submitted with host buffers (input copied into staging before
+ * shc_server_submit returns; the result written to h_out before the request is
+ * reported done).  A free lane takes the type of the oldest waiting request and admits
+ * every waiting request of that type, up to `max_batch`, whose batch fits the device
+ * memory free at that moment (when none fits, it waits for another lane to release
+ * memory); the batch runs on the lane's stream (upload, batched operations, download).
+ * Batches of different lanes overlap on the device.  A request's words do not depend on
+ * the batch it ran in.  Contexts, keys and diagonals a type refers to stay owned by the
+ * caller and must outlive the server.  Safe to call from several host threads. */
+typedef struct shc_server shc_server;
+
+typedef struct {
+    double submit_s, admit_s, done_s; /* seconds on the server's clock (shc_server_clock) */
+    uint32_t batch_size, lane;        /* the batch the request ran in, and its lane */
+    uint64_t batch_id;
+} shc_request_info;
+
+typedef struct {
+    uint64_t submitted, completed, failed, batches;
+    uint32_t max_batches_in_flight; /* most batches that ran at once */
+    size_t peak_device_bytes;       /* most device arena bytes taken at once */
+} shc_server_stats;
+
+int shc_server_create(int device, size_t device_bytes, size_t staging_bytes, uint32_t lanes, uint32_t max_batch,
+                      shc_server **out);
+/* Stops the lanes after their running batches; requests still waiting are dropped. */
+void shc_server_destroy(shc_server *server);
+
+/* Request type "diagonal matrix-vector product" (app/matmul.cuh:98-121 with baby-step
+ * giant-step rotations): input and output one size-2 ciphertext [2][L][N] (CKKS: NTT
+ * form; BFV: coefficient form); d_diagonals [giant][baby][L][N] the pre-rotated
+ * diagonals in NTT form; d_baby_keys / d_baby_hoisted: Galois keys of steps
+ * 1..baby-1 and their forms prepared for level L (shc_hoisted_key_prepare);
+ * d_giant_keys: Galois keys of steps baby, 2 baby, ..., (giant-1) baby (host arrays of
+ * device pointers; copied). */
+int shc_server_add_matvec(shc_server *server, const shc_context *ctx, uint32_t limb_count, uint32_t baby,
+                          uint32_t giant, const uint64_t *d_diagonals, const uint64_t *const *d_baby_keys,
+                          const uint64_t *const *d_baby_hoisted, const uint64_t *const *d_giant_keys,
+                          uint32_t *type);
+/* Request type "multiply, relinearize[, rescale]" (EvaluatorCuda::multiplyInplace then
+ * relinearizeInplace, src/evaluator_cuda.cu:262-281, 703-744): input two size-2
+ * ciphertexts a, b back to back [2][2][L][N], output [2][L][N] ([2][L-1][N] after the
+ * rescale / modulus switch when rescale != 0). */
+int shc_server_add_mulrelin(shc_server *server, const shc_context *ctx, uint32_t limb_count,
+                            const uint64_t *d_relin_key, int rescale, uint32_t *type);
+/* Words of one request's input and output, and the device bytes a batch of one takes. */
+int shc_server_request_size(const shc_server *server, uint32_t type, size_t *in_words, size_t *out_words,
+                            size_t *device_bytes);
+/* Queues a request (blocks while the staging arena is full); *id names it. */
+int shc_server_submit(shc_server *server, uint32_t type, const uint64_t *h_in, uint64_t *h_out, uint64_t *id);
+/* Waits up to timeout_s seconds (< 0: no limit) for request id: SHC_OK when done (the
+ * request is then forgotten; info, if not NULL, describes it), SHC_TIMEOUT when not
+ * yet, or the error the request's batch failed with. */
+int shc_server_wait(shc_server *server, uint64_t id, double timeout_s, shc_request_info *info);
+/* Seconds since the server was created (the clock of shc_request_info). */
+double shc_server_clock(const shc_server *server);
+int shc_server_get_stats(const shc_server *server, shc_server_stats *out);
 
 #ifdef __cplusplus
 }

@@ -312,6 +312,21 @@ public:
         if (n) ptr_ = static_cast<T *>(MemoryPool::get(bytes, cap_));
         size_ = n;
     }
+    // Resizes keeping the first min(size, n) elements and zeroing any new ones (the
+    // reference's DeviceDynamicArray::resize, src/utils/devicearray.cuh:208-216, 289-292).
+    void resizeKeep(size_t n) {
+        if (n == size_) return;
+        if (ptr_ && n * sizeof(T) <= cap_) {
+            if (n > size_) hip_check(hipMemset(ptr_ + size_, 0, (n - size_) * sizeof(T)), "memset");
+            size_ = n;
+            return;
+        }
+        DeviceArray t(n);
+        const size_t keep = std::min(n, size_);
+        if (keep) hip_check(hipMemcpy(t.ptr_, ptr_, keep * sizeof(T), hipMemcpyDeviceToDevice), "copy");
+        if (n > keep) hip_check(hipMemset(t.ptr_ + keep, 0, (n - keep) * sizeof(T)), "memset");
+        *this = std::move(t);
+    }
     // Grows to at least n elements, keeping nothing.
     void ensure(size_t n) {
         if (n > size_) resize(n);
@@ -854,7 +869,7 @@ public:
 
     void resize(size_t coeff_count) {
         if (isNttForm()) throw std::logic_error("cannot resize an NTT transformed Plaintext");
-        data_.resize(coeff_count);
+        data_.resizeKeep(coeff_count);
     }
     void release() {
         parms_id_ = parmsIDZero;

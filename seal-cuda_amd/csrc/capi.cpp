@@ -8,11 +8,16 @@
 #include "galois.h"
 #include "rns_ops.h"
 #include "rotations.h"
+#include "server.h"
 #include <algorithm>
 #include <string>
 
 struct shc_context {
     std::unique_ptr<shc::Context> impl;
+};
+
+struct shc_server {
+    std::unique_ptr<shc::Server> impl;
 };
 
 namespace {
@@ -314,6 +319,16 @@ int shc_ckks_encode(const shc_context *ctx, const double *values, uint32_t count
     });
 }
 
+int shc_ckks_encode_deferred(const shc_context *ctx, const double *values, uint32_t count, double scale,
+                             uint32_t limbs, uint64_t *plain, int32_t *d_status, void *ws, size_t ws_bytes,
+                             void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ckks_encode_deferred(c, values, count, scale, limbs, plain, d_status, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
 int shc_ckks_decode_workspace_size(const shc_context *ctx, uint32_t limbs, size_t *bytes) {
     return guarded([&] {
         const auto &c = ctx_of(ctx);
@@ -466,6 +481,110 @@ int shc_probe_keyswitch(shc_context *ctx, void *begin_event, void *end_event) {
         const auto &c = ctx_of(ctx);
         if (!begin_event || !end_event) shc::fail(shc::ERR_INVALID_ARGUMENT, "null event");
         c.arm_probe(shc::KernelProbe{(hipEvent_t)begin_event, (hipEvent_t)end_event});
+    });
+}
+
+int shc_server_create(int device, size_t device_bytes, size_t staging_bytes, uint32_t lanes, uint32_t max_batch,
+                      shc_server **out) {
+    return guarded([&] {
+        if (!out) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        auto impl = std::make_unique<shc::Server>(device, device_bytes, staging_bytes, lanes, max_batch);
+        *out = new shc_server{std::move(impl)};
+    });
+}
+
+void shc_server_destroy(shc_server *server) { delete server; }
+
+static shc::Server &server_of(const shc_server *s) {
+    if (!s || !s->impl) shc::fail(shc::ERR_INVALID_ARGUMENT, "server is null");
+    return *s->impl;
+}
+
+int shc_server_add_matvec(shc_server *server, const shc_context *ctx, uint32_t limbs, uint32_t baby, uint32_t giant,
+                          const uint64_t *d_diagonals, const uint64_t *const *d_baby_keys,
+                          const uint64_t *const *d_baby_hoisted, const uint64_t *const *d_giant_keys,
+                          uint32_t *type) {
+    return guarded([&] {
+        if (!type) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        if (baby < 1 || giant < 1) shc::fail(shc::ERR_INVALID_ARGUMENT, "baby and giant steps must be >= 1");
+        if ((baby > 1 && (!d_baby_keys || !d_baby_hoisted)) || (giant > 1 && !d_giant_keys))
+            shc::fail(shc::ERR_INVALID_ARGUMENT, "null key array");
+        shc::ServerProgram p;
+        p.kind = shc::ServerProgram::MATVEC;
+        p.ctx = &ctx_of(ctx);
+        p.L = limbs;
+        p.b = baby;
+        p.g = giant;
+        p.plains = d_diagonals;
+        if (baby > 1) {
+            p.baby_keys.assign(d_baby_keys, d_baby_keys + (baby - 1));
+            p.baby_hoisted.assign(d_baby_hoisted, d_baby_hoisted + (baby - 1));
+        }
+        if (giant > 1) p.giant_keys.assign(d_giant_keys, d_giant_keys + (giant - 1));
+        *type = server_of(server).add_program(std::move(p));
+    });
+}
+
+int shc_server_add_mulrelin(shc_server *server, const shc_context *ctx, uint32_t limbs, const uint64_t *d_relin_key,
+                            int rescale, uint32_t *type) {
+    return guarded([&] {
+        if (!type) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        shc::ServerProgram p;
+        p.kind = shc::ServerProgram::MULRELIN;
+        p.ctx = &ctx_of(ctx);
+        p.L = limbs;
+        p.relin_key = d_relin_key;
+        p.rescale = rescale != 0;
+        *type = server_of(server).add_program(std::move(p));
+    });
+}
+
+int shc_server_request_size(const shc_server *server, uint32_t type, size_t *in_words, size_t *out_words,
+                            size_t *device_bytes) {
+    return guarded([&] {
+        const shc::ServerProgram &p = server_of(server).program(type);
+        if (in_words) *in_words = p.in_words();
+        if (out_words) *out_words = p.out_words();
+        if (device_bytes) *device_bytes = p.footprint(1);
+    });
+}
+
+int shc_server_submit(shc_server *server, uint32_t type, const uint64_t *h_in, uint64_t *h_out, uint64_t *id) {
+    return guarded([&] {
+        if (!id) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        *id = server_of(server).submit(type, h_in, h_out);
+    });
+}
+
+int shc_server_wait(shc_server *server, uint64_t id, double timeout_s, shc_request_info *info) {
+    bool done = false;
+    const int rc = guarded([&] {
+        shc::RequestInfo ri;
+        done = server_of(server).wait(id, timeout_s, &ri);
+        if (done && info) {
+            info->submit_s = ri.submit_s;
+            info->admit_s = ri.admit_s;
+            info->done_s = ri.done_s;
+            info->batch_size = ri.batch_size;
+            info->lane = ri.lane;
+            info->batch_id = ri.batch_id;
+        }
+    });
+    return rc == SHC_OK && !done ? SHC_TIMEOUT : rc;
+}
+
+double shc_server_clock(const shc_server *server) { return server && server->impl ? server->impl->now() : 0.0; }
+
+int shc_server_get_stats(const shc_server *server, shc_server_stats *out) {
+    return guarded([&] {
+        if (!out) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        const shc::ServerStats s = server_of(server).stats();
+        out->submitted = s.submitted;
+        out->completed = s.completed;
+        out->failed = s.failed;
+        out->batches = s.batches;
+        out->max_batches_in_flight = s.max_batches_in_flight;
+        out->peak_device_bytes = s.peak_device_bytes;
     });
 }
 

@@ -26,6 +26,10 @@ size_t ckks_encode_workspace(const Context &c);
 // values: `count` (re, im) double pairs (device), count <= N/2 -> plain [L][N] NTT form.
 void ckks_encode(const Context &c, const double *values, uint32_t count, double scale, uint32_t L, uint64_t *plain,
                  void *ws, size_t ws_bytes, hipStream_t st);
+// The same, with the range check written to *d_status on the device (0, or -1 with a
+// zero plaintext) instead of raised: nothing waits for the stream.
+void ckks_encode_deferred(const Context &c, const double *values, uint32_t count, double scale, uint32_t L,
+                          uint64_t *plain, int *d_status, void *ws, size_t ws_bytes, hipStream_t st);
 size_t ckks_decode_workspace(const Context &c, uint32_t L);
 // plain [L][N] NTT form -> N/2 (re, im) double pairs.
 void ckks_decode(const Context &c, const uint64_t *plain, uint32_t L, double scale, double *values, void *ws,

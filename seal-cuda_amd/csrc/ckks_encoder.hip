@@ -105,6 +105,29 @@ __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__r
     }
 }
 
+__global__ void clear_word_kernel(unsigned long long *w) { *w = 0; }
+
+// The encoder's range check on the device: status = 0, or SHC_INVALID_ARGUMENT (-1) when
+// the largest |coefficient| is not finite or needs total_bits or more bits, or more than
+// 128 (CKKSEncoder::encodeInternal's "encoded values are too large", src/ckks.cpp:167-171).
+__global__ void encode_check_kernel(const unsigned long long *maxbits, int total_bits, int *status) {
+    const double maxc = __longlong_as_double((long long)*maxbits);
+    int st = 0;
+    if (!isfinite(maxc)) {
+        st = -1;
+    } else {
+        const int bits = (int)ceil(log2(fmax(maxc, 1.0))) + 1;
+        if (bits >= total_bits || bits > 128) st = -1;
+    }
+    *status = st;
+}
+
+// A plaintext the check rejected is left all zero.
+__global__ void zero_if_rejected_kernel(uint64_t *__restrict__ plain, const int *__restrict__ status, size_t words) {
+    if (*status == 0) return;
+    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < words; i += (size_t)gridDim.x * TPB) plain[i] = 0;
+}
+
 __global__ void compose_kernel(const uint64_t *__restrict__ c, double2 *__restrict__ a, CkksLevelDev T,
                                const DevPrime *__restrict__ primes, int logn, double inv_scale) {
     const size_t n = (size_t)1 << logn;
@@ -222,8 +245,13 @@ void build_ckks_tables(Context &c) {
 
 size_t ckks_encode_workspace(const Context &c) { return c.n * 16 + 256; }
 
-void ckks_encode(const Context &c, const double *values, uint32_t count, double scale, uint32_t L, uint64_t *plain,
-                 void *ws, size_t ws_bytes, hipStream_t st) {
+namespace {
+
+// Encode up to the forward NTT.  deferred: the range check goes to *d_status on the
+// device (a rejected plaintext is zero) and nothing waits for the stream; otherwise the
+// check is read back and raised like the reference's.
+void encode_body(const Context &c, const double *values, uint32_t count, double scale, uint32_t L, uint64_t *plain,
+                 void *ws, size_t ws_bytes, int *d_status, hipStream_t st) {
     if (c.scheme != SCHEME_CKKS || !c.ckks) fail(ERR_INVALID_ARGUMENT, "unsupported scheme");
     if (c.logn < 1) fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree too small");
     if (count > c.n / 2) fail(ERR_INVALID_ARGUMENT, "values_size is too large");
@@ -242,9 +270,15 @@ void ckks_encode(const Context &c, const double *values, uint32_t count, double 
                                                          (uint32_t)(n / 2));
         ridx += m;
     }
-    SHC_HIP(hipMemsetAsync(maxbits, 0, 8, st));
+    clear_word_kernel<<<1, 1, 0, st>>>(maxbits);
     round_reduce_kernel<<<blocks_for(n), TPB, 0, st>>>(a, plain, c.primes(), L, c.logn, maxbits);
     launch_check("ckks encode");
+    if (d_status) {
+        encode_check_kernel<<<1, 1, 0, st>>>(maxbits, T.total_bits, d_status);
+        zero_if_rejected_kernel<<<blocks_for((size_t)L * n), TPB, 0, st>>>(plain, d_status, (size_t)L * n);
+        launch_check("ckks encode check");
+        return;
+    }
     unsigned long long mb = 0;
     SHC_HIP(hipMemcpyAsync(&mb, maxbits, 8, hipMemcpyDeviceToHost, st));
     SHC_HIP(hipStreamSynchronize(st));
@@ -255,6 +289,20 @@ void ckks_encode(const Context &c, const double *values, uint32_t count, double 
     if (!std::isfinite(maxc)) fail(ERR_INVALID_ARGUMENT, "encoded values are not finite");
     const int bits = (int)std::ceil(std::log2(std::max(maxc, 1.0))) + 1;
     if (bits >= T.total_bits || bits > 128) fail(ERR_INVALID_ARGUMENT, "encoded values are too large");
+}
+
+}  // namespace
+
+void ckks_encode(const Context &c, const double *values, uint32_t count, double scale, uint32_t L, uint64_t *plain,
+                 void *ws, size_t ws_bytes, hipStream_t st) {
+    encode_body(c, values, count, scale, L, plain, ws, ws_bytes, nullptr, st);
+    ntt_forward(plain, L, L, c.primes(), c.logn, NTT_FULL, st);
+}
+
+void ckks_encode_deferred(const Context &c, const double *values, uint32_t count, double scale, uint32_t L,
+                          uint64_t *plain, int *d_status, void *ws, size_t ws_bytes, hipStream_t st) {
+    if (!d_status) fail(ERR_INVALID_ARGUMENT, "null status word");
+    encode_body(c, values, count, scale, L, plain, ws, ws_bytes, d_status, st);
     ntt_forward(plain, L, L, c.primes(), c.logn, NTT_FULL, st);
 }
 

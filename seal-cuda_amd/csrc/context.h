@@ -77,6 +77,12 @@ struct Context {
         return true;
     }
 
+    // Hoisted keys prepared through this context (device address -> level, Galois
+    // element), so that rotation sums check a prepared key on the host, without reading
+    // its header words back from the device.
+    mutable std::mutex hoisted_mu;
+    mutable std::map<const void *, std::pair<uint32_t, uint32_t>> hoisted_keys;
+
     size_t key_count() const { return moduli.size(); }
     const DevPrime *primes() const { return d_primes.as<DevPrime>(); }
     const DevPrime *keyswitch_primes(size_t level) const {

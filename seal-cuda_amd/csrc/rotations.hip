@@ -12,11 +12,14 @@
 //     = sigma(sum_j T_ij sigma^-1(key_j)) + NTT_i(S_sigma) sum_j (q_j mod q_i) key_j
 //     = sigma(sum_j T_ij key'_j + corr')
 // with key' = sigma^-1(key) and corr' = sigma^-1(NTT_i(S_sigma) sum_j (q_j mod q_i) key_j),
-// prepared once per key and level.  A zero digit (d = 0) needs no correction, so calls
-// whose c1 has one fall back to separate rotations and the words always equal the
-// reference's.  Each rotation then costs one inner product, one permutation of the
-// 2(L+1) product limbs and one ModDown instead of a full ModUp (L x L transforms for
-// CKKS, L(L+1) for BFV).
+// prepared once per key and level.  A zero digit d_j[p] = 0 at a position sigma negates
+// has the reference digit 0, not q_j: corr' counted (q_j mod q_i) X^sigma(p) there, whose
+// preimage under sigma is -X^p.  Each such zero is put right in the products before the
+// permutation, P += (q_j mod q_i) NTT_i(X^p) key'_j (the NTT of a monomial is a power of
+// the root at each slot), and a ciphertext whose digits are all zero has products zero.
+// The zeros are listed on the device, so no decision waits for the host.  Each rotation
+// then costs one inner product, one permutation of the 2(L+1) product limbs and one
+// ModDown instead of a full ModUp (L x L transforms for CKKS, L(L+1) for BFV).
 #include "rotations.h"
 #include "elementwise.h"
 #include "galois.h"
@@ -87,6 +90,7 @@ constexpr int RG = 2;
 struct RotBatch {
     const uint64_t *key[RMAX];   // prepared keys; corr' follows at corr_off words
     uint64_t *P[RMAX], *S[RMAX];
+    uint32_t elt[RMAX];          // their Galois elements
 };
 
 template <int R>
@@ -214,16 +218,86 @@ __global__ void check_hoisted_headers_kernel(HeaderBatch hb, uint32_t nr, uint32
     if (hb.hdr[r][0] != (HOISTED_TAG | L) || hb.hdr[r][1] != hb.elt[r]) atomicOr(flag, 2ull);
 }
 
-// flag |= 1 if any digit word is zero; digits of ct at digits + ct * dstride.
-__global__ void zero_digit_kernel(const uint64_t *__restrict__ digits, size_t dstride, size_t per_ct,
-                                  unsigned long long *flag, size_t total) {
+// Zero digits of the batch: zc[0] counts them and entry e < zc[0] of `list` is
+// (ct << 32) | (j << 24) | p; zc[2 + ct] counts ciphertext ct's (the list has room for
+// every digit word, so it cannot overflow).
+__global__ void zero_list_kernel(const uint64_t *__restrict__ digits, size_t dstride, uint32_t L, int logn,
+                                 unsigned long long *__restrict__ zc, uint64_t *__restrict__ list, size_t total) {
+    const size_t per_ct = (size_t)L << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        if (digits[(idx / per_ct) * dstride + idx % per_ct] == 0) atomicOr(flag, 1ull);
+        const size_t ct = idx / per_ct, r = idx % per_ct;
+        if (digits[ct * dstride + r] != 0) continue;
+        const unsigned long long e = atomicAdd(zc, 1ull);
+        atomicAdd(zc + 2 + ct, 1ull);
+        list[e] = ((uint64_t)ct << 32) | ((uint64_t)(r >> logn) << 24) | (r & ((size_t(1) << logn) - 1));
+    }
+}
+
+__global__ void clear_words_kernel(unsigned long long *w, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < words; i += (size_t)gridDim.x * TPB) w[i] = 0;
+}
+
+// The zero-digit fix of R rotations' products (before their permutations): at target
+// i, slot s (one thread each),
+//   P_r[ct][k][i][s] += sum over zeros (ct, j, p) that sigma_r negates, j's prime not i,
+//                       of (q_j mod q_i) psi_i^((2 brev(s) + 1) p) key'_r[j][k][ki][s],
+// psi_i^((2 brev(s)+1) p) being slot s of NTT_i(X^p); products of a ciphertext whose
+// L N digit words are all zero are set to zero (its exact products).  Exits at once
+// when the batch has no zero digit.
+template <int R>
+__global__ __launch_bounds__(TPB) void zero_digit_fix_kernel(RotBatch rb, const unsigned long long *__restrict__ zc,
+                                                             const uint64_t *__restrict__ list,
+                                                             const DevPrime *__restrict__ targets,
+                                                             const DevPrime *__restrict__ data, uint32_t L, uint32_t K,
+                                                             int logn, uint32_t count) {
+    const unsigned long long z = zc[0];
+    if (z == 0) return;
+    const size_t n = (size_t)1 << logn;
+    const size_t flat = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (flat >= (size_t)(L + 1) << logn) return;
+    const uint32_t i = (uint32_t)(flat >> logn);
+    const uint32_t s = (uint32_t)(flat & (n - 1));
+    const uint32_t ki = (i == L) ? K - 1 : i;
+    const DevPrime &Q = targets[i];
+    const uint64_t all = (uint64_t)L << logn;
+    auto out = [&](int r, size_t ct, int k) -> uint64_t * {
+        return i == L ? rb.S[r] + (ct * 2 + k) * n + s : rb.P[r] + ((ct * 2 + k) * L + i) * n + s;
+    };
+    for (uint32_t ct = 0; ct < count; ct++) {
+        if (zc[2 + ct] != all) continue;
+#pragma unroll
+        for (int r = 0; r < R; r++) *out(r, ct, 0) = *out(r, ct, 1) = 0;
+    }
+    const uint64_t two_bs1 = 2 * (uint64_t)(__brev(s) >> (32 - logn)) + 1;
+    const size_t kstride = (size_t)K << logn;
+    for (unsigned long long e = 0; e < z; e++) {
+        const uint64_t w = list[e];
+        const size_t ct = w >> 32;
+        if (zc[2 + ct] == all) continue;
+        const uint32_t j = (uint32_t)(w >> 24) & 0xff, p = (uint32_t)w & 0xffffff;
+        if (j == i) continue;  // q_j mod q_j = 0
+        const uint64_t qji = barrett64(data[j].q, Q.q, Q.ratio_hi);
+        const uint64_t ex = (two_bs1 * p) & (2 * n - 1);
+        const uint64_t m = ex & (n - 1);
+        const uint64_t root = Q.fwd[__brev((uint32_t)m) >> (32 - logn)].x;
+        const uint64_t psi = ex >= n ? Q.q - root : root;
+        const uint64_t f = mulmod(qji, psi, Q.q, Q.ratio_lo, Q.ratio_hi);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (((((uint64_t)p * rb.elt[r]) >> logn) & 1) == 0) continue;  // sigma_r keeps p's sign
+            const uint64_t *kp = rb.key[r] + (size_t)ki * n + s;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const uint64_t t = mulmod(f, kp[(2 * (size_t)j + k) * kstride], Q.q, Q.ratio_lo, Q.ratio_hi);
+                uint64_t *o = out(r, ct, k);
+                *o = addmod(*o, t, Q.q);
+            }
+        }
     }
 }
 
 struct RotLayout {
-    size_t gal, tmpct, P2, S2, C0, Pm, Sm, flag, total;
+    size_t gal, tmpct, P2, S2, C0, Pm, Sm, flag, zc, total;
 };
 
 RotLayout rot_layout(const Context &c, uint32_t count, uint32_t L) {
@@ -239,6 +313,7 @@ RotLayout rot_layout(const Context &c, uint32_t count, uint32_t L) {
     l.Pm = take((size_t)RMAX * count * 2 * L * limb);
     l.Sm = take((size_t)RMAX * count * 2 * limb);
     l.flag = take(8);
+    l.zc = take(8 * (2 + (size_t)count));
     l.total = off;
     return l;
 }
@@ -282,6 +357,8 @@ void hoisted_key_prepare(const Context &c, const uint64_t *galois_key, uint32_t 
     apply_galois(c, C, nw, out + (size_t)(K - 1) * 2 * K * n, nw, 2, L + 1, elt_inv, true, st);
     write_hoisted_header_kernel<<<1, 1, 0, st>>>(out + hoisted_key_words(c, L) - 2, L, elt);
     launch_check("hoisted key header");
+    std::lock_guard<std::mutex> g(c.hoisted_mu);
+    c.hoisted_keys[out] = {L, elt};
 }
 
 size_t rotate_sum_workspace(const Context &c, uint32_t count, uint32_t L) { return rot_layout(c, count, L).total; }
@@ -307,8 +384,6 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
     const int logn = c.logn;
     const size_t n = c.n, lw = (size_t)L * n;
     char *base = static_cast<char *>(ws);
-    void *gal_ws = base + lay.gal;
-    const size_t gal_bytes = galois_keyswitch_workspace(c, count, L);
     uint64_t *tmpct = reinterpret_cast<uint64_t *>(base + lay.tmpct);
     uint64_t *P2 = reinterpret_cast<uint64_t *>(base + lay.P2);
     uint64_t *S2 = reinterpret_cast<uint64_t *>(base + lay.S2);
@@ -318,6 +393,39 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
     const size_t sigma_bytes = align256((size_t)count * lw * 8);
     const KsRegions ks = keyswitch_regions(c, count, L, base + lay.gal + sigma_bytes);
 
+    // each prepared key must be the one of this level and element: checked on the
+    // host for keys prepared through this context; other keys' header words are read
+    // back from the device (the only case that waits for the stream)
+    const size_t hdr_off = hoisted_key_words(c, L) - 2;
+    std::vector<uint32_t> unknown;
+    {
+        std::lock_guard<std::mutex> g(c.hoisted_mu);
+        for (uint32_t r = 0; r < rotations; r++) {
+            if (!hoisted[r] || !keys[r]) fail(ERR_INVALID_ARGUMENT, "null key pointer");
+            auto it = c.hoisted_keys.find(hoisted[r]);
+            if (it == c.hoisted_keys.end()) unknown.push_back(r);
+            else if (it->second.first != L || it->second.second != elts[r])
+                fail(ERR_INVALID_ARGUMENT, "a hoisted key was prepared for another level or Galois element");
+        }
+    }
+    if (!unknown.empty()) {
+        clear_flag_kernel<<<1, 1, 0, st>>>(flag);
+        for (size_t r0 = 0; r0 < unknown.size(); r0 += HDR_CHUNK) {
+            HeaderBatch hb{};
+            const uint32_t nr = (uint32_t)std::min<size_t>(unknown.size() - r0, HDR_CHUNK);
+            for (uint32_t q = 0; q < nr; q++) {
+                hb.hdr[q] = hoisted[unknown[r0 + q]] + hdr_off;
+                hb.elt[q] = elts[unknown[r0 + q]];
+            }
+            check_hoisted_headers_kernel<<<1, 64, 0, st>>>(hb, nr, L, flag);
+            launch_check("hoisted key headers");
+        }
+        unsigned long long flags = 0;
+        SHC_HIP(hipMemcpyAsync(&flags, flag, 8, hipMemcpyDeviceToHost, st));
+        SHC_HIP(hipStreamSynchronize(st));
+        if (flags & 2)
+            fail(ERR_INVALID_ARGUMENT, "a hoisted key was prepared for another level or Galois element");
+    }
     // 1. digits of c1 in coefficient form (CKKS: transformed into ks.d; c1 itself, in NTT
     // form, supplies the own-prime terms of the products)
     const uint64_t *digits = in + lw;
@@ -328,40 +436,12 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
         digits = ks.d;
         dstride = lw;
     }
-    // 2. a zero digit has no sign correction: separate rotations then
-    clear_flag_kernel<<<1, 1, 0, st>>>(flag);
-    zero_digit_kernel<<<blocks_for(count * lw), TPB, 0, st>>>(digits, dstride, lw, flag, count * lw);
-    launch_check("zero_digit");
-    // each prepared key must be the one of this level and element (its header words)
-    const size_t hdr_off = hoisted_key_words(c, L) - 2;
-    for (uint32_t r0 = 0; r0 < rotations; r0 += HDR_CHUNK) {
-        HeaderBatch hb{};
-        const uint32_t nr = rotations - r0 < (uint32_t)HDR_CHUNK ? rotations - r0 : (uint32_t)HDR_CHUNK;
-        for (uint32_t q = 0; q < nr; q++) {
-            if (!hoisted[r0 + q] || !keys[r0 + q]) fail(ERR_INVALID_ARGUMENT, "null key pointer");
-            hb.hdr[q] = hoisted[r0 + q] + hdr_off;
-            hb.elt[q] = elts[r0 + q];
-        }
-        check_hoisted_headers_kernel<<<1, 64, 0, st>>>(hb, nr, L, flag);
-        launch_check("hoisted key headers");
-    }
-    unsigned long long flags = 0;
-    SHC_HIP(hipMemcpyAsync(&flags, flag, 8, hipMemcpyDeviceToHost, st));
-    SHC_HIP(hipStreamSynchronize(st));
-    if (flags & 2)
-        fail(ERR_INVALID_ARGUMENT, "a hoisted key was prepared for another level or Galois element");
+    // 2. zero digits, listed for the products' fix (the list has room for every word)
+    auto *zc = reinterpret_cast<unsigned long long *>(base + lay.zc);
+    clear_words_kernel<<<1, TPB, 0, st>>>(zc, 2 + (size_t)count);
+    zero_list_kernel<<<blocks_for(count * lw), TPB, 0, st>>>(digits, dstride, L, logn, zc, tmpct, count * lw);
+    launch_check("zero digits");
     const size_t ct_words = (size_t)count * 2 * lw;
-    if (flags & 1) {
-        for (uint32_t r = 0; r < rotations; r++) {
-            uint64_t *dst = sum ? tmpct : out + r * ct_words;
-            SHC_HIP(hipMemcpyAsync(dst, in, ct_words * 8, hipMemcpyDeviceToDevice, st));
-            apply_galois_keyswitch(c, dst, count, L, elts[r], keys[r], gal_ws, gal_bytes, st);
-            if (!sum) continue;
-            if (r == 0) SHC_HIP(hipMemcpyAsync(out, tmpct, ct_words * 8, hipMemcpyDeviceToDevice, st));
-            else poly_add(c, out, tmpct, out, count * 2, L, st);
-        }
-        return;
-    }
     // 3. one ModUp for every rotation
     const bool skipped = modup_to_T(c, digits, dstride, ks.T, count, L, st, nullptr, ckks);
     const uint64_t *diag = skipped ? in + lw : nullptr;
@@ -381,6 +461,7 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
             rb.key[q] = hoisted[r0 + q];
             rb.P[q] = Pm + (size_t)q * count * 2 * lw;
             rb.S[q] = Sm + (size_t)q * count * 2 * n;
+            rb.elt[q] = elts[r0 + q];
         }
         if (armed && r0 == 0) SHC_HIP(hipEventRecord(probe.begin, st));
         switch (nr) {
@@ -391,6 +472,14 @@ void rotate_hoisted(const Context &c, const uint64_t *in, uint32_t count, uint32
         }
         launch_check("rotation_products");
         if (armed && r0 == 0) SHC_HIP(hipEventRecord(probe.end, st));
+        const uint32_t fix_blocks = (uint32_t)((((size_t)(L + 1) << logn) + TPB - 1) / TPB);
+        switch (nr) {
+            case 1: zero_digit_fix_kernel<1><<<fix_blocks, TPB, 0, st>>>(rb, zc, tmpct, targets, c.primes(), L, (uint32_t)K, logn, count); break;
+            case 2: zero_digit_fix_kernel<2><<<fix_blocks, TPB, 0, st>>>(rb, zc, tmpct, targets, c.primes(), L, (uint32_t)K, logn, count); break;
+            case 3: zero_digit_fix_kernel<3><<<fix_blocks, TPB, 0, st>>>(rb, zc, tmpct, targets, c.primes(), L, (uint32_t)K, logn, count); break;
+            default: zero_digit_fix_kernel<4><<<fix_blocks, TPB, 0, st>>>(rb, zc, tmpct, targets, c.primes(), L, (uint32_t)K, logn, count); break;
+        }
+        launch_check("zero_digit_fix");
         for (uint32_t q = 0; q < nr; q++) {
             const uint32_t r = r0 + q;
             uint64_t *dst = sum ? out : out + r * ct_words;

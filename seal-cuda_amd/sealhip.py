@@ -69,6 +69,7 @@ def lib():
         L.shc_batch_decode.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.shc_ckks_encode_workspace_size.argtypes = [_vp, ctypes.POINTER(sz)]
         L.shc_ckks_encode.argtypes = [_vp, _vp, u32, ctypes.c_double, u32, _vp, _vp, sz, _vp]
+        L.shc_ckks_encode_deferred.argtypes = [_vp, _vp, u32, ctypes.c_double, u32, _vp, _vp, _vp, sz, _vp]
         L.shc_ckks_decode_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
         L.shc_ckks_decode.argtypes = [_vp, _vp, u32, ctypes.c_double, _vp, _vp, sz, _vp]
         L.shc_rescale_workspace_size.argtypes = [_vp, u32, u32, u32, ctypes.POINTER(sz)]
@@ -92,8 +93,34 @@ def lib():
         L.shc_rotate_many.argtypes = L.shc_rotate_sum.argtypes
         L.shc_multiply_plain_accumulate.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, _vp]
         L.shc_multiply_plain_accumulate_many.argtypes = [_vp, _vp, _vp, u32, u32, u32, u32, _vp, _vp]
+        L.shc_server_create.argtypes = [ctypes.c_int, sz, sz, u32, u32, ctypes.POINTER(_vp)]
+        L.shc_server_destroy.argtypes = [_vp]
+        L.shc_server_destroy.restype = None
+        L.shc_server_add_matvec.argtypes = [_vp, _vp, u32, u32, u32, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                            ctypes.POINTER(_vp), ctypes.POINTER(u32)]
+        L.shc_server_add_mulrelin.argtypes = [_vp, _vp, u32, _vp, ctypes.c_int, ctypes.POINTER(u32)]
+        L.shc_server_request_size.argtypes = [_vp, u32, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        L.shc_server_submit.argtypes = [_vp, u32, _vp, _vp, ctypes.POINTER(u64)]
+        L.shc_server_wait.argtypes = [_vp, u64, ctypes.c_double, ctypes.POINTER(RequestInfo)]
+        L.shc_server_clock.argtypes = [_vp]
+        L.shc_server_clock.restype = ctypes.c_double
+        L.shc_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
         _lib = L
     return _lib
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("batch_size", ctypes.c_uint32), ("lane", ctypes.c_uint32), ("batch_id", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("max_batches_in_flight", ctypes.c_uint32),
+                ("peak_device_bytes", ctypes.c_size_t)]
+
+
+SHC_TIMEOUT = 1
 
 
 def check(status):
@@ -273,6 +300,11 @@ class Context:
     def ckks_encode(self, d_values, count, scale, limbs, d_plain, d_ws, ws_bytes, stream=None):
         check(lib().shc_ckks_encode(self.handle, d_values, count, scale, limbs, d_plain, d_ws, ws_bytes, stream))
 
+    def ckks_encode_deferred(self, d_values, count, scale, limbs, d_plain, d_status, d_ws, ws_bytes, stream=None):
+        """Encode with the range check written to the int32 device word d_status."""
+        check(lib().shc_ckks_encode_deferred(self.handle, d_values, count, scale, limbs, d_plain, d_status, d_ws,
+                                             ws_bytes, stream))
+
     def ckks_decode_workspace_size(self, limbs):
         return self._size(lib().shc_ckks_decode_workspace_size, limbs)
 
@@ -350,3 +382,76 @@ class Context:
     def probe_keyswitch(self, begin, end):
         """Arm the next key switch to record HipEvents around its ModUp rows pass."""
         check(lib().shc_probe_keyswitch(self.handle, begin.handle, end.handle))
+
+
+class Server:
+    """The library's continuous-batching request server (shc_server_*): requests with
+    host numpy buffers (uint64), admitted by device-memory footprint into batches that
+    run on `lanes` streams at once.  The arrays passed to submit() are kept alive here
+    until the request is waited for."""
+
+    def __init__(self, device_bytes, staging_bytes, lanes=2, max_batch=16, device=0):
+        h = _vp()
+        check(lib().shc_server_create(device, device_bytes, staging_bytes, lanes, max_batch, ctypes.byref(h)))
+        self.handle = h
+        self._held = {}
+        self._keep = []
+
+    def close(self):
+        if self.handle:
+            lib().shc_server_destroy(self.handle)
+            self.handle = None
+            self._held.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_matvec(self, ctx, limbs, baby, giant, d_diagonals, baby_keys, baby_hoisted, giant_keys):
+        t = ctypes.c_uint32()
+        bk = (_vp * max(len(baby_keys), 1))(*baby_keys)
+        bh = (_vp * max(len(baby_hoisted), 1))(*baby_hoisted)
+        gk = (_vp * max(len(giant_keys), 1))(*giant_keys)
+        check(lib().shc_server_add_matvec(self.handle, ctx.handle, limbs, baby, giant, d_diagonals, bk, bh, gk,
+                                          ctypes.byref(t)))
+        self._keep.append(ctx)
+        return t.value
+
+    def add_mulrelin(self, ctx, limbs, d_relin_key, rescale=False):
+        t = ctypes.c_uint32()
+        check(lib().shc_server_add_mulrelin(self.handle, ctx.handle, limbs, d_relin_key, int(rescale), ctypes.byref(t)))
+        self._keep.append(ctx)
+        return t.value
+
+    def request_size(self, type_id):
+        """(input words, output words, device bytes of a batch of one)."""
+        i, o, d = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().shc_server_request_size(self.handle, type_id, ctypes.byref(i), ctypes.byref(o), ctypes.byref(d)))
+        return i.value, o.value, d.value
+
+    def submit(self, type_id, h_in, h_out):
+        """h_in, h_out: C-contiguous uint64 numpy arrays of the type's input / output words."""
+        rid = ctypes.c_uint64()
+        check(lib().shc_server_submit(self.handle, type_id, h_in.ctypes.data, h_out.ctypes.data, ctypes.byref(rid)))
+        self._held[rid.value] = h_out
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        """The request's RequestInfo once done, or None when `timeout` seconds passed first."""
+        info = RequestInfo()
+        st = lib().shc_server_wait(self.handle, rid, timeout, ctypes.byref(info))
+        if st == SHC_TIMEOUT:
+            return None
+        self._held.pop(rid, None)
+        check(st)
+        return info
+
+    def clock(self):
+        return lib().shc_server_clock(self.handle)
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().shc_server_get_stats(self.handle, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}

@@ -158,7 +158,7 @@ static void plaintext_strings_and_files() {
 }
 
 static void containers_and_keys() {
-    const size_t n = 4096;
+    const size_t n = 8192;  // 60 + 40 + 40 + 60 = 200 bits: within the 128-bit security bound (218)
     EncryptionParameters parms(SchemeType::ckks);
     parms.setPolyModulusDegree(n);
     parms.setCoeffModulus(CoeffModulus::Create(n, {60, 40, 40, 60}));

@@ -180,3 +180,56 @@ def test_ckks_chain_end_to_end():
     assert np.max(np.abs(got - ref)) < 2.0 ** -20
     # and the chain computed x * y^3 (CKKS noise at scale 2^40 after three rescales)
     assert np.max(np.abs(got - want)) < 2.0 ** -10
+
+
+def test_encode_deferred_check_and_graph_capture():
+    """shc_ckks_encode_deferred: the same plaintext words as shc_ckks_encode, the range
+    check as a device status word (0; -1 with a zero plaintext where shc_ckks_encode
+    raises), and a captured graph of it replays to the eager words."""
+    logn, n = 12, 1 << 12
+    moduli = ck.coeff_modulus_create(n, [60, 40, 40, 60])
+    L = 3
+    ctx = hip().Context(CKKS, n, moduli)
+    rng = np.random.default_rng(77)
+    v = slots(rng, n // 2)
+    dv = to_dev_f64(interleave(v))
+    ew = ctx.ckks_encode_workspace_size()
+    eb = ws(ew)
+    want = empty_dev((L, n))
+    ctx.ckks_encode(dv.data_ptr(), n // 2, 2.0 ** 40, L, want.data_ptr(), eb.data_ptr(), ew, stream())
+    got = empty_dev((L, n))
+    status = torch.full((1,), 7, dtype=torch.int32, device="cuda")
+    ctx.ckks_encode_deferred(dv.data_ptr(), n // 2, 2.0 ** 40, L, got.data_ptr(), status.data_ptr(), eb.data_ptr(),
+                             ew, stream())
+    assert int(status.item()) == 0
+    np.testing.assert_array_equal(from_dev(got), from_dev(want))
+    # values far beyond the modulus: shc_ckks_encode raises, the deferred form reports
+    big = to_dev_f64(interleave(v * 2.0 ** 100))
+    with pytest.raises(hip().SealHipError, match="too large"):
+        ctx.ckks_encode(big.data_ptr(), n // 2, 2.0 ** 40, L, want.data_ptr(), eb.data_ptr(), ew, stream())
+    ctx.ckks_encode_deferred(big.data_ptr(), n // 2, 2.0 ** 40, L, got.data_ptr(), status.data_ptr(), eb.data_ptr(),
+                             ew, stream())
+    assert int(status.item()) == -1
+    assert not from_dev(got).any()
+    # captured: replays give the eager words, with eager encodes between the replays
+    ctx.ckks_encode(dv.data_ptr(), n // 2, 2.0 ** 40, L, want.data_ptr(), eb.data_ptr(), ew, stream())
+    eager = from_dev(want).copy()
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(graph, stream=cap):
+        ctx.ckks_encode_deferred(dv.data_ptr(), n // 2, 2.0 ** 40, L, got.data_ptr(), status.data_ptr(),
+                                 eb.data_ptr(), ew, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.current_stream().wait_stream(cap)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        got.zero_()
+        status.fill_(7)
+        graph.replay()
+        torch.cuda.synchronize()
+        assert int(status.item()) == 0
+        np.testing.assert_array_equal(from_dev(got), eager)
+        ctx.ckks_encode_deferred(big.data_ptr(), n // 2, 2.0 ** 40, L, want.data_ptr(), status.data_ptr(),
+                                 eb.data_ptr(), ew, stream())
+        torch.cuda.synchronize()
+        assert int(status.item()) == -1
