@@ -185,12 +185,13 @@ def served_mulrelin(torch, sealhip, ctx, a, b, key, L, n, seconds):
         pair = torch.stack([a[i], b[i]]).cpu()
         pool.append(pair.reshape(-1).numpy().view(np.uint64).copy())
     ring, B = 24, 32
-    staging = ring * 3 * pool[0].nbytes // 2 + (64 << 20)
+    staging = 4 * pool[0].nbytes  # requests use pinned buffers in place; staging serves the self-check
     batched_sv = sealhip.Server(32 << 30, staging, lanes=2, max_batch=B)
     plain_sv = sealhip.Server(2 << 30, staging, lanes=1, max_batch=1)
     tb, tp = batched_sv.add_mulrelin(ctx, L, key.data_ptr()), plain_sv.add_mulrelin(ctx, L, key.data_ptr())
-    serve_open_loop(batched_sv, [(tb, pool)], 0, 8, seed=1, ring=ring, burst=True)
-    cap = serve_open_loop(batched_sv, [(tb, pool)], 0, 48, seed=2, ring=ring, burst=True)
+    pool_b, pool_p = pinned_copy(batched_sv, pool), pinned_copy(plain_sv, pool)
+    serve_open_loop(batched_sv, [(tb, pool_b)], 0, 8, seed=1, ring=ring, burst=True)
+    cap = serve_open_loop(batched_sv, [(tb, pool_b)], 0, 48, seed=2, ring=ring, burst=True)
     words = batched_sv.request_size(tb)[1]
     outs = [np.empty(words, dtype=np.uint64) for _ in range(8)]
     ids = [batched_sv.submit(tb, pool[i % 4], outs[i]) for i in range(8)]
@@ -202,15 +203,35 @@ def served_mulrelin(torch, sealhip, ctx, a, b, key, L, n, seconds):
         raise SystemExit("bench self-check failed: a served mul+relin request differs alone and in a batch")
     rate = 0.5 * cap["served_per_s"]
     reqs = max(100, int(rate * seconds))
-    res = {"server": "library (shc_server_*): host ciphertexts in and out, 2 lanes, batches of at most "
-                     f"{B}, 32 GiB device arena; plain = 1 lane, one request at a time",
+    res = {"server": "library (shc_server_*): host ciphertexts in and out (pinned buffers used in place), 2 "
+                     f"lanes, batches of at most {B}, 32 GiB device arena; plain = 1 lane, one request at a time",
            "capacity_per_s": cap["served_per_s"], "arrivals": "poisson", "rate_per_s": rate, "requests": reqs,
-           "batched": serve_open_loop(batched_sv, [(tb, pool)], rate, reqs, seed=7, ring=ring),
-           "plain": serve_open_loop(plain_sv, [(tp, pool)], rate, reqs, seed=7, ring=ring),
+           "batched": serve_open_loop(batched_sv, [(tb, pool_b)], rate, reqs, seed=7, ring=ring),
+           "plain": serve_open_loop(plain_sv, [(tp, pool_p)], rate, reqs, seed=7, ring=ring),
            "max_batches_in_flight": batched_sv.stats()["max_batches_in_flight"]}
     batched_sv.close()
     plain_sv.close()
     return res, alone
+
+
+def facade_leg(ops=128, warmup=16):
+    """The plain drop-in path as a reference program would call it: the troyn:: class API
+    (include/troyhip.hpp) one ciphertext at a time on the default stream
+    (evaluator.multiply(a, b, out); evaluator.relinearizeInplace(out, rk)), at the same
+    shape, timed by seal-cuda_amd/bench_facade (a separate process, built with the
+    library)."""
+    import subprocess
+    exe = os.path.join(HERE, "seal-cuda_amd", "bench_facade")
+    if not os.path.exists(exe):
+        return {"error": f"{exe} not built"}
+    r = subprocess.run([exe, str(ops), str(warmup)], capture_output=True, text=True, timeout=300)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"bench_facade exited {r.returncode}: {r.stderr[-300:]}"}
+    res = json.loads(lines[-1])
+    res["path"] = ("troyn::Evaluator multiply + relinearizeInplace per ciphertext (C++ API over the C ABI, "
+                   "default stream, pooled device blocks), CKKS N=2^16, 24+1 primes")
+    return res
 
 
 def bench_mulrelin(args):
@@ -280,6 +301,10 @@ def bench_mulrelin(args):
         plain = {"ops_s": B / (tp.ms * 1e-3), "ms_per_step": tp.ms, "calls": 2 * B,
                  "self_check": f"batched == per-ciphertext words on ciphertexts {probe}"}
 
+    facade = None
+    if not args.no_plain:
+        facade = facade_leg()
+
     served = None
     if not args.no_serve:
         served, served_words = served_mulrelin(torch, sealhip, ctx, a, b, key, L, n, args.serve_seconds)
@@ -329,6 +354,8 @@ def bench_mulrelin(args):
                           "algorithmic_bytes_per_op": unit_bytes},
         "plain": plain,
         "batched_over_plain": (ops_s / plain["ops_s"]) if plain else None,
+        "plain_facade": facade,
+        "batched_over_facade": (ops_s / facade["ops_s"]) if facade and "ops_s" in facade else None,
         "served_latency": served,
         "wall_s": wall,
     }
@@ -899,6 +926,16 @@ class MatvecResident:
                                  [k.data_ptr() for k in self.giant])
 
 
+def pinned_copy(server, arrays):
+    """Copies of host arrays in the server's pinned memory (used in place by requests)."""
+    out = []
+    for a in arrays:
+        p = server.host_array(a.size)
+        p[:] = a
+        out.append(p)
+    return out
+
+
 def serve_open_loop(server, kinds, rate, requests, seed, ring=64, burst=False):
     """Open-loop serving through the library's server.  kinds: [(type id, [host input
     arrays])]; request r picks a kind uniformly at random and an input from its list, and
@@ -907,14 +944,14 @@ def serve_open_loop(server, kinds, rate, requests, seed, ring=64, burst=False):
     pinned staging), and its latency runs from arrival to the server's completion, when
     the result has been copied back into host memory -- upload, queueing, batching,
     device work and download included.  At most `ring` requests are outstanding (their
-    output buffers are reused)."""
+    output buffers, pinned memory of the server, are reused)."""
     import collections
     import numpy as np
     rng = np.random.default_rng(seed)
     arrivals = np.zeros(requests) if burst else np.cumsum(rng.exponential(1.0 / rate, size=requests))
     kind = rng.integers(0, len(kinds), size=requests)
     pick = rng.integers(0, 1 << 30, size=requests)
-    outs = [[np.empty(server.request_size(t)[1], dtype=np.uint64) for _ in range(ring)] for t, _ in kinds]
+    outs = [[server.host_array(server.request_size(t)[1]) for _ in range(ring)] for t, _ in kinds]
     lat = np.empty(requests)
     bsz = np.empty(requests)
     done_at = np.empty(requests)
@@ -982,11 +1019,11 @@ def bench_serve(args):
     res = [MatvecResident(torch, sealhip, sealhip.SCHEME_BFV, 8192, bfv_moduli, 65537, 64, 64, 8, gen, st),
            MatvecResident(torch, sealhip, sealhip.SCHEME_CKKS, 1 << 15, ckks_moduli, 0, 32, 32, 8, gen, st)]
     ring = 64
-    staging = ring * max(2 * r.inputs[0].size * 8 for r in res) + (64 << 20)
+    staging = 8 * max(2 * r.inputs[0].size * 8 for r in res)  # the self-check's unpinned outputs
     batched_sv = sealhip.Server(arena, staging, lanes=lanes, max_batch=B)
     plain_sv = sealhip.Server(arena // 4, staging, lanes=1, max_batch=1)
-    kinds_b = [(r.register(batched_sv), r.inputs) for r in res]
-    kinds_p = [(r.register(plain_sv), r.inputs) for r in res]
+    kinds_b = [(r.register(batched_sv), pinned_copy(batched_sv, r.inputs)) for r in res]
+    kinds_p = [(r.register(plain_sv), pinned_copy(plain_sv, r.inputs)) for r in res]
     # warm up, then capacity per type with a burst of 4 full batches
     for t, pool in kinds_b:
         serve_open_loop(batched_sv, [(t, pool)], 0, 2 * B, seed=1, burst=True)

@@ -394,8 +394,16 @@ int shc_server_add_mulrelin(shc_server *server, const shc_context *ctx, uint32_t
 /* Words of one request's input and output, and the device bytes a batch of one takes. */
 int shc_server_request_size(const shc_server *server, uint32_t type, size_t *in_words, size_t *out_words,
                             size_t *device_bytes);
-/* Queues a request (blocks while the staging arena is full); *id names it. */
+/* Queues a request; *id names it.  A buffer inside a block from shc_server_host_alloc is
+ * used in place: the input is uploaded from it (it must stay unchanged until the request
+ * is done) and the output downloaded into it.  Other buffers go through the staging
+ * arena: the input is copied before the call returns (blocking while staging is full)
+ * and the output copied out before the request is reported done. */
 int shc_server_submit(shc_server *server, uint32_t type, const uint64_t *h_in, uint64_t *h_out, uint64_t *id);
+/* Pinned host memory (hipHostMalloc on this call) that requests can use in place; freed
+ * by shc_server_host_free or with the server. */
+int shc_server_host_alloc(shc_server *server, size_t bytes, void **ptr);
+int shc_server_host_free(shc_server *server, void *ptr);
 /* Waits up to timeout_s seconds (< 0: no limit) for request id: SHC_OK when done (the
  * request is then forgotten; info, if not NULL, describes it), SHC_TIMEOUT when not
  * yet, or the error the request's batch failed with. */

@@ -573,6 +573,17 @@ int shc_server_wait(shc_server *server, uint64_t id, double timeout_s, shc_reque
     return rc == SHC_OK && !done ? SHC_TIMEOUT : rc;
 }
 
+int shc_server_host_alloc(shc_server *server, size_t bytes, void **out) {
+    return guarded([&] {
+        if (!out) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        *out = server_of(server).host_alloc(bytes);
+    });
+}
+
+int shc_server_host_free(shc_server *server, void *ptr) {
+    return guarded([&] { server_of(server).host_free(ptr); });
+}
+
 double shc_server_clock(const shc_server *server) { return server && server->impl ? server->impl->now() : 0.0; }
 
 int shc_server_get_stats(const shc_server *server, shc_server_stats *out) {

@@ -208,6 +208,7 @@ Server::~Server() {
     staging_cv_.notify_all();
     for (auto &t : threads_) t.join();
     for (auto s : streams_) (void)hipStreamDestroy(s);
+    for (auto &b : host_blocks_) (void)hipHostFree(const_cast<char *>(b.first));
     (void)hipHostFree(h_staging_);
     (void)hipFree(d_arena_);
 }
@@ -252,6 +253,31 @@ const ServerProgram &Server::program(uint32_t type) const {
     return programs_[type];
 }
 
+void *Server::host_alloc(size_t bytes) {
+    if (!bytes) fail(ERR_INVALID_ARGUMENT, "host_alloc of zero bytes");
+    void *p = nullptr;
+    SHC_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    std::lock_guard<std::mutex> g(mu_);
+    host_blocks_[static_cast<const char *>(p)] = bytes;
+    return p;
+}
+
+void Server::host_free(void *p) {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = host_blocks_.find(static_cast<const char *>(p));
+    if (it == host_blocks_.end()) fail(ERR_INVALID_ARGUMENT, "not a block from this server's host_alloc");
+    SHC_HIP(hipHostFree(p));
+    host_blocks_.erase(it);
+}
+
+bool Server::pinned_range(const void *p, size_t bytes) const {
+    const char *c = static_cast<const char *>(p);
+    auto it = host_blocks_.upper_bound(c);
+    if (it == host_blocks_.begin()) return false;
+    --it;
+    return c >= it->first && c + bytes <= it->first + it->second;
+}
+
 uint64_t Server::submit(uint32_t type, const uint64_t *h_in, uint64_t *h_out) {
     if (!h_in || !h_out) fail(ERR_INVALID_ARGUMENT, "null request buffer");
     auto r = std::make_unique<Request>();
@@ -261,16 +287,25 @@ uint64_t Server::submit(uint32_t type, const uint64_t *h_in, uint64_t *h_out) {
     r->info.submit_s = now();
     if (type >= programs_.size()) fail(ERR_INVALID_ARGUMENT, "no such request type");
     const ServerProgram &p = programs_[type];
-    const size_t bytes = (p.in_words() + p.out_words()) * 8;
-    size_t off;
-    while ((off = staging_alloc_.take(bytes)) == SIZE_MAX) {
-        if (stopping_) fail(ERR_LOGIC, "server is stopping");
-        staging_cv_.wait(lk);
+    const size_t in_b = p.in_words() * 8, out_b = p.out_words() * 8;
+    const bool in_direct = pinned_range(h_in, in_b), out_direct = pinned_range(h_out, out_b);
+    const size_t bytes = (in_direct ? 0 : in_b) + (out_direct ? 0 : out_b);
+    if (bytes) {
+        size_t off;
+        while ((off = staging_alloc_.take(bytes)) == SIZE_MAX) {
+            if (stopping_) fail(ERR_LOGIC, "server is stopping");
+            staging_cv_.wait(lk);
+        }
+        r->staging_off = off;
     }
-    r->staging_off = off;
-    lk.unlock();
-    std::memcpy(h_staging_ + off, h_in, p.in_words() * 8);
-    lk.lock();
+    char *stage = r->staging_off == SIZE_MAX ? nullptr : h_staging_ + r->staging_off;
+    r->src = in_direct ? reinterpret_cast<const char *>(h_in) : stage;
+    r->dst = out_direct ? reinterpret_cast<char *>(h_out) : stage + (in_direct ? 0 : in_b);
+    if (!in_direct) {
+        lk.unlock();
+        std::memcpy(stage, h_in, in_b);
+        lk.lock();
+    }
     r->id = next_id_++;
     Request *raw = r.get();
     requests_[raw->id] = std::move(r);
@@ -366,12 +401,12 @@ void Server::lane_main(uint32_t lane) {
         // results leave staging for the callers' buffers outside the lock
         if (error.empty())
             for (Request *r : batch)
-                std::memcpy(r->h_out, h_staging_ + r->staging_off + p.in_words() * 8, p.out_words() * 8);
+                if (r->dst != reinterpret_cast<char *>(r->h_out)) std::memcpy(r->h_out, r->dst, p.out_words() * 8);
         lk.lock();
         const double t_done = now();
         dev_alloc_.give(dev_off);
         for (Request *r : batch) {
-            staging_alloc_.give(r->staging_off);
+            if (r->staging_off != SIZE_MAX) staging_alloc_.give(r->staging_off);
             r->info.done_s = t_done;
             r->error = error;
             r->done = true;
@@ -395,7 +430,7 @@ void Server::run_batch(uint32_t, hipStream_t st, const ServerProgram &p, std::ve
     const uint32_t parts = p.kind == ServerProgram::MULRELIN ? 2 : 1;
     const size_t part_w = in_w / parts;
     for (uint32_t k = 0; k < m; k++) {
-        const char *src = h_staging_ + batch[k]->staging_off;
+        const char *src = batch[k]->src;
         for (uint32_t part = 0; part < parts; part++)
             SHC_HIP(hipMemcpyAsync(d_in + (size_t)part * m * part_w + k * part_w, src + part * part_w * 8,
                                    part_w * 8, hipMemcpyHostToDevice, st));
@@ -403,8 +438,7 @@ void Server::run_batch(uint32_t, hipStream_t st, const ServerProgram &p, std::ve
     size_t stride = 0;
     const uint64_t *res = run_program(p, d_in, scratch, m, st, &stride);
     for (uint32_t k = 0; k < m; k++)
-        SHC_HIP(hipMemcpyAsync(h_staging_ + batch[k]->staging_off + in_w * 8, res + k * stride, out_w * 8,
-                               hipMemcpyDeviceToHost, st));
+        SHC_HIP(hipMemcpyAsync(batch[k]->dst, res + k * stride, out_w * 8, hipMemcpyDeviceToHost, st));
     SHC_HIP(hipStreamSynchronize(st));
 }
 

@@ -85,8 +85,14 @@ public:
     ~Server();
     uint32_t add_program(ServerProgram p);
     const ServerProgram &program(uint32_t type) const;
-    // Copies h_in into staging and queues the request; blocks while staging is full.
+    // Queues a request.  An input or output buffer inside a block from host_alloc is
+    // used in place (uploaded from / downloaded into directly; the input must stay
+    // unchanged until the request is done); other buffers go through the staging arena
+    // (the input copied before submit returns; blocks while staging is full).
     uint64_t submit(uint32_t type, const uint64_t *h_in, uint64_t *h_out);
+    // Pinned host memory the server uploads from and downloads into without staging.
+    void *host_alloc(size_t bytes);
+    void host_free(void *p);
     // true when done (info filled), false on timeout; throws the request's error.
     bool wait(uint64_t id, double timeout_s, RequestInfo *info);
     double now() const;
@@ -97,11 +103,14 @@ private:
         uint64_t id;
         uint32_t type;
         uint64_t *h_out;
-        size_t staging_off;  // input then output words in the pinned arena
+        const char *src = nullptr;  // pinned input (a host_alloc block or staging)
+        char *dst = nullptr;        // pinned output (a host_alloc block or staging)
+        size_t staging_off = SIZE_MAX;  // staging block, if any
         RequestInfo info;
         bool done = false;
         std::string error;
     };
+    bool pinned_range(const void *p, size_t bytes) const;  // inside one host_alloc block
     void lane_main(uint32_t lane);
     void run_batch(uint32_t lane, hipStream_t st, const ServerProgram &p, std::vector<Request *> &batch,
                    size_t dev_off);
@@ -118,6 +127,7 @@ private:
     std::condition_variable work_cv_, done_cv_, staging_cv_;
     std::deque<Request *> pending_;
     std::map<uint64_t, std::unique_ptr<Request>> requests_;
+    std::map<const char *, size_t> host_blocks_;  // host_alloc blocks: address -> bytes
     uint64_t next_id_ = 1, next_batch_ = 1;
     uint32_t in_flight_ = 0;
     ServerStats stats_;

@@ -102,6 +102,8 @@ def lib():
         L.shc_server_request_size.argtypes = [_vp, u32, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)]
         L.shc_server_submit.argtypes = [_vp, u32, _vp, _vp, ctypes.POINTER(u64)]
         L.shc_server_wait.argtypes = [_vp, u64, ctypes.c_double, ctypes.POINTER(RequestInfo)]
+        L.shc_server_host_alloc.argtypes = [_vp, sz, ctypes.POINTER(_vp)]
+        L.shc_server_host_free.argtypes = [_vp, _vp]
         L.shc_server_clock.argtypes = [_vp]
         L.shc_server_clock.restype = ctypes.c_double
         L.shc_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
@@ -447,6 +449,18 @@ class Server:
         self._held.pop(rid, None)
         check(st)
         return info
+
+    def host_array(self, words):
+        """A uint64 numpy array of `words` in pinned memory from the server: requests
+        whose buffers are such arrays are uploaded from / downloaded into them in place
+        (an input must then stay unchanged until its request is done)."""
+        import numpy as np
+        p = _vp()
+        check(lib().shc_server_host_alloc(self.handle, words * 8, ctypes.byref(p)))
+        buf = (ctypes.c_uint64 * words).from_address(p.value)
+        arr = np.frombuffer(buf, dtype=np.uint64)
+        self._keep.append(buf)
+        return arr
 
     def clock(self):
         return lib().shc_server_clock(self.handle)

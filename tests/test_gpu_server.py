@@ -188,3 +188,30 @@ def test_server_rejects_requests_that_cannot_fit():
     with pytest.raises(sh.SealHipError, match="does not fit"):
         mr.register(server)
     server.close()
+
+
+def test_server_pinned_buffers_in_place():
+    """Inputs and outputs in the server's pinned memory are uploaded from and downloaded
+    into in place; the words equal those of staged (ordinary host) buffers."""
+    sh = hip()
+    mr = MulRelinType(4096, ck.coeff_modulus_create(4096, [60, 40, 40, 60]), seed=12)
+    server = sh.Server(device_bytes=1 << 30, staging_bytes=16 << 20, lanes=2, max_batch=4)
+    t = mr.register(server)
+    in_w, out_w, _ = server.request_size(t)
+    rng = np.random.default_rng(13)
+    xs = mr.inputs(rng, 6)
+    pins, pouts, ids = [], [], []
+    for x in xs:
+        p = server.host_array(in_w)
+        p[:] = x.reshape(-1)
+        o = server.host_array(out_w)
+        pins.append(p)
+        pouts.append(o)
+        ids.append(server.submit(t, p, o))
+    for i in ids:
+        assert server.wait(i, 60.0) is not None
+    staged, _ = serve(server, [(t, x) for x in xs], {t: out_w})
+    for o, st in zip(pouts, staged):
+        assert np.array_equal(o, st)
+    assert np.array_equal(pouts[0], mr.oracle(xs[0]).reshape(-1))
+    server.close()
