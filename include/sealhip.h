@@ -113,6 +113,19 @@ int shc_switch_key(const shc_context *ctx, const uint64_t *d_target, size_t targ
 int shc_relinearize(const shc_context *ctx, uint64_t *d_ct, const uint64_t *d_relin_key, uint32_t count,
                     uint32_t limb_count, void *d_workspace, size_t workspace_bytes, void *stream);
 
+/* shc_relinearize without a workspace argument: the key-switch scratch is a fixed
+ * 128 MiB device array inside the library (one per device, held until the process
+ * ends), so the call allocates nothing and takes no allocating library routine.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Fails with
+ * SHC_OUT_OF_RANGE when the key switch of `count` ciphertexts at this level needs more
+ * scratch than that (shc_keyswitch_workspace_size > 134217728 bytes; e.g. one
+ * ciphertext at N=2^16 with 24 data primes).  Every call on a device shares that
+ * device's array: calls that overlap on one device (other streams, other threads) race;
+ * the array keeps the last call's words, which no call reads before writing. */
+int shc_relinearize_noalloc(const shc_context *ctx, uint64_t *d_ct, const uint64_t *d_relin_key, uint32_t count,
+                            uint32_t limb_count, void *stream);
+
 /* GaloisTool::getEltFromStep (src/utils/galois_cuda.cu:44-86): 3^step mod 2N, a
  * negative step s maps to N/2 - |s|, step 0 gives 2N - 1 (complex conjugation). */
 int shc_galois_elt_from_step(uint32_t poly_modulus_degree, int step, uint32_t *elt);

@@ -8,6 +8,7 @@
 #include "galois.h"
 #include "rns_ops.h"
 #include "rotations.h"
+#include "scratch.h"
 #include "server.h"
 #include <algorithm>
 #include <string>
@@ -63,6 +64,7 @@ int shc_init(int device) {
         if (device < 0 || device >= count) shc::fail(shc::ERR_INVALID_ARGUMENT, "no such device");
         SHC_HIP(hipSetDevice(device));
         SHC_HIP(hipFree(nullptr));
+        shc::noalloc_scratch_setup();  // this device's address of the fixed scratch array
     });
 }
 
@@ -159,6 +161,15 @@ int shc_relinearize(const shc_context *ctx, uint64_t *ct, const uint64_t *key, u
         check_limbs(c, limbs);
         const size_t poly = (size_t)limbs * c.n;
         shc::switch_key(c, ct + 2 * poly, 3 * poly, ct, 3 * poly, key, count, limbs, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_relinearize_noalloc(const shc_context *ctx, uint64_t *ct, const uint64_t *key, uint32_t count,
+                            uint32_t limbs, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::relinearize_noalloc(c, ct, key, count, limbs, (hipStream_t)stream);
     });
 }
 

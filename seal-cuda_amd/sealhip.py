@@ -47,6 +47,7 @@ def lib():
         L.shc_keyswitch_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
         L.shc_switch_key.argtypes = [_vp, _vp, sz, _vp, sz, _vp, u32, u32, _vp, sz, _vp]
         L.shc_relinearize.argtypes = [_vp, _vp, _vp, u32, u32, _vp, sz, _vp]
+        L.shc_relinearize_noalloc.argtypes = [_vp, _vp, _vp, u32, u32, _vp]
         L.shc_galois_elt_from_step.argtypes = [u32, ctypes.c_int, ctypes.POINTER(u32)]
         L.shc_apply_galois.argtypes = [_vp, _vp, _vp, u32, u32, u32, ctypes.c_int, _vp]
         L.shc_galois_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
@@ -224,6 +225,10 @@ class Context:
 
     def relinearize(self, d_ct, d_key, count, limbs, d_ws, ws_bytes, stream=None):
         check(lib().shc_relinearize(self.handle, d_ct, d_key, count, limbs, d_ws, ws_bytes, stream))
+
+    def relinearize_noalloc(self, d_ct, d_key, count, limbs, stream=None):
+        """shc_relinearize with the library's fixed device scratch (no workspace)."""
+        check(lib().shc_relinearize_noalloc(self.handle, d_ct, d_key, count, limbs, stream))
 
     def galois_elt_from_step(self, step):
         e = ctypes.c_uint32()
