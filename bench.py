@@ -68,6 +68,7 @@ def parse():
     p.add_argument("--headline-only", action="store_true",
                    help="mulrelin: skip the other workloads' sub-objects in the JSON line")
     p.add_argument("--sub-steps", type=int, default=5, help="timed steps of each sub-workload in the default line")
+    p.add_argument("--no-tune", action="store_true", help="keep the built-in key-switch launch shapes")
     return p.parse_args()
 
 
@@ -270,6 +271,15 @@ def bench_mulrelin(args):
                               1, L, st)
             ctx.relinearize(out.data_ptr() + i * stride3, key.data_ptr(), 1, L, ws.data_ptr(), ws1_bytes, st)
 
+    # launch shapes of the key switch's ModUp kernels, chosen by the library's autotuner
+    # for this batch and for the per-ciphertext path (the products buffer is its scratch)
+    tune = None
+    if not args.no_tune:
+        ctx.ckks_multiply(a.data_ptr(), b.data_ptr(), out.data_ptr(), B, L, st)
+        tune = {"batched": ctx.autotune_relinearize(out.data_ptr(), key.data_ptr(), B, L, ws.data_ptr(), ws_bytes,
+                                                    reps=2, stream=st),
+                "per_ciphertext": ctx.autotune_relinearize(out.data_ptr(), key.data_ptr(), 1, L, ws.data_ptr(),
+                                                           ws_bytes, reps=8, stream=st)}
     for _ in range(args.warmup):
         step_batched()
     torch.cuda.synchronize()
@@ -317,6 +327,9 @@ def bench_mulrelin(args):
     # prime), those B x L input limbs, the key's 2 L (L+1) limbs and the L+1 twiddle
     # tables once, and writes the 2 B (L+1) product limbs; it runs the last 8 butterfly
     # layers of every ModUp limb and 2 B L (L+1) N multiply-adds.
+    grp = ctx.keyswitch_shape(L, B)[1] or 4
+    kname = {4: "4,1", 2: "2,1", 1: "1,2"}[grp]
+    valu_known = B == 128 and grp == 4  # the SQ_INSTS_VALU count was measured for groups of 4
     tlimbs = B * L * L
     rows_bytes = (tlimbs + B * L + 2 * L * (L + 1) + 2 * B * (L + 1)) * n * 8 + K * n * 16
     rows_bfly = tlimbs * (n // 2) * (logn - logn // 2)
@@ -338,9 +351,9 @@ def bench_mulrelin(args):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved_rows, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_rows / HBM_PEAK_GBS,
-                     "traffic": KEYPROD_TRAFFIC_B128 if B == 128 else None,
+                     "traffic": KEYPROD_TRAFFIC_B128 if valu_known else None,
                      "butterflies_per_launch": rows_bfly, "macs_per_launch": rows_macs,
-                     "kernel": f"ntt_rows_keyprod_kernel<16,8,4,16,4,1,false>: key-switch ModUp NTT rows pass over "
+                     "kernel": f"ntt_rows_keyprod_kernel<16,8,4,16,{kname},false>: key-switch ModUp NTT rows pass over "
                                f"{tlimbs} limbs (last {logn - logn // 2} layers) fused with the key products",
                      "algorithmic_bytes": rows_bytes, "avg_ms": rows_ms,
                      "butterflies_per_s": rows_bfly / (rows_ms * 1e-3),
@@ -349,12 +362,13 @@ def bench_mulrelin(args):
                                "achieved_per_s": KEYPROD_VALU_B128 / (rows_ms * 1e-3),
                                "peak_per_s": VALU_PEAK_WAVE_INSTR_S,
                                "frac": KEYPROD_VALU_B128 / (rows_ms * 1e-3) / VALU_PEAK_WAVE_INSTR_S}
-                              if B == 128 else None)},
+                              if valu_known else None)},
         "step_roofline": {"achieved": achieved_step, "unit": "GB/s", "frac": achieved_step / HBM_PEAK_GBS,
                           "algorithmic_bytes_per_op": unit_bytes},
         "plain": plain,
         "batched_over_plain": (ops_s / plain["ops_s"]) if plain else None,
         "plain_facade": facade,
+        "autotune": tune,
         "batched_over_facade": (ops_s / facade["ops_s"]) if facade and "ops_s" in facade else None,
         "served_latency": served,
         "wall_s": wall,

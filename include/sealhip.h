@@ -126,6 +126,36 @@ int shc_relinearize(const shc_context *ctx, uint64_t *d_ct, const uint64_t *d_re
 int shc_relinearize_noalloc(const shc_context *ctx, uint64_t *d_ct, const uint64_t *d_relin_key, uint32_t count,
                             uint32_t limb_count, void *stream);
 
+/* ---- launch-shape autotuner ----------------------------------------------------
+ * The key switch's two dominant kernels (the ModUp columns pass and the fused ModUp
+ * rows pass + key products) have launch-shape choices whose best value depends on the
+ * level, the batch and the degree: target primes per columns-pass block (1..L+1) and
+ * ciphertexts per rows-pass block (4, 2 or 1).  Zero means the built-in choice.  A
+ * context keeps a table of shapes by (limb_count, count); every key switch of a shape
+ * in the table uses it.  No counterpart in the reference, which launches every kernel
+ * as <<<ceil(n/256), 256>>> (src/kernelutils.cu:3). */
+typedef struct {
+    uint32_t cols_targets; /* target primes per ModUp columns-pass block; 0: built-in */
+    uint32_t rows_group;   /* ciphertexts per fused rows-pass block (4, 2, 1); 0: built-in */
+} shc_ks_shape;
+
+/* Times `reps` relinearizations of d_ct [count][3][L][N] (scratch: its words are
+ * overwritten) with each candidate shape -- the built-in choice and every
+ * (cols_targets, rows_group) pair -- and stores the fastest in the context's table for
+ * (limb_count, count) (the built-in choice unless another is more than 1% faster).  Synchronises `stream`; key switches on this context that run
+ * during the call may take any candidate shape (their words do not depend on it). */
+int shc_autotune_relinearize(const shc_context *ctx, uint64_t *d_ct, const uint64_t *d_relin_key, uint32_t count,
+                             uint32_t limb_count, void *d_workspace, size_t workspace_bytes, int reps, void *stream,
+                             shc_ks_shape *best, float *best_ms, float *builtin_ms);
+/* Sets (shape NULL or all zero: removes) / reads the table entry of (limb_count, count). */
+int shc_keyswitch_shape_set(const shc_context *ctx, uint32_t limb_count, uint32_t count, const shc_ks_shape *shape);
+int shc_keyswitch_shape_get(const shc_context *ctx, uint32_t limb_count, uint32_t count, shc_ks_shape *shape);
+/* The table as text, one line "limb_count count cols_targets rows_group" per entry:
+ * export writes it (NUL-terminated) to buf when capacity allows and its size with the
+ * NUL to *length; import adds a table written by export (lines are validated first). */
+int shc_keyswitch_shapes_export(const shc_context *ctx, char *buf, size_t capacity, size_t *length);
+int shc_keyswitch_shapes_import(const shc_context *ctx, const char *text);
+
 /* GaloisTool::getEltFromStep (src/utils/galois_cuda.cu:44-86): 3^step mod 2N, a
  * negative step s maps to N/2 - |s|, step 0 gives 2N - 1 (complex conjugation). */
 int shc_galois_elt_from_step(uint32_t poly_modulus_degree, int step, uint32_t *elt);

@@ -1,5 +1,6 @@
 // capi.cpp -- extern "C" entry points declared in include/sealhip.h.
 #include "../../include/sealhip.h"
+#include "autotune.h"
 #include "context.h"
 #include "ntt.h"
 #include "ckks_encoder.h"
@@ -11,6 +12,7 @@
 #include "scratch.h"
 #include "server.h"
 #include <algorithm>
+#include <cstring>
 #include <string>
 
 struct shc_context {
@@ -170,6 +172,59 @@ int shc_relinearize_noalloc(const shc_context *ctx, uint64_t *ct, const uint64_t
         const auto &c = ctx_of(ctx);
         check_limbs(c, limbs);
         shc::relinearize_noalloc(c, ct, key, count, limbs, (hipStream_t)stream);
+    });
+}
+
+int shc_autotune_relinearize(const shc_context *ctx, uint64_t *ct, const uint64_t *key, uint32_t count,
+                             uint32_t limbs, void *ws, size_t ws_bytes, int reps, void *stream, shc_ks_shape *best,
+                             float *best_ms, float *builtin_ms) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        const shc::TuneResult r = shc::autotune_relinearize(c, ct, key, count, limbs, ws, ws_bytes,
+                                                            (hipStream_t)stream, reps);
+        if (best) *best = shc_ks_shape{r.best.cols_targets, r.best.rows_group};
+        if (best_ms) *best_ms = r.best_ms;
+        if (builtin_ms) *builtin_ms = r.builtin_ms;
+    });
+}
+
+int shc_keyswitch_shape_set(const shc_context *ctx, uint32_t limbs, uint32_t count, const shc_ks_shape *shape) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        if (!shape) {
+            shc::keyswitch_shape_set(c, limbs, count, nullptr);
+            return;
+        }
+        const shc::KsShape s{shape->cols_targets, shape->rows_group};
+        shc::keyswitch_shape_set(c, limbs, count, &s);
+    });
+}
+
+int shc_keyswitch_shape_get(const shc_context *ctx, uint32_t limbs, uint32_t count, shc_ks_shape *shape) {
+    return guarded([&] {
+        if (!shape) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        const shc::KsShape s = ctx_of(ctx).ks_shape(limbs, count);
+        *shape = shc_ks_shape{s.cols_targets, s.rows_group};
+    });
+}
+
+int shc_keyswitch_shapes_export(const shc_context *ctx, char *buf, size_t capacity, size_t *length) {
+    return guarded([&] {
+        const std::string t = shc::keyswitch_shapes_export(ctx_of(ctx));
+        if (length) *length = t.size() + 1;
+        if (buf && capacity) {
+            if (capacity < t.size() + 1) shc::fail(shc::ERR_OUT_OF_RANGE, "buffer too small for the table");
+            std::memcpy(buf, t.c_str(), t.size() + 1);
+        }
+    });
+}
+
+int shc_keyswitch_shapes_import(const shc_context *ctx, const char *text) {
+    return guarded([&] {
+        if (!text) shc::fail(shc::ERR_INVALID_ARGUMENT, "null argument");
+        shc::keyswitch_shapes_import(ctx_of(ctx), text);
     });
 }
 

@@ -31,6 +31,16 @@ struct alignas(32) DivRoundConst {
     uint64_t pad;
 };
 
+// Launch shape of a key switch's two ModUp kernels (0 = the built-in choice):
+//   cols_targets  target primes one columns-pass block transforms a digit tile for
+//                 (1 .. L+1; the built-in choice aims at ~16k blocks);
+//   rows_group    ciphertexts one fused rows-pass block takes: 4 (then 2 and 1 for the
+//                 rest of the batch), 2 (then 1), or 1 (two rows of one ciphertext).
+struct KsShape {
+    uint32_t cols_targets = 0;
+    uint32_t rows_group = 0;
+};
+
 // Instrumentation: a pair of caller-created events recorded around one kernel launch.
 struct KernelProbe {
     hipEvent_t begin, end;

@@ -83,6 +83,16 @@ struct Context {
     mutable std::mutex hoisted_mu;
     mutable std::map<const void *, std::pair<uint32_t, uint32_t>> hoisted_keys;
 
+    // Key-switch launch shapes by (level, ciphertexts per launch), written by the
+    // autotuner (or set by the caller); shapes not in the table take the built-in choice.
+    mutable std::mutex tune_mu;
+    mutable std::map<std::pair<uint32_t, uint32_t>, KsShape> ks_shapes;
+    KsShape ks_shape(uint32_t L, uint32_t count) const {
+        std::lock_guard<std::mutex> g(tune_mu);
+        auto it = ks_shapes.find({L, count});
+        return it == ks_shapes.end() ? KsShape{} : it->second;
+    }
+
     size_t key_count() const { return moduli.size(); }
     const DevPrime *primes() const { return d_primes.as<DevPrime>(); }
     const DevPrime *keyswitch_primes(size_t level) const {

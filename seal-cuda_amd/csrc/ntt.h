@@ -46,9 +46,10 @@ void ntt_inverse_from(const uint64_t *src, LimbGrid src_grid, uint64_t *data, Li
 // false (and does nothing) for N < 2^13, where the caller gathers first and runs the
 // mapped transform.
 // With a probe, its events are recorded right before and after the rows-pass launch.
+// `shape` overrides the launch shape (KsShape; rows_group applies to the fused form).
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
                        const DevPrime *targets, const DevPrime *digits, int logn, bool skip_diag, bool reduce_4q,
-                       hipStream_t stream, const KernelProbe *probe = nullptr);
+                       hipStream_t stream, const KernelProbe *probe = nullptr, KsShape shape = KsShape{});
 
 // The same ModUp with its rows pass fused into the products by the key (N = 2^15, 2^16;
 // returns false and does nothing otherwise): the columns pass fills T as above (diag !=
@@ -59,7 +60,7 @@ bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint3
 bool ntt_modup_keyprod(const uint64_t *d, size_t d_ct_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
                        uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t count, uint32_t L,
                        uint32_t key_count, const DevPrime *targets, const DevPrime *digits, int logn,
-                       hipStream_t stream, const KernelProbe *probe = nullptr);
+                       hipStream_t stream, const KernelProbe *probe = nullptr, KsShape shape = KsShape{});
 
 // ModDown by the special prime p fused into the last transforms of a key switch, for
 // `count` ciphertexts at L limbs (N >= 2^13; returns false and does nothing otherwise):

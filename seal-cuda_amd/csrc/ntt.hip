@@ -767,7 +767,8 @@ void dispatch(uint64_t *data, LimbGrid lg, const DevPrime *primes, int logn, hip
 // Targets per block of the ModUp columns pass: all L + 1 when the (digit, tile) blocks
 // alone fill the GPU, fewer (chunks of them) so that a small batch still has about
 // 16384 blocks.
-inline uint32_t modup_targets_per_block(uint32_t tile_blocks, uint32_t L) {
+inline uint32_t modup_targets_per_block(uint32_t tile_blocks, uint32_t L, uint32_t forced = 0) {
+    if (forced) return forced < L + 1 ? forced : L + 1;
     const uint32_t want = (16384 + tile_blocks - 1) / tile_blocks;
     const uint32_t chunks = want < 1 ? 1 : (want > L + 1 ? L + 1 : want);
     return (L + chunks) / chunks;  // ceil((L + 1) / chunks)
@@ -775,13 +776,14 @@ inline uint32_t modup_targets_per_block(uint32_t tile_blocks, uint32_t L) {
 
 template <int LOGN>
 void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts, uint32_t L, const DevPrime *targets,
-                  const DevPrime *digits, bool skip_diag, bool reduce_4q, const KernelProbe *probe, hipStream_t st) {
+                  const DevPrime *digits, bool skip_diag, bool reduce_4q, const KernelProbe *probe, hipStream_t st,
+                  KsShape shape) {
     constexpr int K1 = LOGN / 2;
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
     constexpr int CS = 16;
     constexpr int RS = 4096 >> K2;
-    const uint32_t tc = modup_targets_per_block(cts * L * ((1u << K2) / CS), L);
+    const uint32_t tc = modup_targets_per_block(cts * L * ((1u << K2) / CS), L, shape.cols_targets);
     const uint32_t col_grid = cts * L * ((L + tc) / tc) * ((1u << K2) / CS);
     if (tc == 1)
         ntt_cols_modup_kernel<LOGN, K1, E, CS, true><<<col_grid, CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits,
@@ -804,16 +806,16 @@ void launch_modup(const uint64_t *d, size_t d_stride, uint64_t *T, uint32_t cts,
 
 bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint32_t count, uint32_t L,
                        const DevPrime *targets, const DevPrime *digits, int logn, bool skip_diag, bool reduce_4q,
-                       hipStream_t st, const KernelProbe *probe) {
+                       hipStream_t st, const KernelProbe *probe, KsShape shape) {
     if (logn < 13 || logn > 17 || count == 0) return false;
     if ((uint64_t)count * L * (L + 1) > ((uint64_t(1) << 34) >> logn))
         fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
     switch (logn) {
-        case 13: launch_modup<13>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
-        case 14: launch_modup<14>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
-        case 15: launch_modup<15>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
-        case 16: launch_modup<16>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
-        case 17: launch_modup<17>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st); break;
+        case 13: launch_modup<13>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st, shape); break;
+        case 14: launch_modup<14>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st, shape); break;
+        case 15: launch_modup<15>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st, shape); break;
+        case 16: launch_modup<16>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st, shape); break;
+        case 17: launch_modup<17>(d, d_ct_stride, T, count, L, targets, digits, skip_diag, reduce_4q, probe, st, shape); break;
     }
     launch_check("ntt modup launch");
     return true;
@@ -826,23 +828,27 @@ namespace {
 template <int LOGN, bool R4>
 void launch_keyprod_groups(const uint64_t *T, const uint64_t *key, uint64_t *P, uint64_t *S, const uint64_t *diag,
                            size_t diag_stride, uint32_t cts, uint32_t L, uint32_t K_key, const DevPrime *targets,
-                           uint32_t tiles, uint64_t tile_slots, hipStream_t st) {
+                           uint32_t tiles, uint64_t tile_slots, uint32_t group, hipStream_t st) {
     constexpr int K2 = LOGN - LOGN / 2;
     constexpr int E = 4;
     constexpr int RS = 4096 >> K2;
-    const uint32_t g4 = cts / 4, rest = cts % 4;
+    // groups of `group` ciphertexts (4 by default), then a group of 2, then a single one
+    const uint32_t g = group == 1 || group == 2 ? group : 4;
+    const uint32_t g4 = g == 4 ? cts / 4 : 0;
+    const uint32_t g2 = g == 1 ? 0 : (cts - 4 * g4) / 2;
+    const uint32_t singles = cts - 4 * g4 - 2 * g2;
     if (g4)
         ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 4, 1, R4><<<(uint32_t)(tile_slots * g4), RS << (K2 - E), 0, st>>>(
             T, key, P, S, targets, diag, diag_stride, cts, L, K_key, 0, g4, tiles);
-    if (rest >= 2)
-        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 2, 1, R4><<<(uint32_t)tile_slots, RS << (K2 - E), 0, st>>>(
-            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, g4 * 4, 1, tiles);
-    if (rest & 1) {
-        // the single ciphertext: two rows per block
+    if (g2)
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 2, 1, R4><<<(uint32_t)(tile_slots * g2), RS << (K2 - E), 0, st>>>(
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, g4 * 4, g2, tiles);
+    if (singles) {
+        // single ciphertexts: two rows per block
         const uint32_t tiles2 = tiles / 2;
         const uint32_t slots2 = (tiles2 + 7) / 8 * 8;
-        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 1, 2, R4><<<slots2, RS << (K2 - E), 0, st>>>(
-            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, cts - 1, 1, tiles2);
+        ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 1, 2, R4><<<slots2 * singles, RS << (K2 - E), 0, st>>>(
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, cts - singles, singles, tiles2);
     }
 }
 
@@ -850,12 +856,12 @@ template <int LOGN>
 void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
                           uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t cts, uint32_t L,
                           uint32_t K_key, const DevPrime *targets, const DevPrime *digits, const KernelProbe *probe,
-                          hipStream_t st) {
+                          hipStream_t st, KsShape shape) {
     constexpr int K1 = LOGN / 2;
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
     constexpr int CS = 16;
-    const uint32_t tc = modup_targets_per_block(cts * L * ((1u << K2) / CS), L);
+    const uint32_t tc = modup_targets_per_block(cts * L * ((1u << K2) / CS), L, shape.cols_targets);
     const uint32_t col_grid = cts * L * ((L + tc) / tc) * ((1u << K2) / CS);
     if (tc == 1)
         ntt_cols_modup_kernel<LOGN, K1, E, CS, true><<<col_grid, CS << (K1 - E), 0, st>>>(d, d_stride, T, targets, digits,
@@ -865,16 +871,16 @@ void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const
                                                                                         cts, L, diag != nullptr, tc);
     const uint32_t tiles = (L + 1) << K1;
     const uint64_t tile_slots = (uint64_t)((tiles + 7) / 8) * 8;
-    if (tile_slots * ((cts + 3) / 4) >= (uint64_t(1) << 32))
+    if (tile_slots * cts >= (uint64_t(1) << 32))
         fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
     if (probe) SHC_HIP(hipEventRecord(probe->begin, st));
     // up to 31 digits the transformed words (< 8q) enter the sums unreduced
     if (L > 31)
         launch_keyprod_groups<LOGN, true>(T, key, P, S, diag, diag_stride, cts, L, K_key, targets, tiles, tile_slots,
-                                          st);
+                                          shape.rows_group, st);
     else
         launch_keyprod_groups<LOGN, false>(T, key, P, S, diag, diag_stride, cts, L, K_key, targets, tiles,
-                                           tile_slots, st);
+                                           tile_slots, shape.rows_group, st);
     if (probe) SHC_HIP(hipEventRecord(probe->end, st));
 }
 
@@ -887,16 +893,16 @@ LimbGrid poly_grid(uint32_t count, uint32_t lpp) {
 bool ntt_modup_keyprod(const uint64_t *d, size_t d_ct_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
                        uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t count, uint32_t L,
                        uint32_t key_count, const DevPrime *targets, const DevPrime *digits, int logn,
-                       hipStream_t st, const KernelProbe *probe) {
+                       hipStream_t st, const KernelProbe *probe, KsShape shape) {
     if ((logn != 15 && logn != 16) || count == 0) return false;
     if ((uint64_t)count * L * (L + 1) > ((uint64_t(1) << 34) >> logn))
         fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
     if (logn == 15)
         launch_modup_keyprod<15>(d, d_ct_stride, T, key, P, S, diag, diag_stride, count, L, key_count, targets,
-                                 digits, probe, st);
+                                 digits, probe, st, shape);
     else
         launch_modup_keyprod<16>(d, d_ct_stride, T, key, P, S, diag, diag_stride, count, L, key_count, targets,
-                                 digits, probe, st);
+                                 digits, probe, st, shape);
     launch_check("ntt modup keyprod launch");
     return true;
 }

@@ -323,8 +323,9 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
     }
     // 2-4. ModUp and the inner product with the key: fused past the columns pass where
     // the transform shape allows (N = 2^15, 2^16), else T goes through HBM once more
+    const KsShape shape = c.ks_shape(L, count);
     if (!ntt_modup_keyprod(digits, dstride, T, key, P, S, diag, target_stride, count, L, (uint32_t)c.key_count(),
-                           c.keyswitch_primes(L), c.primes(), logn, st, probe)) {
+                           c.keyswitch_primes(L), c.primes(), logn, st, probe, shape)) {
         const bool skipped = modup_to_T(c, digits, dstride, T, count, L, st, probe, diag != nullptr);
         keyswitch_inner_product(c, T, key, P, S, count, L, st, skipped ? diag : nullptr, target_stride);
     }
@@ -341,7 +342,8 @@ bool modup_to_T(const Context &c, const uint64_t *digits, size_t dstride, uint64
     // T's consumers (inner products with the key, hoisted rotation products) sum L
     // products T * key (key < q < 2^60) and one more term < q in 128 bits: T may stay
     // in [0, 8q) up to L = 31 (31 * 8q^2 < 2^127.95), in [0, 4q) beyond.
-    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, skip_diag, L > 31, st, probe))
+    if (ntt_forward_modup(digits, dstride, T, count, L, targets, c.primes(), logn, skip_diag, L > 31, st, probe,
+                          c.ks_shape(L, count)))
         return skip_diag;
     const size_t twords = ((size_t)count * L * (L + 1)) << logn;
     modup_gather_kernel<<<blocks_for(twords), TPB, 0, st>>>(digits, dstride, T, targets, c.primes(), count, L, logn,

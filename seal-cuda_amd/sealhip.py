@@ -94,6 +94,13 @@ def lib():
         L.shc_rotate_many.argtypes = L.shc_rotate_sum.argtypes
         L.shc_multiply_plain_accumulate.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, _vp]
         L.shc_multiply_plain_accumulate_many.argtypes = [_vp, _vp, _vp, u32, u32, u32, u32, _vp, _vp]
+        L.shc_autotune_relinearize.argtypes = [_vp, _vp, _vp, u32, u32, _vp, sz, ctypes.c_int, _vp,
+                                               ctypes.POINTER(KsShape), ctypes.POINTER(ctypes.c_float),
+                                               ctypes.POINTER(ctypes.c_float)]
+        L.shc_keyswitch_shape_set.argtypes = [_vp, u32, u32, ctypes.POINTER(KsShape)]
+        L.shc_keyswitch_shape_get.argtypes = [_vp, u32, u32, ctypes.POINTER(KsShape)]
+        L.shc_keyswitch_shapes_export.argtypes = [_vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+        L.shc_keyswitch_shapes_import.argtypes = [_vp, ctypes.c_char_p]
         L.shc_server_create.argtypes = [ctypes.c_int, sz, sz, u32, u32, ctypes.POINTER(_vp)]
         L.shc_server_destroy.argtypes = [_vp]
         L.shc_server_destroy.restype = None
@@ -110,6 +117,10 @@ def lib():
         L.shc_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
         _lib = L
     return _lib
+
+
+class KsShape(ctypes.Structure):
+    _fields_ = [("cols_targets", ctypes.c_uint32), ("rows_group", ctypes.c_uint32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -385,6 +396,37 @@ class Context:
     def multiply_plain_accumulate_many(self, d_cts, d_plains, terms, outputs, polys, limbs, d_out, stream=None):
         check(lib().shc_multiply_plain_accumulate_many(self.handle, d_cts, d_plains, terms, outputs, polys, limbs,
                                                        d_out, stream))
+
+    def autotune_relinearize(self, d_ct, d_key, count, limbs, d_ws, ws_bytes, reps=2, stream=None):
+        """Times relinearizations of d_ct (scratch) with every launch shape and keeps the
+        fastest for (limbs, count); returns {"best": (cols_targets, rows_group),
+        "best_ms": ..., "builtin_ms": ...}."""
+        best, bms, dms = KsShape(), ctypes.c_float(), ctypes.c_float()
+        check(lib().shc_autotune_relinearize(self.handle, d_ct, d_key, count, limbs, d_ws, ws_bytes, reps, stream,
+                                             ctypes.byref(best), ctypes.byref(bms), ctypes.byref(dms)))
+        return {"best": (best.cols_targets, best.rows_group), "best_ms": bms.value, "builtin_ms": dms.value}
+
+    def keyswitch_shape(self, limbs, count, shape=None, clear=False):
+        """The launch shape used for (limbs, count); sets it first when shape =
+        (cols_targets, rows_group) is given, removes it with clear=True."""
+        if clear:
+            check(lib().shc_keyswitch_shape_set(self.handle, limbs, count, None))
+        elif shape is not None:
+            s = KsShape(*shape)
+            check(lib().shc_keyswitch_shape_set(self.handle, limbs, count, ctypes.byref(s)))
+        out = KsShape()
+        check(lib().shc_keyswitch_shape_get(self.handle, limbs, count, ctypes.byref(out)))
+        return (out.cols_targets, out.rows_group)
+
+    def keyswitch_shapes_export(self):
+        n = ctypes.c_size_t()
+        check(lib().shc_keyswitch_shapes_export(self.handle, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        check(lib().shc_keyswitch_shapes_export(self.handle, buf, n.value, ctypes.byref(n)))
+        return buf.value.decode()
+
+    def keyswitch_shapes_import(self, text):
+        check(lib().shc_keyswitch_shapes_import(self.handle, text.encode()))
 
     def probe_keyswitch(self, begin, end):
         """Arm the next key switch to record HipEvents around its ModUp rows pass."""
