@@ -570,10 +570,38 @@ def bench_chain(args):
                 ctx.rescale_to_next(rels[s].data_ptr(), accs[s].data_ptr(), c, 2, ll, ws.data_ptr(), max_ws, st)
                 cur = accs[s]
 
+    # the same chain with every level group in one launch per stage (shc_*_groups): per
+    # round, one grouped multiply, relinearize and rescale (read straight from the
+    # size-3 products)
+    order = sorted(bufs)
+    gw = 0
+    for s in range(3):
+        pr = [(bufs[l][3][s].data_ptr(), bufs[l][0], l - s) for l in order]
+        gw = max(gw, ctx.relinearize_groups_workspace_size(pr), ctx.rescale_groups_workspace_size(pr))
+    gws = torch.empty(((gw + 7) // 8,), dtype=torch.int64, device="cuda")
+
+    def chain_grouped(st):
+        for s in range(3):
+            cur = [(bufs[l][1] if s == 0 else bufs[l][4][s - 1]).data_ptr() for l in order]
+            a = [(p_, bufs[l][0], l - s) for p_, l in zip(cur, order)]
+            b = [(bufs[l][2][s].data_ptr(), bufs[l][0], l - s) for l in order]
+            pr = [(bufs[l][3][s].data_ptr(), bufs[l][0], l - s) for l in order]
+            out = [(bufs[l][4][s].data_ptr(), bufs[l][0], l - s - 1) for l in order]
+            ctx.ckks_multiply_groups(a, b, pr, st)
+            ctx.relinearize_groups(pr, key.data_ptr(), gws.data_ptr(), gw, st)
+            ctx.rescale_to_next_groups(pr, 3, out, gws.data_ptr(), gw, st)
+
     st = torch.cuda.current_stream().cuda_stream
     chain(st)
     torch.cuda.synchronize()
     eager = {l: b[4][2].clone() for l, b in bufs.items()}
+    for b in bufs.values():
+        b[4][2].zero_()
+    chain_grouped(st)
+    torch.cuda.synchronize()
+    for l, b in bufs.items():
+        if not torch.equal(b[4][2], eager[l]):
+            raise SystemExit("bench self-check failed: the grouped chain differs from the per-level chain")
     # dominant kernel: the fused ModUp rows pass + key products of the largest level's
     # first key switch, bracketed by HIP events on the library's launch stream (eager)
     top = max(groups)
@@ -608,8 +636,35 @@ def bench_chain(args):
     with Timer(torch) as tg:
         for _ in range(args.steps):
             graph.replay()
+    per_level_ms = tg.ms / args.steps
+    # the grouped chain: eager, then captured; every replay must give the same words
+    with Timer(torch) as tge:
+        for _ in range(args.steps):
+            chain_grouped(st)
+    grouped_eager_ms = tge.ms / args.steps
+    ggraph = torch.cuda.CUDAGraph()
+    cap2 = torch.cuda.Stream()
+    cap2.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cap2):
+        with torch.cuda.graph(ggraph, stream=cap2):
+            chain_grouped(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.current_stream().wait_stream(cap2)
+    for _ in range(2):
+        for b in bufs.values():
+            b[4][2].zero_()
+        ggraph.replay()
+        torch.cuda.synchronize()
+        for l, b in bufs.items():
+            if not torch.equal(b[4][2], eager[l]):
+                raise SystemExit("bench self-check failed: a grouped graph replay differs from the eager chain")
+        chain(st)
+        torch.cuda.synchronize()
+    with Timer(torch) as tgg:
+        for _ in range(args.steps):
+            ggraph.replay()
+    grouped_ms = tgg.ms / args.steps
     wall = time.perf_counter() - t0
-    ms = tg.ms / args.steps
+    ms = min(per_level_ms, grouped_ms)
     ops = pairs * 3
     limb = n * 8
     # step: per (group, round) at level m, inputs 4m limbs and output 2(m-1) limbs per
@@ -636,9 +691,13 @@ def bench_chain(args):
            "step_roofline": {"achieved": step_bytes / (ms * 1e-3) / 1e9, "unit": "GB/s",
                              "frac": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                              "algorithmic_bytes_per_step": step_bytes},
-           "eager_ms_per_step": eager_ms, "graph_over_eager": eager_ms / ms,
-           "self_check": "every graph replay equals the eager chain's words", "wall_s": wall}
-    del bufs, ws, key
+           "eager_ms_per_step": eager_ms, "graph_over_eager": eager_ms / per_level_ms,
+           "per_level_graph_ms": per_level_ms, "grouped_graph_ms": grouped_ms, "grouped_eager_ms": grouped_eager_ms,
+           "path": ("level groups in one launch per stage (shc_*_groups), HIP graph" if grouped_ms <= per_level_ms
+                    else "one batched call per level group and stage, HIP graph"),
+           "self_check": "every graph replay (per-level and grouped) equals the eager per-level chain's words",
+           "wall_s": wall}
+    del bufs, ws, gws, key, graph, ggraph
     torch.cuda.empty_cache()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_chain(moduli, logn, [int(x) for x in levels], args.cpu_seconds)
@@ -1311,7 +1370,7 @@ def bench_default(args):
             r = fn(sub)
             d = {k: r[k] for k in SUB_KEYS if k in r}
             for k in ("batched", "plain", "capacity_per_s", "admission", "hoisted_over_separate", "graph_over_eager",
-                      "latency_ms", "self_check"):
+                      "latency_ms", "self_check", "per_level_graph_ms", "grouped_graph_ms", "path"):
                 if k in r:
                     d[k] = r[k]
         except (RuntimeError, MemoryError, OSError) as e:

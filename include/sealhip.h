@@ -156,6 +156,36 @@ int shc_keyswitch_shape_get(const shc_context *ctx, uint32_t limb_count, uint32_
 int shc_keyswitch_shapes_export(const shc_context *ctx, char *buf, size_t capacity, size_t *length);
 int shc_keyswitch_shapes_import(const shc_context *ctx, const char *text);
 
+/* ---- level groups -------------------------------------------------------------
+ * A batch of ciphertexts at uneven levels, as groups of one level each (a work table
+ * of up to 16 groups per launch; more are taken 16 at a time).  Each grouped call gives
+ * every group the words of the one-level call on it, and runs each stage with one
+ * launch for all groups: CKKS at N = 2^15 and 2^16 (the fused key switch); other
+ * degrees and BFV key switches run group by group.  No counterpart in the reference
+ * (one ciphertext per call, src/evaluator_cuda.cu:262-281). */
+typedef struct {
+    uint64_t *d_ct;      /* the group's ciphertexts back to back */
+    uint32_t count;      /* ciphertexts in the group */
+    uint32_t limb_count; /* their level (data primes) */
+} shc_ct_group;
+
+/* CKKS tensor products of groups: out[g] = a[g] x b[g] (size 2 x size 2 -> size 3). */
+int shc_ckks_multiply_groups(const shc_context *ctx, const shc_ct_group *a, const shc_ct_group *b,
+                             const shc_ct_group *out, uint32_t ngroups, void *stream);
+/* Relinearize size-3 ciphertext groups in place (as shc_relinearize on each group). */
+int shc_relinearize_groups_workspace_size(const shc_context *ctx, const shc_ct_group *cts, uint32_t ngroups,
+                                          size_t *bytes);
+int shc_relinearize_groups(const shc_context *ctx, const shc_ct_group *cts, uint32_t ngroups,
+                           const uint64_t *d_relin_key, void *d_workspace, size_t workspace_bytes, void *stream);
+/* Rescale / modulus-switch polys 0 and 1 of each ciphertext of in[g] (in_size polys per
+ * ciphertext in memory: 2, or 3 straight after a relinearization) into out[g] (size 2,
+ * one level down; must not overlap in). */
+int shc_rescale_groups_workspace_size(const shc_context *ctx, const shc_ct_group *in, uint32_t ngroups,
+                                      size_t *bytes);
+int shc_rescale_to_next_groups(const shc_context *ctx, const shc_ct_group *in, uint32_t in_size,
+                               const shc_ct_group *out, uint32_t ngroups, void *d_workspace, size_t workspace_bytes,
+                               void *stream);
+
 /* GaloisTool::getEltFromStep (src/utils/galois_cuda.cu:44-86): 3^step mod 2N, a
  * negative step s maps to N/2 - |s|, step 0 gives 2N - 1 (complex conjugation). */
 int shc_galois_elt_from_step(uint32_t poly_modulus_degree, int step, uint32_t *elt);
@@ -263,6 +293,20 @@ int shc_ckks_encode(const shc_context *ctx, const double *d_values, uint32_t cou
 int shc_ckks_encode_deferred(const shc_context *ctx, const double *d_values, uint32_t count, double scale,
                              uint32_t limb_count, uint64_t *d_plain, int32_t *d_status, void *d_workspace,
                              size_t workspace_bytes, void *stream);
+/* Batches of plaintexts, one launch per FFT layer for the whole batch: plaintext p's
+ * `count` (re, im) pairs at d_values + p * values_stride (doubles), its NTT-form words at
+ * d_plain + p * limb_count * N.  d_status: NULL raises the range check like
+ * shc_ckks_encode (one wait for the batch), else one int32 per plaintext as
+ * shc_ckks_encode_deferred.  Decode writes plaintext p's N/2 pairs at d_values + p * N.
+ * No counterpart in the reference (one plaintext per call). */
+int shc_ckks_encode_many_workspace_size(const shc_context *ctx, uint32_t plaintexts, size_t *bytes);
+int shc_ckks_encode_many(const shc_context *ctx, const double *d_values, uint32_t count, size_t values_stride,
+                         uint32_t plaintexts, double scale, uint32_t limb_count, uint64_t *d_plain,
+                         int32_t *d_status, void *d_workspace, size_t workspace_bytes, void *stream);
+int shc_ckks_decode_many_workspace_size(const shc_context *ctx, uint32_t limb_count, uint32_t plaintexts,
+                                        size_t *bytes);
+int shc_ckks_decode_many(const shc_context *ctx, const uint64_t *d_plain, uint32_t limb_count, uint32_t plaintexts,
+                         double scale, double *d_values, void *d_workspace, size_t workspace_bytes, void *stream);
 int shc_ckks_decode_workspace_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
 int shc_ckks_decode(const shc_context *ctx, const uint64_t *d_plain, uint32_t limb_count, double scale,
                     double *d_values, void *d_workspace, size_t workspace_bytes, void *stream);

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 struct shc_context {
     std::unique_ptr<shc::Context> impl;
@@ -228,6 +229,89 @@ int shc_keyswitch_shapes_import(const shc_context *ctx, const char *text) {
     });
 }
 
+static void check_groups(const void *p, uint32_t n) {
+    if (n && !p) shc::fail(shc::ERR_INVALID_ARGUMENT, "null group array");
+}
+
+int shc_ckks_multiply_groups(const shc_context *ctx, const shc_ct_group *a, const shc_ct_group *b,
+                             const shc_ct_group *out, uint32_t ngroups, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_groups(a, ngroups);
+        check_groups(b, ngroups);
+        check_groups(out, ngroups);
+        std::vector<shc::TensorGroup> g(ngroups);
+        for (uint32_t i = 0; i < ngroups; i++) {
+            if (a[i].count != b[i].count || a[i].count != out[i].count || a[i].limb_count != b[i].limb_count ||
+                a[i].limb_count != out[i].limb_count)
+                shc::fail(shc::ERR_INVALID_ARGUMENT, "groups of a, b and out differ in count or level");
+            check_limbs(c, a[i].limb_count);
+            g[i] = shc::TensorGroup{a[i].d_ct, b[i].d_ct, out[i].d_ct, a[i].count, a[i].limb_count};
+        }
+        shc::ckks_tensor_groups(c, g.data(), ngroups, (hipStream_t)stream);
+    });
+}
+
+static std::vector<shc::RelinGroup> relin_groups(const shc::Context &c, const shc_ct_group *cts, uint32_t n) {
+    check_groups(cts, n);
+    std::vector<shc::RelinGroup> g(n);
+    for (uint32_t i = 0; i < n; i++) {
+        check_limbs(c, cts[i].limb_count);
+        g[i] = shc::RelinGroup{cts[i].d_ct, cts[i].count, cts[i].limb_count};
+    }
+    return g;
+}
+
+int shc_relinearize_groups_workspace_size(const shc_context *ctx, const shc_ct_group *cts, uint32_t ngroups,
+                                          size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        const auto g = relin_groups(c, cts, ngroups);
+        *bytes = shc::relinearize_groups_workspace(c, g.data(), ngroups);
+    });
+}
+
+int shc_relinearize_groups(const shc_context *ctx, const shc_ct_group *cts, uint32_t ngroups, const uint64_t *key,
+                           void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        const auto g = relin_groups(c, cts, ngroups);
+        shc::relinearize_groups(c, g.data(), ngroups, key, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+static std::vector<shc::RescaleGroup> rescale_groups_of(const shc::Context &c, const shc_ct_group *in,
+                                                        const shc_ct_group *out, uint32_t n) {
+    check_groups(in, n);
+    std::vector<shc::RescaleGroup> g(n);
+    for (uint32_t i = 0; i < n; i++) {
+        check_limbs(c, in[i].limb_count);
+        if (out && (out[i].count != in[i].count || out[i].limb_count + 1 != in[i].limb_count))
+            shc::fail(shc::ERR_INVALID_ARGUMENT, "an output group must hold the same count one level down");
+        g[i] = shc::RescaleGroup{in[i].d_ct, out ? out[i].d_ct : nullptr, in[i].count, in[i].limb_count};
+    }
+    return g;
+}
+
+int shc_rescale_groups_workspace_size(const shc_context *ctx, const shc_ct_group *in, uint32_t ngroups,
+                                      size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        const auto g = rescale_groups_of(c, in, nullptr, ngroups);
+        *bytes = shc::rescale_groups_workspace(c, g.data(), ngroups);
+    });
+}
+
+int shc_rescale_to_next_groups(const shc_context *ctx, const shc_ct_group *in, uint32_t in_size,
+                               const shc_ct_group *out, uint32_t ngroups, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_groups(out, ngroups);
+        const auto g = rescale_groups_of(c, in, out, ngroups);
+        shc::rescale_groups(c, g.data(), ngroups, in_size, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
 int shc_galois_elt_from_step(uint32_t n, int step, uint32_t *elt) {
     return guarded([&] {
         if (n < 2 || (n & (n - 1))) shc::fail(shc::ERR_INVALID_ARGUMENT, "poly_modulus_degree must be a power of two");
@@ -392,6 +476,38 @@ int shc_ckks_encode_deferred(const shc_context *ctx, const double *values, uint3
         const auto &c = ctx_of(ctx);
         check_limbs(c, limbs);
         shc::ckks_encode_deferred(c, values, count, scale, limbs, plain, d_status, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_ckks_encode_many_workspace_size(const shc_context *ctx, uint32_t plaintexts, size_t *bytes) {
+    return guarded([&] { *bytes = shc::ckks_encode_workspace_many(ctx_of(ctx), plaintexts); });
+}
+
+int shc_ckks_encode_many(const shc_context *ctx, const double *values, uint32_t count, size_t values_stride,
+                         uint32_t plaintexts, double scale, uint32_t limbs, uint64_t *plain, int32_t *d_status,
+                         void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ckks_encode_many(c, values, count, values_stride, plaintexts, scale, limbs, plain, d_status, ws, ws_bytes,
+                              (hipStream_t)stream);
+    });
+}
+
+int shc_ckks_decode_many_workspace_size(const shc_context *ctx, uint32_t limbs, uint32_t plaintexts, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = shc::ckks_decode_workspace_many(c, limbs, plaintexts);
+    });
+}
+
+int shc_ckks_decode_many(const shc_context *ctx, const uint64_t *plain, uint32_t limbs, uint32_t plaintexts,
+                         double scale, double *values, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ckks_decode_many(c, plain, limbs, plaintexts, scale, values, ws, ws_bytes, (hipStream_t)stream);
     });
 }
 

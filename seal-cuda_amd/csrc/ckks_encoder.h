@@ -30,6 +30,17 @@ void ckks_encode(const Context &c, const double *values, uint32_t count, double 
 // zero plaintext) instead of raised: nothing waits for the stream.
 void ckks_encode_deferred(const Context &c, const double *values, uint32_t count, double scale, uint32_t L,
                           uint64_t *plain, int *d_status, void *ws, size_t ws_bytes, hipStream_t st);
+// Batches: `batch` plaintexts in one launch per FFT layer.  Encode: values of plaintext p
+// at values + p * vstride (count pairs each), plain [batch][L][N]; d_status (optional, one
+// int per plaintext) takes the range checks on the device.  Decode: plain [batch][L][N]
+// -> values [batch][N/2] pairs.
+size_t ckks_encode_workspace_many(const Context &c, uint32_t batch);
+void ckks_encode_many(const Context &c, const double *values, uint32_t count, size_t vstride, uint32_t batch,
+                      double scale, uint32_t L, uint64_t *plain, int *d_status, void *ws, size_t ws_bytes,
+                      hipStream_t st);
+size_t ckks_decode_workspace_many(const Context &c, uint32_t L, uint32_t batch);
+void ckks_decode_many(const Context &c, const uint64_t *plain, uint32_t L, uint32_t batch, double scale,
+                      double *values, void *ws, size_t ws_bytes, hipStream_t st);
 size_t ckks_decode_workspace(const Context &c, uint32_t L);
 // plain [L][N] NTT form -> N/2 (re, im) double pairs.
 void ckks_decode(const Context &c, const uint64_t *plain, uint32_t L, double scale, double *values, void *ws,

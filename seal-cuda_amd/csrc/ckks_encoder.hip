@@ -14,6 +14,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <vector>
 
 namespace shc {
 namespace {
@@ -42,8 +43,12 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-__global__ void scatter_kernel(const double *__restrict__ values, uint32_t count, double2 *__restrict__ a, int logn) {
+// Every kernel here takes plaintext blockIdx.y of a batch (its data at a fixed stride).
+__global__ void scatter_kernel(const double *__restrict__ values, uint32_t count, size_t vstride,
+                               double2 *__restrict__ a, int logn) {
     const uint32_t slots = 1u << (logn - 1);
+    values += blockIdx.y * vstride;
+    a += (size_t)blockIdx.y << logn;
     for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < slots; i += gridDim.x * TPB) {
         double2 v = make_double2(0.0, 0.0);
         if (i < count) v = make_double2(values[2 * i], values[2 * i + 1]);
@@ -55,6 +60,7 @@ __global__ void scatter_kernel(const double *__restrict__ values, uint32_t count
 // one Gentleman-Sande layer of DWTHandler::transformFromRev with m groups of size 2*gap
 __global__ void fft_inv_layer(double2 *__restrict__ a, const double2 *__restrict__ r, uint32_t m, uint32_t gap,
                               uint32_t ridx, double scalar, int last, uint32_t half) {
+    a += (size_t)blockIdx.y * 2 * half;
     for (uint32_t b = blockIdx.x * TPB + threadIdx.x; b < half; b += gridDim.x * TPB) {
         const uint32_t i = b / gap, j = b % gap;
         double2 *x = a + 2 * gap * i + j, *y = x + gap;
@@ -73,6 +79,7 @@ __global__ void fft_inv_layer(double2 *__restrict__ a, const double2 *__restrict
 // one Cooley-Tukey layer of DWTHandler::transformToRev
 __global__ void fft_fwd_layer(double2 *__restrict__ a, const double2 *__restrict__ r, uint32_t gap, uint32_t ridx,
                               uint32_t half) {
+    a += (size_t)blockIdx.y * 2 * half;
     for (uint32_t b = blockIdx.x * TPB + threadIdx.x; b < half; b += gridDim.x * TPB) {
         const uint32_t i = b / gap, j = b % gap;
         double2 *x = a + 2 * gap * i + j, *y = x + gap;
@@ -89,6 +96,9 @@ __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__r
                                     unsigned long long *maxbits) {
     const size_t n = (size_t)1 << logn;
     const double two64 = 18446744073709551616.0;
+    a += blockIdx.y * n;
+    out += blockIdx.y * L * n;
+    maxbits += blockIdx.y;
     for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < n; c += (size_t)gridDim.x * TPB) {
         double d = rint(a[c].x);
         const bool neg = signbit(d);
@@ -105,12 +115,19 @@ __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__r
     }
 }
 
-__global__ void clear_word_kernel(unsigned long long *w) { *w = 0; }
+__global__ void clear_words_kernel(unsigned long long *w, uint32_t words) {
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) w[i] = 0;
+}
 
 // The encoder's range check on the device: status = 0, or SHC_INVALID_ARGUMENT (-1) when
 // the largest |coefficient| is not finite or needs total_bits or more bits, or more than
 // 128 (CKKSEncoder::encodeInternal's "encoded values are too large", src/ckks.cpp:167-171).
-__global__ void encode_check_kernel(const unsigned long long *maxbits, int total_bits, int *status) {
+// (one thread per plaintext: plaintext p's maximum and status word)
+__global__ void encode_check_kernel(const unsigned long long *maxbits, int total_bits, int *status, uint32_t count) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    maxbits += p;
+    status += p;
     const double maxc = __longlong_as_double((long long)*maxbits);
     int st = 0;
     if (!isfinite(maxc)) {
@@ -124,7 +141,8 @@ __global__ void encode_check_kernel(const unsigned long long *maxbits, int total
 
 // A plaintext the check rejected is left all zero.
 __global__ void zero_if_rejected_kernel(uint64_t *__restrict__ plain, const int *__restrict__ status, size_t words) {
-    if (*status == 0) return;
+    plain += blockIdx.y * words;
+    if (status[blockIdx.y] == 0) return;
     for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < words; i += (size_t)gridDim.x * TPB) plain[i] = 0;
 }
 
@@ -132,6 +150,8 @@ __global__ void compose_kernel(const uint64_t *__restrict__ c, double2 *__restri
                                const DevPrime *__restrict__ primes, int logn, double inv_scale) {
     const size_t n = (size_t)1 << logn;
     const double two64 = 18446744073709551616.0;
+    c += blockIdx.y * T.L * n;
+    a += blockIdx.y * n;
     for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
         double f = 0.0;
         uint64_t lo = 0, hi = 0;
@@ -163,6 +183,8 @@ __global__ void compose_kernel(const uint64_t *__restrict__ c, double2 *__restri
 
 __global__ void gather_kernel(const double2 *__restrict__ a, double *__restrict__ values, int logn) {
     const uint32_t slots = 1u << (logn - 1);
+    a += (size_t)blockIdx.y << logn;
+    values += (size_t)blockIdx.y << logn;
     for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < slots; i += gridDim.x * TPB) {
         const double2 v = a[slot_pos(i, logn)];
         values[2 * i] = v.x;
@@ -243,91 +265,121 @@ void build_ckks_tables(Context &c) {
     c.ckks = t;
 }
 
-size_t ckks_encode_workspace(const Context &c) { return c.n * 16 + 256; }
+size_t ckks_encode_workspace(const Context &c) { return ckks_encode_workspace_many(c, 1); }
 
 namespace {
 
-// Encode up to the forward NTT.  deferred: the range check goes to *d_status on the
-// device (a rejected plaintext is zero) and nothing waits for the stream; otherwise the
-// check is read back and raised like the reference's.
-void encode_body(const Context &c, const double *values, uint32_t count, double scale, uint32_t L, uint64_t *plain,
-                 void *ws, size_t ws_bytes, int *d_status, hipStream_t st) {
+// Encode `batch` plaintexts up to the forward NTT: values of plaintext p at
+// values + p * vstride, plaintext p at plain + p * L * N.  d_status: the range checks go
+// to d_status[p] on the device (a rejected plaintext is zero) and nothing waits for the
+// stream; otherwise they are read back and raised like the reference's.
+void encode_body(const Context &c, const double *values, uint32_t count, size_t vstride, uint32_t batch, double scale,
+                 uint32_t L, uint64_t *plain, void *ws, size_t ws_bytes, int *d_status, hipStream_t st) {
     if (c.scheme != SCHEME_CKKS || !c.ckks) fail(ERR_INVALID_ARGUMENT, "unsupported scheme");
     if (c.logn < 1) fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree too small");
     if (count > c.n / 2) fail(ERR_INVALID_ARGUMENT, "values_size is too large");
-    if (!ws || ws_bytes < ckks_encode_workspace(c)) fail(ERR_INVALID_ARGUMENT, "workspace too small for encoding");
+    if (batch == 0) return;
+    if (batch > 65535) fail(ERR_INVALID_ARGUMENT, "at most 65535 plaintexts per call");
+    if (!ws || ws_bytes < ckks_encode_workspace_many(c, batch))
+        fail(ERR_INVALID_ARGUMENT, "workspace too small for encoding");
     const CkksLevelDev &T = c.ckks->levels[L];
     // CKKSEncoder::encodeInternal scale check (src/ckks.cpp:120-123)
     if (!(scale > 0) || (int)std::log2(scale) + 1 >= T.total_bits) fail(ERR_INVALID_ARGUMENT, "scale out of bounds");
     const size_t n = c.n;
     double2 *a = static_cast<double2 *>(ws);
-    auto *maxbits = reinterpret_cast<unsigned long long *>(static_cast<char *>(ws) + n * 16);
+    auto *maxbits = reinterpret_cast<unsigned long long *>(static_cast<char *>(ws) + batch * n * 16);
     const double2 *irp = c.ckks->roots.as<const double2>() + n;
-    scatter_kernel<<<blocks_for(n / 2), TPB, 0, st>>>(values, count, a, c.logn);
+    scatter_kernel<<<dim3(blocks_for(n / 2), batch), TPB, 0, st>>>(values, count, vstride, a, c.logn);
     uint32_t gap = 1, ridx = 1;
     for (uint32_t m = (uint32_t)(n >> 1); m >= 1; m >>= 1, gap <<= 1) {
-        fft_inv_layer<<<blocks_for(n / 2), TPB, 0, st>>>(a, irp, m, gap, ridx, scale / (double)n, m == 1,
-                                                         (uint32_t)(n / 2));
+        fft_inv_layer<<<dim3(blocks_for(n / 2), batch), TPB, 0, st>>>(a, irp, m, gap, ridx, scale / (double)n, m == 1,
+                                                                      (uint32_t)(n / 2));
         ridx += m;
     }
-    clear_word_kernel<<<1, 1, 0, st>>>(maxbits);
-    round_reduce_kernel<<<blocks_for(n), TPB, 0, st>>>(a, plain, c.primes(), L, c.logn, maxbits);
+    clear_words_kernel<<<1, TPB, 0, st>>>(maxbits, batch);
+    round_reduce_kernel<<<dim3(blocks_for(n), batch), TPB, 0, st>>>(a, plain, c.primes(), L, c.logn, maxbits);
     launch_check("ckks encode");
     if (d_status) {
-        encode_check_kernel<<<1, 1, 0, st>>>(maxbits, T.total_bits, d_status);
-        zero_if_rejected_kernel<<<blocks_for((size_t)L * n), TPB, 0, st>>>(plain, d_status, (size_t)L * n);
+        encode_check_kernel<<<(batch + TPB - 1) / TPB, TPB, 0, st>>>(maxbits, T.total_bits, d_status, batch);
+        zero_if_rejected_kernel<<<dim3(blocks_for((size_t)L * n), batch), TPB, 0, st>>>(plain, d_status,
+                                                                                         (size_t)L * n);
         launch_check("ckks encode check");
         return;
     }
-    unsigned long long mb = 0;
-    SHC_HIP(hipMemcpyAsync(&mb, maxbits, 8, hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> mb(batch);
+    SHC_HIP(hipMemcpyAsync(mb.data(), maxbits, 8 * (size_t)batch, hipMemcpyDeviceToHost, st));
     SHC_HIP(hipStreamSynchronize(st));
-    double maxc;
-    std::memcpy(&maxc, &mb, 8);
-    // an infinite or NaN input makes the maximum non-finite (NaN orders above +inf as
-    // an unsigned bit pattern), which no coefficient modulus can hold
-    if (!std::isfinite(maxc)) fail(ERR_INVALID_ARGUMENT, "encoded values are not finite");
-    const int bits = (int)std::ceil(std::log2(std::max(maxc, 1.0))) + 1;
-    if (bits >= T.total_bits || bits > 128) fail(ERR_INVALID_ARGUMENT, "encoded values are too large");
+    for (uint32_t p = 0; p < batch; p++) {
+        double maxc;
+        std::memcpy(&maxc, &mb[p], 8);
+        // an infinite or NaN input makes the maximum non-finite (NaN orders above +inf as
+        // an unsigned bit pattern), which no coefficient modulus can hold
+        if (!std::isfinite(maxc)) fail(ERR_INVALID_ARGUMENT, "encoded values are not finite");
+        const int bits = (int)std::ceil(std::log2(std::max(maxc, 1.0))) + 1;
+        if (bits >= T.total_bits || bits > 128) fail(ERR_INVALID_ARGUMENT, "encoded values are too large");
+    }
 }
 
 }  // namespace
 
+size_t ckks_encode_workspace_many(const Context &c, uint32_t batch) {
+    return (size_t)batch * c.n * 16 + ((8 * (size_t)batch + 255) & ~size_t(255));
+}
+
 void ckks_encode(const Context &c, const double *values, uint32_t count, double scale, uint32_t L, uint64_t *plain,
                  void *ws, size_t ws_bytes, hipStream_t st) {
-    encode_body(c, values, count, scale, L, plain, ws, ws_bytes, nullptr, st);
+    encode_body(c, values, count, 0, 1, scale, L, plain, ws, ws_bytes, nullptr, st);
     ntt_forward(plain, L, L, c.primes(), c.logn, NTT_FULL, st);
 }
 
 void ckks_encode_deferred(const Context &c, const double *values, uint32_t count, double scale, uint32_t L,
                           uint64_t *plain, int *d_status, void *ws, size_t ws_bytes, hipStream_t st) {
     if (!d_status) fail(ERR_INVALID_ARGUMENT, "null status word");
-    encode_body(c, values, count, scale, L, plain, ws, ws_bytes, d_status, st);
+    encode_body(c, values, count, 0, 1, scale, L, plain, ws, ws_bytes, d_status, st);
     ntt_forward(plain, L, L, c.primes(), c.logn, NTT_FULL, st);
 }
 
-size_t ckks_decode_workspace(const Context &c, uint32_t L) { return (size_t)L * c.n * 8 + c.n * 16 + 256; }
+void ckks_encode_many(const Context &c, const double *values, uint32_t count, size_t vstride, uint32_t batch,
+                      double scale, uint32_t L, uint64_t *plain, int *d_status, void *ws, size_t ws_bytes,
+                      hipStream_t st) {
+    encode_body(c, values, count, vstride, batch, scale, L, plain, ws, ws_bytes, d_status, st);
+    if (batch) ntt_forward(plain, batch * L, L, c.primes(), c.logn, NTT_FULL, st);
+}
 
-void ckks_decode(const Context &c, const uint64_t *plain, uint32_t L, double scale, double *values, void *ws,
-                 size_t ws_bytes, hipStream_t st) {
+size_t ckks_decode_workspace(const Context &c, uint32_t L) { return ckks_decode_workspace_many(c, L, 1); }
+
+size_t ckks_decode_workspace_many(const Context &c, uint32_t L, uint32_t batch) {
+    return (((size_t)batch * L * c.n * 8 + 255) & ~size_t(255)) + (size_t)batch * c.n * 16;
+}
+
+void ckks_decode_many(const Context &c, const uint64_t *plain, uint32_t L, uint32_t batch, double scale,
+                      double *values, void *ws, size_t ws_bytes, hipStream_t st) {
     if (c.scheme != SCHEME_CKKS || !c.ckks) fail(ERR_INVALID_ARGUMENT, "unsupported scheme");
-    if (!ws || ws_bytes < ckks_decode_workspace(c, L)) fail(ERR_INVALID_ARGUMENT, "workspace too small for decoding");
+    if (batch == 0) return;
+    if (batch > 65535) fail(ERR_INVALID_ARGUMENT, "at most 65535 plaintexts per call");
+    if (!ws || ws_bytes < ckks_decode_workspace_many(c, L, batch))
+        fail(ERR_INVALID_ARGUMENT, "workspace too small for decoding");
     const CkksLevelDev &T = c.ckks->levels[L];
     if (!(scale > 0) || (int)std::log2(scale) >= T.total_bits) fail(ERR_INVALID_ARGUMENT, "scale out of bounds");
     const size_t n = c.n;
     uint64_t *cp = static_cast<uint64_t *>(ws);
-    double2 *a = reinterpret_cast<double2 *>(static_cast<char *>(ws) + (((size_t)L * n * 8 + 255) & ~size_t(255)));
-    SHC_HIP(hipMemcpyAsync(cp, plain, (size_t)L * n * 8, hipMemcpyDeviceToDevice, st));
-    ntt_inverse(cp, L, L, c.primes(), c.logn, NTT_FULL, st);
-    compose_kernel<<<blocks_for(n), TPB, 0, st>>>(cp, a, T, c.primes(), c.logn, 1.0 / scale);
+    double2 *a = reinterpret_cast<double2 *>(static_cast<char *>(ws) + (((size_t)batch * L * n * 8 + 255) & ~size_t(255)));
+    SHC_HIP(hipMemcpyAsync(cp, plain, (size_t)batch * L * n * 8, hipMemcpyDeviceToDevice, st));
+    ntt_inverse(cp, batch * L, L, c.primes(), c.logn, NTT_FULL, st);
+    compose_kernel<<<dim3(blocks_for(n), batch), TPB, 0, st>>>(cp, a, T, c.primes(), c.logn, 1.0 / scale);
     const double2 *rp = c.ckks->roots.as<const double2>();
     uint32_t gap = (uint32_t)(n >> 1), ridx = 1;
     for (uint32_t m = 1; m < n; m <<= 1, gap >>= 1) {
-        fft_fwd_layer<<<blocks_for(n / 2), TPB, 0, st>>>(a, rp, gap, ridx, (uint32_t)(n / 2));
+        fft_fwd_layer<<<dim3(blocks_for(n / 2), batch), TPB, 0, st>>>(a, rp, gap, ridx, (uint32_t)(n / 2));
         ridx += m;
     }
-    gather_kernel<<<blocks_for(n / 2), TPB, 0, st>>>(a, values, c.logn);
+    gather_kernel<<<dim3(blocks_for(n / 2), batch), TPB, 0, st>>>(a, values, c.logn);
     launch_check("ckks decode");
+}
+
+void ckks_decode(const Context &c, const uint64_t *plain, uint32_t L, double scale, double *values, void *ws,
+                 size_t ws_bytes, hipStream_t st) {
+    ckks_decode_many(c, plain, L, 1, scale, values, ws, ws_bytes, st);
 }
 
 }  // namespace shc

@@ -80,4 +80,43 @@ struct ModDownArgs {
 bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &args, uint32_t count, const DevPrime *primes, int logn,
                  hipStream_t stream);
 
+// ---- grouped launches (ciphertexts at several levels, one launch per stage) ----------
+// At most MAX_GROUPS groups per call; every group is one level's batch with its own
+// buffers.  A grouped call gives each group the words of the one-level call.
+constexpr uint32_t MAX_GROUPS = 16;
+
+// A transform group: limbs of data (grid lg, primes[i] for limb (i, j)); src != nullptr:
+// the first pass reads the limbs from src (grid slg, same shape) instead.
+struct NttGroup {
+    uint64_t *data;
+    const uint64_t *src;
+    LimbGrid lg, slg;
+    const DevPrime *primes;
+};
+void ntt_forward_groups(const NttGroup *groups, uint32_t n, int logn, NttOut out, hipStream_t stream);
+void ntt_inverse_groups(const NttGroup *groups, uint32_t n, int logn, NttOut out, hipStream_t stream);
+
+// A key-switch group for ntt_modup_keyprod_groups (the arguments of ntt_modup_keyprod).
+struct KsGroup {
+    const uint64_t *d;
+    size_t d_ct_stride;
+    uint64_t *T, *P, *S;
+    const uint64_t *diag;
+    size_t diag_stride;
+    const DevPrime *targets;
+    uint32_t count, L;
+};
+// The fused ModUp + key products of every group (N = 2^15, 2^16; false otherwise): one
+// columns-pass launch and one rows-pass launch per ciphertext grouping (4, 2, 1).
+bool ntt_modup_keyprod_groups(const KsGroup *groups, uint32_t n, const uint64_t *key, uint32_t key_count,
+                              const DevPrime *digits, int logn, hipStream_t stream);
+
+// ModDown groups (CKKS, N >= 2^13; false otherwise): buf = U scratch of the group.
+struct ModDownGroup {
+    uint64_t *U;
+    ModDownArgs a;
+    uint32_t count;
+};
+bool ntt_moddown_groups(const ModDownGroup *groups, uint32_t n, const DevPrime *primes, int logn, hipStream_t stream);
+
 }  // namespace shc

@@ -27,6 +27,7 @@
 #include "ntt.h"
 #include "accum.h"
 #include "butterfly.h"
+#include <vector>
 
 namespace shc {
 namespace {
@@ -162,8 +163,9 @@ __host__ __device__ constexpr bool rows_mixed(uint32_t jn) {
 }
 
 template <int LOGN, int K, int E, int SUBS, bool INV, bool W = false, bool MX = false, class Load, class Store>
-__device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, uint32_t I, uint32_t j0, uint32_t jn,
-                                          const Load &load_elem, const Store &store_elem, uint32_t jn_short = 0) {
+__device__ __forceinline__ void rows_body(uint32_t bid, const DevPrime *__restrict__ primes, uint32_t I, uint32_t j0,
+                                          uint32_t jn, const Load &load_elem, const Store &store_elem,
+                                          uint32_t jn_short = 0) {
     constexpr int T = 1 << (K - E);
     constexpr int SZ = 1 << K;
     constexpr int RPL = LOGN - K;  // log2(rows per limb)
@@ -178,8 +180,8 @@ __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, u
     uint32_t j;
     if constexpr (MX) {
         const uint32_t per = rows_blocks_per_prime<LOGN, K, E, SUBS>(jn, true);
-        i = blockIdx.x / per;
-        q0 = (blockIdx.x % per) * SUBS;
+        i = bid / per;
+        q0 = (bid % per) * SUBS;
         nsub = (int)min<uint32_t>(SUBS, (jn << RPL) - q0);
         active = sub < nsub;
         const uint32_t q = q0 + (uint32_t)(active ? sub : 0);
@@ -187,8 +189,8 @@ __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, u
         g = (int)(q / jn);
     } else {
         const uint32_t jc_count = (jn + SUBS - 1) / SUBS;
-        const uint32_t jc = blockIdx.x % jc_count;
-        const uint32_t r = blockIdx.x / jc_count;
+        const uint32_t jc = bid % jc_count;
+        const uint32_t r = bid / jc_count;
         g = (int)(r & ((1u << RPL) - 1));
         i = r >> RPL;
         jfirst = j0 + jc * SUBS;
@@ -242,7 +244,7 @@ __device__ __forceinline__ void rows_body(const DevPrime *__restrict__ primes, u
 // Blocks run limb (i, j) by limb with j fastest, or with i fastest when I_FAST (then I
 // is the prime count) -- the order for loads shared by all primes of one j.
 template <int LOGN, int K, int E, int SUBS, bool INV, bool I_FAST = false, class Load, class Store>
-__device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, uint32_t j0, uint32_t jn,
+__device__ __forceinline__ void cols_body(uint32_t bid, const DevPrime *__restrict__ primes, uint32_t j0, uint32_t jn,
                                           const Load &load_elem, const Store &store_elem, uint32_t I = 1) {
     constexpr int SZ = 1 << K;
     constexpr int C = 1 << (LOGN - K);
@@ -250,8 +252,8 @@ __device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, u
     __shared__ uint64_t tile[SUBS * SZ];
     const int tid = threadIdx.x;
     const int col = tid % SUBS, t = tid / SUBS;
-    const uint32_t r = blockIdx.x / TILES;
-    const size_t c = (size_t)(blockIdx.x % TILES) * SUBS + col;
+    const uint32_t r = bid / TILES;
+    const size_t c = (size_t)(bid % TILES) * SUBS + col;
     const uint32_t i = I_FAST ? r % I : r / jn, j = j0 + (I_FAST ? r / I : r % jn);
     const DevPrime &P = primes[i];
     const ulonglong2 *tw = INV ? P.inv : P.fwd;
@@ -281,8 +283,8 @@ __device__ __forceinline__ void cols_body(const DevPrime *__restrict__ primes, u
 // Sum bound: up to 31 digits the transformed words (< 8q) enter the sums directly
 // (31 * 8q^2 + q^2 < 2^128 for q < 2^60); beyond, R4 takes them below 4q first.
 template <int LOGN, int K, int E, int SUBS, int CG, int RW, bool R4>
-__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
-    const uint64_t *__restrict__ T, const uint64_t *__restrict__ key, uint64_t *__restrict__ Pout,
+__device__ __forceinline__ void rows_keyprod_body(
+    uint32_t bid, const uint64_t *__restrict__ T, const uint64_t *__restrict__ key, uint64_t *__restrict__ Pout,
     uint64_t *__restrict__ Sout, const DevPrime *__restrict__ primes, const uint64_t *__restrict__ diag,
     size_t diag_stride, uint32_t count, uint32_t L, uint32_t K_key, uint32_t ct_base, uint32_t groups,
     uint32_t tiles) {
@@ -297,7 +299,8 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
     constexpr int DJ = SUBS / NS;  // digits per step
     static_assert(RW == 1 || DJ % (64 / TT) == 0, "slot-major: one slot per wave");
     __shared__ uint64_t tile[SUBS * SZ];
-    const uint32_t xcd = blockIdx.x & 7, kk = blockIdx.x >> 3;
+    // bid & 7 is the XCD (launches, and groups of a grouped launch, start at multiples of 8)
+    const uint32_t xcd = bid & 7, kk = bid >> 3;
     const uint32_t tl = (kk / groups) * 8 + xcd;
     if (tl >= tiles) return;  // uniform: the whole block
     const uint32_t grp = kk % groups;
@@ -408,6 +411,51 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
     }
 }
 
+template <int LOGN, int K, int E, int SUBS, int CG, int RW, bool R4>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
+    const uint64_t *__restrict__ T, const uint64_t *__restrict__ key, uint64_t *__restrict__ Pout,
+    uint64_t *__restrict__ Sout, const DevPrime *__restrict__ primes, const uint64_t *__restrict__ diag,
+    size_t diag_stride, uint32_t count, uint32_t L, uint32_t K_key, uint32_t ct_base, uint32_t groups,
+    uint32_t tiles) {
+    rows_keyprod_body<LOGN, K, E, SUBS, CG, RW, R4>(blockIdx.x, T, key, Pout, Sout, primes, diag, diag_stride, count,
+                                                    L, K_key, ct_base, groups, tiles);
+}
+
+// ---- grouped launches: ciphertexts at several levels in one launch per stage.  Group
+// g owns blocks [start[g], start[g+1]) and runs the one-level body with its own
+// arguments on block (blockIdx.x - start[g]); the argument tables travel by value (in
+// the kernel-argument segment; the group index is uniform, so they are scalar loads).
+struct KeyprodGroup {
+    const uint64_t *T;
+    uint64_t *P, *S;
+    const DevPrime *primes;
+    const uint64_t *diag;
+    size_t diag_stride;
+    uint32_t count, L, ct_base, groups, tiles;
+};
+struct KeyprodGroups {
+    uint32_t n;
+    uint32_t start[MAX_GROUPS + 1];
+    const uint64_t *key;
+    uint32_t K_key;
+    KeyprodGroup g[MAX_GROUPS];
+};
+__device__ __forceinline__ uint32_t group_of(const uint32_t *start, uint32_t n, uint32_t b) {
+    uint32_t g = 0;
+#pragma unroll 1
+    while (g + 1 < n && b >= start[g + 1]) g++;
+    return g;
+}
+
+template <int LOGN, int K, int E, int SUBS, int CG, int RW, bool R4>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_grouped_kernel(KeyprodGroups kg) {
+    const uint32_t gi = group_of(kg.start, kg.n, blockIdx.x);
+    const KeyprodGroup &a = kg.g[gi];
+    rows_keyprod_body<LOGN, K, E, SUBS, CG, RW, R4>(blockIdx.x - kg.start[gi], a.T, kg.key, a.P, a.S, a.primes,
+                                                    a.diag, a.diag_stride, a.count, a.L, kg.K_key, a.ct_base,
+                                                    a.groups, a.tiles);
+}
+
 // Plain in-place passes over a LimbGrid buffer.
 struct GridAccess {
     uint64_t *data;
@@ -424,7 +472,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_kernel(uint64_t *__r
                                                                   uint32_t j0, uint32_t jn) {
     const GridAccess g{data, lg, LOGN};
     rows_body<LOGN, K, E, SUBS, INV, false, MX>(
-        primes, lg.I, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
+        blockIdx.x, primes, lg.I, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
         });
@@ -440,7 +488,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_from_kernel(const ui
     const GridAccess g{data, lg, LOGN};
     const GridAccess s{const_cast<uint64_t *>(src), slg, LOGN};
     rows_body<LOGN, K, E, SUBS, INV, false, MX>(
-        primes, lg.I, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *s.at(i, j, off); },
+        blockIdx.x, primes, lg.I, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *s.at(i, j, off); },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
         });
@@ -456,7 +504,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_modup_kernel(uint64_
                                                                         LimbGrid lg, uint32_t jn_short) {
     const GridAccess g{data, lg, LOGN};
     rows_body<LOGN, K, E, SUBS, false, true>(
-        primes, lg.I, 0, lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
+        blockIdx.x, primes, lg.I, 0, lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             *g.at(i, j, off) = FIN ? finish<false, false>(x, primes[i]) : x;
         },
@@ -469,7 +517,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
                                                                   uint32_t j0, uint32_t jn) {
     const GridAccess g{data, lg, LOGN};
     cols_body<LOGN, K, E, SUBS, INV>(
-        primes, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
+        blockIdx.x, primes, j0, jn, [&](uint32_t i, uint32_t j, size_t off) { return *g.at(i, j, off); },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
         });
@@ -488,13 +536,10 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_kernel(uint64_t *__r
 // skip_diag: limb (i, i) is not produced (its NTT is the NTT-form input limb) and the
 // targets i < L keep L - 1 limbs per ciphertext, T[i][ct][j - (j > i)].
 template <int LOGN, int K, int E, int SUBS, bool ONE>
-__global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const uint64_t *__restrict__ d,
-                                                                        size_t d_ct_stride,
-                                                                        uint64_t *__restrict__ T,
-                                                                        const DevPrime *__restrict__ targets,
-                                                                        const DevPrime *__restrict__ digits,
-                                                                        uint32_t count, uint32_t L, bool skip_diag,
-                                                                        uint32_t tc) {
+__device__ __forceinline__ void cols_modup_body(uint32_t bid, const uint64_t *__restrict__ d, size_t d_ct_stride,
+                                                uint64_t *__restrict__ T, const DevPrime *__restrict__ targets,
+                                                const DevPrime *__restrict__ digits, uint32_t count, uint32_t L,
+                                                bool skip_diag, uint32_t tc) {
     static_assert(K >= E, "the first phase fills all 2^E registers");
     constexpr int C = 1 << (LOGN - K);
     constexpr int TILES = C / SUBS;
@@ -502,10 +547,10 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
     const int tid = threadIdx.x;
     const int col = tid % SUBS, t = tid / SUBS;
     const uint32_t chunks = (L + tc) / tc;  // ceil((L + 1) / tc)
-    const uint32_t rc = blockIdx.x / TILES;  // (ct * L + j) * chunks + chunk
+    const uint32_t rc = bid / TILES;  // (ct * L + j) * chunks + chunk
     const uint32_t r = rc / chunks;          // ct * L + j
     const uint32_t i0 = (rc % chunks) * tc, i1 = min<uint32_t>(i0 + tc, L + 1);
-    const int c0 = (int)(blockIdx.x % TILES) * SUBS;
+    const int c0 = (int)(bid % TILES) * SUBS;
     const uint32_t j = r % L;
     const size_t ct = r / L;
     const uint64_t *in = d + ct * d_ct_stride + ((size_t)j << LOGN) + c0 + col;
@@ -546,6 +591,83 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const u
     }
 }
 
+template <int LOGN, int K, int E, int SUBS, bool ONE>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_kernel(const uint64_t *__restrict__ d,
+                                                                        size_t d_ct_stride,
+                                                                        uint64_t *__restrict__ T,
+                                                                        const DevPrime *__restrict__ targets,
+                                                                        const DevPrime *__restrict__ digits,
+                                                                        uint32_t count, uint32_t L, bool skip_diag,
+                                                                        uint32_t tc) {
+    cols_modup_body<LOGN, K, E, SUBS, ONE>(blockIdx.x, d, d_ct_stride, T, targets, digits, count, L, skip_diag, tc);
+}
+
+struct ModupGroup {
+    const uint64_t *d;
+    size_t d_ct_stride;
+    uint64_t *T;
+    const DevPrime *targets;
+    uint32_t count, L, tc;
+    bool skip_diag;
+};
+struct ModupGroups {
+    uint32_t n;
+    uint32_t start[MAX_GROUPS + 1];
+    const DevPrime *digits;
+    ModupGroup g[MAX_GROUPS];
+};
+
+template <int LOGN, int K, int E, int SUBS, bool ONE>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_modup_grouped_kernel(ModupGroups mg) {
+    const uint32_t gi = group_of(mg.start, mg.n, blockIdx.x);
+    const ModupGroup &a = mg.g[gi];
+    cols_modup_body<LOGN, K, E, SUBS, ONE>(blockIdx.x - mg.start[gi], a.d, a.d_ct_stride, a.T, a.targets, mg.digits,
+                                           a.count, a.L, a.skip_diag, a.tc);
+}
+
+// Grouped transform passes: group g's limbs (i, j) of its grid, j < lg.J, all primes of
+// its grid; the first pass of an inverse reads from src when the group has one.
+struct NttGroupDev {
+    uint64_t *data;
+    const uint64_t *src;
+    LimbGrid lg, slg;
+    const DevPrime *primes;
+};
+struct NttGroups {
+    uint32_t n;
+    uint32_t start[MAX_GROUPS + 1];
+    NttGroupDev g[MAX_GROUPS];
+};
+
+template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL, bool MX>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_grouped_kernel(NttGroups ng) {
+    const uint32_t gi = group_of(ng.start, ng.n, blockIdx.x);
+    const NttGroupDev &a = ng.g[gi];
+    const GridAccess g{a.data, a.lg, LOGN};
+    const GridAccess s{const_cast<uint64_t *>(a.src ? a.src : a.data), a.src ? a.slg : a.lg, LOGN};
+    const DevPrime *primes = a.primes;
+    rows_body<LOGN, K, E, SUBS, INV, false, MX>(
+        blockIdx.x - ng.start[gi], primes, a.lg.I, 0, a.lg.J,
+        [&](uint32_t i, uint32_t j, size_t off) { return *s.at(i, j, off); },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
+        });
+}
+
+template <int LOGN, int K, int E, int SUBS, bool INV, bool LAST, bool FULL>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_grouped_kernel(NttGroups ng) {
+    const uint32_t gi = group_of(ng.start, ng.n, blockIdx.x);
+    const NttGroupDev &a = ng.g[gi];
+    const GridAccess g{a.data, a.lg, LOGN};
+    const GridAccess s{const_cast<uint64_t *>(a.src ? a.src : a.data), a.src ? a.slg : a.lg, LOGN};
+    const DevPrime *primes = a.primes;
+    cols_body<LOGN, K, E, SUBS, INV>(
+        blockIdx.x - ng.start[gi], primes, 0, a.lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *s.at(i, j, off); },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
+        });
+}
+
 // ---- ModDown by the special prime p, fused into the transforms (switch_key step 5):
 //   u_i = ([s + floor(p/2)]_p mod q_i) - (floor(p/2) mod q_i)     (s: special limb, coeff)
 //   ct_k[i] += (P_k[i] - u_i) * p^-1 mod q_i
@@ -572,7 +694,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_moddown_prep_kernel(
                                                                                ModDownArgs a, uint32_t jn) {
     // every prime reads the same special limb: i fastest keeps it in L2
     cols_body<LOGN, K, E, SUBS, false, true>(
-        primes, 0, jn,
+        blockIdx.x, primes, 0, jn,
         [&](uint32_t i, uint32_t j, size_t off) {
             return moddown_prep(a.S[((size_t)j << LOGN) + off], a.p, primes[i], a.dc[i]);
         },
@@ -584,7 +706,49 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_moddown_kernel(const
                                                                           const DevPrime *__restrict__ primes,
                                                                           ModDownArgs a, uint32_t jn) {
     rows_body<LOGN, K, E, SUBS, false, false, MX>(
-        primes, a.L, 0, jn, [&](uint32_t i, uint32_t j, size_t off) { return U[((size_t)(j * a.L + i) << LOGN) + off]; },
+        blockIdx.x, primes, a.L, 0, jn, [&](uint32_t i, uint32_t j, size_t off) { return U[((size_t)(j * a.L + i) << LOGN) + off]; },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            const DevPrime &Q = primes[i];
+            moddown_add<LOGN>(a, i, j, off, a.P[((size_t)(j * a.L + i) << LOGN) + off], finish<false, true>(x, Q), Q);
+        });
+}
+
+struct ModDownGroupDev {
+    uint64_t *U;
+    ModDownArgs a;
+    uint32_t jn;
+};
+struct ModDownGroups {
+    uint32_t n;
+    uint32_t start[MAX_GROUPS + 1];
+    ModDownGroupDev g[MAX_GROUPS];
+};
+
+template <int LOGN, int K, int E, int SUBS>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_moddown_prep_grouped_kernel(
+    const DevPrime *__restrict__ primes, ModDownGroups mg) {
+    const uint32_t gi = group_of(mg.start, mg.n, blockIdx.x);
+    const ModDownGroupDev &gr = mg.g[gi];
+    const ModDownArgs &a = gr.a;
+    uint64_t *U = gr.U;
+    cols_body<LOGN, K, E, SUBS, false, true>(
+        blockIdx.x - mg.start[gi], primes, 0, gr.jn,
+        [&](uint32_t i, uint32_t j, size_t off) {
+            return moddown_prep(a.S[((size_t)j << LOGN) + off], a.p, primes[i], a.dc[i]);
+        },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { U[((size_t)(j * a.L + i) << LOGN) + off] = x; }, a.L);
+}
+
+template <int LOGN, int K, int E, int SUBS, bool MX>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_moddown_grouped_kernel(const DevPrime *__restrict__ primes,
+                                                                                  ModDownGroups mg) {
+    const uint32_t gi = group_of(mg.start, mg.n, blockIdx.x);
+    const ModDownGroupDev &gr = mg.g[gi];
+    const ModDownArgs &a = gr.a;
+    const uint64_t *U = gr.U;
+    rows_body<LOGN, K, E, SUBS, false, false, MX>(
+        blockIdx.x - mg.start[gi], primes, a.L, 0, gr.jn,
+        [&](uint32_t i, uint32_t j, size_t off) { return U[((size_t)(j * a.L + i) << LOGN) + off]; },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             const DevPrime &Q = primes[i];
             moddown_add<LOGN>(a, i, j, off, a.P[((size_t)(j * a.L + i) << LOGN) + off], finish<false, true>(x, Q), Q);
@@ -596,7 +760,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_moddown_kernel(const
                                                                           const DevPrime *__restrict__ primes,
                                                                           ModDownArgs a, uint32_t jn) {
     cols_body<LOGN, K, E, SUBS, true>(
-        primes, 0, jn, [&](uint32_t i, uint32_t j, size_t off) { return Pt[((size_t)(j * a.L + i) << LOGN) + off]; },
+        blockIdx.x, primes, 0, jn, [&](uint32_t i, uint32_t j, size_t off) { return Pt[((size_t)(j * a.L + i) << LOGN) + off]; },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             const DevPrime &Q = primes[i];
             const uint64_t u = moddown_prep(a.S[((size_t)j << LOGN) + off], a.p, Q, a.dc[i]);
@@ -884,6 +1048,183 @@ void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const
     if (probe) SHC_HIP(hipEventRecord(probe->end, st));
 }
 
+// Block prefix table of a grouped launch; sizes[g] blocks for group g.
+template <class Table>
+uint32_t fill_starts(Table &t, const std::vector<uint64_t> &sizes) {
+    uint64_t acc = 0;
+    t.n = (uint32_t)sizes.size();
+    for (size_t g = 0; g < sizes.size(); g++) {
+        t.start[g] = (uint32_t)acc;
+        acc += sizes[g];
+    }
+    t.start[sizes.size()] = (uint32_t)acc;
+    if (acc >= (uint64_t(1) << 32) / 1024) fail(ERR_INVALID_ARGUMENT, "grouped launch too large");
+    return (uint32_t)acc;
+}
+
+template <int LOGN, bool INV, bool FULL>
+void launch_ntt_groups(const NttGroup *gs, uint32_t n, hipStream_t st) {
+    static_assert(LOGN > 12, "grouped transforms run the two-pass schedule");
+    constexpr int K1 = LOGN / 2;
+    constexpr int K2 = LOGN - K1;
+    constexpr int E = 4;
+    constexpr int CS = 16;
+    constexpr int RS = 4096 >> K2;
+    // the first pass reads src where a group has one; the second runs in place
+    NttGroups cols{}, rows_mx{}, rows_full{};
+    std::vector<uint64_t> csz, msz, fsz;
+    for (uint32_t g = 0; g < n; g++) {
+        const NttGroup &a = gs[g];
+        const bool first_is_cols = !INV;
+        NttGroupDev d{a.data, nullptr, a.lg, a.lg, a.primes};
+        NttGroupDev d_first{a.data, a.src, a.lg, a.src ? a.slg : a.lg, a.primes};
+        cols.g[csz.size()] = first_is_cols ? d_first : d;
+        csz.push_back((uint64_t)a.lg.I * a.lg.J * ((1u << K2) / CS));
+        const bool mx = rows_mixed<LOGN, K2, RS>(a.lg.J);
+        const uint64_t rb = (uint64_t)a.lg.I * rows_blocks_per_prime<LOGN, K2, E, RS>(a.lg.J, mx);
+        if (mx) {
+            rows_mx.g[msz.size()] = first_is_cols ? d : d_first;
+            msz.push_back(rb);
+        } else {
+            rows_full.g[fsz.size()] = first_is_cols ? d : d_first;
+            fsz.push_back(rb);
+        }
+    }
+    const uint32_t cb = fill_starts(cols, csz);
+    auto rows = [&](bool last) {
+        if (!msz.empty()) {
+            const uint32_t b = fill_starts(rows_mx, msz);
+            if (last) ntt_rows_grouped_kernel<LOGN, K2, E, RS, INV, true, FULL, true><<<b, RS << (K2 - E), 0, st>>>(rows_mx);
+            else ntt_rows_grouped_kernel<LOGN, K2, E, RS, INV, false, FULL, true><<<b, RS << (K2 - E), 0, st>>>(rows_mx);
+        }
+        if (!fsz.empty()) {
+            const uint32_t b = fill_starts(rows_full, fsz);
+            if (last) ntt_rows_grouped_kernel<LOGN, K2, E, RS, INV, true, FULL, false><<<b, RS << (K2 - E), 0, st>>>(rows_full);
+            else ntt_rows_grouped_kernel<LOGN, K2, E, RS, INV, false, FULL, false><<<b, RS << (K2 - E), 0, st>>>(rows_full);
+        }
+    };
+    if (!INV) {
+        ntt_cols_grouped_kernel<LOGN, K1, E, CS, false, false, FULL><<<cb, CS << (K1 - E), 0, st>>>(cols);
+        rows(true);
+    } else {
+        rows(false);
+        ntt_cols_grouped_kernel<LOGN, K1, E, CS, true, true, FULL><<<cb, CS << (K1 - E), 0, st>>>(cols);
+    }
+    launch_check("grouped ntt launch");
+}
+
+template <int LOGN>
+void launch_modup_keyprod_groups(const KsGroup *gs, uint32_t n, const uint64_t *key, uint32_t K_key,
+                                 const DevPrime *digits, hipStream_t st) {
+    constexpr int K1 = LOGN / 2;
+    constexpr int K2 = LOGN - K1;
+    constexpr int E = 4;
+    constexpr int CS = 16;
+    constexpr int RS = 4096 >> K2;
+    // columns pass: targets per block chosen for the whole launch's tile blocks
+    uint64_t tile_blocks = 0;
+    uint32_t maxL = 0;
+    for (uint32_t g = 0; g < n; g++) {
+        tile_blocks += (uint64_t)gs[g].count * gs[g].L * ((1u << K2) / CS);
+        maxL = gs[g].L > maxL ? gs[g].L : maxL;
+    }
+    const uint64_t want = (16384 + tile_blocks - 1) / tile_blocks;
+    ModupGroups mg{};
+    mg.digits = digits;
+    std::vector<uint64_t> msz;
+    bool all_one = true;
+    for (uint32_t g = 0; g < n; g++) {
+        const KsGroup &a = gs[g];
+        const uint32_t chunks = (uint32_t)(want < 1 ? 1 : (want > a.L + 1 ? a.L + 1 : want));
+        const uint32_t tc = (a.L + chunks) / chunks;
+        all_one = all_one && tc == 1;
+        mg.g[g] = ModupGroup{a.d, a.d_ct_stride, a.T, a.targets, a.count, a.L, tc, a.diag != nullptr};
+        msz.push_back((uint64_t)a.count * a.L * ((a.L + tc) / tc) * ((1u << K2) / CS));
+    }
+    const uint32_t cb = fill_starts(mg, msz);
+    if (all_one) ntt_cols_modup_grouped_kernel<LOGN, K1, E, CS, true><<<cb, CS << (K1 - E), 0, st>>>(mg);
+    else ntt_cols_modup_grouped_kernel<LOGN, K1, E, CS, false><<<cb, CS << (K1 - E), 0, st>>>(mg);
+    // fused rows pass + key products: groups of 4, then of 2, then single ciphertexts
+    KeyprodGroups t4{}, t2{}, t1{};
+    t4.key = t2.key = t1.key = key;
+    t4.K_key = t2.K_key = t1.K_key = K_key;
+    std::vector<uint64_t> s4, s2, s1;
+    for (uint32_t g = 0; g < n; g++) {
+        const KsGroup &a = gs[g];
+        const uint32_t tiles = (a.L + 1) << K1;
+        const uint64_t tile_slots = (uint64_t)((tiles + 7) / 8) * 8;
+        const uint32_t g4 = a.count / 4, g2 = (a.count - 4 * g4) / 2, singles = a.count - 4 * g4 - 2 * g2;
+        if (g4) {
+            t4.g[s4.size()] = KeyprodGroup{a.T, a.P, a.S, a.targets, a.diag, a.diag_stride, a.count, a.L, 0, g4, tiles};
+            s4.push_back(tile_slots * g4);
+        }
+        if (g2) {
+            t2.g[s2.size()] =
+                KeyprodGroup{a.T, a.P, a.S, a.targets, a.diag, a.diag_stride, a.count, a.L, 4 * g4, g2, tiles};
+            s2.push_back(tile_slots * g2);
+        }
+        if (singles) {
+            const uint32_t tiles2 = tiles / 2;
+            const uint64_t slots2 = (tiles2 + 7) / 8 * 8;
+            t1.g[s1.size()] = KeyprodGroup{a.T, a.P, a.S, a.targets, a.diag, a.diag_stride, a.count, a.L,
+                                           a.count - singles, singles, tiles2};
+            s1.push_back(slots2 * singles);
+        }
+    }
+    const bool r4 = maxL > 31;
+    if (!s4.empty()) {
+        const uint32_t b = fill_starts(t4, s4);
+        if (r4) ntt_rows_keyprod_grouped_kernel<LOGN, K2, E, RS, 4, 1, true><<<b, RS << (K2 - E), 0, st>>>(t4);
+        else ntt_rows_keyprod_grouped_kernel<LOGN, K2, E, RS, 4, 1, false><<<b, RS << (K2 - E), 0, st>>>(t4);
+    }
+    if (!s2.empty()) {
+        const uint32_t b = fill_starts(t2, s2);
+        if (r4) ntt_rows_keyprod_grouped_kernel<LOGN, K2, E, RS, 2, 1, true><<<b, RS << (K2 - E), 0, st>>>(t2);
+        else ntt_rows_keyprod_grouped_kernel<LOGN, K2, E, RS, 2, 1, false><<<b, RS << (K2 - E), 0, st>>>(t2);
+    }
+    if (!s1.empty()) {
+        const uint32_t b = fill_starts(t1, s1);
+        if (r4) ntt_rows_keyprod_grouped_kernel<LOGN, K2, E, RS, 1, 2, true><<<b, RS << (K2 - E), 0, st>>>(t1);
+        else ntt_rows_keyprod_grouped_kernel<LOGN, K2, E, RS, 1, 2, false><<<b, RS << (K2 - E), 0, st>>>(t1);
+    }
+}
+
+template <int LOGN>
+void launch_moddown_groups(const ModDownGroup *gs, uint32_t n, const DevPrime *primes, hipStream_t st) {
+    constexpr int K1 = LOGN / 2;
+    constexpr int K2 = LOGN - K1;
+    constexpr int E = 4;
+    constexpr int CS = 16;
+    constexpr int RS = 4096 >> K2;
+    ModDownGroups cols{}, rmx{}, rfull{};
+    std::vector<uint64_t> csz, msz, fsz;
+    for (uint32_t g = 0; g < n; g++) {
+        const uint32_t jn = 2 * gs[g].count;
+        const ModDownGroupDev d{gs[g].U, gs[g].a, jn};
+        cols.g[g] = d;
+        csz.push_back((uint64_t)gs[g].a.L * jn * ((1u << K2) / CS));
+        const bool mx = rows_mixed<LOGN, K2, RS>(jn);
+        const uint64_t rb = (uint64_t)gs[g].a.L * rows_blocks_per_prime<LOGN, K2, E, RS>(jn, mx);
+        if (mx) {
+            rmx.g[msz.size()] = d;
+            msz.push_back(rb);
+        } else {
+            rfull.g[fsz.size()] = d;
+            fsz.push_back(rb);
+        }
+    }
+    const uint32_t cb = fill_starts(cols, csz);
+    ntt_cols_moddown_prep_grouped_kernel<LOGN, K1, E, CS><<<cb, CS << (K1 - E), 0, st>>>(primes, cols);
+    if (!msz.empty()) {
+        const uint32_t b = fill_starts(rmx, msz);
+        ntt_rows_moddown_grouped_kernel<LOGN, K2, E, RS, true><<<b, RS << (K2 - E), 0, st>>>(primes, rmx);
+    }
+    if (!fsz.empty()) {
+        const uint32_t b = fill_starts(rfull, fsz);
+        ntt_rows_moddown_grouped_kernel<LOGN, K2, E, RS, false><<<b, RS << (K2 - E), 0, st>>>(primes, rfull);
+    }
+}
+
 LimbGrid poly_grid(uint32_t count, uint32_t lpp) {
     if (lpp == 0 || count % lpp) fail(ERR_INVALID_ARGUMENT, "limb count must be a multiple of limbs per polynomial");
     return LimbGrid{1, lpp, lpp, count / lpp};
@@ -949,6 +1290,78 @@ void ntt_inverse_from(const uint64_t *src, LimbGrid src_grid, uint64_t *data, Li
                       int logn, NttOut out, hipStream_t st) {
     if (out == NTT_FULL) dispatch_inverse_from<true>(src, src_grid, data, grid, primes, logn, st);
     else dispatch_inverse_from<false>(src, src_grid, data, grid, primes, logn, st);
+}
+
+void ntt_forward_groups(const NttGroup *gs, uint32_t n, int logn, NttOut out, hipStream_t st) {
+    if (n > MAX_GROUPS) fail(ERR_INVALID_ARGUMENT, "too many groups for one launch");
+    if (n == 0) return;
+    if (logn <= 12) {
+        for (uint32_t g = 0; g < n; g++) {
+            if (gs[g].src) fail(ERR_INVALID_ARGUMENT, "a forward transform runs in place");
+            ntt_forward_grid(gs[g].data, gs[g].lg, gs[g].primes, logn, out, st);
+        }
+        return;
+    }
+    for (uint32_t g = 0; g < n; g++)
+        if (gs[g].src) fail(ERR_INVALID_ARGUMENT, "a forward transform runs in place");
+    switch (logn) {
+#define SHC_NTT_CASE(L)                                                          \
+    case L:                                                                      \
+        if (out == NTT_FULL) launch_ntt_groups<L, false, true>(gs, n, st);       \
+        else launch_ntt_groups<L, false, false>(gs, n, st);                      \
+        break;
+        SHC_NTT_CASE(13) SHC_NTT_CASE(14) SHC_NTT_CASE(15) SHC_NTT_CASE(16) SHC_NTT_CASE(17)
+#undef SHC_NTT_CASE
+        default: fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree must be 2^1 .. 2^17");
+    }
+}
+
+void ntt_inverse_groups(const NttGroup *gs, uint32_t n, int logn, NttOut out, hipStream_t st) {
+    if (n > MAX_GROUPS) fail(ERR_INVALID_ARGUMENT, "too many groups for one launch");
+    if (n == 0) return;
+    if (logn <= 12) {
+        for (uint32_t g = 0; g < n; g++) {
+            if (gs[g].src) ntt_inverse_from(gs[g].src, gs[g].slg, gs[g].data, gs[g].lg, gs[g].primes, logn, out, st);
+            else ntt_inverse_grid(gs[g].data, gs[g].lg, gs[g].primes, logn, out, st);
+        }
+        return;
+    }
+    switch (logn) {
+#define SHC_NTT_CASE(L)                                                          \
+    case L:                                                                      \
+        if (out == NTT_FULL) launch_ntt_groups<L, true, true>(gs, n, st);        \
+        else launch_ntt_groups<L, true, false>(gs, n, st);                       \
+        break;
+        SHC_NTT_CASE(13) SHC_NTT_CASE(14) SHC_NTT_CASE(15) SHC_NTT_CASE(16) SHC_NTT_CASE(17)
+#undef SHC_NTT_CASE
+        default: fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree must be 2^1 .. 2^17");
+    }
+}
+
+bool ntt_modup_keyprod_groups(const KsGroup *gs, uint32_t n, const uint64_t *key, uint32_t key_count,
+                              const DevPrime *digits, int logn, hipStream_t st) {
+    if (logn != 15 && logn != 16) return false;
+    if (n > MAX_GROUPS) fail(ERR_INVALID_ARGUMENT, "too many groups for one launch");
+    if (n == 0) return true;
+    if (logn == 15) launch_modup_keyprod_groups<15>(gs, n, key, key_count, digits, st);
+    else launch_modup_keyprod_groups<16>(gs, n, key, key_count, digits, st);
+    launch_check("grouped modup keyprod launch");
+    return true;
+}
+
+bool ntt_moddown_groups(const ModDownGroup *gs, uint32_t n, const DevPrime *primes, int logn, hipStream_t st) {
+    if (logn < 13 || logn > 17) return false;
+    if (n > MAX_GROUPS) fail(ERR_INVALID_ARGUMENT, "too many groups for one launch");
+    if (n == 0) return true;
+    switch (logn) {
+        case 13: launch_moddown_groups<13>(gs, n, primes, st); break;
+        case 14: launch_moddown_groups<14>(gs, n, primes, st); break;
+        case 15: launch_moddown_groups<15>(gs, n, primes, st); break;
+        case 16: launch_moddown_groups<16>(gs, n, primes, st); break;
+        case 17: launch_moddown_groups<17>(gs, n, primes, st); break;
+    }
+    launch_check("grouped moddown launch");
+    return true;
 }
 
 }  // namespace shc

@@ -59,4 +59,31 @@ size_t rescale_workspace(const Context &c, uint32_t count, uint32_t polys, uint3
 void rescale_to_next(const Context &c, const uint64_t *in, uint64_t *out, uint32_t count, uint32_t polys, uint32_t L,
                      void *ws, size_t ws_bytes, hipStream_t st);
 
+// Level groups: each group is a batch at one level with its own buffers; a grouped call
+// gives every group the words of the one-level call, with one launch per stage for all
+// groups (CKKS, N = 2^15 and 2^16 for the key switch; other shapes run group by group).
+struct TensorGroup {
+    const uint64_t *a, *b;  // [count][2][L][N]
+    uint64_t *out;          // [count][3][L][N]
+    uint32_t count, L;
+};
+void ckks_tensor_groups(const Context &c, const TensorGroup *groups, uint32_t n, hipStream_t st);
+
+struct RelinGroup {
+    uint64_t *ct;  // [count][3][L][N], relinearized in place (polys 0, 1)
+    uint32_t count, L;
+};
+size_t relinearize_groups_workspace(const Context &c, const RelinGroup *groups, uint32_t n);
+void relinearize_groups(const Context &c, const RelinGroup *groups, uint32_t n, const uint64_t *key, void *ws,
+                        size_t ws_bytes, hipStream_t st);
+
+struct RescaleGroup {
+    const uint64_t *in;  // [count][in_size][L][N]: polys 0 and 1 are rescaled
+    uint64_t *out;       // [count][2][L-1][N]
+    uint32_t count, L;
+};
+size_t rescale_groups_workspace(const Context &c, const RescaleGroup *groups, uint32_t n);
+void rescale_groups(const Context &c, const RescaleGroup *groups, uint32_t n, uint32_t in_size, void *ws,
+                    size_t ws_bytes, hipStream_t st);
+
 }  // namespace shc

@@ -22,6 +22,7 @@
 #include "modarith.h"
 #include "ntt.h"
 #include "rns_ops.h"
+#include <vector>
 
 namespace shc {
 namespace {
@@ -239,6 +240,94 @@ __global__ void gather_polys_kernel(const uint64_t *__restrict__ src, size_t src
     }
 }
 
+// ---- grouped elementwise passes (one launch over several level groups) -------------
+// Group g owns blocks [start[g], start[g+1]), one element per thread.
+struct ElemGroup {
+    const uint64_t *a, *b;
+    uint64_t *out, *aux;
+    size_t sa, sb, so;       // strides (words) between ciphertexts
+    const DivRoundConst *dc;
+    uint64_t p;
+    uint32_t L, count;
+    size_t total;
+};
+struct ElemGroups {
+    uint32_t n;
+    uint32_t start[MAX_GROUPS + 1];
+    ElemGroup g[MAX_GROUPS];
+};
+enum ElemOp { OP_TENSOR, OP_LAST_LIMB, OP_DIVROUND_PREP, OP_RESCALE_FINISH };
+
+__device__ __forceinline__ uint32_t elem_group(const ElemGroups &eg, uint32_t b) {
+    uint32_t g = 0;
+#pragma unroll 1
+    while (g + 1 < eg.n && b >= eg.start[g + 1]) g++;
+    return g;
+}
+
+//   OP_TENSOR:         out[ct] = (a0 b0, a0 b1 + a1 b0, a1 b1) at level L (a, b: 2 polys, out: 3)
+//   OP_LAST_LIMB:      out[ct][k] = a[ct][k][L-1]              (k < 2; a: stride sa)
+//   OP_DIVROUND_PREP:  out[ct][k][i] = ([b[ct][k] + p/2]_p mod q_i) - (p/2 mod q_i), i < L
+//   OP_RESCALE_FINISH: out[ct][k][i] = (a[ct][k][i] - aux[ct][k][i]) p^-1, i < L (a: L+1 limbs)
+template <int OP>
+__global__ void elem_grouped_kernel(ElemGroups eg, const DevPrime *__restrict__ primes, int logn) {
+    const uint32_t gi = elem_group(eg, blockIdx.x);
+    const ElemGroup &e = eg.g[gi];
+    const size_t idx = (size_t)(blockIdx.x - eg.start[gi]) * TPB + threadIdx.x;
+    if (idx >= e.total) return;
+    const size_t n = (size_t)1 << logn;
+    const size_t c = idx & (n - 1);
+    const size_t r = idx >> logn;
+    if constexpr (OP == OP_TENSOR) {
+        const size_t ct = r / e.L;
+        const uint32_t i = (uint32_t)(r % e.L);
+        const size_t lw = (size_t)e.L << logn;
+        const DevPrime &P = primes[i];
+        const uint64_t *pa = e.a + ct * e.sa + ((size_t)i << logn) + c, *pb = e.b + ct * e.sb + ((size_t)i << logn) + c;
+        const uint64_t a0 = pa[0], a1 = pa[lw], b0 = pb[0], b1 = pb[lw];
+        uint64_t *po = e.out + ct * e.so + ((size_t)i << logn) + c;
+        po[0] = mulmod(a0, b0, P.q, P.ratio_lo, P.ratio_hi);
+        u128 cross = mul64x64(a0, b1);
+        add128(cross, mul64x64(a1, b0));
+        po[lw] = barrett128(cross, P.q, P.ratio_lo, P.ratio_hi);
+        po[2 * lw] = mulmod(a1, b1, P.q, P.ratio_lo, P.ratio_hi);
+    } else if constexpr (OP == OP_LAST_LIMB) {
+        const size_t ct = r >> 1, k = r & 1;
+        e.out[idx] = e.a[ct * e.sa + ((k * e.L + e.L - 1) << logn) + c];
+    } else if constexpr (OP == OP_DIVROUND_PREP) {
+        const uint32_t i = (uint32_t)(r % e.L);
+        const size_t g = r / e.L;
+        const DevPrime &Q = primes[i];
+        uint64_t v = e.b[g * n + c] + (e.p >> 1);
+        v = v >= e.p ? v - e.p : v;
+        v = barrett64(v, Q.q, Q.ratio_hi);
+        e.out[idx] = submod(v, e.dc[i].half_mod, Q.q);
+    } else {
+        const uint32_t i = (uint32_t)(r % e.L);
+        const size_t g = r / e.L, ct = g >> 1, k = g & 1;
+        const DevPrime &Q = primes[i];
+        const uint64_t x = submod(e.a[ct * e.sa + ((k * (e.L + 1) + i) << logn) + c], e.aux[idx], Q.q);
+        e.out[ct * e.so + ((k * e.L + i) << logn) + c] = shoup(x, e.dc[i].inv, e.dc[i].inv_shoup, Q.q);
+    }
+}
+
+template <int OP>
+void launch_elem_groups(const std::vector<ElemGroup> &gs, const DevPrime *primes, int logn, hipStream_t st) {
+    ElemGroups eg{};
+    eg.n = (uint32_t)gs.size();
+    uint64_t acc = 0;
+    for (size_t g = 0; g < gs.size(); g++) {
+        eg.g[g] = gs[g];
+        eg.start[g] = (uint32_t)acc;
+        acc += (gs[g].total + TPB - 1) / TPB;
+    }
+    eg.start[gs.size()] = (uint32_t)acc;
+    if (acc == 0) return;
+    if (acc >= (uint64_t(1) << 32) / TPB) fail(ERR_INVALID_ARGUMENT, "grouped launch too large");
+    elem_grouped_kernel<OP><<<(uint32_t)acc, TPB, 0, st>>>(eg, primes, logn);
+    launch_check("grouped elementwise");
+}
+
 struct KsLayout {
     size_t d, T, P, S, U, total;
 };
@@ -440,6 +529,184 @@ void rescale_to_next(const Context &c, const uint64_t *in, uint64_t *out, uint32
     rescale_finish_kernel<<<blocks_for(uwords), TPB, 0, st>>>(in, U, out, primes, c.rescale_consts(L), L - 1, logn,
                                                              uwords);
     launch_check("rescale_finish");
+}
+
+// ---- level groups in one launch per stage ------------------------------------------
+
+namespace {
+template <class F>
+void in_chunks(uint32_t n, F &&f) {
+    for (uint32_t g0 = 0; g0 < n; g0 += MAX_GROUPS) f(g0, n - g0 < MAX_GROUPS ? n - g0 : MAX_GROUPS);
+}
+void check_group_level(const Context &c, uint32_t L, uint32_t lo) {
+    if (L < lo || L > c.key_count()) fail(ERR_INVALID_ARGUMENT, "limb_count out of range for a group");
+}
+}  // namespace
+
+void ckks_tensor_groups(const Context &c, const TensorGroup *gs, uint32_t n, hipStream_t st) {
+    if (c.scheme != SCHEME_CKKS) fail(ERR_INVALID_ARGUMENT, "grouped tensor products are CKKS (NTT form)");
+    in_chunks(n, [&](uint32_t g0, uint32_t m) {
+        std::vector<ElemGroup> eg;
+        for (uint32_t g = g0; g < g0 + m; g++) {
+            const TensorGroup &t = gs[g];
+            check_group_level(c, t.L, 1);
+            const size_t lw = (size_t)t.L * c.n;
+            ElemGroup e{};
+            e.a = t.a;
+            e.b = t.b;
+            e.out = t.out;
+            e.sa = e.sb = 2 * lw;
+            e.so = 3 * lw;
+            e.L = t.L;
+            e.count = t.count;
+            e.total = (size_t)t.count * lw;
+            eg.push_back(e);
+        }
+        launch_elem_groups<OP_TENSOR>(eg, c.primes(), c.logn, st);
+    });
+}
+
+size_t relinearize_groups_workspace(const Context &c, const RelinGroup *gs, uint32_t n) {
+    size_t total = 0;
+    for (uint32_t g = 0; g < n; g++) total += ks_layout(c, gs[g].count, gs[g].L).total;
+    return total;
+}
+
+void relinearize_groups(const Context &c, const RelinGroup *gs, uint32_t n, const uint64_t *key, void *ws,
+                        size_t ws_bytes, hipStream_t st) {
+    const size_t K = c.key_count();
+    if (K < 2) fail(ERR_LOGIC, "keyswitching is not supported by the context");
+    for (uint32_t g = 0; g < n; g++) {
+        if (gs[g].L < 1 || gs[g].L > K - 1) fail(ERR_INVALID_ARGUMENT, "level out of range for key switching");
+        if (!gs[g].ct && gs[g].count) fail(ERR_INVALID_ARGUMENT, "null ciphertext group");
+    }
+    if (!ws || ws_bytes < relinearize_groups_workspace(c, gs, n))
+        fail(ERR_INVALID_ARGUMENT, "workspace too small for key switching");
+    const int logn = c.logn;
+    const size_t nn = c.n;
+    // one launch per stage needs the fused CKKS path and every group within one launch
+    bool grouped = c.scheme == SCHEME_CKKS && (logn == 15 || logn == 16);
+    for (uint32_t g = 0; g < n && grouped; g++)
+        grouped = (uint64_t)gs[g].count * gs[g].L * (gs[g].L + 1) <= ((uint64_t(1) << 34) >> logn);
+    char *base = static_cast<char *>(ws);
+    if (!grouped) {
+        for (uint32_t g = 0; g < n; g++) {
+            const size_t poly = (size_t)gs[g].L * nn, bytes = ks_layout(c, gs[g].count, gs[g].L).total;
+            switch_key(c, gs[g].ct + 2 * poly, 3 * poly, gs[g].ct, 3 * poly, key, gs[g].count, gs[g].L, base, bytes, st);
+            base += bytes;
+        }
+        return;
+    }
+    in_chunks(n, [&](uint32_t g0, uint32_t m) {
+        NttGroup dg[MAX_GROUPS], sg[MAX_GROUPS];
+        KsGroup kg[MAX_GROUPS];
+        ModDownGroup mg[MAX_GROUPS];
+        uint32_t k = 0;
+        for (uint32_t g = g0; g < g0 + m; g++) {
+            const RelinGroup &r = gs[g];
+            const uint32_t L = r.L;
+            const KsLayout lay = ks_layout(c, r.count, L);
+            char *const gb = base;  // this group's workspace
+            auto at = [gb](size_t off) { return reinterpret_cast<uint64_t *>(gb + off); };
+            base += lay.total;
+            if (!r.count) continue;
+            const size_t poly = (size_t)L * nn;
+            uint64_t *target = r.ct + 2 * poly;
+            dg[k] = NttGroup{at(lay.d), target, LimbGrid{1, L, L, r.count}, LimbGrid{1, 3 * L, L, r.count}, c.primes()};
+            kg[k] = KsGroup{at(lay.d), poly, at(lay.T), at(lay.P), at(lay.S), target, 3 * poly, c.keyswitch_primes(L),
+                            r.count, L};
+            sg[k] = NttGroup{at(lay.S), nullptr, LimbGrid{1, 1, 1, 2 * r.count}, LimbGrid{1, 1, 1, 2 * r.count},
+                             c.keyswitch_primes(L) + L};
+            mg[k] = ModDownGroup{at(lay.U),
+                                 ModDownArgs{at(lay.S), at(lay.P), r.ct, 3 * poly, c.ks_consts(), c.moduli[K - 1], L},
+                                 r.count};
+            k++;
+        }
+        if (!k) return;
+        // 1. digits: the c2 polynomials to coefficient form (they stay in NTT form for the
+        // own-prime terms); 2. ModUp fused with the key products; 3. ModDown
+        ntt_inverse_groups(dg, k, logn, NTT_FULL, st);
+        ntt_modup_keyprod_groups(kg, k, key, (uint32_t)K, c.primes(), logn, st);
+        ntt_inverse_groups(sg, k, logn, NTT_FULL, st);
+        ntt_moddown_groups(mg, k, c.primes(), logn, st);
+    });
+}
+
+size_t rescale_groups_workspace(const Context &c, const RescaleGroup *gs, uint32_t n) {
+    size_t total = 0;
+    for (uint32_t g = 0; g < n; g++) {
+        if (gs[g].L < 2) continue;
+        total += rescale_workspace(c, gs[g].count, 2, gs[g].L);
+    }
+    return total;
+}
+
+void rescale_groups(const Context &c, const RescaleGroup *gs, uint32_t n, uint32_t in_size, void *ws, size_t ws_bytes,
+                    hipStream_t st) {
+    if (in_size < 2) fail(ERR_INVALID_ARGUMENT, "ciphertexts of at least two polynomials are rescaled");
+    for (uint32_t g = 0; g < n; g++) {
+        if (gs[g].L < 2 || gs[g].L > c.key_count()) fail(ERR_INVALID_ARGUMENT, "end of modulus switching chain reached");
+        if (gs[g].count && (!gs[g].in || !gs[g].out)) fail(ERR_INVALID_ARGUMENT, "null ciphertext group");
+        if (gs[g].in == gs[g].out && gs[g].count) fail(ERR_INVALID_ARGUMENT, "rescale cannot run in place");
+    }
+    if (!ws || ws_bytes < rescale_groups_workspace(c, gs, n)) fail(ERR_INVALID_ARGUMENT, "workspace too small for rescale");
+    const bool ckks = c.scheme == SCHEME_CKKS;
+    const int logn = c.logn;
+    const size_t nn = c.n;
+    const DevPrime *primes = c.primes();
+    char *base = static_cast<char *>(ws);
+    in_chunks(n, [&](uint32_t g0, uint32_t m) {
+        std::vector<ElemGroup> last, prep, fin;
+        NttGroup sg[MAX_GROUPS], ug[MAX_GROUPS];
+        uint32_t k = 0;
+        for (uint32_t g = g0; g < g0 + m; g++) {
+            const RescaleGroup &r = gs[g];
+            const uint32_t L = r.L, Lo = L - 1;
+            const size_t bytes = rescale_workspace(c, r.count, 2, L);
+            uint64_t *S = reinterpret_cast<uint64_t *>(base);
+            uint64_t *U = reinterpret_cast<uint64_t *>(base + ((2 * (size_t)r.count * nn * 8 + 255) & ~size_t(255)));
+            base += bytes;
+            if (!r.count) continue;
+            const size_t sa = (size_t)in_size * L * nn;
+            ElemGroup e{};
+            e.a = r.in;
+            e.sa = sa;
+            e.out = S;
+            e.L = L;
+            e.count = r.count;
+            e.total = 2 * (size_t)r.count * nn;
+            last.push_back(e);
+            ElemGroup p{};
+            p.b = S;
+            p.out = U;
+            p.dc = c.rescale_consts(L);
+            p.p = c.moduli[L - 1];
+            p.L = Lo;
+            p.count = r.count;
+            p.total = 2 * (size_t)r.count * Lo * nn;
+            prep.push_back(p);
+            ElemGroup f{};
+            f.a = r.in;
+            f.sa = sa;
+            f.aux = U;
+            f.out = r.out;
+            f.so = 2 * (size_t)Lo * nn;
+            f.dc = c.rescale_consts(L);
+            f.L = Lo;
+            f.count = r.count;
+            f.total = 2 * (size_t)r.count * Lo * nn;
+            fin.push_back(f);
+            sg[k] = NttGroup{S, nullptr, LimbGrid{1, 1, 1, 2 * r.count}, LimbGrid{1, 1, 1, 2 * r.count}, primes + (L - 1)};
+            ug[k] = NttGroup{U, nullptr, LimbGrid{1, Lo, Lo, 2 * r.count}, LimbGrid{1, Lo, Lo, 2 * r.count}, primes};
+            k++;
+        }
+        if (!k) return;
+        launch_elem_groups<OP_LAST_LIMB>(last, primes, logn, st);
+        if (ckks) ntt_inverse_groups(sg, k, logn, NTT_FULL, st);
+        launch_elem_groups<OP_DIVROUND_PREP>(prep, primes, logn, st);
+        if (ckks) ntt_forward_groups(ug, k, logn, NTT_FULL, st);
+        launch_elem_groups<OP_RESCALE_FINISH>(fin, primes, logn, st);
+    });
 }
 
 }  // namespace shc

@@ -16,57 +16,6 @@ constexpr size_t GRAIN = 256;
 size_t round_up(size_t x) { return (x + GRAIN - 1) / GRAIN * GRAIN; }
 }  // namespace
 
-// ------------------------------------------------------------------ RangeAllocator
-
-void RangeAllocator::reset(size_t bytes) {
-    free_.clear();
-    taken_.clear();
-    total_ = bytes / GRAIN * GRAIN;
-    used_ = 0;
-    if (total_) free_[0] = total_;
-}
-
-size_t RangeAllocator::take(size_t bytes) {
-    bytes = round_up(std::max<size_t>(bytes, 1));
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-        if (it->second < bytes) continue;
-        const size_t off = it->first, rest = it->second - bytes;
-        free_.erase(it);
-        if (rest) free_[off + bytes] = rest;
-        taken_[off] = bytes;
-        used_ += bytes;
-        return off;
-    }
-    return SIZE_MAX;
-}
-
-void RangeAllocator::give(size_t off) {
-    auto t = taken_.find(off);
-    if (t == taken_.end()) fail(ERR_LOGIC, "release of a block that is not taken");
-    size_t size = t->second;
-    taken_.erase(t);
-    used_ -= size;
-    auto next = free_.lower_bound(off);
-    if (next != free_.end() && off + size == next->first) {
-        size += next->second;
-        next = free_.erase(next);
-    }
-    if (next != free_.begin()) {
-        auto prev = std::prev(next);
-        if (prev->first + prev->second == off) {
-            prev->second += size;
-            return;
-        }
-    }
-    free_[off] = size;
-}
-
-size_t RangeAllocator::largest_free() const {
-    size_t best = 0;
-    for (const auto &f : free_) best = std::max(best, f.second);
-    return best;
-}
-
 // ------------------------------------------------------------------ programs
 
 namespace {

@@ -12,6 +12,7 @@
 // run are admitted as soon as a lane and memory free up.
 #pragma once
 #include "context.h"
+#include "range_alloc.h"
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -23,25 +24,6 @@
 #include <vector>
 
 namespace shc {
-
-// First-fit allocator over one contiguous range (offsets in bytes, 256-byte granules),
-// coalescing neighbours on release.
-class RangeAllocator {
-public:
-    explicit RangeAllocator(size_t bytes = 0) { reset(bytes); }
-    void reset(size_t bytes);
-    // Offset of a free block of `bytes`, or SIZE_MAX when none is large enough.
-    size_t take(size_t bytes);
-    void give(size_t offset);
-    size_t largest_free() const;
-    size_t in_use() const { return used_; }
-    size_t capacity() const { return total_; }
-
-private:
-    std::map<size_t, size_t> free_;   // offset -> size
-    std::map<size_t, size_t> taken_;  // offset -> size
-    size_t total_ = 0, used_ = 0;
-};
 
 // One kind of request the server runs in batches.
 struct ServerProgram {

@@ -70,6 +70,10 @@ def lib():
         L.shc_batch_decode.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.shc_ckks_encode_workspace_size.argtypes = [_vp, ctypes.POINTER(sz)]
         L.shc_ckks_encode.argtypes = [_vp, _vp, u32, ctypes.c_double, u32, _vp, _vp, sz, _vp]
+        L.shc_ckks_encode_many_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
+        L.shc_ckks_encode_many.argtypes = [_vp, _vp, u32, sz, u32, ctypes.c_double, u32, _vp, _vp, _vp, sz, _vp]
+        L.shc_ckks_decode_many_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
+        L.shc_ckks_decode_many.argtypes = [_vp, _vp, u32, u32, ctypes.c_double, _vp, _vp, sz, _vp]
         L.shc_ckks_encode_deferred.argtypes = [_vp, _vp, u32, ctypes.c_double, u32, _vp, _vp, _vp, sz, _vp]
         L.shc_ckks_decode_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
         L.shc_ckks_decode.argtypes = [_vp, _vp, u32, ctypes.c_double, _vp, _vp, sz, _vp]
@@ -101,6 +105,12 @@ def lib():
         L.shc_keyswitch_shape_get.argtypes = [_vp, u32, u32, ctypes.POINTER(KsShape)]
         L.shc_keyswitch_shapes_export.argtypes = [_vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         L.shc_keyswitch_shapes_import.argtypes = [_vp, ctypes.c_char_p]
+        gp = ctypes.POINTER(CtGroup)
+        L.shc_ckks_multiply_groups.argtypes = [_vp, gp, gp, gp, u32, _vp]
+        L.shc_relinearize_groups_workspace_size.argtypes = [_vp, gp, u32, ctypes.POINTER(sz)]
+        L.shc_relinearize_groups.argtypes = [_vp, gp, u32, _vp, _vp, sz, _vp]
+        L.shc_rescale_groups_workspace_size.argtypes = [_vp, gp, u32, ctypes.POINTER(sz)]
+        L.shc_rescale_to_next_groups.argtypes = [_vp, gp, u32, gp, u32, _vp, sz, _vp]
         L.shc_server_create.argtypes = [ctypes.c_int, sz, sz, u32, u32, ctypes.POINTER(_vp)]
         L.shc_server_destroy.argtypes = [_vp]
         L.shc_server_destroy.restype = None
@@ -117,6 +127,18 @@ def lib():
         L.shc_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
         _lib = L
     return _lib
+
+
+class CtGroup(ctypes.Structure):
+    _fields_ = [("d_ct", ctypes.c_void_p), ("count", ctypes.c_uint32), ("limb_count", ctypes.c_uint32)]
+
+
+def groups(items):
+    """A ctypes array of CtGroup from (device pointer, count, limb_count) triples."""
+    arr = (CtGroup * max(len(items), 1))()
+    for k, (p, c, l) in enumerate(items):
+        arr[k] = CtGroup(p, c, l)
+    return arr
 
 
 class KsShape(ctypes.Structure):
@@ -323,6 +345,21 @@ class Context:
         check(lib().shc_ckks_encode_deferred(self.handle, d_values, count, scale, limbs, d_plain, d_status, d_ws,
                                              ws_bytes, stream))
 
+    def ckks_encode_many_workspace_size(self, plaintexts):
+        return self._size(lib().shc_ckks_encode_many_workspace_size, plaintexts)
+
+    def ckks_encode_many(self, d_values, count, values_stride, plaintexts, scale, limbs, d_plain, d_status, d_ws,
+                         ws_bytes, stream=None):
+        check(lib().shc_ckks_encode_many(self.handle, d_values, count, values_stride, plaintexts, scale, limbs,
+                                         d_plain, d_status, d_ws, ws_bytes, stream))
+
+    def ckks_decode_many_workspace_size(self, limbs, plaintexts):
+        return self._size(lib().shc_ckks_decode_many_workspace_size, limbs, plaintexts)
+
+    def ckks_decode_many(self, d_plain, limbs, plaintexts, scale, d_values, d_ws, ws_bytes, stream=None):
+        check(lib().shc_ckks_decode_many(self.handle, d_plain, limbs, plaintexts, scale, d_values, d_ws, ws_bytes,
+                                         stream))
+
     def ckks_decode_workspace_size(self, limbs):
         return self._size(lib().shc_ckks_decode_workspace_size, limbs)
 
@@ -427,6 +464,27 @@ class Context:
 
     def keyswitch_shapes_import(self, text):
         check(lib().shc_keyswitch_shapes_import(self.handle, text.encode()))
+
+    # ---- level groups: lists of (device pointer, count, limb_count)
+    def ckks_multiply_groups(self, a, b, out, stream=None):
+        check(lib().shc_ckks_multiply_groups(self.handle, groups(a), groups(b), groups(out), len(a), stream))
+
+    def relinearize_groups_workspace_size(self, cts):
+        n = ctypes.c_size_t()
+        check(lib().shc_relinearize_groups_workspace_size(self.handle, groups(cts), len(cts), ctypes.byref(n)))
+        return n.value
+
+    def relinearize_groups(self, cts, d_key, d_ws, ws_bytes, stream=None):
+        check(lib().shc_relinearize_groups(self.handle, groups(cts), len(cts), d_key, d_ws, ws_bytes, stream))
+
+    def rescale_groups_workspace_size(self, cts):
+        n = ctypes.c_size_t()
+        check(lib().shc_rescale_groups_workspace_size(self.handle, groups(cts), len(cts), ctypes.byref(n)))
+        return n.value
+
+    def rescale_to_next_groups(self, cts, in_size, out, d_ws, ws_bytes, stream=None):
+        check(lib().shc_rescale_to_next_groups(self.handle, groups(cts), in_size, groups(out), len(cts), d_ws,
+                                               ws_bytes, stream))
 
     def probe_keyswitch(self, begin, end):
         """Arm the next key switch to record HipEvents around its ModUp rows pass."""

@@ -233,3 +233,56 @@ def test_encode_deferred_check_and_graph_capture():
                                  eb.data_ptr(), ew, stream())
         torch.cuda.synchronize()
         assert int(status.item()) == -1
+
+
+def test_encode_decode_many_equal_single_calls():
+    """A batch of plaintexts in one call (one launch per FFT layer for all of them) gives
+    each plaintext the words of its own shc_ckks_encode, and the batched decode the
+    values of shc_ckks_decode; per-plaintext status words flag exactly the plaintext
+    whose values are too large."""
+    logn, n = 13, 1 << 13
+    moduli = ck.coeff_modulus_create(n, [60, 40, 40, 40, 60])
+    L = 4
+    ctx = hip().Context(CKKS, n, moduli)
+    rng = np.random.default_rng(91)
+    P, count, stride = 5, 3000, 2 * 4096
+    vals = np.zeros((P, stride))
+    for p in range(P):
+        vals[p, :2 * count] = interleave(slots(rng, count))
+    dv = to_dev_f64(vals)
+    ew = ctx.ckks_encode_many_workspace_size(P)
+    eb = ws(ew)
+    plains = empty_dev((P, L, n))
+    ctx.ckks_encode_many(dv.data_ptr(), count, stride, P, 2.0 ** 40, L, plains.data_ptr(), None, eb.data_ptr(), ew,
+                         stream())
+    got = from_dev(plains)
+    ew1 = ctx.ckks_encode_workspace_size()
+    eb1 = ws(ew1)
+    one = empty_dev((L, n))
+    for p in range(P):
+        ctx.ckks_encode(dv[p].data_ptr(), count, 2.0 ** 40, L, one.data_ptr(), eb1.data_ptr(), ew1, stream())
+        np.testing.assert_array_equal(got[p], from_dev(one), err_msg=f"plaintext {p}")
+    dw = ctx.ckks_decode_many_workspace_size(L, P)
+    db = ws(dw)
+    out = torch.empty((P, n), dtype=torch.float64, device="cuda")
+    ctx.ckks_decode_many(plains.data_ptr(), L, P, 2.0 ** 40, out.data_ptr(), db.data_ptr(), dw, stream())
+    dw1 = ctx.ckks_decode_workspace_size(L)
+    db1 = ws(dw1)
+    o1 = torch.empty((n,), dtype=torch.float64, device="cuda")
+    for p in range(P):
+        ctx.ckks_decode(plains[p].data_ptr(), L, 2.0 ** 40, o1.data_ptr(), db1.data_ptr(), dw1, stream())
+        torch.cuda.synchronize()
+        assert torch.equal(out[p], o1), f"decode of plaintext {p}"
+    # one plaintext far beyond the modulus: its status alone is -1 and it is left zero
+    vals[3, :2 * count] *= 2.0 ** 100
+    dv = to_dev_f64(vals)
+    status = torch.full((P,), 7, dtype=torch.int32, device="cuda")
+    ctx.ckks_encode_many(dv.data_ptr(), count, stride, P, 2.0 ** 40, L, plains.data_ptr(), status.data_ptr(),
+                         eb.data_ptr(), ew, stream())
+    assert status.cpu().tolist() == [0, 0, 0, -1, 0]
+    g2 = from_dev(plains)
+    assert not g2[3].any()
+    np.testing.assert_array_equal(g2[0], got[0])
+    with pytest.raises(hip().SealHipError, match="too large"):
+        ctx.ckks_encode_many(dv.data_ptr(), count, stride, P, 2.0 ** 40, L, plains.data_ptr(), None, eb.data_ptr(),
+                             ew, stream())
