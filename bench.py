@@ -44,7 +44,7 @@ KEYPROD_TRAFFIC_B128 = 40.93e9 + 3.36e9
 # file) and the VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles
 # per SIMD at the 2.4 GHz peak engine clock.  The kernel is bound by VALU issue (exact
 # 64-bit modular arithmetic from 32-bit multiplies), so this is its figure of merit.
-KEYPROD_VALU_B128 = 8084848640
+KEYPROD_VALU_B128 = 8081571840
 VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 CKKS = 2
 T_BFV = 65537
