@@ -56,7 +56,7 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="mulrelin",
-                   choices=["mulrelin", "ntt", "copy", "chain", "rotsum", "matvec", "serve", "bfv8192"])
+                   choices=["mulrelin", "ntt", "copy", "chain", "rotsum", "matvec", "serve", "bfv8192", "bfv"])
     p.add_argument("--no-bfv", action="store_true", help="skip the BFV leg of the mulrelin workload")
     p.add_argument("--batch", type=int, default=128)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -516,6 +516,19 @@ def bench_bfv_leg(torch, sealhip, B, steps, warmup):
             "self_check": f"batched == per-ciphertext words on pairs {probe}",
             "config": f"BFV multiply (BEHZ) + relinearize, N=2^16, 24 data + 1 special 60-bit primes, t={t}, "
                       f"batch of {B} ciphertext pairs, coefficient form, key resident"}
+
+
+def bench_bfv(args):
+    """The BFV leg of the headline alone (for profiling): BFV multiply (BEHZ) + relinearize,
+    N=2^16, 24+1 primes of 60 bits, t = 65537, batch --batch."""
+    import torch
+    import sealhip
+    sealhip.init(0)
+    r = bench_bfv_leg(torch, sealhip, args.batch, args.steps, args.warmup)
+    return {"metric": "BFV mul+relin ops/s at N=2^16, L=24", "value": r["ops_s"], "unit": r["unit"], "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded uniform residues)",
+            "config": {"workload": r["config"], "parallelism": "single GPU"}, "self_check": r["self_check"]}
 
 
 def bench_chain(args):
@@ -1411,7 +1424,7 @@ def main():
         sys.exit(2)
     out = {"mulrelin": bench_default, "ntt": bench_ntt, "copy": bench_copy, "chain": bench_chain,
            "rotsum": bench_rotsum, "matvec": bench_matvec, "serve": bench_serve,
-           "bfv8192": bench_bfv8192}[args.workload](args)
+           "bfv8192": bench_bfv8192, "bfv": bench_bfv}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

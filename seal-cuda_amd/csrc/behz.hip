@@ -359,9 +359,6 @@ void bfv_multiply(const Context &c, const uint64_t *a, const uint64_t *b, uint64
     const size_t n = c.n;
     const DevPrime *qp = c.primes();
     const DevPrime *bp = B.bsk_primes;
-    const size_t qwords = (size_t)count * 2 * L * n;
-    SHC_HIP(hipMemcpyAsync(Aq, a, qwords * 8, hipMemcpyDeviceToDevice, st));
-    SHC_HIP(hipMemcpyAsync(Bq, b, qwords * 8, hipMemcpyDeviceToDevice, st));
     const size_t ext = (size_t)count * 2 * n;
     const size_t lds = (size_t)(L > B.bsz ? L : B.bsz) * BT * 8;
     if (out_group(B.bsk) == 5) {
@@ -372,14 +369,33 @@ void bfv_multiply(const Context &c, const uint64_t *a, const uint64_t *b, uint64
         behz_extend_kernel<4><<<grid_for(ext), BT, lds, st>>>(b, Bb, B, qp, logn, ext);
     }
     launch_check("behz_extend");
-    ntt_forward(Aq, count * 2 * L, L, qp, logn, NTT_FULL, st);
-    ntt_forward(Bq, count * 2 * L, L, qp, logn, NTT_FULL, st);
-    ntt_forward(Ab, count * 2 * bsk, bsk, bp, logn, NTT_FULL, st);
-    ntt_forward(Bb, count * 2 * bsk, bsk, bp, logn, NTT_FULL, st);
+    // the four forward transforms in one launch per pass, a and b read in place (their
+    // transforms land in Aq, Bq); then the two tensor products and the two inverse
+    // transforms
+    const LimbGrid gq{1, L, L, count * 2}, gb{1, bsk, bsk, count * 2};
+    const NttGroup fwd[4] = {{Aq, a, gq, gq, qp}, {Bq, b, gq, gq, qp}, {Ab, nullptr, gb, gb, bp},
+                             {Bb, nullptr, gb, gb, bp}};
+    if ((uint64_t)count * 2 * (L > bsk ? L : bsk) <= ((uint64_t(1) << 30) >> logn)) {
+        ntt_forward_groups(fwd, 4, logn, NTT_FULL, st);
+    } else {
+        const size_t qwords = (size_t)count * 2 * L * n;
+        SHC_HIP(hipMemcpyAsync(Aq, a, qwords * 8, hipMemcpyDeviceToDevice, st));
+        SHC_HIP(hipMemcpyAsync(Bq, b, qwords * 8, hipMemcpyDeviceToDevice, st));
+        ntt_forward(Aq, count * 2 * L, L, qp, logn, NTT_FULL, st);
+        ntt_forward(Bq, count * 2 * L, L, qp, logn, NTT_FULL, st);
+        ntt_forward(Ab, count * 2 * bsk, bsk, bp, logn, NTT_FULL, st);
+        ntt_forward(Bb, count * 2 * bsk, bsk, bp, logn, NTT_FULL, st);
+    }
     tensor_strided(qp, Aq, 2 * L * n, Bq, 2 * L * n, Tq, 3 * L * n, count, L, logn, st);
     tensor_strided(bp, Ab, 2 * bsk * n, Bb, 2 * bsk * n, Tb, 3 * bsk * n, count, bsk, logn, st);
-    ntt_inverse(Tq, count * 3 * L, L, qp, logn, NTT_FULL, st);
-    ntt_inverse(Tb, count * 3 * bsk, bsk, bp, logn, NTT_FULL, st);
+    const LimbGrid iq{1, L, L, count * 3}, ib{1, bsk, bsk, count * 3};
+    const NttGroup inv[2] = {{Tq, nullptr, iq, iq, qp}, {Tb, nullptr, ib, ib, bp}};
+    if ((uint64_t)count * 3 * (L > bsk ? L : bsk) <= ((uint64_t(1) << 30) >> logn)) {
+        ntt_inverse_groups(inv, 2, logn, NTT_FULL, st);
+    } else {
+        ntt_inverse(Tq, count * 3 * L, L, qp, logn, NTT_FULL, st);
+        ntt_inverse(Tb, count * 3 * bsk, bsk, bp, logn, NTT_FULL, st);
+    }
     const size_t fl = (size_t)count * 3 * n;
     if (out_group(B.bsk) == 5)
         behz_floor_kernel<5><<<grid_for(fl), BT, lds, st>>>(Tq, Tb, Z, B, qp, logn, fl);

@@ -86,7 +86,8 @@ bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &args, uint32_t cou
 constexpr uint32_t MAX_GROUPS = 16;
 
 // A transform group: limbs of data (grid lg, primes[i] for limb (i, j)); src != nullptr:
-// the first pass reads the limbs from src (grid slg, same shape) instead.
+// the first pass reads the limbs from src (grid slg, same shape) instead, so the
+// transform lands in data and src is left as it was.
 struct NttGroup {
     uint64_t *data;
     const uint64_t *src;

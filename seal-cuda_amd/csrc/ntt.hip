@@ -1296,14 +1296,26 @@ void ntt_forward_groups(const NttGroup *gs, uint32_t n, int logn, NttOut out, hi
     if (n > MAX_GROUPS) fail(ERR_INVALID_ARGUMENT, "too many groups for one launch");
     if (n == 0) return;
     if (logn <= 12) {
+        // one-tile transforms: a group with a source is copied first, then transformed
         for (uint32_t g = 0; g < n; g++) {
-            if (gs[g].src) fail(ERR_INVALID_ARGUMENT, "a forward transform runs in place");
+            if (gs[g].src) {
+                const LimbGrid &d = gs[g].lg, &s = gs[g].slg;
+                if (d.A == 1 && s.A == 1 && d.B == d.I && s.B == s.I) {  // both dense: one copy
+                    SHC_HIP(hipMemcpyAsync(gs[g].data, gs[g].src, ((size_t)d.I * d.J * 8) << logn,
+                                           hipMemcpyDeviceToDevice, st));
+                    ntt_forward_grid(gs[g].data, gs[g].lg, gs[g].primes, logn, out, st);
+                    continue;
+                }
+                for (uint32_t j = 0; j < d.J; j++)
+                    for (uint32_t i = 0; i < d.I; i++)
+                        SHC_HIP(hipMemcpyAsync(gs[g].data + ((size_t)(i * d.A + j * d.B) << logn),
+                                               gs[g].src + ((size_t)(i * s.A + j * s.B) << logn), (size_t)8 << logn,
+                                               hipMemcpyDeviceToDevice, st));
+            }
             ntt_forward_grid(gs[g].data, gs[g].lg, gs[g].primes, logn, out, st);
         }
         return;
     }
-    for (uint32_t g = 0; g < n; g++)
-        if (gs[g].src) fail(ERR_INVALID_ARGUMENT, "a forward transform runs in place");
     switch (logn) {
 #define SHC_NTT_CASE(L)                                                          \
     case L:                                                                      \

@@ -74,18 +74,22 @@ class MatvecType:
 
 
 class MulRelinType:
-    def __init__(self, n, moduli, seed):
+    """multiply + relinearize (+ rescale / modulus switch) requests; BFV inputs are
+    coefficient form (BEHZ multiply), CKKS NTT form."""
+
+    def __init__(self, n, moduli, seed, scheme=CKKS, rescale=True):
         sh = hip()
         rng = np.random.default_rng(seed)
         K = len(moduli)
         self.L, self.n, self.moduli = K - 1, n, moduli
         self.logn = n.bit_length() - 1
-        self.ctx = sh.Context(CKKS, n, moduli)
+        self.scheme, self.rescale = scheme, rescale
+        self.ctx = sh.Context(scheme, n, moduli, 65537 if scheme == BFV else 0)
         self.key_h = ck.random_key(rng, moduli, self.logn)
         self.key = to_dev(self.key_h)
 
     def register(self, server):
-        return server.add_mulrelin(self.ctx, self.L, self.key.data_ptr(), rescale=True)
+        return server.add_mulrelin(self.ctx, self.L, self.key.data_ptr(), rescale=self.rescale)
 
     def inputs(self, rng, count):
         return [ck.uniform_residues(rng, (2, 2, self.L, self.n), self.moduli[:self.L]) for _ in range(count)]
@@ -94,11 +98,15 @@ class MulRelinType:
         ctx, L, n = self.ctx, self.L, self.n
         a, b = to_dev(x[0][None]), to_dev(x[1][None])
         prod = empty_dev((1, 3, L, n))
-        ctx.ckks_multiply(a.data_ptr(), b.data_ptr(), prod.data_ptr(), 1, L, stream())
+        mw = ctx.multiply_workspace_size(1, L)
+        mb = ws(mw)
+        ctx.multiply(a.data_ptr(), b.data_ptr(), prod.data_ptr(), 1, L, mb.data_ptr(), mw, stream())
         kw = ctx.keyswitch_workspace_size(1, L)
         kb = ws(kw)
         ctx.relinearize(prod.data_ptr(), self.key.data_ptr(), 1, L, kb.data_ptr(), kw, stream())
         relin = prod[:, :2].contiguous()
+        if not self.rescale:
+            return from_dev(relin)[0]
         out = empty_dev((1, 2, L - 1, n))
         rw = ctx.rescale_workspace_size(1, 2, L)
         rb = ws(rw)
@@ -106,9 +114,12 @@ class MulRelinType:
         return from_dev(out)[0]
 
     def oracle(self, x):
-        o3 = ck.ckks_multiply(x[0], x[1], self.moduli[:self.L], self.logn)
-        o2 = ck.relinearize(CKKS, o3, self.key_h, self.moduli, self.L, self.logn)
-        return ck.rescale(CKKS, o2, self.moduli, self.logn)
+        if self.scheme == BFV:
+            o3 = ck.bfv_multiply(x[0], x[1], self.moduli[:self.L], 65537, self.logn)
+        else:
+            o3 = ck.ckks_multiply(x[0], x[1], self.moduli[:self.L], self.logn)
+        o2 = ck.relinearize(self.scheme, o3, self.key_h, self.moduli, self.L, self.logn)
+        return ck.rescale(self.scheme, o2, self.moduli, self.logn) if self.rescale else o2
 
 
 def serve(server, reqs, out_words):
@@ -214,4 +225,23 @@ def test_server_pinned_buffers_in_place():
     for o, st in zip(pouts, staged):
         assert np.array_equal(o, st)
     assert np.array_equal(pouts[0], mr.oracle(xs[0]).reshape(-1))
+    server.close()
+
+
+@pytest.mark.parametrize("scheme,rescale", [(BFV, True), (BFV, False), (CKKS, False)])
+def test_server_mulrelin_types(scheme, rescale):
+    """BFV (BEHZ multiply, modulus switch) and CKKS without rescale through the server:
+    batches equal the direct calls and one request the oracle."""
+    sh = hip()
+    bits = [40, 40, 50] if scheme == BFV else [60, 40, 40, 60]
+    mr = MulRelinType(4096, ck.coeff_modulus_create(4096, bits), seed=20 + scheme, scheme=scheme, rescale=rescale)
+    server = sh.Server(device_bytes=1 << 30, staging_bytes=64 << 20, lanes=2, max_batch=4)
+    t = mr.register(server)
+    rng = np.random.default_rng(21)
+    xs = mr.inputs(rng, 5)
+    outs, infos = serve(server, [(t, x) for x in xs], {t: server.request_size(t)[1]})
+    assert max(inf.batch_size for inf in infos) > 1
+    for x, o in zip(xs, outs):
+        assert np.array_equal(o, mr.direct(x).reshape(-1))
+    assert np.array_equal(outs[2], mr.oracle(xs[2]).reshape(-1))
     server.close()
