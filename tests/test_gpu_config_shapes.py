@@ -106,6 +106,49 @@ def test_rotate_sum_n65536_15_rotations():
         np.testing.assert_array_equal(got[c], acc, err_msg=f"ciphertext {c}: sum vs listed rotations")
 
 
+def test_rotate_many_n65536_zero_digits():
+    """Config 4's primes with zero digits in c1 (ten coefficient-form digit words of
+    ciphertext 0 set to zero, in several limbs): the device-listed fix of the hoisted
+    products keeps every rotation equal to the oracle's separate rotation."""
+    logn, n = 16, 1 << 16
+    moduli = ck.coeff_modulus_create(n, [60] + [50] * 23 + [60])
+    K = len(moduli)
+    L = K - 1
+    ctx = hip().Context(CKKS, n, moduli)
+    steps = [1, 4, 64, 1024, -3]
+    elts = [ctx.galois_elt_from_step(s) for s in steps]
+    count = 2
+    cts = from_dev(device_uniform(ctx, (count, 2, L, n), L, 7)).copy()
+    c1 = ck.ntt_rns(cts[0, 1], moduli[:L], logn, inverse=True)
+    for k, (limb, pos) in enumerate([(0, 5), (0, 40000), (3, 17), (11, 65535), (23, 1), (7, 32768), (7, 32769),
+                                     (19, 123), (2, 4096), (15, 60001)]):
+        c1[limb, pos] = 0
+    cts[0, 1] = ck.ntt_rns(c1, moduli[:L], logn)
+    din = to_dev(cts)
+    dkeys = [device_key(ctx, K, 40 + r) for r in range(len(steps))]
+    hb, hw = ctx.hoisted_key_size(L), ctx.hoisted_key_workspace_size(L)
+    hws = ws(hw)
+    hkeys = []
+    for e, dk in zip(elts, dkeys):
+        h = empty_dev(((hb + 7) // 8,))
+        ctx.hoisted_key_prepare(dk.data_ptr(), e, L, h.data_ptr(), hws.data_ptr(), hw, stream())
+        hkeys.append(h)
+    del hws
+    many = empty_dev((len(elts), count, 2, L, n))
+    rw = ctx.rotate_sum_workspace_size(count, L)
+    rws = ws(rw)
+    ctx.rotate_many(din.data_ptr(), count, L, elts, [h.data_ptr() for h in hkeys], [k.data_ptr() for k in dkeys],
+                    many.data_ptr(), rws.data_ptr(), rw, stream())
+    got = from_dev(many)
+    keys = [from_dev(k) for k in dkeys]
+    del hkeys, rws, many, dkeys
+    torch.cuda.empty_cache()
+    rots = ck.parallel_map(lambda r: ck.apply_galois_keyswitch(CKKS, cts[0], elts[r], keys[r], moduli, L, logn),
+                           list(range(len(elts))))
+    for r, rot in enumerate(rots):
+        np.testing.assert_array_equal(got[r, 0], rot, err_msg=f"rotation by {steps[r]} with zero digits")
+
+
 # ---------------------------------------------------------------- batched relinearize (N=2^16, L=24)
 
 @pytest.mark.parametrize("scheme", [CKKS, BFV])

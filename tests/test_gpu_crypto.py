@@ -280,3 +280,44 @@ def test_keyed_context_pipeline():
     ctx.set_sampler_key(None)   # back to the splitmix64 stream
     ctx.keygen_secret(5, sk.data_ptr(), stream())
     np.testing.assert_array_equal(from_dev(sk), from_dev(sk_plain))
+
+
+def test_key_generation_in_a_captured_graph():
+    """Relinearization and Galois key generation launch kernels only (no allocation, no
+    wait for the host), so they can be captured: replays of the captured graph give the
+    eager keys, with eager key generation between the replays.  Keyed (ChaCha20) sampler."""
+    import torch
+    ctx, moduli = bfv_ctx()
+    ctx.set_sampler_key(bytes(range(32)))
+    n, K = ctx.n, len(moduli)
+    sk = empty_dev((K, n))
+    ctx.keygen_secret(5, sk.data_ptr(), stream())
+    kw = ctx.keygen_workspace_size()
+    kb = ws(kw)
+    rk, gk = empty_dev((K - 1, 2, K, n)), empty_dev((K - 1, 2, K, n))
+    elt = ctx.galois_elt_from_step(3)
+
+    def gen(st, rk_, gk_):
+        ctx.keygen_relin(sk.data_ptr(), 11, rk_.data_ptr(), kb.data_ptr(), kw, st)
+        ctx.keygen_galois(sk.data_ptr(), elt, 12, gk_.data_ptr(), kb.data_ptr(), kw, st)
+
+    gen(stream(), rk, gk)
+    want_r, want_g = from_dev(rk).copy(), from_dev(gk).copy()
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(graph, stream=cap):
+        gen(torch.cuda.current_stream().cuda_stream, rk, gk)
+    torch.cuda.current_stream().wait_stream(cap)
+    torch.cuda.synchronize()
+    rk2, gk2 = empty_dev((K - 1, 2, K, n)), empty_dev((K - 1, 2, K, n))
+    for _ in range(3):
+        rk.zero_()
+        gk.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(from_dev(rk), want_r)
+        np.testing.assert_array_equal(from_dev(gk), want_g)
+        gen(stream(), rk2, gk2)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(from_dev(rk2), want_r)
