@@ -643,6 +643,49 @@ inline T get(std::istream &is, const char *what = "stream ended early") {
     return v;
 }
 
+// The fields that open a saved ciphertext (src/ciphertext_cuda.cu:8-17): parms_id (32 B),
+// is_ntt_form (1 B), size, N, coefficient modulus count (8 B each), scale (double),
+// correction factor, seed, terms flag (1 B); the word count and the words follow.
+struct CtHeader {
+    ParmsID id{};
+    bool ntt = false;
+    uint64_t size = 0, n = 0, L = 0;
+    double scale = 1.0;
+    uint64_t correction = 1, seed = 0;
+    bool terms = false;
+};
+inline void put_header(std::ostream &os, const CtHeader &h) {
+    put(os, h.id);
+    put(os, h.ntt);
+    put(os, h.size);
+    put(os, h.n);
+    put(os, h.L);
+    put(os, h.scale);
+    put(os, h.correction);
+    put(os, h.seed);
+    put(os, h.terms);
+}
+inline CtHeader get_header(std::istream &is) {
+    const char *what = "stream ended before the ciphertext did";
+    CtHeader h;
+    h.id = get<ParmsID>(is, what);
+    h.ntt = get<bool>(is, what);
+    h.size = get<uint64_t>(is, what);
+    h.n = get<uint64_t>(is, what);
+    h.L = get<uint64_t>(is, what);
+    h.scale = get<double>(is, what);
+    h.correction = get<uint64_t>(is, what);
+    h.seed = get<uint64_t>(is, what);
+    h.terms = get<bool>(is, what);
+    return h;
+}
+// `words` words into h (the count was read and checked by the caller).
+inline void get_words(std::istream &is, std::vector<uint64_t> &h, size_t words) {
+    h.resize(words);
+    is.read(reinterpret_cast<char *>(h.data()), (std::streamsize)(words * 8));
+    if (!is) throw std::invalid_argument("stream ended before the ciphertext did");
+}
+
 // Plaintext::operator=(const std::string &) (src/plaintext.cpp:91-200): "7FFx^3 + 1x^1
 // + 3" -- hexadecimal coefficients, decimal exponents strictly decreasing, "x^1" for
 // the linear term, a bare constant term, terms joined by exactly " + ".
@@ -993,12 +1036,24 @@ public:
     }
     void load(std::istream &is, const SEALContext &ctx) {
         loadHeader(is, ctx, false);
-        const uint64_t words = detail::get<uint64_t>(is, "stream ended before the ciphertext did");
-        if (words != size_ * polyWords()) throw std::invalid_argument("ciphertext word count does not match its shape");
-        std::vector<uint64_t> h(words);
-        is.read(reinterpret_cast<char *>(h.data()), (std::streamsize)(words * 8));
-        if (!is) throw std::invalid_argument("stream ended before the ciphertext did");
-        data_.fromHost(h.data(), h.size());
+        loadWords(is);
+    }
+    // CiphertextCuda::load(stream) (src/ciphertext_cuda.cu:74-96): the shape comes from
+    // the header alone, with no context to check it against (keys are stored this way).
+    void load(std::istream &is) {
+        const detail::CtHeader h = detail::get_header(is);
+        if (h.terms) throw std::invalid_argument("Trying to load a termed ciphertext, but indices is not specified");
+        if (h.seed != 0) throw std::invalid_argument("seed is not zero.");
+        if (h.n == 0 || (h.n & (h.n - 1)) || h.n > (1u << 17) || h.L == 0 || h.L > 64 || h.size > 16)
+            throw std::invalid_argument("ciphertext header is not valid");
+        parms_id_ = h.id;
+        ntt_form_ = h.ntt;
+        n_ = (size_t)h.n;
+        L_ = (size_t)h.L;
+        scale_ = h.scale;
+        correction_ = h.correction;
+        resize((size_t)h.size);
+        loadWords(is);
     }
     // saveTerms / loadTerms (src/ciphertext_cuda.cu:37-136): only the listed coefficients
     // of c0 (coefficient form), then every other polynomial in full.
@@ -1034,36 +1089,36 @@ private:
         n_ = ctx.polyModulusDegree();
     }
     void saveHeader(std::ostream &os, bool terms) const {
-        detail::put(os, parms_id_);
-        detail::put(os, ntt_form_);
-        detail::put(os, (uint64_t)size_);
-        detail::put(os, (uint64_t)n_);
-        detail::put(os, (uint64_t)L_);
-        detail::put(os, scale_);
-        detail::put(os, (uint64_t)correction_);
-        detail::put(os, (uint64_t)0);  // seed
-        detail::put(os, terms);
+        detail::CtHeader h;
+        h.id = parms_id_;
+        h.ntt = ntt_form_;
+        h.size = size_;
+        h.n = n_;
+        h.L = L_;
+        h.scale = scale_;
+        h.correction = correction_;
+        h.terms = terms;
+        detail::put_header(os, h);
     }
     void loadHeader(std::istream &is, const SEALContext &ctx, bool terms) {
-        const char *what = "stream ended before the ciphertext did";
-        const ParmsID id = detail::get<ParmsID>(is, what);
-        const bool ntt = detail::get<bool>(is, what);
-        const uint64_t size = detail::get<uint64_t>(is, what), n = detail::get<uint64_t>(is, what),
-                       L = detail::get<uint64_t>(is, what);
-        const double scale = detail::get<double>(is, what);
-        const uint64_t correction = detail::get<uint64_t>(is, what);
-        const uint64_t seed = detail::get<uint64_t>(is, what);
-        const bool is_terms = detail::get<bool>(is, what);
-        if (seed != 0) throw std::invalid_argument("seeded ciphertexts are not supported");
-        if (is_terms != terms)
+        const detail::CtHeader h = detail::get_header(is);
+        if (h.seed != 0) throw std::invalid_argument("seeded ciphertexts are not supported");
+        if (h.terms != terms)
             throw std::invalid_argument(terms ? "Trying to load a normal ciphertext, but term indices is specified"
                                               : "Trying to load a termed ciphertext, but indices is not specified");
-        if (ctx.levelOf(id) == 0 || ctx.levelOf(id) != L || n != ctx.polyModulusDegree())
+        if (ctx.levelOf(h.id) == 0 || ctx.levelOf(h.id) != h.L || h.n != ctx.polyModulusDegree())
             throw std::invalid_argument("ciphertext is not valid for encryption parameters");
-        resize(ctx, id, size);
-        ntt_form_ = ntt;
-        scale_ = scale;
-        correction_ = correction;
+        resize(ctx, h.id, h.size);
+        ntt_form_ = h.ntt;
+        scale_ = h.scale;
+        correction_ = h.correction;
+    }
+    void loadWords(std::istream &is) {
+        const uint64_t words = detail::get<uint64_t>(is, "stream ended before the ciphertext did");
+        if (words != size_ * polyWords()) throw std::invalid_argument("ciphertext word count does not match its shape");
+        std::vector<uint64_t> h;
+        detail::get_words(is, h, words);
+        data_.fromHost(h.data(), h.size());
     }
 
     DeviceArray<uint64_t> data_;
@@ -1081,6 +1136,9 @@ public:
     const Plaintext &data() const { return data_; }
     ParmsID &parmsID() { return data_.parmsID(); }
     const ParmsID &parmsID() const { return data_.parmsID(); }
+    // SecretKeyCuda::save / load (src/secretkey_cuda.cuh:292-297): the plaintext's layout.
+    void save(std::ostream &os) const { data_.save(os); }
+    void load(std::istream &is) { data_.load(is); }
 
 private:
     Plaintext data_;
@@ -1093,6 +1151,9 @@ public:
     const Ciphertext &data() const { return data_; }
     ParmsID &parmsID() { return data_.parmsID(); }
     const ParmsID &parmsID() const { return data_.parmsID(); }
+    // PublicKeyCuda::save / load (src/publickey_cuda.cuh:252-257): the ciphertext's layout.
+    void save(std::ostream &os) const { data_.save(os); }
+    void load(std::istream &is) { data_.load(is); }
 
 private:
     Ciphertext data_;
@@ -1119,10 +1180,88 @@ public:
     }
     ParmsID &parmsID() { return parms_id_; }
     const ParmsID &parmsID() const { return parms_id_; }
+    // The shape of every key: degree N and key-level modulus count K (set by KeyGenerator
+    // and by load; save needs it).
+    void setShape(size_t n, size_t key_moduli) {
+        n_ = n;
+        K_ = key_moduli;
+    }
+    size_t polyModulusDegree() const { return n_; }
+    size_t keyModulusSize() const { return K_; }
+
+    // KSwitchKeysCuda::save / load (src/kswitchkeys_cuda.cuh:330-355): parms_id (32 B), the
+    // number of key slots (8 B), then per slot the number of its public keys (8 B; 0 for
+    // an absent key) and each public key in the ciphertext layout (size 2, K limbs, NTT
+    // form, scale 1, correction 1; its parms_id is the key level's).  Public key j of a
+    // slot is part j of this build's [K-1][2][K][N] key.
+    void save(std::ostream &os) const {
+        detail::put(os, parms_id_);
+        detail::put(os, (uint64_t)keys_.size());
+        const size_t pk_words = 2 * K_ * n_;
+        std::vector<uint64_t> h;
+        for (const DeviceArray<uint64_t> &k : keys_) {
+            if (!k.size()) {
+                detail::put(os, (uint64_t)0);
+                continue;
+            }
+            if (K_ < 2 || k.size() != (K_ - 1) * pk_words)
+                throw std::logic_error("key shape is not set or does not match the key");
+            h = k.toHost();
+            detail::put(os, (uint64_t)(K_ - 1));
+            detail::CtHeader hd;
+            hd.id = parms_id_;
+            hd.ntt = true;
+            hd.size = 2;
+            hd.n = n_;
+            hd.L = K_;
+            for (size_t j = 0; j + 1 < K_; j++) {
+                detail::put_header(os, hd);
+                detail::put(os, (uint64_t)pk_words);
+                os.write(reinterpret_cast<const char *>(h.data() + j * pk_words), (std::streamsize)(pk_words * 8));
+            }
+        }
+    }
+    void load(std::istream &is) {
+        const char *what = "stream ended before the keys did";
+        const ParmsID id = detail::get<ParmsID>(is, what);
+        const uint64_t slots = detail::get<uint64_t>(is, what);
+        if (slots > (1u << 18)) throw std::invalid_argument("key slot count is not valid");
+        std::vector<DeviceArray<uint64_t>> keys(slots);
+        size_t n = 0, K = 0;
+        std::vector<uint64_t> all, part;
+        for (uint64_t s = 0; s < slots; s++) {
+            const uint64_t m = detail::get<uint64_t>(is, what);
+            if (!m) continue;
+            all.clear();
+            for (uint64_t j = 0; j < m; j++) {
+                const detail::CtHeader hd = detail::get_header(is);
+                if (hd.seed != 0 || hd.terms || hd.size != 2 || !hd.ntt || hd.id != id)
+                    throw std::invalid_argument("key part is not a size-2 NTT-form ciphertext of the key level");
+                if (!n) {
+                    n = (size_t)hd.n;
+                    K = (size_t)hd.L;
+                    if (!n || (n & (n - 1)) || n > (1u << 17) || K < 2 || K > 64)
+                        throw std::invalid_argument("key part header is not valid");
+                }
+                if (hd.n != n || hd.L != K || m != K - 1)
+                    throw std::invalid_argument("key parts differ in shape or count");
+                const uint64_t words = detail::get<uint64_t>(is, what);
+                if (words != 2 * K * n) throw std::invalid_argument("key part word count does not match its shape");
+                detail::get_words(is, part, words);
+                all.insert(all.end(), part.begin(), part.end());
+            }
+            keys[s].fromHost(all.data(), all.size());
+        }
+        keys_ = std::move(keys);
+        parms_id_ = id;
+        n_ = n;
+        K_ = K;
+    }
 
 protected:
     std::vector<DeviceArray<uint64_t>> keys_;
     ParmsID parms_id_ = parmsIDZero;
+    size_t n_ = 0, K_ = 0;
 };
 
 // RelinKeysCuda (src/relinkeys_cuda.cuh:11-60): key for s^p at index p - 2.
@@ -1203,6 +1342,7 @@ public:
         check(shc_keygen_relin(ctx_.handle(), sk_.data().data(), detail::os_random64(), rk.data()[0].get(),
                                ctx_.workspace(ws), ws, nullptr));
         rk.parmsID() = ctx_.keyParmsID();
+        rk.setShape(ctx_.polyModulusDegree(), ctx_.keyCount());
     }
     RelinKeys createRelinKeys() const {
         RelinKeys rk;
@@ -1221,6 +1361,7 @@ public:
                                     ctx_.workspace(ws), ws, nullptr));
         }
         gk.parmsID() = ctx_.keyParmsID();
+        gk.setShape(ctx_.polyModulusDegree(), ctx_.keyCount());
     }
     GaloisKeys createGaloisKeys(const std::vector<uint32_t> &elts) const {
         GaloisKeys gk;
@@ -1602,7 +1743,7 @@ public:
         const uint32_t L = level(a);
         size_t ws = 0;
         check(shc_keyswitch_workspace_size(ctx_.handle(), 1, L, &ws));
-        check(shc_relinearize(ctx_.handle(), a.data(), rk.key(2).get(), 1, L, ctx_.workspace(ws), ws, nullptr));
+        check(shc_relinearize(ctx_.handle(), a.data(), keyWords(rk.key(2)), 1, L, ctx_.workspace(ws), ws, nullptr));
         a.shrink(2);
     }
     void relinearize(const Ciphertext &a, const RelinKeys &rk, Ciphertext &out) const {
@@ -1745,7 +1886,7 @@ public:
         const uint32_t L = level(a);
         size_t ws = 0;
         check(shc_galois_workspace_size(ctx_.handle(), 1, L, &ws));
-        check(shc_apply_galois_keyswitch(ctx_.handle(), a.data(), gk.key(elt).get(), elt, 1, L, ctx_.workspace(ws), ws,
+        check(shc_apply_galois_keyswitch(ctx_.handle(), a.data(), keyWords(gk.key(elt)), elt, 1, L, ctx_.workspace(ws), ws,
                                          nullptr));
     }
     void applyGalois(const Ciphertext &a, uint32_t elt, const GaloisKeys &gk, Ciphertext &out) const {
@@ -1787,6 +1928,13 @@ public:
 
 private:
     uint32_t level(const Ciphertext &a) const { return ctx_.levelOrThrow(a.parmsID(), "encrypted"); }
+    // A key's words, after checking that it has this context's [K-1][2][K][N] shape (a
+    // loaded key carries its own shape).
+    const uint64_t *keyWords(const DeviceArray<uint64_t> &k) const {
+        const size_t K = ctx_.keyCount(), n = ctx_.polyModulusDegree();
+        if (k.size() != (K - 1) * 2 * K * n) throw std::invalid_argument("keyswitching key has the wrong shape");
+        return k.get();
+    }
     uint32_t targetLevel(const Ciphertext &a, ParmsID id) const {
         const uint32_t target = ctx_.levelOrThrow(id, "parms_id");
         if (level(a) < target) throw std::invalid_argument("cannot switch to higher level modulus");

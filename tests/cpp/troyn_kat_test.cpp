@@ -8,7 +8,8 @@
 // (src/plaintext.h:99-130, 470-498), PlaintextCuda::save / load
 // (src/plaintext_cuda.cu:6-27), parms_id hash blocks (src/encryptionparams.h:36),
 // cpu() / conversion constructors (src/ciphertext_cuda.cuh:20-49,
-// src/plaintext_cuda.cuh:27-42), KSwitchKeys (src/kswitchkeys_cuda.cuh:31-170), fresh
+// src/plaintext_cuda.cuh:27-42), KSwitchKeys (src/kswitchkeys_cuda.cuh:31-170), key
+// save / load in the reference's layouts (src/kswitchkeys_cuda.cuh:330-355), fresh
 // key material per KeyGenerator, and a memory pool that serves steady-state operations
 // without new device allocations.  Prints "ok <name>" per check, "troyn kat ok" at the
 // end; exits 1 on the first failure.
@@ -237,11 +238,118 @@ static void containers_and_keys() {
     EXPECT(MemoryPool::release() == cached && MemoryPool::cachedBytes() == 0, "pool release");
 }
 
+// SecretKeyCuda / PublicKeyCuda / KSwitchKeysCuda save / load (src/secretkey_cuda.cuh:292-297,
+// src/publickey_cuda.cuh:252-257, src/kswitchkeys_cuda.cuh:330-355): the reference's byte
+// layouts, and loaded keys that decrypt, encrypt, relinearize and rotate as the originals.
+static void key_files() {
+    const size_t n = 8192;
+    EncryptionParameters parms(SchemeType::ckks);
+    parms.setPolyModulusDegree(n);
+    parms.setCoeffModulus(CoeffModulus::Create(n, {60, 40, 40, 60}));
+    SEALContext context(parms);
+    const size_t K = 4;
+    const ParmsID key = context.keyParmsID();
+    KeyGenerator keygen(context);
+    RelinKeys rlk = keygen.createRelinKeys();
+    GaloisKeys gk = keygen.createGaloisKeys(std::vector<int>{1});  // one key among N slots
+
+    // relin keys: parms_id, slot count, then per slot the public-key count and each
+    // public key as a size-2 ciphertext of the key level
+    std::stringstream ks;
+    rlk.save(ks);
+    const std::string kb = ks.str();
+    const size_t pk_bytes = 82 + 8 + 2 * K * n * 8;  // header, word count, words
+    uint64_t slots = 0, m = 0, size = 0, nn = 0, L = 0, words = 0;
+    std::memcpy(&slots, kb.data() + 32, 8);
+    std::memcpy(&m, kb.data() + 40, 8);
+    const char *pk0 = kb.data() + 48;
+    ParmsID pid{};
+    std::memcpy(pid.data(), pk0, 32);
+    std::memcpy(&size, pk0 + 33, 8);
+    std::memcpy(&nn, pk0 + 41, 8);
+    std::memcpy(&L, pk0 + 49, 8);
+    std::memcpy(&words, pk0 + 82, 8);
+    double scale = 0;
+    std::memcpy(&scale, pk0 + 57, 8);
+    EXPECT(kb.size() == 48 + (K - 1) * pk_bytes && slots == 1 && m == K - 1 && pid == key && pk0[32] == 1 &&
+               size == 2 && nn == n && L == K && scale == 1.0 && words == 2 * K * n,
+           "relin keys save layout");
+    const std::vector<uint64_t> rw = rlk.data()[0].toHost();
+    EXPECT(std::memcmp(pk0 + 90, rw.data(), 2 * K * n * 8) == 0 &&
+               std::memcmp(kb.data() + 48 + pk_bytes + 90, rw.data() + 2 * K * n, 2 * K * n * 8) == 0,
+           "relin key words in public-key order");
+    RelinKeys rlk2;
+    rlk2.load(ks);
+    EXPECT(rlk2.parmsID() == key && rlk2.hasKey(2) && rlk2.data()[0].toHost() == rw &&
+               rlk2.polyModulusDegree() == n && rlk2.keyModulusSize() == K,
+           "relin keys load round trip");
+
+    // galois keys: absent slots are saved as empty
+    std::stringstream gs;
+    gk.save(gs);
+    GaloisKeys gk2;
+    gk2.load(gs);
+    const uint32_t e1 = 3;
+    EXPECT(gk2.size() == 1 && gk2.hasKey(e1) && !gk2.hasKey(5) && gk2.data().size() == gk.data().size() &&
+               gk2.key(e1).toHost() == gk.key(e1).toHost(),
+           "galois keys load round trip");
+
+    // secret and public keys: plaintext and ciphertext layouts
+    std::stringstream ss, ps;
+    keygen.secretKey().save(ss);
+    const PublicKey pk = keygen.createPublicKey();
+    pk.save(ps);
+    SecretKey sk2;
+    sk2.load(ss);
+    PublicKey pk2;
+    pk2.load(ps);
+    EXPECT(sk2.parmsID() == key && sk2.data() == keygen.secretKey().data(), "secret key load round trip");
+    EXPECT(pk2.parmsID() == key && pk2.data().size() == 2 && pk2.data().isNttForm() &&
+               pk2.data().toHost().data() == pk.data().toHost().data(),
+           "public key load round trip");
+
+    // the loaded keys at work: encrypt with pk2, square, relinearize with rlk2, rotate
+    // with gk2, decrypt with sk2
+    CKKSEncoder encoder(context);
+    Encryptor encryptor(context, pk2);
+    Decryptor decryptor(context, sk2);
+    Evaluator evaluator(context);
+    std::vector<std::complex<double>> v(encoder.slotCount());
+    for (size_t i = 0; i < v.size(); i++) v[i] = {std::sin(0.1 * i), std::cos(0.3 * i)};
+    Plaintext p;
+    encoder.encode(v, std::pow(2.0, 40), p);
+    Ciphertext ct = encryptor.encrypt(p);
+    evaluator.squareInplace(ct);
+    evaluator.relinearizeInplace(ct, rlk2);
+    evaluator.rescaleToNextInplace(ct);
+    evaluator.rotateVectorInplace(ct, 1, gk2);
+    Plaintext out;
+    decryptor.decrypt(ct, out);
+    std::vector<std::complex<double>> got;
+    encoder.decode(out, got);
+    double err = 0;
+    for (size_t i = 0; i < v.size(); i++) {
+        const std::complex<double> w = v[(i + 1) % v.size()];
+        err = std::max(err, std::abs(got[i] - w * w));
+    }
+    EXPECT(err < 1e-4, "loaded keys: square, relinearize, rotate, decrypt");
+
+    // a truncated key stream and a key of another shape are rejected
+    std::stringstream cut(kb.substr(0, kb.size() - 5));
+    EXPECT(throws<std::invalid_argument>([&] { RelinKeys r; r.load(cut); }), "truncated keys throw");
+    std::string bad = kb;
+    uint64_t wrong_m = K;
+    std::memcpy(&bad[40], &wrong_m, 8);
+    std::stringstream bs(bad);
+    EXPECT(throws<std::invalid_argument>([&] { RelinKeys r; r.load(bs); }), "public-key count must be K - 1");
+}
+
 int main() {
     KernelProvider::initialize();
     bfv_relinearize();
     plaintext_strings_and_files();
     containers_and_keys();
+    key_files();
     std::printf("troyn kat ok\n");
     return 0;
 }
