@@ -1767,6 +1767,11 @@ public:
         const uint32_t target = targetLevel(a, id);
         while (level(a) > target) rescaleToNextInplace(a);
     }
+    void rescaleTo(const Ciphertext &a, ParmsID id, Ciphertext &out) const {
+        Ciphertext r = a;
+        rescaleToInplace(r, id);
+        out = std::move(r);
+    }
     void modSwitchToNext(const Ciphertext &a, Ciphertext &out) const {
         if (ctx_.scheme() == SchemeType::bfv) {
             const double scale = a.scale();
@@ -1811,6 +1816,65 @@ public:
         if (ctx_.levelOrThrow(p.parmsID(), "plain") < target)
             throw std::invalid_argument("cannot switch to higher level modulus");
         while (ctx_.levelOf(p.parmsID()) > target) modSwitchToNextInplace(p);
+    }
+    void modSwitchTo(const Ciphertext &a, ParmsID id, Ciphertext &out) const {
+        Ciphertext r = a;
+        modSwitchToInplace(r, id);
+        out = std::move(r);
+    }
+    void modSwitchTo(const Plaintext &p, ParmsID id, Plaintext &out) const {
+        Plaintext r = p;
+        modSwitchToInplace(r, id);
+        out = std::move(r);
+    }
+
+    // EvaluatorCuda::multiplyMany (src/evaluator_cuda.cu:1434-1510), BFV: the same product
+    // tree -- neighbours (0,1), (2,3), ... multiplied and relinearized, an odd last one
+    // carried, then the front two of the list multiplied onto its back until one is left
+    // -- so the result has the same words as the reference's.
+    void multiplyMany(const std::vector<Ciphertext> &cts, const RelinKeys &rk, Ciphertext &out) const {
+        if (cts.empty()) throw std::invalid_argument("encrypteds vector must not be empty");
+        for (const Ciphertext &c : cts)
+            if (&c == &out) throw std::invalid_argument("encrypteds must be different from destination");
+        ctx_.levelOrThrow(cts[0].parmsID(), "encrypteds");
+        if (ctx_.scheme() != SchemeType::bfv) throw std::logic_error("unsupported scheme");
+        if (cts.size() == 1) {
+            out = cts[0];
+            return;
+        }
+        std::vector<Ciphertext> prod;
+        prod.reserve(2 * cts.size());
+        for (size_t i = 0; i + 1 < cts.size(); i += 2) {
+            Ciphertext t;
+            multiply(cts[i], cts[i + 1], t);
+            relinearizeInplace(t, rk);
+            prod.push_back(std::move(t));
+        }
+        if (cts.size() & 1) prod.push_back(cts.back());
+        for (size_t i = 0; i + 1 < prod.size(); i += 2) {
+            Ciphertext t;
+            multiply(prod[i], prod[i + 1], t);
+            relinearizeInplace(t, rk);
+            prod.push_back(std::move(t));
+        }
+        out = std::move(prod.back());
+    }
+    // EvaluatorCuda::exponentiateInplace (src/evaluator_cuda.cu:1512-1540): the product of
+    // `exponent` copies through multiplyMany.
+    void exponentiateInplace(Ciphertext &a, uint64_t exponent, const RelinKeys &rk) const {
+        level(a);
+        if (rk.parmsID() != ctx_.keyParmsID()) throw std::invalid_argument("relin_keys is not valid for encryption parameters");
+        if (exponent == 0) throw std::invalid_argument("exponent cannot be 0");
+        if (exponent == 1) return;
+        const std::vector<Ciphertext> copies((size_t)exponent, a);
+        Ciphertext r;
+        multiplyMany(copies, rk, r);
+        a = std::move(r);
+    }
+    void exponentiate(const Ciphertext &a, uint64_t exponent, const RelinKeys &rk, Ciphertext &out) const {
+        Ciphertext r = a;
+        exponentiateInplace(r, exponent, rk);
+        out = std::move(r);
     }
 
     // ---- plaintext operations

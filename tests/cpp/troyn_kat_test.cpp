@@ -344,12 +344,173 @@ static void key_files() {
     EXPECT(throws<std::invalid_argument>([&] { RelinKeys r; r.load(bs); }), "public-key count must be K - 1");
 }
 
+// EvaluatorCudaTest.BFVEncryptMultiplyManyDecrypt and BFVEncryptExponentiateDecrypt
+// (test/evaluator_cuda.cu:4575-4732), statement for statement (ASSERT_EQ / ASSERT_TRUE as
+// EXPECT), and the first two cases of BFVEncryptAddManyDecrypt (:4734-4790) in the same
+// context.
+static void bfv_many_and_exponentiate() {
+    EncryptionParameters parms(SchemeType::bfv);
+    Modulus plain_modulus(1 << 6);
+    parms.setPolyModulusDegree(128);
+    parms.setPlainModulus(plain_modulus);
+    parms.setCoeffModulus(CoeffModulus::Create(128, {40, 40, 40}));
+
+    SEALContext context(parms, false, SecurityLevel::none);
+    KeyGenerator keygen(context);
+    PublicKey pk;
+    keygen.createPublicKey(pk);
+    RelinKeys rlk;
+    keygen.createRelinKeys(rlk);
+
+    Encryptor encryptor(context, pk);
+    Evaluator evaluator(context);
+    Decryptor decryptor(context, keygen.secretKey());
+
+    Ciphertext encrypted1, encrypted2, encrypted3, encrypted4, product;
+    Plaintext plain, plain1, plain2, plain3, plain4;
+    auto same_level = [&](const std::vector<Ciphertext> &v, const Ciphertext &r) {
+        for (const Ciphertext &c : v)
+            if (!(c.parmsID() == r.parmsID())) return false;
+        return r.parmsID() == context.firstParmsID();
+    };
+
+    plain1 = "1x^2 + 1";
+    plain2 = "1x^2 + 1x^1";
+    plain3 = "1x^2 + 1x^1 + 1";
+    encryptor.encrypt(plain1, encrypted1);
+    encryptor.encrypt(plain2, encrypted2);
+    encryptor.encrypt(plain3, encrypted3);
+    std::vector<Ciphertext> encrypteds{encrypted1, encrypted2, encrypted3};
+    evaluator.multiplyMany(encrypteds, rlk, product);
+    decryptor.decrypt(product, plain);
+    EXPECT(encrypteds.size() == 3 && plain.to_string() == "1x^6 + 2x^5 + 3x^4 + 3x^3 + 2x^2 + 1x^1" &&
+               same_level(encrypteds, product),
+           "BFVEncryptMultiplyManyDecrypt: three");
+
+    plain1 = "3Fx^3 + 3F";
+    plain2 = "3Fx^4 + 3F";
+    encryptor.encrypt(plain1, encrypted1);
+    encryptor.encrypt(plain2, encrypted2);
+    encrypteds = {encrypted1, encrypted2};
+    evaluator.multiplyMany(encrypteds, rlk, product);
+    decryptor.decrypt(product, plain);
+    EXPECT(encrypteds.size() == 2 && plain.to_string() == "1x^7 + 1x^4 + 1x^3 + 1" && same_level(encrypteds, product),
+           "BFVEncryptMultiplyManyDecrypt: two");
+
+    plain1 = "1x^1";
+    plain2 = "3Fx^4 + 3Fx^3 + 3Fx^2 + 3Fx^1 + 3F";
+    plain3 = "1x^2 + 1x^1 + 1";
+    encryptor.encrypt(plain1, encrypted1);
+    encryptor.encrypt(plain2, encrypted2);
+    encryptor.encrypt(plain3, encrypted3);
+    encrypteds = {encrypted1, encrypted2, encrypted3};
+    evaluator.multiplyMany(encrypteds, rlk, product);
+    decryptor.decrypt(product, plain);
+    EXPECT(encrypteds.size() == 3 && plain.to_string() == "3Fx^7 + 3Ex^6 + 3Dx^5 + 3Dx^4 + 3Dx^3 + 3Ex^2 + 3Fx^1" &&
+               same_level(encrypteds, product),
+           "BFVEncryptMultiplyManyDecrypt: three, negative coefficients");
+
+    plain1 = "1";
+    plain2 = "3F";
+    plain3 = "1";
+    plain4 = "3F";
+    encryptor.encrypt(plain1, encrypted1);
+    encryptor.encrypt(plain2, encrypted2);
+    encryptor.encrypt(plain3, encrypted3);
+    encryptor.encrypt(plain4, encrypted4);
+    encrypteds = {encrypted1, encrypted2, encrypted3, encrypted4};
+    evaluator.multiplyMany(encrypteds, rlk, product);
+    decryptor.decrypt(product, plain);
+    EXPECT(encrypteds.size() == 4 && plain.to_string() == "1" && same_level(encrypteds, product),
+           "BFVEncryptMultiplyManyDecrypt: four constants");
+
+    plain1 = "1x^16 + 1x^15 + 1x^8 + 1x^7 + 1x^6 + 1x^3 + 1x^2 + 1";
+    plain2 = "0";
+    plain3 = "1x^13 + 1x^12 + 1x^5 + 1x^4 + 1x^3 + 1";
+    plain4 = "1x^15 + 1x^10 + 1x^9 + 1x^8 + 1x^2 + 1x^1 + 1";
+    encryptor.encrypt(plain1, encrypted1);
+    encryptor.encrypt(plain2, encrypted2);
+    encryptor.encrypt(plain3, encrypted3);
+    encryptor.encrypt(plain4, encrypted4);
+    encrypteds = {encrypted1, encrypted2, encrypted3, encrypted4};
+    evaluator.multiplyMany(encrypteds, rlk, product);
+    decryptor.decrypt(product, plain);
+    EXPECT(encrypteds.size() == 4 && plain.to_string() == "0" && same_level(encrypteds, product),
+           "BFVEncryptMultiplyManyDecrypt: a zero factor");
+
+    // the product tree: ((c1 c2) (c3 c4)) with a relinearization after each product,
+    // word for word
+    Ciphertext p12, p34, tree;
+    evaluator.multiply(encrypted1, encrypted2, p12);
+    evaluator.relinearizeInplace(p12, rlk);
+    evaluator.multiply(encrypted3, encrypted4, p34);
+    evaluator.relinearizeInplace(p34, rlk);
+    evaluator.multiply(p12, p34, tree);
+    evaluator.relinearizeInplace(tree, rlk);
+    EXPECT(tree.toHost().data() == product.toHost().data(), "multiplyMany follows the reference's product tree");
+    EXPECT(throws<std::invalid_argument>([&] { evaluator.multiplyMany({}, rlk, product); }), "empty list throws");
+
+    Ciphertext encrypted;
+    plain = "1x^2 + 1";
+    encryptor.encrypt(plain, encrypted);
+    evaluator.exponentiateInplace(encrypted, 1, rlk);
+    decryptor.decrypt(encrypted, plain);
+    EXPECT(plain.to_string() == "1x^2 + 1" && encrypted.parmsID() == context.firstParmsID(),
+           "BFVEncryptExponentiateDecrypt: 1");
+
+    plain = "1x^2 + 1x^1 + 1";
+    encryptor.encrypt(plain, encrypted);
+    evaluator.exponentiateInplace(encrypted, 2, rlk);
+    decryptor.decrypt(encrypted, plain);
+    EXPECT(plain.to_string() == "1x^4 + 2x^3 + 3x^2 + 2x^1 + 1" && encrypted.parmsID() == context.firstParmsID(),
+           "BFVEncryptExponentiateDecrypt: 2");
+
+    plain = "3Fx^2 + 3Fx^1 + 3F";
+    encryptor.encrypt(plain, encrypted);
+    evaluator.exponentiateInplace(encrypted, 3, rlk);
+    decryptor.decrypt(encrypted, plain);
+    EXPECT(plain.to_string() == "3Fx^6 + 3Dx^5 + 3Ax^4 + 39x^3 + 3Ax^2 + 3Dx^1 + 3F" &&
+               encrypted.parmsID() == context.firstParmsID(),
+           "BFVEncryptExponentiateDecrypt: 3");
+
+    plain = "1x^8";
+    encryptor.encrypt(plain, encrypted);
+    evaluator.exponentiateInplace(encrypted, 4, rlk);
+    decryptor.decrypt(encrypted, plain);
+    EXPECT(plain.to_string() == "1x^32" && encrypted.parmsID() == context.firstParmsID(),
+           "BFVEncryptExponentiateDecrypt: 4");
+    EXPECT(throws<std::invalid_argument>([&] { evaluator.exponentiateInplace(encrypted, 0, rlk); }),
+           "exponent 0 throws");
+
+    Ciphertext sum;
+    plain1 = "1x^2 + 1";
+    plain2 = "1x^2 + 1x^1";
+    plain3 = "1x^2 + 1x^1 + 1";
+    encryptor.encrypt(plain1, encrypted1);
+    encryptor.encrypt(plain2, encrypted2);
+    encryptor.encrypt(plain3, encrypted3);
+    encrypteds = {encrypted1, encrypted2, encrypted3};
+    evaluator.addMany(encrypteds, sum);
+    decryptor.decrypt(sum, plain);
+    EXPECT(plain.to_string() == "3x^2 + 2x^1 + 2" && same_level(encrypteds, sum), "BFVEncryptAddManyDecrypt: three");
+
+    plain1 = "3Fx^3 + 3F";
+    plain2 = "3Fx^4 + 3F";
+    encryptor.encrypt(plain1, encrypted1);
+    encryptor.encrypt(plain2, encrypted2);
+    encrypteds = {encrypted1, encrypted2};
+    evaluator.addMany(encrypteds, sum);
+    decryptor.decrypt(sum, plain);
+    EXPECT(plain.to_string() == "3Fx^4 + 3Fx^3 + 3E" && same_level(encrypteds, sum), "BFVEncryptAddManyDecrypt: two");
+}
+
 int main() {
     KernelProvider::initialize();
     bfv_relinearize();
     plaintext_strings_and_files();
     containers_and_keys();
     key_files();
+    bfv_many_and_exponentiate();
     std::printf("troyn kat ok\n");
     return 0;
 }
