@@ -310,6 +310,17 @@ int shc_ckks_decode_many(const shc_context *ctx, const uint64_t *d_plain, uint32
 int shc_ckks_decode_workspace_size(const shc_context *ctx, uint32_t limb_count, size_t *bytes);
 int shc_ckks_decode(const shc_context *ctx, const uint64_t *d_plain, uint32_t limb_count, double scale,
                     double *d_values, void *d_workspace, size_t workspace_bytes, void *stream);
+/* CKKSEncoderCuda::encodePolynomial / decodePolynomial (src/ckks_cuda.cuh:34-40, 84-87;
+ * src/ckks_cuda.cu:455-572, 983-1053): the coefficients themselves, with no FFT.  Encode
+ * takes count <= N doubles at d_values, multiplies them by scale, rounds half away from
+ * zero and writes the [L][N] NTT-form plaintext (workspace: shc_ckks_encode_workspace_size;
+ * waits for the range check like shc_ckks_encode).  Decode writes the N centred
+ * coefficients divided by scale (workspace: shc_ckks_decode_workspace_size). */
+int shc_ckks_encode_polynomial(const shc_context *ctx, const double *d_values, uint32_t count, double scale,
+                               uint32_t limb_count, uint64_t *d_plain, void *d_workspace, size_t workspace_bytes,
+                               void *stream);
+int shc_ckks_decode_polynomial(const shc_context *ctx, const uint64_t *d_plain, uint32_t limb_count, double scale,
+                               double *d_values, void *d_workspace, size_t workspace_bytes, void *stream);
 
 int shc_rescale_workspace_size(const shc_context *ctx, uint32_t count, uint32_t poly_count, uint32_t limb_count,
                                size_t *bytes);

@@ -1471,6 +1471,43 @@ public:
         for (size_t i = 0; i < u.size(); i++)
             out[i] = u[i] >= (t + 1) / 2 ? (int64_t)u[i] - (int64_t)t : (int64_t)u[i];
     }
+    // BatchEncoderCuda::encodePolynomial / decodePolynomial (src/batchencoder_cuda.cu:124-170,
+    // 267-285): the values are the plaintext's coefficients, reduced mod t, with no slot
+    // permutation.  As in the reference, the unsigned form keeps values.size()
+    // coefficients and the signed form pads to N, a negative v becoming t - (|v| mod t).
+    void encodePolynomial(const std::vector<uint64_t> &values, Plaintext &plain) const {
+        if (values.size() > slotCount()) throw std::invalid_argument("values_matrix size is too large");
+        const uint64_t t = ctx_.parms().plainModulus().value();
+        std::vector<uint64_t> v(values.size());
+        for (size_t i = 0; i < v.size(); i++) v[i] = values[i] % t;
+        plain.parmsID() = parmsIDZero;
+        plain.dynArray().fromHost(v.data(), v.size());
+    }
+    void encodePolynomial(const std::vector<int64_t> &values, Plaintext &plain) const {
+        if (values.size() > slotCount()) throw std::invalid_argument("values_matrix size is too large");
+        const uint64_t t = ctx_.parms().plainModulus().value();
+        std::vector<uint64_t> v(slotCount(), 0);
+        for (size_t i = 0; i < values.size(); i++) {
+            const int64_t x = values[i];
+            // |x| without overflow at INT64_MIN
+            v[i] = x < 0 ? t - ((uint64_t)(-(x + 1)) + 1) % t : (uint64_t)x % t;
+        }
+        plain.parmsID() = parmsIDZero;
+        plain.dynArray().fromHost(v.data(), v.size());
+    }
+    // The first N coefficients (zero past a shorter plaintext).
+    void decodePolynomial(const Plaintext &plain, std::vector<uint64_t> &out) const {
+        out = plain.dynArray().toHost();
+        out.resize(slotCount(), 0);
+    }
+    // Centred: a coefficient above t / 2 (rounded down) becomes negative.
+    void decodePolynomial(const Plaintext &plain, std::vector<int64_t> &out) const {
+        std::vector<uint64_t> u;
+        decodePolynomial(plain, u);
+        const uint64_t t = ctx_.parms().plainModulus().value();
+        out.resize(u.size());
+        for (size_t i = 0; i < u.size(); i++) out[i] = u[i] > (t >> 1) ? (int64_t)(u[i] - t) : (int64_t)u[i];
+    }
 
 private:
     SEALContext ctx_;
@@ -1537,6 +1574,36 @@ public:
         out.resize(slotCount());
         for (size_t i = 0; i < out.size(); i++) out[i] = {h[2 * i], h[2 * i + 1]};
     }
+    // CKKSEncoderCuda::encodePolynomial / decodePolynomial (src/ckks_cuda.cuh:34-40, 84-87):
+    // the values are the coefficients themselves (times scale, rounded half away from
+    // zero), no FFT; decode returns all N coefficients over the plaintext's scale.
+    void encodePolynomial(const std::vector<double> &values, ParmsID id, double scale, Plaintext &plain) const {
+        const uint32_t L = ctx_.levelOrThrow(id, "parms_id");
+        if (values.size() > ctx_.polyModulusDegree()) throw std::invalid_argument("values_size is too large");
+        DeviceArray<double> dv;
+        dv.fromHost(values.data(), values.size());
+        plain.parmsID() = parmsIDZero;
+        plain.resize(L * ctx_.polyModulusDegree());
+        size_t ws = 0;
+        check(shc_ckks_encode_workspace_size(ctx_.handle(), &ws));
+        check(shc_ckks_encode_polynomial(ctx_.handle(), dv.get(), (uint32_t)values.size(), scale, L, plain.data(),
+                                         ctx_.workspace(ws), ws, nullptr));
+        plain.parmsID() = id;
+        plain.scale() = scale;
+    }
+    void encodePolynomial(const std::vector<double> &values, double scale, Plaintext &plain) const {
+        encodePolynomial(values, ctx_.firstParmsID(), scale, plain);
+    }
+    void decodePolynomial(const Plaintext &plain, std::vector<double> &out) const {
+        if (!plain.isNttForm()) throw std::invalid_argument("plain is not in NTT form");
+        const uint32_t L = ctx_.levelOrThrow(plain.parmsID(), "plain");
+        DeviceArray<double> dv(ctx_.polyModulusDegree());
+        size_t ws = 0;
+        check(shc_ckks_decode_workspace_size(ctx_.handle(), L, &ws));
+        check(shc_ckks_decode_polynomial(ctx_.handle(), plain.data(), L, plain.scale(), dv.get(), ctx_.workspace(ws),
+                                         ws, nullptr));
+        out = dv.toHost();
+    }
 
 private:
     SEALContext ctx_;
@@ -1574,6 +1641,17 @@ public:
     Encryptor(const SEALContext &ctx, const PublicKey &pk, const SecretKey &sk) : ctx_(ctx), pk_(pk), sk_(sk) {
         ctx_.checkSet();
         has_pk_ = has_sk_ = true;
+    }
+    // EncryptorCuda::setPublicKey / setSecretKey (src/encryptor_cuda.cuh:87-101).
+    void setPublicKey(const PublicKey &pk) {
+        if (pk.parmsID() != ctx_.keyParmsID()) throw std::invalid_argument("public key is not valid for encryption parameters");
+        pk_ = pk;
+        has_pk_ = true;
+    }
+    void setSecretKey(const SecretKey &sk) {
+        if (sk.parmsID() != ctx_.keyParmsID()) throw std::invalid_argument("secret key is not valid for encryption parameters");
+        sk_ = sk;
+        has_sk_ = true;
     }
 
     void encrypt(const Plaintext &plain, Ciphertext &ct) const {
@@ -1613,21 +1691,49 @@ public:
         else
             check(shc_bfv_add_plain(ctx_.handle(), ct.data(), p, ct.data(), 1, 2, L, 0, nullptr));
     }
+    Ciphertext encryptSymmetric(const Plaintext &plain) const {
+        Ciphertext ct;
+        encryptSymmetric(plain, ct);
+        return ct;
+    }
+    // EncryptorCuda::encryptZero (src/encryptor_cuda.cuh:170-236, encryptZeroInternal at
+    // src/encryptor_cuda.cu:92-166): an encryption of zero at the level of `id` (below the
+    // first level: one level above, then divide-and-round), in the scheme's default form.
     void encryptZero(ParmsID id, Ciphertext &ct) const {
         if (!has_pk_) throw std::logic_error("public key is not set");
         const uint32_t L = ctx_.levelOrThrow(id, "parms_id");
-        Plaintext zero;
-        if (ctx_.scheme() == SchemeType::ckks) {
-            zero.resize(L * ctx_.polyModulusDegree());
-            zero.setZero();
-            zero.parmsID() = id;
-        } else {
-            zero.resize(ctx_.polyModulusDegree());
-            zero.setZero();
-        }
-        encrypt(zero, ct);
+        ct.resize(ctx_, id, 2);
+        ct.isNttForm() = ctx_.scheme() == SchemeType::ckks;
+        ct.scale() = 1.0;
+        size_t ws = 0;
+        check(shc_encrypt_workspace_size(ctx_.handle(), L, &ws));
+        check(shc_encrypt(ctx_.handle(), pk_.data().data(), nullptr, L, detail::os_random64(), ct.data(),
+                          ctx_.workspace(ws), ws, nullptr));
     }
     void encryptZero(Ciphertext &ct) const { encryptZero(ctx_.firstParmsID(), ct); }
+    Ciphertext encryptZero(ParmsID id) const {
+        Ciphertext ct;
+        encryptZero(id, ct);
+        return ct;
+    }
+    Ciphertext encryptZero() const { return encryptZero(ctx_.firstParmsID()); }
+    // EncryptorCuda::encryptZeroSymmetric (src/encryptor_cuda.cuh:312-388).
+    void encryptZeroSymmetric(ParmsID id, Ciphertext &ct) const {
+        if (!has_sk_) throw std::logic_error("secret key is not set");
+        const uint32_t L = ctx_.levelOrThrow(id, "parms_id");
+        ct.resize(ctx_, id, 2);
+        ct.isNttForm() = ctx_.scheme() == SchemeType::ckks;
+        ct.scale() = 1.0;
+        check(shc_encrypt_zero_symmetric(ctx_.handle(), sk_.data().data(), L, detail::os_random64(), ct.data(),
+                                         nullptr));
+    }
+    void encryptZeroSymmetric(Ciphertext &ct) const { encryptZeroSymmetric(ctx_.firstParmsID(), ct); }
+    Ciphertext encryptZeroSymmetric(ParmsID id) const {
+        Ciphertext ct;
+        encryptZeroSymmetric(id, ct);
+        return ct;
+    }
+    Ciphertext encryptZeroSymmetric() const { return encryptZeroSymmetric(ctx_.firstParmsID()); }
 
 private:
     ParmsID levelOf(const Plaintext &plain) const {

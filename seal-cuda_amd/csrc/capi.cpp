@@ -528,6 +528,24 @@ int shc_ckks_decode(const shc_context *ctx, const uint64_t *plain, uint32_t limb
     });
 }
 
+int shc_ckks_encode_polynomial(const shc_context *ctx, const double *values, uint32_t count, double scale,
+                               uint32_t limbs, uint64_t *plain, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ckks_encode_polynomial(c, values, count, scale, limbs, plain, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
+int shc_ckks_decode_polynomial(const shc_context *ctx, const uint64_t *plain, uint32_t limbs, double scale,
+                               double *values, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::ckks_decode_polynomial(c, plain, limbs, scale, values, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
 int shc_rescale_workspace_size(const shc_context *ctx, uint32_t count, uint32_t polys, uint32_t limbs, size_t *bytes) {
     return guarded([&] {
         const auto &c = ctx_of(ctx);

@@ -45,5 +45,13 @@ size_t ckks_decode_workspace(const Context &c, uint32_t L);
 // plain [L][N] NTT form -> N/2 (re, im) double pairs.
 void ckks_decode(const Context &c, const uint64_t *plain, uint32_t L, double scale, double *values, void *ws,
                  size_t ws_bytes, hipStream_t st);
+// CKKSEncoderCuda::encodePolynomial / decodePolynomial (src/ckks_cuda.cu:455-572,
+// 983-1053): count <= N real coefficients times scale, rounded half away from zero, in
+// NTT form (workspace: ckks_encode_workspace); decode gives the N real coefficients over
+// scale (workspace: ckks_decode_workspace).
+void ckks_encode_polynomial(const Context &c, const double *values, uint32_t count, double scale, uint32_t L,
+                            uint64_t *plain, void *ws, size_t ws_bytes, hipStream_t st);
+void ckks_decode_polynomial(const Context &c, const uint64_t *plain, uint32_t L, double scale, double *values,
+                            void *ws, size_t ws_bytes, hipStream_t st);
 
 }  // namespace shc

@@ -89,8 +89,10 @@ __global__ void fft_fwd_layer(double2 *__restrict__ a, const double2 *__restrict
     }
 }
 
-// round the real parts, reduce into each limb; record max |coefficient| (positive
-// doubles order like their bit patterns, so an integer atomicMax works)
+// round the real parts (half away from zero, as std::round in the reference's
+// src/ckks.cpp:176 and src/ckks_cuda.cu:220), reduce into each limb; record max
+// |coefficient| (positive doubles order like their bit patterns, so an integer
+// atomicMax works)
 __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__restrict__ out,
                                     const DevPrime *__restrict__ primes, uint32_t L, int logn,
                                     unsigned long long *maxbits) {
@@ -100,7 +102,7 @@ __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__r
     out += blockIdx.y * L * n;
     maxbits += blockIdx.y;
     for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < n; c += (size_t)gridDim.x * TPB) {
-        double d = rint(a[c].x);
+        double d = round(a[c].x);
         const bool neg = signbit(d);
         d = fabs(d);
         atomicMax(maxbits, (unsigned long long)__double_as_longlong(d));
@@ -113,6 +115,24 @@ __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__r
             out[l * n + c] = neg ? negmod(r, P.q) : r;
         }
     }
+}
+
+// Polynomial encoding: coefficient i is values[i] * scale (zero past count); the
+// maximum is taken before rounding, as the reference's (src/ckks_cuda.cu:490-522).
+__global__ void poly_scale_kernel(const double *__restrict__ values, uint32_t count, double scale,
+                                  double2 *__restrict__ a, int logn, unsigned long long *maxbits) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
+        const double v = i < count ? values[i] * scale : 0.0;
+        a[i] = make_double2(v, 0.0);
+        atomicMax(maxbits, (unsigned long long)__double_as_longlong(fabs(v)));
+    }
+}
+
+// Polynomial decoding: the real coefficients of a composed plaintext.
+__global__ void real_parts_kernel(const double2 *__restrict__ a, double *__restrict__ values, int logn) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) values[i] = a[i].x;
 }
 
 __global__ void clear_words_kernel(unsigned long long *w, uint32_t words) {
@@ -269,6 +289,23 @@ size_t ckks_encode_workspace(const Context &c) { return ckks_encode_workspace_ma
 
 namespace {
 
+// Reads the per-plaintext maxima back and raises the reference's range errors
+// (CKKSEncoder::encodeInternal, src/ckks.cpp:167-171).
+void host_range_check(const unsigned long long *maxbits, uint32_t batch, const CkksLevelDev &T, hipStream_t st) {
+    std::vector<unsigned long long> mb(batch);
+    SHC_HIP(hipMemcpyAsync(mb.data(), maxbits, 8 * (size_t)batch, hipMemcpyDeviceToHost, st));
+    SHC_HIP(hipStreamSynchronize(st));
+    for (uint32_t p = 0; p < batch; p++) {
+        double maxc;
+        std::memcpy(&maxc, &mb[p], 8);
+        // an infinite or NaN input makes the maximum non-finite (NaN orders above +inf as
+        // an unsigned bit pattern), which no coefficient modulus can hold
+        if (!std::isfinite(maxc)) fail(ERR_INVALID_ARGUMENT, "encoded values are not finite");
+        const int bits = (int)std::ceil(std::log2(std::max(maxc, 1.0))) + 1;
+        if (bits >= T.total_bits || bits > 128) fail(ERR_INVALID_ARGUMENT, "encoded values are too large");
+    }
+}
+
 // Encode `batch` plaintexts up to the forward NTT: values of plaintext p at
 // values + p * vstride, plaintext p at plain + p * L * N.  d_status: the range checks go
 // to d_status[p] on the device (a rejected plaintext is zero) and nothing waits for the
@@ -306,18 +343,7 @@ void encode_body(const Context &c, const double *values, uint32_t count, size_t 
         launch_check("ckks encode check");
         return;
     }
-    std::vector<unsigned long long> mb(batch);
-    SHC_HIP(hipMemcpyAsync(mb.data(), maxbits, 8 * (size_t)batch, hipMemcpyDeviceToHost, st));
-    SHC_HIP(hipStreamSynchronize(st));
-    for (uint32_t p = 0; p < batch; p++) {
-        double maxc;
-        std::memcpy(&maxc, &mb[p], 8);
-        // an infinite or NaN input makes the maximum non-finite (NaN orders above +inf as
-        // an unsigned bit pattern), which no coefficient modulus can hold
-        if (!std::isfinite(maxc)) fail(ERR_INVALID_ARGUMENT, "encoded values are not finite");
-        const int bits = (int)std::ceil(std::log2(std::max(maxc, 1.0))) + 1;
-        if (bits >= T.total_bits || bits > 128) fail(ERR_INVALID_ARGUMENT, "encoded values are too large");
-    }
+    host_range_check(maxbits, batch, T, st);
 }
 
 }  // namespace
@@ -380,6 +406,40 @@ void ckks_decode_many(const Context &c, const uint64_t *plain, uint32_t L, uint3
 void ckks_decode(const Context &c, const uint64_t *plain, uint32_t L, double scale, double *values, void *ws,
                  size_t ws_bytes, hipStream_t st) {
     ckks_decode_many(c, plain, L, 1, scale, values, ws, ws_bytes, st);
+}
+
+void ckks_encode_polynomial(const Context &c, const double *values, uint32_t count, double scale, uint32_t L,
+                            uint64_t *plain, void *ws, size_t ws_bytes, hipStream_t st) {
+    if (c.scheme != SCHEME_CKKS || !c.ckks) fail(ERR_INVALID_ARGUMENT, "unsupported scheme");
+    if (count > c.n) fail(ERR_INVALID_ARGUMENT, "values_size is too large");
+    if (count && !values) fail(ERR_INVALID_ARGUMENT, "values cannot be null");
+    if (!ws || ws_bytes < ckks_encode_workspace_many(c, 1)) fail(ERR_INVALID_ARGUMENT, "workspace too small for encoding");
+    const CkksLevelDev &T = c.ckks->levels[L];
+    const size_t n = c.n;
+    double2 *a = static_cast<double2 *>(ws);
+    auto *maxbits = reinterpret_cast<unsigned long long *>(static_cast<char *>(ws) + n * 16);
+    clear_words_kernel<<<1, TPB, 0, st>>>(maxbits, 1);
+    poly_scale_kernel<<<blocks_for(n), TPB, 0, st>>>(values, count, scale, a, c.logn, maxbits);
+    round_reduce_kernel<<<dim3(blocks_for(n), 1), TPB, 0, st>>>(a, plain, c.primes(), L, c.logn, maxbits);
+    launch_check("ckks polynomial encode");
+    host_range_check(maxbits, 1, T, st);
+    ntt_forward(plain, L, L, c.primes(), c.logn, NTT_FULL, st);
+}
+
+void ckks_decode_polynomial(const Context &c, const uint64_t *plain, uint32_t L, double scale, double *values,
+                            void *ws, size_t ws_bytes, hipStream_t st) {
+    if (c.scheme != SCHEME_CKKS || !c.ckks) fail(ERR_INVALID_ARGUMENT, "unsupported scheme");
+    if (!ws || ws_bytes < ckks_decode_workspace_many(c, L, 1)) fail(ERR_INVALID_ARGUMENT, "workspace too small for decoding");
+    const CkksLevelDev &T = c.ckks->levels[L];
+    if (!(scale > 0) || (int)std::log2(scale) >= T.total_bits) fail(ERR_INVALID_ARGUMENT, "scale out of bounds");
+    const size_t n = c.n;
+    uint64_t *cp = static_cast<uint64_t *>(ws);
+    double2 *a = reinterpret_cast<double2 *>(static_cast<char *>(ws) + (((size_t)L * n * 8 + 255) & ~size_t(255)));
+    SHC_HIP(hipMemcpyAsync(cp, plain, (size_t)L * n * 8, hipMemcpyDeviceToDevice, st));
+    ntt_inverse(cp, L, L, c.primes(), c.logn, NTT_FULL, st);
+    compose_kernel<<<dim3(blocks_for(n), 1), TPB, 0, st>>>(cp, a, T, c.primes(), c.logn, 1.0 / scale);
+    real_parts_kernel<<<blocks_for(n), TPB, 0, st>>>(a, values, c.logn);
+    launch_check("ckks polynomial decode");
 }
 
 }  // namespace shc

@@ -77,6 +77,8 @@ def lib():
         L.shc_ckks_encode_deferred.argtypes = [_vp, _vp, u32, ctypes.c_double, u32, _vp, _vp, _vp, sz, _vp]
         L.shc_ckks_decode_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
         L.shc_ckks_decode.argtypes = [_vp, _vp, u32, ctypes.c_double, _vp, _vp, sz, _vp]
+        L.shc_ckks_encode_polynomial.argtypes = [_vp, _vp, u32, ctypes.c_double, u32, _vp, _vp, sz, _vp]
+        L.shc_ckks_decode_polynomial.argtypes = [_vp, _vp, u32, ctypes.c_double, _vp, _vp, sz, _vp]
         L.shc_rescale_workspace_size.argtypes = [_vp, u32, u32, u32, ctypes.POINTER(sz)]
         L.shc_rescale_to_next.argtypes = [_vp, _vp, _vp, u32, u32, u32, _vp, sz, _vp]
         L.shc_probe_keyswitch.argtypes = [_vp, _vp, _vp]
@@ -365,6 +367,15 @@ class Context:
 
     def ckks_decode(self, d_plain, limbs, scale, d_values, d_ws, ws_bytes, stream=None):
         check(lib().shc_ckks_decode(self.handle, d_plain, limbs, scale, d_values, d_ws, ws_bytes, stream))
+
+    def ckks_encode_polynomial(self, d_values, count, scale, limbs, d_plain, d_ws, ws_bytes, stream=None):
+        """count <= N real coefficients times scale (no FFT); workspace as ckks_encode."""
+        check(lib().shc_ckks_encode_polynomial(self.handle, d_values, count, scale, limbs, d_plain, d_ws, ws_bytes,
+                                               stream))
+
+    def ckks_decode_polynomial(self, d_plain, limbs, scale, d_values, d_ws, ws_bytes, stream=None):
+        """The N real coefficients over scale; workspace as ckks_decode."""
+        check(lib().shc_ckks_decode_polynomial(self.handle, d_plain, limbs, scale, d_values, d_ws, ws_bytes, stream))
 
     def rescale_workspace_size(self, count, polys, limbs):
         b = ctypes.c_size_t()

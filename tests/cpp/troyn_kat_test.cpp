@@ -504,6 +504,87 @@ static void bfv_many_and_exponentiate() {
     EXPECT(plain.to_string() == "3Fx^4 + 3Fx^3 + 3E" && same_level(encrypteds, sum), "BFVEncryptAddManyDecrypt: two");
 }
 
+// BatchEncoderCuda / CKKSEncoderCuda encodePolynomial / decodePolynomial
+// (src/batchencoder_cuda.cu:124-170, 267-285; src/ckks_cuda.cu:455-572, 983-1053): the
+// values are coefficients; products of encrypted polynomials decode to the polynomial
+// products (the coefficient-encoded callers, app/LinearHelperCKKS.cuh:139-670).
+static void polynomial_encoders() {
+    {
+        EncryptionParameters parms(SchemeType::bfv);
+        parms.setPolyModulusDegree(4096);
+        parms.setPlainModulus(PlainModulus::Batching(4096, 17));
+        parms.setCoeffModulus(CoeffModulus::Create(4096, {36, 36, 37}));
+        SEALContext context(parms);
+        const uint64_t t = parms.plainModulus().value();
+        BatchEncoder encoder(context);
+        Plaintext p;
+        encoder.encodePolynomial(std::vector<uint64_t>{1, 2, t + 4463}, p);
+        EXPECT(p.coeffCount() == 3 && p.to_string() == "116Fx^2 + 2x^1 + 1" && !p.isNttForm(),
+               "BatchEncoder::encodePolynomial (unsigned): coefficients mod t, values.size() of them");
+        Plaintext q;
+        encoder.encodePolynomial(std::vector<int64_t>{-1, 5, -(int64_t)t, -3}, q);
+        std::vector<uint64_t> qu;
+        encoder.decodePolynomial(q, qu);
+        EXPECT(q.coeffCount() == 4096 && qu.size() == 4096 && qu[0] == t - 1 && qu[1] == 5 && qu[2] == t &&
+                   qu[3] == t - 3 && qu[4] == 0,
+               "BatchEncoder::encodePolynomial (signed): t - (|v| mod t), padded to N");
+        std::vector<int64_t> qs;
+        encoder.decodePolynomial(q, qs);
+        EXPECT(qs[0] == -1 && qs[1] == 5 && qs[3] == -3 && qs[4] == 0, "BatchEncoder::decodePolynomial (signed)");
+        std::vector<uint64_t> pu;
+        encoder.decodePolynomial(p, pu);
+        EXPECT(pu.size() == 4096 && pu[0] == 1 && pu[1] == 2 && pu[2] == 4463 && pu[3] == 0,
+               "decodePolynomial of a short plaintext pads with zeros");
+
+        KeyGenerator keygen(context);
+        Encryptor encryptor(context, keygen.createPublicKey());
+        Decryptor decryptor(context, keygen.secretKey());
+        Evaluator evaluator(context);
+        RelinKeys rk = keygen.createRelinKeys();
+        Plaintext a, b, out;
+        encoder.encodePolynomial(std::vector<int64_t>{2, 1}, a);  // x + 2
+        encoder.encodePolynomial(std::vector<int64_t>{3, -1}, b);  // 3 - x
+        Ciphertext ca = encryptor.encrypt(a), cb = encryptor.encrypt(b);
+        evaluator.multiplyInplace(ca, cb);
+        evaluator.relinearizeInplace(ca, rk);
+        decryptor.decrypt(ca, out);
+        std::vector<int64_t> prod;
+        encoder.decodePolynomial(out, prod);
+        EXPECT(prod[0] == 6 && prod[1] == 1 && prod[2] == -1 && prod[3] == 0,
+               "BFV: (x + 2)(3 - x) through encodePolynomial, multiply, relinearize, decodePolynomial");
+    }
+    {
+        EncryptionParameters parms(SchemeType::ckks);
+        parms.setPolyModulusDegree(8192);
+        parms.setCoeffModulus(CoeffModulus::Create(8192, {60, 40, 40, 60}));
+        SEALContext context(parms);
+        CKKSEncoder encoder(context);
+        const double scale = std::pow(2.0, 40);
+        Plaintext p;
+        encoder.encodePolynomial(std::vector<double>{0.5, 0.25}, scale, p);
+        std::vector<double> back;
+        encoder.decodePolynomial(p, back);
+        EXPECT(p.isNttForm() && p.parmsID() == context.firstParmsID() && p.scale() == scale && back.size() == 8192 &&
+                   back[0] == 0.5 && back[1] == 0.25 && back[2] == 0.0,
+               "CKKSEncoder::encodePolynomial / decodePolynomial round trip");
+        KeyGenerator keygen(context);
+        Encryptor encryptor(context, keygen.createPublicKey());
+        Decryptor decryptor(context, keygen.secretKey());
+        Evaluator evaluator(context);
+        Plaintext m, out;
+        encoder.encodePolynomial(std::vector<double>{2.0, -1.0}, scale, m);  // 2 - x
+        Ciphertext c = encryptor.encrypt(p);                                   // 0.5 + 0.25 x
+        evaluator.multiplyPlainInplace(c, m);
+        evaluator.rescaleToNextInplace(c);
+        decryptor.decrypt(c, out);
+        std::vector<double> r;
+        encoder.decodePolynomial(out, r);
+        double err = std::fabs(r[0] - 1.0) + std::fabs(r[1]) + std::fabs(r[2] + 0.25);
+        for (size_t i = 3; i < r.size(); i++) err = std::max(err, std::fabs(r[i]));
+        EXPECT(err < 1e-5, "CKKS: (0.5 + 0.25x)(2 - x) = 1 - 0.25x^2 through encodePolynomial and multiplyPlain");
+    }
+}
+
 int main() {
     KernelProvider::initialize();
     bfv_relinearize();
@@ -511,6 +592,7 @@ int main() {
     containers_and_keys();
     key_files();
     bfv_many_and_exponentiate();
+    polynomial_encoders();
     std::printf("troyn kat ok\n");
     return 0;
 }

@@ -9,6 +9,8 @@ decode equals the oracle's decode of the same words within 2^-20 absolute, every
 intermediate ciphertext word equals the oracle's on the same inputs, and the result
 is the product of the inputs up to CKKS noise.
 """
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -286,3 +288,68 @@ def test_encode_decode_many_equal_single_calls():
     with pytest.raises(hip().SealHipError, match="too large"):
         ctx.ckks_encode_many(dv.data_ptr(), count, stride, P, 2.0 ** 40, L, plains.data_ptr(), None, eb.data_ptr(),
                              ew, stream())
+
+
+def round_half_away(x):
+    """std::round of a double, as a Python integer (exact for every finite double)."""
+    a = abs(x)
+    f = math.floor(a)
+    r = f + 1 if a - f >= 0.5 else f
+    return -int(r) if x < 0 else int(r)
+
+
+@pytest.mark.parametrize("logn,bits,L,scale", [
+    (10, [60, 40, 40, 60], 3, 2.0 ** 40),
+    (12, [60, 60, 60, 60], 3, 2.0 ** 80),
+    (10, [60, 40, 60], 2, 1.0),
+])
+def test_encode_decode_polynomial(logn, bits, L, scale):
+    """CKKSEncoderCuda::encodePolynomial / decodePolynomial (src/ckks_cuda.cu:455-572, 983-1053):
+    coefficient i is std::round(values[i] * scale) (half away from zero; ties included)
+    reduced into every prime and transformed to NTT form, word for word; decode returns
+    every coefficient over scale."""
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, bits)
+    ctx = hip().Context(CKKS, n, moduli)
+    rng = np.random.default_rng(logn + L)
+    count = n - 5  # the last coefficients are zero
+    if scale == 1.0:
+        v = rng.integers(-2000, 2000, count) / 2.0  # exact halves: every odd numerator is a tie
+    else:
+        v = rng.uniform(-1, 1, count)
+        v[:6] = np.array([0.5, -0.5, 2.5, -2.5, 1e6 + 0.5, -(1e6 + 0.5)]) / scale  # ties after scaling
+    coeffs = [round_half_away(float(x) * scale) for x in v] + [0] * (n - count)
+    want = np.array([[c % q for c in coeffs] for q in moduli[:L]], dtype=np.uint64)
+    want = ck.ntt_rns(want, moduli[:L], logn)
+    dv = to_dev_f64(v)
+    plain = empty_dev((L, n))
+    ew = ctx.ckks_encode_workspace_size()
+    eb = ws(ew)
+    ctx.ckks_encode_polynomial(dv.data_ptr(), count, scale, L, plain.data_ptr(), eb.data_ptr(), ew, stream())
+    np.testing.assert_array_equal(from_dev(plain), want)
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    dw = ctx.ckks_decode_workspace_size(L)
+    db = ws(dw)
+    ctx.ckks_decode_polynomial(plain.data_ptr(), L, scale, out.data_ptr(), db.data_ptr(), dw, stream())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    exact = np.array([float(c) for c in coeffs]) / scale
+    assert np.max(np.abs(got - exact) / np.maximum(1.0, np.abs(exact))) < 2.0 ** -50
+    assert np.max(np.abs(got[:count] - v) - 0.5 / scale) <= 2.0 ** -50
+
+
+def test_encode_polynomial_rejects():
+    n = 1 << 10
+    moduli = ck.coeff_modulus_create(n, [30, 30])
+    ctx = hip().Context(CKKS, n, moduli)
+    plain = empty_dev((2, n))
+    ew = ctx.ckks_encode_workspace_size()
+    eb = ws(ew)
+    big = to_dev_f64(np.array([1.0, 1e12]))
+    with pytest.raises(hip().SealHipError, match="too large"):
+        ctx.ckks_encode_polynomial(big.data_ptr(), 2, 2.0 ** 40, 2, plain.data_ptr(), eb.data_ptr(), ew, stream())
+    bad = to_dev_f64(np.array([0.5, np.nan]))
+    with pytest.raises(hip().SealHipError, match="not finite"):
+        ctx.ckks_encode_polynomial(bad.data_ptr(), 2, 2.0 ** 20, 2, plain.data_ptr(), eb.data_ptr(), ew, stream())
+    with pytest.raises(hip().SealHipError, match="too large"):
+        ctx.ckks_encode_polynomial(big.data_ptr(), n + 1, 1.0, 2, plain.data_ptr(), eb.data_ptr(), ew, stream())
