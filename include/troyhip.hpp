@@ -299,12 +299,13 @@ public:
     }
     ~DeviceArray() { release(); }
 
-    // Contents are not kept when the block changes: a block between half and all of
-    // its capacity in use is kept, else it goes back to the pool for one of the new size.
+    // The block is kept while n fits its capacity (as DeviceDynamicArray::resize,
+    // src/utils/devicearray.cuh:282-285, so a reserve lasts); a larger n takes a new
+    // block from the pool and, unlike the reference's, does not copy the contents.
     void resize(size_t n) {
         if (n == size_) return;
         const size_t bytes = n * sizeof(T);
-        if (ptr_ && bytes <= cap_ && 2 * bytes >= cap_) {
+        if (ptr_ && bytes <= cap_) {
             size_ = n;
             return;
         }
@@ -331,6 +332,15 @@ public:
     void ensure(size_t n) {
         if (n > size_) resize(n);
     }
+    // Capacity for n elements, keeping the contents (DeviceDynamicArray::reserve,
+    // src/utils/devicearray.cuh); shrinkToFit moves them to a block of their own size.
+    void reserve(size_t n) {
+        if (n * sizeof(T) <= cap_) return;
+        moveTo(n);
+    }
+    void shrinkToFit() {
+        if (cap_ > size_ * sizeof(T)) moveTo(size_);
+    }
     void fromHost(const T *src, size_t n) {
         resize(n);
         if (n) hip_check(hipMemcpy(ptr_, src, n * sizeof(T), hipMemcpyHostToDevice), "copy to device");
@@ -354,6 +364,18 @@ public:
     size_t capacity() const { return cap_ / sizeof(T); }
 
 private:
+    // A block for `elements` (none when zero) holding the first min(size, elements) ones.
+    void moveTo(size_t elements) {
+        const size_t keep = std::min(size_, elements);
+        T *p = nullptr;
+        size_t cap = 0;
+        if (elements) p = static_cast<T *>(MemoryPool::get(elements * sizeof(T), cap));
+        if (keep) hip_check(hipMemcpy(p, ptr_, keep * sizeof(T), hipMemcpyDeviceToDevice), "copy");
+        if (ptr_) MemoryPool::put(ptr_, cap_);
+        ptr_ = p;
+        cap_ = cap;
+        size_ = keep;
+    }
     T *ptr_ = nullptr;
     size_t size_ = 0;
     size_t cap_ = 0;  // bytes of the pool block
@@ -920,6 +942,13 @@ public:
         data_.release();
     }
     void setZero() { data_.setZero(); }
+    // PlaintextCuda::reserve / shrinkToFit (src/plaintext_cuda.cuh:59-72): capacity for
+    // `capacity` coefficients, keeping the current ones.
+    void reserve(size_t capacity) {
+        if (isNttForm()) throw std::logic_error("cannot reserve for an NTT transformed Plaintext");
+        data_.reserve(capacity);
+    }
+    void shrinkToFit() { data_.shrinkToFit(); }
     void setZero(size_t start_coeff) {
         if (start_coeff >= coeffCount()) throw std::out_of_range("start_coeff must be within [0, coeff_count)");
         hip_check(hipMemset(data_.get() + start_coeff, 0, (coeffCount() - start_coeff) * 8), "memset");
@@ -997,6 +1026,19 @@ public:
         size_ = size;
         data_.resize(size * L_ * n_);
     }
+    // CiphertextCuda::reserve / sizeCapacity (src/ciphertext_cuda.cuh:125-215): room for
+    // size_capacity polynomials of the level; the words that fit are kept and the size is
+    // capped at size_capacity, as the reference's reserveInternal.
+    void reserve(const SEALContext &ctx, ParmsID id, size_t size_capacity) {
+        const uint32_t L = ctx.levelOrThrow(id, "parms_id");
+        parms_id_ = id;
+        reserveWords(size_capacity, ctx.polyModulusDegree(), L);
+    }
+    void reserve(const SEALContext &ctx, size_t size_capacity) { reserve(ctx, ctx.firstParmsID(), size_capacity); }
+    void reserve(size_t size_capacity) { reserveWords(size_capacity, n_, L_); }
+    size_t sizeCapacity() const { return L_ * n_ ? data_.capacity() / (L_ * n_) : 0; }
+    // Ciphertexts here are never seeded (c1 is always stored).
+    uint64_t seed() const { return 0; }
     void release() {
         parms_id_ = parmsIDZero;
         ntt_form_ = false;
@@ -1112,6 +1154,15 @@ private:
         ntt_form_ = h.ntt;
         scale_ = h.scale;
         correction_ = h.correction;
+    }
+    void reserveWords(size_t size_capacity, size_t n, size_t L) {
+        if (size_capacity < 2 || size_capacity > 16) throw std::invalid_argument("invalid size_capacity");
+        const size_t words = size_capacity * n * L;
+        data_.reserve(words);
+        data_.resizeKeep(std::min(words, data_.size()));
+        size_ = std::min(size_capacity, size_);
+        n_ = n;
+        L_ = L;
     }
     void loadWords(std::istream &is) {
         const uint64_t words = detail::get<uint64_t>(is, "stream ended before the ciphertext did");

@@ -211,6 +211,21 @@ static void containers_and_keys() {
     loaded.load(ss, context);
     EXPECT(loaded.parmsID() == first && loaded.coeffModulusSize() == 3, "loaded ciphertext level");
 
+    // reserve / capacity (src/ciphertext_cuda.cuh:125-215, src/plaintext_cuda.cuh:59-72)
+    Plaintext pr("3x^2 + 1");
+    pr.reserve(5000);
+    EXPECT(pr.capacity() >= 5000 && pr.to_string() == "3x^2 + 1", "plaintext reserve keeps its coefficients");
+    pr.shrinkToFit();  // (pool blocks are whole 4 KiB pages)
+    EXPECT(pr.capacity() < 5000 && pr.to_string() == "3x^2 + 1", "plaintext shrinkToFit keeps its coefficients");
+    EXPECT(throws<std::logic_error>([&] { Plaintext q = p; q.reserve(10); }), "no reserve for an NTT-form plaintext");
+    Ciphertext cr = ct;
+    const std::vector<uint64_t> words = cr.toHost().data();
+    cr.reserve(5);
+    EXPECT(cr.sizeCapacity() >= 5 && cr.size() == 2 && cr.toHost().data() == words, "ciphertext reserve keeps its words");
+    cr.resize(3);
+    EXPECT(cr.sizeCapacity() >= 5 && cr.size() == 3, "a resize within the capacity keeps the block");
+    EXPECT(throws<std::invalid_argument>([&] { cr.reserve(17); }), "size_capacity above 16 throws");
+
     // steady state: a mul + relin + rescale loop reuses pooled blocks
     RelinKeys rk = keygen.createRelinKeys();
     auto step = [&] {
@@ -552,6 +567,26 @@ static void polynomial_encoders() {
         encoder.decodePolynomial(out, prod);
         EXPECT(prod[0] == 6 && prod[1] == 1 && prod[2] == -1 && prod[3] == 0,
                "BFV: (x + 2)(3 - x) through encodePolynomial, multiply, relinearize, decodePolynomial");
+
+        // EncryptorCuda::encryptZero at a lower level, encryptZeroSymmetric, setPublicKey
+        // (src/encryptor_cuda.cuh:87-101, 170-388)
+        const ParmsID second = context.getContextData(context.firstParmsID())->nextParmsID();
+        Ciphertext z = encryptor.encryptZero(second);
+        decryptor.decrypt(z, out);
+        EXPECT(z.parmsID() == second && z.coeffModulusSize() == 1 && !z.isNttForm() && out.to_string() == "0",
+               "encryptZero at the second level decrypts to 0");
+        Encryptor sym(context, keygen.secretKey());
+        Ciphertext zs = sym.encryptZeroSymmetric();
+        decryptor.decrypt(zs, out);
+        EXPECT(zs.parmsID() == context.firstParmsID() && out.to_string() == "0" &&
+                   throws<std::logic_error>([&] { (void)sym.encryptZero(); }),
+               "encryptZeroSymmetric decrypts to 0; no public key, no encryptZero");
+        sym.setPublicKey(keygen.createPublicKey());
+        Ciphertext ca2 = sym.encrypt(a);
+        decryptor.decrypt(ca2, out);
+        std::vector<int64_t> av;
+        encoder.decodePolynomial(out, av);
+        EXPECT(av[0] == 2 && av[1] == 1 && av[2] == 0, "setPublicKey enables public-key encryption");
     }
     {
         EncryptionParameters parms(SchemeType::ckks);
