@@ -471,33 +471,125 @@ static const ParmsID parmsIDZero{0, 0, 0, 0};
 
 class SEALContext;
 
+// EncryptionParameterQualifiers (src/context.h:22-215): what the parameters of a level
+// support (every level of a context that was created is valid here).
+struct EncryptionParameterQualifiers {
+    enum class ErrorType : int { none = -1, success = 0 };
+    ErrorType parameter_error = ErrorType::success;
+    bool using_fft = true;
+    bool using_ntt = true;
+    bool using_batching = false;
+    bool using_fast_plain_lift = false;
+    bool using_descending_modulus_chain = false;
+    SecurityLevel sec_level = SecurityLevel::tc128;
+    bool parametersSet() const { return parameter_error == ErrorType::success; }
+};
+
+namespace detail {
+// Deterministic Miller-Rabin for 64-bit values (the bases 2..37 decide every n < 2^64).
+inline bool is_prime64(uint64_t n) {
+    if (n < 2) return false;
+    for (uint64_t p : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull, 17ull, 19ull, 23ull, 29ull, 31ull, 37ull})
+        if (n % p == 0) return n == p;
+    uint64_t d = n - 1;
+    int r = 0;
+    while (!(d & 1)) {
+        d >>= 1;
+        r++;
+    }
+    auto mulm = [n](uint64_t a, uint64_t b) { return (uint64_t)((unsigned __int128)a * b % n); };
+    for (uint64_t a : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull, 17ull, 19ull, 23ull, 29ull, 31ull, 37ull}) {
+        uint64_t x = 1, base = a, e = d;
+        while (e) {
+            if (e & 1) x = mulm(x, base);
+            base = mulm(base, base);
+            e >>= 1;
+        }
+        if (x == 1 || x == n - 1) continue;
+        bool composite = true;
+        for (int k = 1; k < r && composite; k++) {
+            x = mulm(x, x);
+            if (x == n - 1) composite = false;
+        }
+        if (composite) return false;
+    }
+    return true;
+}
+}  // namespace detail
+
 // ContextDataCuda (src/context_cuda.cuh:11-170): one level of the chain.
 class ContextData {
 public:
-    ContextData(ParmsID id, uint32_t level, std::vector<Modulus> q, SchemeType scheme, size_t n, Modulus t,
-                ParmsID next)
-        : id_(id), level_(level), q_(std::move(q)), scheme_(scheme), n_(n), t_(t), next_(next) {}
+    ContextData(std::shared_ptr<const SEALContext> ctx, ParmsID id, uint32_t level, uint32_t last_level,
+                std::vector<Modulus> q, SchemeType scheme, size_t n, Modulus t, ParmsID next, SecurityLevel sec)
+        : ctx_(std::move(ctx)), id_(id), level_(level), last_(last_level), parms_(scheme), next_(next) {
+        parms_.setPolyModulusDegree(n);
+        parms_.setCoeffModulus(q);
+        parms_.setPlainModulus(t);
+        // the reference's qualifier rules (src/context.cpp:205-420)
+        qual_.sec_level = totalCoeffModulusBitCount() > CoeffModulus::MaxBitCount(n, sec) ? SecurityLevel::none : sec;
+        if (scheme == SchemeType::bfv) {
+            qual_.using_batching = detail::is_prime64(t.value()) && t.value() % (2 * n) == 1;
+            qual_.using_fast_plain_lift = true;
+            for (const Modulus &m : q) qual_.using_fast_plain_lift &= m.value() > t.value();
+        } else {
+            qual_.using_batching = true;
+        }
+        qual_.using_descending_modulus_chain = true;
+        for (size_t i = 0; i + 1 < q.size(); i++) qual_.using_descending_modulus_chain &= q[i].value() > q[i + 1].value();
+    }
     ParmsID parmsID() const { return id_; }
     // Position in the chain: 0 for the last level, growing towards the key level.
-    size_t chainIndex() const { return level_ - 1; }
-    const std::vector<Modulus> &coeffModulus() const { return q_; }
-    SchemeType scheme() const { return scheme_; }
-    size_t polyModulusDegree() const { return n_; }
-    const Modulus &plainModulus() const { return t_; }
+    size_t chainIndex() const { return level_ - last_; }
+    const EncryptionParameters &parms() const { return parms_; }
+    const EncryptionParameterQualifiers &qualifiers() const { return qual_; }
+    const std::vector<Modulus> &coeffModulus() const { return parms_.coeffModulus(); }
+    SchemeType scheme() const { return parms_.scheme(); }
+    size_t polyModulusDegree() const { return parms_.polyModulusDegree(); }
+    const Modulus &plainModulus() const { return parms_.plainModulus(); }
     int totalCoeffModulusBitCount() const {
         int b = 0;
-        for (auto &m : q_) b += m.bitCount();
+        for (auto &m : parms_.coeffModulus()) b += m.bitCount();
         return b;
     }
+    // The product of the level's primes as little-endian 64-bit words, on the host (the
+    // reference keeps the same words on the device).
+    std::vector<uint64_t> totalCoeffModulus() const {
+        std::vector<uint64_t> w{1};
+        for (const Modulus &m : parms_.coeffModulus()) {
+            unsigned __int128 carry = 0;
+            for (uint64_t &x : w) {
+                carry += (unsigned __int128)x * m.value();
+                x = (uint64_t)carry;
+                carry >>= 64;
+            }
+            if (carry) w.push_back((uint64_t)carry);
+        }
+        w.resize(parms_.coeffModulus().size(), 0);
+        return w;
+    }
+    // (t + 1) / 2: plaintext coefficients at or above it stand for negative values.
+    uint64_t plainUpperHalfThreshold() const { return (parms_.plainModulus().value() + 1) >> 1; }
+    // The product of the primes modulo t (BFV).
+    uint64_t coeffModulusModPlainModulus() const {
+        const uint64_t t = parms_.plainModulus().value();
+        if (!t) return 0;
+        uint64_t r = 1 % t;
+        for (const Modulus &m : parms_.coeffModulus()) r = (uint64_t)((unsigned __int128)r * (m.value() % t) % t);
+        return r;
+    }
     ParmsID nextParmsID() const { return next_; }
+    // The neighbouring levels (src/context_cuda.cuh:88-97): next drops a prime, prev adds
+    // one (the first level's prev is the key level); null past either end.
+    std::shared_ptr<const ContextData> nextContextData() const;
+    std::shared_ptr<const ContextData> prevContextData() const;
 
 private:
+    std::shared_ptr<const SEALContext> ctx_;
     ParmsID id_;
-    uint32_t level_;
-    std::vector<Modulus> q_;
-    SchemeType scheme_;
-    size_t n_;
-    Modulus t_;
+    uint32_t level_, last_;
+    EncryptionParameters parms_;
+    EncryptionParameterQualifiers qual_;
     ParmsID next_;
 };
 
@@ -522,6 +614,7 @@ public:
         KernelProvider::checkInitialized();
         s_->parms = parms;
         s_->expand = expand_mod_chain;
+        s_->sec = sec;
         const auto &q = parms.coeffModulus();
         int bits = 0;
         for (auto &m : q) bits += m.bitCount();
@@ -581,16 +674,19 @@ public:
     ParmsID firstParmsID() const { return idOf(firstLevel()); }
     ParmsID lastParmsID() const { return idOf(s_->expand ? 1 : firstLevel()); }
     bool usingKeyswitching() const { return keyCount() > 1; }
+    bool using_keyswitching() const { return usingKeyswitching(); }  // the reference's spelling
 
     std::shared_ptr<const ContextData> getContextData(ParmsID id) const {
         const uint32_t l = levelOf(id);
         if (!l || !inChain(l)) return nullptr;
         const auto &q = s_->parms.coeffModulus();
         const uint32_t last = s_->expand ? 1 : firstLevel();
-        const bool has_next = l > last && l != keyCount();
-        return std::make_shared<ContextData>(id, l, std::vector<Modulus>(q.begin(), q.begin() + l), scheme(),
-                                             polyModulusDegree(), s_->parms.plainModulus(),
-                                             has_next ? idOf(l - 1) : parmsIDZero);
+        // the key level's next is the first level (src/context.cpp:464-500)
+        const ParmsID next = l == keyCount() ? (keyCount() > 1 ? firstParmsID() : parmsIDZero)
+                                             : (l > last ? idOf(l - 1) : parmsIDZero);
+        return std::make_shared<ContextData>(std::make_shared<const SEALContext>(*this), id, l, last,
+                                             std::vector<Modulus>(q.begin(), q.begin() + l), scheme(),
+                                             polyModulusDegree(), s_->parms.plainModulus(), next, s_->sec);
     }
     std::shared_ptr<const ContextData> firstContextData() const { return getContextData(firstParmsID()); }
     std::shared_ptr<const ContextData> keyContextData() const { return getContextData(keyParmsID()); }
@@ -640,6 +736,7 @@ private:
     struct State {
         EncryptionParameters parms;
         bool expand = true;
+        SecurityLevel sec = SecurityLevel::tc128;
         std::string error;
         shc_context *handle = nullptr;
         std::vector<ParmsID> ids;               // by level (prime count)
@@ -651,6 +748,14 @@ private:
     };
     std::shared_ptr<State> s_;
 };
+
+inline std::shared_ptr<const ContextData> ContextData::nextContextData() const {
+    return next_ == parmsIDZero ? nullptr : ctx_->getContextData(next_);
+}
+inline std::shared_ptr<const ContextData> ContextData::prevContextData() const {
+    // every level between a chain level and the key level is in the chain
+    return level_ == ctx_->keyCount() ? nullptr : ctx_->getContextData(ctx_->idOf(level_ + 1));
+}
 
 namespace detail {
 template <class T>

@@ -173,6 +173,29 @@ static void containers_and_keys() {
     ParmsID bogus = first;
     bogus[0] ^= 1;
     EXPECT(!context.getContextData(bogus), "unknown parms_id has no context data");
+    // ContextDataCuda neighbours, chain indices, per-level parms and qualifiers
+    // (src/context_cuda.cuh:64-135; the reference's tests walk the chain as
+    // context.firstContextData()->nextContextData()->parmsID())
+    auto fcd = context.firstContextData(), kcd = context.keyContextData(), lcd = context.lastContextData();
+    EXPECT(fcd->nextContextData()->parmsID() == context.idOf(2) && !lcd->nextContextData() &&
+               kcd->nextContextData()->parmsID() == first && fcd->prevContextData()->parmsID() == key &&
+               !kcd->prevContextData() && lcd->prevContextData()->parmsID() == context.idOf(2),
+           "context data next / prev");
+    EXPECT(lcd->chainIndex() == 0 && fcd->chainIndex() == 2 && kcd->chainIndex() == 3, "chain indices");
+    EXPECT(fcd->parms().coeffModulus().size() == 3 && fcd->parms().polyModulusDegree() == n &&
+               fcd->qualifiers().using_batching && fcd->qualifiers().using_ntt && fcd->qualifiers().parametersSet() &&
+               fcd->qualifiers().sec_level == SecurityLevel::tc128 && context.using_keyswitching(),
+           "per-level parms and qualifiers");
+    {
+        const std::vector<uint64_t> w = fcd->totalCoeffModulus();
+        bool divides = w.size() == 3 && w[2] != 0;
+        for (const Modulus &m : fcd->coeffModulus()) {
+            unsigned __int128 r = 0;
+            for (size_t k = w.size(); k-- > 0;) r = ((r << 64) | w[k]) % m.value();
+            divides &= r == 0;
+        }
+        EXPECT(divides, "totalCoeffModulus is the product of the level's primes");
+    }
 
     KeyGenerator keygen(context), keygen2(context);
     EXPECT(keygen.secretKey().data().toHost().data() != keygen2.secretKey().data().toHost().data(),
@@ -371,6 +394,13 @@ static void bfv_many_and_exponentiate() {
     parms.setCoeffModulus(CoeffModulus::Create(128, {40, 40, 40}));
 
     SEALContext context(parms, false, SecurityLevel::none);
+    // a chain that is not expanded: the key level and the first level only
+    EXPECT(context.firstContextData()->chainIndex() == 0 && context.keyContextData()->chainIndex() == 1 &&
+               !context.firstContextData()->nextContextData() && context.lastParmsID() == context.firstParmsID() &&
+               !context.firstContextData()->qualifiers().using_batching &&
+               context.firstContextData()->qualifiers().using_fast_plain_lift &&
+               context.firstContextData()->qualifiers().sec_level == SecurityLevel::none,
+           "unexpanded chain: indices, neighbours, BFV qualifiers (t = 64: no batching)");
     KeyGenerator keygen(context);
     PublicKey pk;
     keygen.createPublicKey(pk);
@@ -531,6 +561,9 @@ static void polynomial_encoders() {
         parms.setCoeffModulus(CoeffModulus::Create(4096, {36, 36, 37}));
         SEALContext context(parms);
         const uint64_t t = parms.plainModulus().value();
+        EXPECT(context.firstContextData()->qualifiers().using_batching &&
+                   context.firstContextData()->plainUpperHalfThreshold() == (t + 1) / 2,
+               "BFV qualifiers: batching with a prime t = 1 mod 2N");
         BatchEncoder encoder(context);
         Plaintext p;
         encoder.encodePolynomial(std::vector<uint64_t>{1, 2, t + 4463}, p);
