@@ -422,6 +422,45 @@ struct CoeffModulus {
     static std::vector<Modulus> Create(size_t n, const std::vector<int> &bit_sizes) {
         return create(n, bit_sizes, 0);
     }
+    // CoeffModulus::BFVDefault (src/modulus.cpp:53-78): the HE-standard default primes for
+    // N = 1024 ... 32768 at 128-, 192- and 256-bit security -- a published table, kept
+    // word for word (src/utils/globals.cpp:10-200), one row per degree.
+    static std::vector<Modulus> BFVDefault(size_t n, SecurityLevel sec = SecurityLevel::tc128) {
+        if (!MaxBitCount(n, sec)) throw std::invalid_argument("non-standard poly_modulus_degree");
+        if (sec == SecurityLevel::none) throw std::invalid_argument("invalid security level");
+        static const std::vector<uint64_t> t128[6] = {
+            {0x7e00001},
+            {0x3fffffff000001},
+            {0xffffee001, 0xffffc4001, 0x1ffffe0001},
+            {0x7fffffd8001, 0x7fffffc8001, 0xfffffffc001, 0xffffff6c001, 0xfffffebc001},
+            {0xfffffffd8001, 0xfffffffa0001, 0xfffffff00001, 0x1fffffff68001, 0x1fffffff50001, 0x1ffffffee8001,
+             0x1ffffffea0001, 0x1ffffffe88001, 0x1ffffffe48001},
+            {0x7fffffffe90001, 0x7fffffffbf0001, 0x7fffffffbd0001, 0x7fffffffba0001, 0x7fffffffaa0001,
+             0x7fffffffa50001, 0x7fffffff9f0001, 0x7fffffff7e0001, 0x7fffffff770001, 0x7fffffff380001,
+             0x7fffffff330001, 0x7fffffff2d0001, 0x7fffffff170001, 0x7fffffff150001, 0x7ffffffef00001,
+             0xfffffffff70001}};
+        static const std::vector<uint64_t> t192[6] = {
+            {0x7f001},
+            {0x1ffffc0001},
+            {0x1ffc001, 0x1fce001, 0x1fc0001},
+            {0x3ffffac001, 0x3ffff54001, 0x3ffff48001, 0x3ffff28001},
+            {0x3ffffffdf0001, 0x3ffffffd48001, 0x3ffffffd20001, 0x3ffffffd18001, 0x3ffffffcd0001, 0x3ffffffc70001},
+            {0x3fffffffd60001, 0x3fffffffca0001, 0x3fffffff6d0001, 0x3fffffff5d0001, 0x3fffffff550001,
+             0x7fffffffe90001, 0x7fffffffbf0001, 0x7fffffffbd0001, 0x7fffffffba0001, 0x7fffffffaa0001,
+             0x7fffffffa50001}};
+        static const std::vector<uint64_t> t256[6] = {
+            {0x3001},
+            {0x1ffc0001},
+            {0x3ffffffff040001},
+            {0x7ffffec001, 0x7ffffb0001, 0xfffffdc001},
+            {0x7ffffffc8001, 0x7ffffff00001, 0x7fffffe70001, 0xfffffffd8001, 0xfffffffa0001},
+            {0xffffffff00001, 0x1fffffffe30001, 0x1fffffffd80001, 0x1fffffffd10001, 0x1fffffffc50001,
+             0x1fffffffbf0001, 0x1fffffffb90001, 0x1fffffffb60001, 0x1fffffffa50001}};
+        int row = 0;
+        while ((size_t(1024) << row) != n) row++;
+        const std::vector<uint64_t> &v = sec == SecurityLevel::tc128 ? t128[row] : sec == SecurityLevel::tc192 ? t192[row] : t256[row];
+        return std::vector<Modulus>(v.begin(), v.end());
+    }
     // Primes that are also 1 mod t (CoeffModulus::Create(n, plain_modulus, bit_sizes)).
     static std::vector<Modulus> Create(size_t n, const Modulus &plain_modulus, const std::vector<int> &bit_sizes) {
         return create(n, bit_sizes, plain_modulus.value());

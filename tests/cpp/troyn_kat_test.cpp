@@ -653,6 +653,69 @@ static void polynomial_encoders() {
     }
 }
 
+// CoeffModulus::BFVDefault (src/modulus.cpp:53-78): every row of the three tables holds
+// distinct primes = 1 mod 2N whose bits add up to at most the security bound (to it, but
+// for two 192-bit rows), and N = 8192 at
+// 128 bits is the BFV plain-path workload's list; that workload (N = 8192, t = 65537,
+// two batched plaintexts, multiply, relinearize, decrypt) runs through the troyn:: API.
+static void bfv_default_moduli() {
+    bool ok = true;
+    for (SecurityLevel sec : {SecurityLevel::tc128, SecurityLevel::tc192, SecurityLevel::tc256})
+        for (size_t n = 1024; n <= 32768; n *= 2) {
+            const std::vector<Modulus> q = CoeffModulus::BFVDefault(n, sec);
+            int bits = 0;
+            for (size_t i = 0; i < q.size(); i++) {
+                bits += q[i].bitCount();
+                ok &= troyhip::detail::is_prime64(q[i].value()) && q[i].value() % (2 * n) == 1;
+                for (size_t j = 0; j < i; j++) ok &= q[i] != q[j];
+            }
+            ok &= bits <= CoeffModulus::MaxBitCount(n, sec) && bits + 12 > CoeffModulus::MaxBitCount(n, sec);
+        }
+    EXPECT(ok, "BFVDefault: primes = 1 mod 2N within the security bound, for every degree and level");
+    const std::vector<Modulus> d = CoeffModulus::BFVDefault(8192);
+    const std::vector<uint64_t> want{0x7fffffd8001, 0x7fffffc8001, 0xfffffffc001, 0xffffff6c001, 0xfffffebc001};
+    bool same = d.size() == want.size();
+    for (size_t i = 0; same && i < d.size(); i++) same = d[i].value() == want[i];
+    EXPECT(same, "BFVDefault(8192) is the BFV plain-path workload's list");
+    EXPECT(throws<std::invalid_argument>([] { (void)CoeffModulus::BFVDefault(65536); }) &&
+               throws<std::invalid_argument>([] { (void)CoeffModulus::BFVDefault(8192, SecurityLevel::none); }),
+           "BFVDefault rejects other degrees and no security level");
+
+    EncryptionParameters parms(SchemeType::bfv);
+    parms.setPolyModulusDegree(8192);
+    parms.setCoeffModulus(CoeffModulus::BFVDefault(8192));
+    parms.setPlainModulus(65537);
+    SEALContext context(parms);
+    KeyGenerator keygen(context);
+    Encryptor encryptor(context, keygen.createPublicKey());
+    Decryptor decryptor(context, keygen.secretKey());
+    Evaluator evaluator(context);
+    BatchEncoder encoder(context);
+    RelinKeys rk = keygen.createRelinKeys();
+    std::vector<uint64_t> x(8192), y(8192);
+    uint64_t st = 1;
+    auto next = [&st] {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (st >> 33) % 65537;
+    };
+    for (size_t i = 0; i < 8192; i++) {
+        x[i] = next();
+        y[i] = next();
+    }
+    Plaintext px, py, out;
+    encoder.encode(x, px);
+    encoder.encode(y, py);
+    Ciphertext cx = encryptor.encrypt(px), cy = encryptor.encrypt(py);
+    evaluator.multiplyInplace(cx, cy);
+    evaluator.relinearizeInplace(cx, rk);
+    decryptor.decrypt(cx, out);
+    std::vector<uint64_t> z;
+    encoder.decode(out, z);
+    bool exact = z.size() == 8192;
+    for (size_t i = 0; exact && i < 8192; i++) exact = z[i] == x[i] * y[i] % 65537;
+    EXPECT(exact, "BFV plain path at N = 8192, BFVDefault, t = 65537: slots of the product exact");
+}
+
 int main() {
     KernelProvider::initialize();
     bfv_relinearize();
@@ -661,6 +724,7 @@ int main() {
     key_files();
     bfv_many_and_exponentiate();
     polynomial_encoders();
+    bfv_default_moduli();
     std::printf("troyn kat ok\n");
     return 0;
 }
