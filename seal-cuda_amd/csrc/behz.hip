@@ -3,12 +3,14 @@
 // Same arithmetic as Evaluator::bfvMultiply (src/evaluator.cpp:385-624) with the
 // RNSTool steps (src/utils/rns.cpp:849-960); GPU twins src/evaluator_cuda.cu:283-382,
 // src/utils/rns_cuda.cu:300-443.  Per coefficient, each base conversion is a short
-// dot product over the input base: the thread stages the scaled inputs in LDS and
-// accumulates the output primes four or five at a time (each staged word read once per group)
-// with three-word product sums (accum.h; inputs < 2^61, at most 64 terms, so every sum
-// stays below 2^128), so intermediate residues never leave the kernel.  Scalar
-// multiplies that precede a conversion in the reference (x m_tilde, x t) are folded into
-// the conversion's first constant.
+// dot product over the input base.  A block takes 64 coefficients, one per lane, and its
+// two waves split the work: each stages half of the scaled inputs in LDS, and then
+// computes its own half of the output primes, OB at a time (each staged word read once
+// per group), with three-word product sums (accum.h; inputs < 2^61, at most 64 terms,
+// so every sum stays below 2^128), so intermediate residues never leave the kernel.
+// Sharing the staged words between two waves halves the LDS per wave, which is what
+// bounds the waves a CU holds.  Scalar multiplies that precede a conversion in the
+// reference (x m_tilde, x t) are folded into the conversion's first constant.
 #include "behz.h"
 #include "accum.h"
 #include "context.h"
@@ -16,38 +18,57 @@
 #include "ntt.h"
 #include "rns_ops.h"
 
+#include <type_traits>
+
 namespace shc {
 namespace {
 
-constexpr int BT = 128;   // threads per block of the conversion kernels
-constexpr int MAXB = 64;  // LDS rows per thread (>= |q| and >= |B|)
+constexpr int CW = 64;       // coefficients per block of the conversion kernels, one per lane
+constexpr int WV = 2;        // waves per block: each takes a slice of the output primes
+constexpr int BT = CW * WV;  // threads per block
+constexpr int MAXB = 64;     // LDS rows per block (>= |q| and >= |B|)
 
-inline uint32_t grid_for(size_t n) {
-    const size_t b = (n + BT - 1) / BT;
+inline uint32_t grid_for(size_t coefficients) {
+    const size_t b = (coefficients + CW - 1) / CW;
     return (uint32_t)(b < (size_t(1) << 20) ? b : (size_t(1) << 20));
 }
 
-// Output primes per conversion pass: 5 when that leaves fewer unused sums than 4.
+// Output primes per pass for a wave's slice of `outputs`: the fewest sums computed, then
+// the fewest passes over the staged inputs.
 inline int out_group(uint32_t outputs) {
-    const uint32_t w4 = (outputs + 3) / 4 * 4 - outputs, w5 = (outputs + 4) / 5 * 5 - outputs;
-    return w5 < w4 ? 5 : 4;
+    const uint32_t S = (outputs + WV - 1) / WV;
+    int best = 4;
+    uint32_t best_slots = ~0u, best_passes = ~0u;
+    for (int ob : {7, 6, 5, 4}) {
+        const uint32_t passes = (S + ob - 1) / ob, slots = passes * ob;
+        if (slots < best_slots || (slots == best_slots && passes < best_passes)) {
+            best = ob;
+            best_slots = slots;
+            best_passes = passes;
+        }
+    }
+    return best;
+}
+
+// Wave w's outputs [ob, oe) out of `outputs` (contiguous slices of ceil(outputs / WV)).
+__device__ __forceinline__ void slice_of(uint32_t outputs, int w, uint32_t &ob, uint32_t &oe) {
+    const uint32_t S = (outputs + WV - 1) / WV;
+    ob = min<uint32_t>(outputs, (uint32_t)w * S);
+    oe = min<uint32_t>(outputs, ob + S);
 }
 
 __device__ __forceinline__ uint64_t shoup_pair(uint64_t x, const uint64_t *pair, uint64_t q) {
     return shoup(x, pair[0], pair[1], q);
 }
 
-// OB: output primes per pass over the staged inputs, 4 or 5, whichever wastes fewer
-// sums for the conversion's output count (25 Bsk primes at 24 data primes: 5 x 5)
-
-// acc[u] = sum_j v[j * BT + tid] * rows[(o0 + u) * terms + j] for u < OB (rows past
+// acc[u] = sum_j v[j * CW + lane] * rows[(o0 + u) * terms + j] for u < OB (rows past
 // `outputs` repeat the last one; the caller does not use those sums).  The rows are
-// read-only tables with block-uniform addresses: read through the constant address
+// read-only tables with wave-uniform addresses: read through the constant address
 // space, so that they become scalar loads.
 typedef const __attribute__((address_space(4))) uint64_t const_u64;
 template <int OB>
 __device__ __forceinline__ void dot_rows(Acc3 (&acc)[OB], const uint64_t *v, const uint64_t *__restrict__ rows,
-                                         uint32_t terms, uint32_t o0, uint32_t outputs, int tid) {
+                                         uint32_t terms, uint32_t o0, uint32_t outputs, int lane) {
     const_u64 *r[OB];
 #pragma unroll
     for (int u = 0; u < OB; u++) {
@@ -60,14 +81,14 @@ __device__ __forceinline__ void dot_rows(Acc3 (&acc)[OB], const uint64_t *v, con
     for (; j + 4 <= terms; j += 4) {
         uint64_t y[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) y[k] = v[(j + k) * BT + tid];
+        for (int k = 0; k < 4; k++) y[k] = v[(j + k) * CW + lane];
 #pragma unroll
         for (int k = 0; k < 4; k++)
 #pragma unroll
             for (int u = 0; u < OB; u++) mac<true>(acc[u], y[k], r[u][j + k]);
     }
     for (; j < terms; j++) {
-        const uint64_t y = v[j * BT + tid];
+        const uint64_t y = v[j * CW + lane];
 #pragma unroll
         for (int u = 0; u < OB; u++) mac<true>(acc[u], y, r[u][j]);
     }
@@ -78,35 +99,42 @@ template <int OB>
 __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restrict__ x, uint64_t *__restrict__ out,
                                                          BehzDev B, const DevPrime *__restrict__ qp, int logn,
                                                          size_t total) {
-    extern __shared__ uint64_t ys[];  // [rows][BT], rows = max(|q|, |B|)
+    extern __shared__ uint64_t ys[];  // [L][CW]
     const size_t n = (size_t)1 << logn;
-    const int tid = threadIdx.x;
-    for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
+    // the wave index, uniform across the wave (so that the slice and its table rows are scalar)
+    const int lane = threadIdx.x % CW, w = __builtin_amdgcn_readfirstlane(threadIdx.x / CW);
+    uint32_t ob, oe;
+    slice_of(B.bsk, w, ob, oe);
+    const uint64_t *row_mt = B.qhat_mod_bskm + (size_t)B.bsk * B.L;
+    for (size_t base = (size_t)blockIdx.x * CW; base < total; base += (size_t)gridDim.x * CW) {
+        const size_t idx = base + lane;
+        const bool live = idx < total;
         const size_t p = idx >> logn, c = idx & (n - 1);
-        for (uint32_t j = 0; j < B.L; j++)
-            ys[j * BT + tid] = shoup_pair(x[(p * B.L + j) * n + c], B.mt_qhat_inv + 2 * j, qp[j].q);
-        // conversion into m_tilde = 2^32: sums wrap modulo 2^64, a multiple of 2^32
+        for (uint32_t j = w; j < B.L; j += WV)
+            ys[j * CW + lane] = live ? shoup_pair(x[(p * B.L + j) * n + c], B.mt_qhat_inv + 2 * j, qp[j].q) : 0;
+        __syncthreads();
+        // conversion into m_tilde = 2^32 (each wave): sums wrap modulo 2^64, a multiple of 2^32
         uint64_t s_mt = 0;
-        const uint64_t *row_mt = B.qhat_mod_bskm + (size_t)B.bsk * B.L;
-        for (uint32_t j = 0; j < B.L; j++) s_mt += ys[j * BT + tid] * row_mt[j];
+        for (uint32_t j = 0; j < B.L; j++) s_mt += ys[j * CW + lane] * row_mt[j];
         const uint64_t r = ((s_mt & 0xffffffffull) * B.m_tilde_neg_inv_q) & 0xffffffffull;
-        for (uint32_t i0 = 0; i0 < B.bsk; i0 += OB) {
+        for (uint32_t i0 = ob; i0 < oe; i0 += OB) {
             Acc3 acc[OB];
-            dot_rows(acc, ys, B.qhat_mod_bskm, B.L, i0, B.bsk, tid);
+            dot_rows(acc, ys, B.qhat_mod_bskm, B.L, i0, B.bsk, lane);
 #pragma unroll
             for (int u = 0; u < OB; u++) {
                 const uint32_t i = i0 + u;
-                if (i >= B.bsk) break;
+                if (i >= oe) break;
                 const DevPrime &P = B.bsk_primes[i];
                 const uint64_t s = acc_reduce(acc[u], P);
                 // centred r (m_tilde is a power of two, so r >= m_tilde/2 is negative)
                 const uint64_t rc = r >= 0x80000000ull ? r + P.q - 0x100000000ull : r;
                 u128 v = mul64x64(rc, B.q_mod_bsk[i]);
                 add128(v, u128{s, 0});
-                const uint64_t w = barrett128(v, P.q, P.ratio_lo, P.ratio_hi);
-                out[(p * B.bsk + i) * n + c] = shoup_pair(w, B.inv_mt_bsk + 2 * i, P.q);
+                const uint64_t wv = barrett128(v, P.q, P.ratio_lo, P.ratio_hi);
+                if (live) out[(p * B.bsk + i) * n + c] = shoup_pair(wv, B.inv_mt_bsk + 2 * i, P.q);
             }
         }
+        __syncthreads();  // the staged words are read before the next coefficients' overwrite them
     }
 }
 
@@ -115,26 +143,34 @@ template <int OB>
 __global__ __launch_bounds__(BT) void behz_floor_kernel(const uint64_t *__restrict__ Tq, const uint64_t *__restrict__ Tb,
                                                         uint64_t *__restrict__ Z, BehzDev B,
                                                         const DevPrime *__restrict__ qp, int logn, size_t total) {
-    extern __shared__ uint64_t ws[];  // [rows][BT], rows = max(|q|, |B|)
+    extern __shared__ uint64_t ws[];  // [L][CW]
     const size_t n = (size_t)1 << logn;
-    const int tid = threadIdx.x;
-    for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
+    // the wave index, uniform across the wave (so that the slice and its table rows are scalar)
+    const int lane = threadIdx.x % CW, w = __builtin_amdgcn_readfirstlane(threadIdx.x / CW);
+    uint32_t ob, oe;
+    slice_of(B.bsk, w, ob, oe);
+    for (size_t base = (size_t)blockIdx.x * CW; base < total; base += (size_t)gridDim.x * CW) {
+        const size_t idx = base + lane;
+        const bool live = idx < total;
         const size_t p = idx >> logn, c = idx & (n - 1);
-        for (uint32_t j = 0; j < B.L; j++)
-            ws[j * BT + tid] = shoup_pair(Tq[(p * B.L + j) * n + c], B.t_qhat_inv + 2 * j, qp[j].q);
-        for (uint32_t i0 = 0; i0 < B.bsk; i0 += OB) {
+        for (uint32_t j = w; j < B.L; j += WV)
+            ws[j * CW + lane] = live ? shoup_pair(Tq[(p * B.L + j) * n + c], B.t_qhat_inv + 2 * j, qp[j].q) : 0;
+        __syncthreads();
+        for (uint32_t i0 = ob; i0 < oe; i0 += OB) {
             Acc3 acc[OB];
-            dot_rows(acc, ws, B.qhat_mod_bskm, B.L, i0, B.bsk, tid);
+            dot_rows(acc, ws, B.qhat_mod_bskm, B.L, i0, B.bsk, lane);
 #pragma unroll
             for (int u = 0; u < OB; u++) {
                 const uint32_t i = i0 + u;
-                if (i >= B.bsk) break;
+                if (i >= oe) break;
+                if (!live) continue;
                 const DevPrime &P = B.bsk_primes[i];
                 const uint64_t conv = acc_reduce(acc[u], P);
                 const uint64_t tb = shoup_pair(Tb[(p * B.bsk + i) * n + c], B.t_bsk + 2 * i, P.q);
                 Z[(p * B.bsk + i) * n + c] = shoup_pair(tb + P.q - conv, B.inv_q_bsk + 2 * i, P.q);
             }
         }
+        __syncthreads();
     }
 }
 
@@ -143,28 +179,37 @@ template <int OB>
 __global__ __launch_bounds__(BT) void behz_sk_kernel(const uint64_t *__restrict__ Z, uint64_t *__restrict__ out,
                                                      BehzDev B, const DevPrime *__restrict__ qp, int logn,
                                                      size_t total) {
-    extern __shared__ uint64_t zs[];  // [rows][BT], rows = max(|q|, |B|)
+    extern __shared__ uint64_t zs[];  // [bsz][CW]
     const size_t n = (size_t)1 << logn;
-    const int tid = threadIdx.x;
+    // the wave index, uniform across the wave (so that the slice and its table rows are scalar)
+    const int lane = threadIdx.x % CW, w = __builtin_amdgcn_readfirstlane(threadIdx.x / CW);
+    uint32_t ob, oe;
+    slice_of(B.L, w, ob, oe);
     const DevPrime &Msk = B.bsk_primes[B.bsz];
-    for (size_t idx = (size_t)blockIdx.x * BT + tid; idx < total; idx += (size_t)gridDim.x * BT) {
+    const_u64 *bm = (const_u64 *)B.bhat_mod_msk;
+    for (size_t base = (size_t)blockIdx.x * CW; base < total; base += (size_t)gridDim.x * CW) {
+        const size_t idx = base + lane;
+        const bool live = idx < total;
         const size_t p = idx >> logn, c = idx & (n - 1);
-        for (uint32_t j = 0; j < B.bsz; j++)
-            zs[j * BT + tid] = shoup_pair(Z[(p * B.bsk + j) * n + c], B.bhat_inv + 2 * j, B.bsk_primes[j].q);
+        for (uint32_t j = w; j < B.bsz; j += WV)
+            zs[j * CW + lane] =
+                live ? shoup_pair(Z[(p * B.bsk + j) * n + c], B.bhat_inv + 2 * j, B.bsk_primes[j].q) : 0;
+        __syncthreads();
+        // alpha_sk (each wave)
         Acc3 acc{0, 0, 0};
-        const_u64 *bm = (const_u64 *)B.bhat_mod_msk;
-        for (uint32_t j = 0; j < B.bsz; j++) mac<true>(acc, zs[j * BT + tid], bm[j]);
+        for (uint32_t j = 0; j < B.bsz; j++) mac<true>(acc, zs[j * CW + lane], bm[j]);
         const uint64_t conv = acc_reduce(acc, Msk);
-        const uint64_t zmsk = Z[(p * B.bsk + B.bsz) * n + c];
+        const uint64_t zmsk = live ? Z[(p * B.bsk + B.bsz) * n + c] : 0;
         const uint64_t alpha = shoup(conv + B.m_sk - zmsk, B.inv_prod_B_msk, B.inv_prod_B_msk_shoup, B.m_sk);
         const bool neg = alpha > B.m_sk_half;
-        for (uint32_t i0 = 0; i0 < B.L; i0 += OB) {
+        for (uint32_t i0 = ob; i0 < oe; i0 += OB) {
             Acc3 s[OB];
-            dot_rows(s, zs, B.bhat_mod_q, B.bsz, i0, B.L, tid);
+            dot_rows(s, zs, B.bhat_mod_q, B.bsz, i0, B.L, lane);
 #pragma unroll
             for (int u = 0; u < OB; u++) {
                 const uint32_t i = i0 + u;
-                if (i >= B.L) break;
+                if (i >= oe) break;
+                if (!live) continue;
                 const DevPrime &Q = qp[i];
                 const uint64_t o = acc_reduce(s[u], Q);
                 // Shenoy-Kumaresan correction with the centred alpha_sk
@@ -174,6 +219,18 @@ __global__ __launch_bounds__(BT) void behz_sk_kernel(const uint64_t *__restrict_
                 out[(p * B.L + i) * n + c] = barrett128(v, Q.q, Q.ratio_lo, Q.ratio_hi);
             }
         }
+        __syncthreads();
+    }
+}
+
+// Runs f(std::integral_constant<int, OB>) for the OB out_group picks.
+template <class F>
+void with_group(int ob, F &&f) {
+    switch (ob) {
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    default: f(std::integral_constant<int, 4>{}); break;
     }
 }
 
@@ -360,14 +417,11 @@ void bfv_multiply(const Context &c, const uint64_t *a, const uint64_t *b, uint64
     const DevPrime *qp = c.primes();
     const DevPrime *bp = B.bsk_primes;
     const size_t ext = (size_t)count * 2 * n;
-    const size_t lds = (size_t)(L > B.bsz ? L : B.bsz) * BT * 8;
-    if (out_group(B.bsk) == 5) {
-        behz_extend_kernel<5><<<grid_for(ext), BT, lds, st>>>(a, Ab, B, qp, logn, ext);
-        behz_extend_kernel<5><<<grid_for(ext), BT, lds, st>>>(b, Bb, B, qp, logn, ext);
-    } else {
-        behz_extend_kernel<4><<<grid_for(ext), BT, lds, st>>>(a, Ab, B, qp, logn, ext);
-        behz_extend_kernel<4><<<grid_for(ext), BT, lds, st>>>(b, Bb, B, qp, logn, ext);
-    }
+    const size_t lds = (size_t)(L > B.bsz ? L : B.bsz) * CW * 8;
+    with_group(out_group(B.bsk), [&](auto ob) {
+        behz_extend_kernel<decltype(ob)::value><<<grid_for(ext), BT, lds, st>>>(a, Ab, B, qp, logn, ext);
+        behz_extend_kernel<decltype(ob)::value><<<grid_for(ext), BT, lds, st>>>(b, Bb, B, qp, logn, ext);
+    });
     launch_check("behz_extend");
     // the four forward transforms in one launch per pass, a and b read in place (their
     // transforms land in Aq, Bq); then the two tensor products and the two inverse
@@ -397,15 +451,13 @@ void bfv_multiply(const Context &c, const uint64_t *a, const uint64_t *b, uint64
         ntt_inverse(Tb, count * 3 * bsk, bsk, bp, logn, NTT_FULL, st);
     }
     const size_t fl = (size_t)count * 3 * n;
-    if (out_group(B.bsk) == 5)
-        behz_floor_kernel<5><<<grid_for(fl), BT, lds, st>>>(Tq, Tb, Z, B, qp, logn, fl);
-    else
-        behz_floor_kernel<4><<<grid_for(fl), BT, lds, st>>>(Tq, Tb, Z, B, qp, logn, fl);
+    with_group(out_group(B.bsk), [&](auto ob) {
+        behz_floor_kernel<decltype(ob)::value><<<grid_for(fl), BT, lds, st>>>(Tq, Tb, Z, B, qp, logn, fl);
+    });
     launch_check("behz_floor");
-    if (out_group(B.L) == 5)
-        behz_sk_kernel<5><<<grid_for(fl), BT, lds, st>>>(Z, out, B, qp, logn, fl);
-    else
-        behz_sk_kernel<4><<<grid_for(fl), BT, lds, st>>>(Z, out, B, qp, logn, fl);
+    with_group(out_group(B.L), [&](auto ob) {
+        behz_sk_kernel<decltype(ob)::value><<<grid_for(fl), BT, lds, st>>>(Z, out, B, qp, logn, fl);
+    });
     launch_check("behz_sk");
 }
 
