@@ -28,7 +28,7 @@ struct alignas(64) DevPrime {
 struct alignas(32) DivRoundConst {
     uint64_t inv, inv_shoup;   // p^-1 mod q_i as a Shoup operand
     uint64_t half_mod;         // floor(p/2) mod q_i
-    uint64_t pad;
+    uint64_t p_mod;            // p mod q_i
 };
 
 // Launch shape of a key switch's two ModUp kernels (0 = the built-in choice):

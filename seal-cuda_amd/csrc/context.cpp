@@ -87,6 +87,7 @@ std::unique_ptr<Context> make_context(Scheme scheme, int logn, const std::vector
         if (!d.inv) fail(ERR_LOGIC, "invalid rns bases");
         d.inv_shoup = hm::shoup(d.inv, qi);
         d.half_mod = (p >> 1) % qi;
+        d.p_mod = p % qi;
         return d;
     };
     std::vector<DivRoundConst> ksc(K), rsc((K + 1) * K);

@@ -57,14 +57,25 @@ bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint3
 // i < L and S[ct][k] for the special target (ki = key_count - 1), the own-prime terms
 // read from diag + ct * diag_stride + i * N.  key: [L.. ][2][key_count][N] rows of the
 // key-switching key.  With a probe, its events bracket the fused launch.
+// With fold.ct (CKKS: the NTT-form ciphertext the switch adds into, [count] at fold.ct_stride
+// words, polynomials k = 0, 1 of L limbs each), P[ct][k][i] also takes ct_k[i] * (p mod q_i)
+// (fold.dc: the special prime's divide-and-round constants), so that the ModDown writes
+// (P - u) p^-1 without reading the ciphertext (ModDownArgs::folded).
+struct KsFold {
+    const uint64_t *ct;
+    size_t ct_stride;
+    const DivRoundConst *dc;
+};
 bool ntt_modup_keyprod(const uint64_t *d, size_t d_ct_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
                        uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t count, uint32_t L,
                        uint32_t key_count, const DevPrime *targets, const DevPrime *digits, int logn,
-                       hipStream_t stream, const KernelProbe *probe = nullptr, KsShape shape = KsShape{});
+                       hipStream_t stream, const KernelProbe *probe = nullptr, KsShape shape = KsShape{},
+                       KsFold fold = KsFold{});
 
 // ModDown by the special prime p fused into the last transforms of a key switch, for
 // `count` ciphertexts at L limbs (N >= 2^13; returns false and does nothing otherwise):
 //   u_i = ([s + floor(p/2)]_p mod q_i) - (floor(p/2) mod q_i),  ct_k[i] += (P_k[i] - u_i) p^-1
+// (folded: P already holds p ct_k[i], and ct_k[i] = (P_k[i] - u_i) p^-1 is written unread)
 // with s = S[ct][k] (special limb, coefficient form) and P = [ct][k][L][N].
 //   CKKS: buf = U scratch [count][2][L][N]; P in NTT form (u is transformed on the way).
 //   BFV:  buf = P itself (coefficient form after the in-place inverse transform).
@@ -76,6 +87,7 @@ struct ModDownArgs {
     const DivRoundConst *dc;
     uint64_t p;
     uint32_t L;
+    bool folded;
 };
 bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &args, uint32_t count, const DevPrime *primes, int logn,
                  hipStream_t stream);
@@ -106,6 +118,7 @@ struct KsGroup {
     size_t diag_stride;
     const DevPrime *targets;
     uint32_t count, L;
+    KsFold fold;
 };
 // The fused ModUp + key products of every group (N = 2^15, 2^16; false otherwise): one
 // columns-pass launch and one rows-pass launch per ciphertext grouping (4, 2, 1).

@@ -287,7 +287,7 @@ __device__ __forceinline__ void rows_keyprod_body(
     uint32_t bid, const uint64_t *__restrict__ T, const uint64_t *__restrict__ key, uint64_t *__restrict__ Pout,
     uint64_t *__restrict__ Sout, const DevPrime *__restrict__ primes, const uint64_t *__restrict__ diag,
     size_t diag_stride, uint32_t count, uint32_t L, uint32_t K_key, uint32_t ct_base, uint32_t groups,
-    uint32_t tiles) {
+    uint32_t tiles, const KsFold &fold) {
     constexpr int TT = 1 << (K - E);  // threads per row
     constexpr int SZ = 1 << K;        // row length
     static_assert(SZ == SUBS * TT, "one coefficient per thread in the product phase");
@@ -395,6 +395,18 @@ __device__ __forceinline__ void rows_keyprod_body(
             mac(a1[c], x, k1);
         }
     }
+    // the ciphertext the switch adds into, times p (CKKS, i < L): one more term per sum,
+    // and the ModDown reads no ciphertext
+    if (fold.ct && i < L) {
+        const uint64_t pm = fold.dc[i].p_mod;
+#pragma unroll
+        for (int c = 0; c < NS; c++) {
+            const uint64_t *cw = fold.ct + (size_t)(ct0 + min<uint32_t>(c % CG, gv - 1)) * fold.ct_stride +
+                                 ((size_t)i << LOGN) + row_off(c / CG) + tid;
+            mac<true>(a0[c], cw[0], pm);
+            mac<true>(a1[c], cw[(size_t)L << LOGN], pm);
+        }
+    }
 #pragma unroll
     for (int c = 0; c < NS; c++) {
         if ((uint32_t)(c % CG) >= gv) continue;
@@ -416,9 +428,9 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_kernel(
     const uint64_t *__restrict__ T, const uint64_t *__restrict__ key, uint64_t *__restrict__ Pout,
     uint64_t *__restrict__ Sout, const DevPrime *__restrict__ primes, const uint64_t *__restrict__ diag,
     size_t diag_stride, uint32_t count, uint32_t L, uint32_t K_key, uint32_t ct_base, uint32_t groups,
-    uint32_t tiles) {
+    uint32_t tiles, KsFold fold) {
     rows_keyprod_body<LOGN, K, E, SUBS, CG, RW, R4>(blockIdx.x, T, key, Pout, Sout, primes, diag, diag_stride, count,
-                                                    L, K_key, ct_base, groups, tiles);
+                                                    L, K_key, ct_base, groups, tiles, fold);
 }
 
 // ---- grouped launches: ciphertexts at several levels in one launch per stage.  Group
@@ -432,6 +444,7 @@ struct KeyprodGroup {
     const uint64_t *diag;
     size_t diag_stride;
     uint32_t count, L, ct_base, groups, tiles;
+    KsFold fold;
 };
 struct KeyprodGroups {
     uint32_t n;
@@ -453,7 +466,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_keyprod_grouped_kern
     const KeyprodGroup &a = kg.g[gi];
     rows_keyprod_body<LOGN, K, E, SUBS, CG, RW, R4>(blockIdx.x - kg.start[gi], a.T, kg.key, a.P, a.S, a.primes,
                                                     a.diag, a.diag_stride, a.count, a.L, kg.K_key, a.ct_base,
-                                                    a.groups, a.tiles);
+                                                    a.groups, a.tiles, a.fold);
 }
 
 // Plain in-place passes over a LimbGrid buffer.
@@ -685,7 +698,8 @@ __device__ __forceinline__ void moddown_add(const ModDownArgs &a, uint32_t i, ui
                                             uint64_t u, const DevPrime &Q) {
     uint64_t *dst = a.ct + (size_t)(j >> 1) * a.ct_stride + ((size_t)((j & 1) * a.L + i) << LOGN) + off;
     const uint64_t v = shoup(submod(pv, u, Q.q), a.dc[i].inv, a.dc[i].inv_shoup, Q.q);
-    *dst = addmod(*dst, v, Q.q);
+    if (a.folded) *dst = v;  // uniform: the ciphertext's words are already in P
+    else *dst = addmod(*dst, v, Q.q);
 }
 
 template <int LOGN, int K, int E, int SUBS>
@@ -992,7 +1006,7 @@ namespace {
 template <int LOGN, bool R4>
 void launch_keyprod_groups(const uint64_t *T, const uint64_t *key, uint64_t *P, uint64_t *S, const uint64_t *diag,
                            size_t diag_stride, uint32_t cts, uint32_t L, uint32_t K_key, const DevPrime *targets,
-                           uint32_t tiles, uint64_t tile_slots, uint32_t group, hipStream_t st) {
+                           uint32_t tiles, uint64_t tile_slots, uint32_t group, const KsFold &fold, hipStream_t st) {
     constexpr int K2 = LOGN - LOGN / 2;
     constexpr int E = 4;
     constexpr int RS = 4096 >> K2;
@@ -1003,16 +1017,16 @@ void launch_keyprod_groups(const uint64_t *T, const uint64_t *key, uint64_t *P, 
     const uint32_t singles = cts - 4 * g4 - 2 * g2;
     if (g4)
         ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 4, 1, R4><<<(uint32_t)(tile_slots * g4), RS << (K2 - E), 0, st>>>(
-            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, 0, g4, tiles);
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, 0, g4, tiles, fold);
     if (g2)
         ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 2, 1, R4><<<(uint32_t)(tile_slots * g2), RS << (K2 - E), 0, st>>>(
-            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, g4 * 4, g2, tiles);
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, g4 * 4, g2, tiles, fold);
     if (singles) {
         // single ciphertexts: two rows per block
         const uint32_t tiles2 = tiles / 2;
         const uint32_t slots2 = (tiles2 + 7) / 8 * 8;
         ntt_rows_keyprod_kernel<LOGN, K2, E, RS, 1, 2, R4><<<slots2 * singles, RS << (K2 - E), 0, st>>>(
-            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, cts - singles, singles, tiles2);
+            T, key, P, S, targets, diag, diag_stride, cts, L, K_key, cts - singles, singles, tiles2, fold);
     }
 }
 
@@ -1020,7 +1034,7 @@ template <int LOGN>
 void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
                           uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t cts, uint32_t L,
                           uint32_t K_key, const DevPrime *targets, const DevPrime *digits, const KernelProbe *probe,
-                          hipStream_t st, KsShape shape) {
+                          hipStream_t st, KsShape shape, const KsFold &fold) {
     constexpr int K1 = LOGN / 2;
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
@@ -1041,10 +1055,10 @@ void launch_modup_keyprod(const uint64_t *d, size_t d_stride, uint64_t *T, const
     // up to 31 digits the transformed words (< 8q) enter the sums unreduced
     if (L > 31)
         launch_keyprod_groups<LOGN, true>(T, key, P, S, diag, diag_stride, cts, L, K_key, targets, tiles, tile_slots,
-                                          shape.rows_group, st);
+                                          shape.rows_group, fold, st);
     else
         launch_keyprod_groups<LOGN, false>(T, key, P, S, diag, diag_stride, cts, L, K_key, targets, tiles,
-                                           tile_slots, shape.rows_group, st);
+                                           tile_slots, shape.rows_group, fold, st);
     if (probe) SHC_HIP(hipEventRecord(probe->end, st));
 }
 
@@ -1155,19 +1169,20 @@ void launch_modup_keyprod_groups(const KsGroup *gs, uint32_t n, const uint64_t *
         const uint64_t tile_slots = (uint64_t)((tiles + 7) / 8) * 8;
         const uint32_t g4 = a.count / 4, g2 = (a.count - 4 * g4) / 2, singles = a.count - 4 * g4 - 2 * g2;
         if (g4) {
-            t4.g[s4.size()] = KeyprodGroup{a.T, a.P, a.S, a.targets, a.diag, a.diag_stride, a.count, a.L, 0, g4, tiles};
+            t4.g[s4.size()] =
+                KeyprodGroup{a.T, a.P, a.S, a.targets, a.diag, a.diag_stride, a.count, a.L, 0, g4, tiles, a.fold};
             s4.push_back(tile_slots * g4);
         }
         if (g2) {
             t2.g[s2.size()] =
-                KeyprodGroup{a.T, a.P, a.S, a.targets, a.diag, a.diag_stride, a.count, a.L, 4 * g4, g2, tiles};
+                KeyprodGroup{a.T, a.P, a.S, a.targets, a.diag, a.diag_stride, a.count, a.L, 4 * g4, g2, tiles, a.fold};
             s2.push_back(tile_slots * g2);
         }
         if (singles) {
             const uint32_t tiles2 = tiles / 2;
             const uint64_t slots2 = (tiles2 + 7) / 8 * 8;
             t1.g[s1.size()] = KeyprodGroup{a.T, a.P, a.S, a.targets, a.diag, a.diag_stride, a.count, a.L,
-                                           a.count - singles, singles, tiles2};
+                                           a.count - singles, singles, tiles2, a.fold};
             s1.push_back(slots2 * singles);
         }
     }
@@ -1234,16 +1249,16 @@ LimbGrid poly_grid(uint32_t count, uint32_t lpp) {
 bool ntt_modup_keyprod(const uint64_t *d, size_t d_ct_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
                        uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t count, uint32_t L,
                        uint32_t key_count, const DevPrime *targets, const DevPrime *digits, int logn,
-                       hipStream_t st, const KernelProbe *probe, KsShape shape) {
+                       hipStream_t st, const KernelProbe *probe, KsShape shape, KsFold fold) {
     if ((logn != 15 && logn != 16) || count == 0) return false;
     if ((uint64_t)count * L * (L + 1) > ((uint64_t(1) << 34) >> logn))
         fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
     if (logn == 15)
         launch_modup_keyprod<15>(d, d_ct_stride, T, key, P, S, diag, diag_stride, count, L, key_count, targets,
-                                 digits, probe, st, shape);
+                                 digits, probe, st, shape, fold);
     else
         launch_modup_keyprod<16>(d, d_ct_stride, T, key, P, S, diag, diag_stride, count, L, key_count, targets,
-                                 digits, probe, st, shape);
+                                 digits, probe, st, shape, fold);
     launch_check("ntt modup keyprod launch");
     return true;
 }

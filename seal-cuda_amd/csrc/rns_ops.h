@@ -38,7 +38,7 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
 //               S[ct][k] the special target's; with diag (after a skipped diagonal)
 //               the term j == i reads the NTT-form input limb diag + ct * diag_stride
 //               + i * N.
-//   moddown_into: ct_k[i] += ModDown(P_k, S_k)[i] (S is consumed; U is scratch
+//   moddown_into: ct_k[i] += ModDown(P_k, S_k)[i] (folded: P holds p ct_k, and ct_k = ModDown; S is consumed; U is scratch
 //               [count][2][L][N]; P in NTT form).
 // The regions of a key-switch workspace of `count` ciphertexts at level L.
 struct KsRegions {
@@ -51,7 +51,7 @@ void keyswitch_inner_product(const Context &c, const uint64_t *T, const uint64_t
                              uint32_t count, uint32_t L, hipStream_t st, const uint64_t *diag = nullptr,
                              size_t diag_stride = 0);
 void moddown_into(const Context &c, uint64_t *P, uint64_t *S, uint64_t *U, uint64_t *ct, size_t ct_stride,
-                  uint32_t count, uint32_t L, hipStream_t st);
+                  uint32_t count, uint32_t L, hipStream_t st, bool folded = false);
 
 // Divide-and-round by the last prime of level L, dropping it (CKKS rescale in NTT form,
 // BFV modulus switch in coefficient form); in: [count][polys][L][N], out: [count][polys][L-1][N].

@@ -412,14 +412,19 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
     }
     // 2-4. ModUp and the inner product with the key: fused past the columns pass where
     // the transform shape allows (N = 2^15, 2^16), else T goes through HBM once more
+    // (CKKS: the fused products also take p times the ciphertext the switch adds into, so
+    // that the ModDown writes it without reading it)
     const KsShape shape = c.ks_shape(L, count);
-    if (!ntt_modup_keyprod(digits, dstride, T, key, P, S, diag, target_stride, count, L, (uint32_t)c.key_count(),
-                           c.keyswitch_primes(L), c.primes(), logn, st, probe, shape)) {
+    const KsFold fold = ckks ? KsFold{ct, ct_stride, c.ks_consts()} : KsFold{};
+    const bool fused = ntt_modup_keyprod(digits, dstride, T, key, P, S, diag, target_stride, count, L,
+                                         (uint32_t)c.key_count(), c.keyswitch_primes(L), c.primes(), logn, st, probe,
+                                         shape, fold);
+    if (!fused) {
         const bool skipped = modup_to_T(c, digits, dstride, T, count, L, st, probe, diag != nullptr);
         keyswitch_inner_product(c, T, key, P, S, count, L, st, skipped ? diag : nullptr, target_stride);
     }
     // 5. ModDown by the special prime
-    moddown_into(c, P, S, U, ct, ct_stride, count, L, st);
+    moddown_into(c, P, S, U, ct, ct_stride, count, L, st, fused && fold.ct);
 }
 
 }  // namespace
@@ -459,14 +464,15 @@ void keyswitch_inner_product(const Context &c, const uint64_t *T, const uint64_t
 }
 
 void moddown_into(const Context &c, uint64_t *P, uint64_t *S, uint64_t *U, uint64_t *ct, size_t ct_stride,
-                  uint32_t count, uint32_t L, hipStream_t st) {
+                  uint32_t count, uint32_t L, hipStream_t st, bool folded) {
     const size_t K = c.key_count();
     const int logn = c.logn;
     const bool ckks = c.scheme == SCHEME_CKKS;
     ntt_inverse(S, count * 2, 1, c.keyswitch_primes(L) + L, logn, NTT_FULL, st);
     const size_t uwords = ((size_t)count * 2 * L) << logn;
-    const ModDownArgs md{S, P, ct, ct_stride, c.ks_consts(), c.moduli[K - 1], L};
+    const ModDownArgs md{S, P, ct, ct_stride, c.ks_consts(), c.moduli[K - 1], L, folded};
     if (ntt_moddown(ckks, ckks ? U : P, md, count, c.primes(), logn, st)) return;
+    if (folded) fail(ERR_LOGIC, "a folded ModDown needs the fused transforms");
     divround_prep_kernel<<<blocks_for(uwords), TPB, 0, st>>>(S, U, c.primes(), c.ks_consts(), c.moduli[K - 1], L, logn,
                                                             uwords);
     launch_check("moddown_prep");
@@ -613,13 +619,14 @@ void relinearize_groups(const Context &c, const RelinGroup *gs, uint32_t n, cons
             const size_t poly = (size_t)L * nn;
             uint64_t *target = r.ct + 2 * poly;
             dg[k] = NttGroup{at(lay.d), target, LimbGrid{1, L, L, r.count}, LimbGrid{1, 3 * L, L, r.count}, c.primes()};
+            // the products take p times (c0, c1), and the ModDown writes them unread
             kg[k] = KsGroup{at(lay.d), poly, at(lay.T), at(lay.P), at(lay.S), target, 3 * poly, c.keyswitch_primes(L),
-                            r.count, L};
+                            r.count, L, KsFold{r.ct, 3 * poly, c.ks_consts()}};
             sg[k] = NttGroup{at(lay.S), nullptr, LimbGrid{1, 1, 1, 2 * r.count}, LimbGrid{1, 1, 1, 2 * r.count},
                              c.keyswitch_primes(L) + L};
-            mg[k] = ModDownGroup{at(lay.U),
-                                 ModDownArgs{at(lay.S), at(lay.P), r.ct, 3 * poly, c.ks_consts(), c.moduli[K - 1], L},
-                                 r.count};
+            mg[k] = ModDownGroup{
+                at(lay.U), ModDownArgs{at(lay.S), at(lay.P), r.ct, 3 * poly, c.ks_consts(), c.moduli[K - 1], L, true},
+                r.count};
             k++;
         }
         if (!k) return;
