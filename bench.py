@@ -39,12 +39,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel at the default workload (batch 128),
 # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB device copy
 # (profiles/mulrelin_pmc_traffic.md); a profiler cannot run inside this process.
-KEYPROD_TRAFFIC_B128 = 40.93e9 + 3.36e9
+KEYPROD_TRAFFIC_B128 = 44.16e9 + 3.36e9
 # VALU wave instructions per launch of the same kernel (rocprofv3 --pmc SQ_INSTS_VALU, same
 # file) and the VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles
 # per SIMD at the 2.4 GHz peak engine clock.  The kernel is bound by VALU issue (exact
 # 64-bit modular arithmetic from 32-bit multiplies), so this is its figure of merit.
-KEYPROD_VALU_B128 = 8081571840
+KEYPROD_VALU_B128 = 8140554240
 VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 CKKS = 2
 T_BFV = 65537
@@ -324,14 +324,15 @@ def bench_mulrelin(args):
     # The dominant kernel: the ModUp rows pass fused with the products by the key.  It
     # reads every ModUp limb once (B x L x L: each data prime's own digit is the NTT-form
     # input limb, so L - 1 digits go to each of the L data primes and L to the special
-    # prime), those B x L input limbs, the key's 2 L (L+1) limbs and the L+1 twiddle
-    # tables once, and writes the 2 B (L+1) product limbs; it runs the last 8 butterfly
-    # layers of every ModUp limb and 2 B L (L+1) N multiply-adds.
+    # prime), those B x L input limbs, the 2 B L limbs of (c0, c1) (their p multiples join
+    # the sums, so that the ModDown writes them unread), the key's 2 L (L+1) limbs and the
+    # L+1 twiddle tables once, and writes the 2 B (L+1) product limbs; it runs the last 8
+    # butterfly layers of every ModUp limb and 2 B L (L+1) N multiply-adds.
     grp = ctx.keyswitch_shape(L, B)[1] or 4
     kname = {4: "4,1", 2: "2,1", 1: "1,2"}[grp]
     valu_known = B == 128 and grp == 4  # the SQ_INSTS_VALU count was measured for groups of 4
     tlimbs = B * L * L
-    rows_bytes = (tlimbs + B * L + 2 * L * (L + 1) + 2 * B * (L + 1)) * n * 8 + K * n * 16
+    rows_bytes = (tlimbs + B * L + 2 * B * L + 2 * L * (L + 1) + 2 * B * (L + 1)) * n * 8 + K * n * 16
     rows_bfly = tlimbs * (n // 2) * (logn - logn // 2)
     rows_macs = 2 * B * L * (L + 1) * n
     limb_bytes = n * 8
@@ -684,10 +685,11 @@ def bench_chain(args):
     # pair, the key's 2m(m+1) limbs once per key-switch call
     step_bytes = sum((c * (4 * (l - s) + 2 * (l - s - 1)) + 2 * (l - s) * (l - s + 1)) * limb
                      for l, c in groups.items() for s in range(3))
-    # the probed launch at level L = top for c pairs: L x L ModUp limbs and L input limbs
-    # per pair, the key's 2L(L+1) limbs, 2(L+1) product limbs per pair, L+1 twiddle tables
+    # the probed launch at level L = top for c pairs: L x L ModUp limbs, L input limbs and
+    # the 2L limbs of (c0, c1) per pair, the key's 2L(L+1) limbs, 2(L+1) product limbs per
+    # pair, L+1 twiddle tables
     c_top, L_top = groups[top], top
-    kp_bytes = ((c_top * (L_top * L_top + L_top + 2 * (L_top + 1)) + 2 * L_top * (L_top + 1)) * limb
+    kp_bytes = ((c_top * (L_top * L_top + 3 * L_top + 2 * (L_top + 1)) + 2 * L_top * (L_top + 1)) * limb
                 + (L_top + 1) * n * 16)
     out = {"metric": "CKKS mul+relin+rescale per second, N=2^15 chain of 3, uneven levels, HIP graph",
            "value": ops / (ms * 1e-3), "unit": "mul+relin+rescale/s", "n_gpus": 1, "steps": args.steps,
