@@ -72,6 +72,20 @@ int shc_ntt_forward(const shc_context *ctx, uint64_t *d_data, uint32_t poly_coun
 int shc_ntt_inverse(const shc_context *ctx, uint64_t *d_data, uint32_t poly_count, uint32_t limb_count,
                     void *stream);
 
+/* A transform job: poly_count polynomials of limb_count limbs at d_data, in ctx's ring. */
+typedef struct shc_ntt_job {
+    const shc_context *ctx;
+    uint64_t *d_data;
+    uint32_t poly_count, limb_count;
+} shc_ntt_job;
+/* shc_ntt_forward / shc_ntt_inverse over the jobs of several contexts -- ring degrees and
+ * limb counts mixed -- with one launch per pass for every job at N = 2^13 .. 2^17 (jobs at
+ * smaller N run their own launches); at most 16 jobs.  The reference transforms one ring
+ * per call (kNttNegacyclicHarvey, src/kernelutils.cuh:578-632); this is its batched form
+ * for ciphertexts of different degrees. */
+int shc_ntt_forward_multi(const shc_ntt_job *jobs, uint32_t n, void *stream);
+int shc_ntt_inverse_multi(const shc_ntt_job *jobs, uint32_t n, void *stream);
+
 /* ------------------------------------------------------------------ evaluator
  * Batched entry points: `count` ciphertexts packed back to back, all at the same
  * level with `limb_count` = L data primes.  Scratch memory is caller-owned; the

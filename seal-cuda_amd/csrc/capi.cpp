@@ -108,6 +108,43 @@ int shc_ntt_inverse(const shc_context *ctx, uint64_t *d, uint32_t polys, uint32_
     });
 }
 
+namespace {
+int ntt_multi(const shc_ntt_job *jobs, uint32_t n, void *stream, bool inverse) {
+    return guarded([&] {
+        if (n > shc::MAX_GROUPS) shc::fail(shc::ERR_INVALID_ARGUMENT, "at most 16 transform jobs per call");
+        if (n && !jobs) shc::fail(shc::ERR_INVALID_ARGUMENT, "null job list");
+        std::vector<shc::NttGroup> gs;
+        std::vector<int> logns;
+        for (uint32_t k = 0; k < n; k++) {
+            const auto &c = ctx_of(jobs[k].ctx);
+            const uint32_t limbs = jobs[k].limb_count, polys = jobs[k].poly_count;
+            check_limbs(c, limbs);
+            if (!polys) continue;
+            if (!jobs[k].d_data) shc::fail(shc::ERR_INVALID_ARGUMENT, "null transform data");
+            if (c.logn < 13) {  // one-tile transforms: a launch of their own
+                if (inverse)
+                    shc::ntt_inverse(jobs[k].d_data, polys * limbs, limbs, c.primes(), c.logn, shc::NTT_FULL,
+                                     (hipStream_t)stream);
+                else
+                    shc::ntt_forward(jobs[k].d_data, polys * limbs, limbs, c.primes(), c.logn, shc::NTT_FULL,
+                                     (hipStream_t)stream);
+                continue;
+            }
+            const shc::LimbGrid lg{1, limbs, limbs, polys};
+            gs.push_back(shc::NttGroup{jobs[k].d_data, nullptr, lg, lg, c.primes()});
+            logns.push_back(c.logn);
+        }
+        if (inverse)
+            shc::ntt_inverse_multi(gs.data(), logns.data(), (uint32_t)gs.size(), shc::NTT_FULL, (hipStream_t)stream);
+        else
+            shc::ntt_forward_multi(gs.data(), logns.data(), (uint32_t)gs.size(), shc::NTT_FULL, (hipStream_t)stream);
+    });
+}
+}  // namespace
+
+int shc_ntt_forward_multi(const shc_ntt_job *jobs, uint32_t n, void *stream) { return ntt_multi(jobs, n, stream, false); }
+int shc_ntt_inverse_multi(const shc_ntt_job *jobs, uint32_t n, void *stream) { return ntt_multi(jobs, n, stream, true); }
+
 int shc_ckks_multiply(const shc_context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out, uint32_t count,
                       uint32_t limbs, void *stream) {
     return guarded([&] {

@@ -108,6 +108,10 @@ struct NttGroup {
 };
 void ntt_forward_groups(const NttGroup *groups, uint32_t n, int logn, NttOut out, hipStream_t stream);
 void ntt_inverse_groups(const NttGroup *groups, uint32_t n, int logn, NttOut out, hipStream_t stream);
+// The same over groups of several ring degrees, group g at N = 2^logns[g] (13 .. 17): one
+// launch per pass for all of them.
+void ntt_forward_multi(const NttGroup *groups, const int *logns, uint32_t n, NttOut out, hipStream_t stream);
+void ntt_inverse_multi(const NttGroup *groups, const int *logns, uint32_t n, NttOut out, hipStream_t stream);
 
 // A key-switch group for ntt_modup_keyprod_groups (the arguments of ntt_modup_keyprod).
 struct KsGroup {

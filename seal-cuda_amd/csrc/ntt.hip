@@ -162,16 +162,16 @@ __host__ __device__ constexpr bool rows_mixed(uint32_t jn) {
     return LOGN > K && jn < (uint32_t)SUBS;
 }
 
+// (tile: SUBS << K words of LDS, the caller's)
 template <int LOGN, int K, int E, int SUBS, bool INV, bool W = false, bool MX = false, class Load, class Store>
-__device__ __forceinline__ void rows_body(uint32_t bid, const DevPrime *__restrict__ primes, uint32_t I, uint32_t j0,
-                                          uint32_t jn, const Load &load_elem, const Store &store_elem,
-                                          uint32_t jn_short = 0) {
+__device__ __forceinline__ void rows_body_t(uint64_t *tile, uint32_t bid, const DevPrime *__restrict__ primes,
+                                            uint32_t I, uint32_t j0, uint32_t jn, const Load &load_elem,
+                                            const Store &store_elem, uint32_t jn_short = 0) {
     constexpr int T = 1 << (K - E);
     constexpr int SZ = 1 << K;
     constexpr int RPL = LOGN - K;  // log2(rows per limb)
     constexpr int NP = (K + E - 1) / E;
     static_assert(!MX || (NP > 1 && RPL > 0), "pairs of (limb, row) need rows and a staged store");
-    __shared__ uint64_t tile[NP > 1 ? SUBS * SZ : 1];
     const int tid = threadIdx.x;
     const int sub = tid / T, t = tid % T;
     uint32_t i, jfirst = 0, q0 = 0;
@@ -239,17 +239,25 @@ __device__ __forceinline__ void rows_body(uint32_t bid, const DevPrime *__restri
     }
 }
 
+template <int LOGN, int K, int E, int SUBS, bool INV, bool W = false, bool MX = false, class Load, class Store>
+__device__ __forceinline__ void rows_body(uint32_t bid, const DevPrime *__restrict__ primes, uint32_t I, uint32_t j0,
+                                          uint32_t jn, const Load &load_elem, const Store &store_elem,
+                                          uint32_t jn_short = 0) {
+    constexpr int NP = (K + E - 1) / E;
+    __shared__ uint64_t tile[NP > 1 ? SUBS << K : 1];
+    rows_body_t<LOGN, K, E, SUBS, INV, W, MX>(tile, bid, primes, I, j0, jn, load_elem, store_elem, jn_short);
+}
+
 // Columns pass: SUBS adjacent columns of one limb per block; a column is the first
 // K layers of the limb (stride 2^(LOGN-K) between its values).
 // Blocks run limb (i, j) by limb with j fastest, or with i fastest when I_FAST (then I
 // is the prime count) -- the order for loads shared by all primes of one j.
 template <int LOGN, int K, int E, int SUBS, bool INV, bool I_FAST = false, class Load, class Store>
-__device__ __forceinline__ void cols_body(uint32_t bid, const DevPrime *__restrict__ primes, uint32_t j0, uint32_t jn,
-                                          const Load &load_elem, const Store &store_elem, uint32_t I = 1) {
-    constexpr int SZ = 1 << K;
+__device__ __forceinline__ void cols_body_t(uint64_t *tile, uint32_t bid, const DevPrime *__restrict__ primes,
+                                            uint32_t j0, uint32_t jn, const Load &load_elem, const Store &store_elem,
+                                            uint32_t I = 1) {
     constexpr int C = 1 << (LOGN - K);
     constexpr int TILES = C / SUBS;
-    __shared__ uint64_t tile[SUBS * SZ];
     const int tid = threadIdx.x;
     const int col = tid % SUBS, t = tid / SUBS;
     const uint32_t r = bid / TILES;
@@ -261,6 +269,13 @@ __device__ __forceinline__ void cols_body(uint32_t bid, const DevPrime *__restri
     auto load = [&](int idx, int) { return load_elem(i, j, (size_t)idx * C + c); };
     auto store = [&](int idx, uint64_t x) { store_elem(i, j, (size_t)idx * C + c, x); };
     run_phases<LOGN, K, E, INV, false>(tile, addr, load, store, t, 0, 0, tw, bf_const(P), true);
+}
+
+template <int LOGN, int K, int E, int SUBS, bool INV, bool I_FAST = false, class Load, class Store>
+__device__ __forceinline__ void cols_body(uint32_t bid, const DevPrime *__restrict__ primes, uint32_t j0, uint32_t jn,
+                                          const Load &load_elem, const Store &store_elem, uint32_t I = 1) {
+    __shared__ uint64_t tile[SUBS << K];
+    cols_body_t<LOGN, K, E, SUBS, INV, I_FAST>(tile, bid, primes, j0, jn, load_elem, store_elem, I);
 }
 
 // The key switch's ModUp rows pass fused with the products by the key (N = 2^15, 2^16):
@@ -679,6 +694,78 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_grouped_kernel(NttGr
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
         });
+}
+
+// ---- Transforms over several ring degrees (N = 2^13 .. 2^17) in one launch per pass.
+// Every degree runs 256-thread blocks over a 32 KB LDS tile -- the rows pass 4096 >> K2
+// rows per block, the columns pass 4096 >> K1 columns -- so blocks of every degree share
+// one kernel; a group's degree is uniform over its blocks.
+struct NttMultiDev {
+    NttGroupDev d;
+    int32_t logn;
+};
+struct NttMulti {
+    uint32_t n;
+    uint32_t start[MAX_GROUPS + 1];
+    NttMultiDev g[MAX_GROUPS];
+};
+
+template <int LOGN, bool INV, bool LAST, bool FULL>
+__device__ __forceinline__ void rows_multi(uint64_t *tile, uint32_t bid, const NttGroupDev &a) {
+    constexpr int K2 = LOGN - LOGN / 2;
+    constexpr int RS = 4096 >> K2;
+    const GridAccess g{a.data, a.lg, LOGN};
+    const GridAccess s{const_cast<uint64_t *>(a.src ? a.src : a.data), a.src ? a.slg : a.lg, LOGN};
+    const DevPrime *primes = a.primes;
+    rows_body_t<LOGN, K2, 4, RS, INV>(
+        tile, bid, primes, a.lg.I, 0, a.lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *s.at(i, j, off); },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
+        });
+}
+
+template <int LOGN, bool INV, bool LAST, bool FULL>
+__device__ __forceinline__ void cols_multi(uint64_t *tile, uint32_t bid, const NttGroupDev &a) {
+    constexpr int K1 = LOGN / 2;
+    constexpr int CS = 4096 >> K1;
+    const GridAccess g{a.data, a.lg, LOGN};
+    const GridAccess s{const_cast<uint64_t *>(a.src ? a.src : a.data), a.src ? a.slg : a.lg, LOGN};
+    const DevPrime *primes = a.primes;
+    cols_body_t<LOGN, K1, 4, CS, INV>(
+        tile, bid, primes, 0, a.lg.J, [&](uint32_t i, uint32_t j, size_t off) { return *s.at(i, j, off); },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            *g.at(i, j, off) = LAST ? finish<INV, FULL>(x, primes[i]) : x;
+        });
+}
+
+template <bool INV, bool LAST, bool FULL>
+__global__ __launch_bounds__(256) void ntt_rows_multi_kernel(NttMulti ng) {
+    __shared__ uint64_t tile[4096];
+    const uint32_t gi = group_of(ng.start, ng.n, blockIdx.x);
+    const NttMultiDev &m = ng.g[gi];
+    const uint32_t bid = blockIdx.x - ng.start[gi];
+    switch (m.logn) {  // uniform: the whole block
+    case 13: rows_multi<13, INV, LAST, FULL>(tile, bid, m.d); break;
+    case 14: rows_multi<14, INV, LAST, FULL>(tile, bid, m.d); break;
+    case 15: rows_multi<15, INV, LAST, FULL>(tile, bid, m.d); break;
+    case 16: rows_multi<16, INV, LAST, FULL>(tile, bid, m.d); break;
+    default: rows_multi<17, INV, LAST, FULL>(tile, bid, m.d); break;
+    }
+}
+
+template <bool INV, bool LAST, bool FULL>
+__global__ __launch_bounds__(256) void ntt_cols_multi_kernel(NttMulti ng) {
+    __shared__ uint64_t tile[4096];
+    const uint32_t gi = group_of(ng.start, ng.n, blockIdx.x);
+    const NttMultiDev &m = ng.g[gi];
+    const uint32_t bid = blockIdx.x - ng.start[gi];
+    switch (m.logn) {
+    case 13: cols_multi<13, INV, LAST, FULL>(tile, bid, m.d); break;
+    case 14: cols_multi<14, INV, LAST, FULL>(tile, bid, m.d); break;
+    case 15: cols_multi<15, INV, LAST, FULL>(tile, bid, m.d); break;
+    case 16: cols_multi<16, INV, LAST, FULL>(tile, bid, m.d); break;
+    default: cols_multi<17, INV, LAST, FULL>(tile, bid, m.d); break;
+    }
 }
 
 // ---- ModDown by the special prime p, fused into the transforms (switch_key step 5):
@@ -1127,6 +1214,34 @@ void launch_ntt_groups(const NttGroup *gs, uint32_t n, hipStream_t st) {
     launch_check("grouped ntt launch");
 }
 
+template <bool INV, bool FULL>
+void launch_ntt_multi(const NttGroup *gs, const int *logns, uint32_t n, hipStream_t st) {
+    NttMulti cols{}, rows{};
+    std::vector<uint64_t> csz, rsz;
+    for (uint32_t g = 0; g < n; g++) {
+        const NttGroup &a = gs[g];
+        const int L = logns[g];
+        const int K1 = L / 2, K2 = L - K1;
+        const uint32_t CS = 4096u >> K1, RS = 4096u >> K2;
+        const NttGroupDev d{a.data, nullptr, a.lg, a.lg, a.primes};
+        const NttGroupDev d_first{a.data, a.src, a.lg, a.src ? a.slg : a.lg, a.primes};
+        // the first pass reads src where a group has one; the second runs in place
+        cols.g[g] = NttMultiDev{INV ? d : d_first, L};
+        rows.g[g] = NttMultiDev{INV ? d_first : d, L};
+        csz.push_back((uint64_t)a.lg.I * a.lg.J * ((1u << (L - K1)) / CS));
+        rsz.push_back((uint64_t)a.lg.I * (1u << (L - K2)) * ((a.lg.J + RS - 1) / RS));
+    }
+    const uint32_t cb = fill_starts(cols, csz), rb = fill_starts(rows, rsz);
+    if (!INV) {
+        ntt_cols_multi_kernel<false, false, FULL><<<cb, 256, 0, st>>>(cols);
+        ntt_rows_multi_kernel<false, true, FULL><<<rb, 256, 0, st>>>(rows);
+    } else {
+        ntt_rows_multi_kernel<true, false, FULL><<<rb, 256, 0, st>>>(rows);
+        ntt_cols_multi_kernel<true, true, FULL><<<cb, 256, 0, st>>>(cols);
+    }
+    launch_check("multi-degree ntt launch");
+}
+
 template <int LOGN>
 void launch_modup_keyprod_groups(const KsGroup *gs, uint32_t n, const uint64_t *key, uint32_t K_key,
                                  const DevPrime *digits, hipStream_t st) {
@@ -1363,6 +1478,38 @@ void ntt_inverse_groups(const NttGroup *gs, uint32_t n, int logn, NttOut out, hi
 #undef SHC_NTT_CASE
         default: fail(ERR_INVALID_ARGUMENT, "poly_modulus_degree must be 2^1 .. 2^17");
     }
+}
+
+namespace {
+void check_multi(const NttGroup *gs, const int *logns, uint32_t n) {
+    if (n > MAX_GROUPS) fail(ERR_INVALID_ARGUMENT, "too many groups for one launch");
+    uint64_t cols = 0, rows = 0;
+    for (uint32_t g = 0; g < n; g++) {
+        const int L = logns[g];
+        if (L < 13 || L > 17) fail(ERR_INVALID_ARGUMENT, "multi-degree transforms take N = 2^13 .. 2^17");
+        const LimbGrid &lg = gs[g].lg;
+        if (!lg.I || !lg.J) fail(ERR_INVALID_ARGUMENT, "empty transform group");
+        const int K1 = L / 2, K2 = L - K1;
+        cols += (uint64_t)lg.I * lg.J * ((1u << (L - K1)) / (4096u >> K1));
+        rows += (uint64_t)lg.I * (1u << (L - K2)) * ((lg.J + (4096u >> K2) - 1) / (4096u >> K2));
+    }
+    if (cols >= (uint64_t(1) << 32) || rows >= (uint64_t(1) << 32))
+        fail(ERR_INVALID_ARGUMENT, "transform batch too large for one launch");
+}
+}  // namespace
+
+void ntt_forward_multi(const NttGroup *gs, const int *logns, uint32_t n, NttOut out, hipStream_t st) {
+    check_multi(gs, logns, n);
+    if (!n) return;
+    if (out == NTT_FULL) launch_ntt_multi<false, true>(gs, logns, n, st);
+    else launch_ntt_multi<false, false>(gs, logns, n, st);
+}
+
+void ntt_inverse_multi(const NttGroup *gs, const int *logns, uint32_t n, NttOut out, hipStream_t st) {
+    check_multi(gs, logns, n);
+    if (!n) return;
+    if (out == NTT_FULL) launch_ntt_multi<true, true>(gs, logns, n, st);
+    else launch_ntt_multi<true, false>(gs, logns, n, st);
 }
 
 bool ntt_modup_keyprod_groups(const KsGroup *gs, uint32_t n, const uint64_t *key, uint32_t key_count,

@@ -40,6 +40,8 @@ def lib():
                                                ctypes.c_uint64, _u64p]
         for name in ("shc_ntt_forward", "shc_ntt_inverse"):
             getattr(L, name).argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp]
+        for name in ("shc_ntt_forward_multi", "shc_ntt_inverse_multi"):
+            getattr(L, name).argtypes = [ctypes.POINTER(NttJob), ctypes.c_uint32, _vp]
         u32, sz = ctypes.c_uint32, ctypes.c_size_t
         L.shc_ckks_multiply.argtypes = [_vp, _vp, _vp, _vp, u32, u32, _vp]
         L.shc_multiply_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
@@ -141,6 +143,26 @@ def groups(items):
     for k, (p, c, l) in enumerate(items):
         arr[k] = CtGroup(p, c, l)
     return arr
+
+
+class NttJob(ctypes.Structure):
+    """shc_ntt_job: poly_count polynomials of limb_count limbs at d_data, in ctx's ring."""
+    _fields_ = [("ctx", _vp), ("d_data", _vp), ("poly_count", ctypes.c_uint32), ("limb_count", ctypes.c_uint32)]
+
+
+def _ntt_multi(fn, jobs, stream):
+    arr = (NttJob * max(1, len(jobs)))(*[NttJob(c.handle, d, p, l) for c, d, p, l in jobs])
+    check(fn(arr, len(jobs), stream))
+
+
+def ntt_forward_multi(jobs, stream=None):
+    """Forward transforms of (Context, d_ptr, polys, limbs) jobs of mixed ring degrees, one
+    launch per pass for those at N >= 2^13."""
+    _ntt_multi(lib().shc_ntt_forward_multi, jobs, stream)
+
+
+def ntt_inverse_multi(jobs, stream=None):
+    _ntt_multi(lib().shc_ntt_inverse_multi, jobs, stream)
 
 
 class KsShape(ctypes.Structure):

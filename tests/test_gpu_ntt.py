@@ -96,3 +96,36 @@ def test_batched_ntt_config_shape():
     ctx.ntt_inverse(dev.data_ptr(), polys, 24, stream())
     torch.cuda.synchronize()
     assert torch.equal(dev, orig)
+
+
+def test_multi_degree_transforms_in_one_launch():
+    """shc_ntt_forward_multi / shc_ntt_inverse_multi: jobs of five ring degrees (2^12 .. 2^17,
+    the 2^12 job on its own launches) and different prime counts and batch sizes give every
+    job the oracle's words, and the inverse returns the inputs."""
+    specs = [(12, [50, 40], 3), (13, [60, 50, 40, 40], 2), (15, [60] * 5, 3), (16, [60, 50, 50], 2),
+             (17, [60, 60], 1), (15, [40, 40], 1)]
+    ctxs, inputs, devs, jobs = [], [], [], []
+    rng = np.random.default_rng(77)
+    for logn, bits, polys in specs:
+        n = 1 << logn
+        moduli = ck.coeff_modulus_create(n, bits)
+        ctx = hip().Context(hip().SCHEME_CKKS, n, moduli)
+        a = ck.uniform_residues(rng, (polys, len(bits), n), moduli)
+        d = to_dev(a)
+        ctxs.append((ctx, moduli, logn))
+        inputs.append(a)
+        devs.append(d)
+        jobs.append((ctx, d.data_ptr(), polys, len(bits)))
+    hip().ntt_forward_multi(jobs, stream())
+    outs = [from_dev(d) for d in devs]
+    for (ctx, moduli, logn), a, f in zip(ctxs, inputs, outs):
+        np.testing.assert_array_equal(f, ck.ntt_rns(a, moduli, logn), err_msg=f"forward, N=2^{logn}")
+    hip().ntt_inverse_multi(jobs, stream())
+    for (ctx, moduli, logn), a, d in zip(ctxs, inputs, devs):
+        np.testing.assert_array_equal(from_dev(d), a, err_msg=f"round trip, N=2^{logn}")
+    # errors: a level beyond the context, too many jobs
+    ctx, moduli, logn = ctxs[2]
+    with pytest.raises(hip().SealHipError):
+        hip().ntt_forward_multi([(ctx, devs[2].data_ptr(), 1, 9)], stream())
+    with pytest.raises(hip().SealHipError, match="16"):
+        hip().ntt_forward_multi([(ctx, devs[2].data_ptr(), 1, 1)] * 17, stream())
