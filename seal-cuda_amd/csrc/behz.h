@@ -21,12 +21,11 @@ struct BehzDev {
     uint64_t t;
     const DevPrime *bsk_primes;      // [bsk]: B..., m_sk (with NTT tables)
     const uint64_t *mt_qhat_inv;     // [L][2]   (m_tilde * qhat_j^-1) mod q_j
-    const uint64_t *qhat_mod_bskm;   // [bsk+1][L] qhat_j mod p, p in Bsk then m_tilde
-    const uint64_t *q_mod_bsk;       // [bsk]    prod(q) mod b
-    const uint64_t *inv_mt_bsk;      // [bsk][2] m_tilde^-1 mod b
+    const uint64_t *qhat_ext;        // [bsk+1][L] qhat_j m_tilde^-1 mod b (b in Bsk), then qhat_j mod m_tilde
+    const uint64_t *q_ext;           // [bsk]    prod(q) m_tilde^-1 mod b
+    const uint64_t *qhat_floor;      // [bsk][L] qhat_j prod(q)^-1 mod b
+    const uint64_t *t_inv_q_bsk;     // [bsk][2] t prod(q)^-1 mod b
     const uint64_t *t_qhat_inv;      // [L][2]   (t * qhat_j^-1) mod q_j
-    const uint64_t *t_bsk;           // [bsk][2] t mod b
-    const uint64_t *inv_q_bsk;       // [bsk][2] prod(q)^-1 mod b
     const uint64_t *bhat_inv;        // [bsz][2] Bhat_j^-1 mod B_j
     const uint64_t *bhat_mod_q;      // [L][bsz] Bhat_j mod q_i
     const uint64_t *bhat_mod_msk;    // [bsz]    Bhat_j mod m_sk

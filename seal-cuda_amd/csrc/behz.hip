@@ -6,7 +6,8 @@
 // dot product over the input base.  A block takes 64 coefficients, one per lane, and its
 // two waves split the work: each stages half of the scaled inputs in LDS, and then
 // computes its own half of the output primes, OB at a time (each staged word read once
-// per group), with three-word product sums (accum.h; inputs < 2^61, at most 64 terms,
+// per group), with three-word product sums (accum.h; inputs < 2^61, at most 63 terms:
+// 62 inputs and the correction product that smMrq / Shenoy-Kumaresan add,
 // so every sum stays below 2^128), so intermediate residues never leave the kernel.
 // Sharing the staged words between two waves halves the LDS per wave, which is what
 // bounds the waves a CU holds.  Scalar multiplies that precede a conversion in the
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restr
     const int lane = threadIdx.x % CW, w = __builtin_amdgcn_readfirstlane(threadIdx.x / CW);
     uint32_t ob, oe;
     slice_of(B.bsk, w, ob, oe);
-    const uint64_t *row_mt = B.qhat_mod_bskm + (size_t)B.bsk * B.L;
+    const uint64_t *row_mt = B.qhat_ext + (size_t)B.bsk * B.L;
     for (size_t base = (size_t)blockIdx.x * CW; base < total; base += (size_t)gridDim.x * CW) {
         const size_t idx = base + lane;
         const bool live = idx < total;
@@ -118,20 +119,20 @@ __global__ __launch_bounds__(BT) void behz_extend_kernel(const uint64_t *__restr
         for (uint32_t j = 0; j < B.L; j++) s_mt += ys[j * CW + lane] * row_mt[j];
         const uint64_t r = ((s_mt & 0xffffffffull) * B.m_tilde_neg_inv_q) & 0xffffffffull;
         for (uint32_t i0 = ob; i0 < oe; i0 += OB) {
+            // the rows carry m_tilde^-1: the sums are (s + r q) m_tilde^-1 mod p directly
             Acc3 acc[OB];
-            dot_rows(acc, ys, B.qhat_mod_bskm, B.L, i0, B.bsk, lane);
+            dot_rows(acc, ys, B.qhat_ext, B.L, i0, B.bsk, lane);
 #pragma unroll
             for (int u = 0; u < OB; u++) {
                 const uint32_t i = i0 + u;
                 if (i >= oe) break;
                 const DevPrime &P = B.bsk_primes[i];
-                const uint64_t s = acc_reduce(acc[u], P);
-                // centred r (m_tilde is a power of two, so r >= m_tilde/2 is negative)
+                // the centred r (m_tilde is a power of two, so r >= m_tilde/2 is negative)
+                // times q m_tilde^-1 joins the sum (one more term below 2^122), so one
+                // reduction gives the output residue
                 const uint64_t rc = r >= 0x80000000ull ? r + P.q - 0x100000000ull : r;
-                u128 v = mul64x64(rc, B.q_mod_bsk[i]);
-                add128(v, u128{s, 0});
-                const uint64_t wv = barrett128(v, P.q, P.ratio_lo, P.ratio_hi);
-                if (live) out[(p * B.bsk + i) * n + c] = shoup_pair(wv, B.inv_mt_bsk + 2 * i, P.q);
+                mac(acc[u], rc, B.q_ext[i]);
+                if (live) out[(p * B.bsk + i) * n + c] = acc_reduce(acc[u], P);
             }
         }
         __syncthreads();  // the staged words are read before the next coefficients' overwrite them
@@ -157,8 +158,9 @@ __global__ __launch_bounds__(BT) void behz_floor_kernel(const uint64_t *__restri
             ws[j * CW + lane] = live ? shoup_pair(Tq[(p * B.L + j) * n + c], B.t_qhat_inv + 2 * j, qp[j].q) : 0;
         __syncthreads();
         for (uint32_t i0 = ob; i0 < oe; i0 += OB) {
+            // the rows carry prod(q)^-1: Z = t Tb q^-1 - conv q^-1 mod p
             Acc3 acc[OB];
-            dot_rows(acc, ws, B.qhat_mod_bskm, B.L, i0, B.bsk, lane);
+            dot_rows(acc, ws, B.qhat_floor, B.L, i0, B.bsk, lane);
 #pragma unroll
             for (int u = 0; u < OB; u++) {
                 const uint32_t i = i0 + u;
@@ -166,8 +168,8 @@ __global__ __launch_bounds__(BT) void behz_floor_kernel(const uint64_t *__restri
                 if (!live) continue;
                 const DevPrime &P = B.bsk_primes[i];
                 const uint64_t conv = acc_reduce(acc[u], P);
-                const uint64_t tb = shoup_pair(Tb[(p * B.bsk + i) * n + c], B.t_bsk + 2 * i, P.q);
-                Z[(p * B.bsk + i) * n + c] = shoup_pair(tb + P.q - conv, B.inv_q_bsk + 2 * i, P.q);
+                const uint64_t tb = shoup_pair(Tb[(p * B.bsk + i) * n + c], B.t_inv_q_bsk + 2 * i, P.q);
+                Z[(p * B.bsk + i) * n + c] = submod(tb, conv, P.q);
             }
         }
         __syncthreads();
@@ -211,12 +213,11 @@ __global__ __launch_bounds__(BT) void behz_sk_kernel(const uint64_t *__restrict_
                 if (i >= oe) break;
                 if (!live) continue;
                 const DevPrime &Q = qp[i];
-                const uint64_t o = acc_reduce(s[u], Q);
-                // Shenoy-Kumaresan correction with the centred alpha_sk
+                // Shenoy-Kumaresan correction with the centred alpha_sk: one more term of
+                // the sum (below 2^122), so one reduction gives the output residue
                 const uint64_t pb = B.prod_B_mod_q[i];
-                u128 v = neg ? mul64x64(B.m_sk - alpha, pb) : mul64x64(alpha, Q.q - pb);
-                add128(v, u128{o, 0});
-                out[(p * B.L + i) * n + c] = barrett128(v, Q.q, Q.ratio_lo, Q.ratio_hi);
+                mac(s[u], neg ? B.m_sk - alpha : alpha, neg ? pb : Q.q - pb);
+                out[(p * B.L + i) * n + c] = acc_reduce(s[u], Q);
             }
         }
         __syncthreads();
@@ -293,26 +294,32 @@ void build_behz_level(Context &c, uint32_t L, BehzLevel &out) {
     v.clear();
     for (uint32_t j = 0; j < L; j++) v.push_back(mulmod(mt % q[j], qhat_inv[j], q[j]));
     const size_t o_mt_qhat_inv = put(shoup_vec(v, q));
+    // the q -> Bsk conversion rows (qhat_j mod b) with the step that follows each
+    // conversion folded in: smMrq's m_tilde^-1 (extend, whose last row is qhat_j mod
+    // m_tilde) and fastFloor's prod(q)^-1 (floor)
+    std::vector<uint64_t> inv_mt(bsk), inv_q(bsk);
+    for (uint32_t i = 0; i < bsk; i++) {
+        inv_mt[i] = invmod(mt % Bsk[i], Bsk[i]);
+        inv_q[i] = invmod(prod_mod(q, NONE, Bsk[i]), Bsk[i]);
+    }
     v.clear();
     for (uint32_t i = 0; i < bsk; i++)
-        for (uint32_t j = 0; j < L; j++) v.push_back(prod_mod(q, j, Bsk[i]));
+        for (uint32_t j = 0; j < L; j++) v.push_back(mulmod(prod_mod(q, j, Bsk[i]), inv_mt[i], Bsk[i]));
     for (uint32_t j = 0; j < L; j++) v.push_back(prod_mod(q, j, mt));
-    const size_t o_qhat = put(v);
+    const size_t o_qhat_ext = put(v);
     v.clear();
-    for (uint32_t i = 0; i < bsk; i++) v.push_back(prod_mod(q, NONE, Bsk[i]));
-    const size_t o_q_bsk = put(v);
+    for (uint32_t i = 0; i < bsk; i++) v.push_back(mulmod(prod_mod(q, NONE, Bsk[i]), inv_mt[i], Bsk[i]));
+    const size_t o_q_ext = put(v);
     v.clear();
-    for (uint32_t i = 0; i < bsk; i++) v.push_back(invmod(mt % Bsk[i], Bsk[i]));
-    const size_t o_inv_mt = put(shoup_vec(v, Bsk));
+    for (uint32_t i = 0; i < bsk; i++)
+        for (uint32_t j = 0; j < L; j++) v.push_back(mulmod(prod_mod(q, j, Bsk[i]), inv_q[i], Bsk[i]));
+    const size_t o_qhat_floor = put(v);
+    v.clear();
+    for (uint32_t i = 0; i < bsk; i++) v.push_back(mulmod(t % Bsk[i], inv_q[i], Bsk[i]));
+    const size_t o_t_inv_q = put(shoup_vec(v, Bsk));
     v.clear();
     for (uint32_t j = 0; j < L; j++) v.push_back(mulmod(t % q[j], qhat_inv[j], q[j]));
     const size_t o_t_qhat_inv = put(shoup_vec(v, q));
-    v.clear();
-    for (uint32_t i = 0; i < bsk; i++) v.push_back(t % Bsk[i]);
-    const size_t o_t_bsk = put(shoup_vec(v, Bsk));
-    v.clear();
-    for (uint32_t i = 0; i < bsk; i++) v.push_back(invmod(prod_mod(q, NONE, Bsk[i]), Bsk[i]));
-    const size_t o_inv_q_bsk = put(shoup_vec(v, Bsk));
     v.clear();
     for (size_t j = 0; j < bsz; j++) v.push_back(invmod(prod_mod(Bv, j, Bv[j]), Bv[j]));
     const size_t o_bhat_inv = put(shoup_vec(v, Bv));
@@ -367,12 +374,11 @@ void build_behz_level(Context &c, uint32_t L, BehzLevel &out) {
     d.t = t;
     d.bsk_primes = out.bsk_primes.as<const DevPrime>();
     d.mt_qhat_inv = base + o_mt_qhat_inv;
-    d.qhat_mod_bskm = base + o_qhat;
-    d.q_mod_bsk = base + o_q_bsk;
-    d.inv_mt_bsk = base + o_inv_mt;
+    d.qhat_ext = base + o_qhat_ext;
+    d.q_ext = base + o_q_ext;
+    d.qhat_floor = base + o_qhat_floor;
+    d.t_inv_q_bsk = base + o_t_inv_q;
     d.t_qhat_inv = base + o_t_qhat_inv;
-    d.t_bsk = base + o_t_bsk;
-    d.inv_q_bsk = base + o_inv_q_bsk;
     d.bhat_inv = base + o_bhat_inv;
     d.bhat_mod_q = base + o_bhat_q;
     d.bhat_mod_msk = base + o_bhat_msk;
