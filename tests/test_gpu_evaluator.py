@@ -7,6 +7,7 @@ injected uniform words (the same words go to both paths).
 """
 import numpy as np
 import pytest
+import torch
 
 import checkers as ck
 from gpu_util import empty_dev, from_dev, hip, stream, to_dev
@@ -65,6 +66,9 @@ def test_ckks_multiply(logn, bits, L, count):
     # 33 digits: the fused kernel reduces the transformed words below 4q before the
     # products (the 128-bit sum bound); groups of 4, 2 and 1
     (CKKS, 15, [60] * 34, 33, 7),
+    # the largest ring degree, and a level of one prime
+    (CKKS, 17, [60] * 4, 3, 2),
+    (CKKS, 12, [50, 50], 1, 2),
 ])
 def test_relinearize(scheme, logn, bits, L, count):
     ctx, moduli = make_ctx(scheme, logn, bits, 65537 if scheme == BFV else 0)
@@ -141,8 +145,14 @@ def test_rescale(scheme, logn, bits, L):
     (11, [60, 60, 60], 2, 1024, 1),
     (10, [30, 30, 30, 40], 3, 786433, 3),
     (4, [30, 30, 40], 2, 97, 2),
+    # a 40-bit t with two 60-bit primes: 32 + bits(t) + bits(q) >= 61 (L + 1), so B takes
+    # one more prime (|B| = L + 1, |Bsk| = L + 2: uneven output slices in the conversions)
+    (12, [60, 60, 60], 2, None, 3),
+    (15, [60] * 5, 4, None, 5),
 ])
 def test_bfv_multiply(logn, bits, L, t, count):
+    if t is None:  # a 40-bit prime plain modulus (= 1 mod 2N, coprime to the 60-bit q)
+        t = ck.coeff_modulus_create(1 << logn, [40])[0]
     ctx, moduli = make_ctx(BFV, logn, bits, t)
     n = 1 << logn
     rng = np.random.default_rng(300 + logn)
@@ -178,3 +188,16 @@ def test_keyswitch_rejects_small_workspace():
     ws = empty_dev(16)
     with pytest.raises(hip().SealHipError):
         ctx.relinearize(d.data_ptr(), k.data_ptr(), 1, 2, ws.data_ptr(), 128, stream())
+
+
+def test_empty_batches():
+    """count = 0 is a valid batch for every batched entry point: nothing runs, no error."""
+    ctx, moduli = make_ctx(CKKS, 12, [60, 40, 60])
+    bctx, _ = make_ctx(BFV, 12, [60, 60, 60], 65537)
+    ws = empty_dev((1 << 16,))
+    d = empty_dev((8,))
+    ctx.ckks_multiply(d.data_ptr(), d.data_ptr(), d.data_ptr(), 0, 2, stream())
+    ctx.relinearize(d.data_ptr(), d.data_ptr(), 0, 2, ws.data_ptr(), 8 << 16, stream())
+    ctx.rescale_to_next(d.data_ptr(), d.data_ptr(), 0, 2, 2, ws.data_ptr(), 8 << 16, stream())
+    bctx.multiply(d.data_ptr(), d.data_ptr(), d.data_ptr(), 0, 2, ws.data_ptr(), 8 << 16, stream())
+    torch.cuda.synchronize()
