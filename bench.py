@@ -20,6 +20,9 @@ against the same sum through separate rotations.
 `--workload matvec`: CKKS 1024 x 1024 matrix-vector products by baby-step giant-step
 rotations (N=2^15, batches of 16 requests).
 
+`--workload primitives`: rotation, rescale, encryption, decryption and both encoders,
+each timed alone (CKKS at the headline shape, BFV at N=8192), batches of 128.
+
 `--workload copy`: a 1 GiB device copy, the calibration run for HBM byte counters.
 
 Prints one JSON line.  Data are seeded uniform residues (synthetic) and injected
@@ -47,6 +50,7 @@ KEYPROD_TRAFFIC_B128 = 44.16e9 + 3.36e9
 KEYPROD_VALU_B128 = 8140554240
 VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 CKKS = 2
+BFV_SCHEME = 1
 T_BFV = 65537
 
 
@@ -56,7 +60,8 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="mulrelin",
-                   choices=["mulrelin", "ntt", "copy", "chain", "rotsum", "matvec", "serve", "bfv8192", "bfv"])
+                   choices=["mulrelin", "ntt", "copy", "chain", "rotsum", "matvec", "serve", "bfv8192", "bfv",
+                            "primitives"])
     p.add_argument("--no-bfv", action="store_true", help="skip the BFV leg of the mulrelin workload")
     p.add_argument("--batch", type=int, default=128)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -1359,14 +1364,330 @@ def bench_bfv8192(args):
     return out_json
 
 
+def bench_primitives(args):
+    """The path's other entry points, each timed alone with its inputs resident in HBM.
+
+    CKKS at the headline shape (N=2^16, CoeffModulus::Create(65536, {60 x 25}), level 24,
+    128 ciphertexts): one rotation per ciphertext (shc_apply_galois_keyswitch), rescale to
+    level 23, public-key encryption of 128 NTT-form plaintexts (shc_encrypt_many),
+    decryption (shc_decrypt_many), CKKSEncoder encode and decode of 128 x 32768 slots
+    (shc_ckks_encode_many / shc_ckks_decode_many).  BFV at the CPU-runnable shape (N=8192,
+    BFVDefault(8192), t=65537, 128 ciphertexts): encryption, decryption and the
+    BatchEncoder (batched; the BatchEncoder also one call per plaintext, as the
+    reference's).  Each operation reports
+    its rate, the whole call's algorithmic bytes over its time against the HBM peak, and
+    the reference CPU path on the same operation (its own randomness for encryption).
+    Self-checks: BFV encode -> encrypt -> decrypt -> decode returns the slots; CKKS
+    decrypt(encrypt(m)) - m is small in every coefficient of its first limb; CKKS
+    decode(encode(x)) is within 2^-20 of x."""
+    import numpy as np
+    import torch
+    import sealhip
+    import checkers as ck
+    sealhip.init(0)
+    st = torch.cuda.current_stream().cuda_stream
+    dev = lambda shape: torch.empty(shape, dtype=torch.int64, device="cuda")
+    wsb = lambda nbytes: dev(((max(nbytes, 8) + 7) // 8,))
+    B = 128
+    steps, warmup = max(args.steps, 3), max(args.warmup, 1)
+    cpu_s = args.cpu_seconds / 3
+    ops = {}
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        with Timer(torch) as tm:
+            for _ in range(steps):
+                fn()
+        return tm.ms / steps
+
+    def record(name, metric, unit, units, ms, nbytes, workload, entry, make=None, sample=None):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        d = {"metric": metric, "value": units / (ms * 1e-3), "unit": unit, "ms_per_step": ms, "steps": steps,
+             "config": {"workload": workload, "parallelism": "single GPU"},
+             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": f"{entry}: the whole call",
+                          "algorithmic_bytes": nbytes, "avg_ms": ms}}
+        if make is not None and not args.no_cpu:
+            d["cpu_baseline"] = cpu_baseline(make, unit, 1, cpu_s, sample)
+        ops[name] = d
+        print(f"[bench] primitives: {name} {d['value']:.1f} {unit}", file=sys.stderr, flush=True)
+
+    # ---------------------------------------------------------------- CKKS, N = 2^16, L = 24
+    logn, n = 16, 1 << 16
+    moduli = sealhip.coeff_modulus_create(n, [60] * 25)
+    K, L = len(moduli), len(moduli) - 1
+    limb = n * 8
+    shape = f"N=2^16, 24+1 primes of 60 bits, level 24, batch of {B}"
+    ctx = sealhip.Context(sealhip.SCHEME_CKKS, n, moduli)
+    gen = torch.Generator(device="cuda").manual_seed(17)
+    rng = np.random.default_rng(17)
+    h_ct = ck.uniform_residues(rng, (2, L, n), moduli[:L])  # CPU-baseline inputs
+    h_key = ck.random_key(rng, moduli, logn)
+
+    x = dev((B, 2, L, n))
+    fill_uniform(torch, x, moduli[:L], 2, gen)
+    key = dev((K - 1, 2, K, n))
+    fill_uniform(torch, key, moduli, 2, gen)
+    elt = ctx.galois_elt_from_step(1)
+    gw = ctx.galois_workspace_size(B, L)
+    gb = wsb(gw)
+    ms = timed(lambda: ctx.apply_galois_keyswitch(x.data_ptr(), key.data_ptr(), elt, B, L, gb.data_ptr(), gw, st))
+    sess = {}
+
+    def ref_sess(kind, scheme, nn, mods, t=0):
+        if kind != "reference":
+            return None
+        return ck.RefSession(scheme, nn, mods, t)
+
+    def make_rot(kind):
+        s_ = ref_sess(kind, CKKS, n, moduli)
+        if s_ is not None:
+            return lambda: s_.apply_galois(h_ct, ck.galois_elt(logn, 1), h_key, L)
+        return lambda: ck.apply_galois_keyswitch(CKKS, h_ct, ck.galois_elt(logn, 1), h_key, moduli, L, logn)
+
+    record("ckks_rotate", "CKKS rotations/s (one key switch each), " + shape, "rotations/s", B, ms,
+           (B * 4 * L + 2 * L * (L + 1)) * limb, "CKKS rotation by one slot of each ciphertext, " + shape,
+           "shc_apply_galois_keyswitch", make_rot, "one rotation (applyGaloisInplace) per call")
+    del gb, key
+    torch.cuda.empty_cache()
+
+    out = dev((B, 2, L - 1, n))
+    rw = ctx.rescale_workspace_size(B, 2, L)
+    rb = wsb(rw)
+    ms = timed(lambda: ctx.rescale_to_next(x.data_ptr(), out.data_ptr(), B, 2, L, rb.data_ptr(), rw, st))
+
+    def make_rescale(kind):
+        s_ = ref_sess(kind, CKKS, n, moduli)
+        if s_ is not None:
+            return lambda: s_.rescale(h_ct, L)
+        return lambda: ck.rescale(CKKS, h_ct, moduli[:L], logn)
+
+    record("ckks_rescale", "CKKS rescales/s, " + shape, "rescales/s", B, ms, B * (2 * L + 2 * (L - 1)) * limb,
+           "CKKS rescale to the next level, " + shape, "shc_rescale_to_next", make_rescale,
+           "one rescaleToNext per call")
+    del out, rb
+    torch.cuda.empty_cache()
+
+    # real keys for the round trip
+    sk, pk = dev((K, n)), dev((2, K, n))
+    ctx.keygen_secret(5, sk.data_ptr(), st)
+    ctx.keygen_public(sk.data_ptr(), 6, pk.data_ptr(), st)
+    plain = dev((B, L, n))
+    fill_uniform(torch, plain, moduli[:L], 1, gen)
+    ew = ctx.encrypt_many_workspace_size(L, B)
+    eb = wsb(ew)
+    ms = timed(lambda: ctx.encrypt_many(pk.data_ptr(), plain.data_ptr(), B, L, 99, x.data_ptr(), eb.data_ptr(), ew,
+                                        st))
+    h_pk = pk.cpu().numpy().view(np.uint64).copy()
+    h_plain = plain[0].cpu().numpy().view(np.uint64).copy()
+
+    def make_enc(kind):
+        s_ = ref_sess(kind, CKKS, n, moduli)
+        if s_ is not None:
+            return lambda: s_.encrypt(h_pk, h_plain, L)
+        u = ck.rns_of_small(np.random.default_rng(1).integers(-1, 2, size=n), moduli)
+        e = ck.rns_of_small(np.random.default_rng(2).integers(-3, 4, size=n), moduli)
+
+        def run():
+            z = ck.encrypt_zero_asymmetric(CKKS, h_pk, u, e, e, moduli, logn)
+            ck.mod_add(ck.rescale(CKKS, z, moduli, logn)[0], h_plain, moduli)
+        return run
+
+    record("ckks_encrypt", "CKKS public-key encryptions/s, " + shape, "encryptions/s", B, ms,
+           (2 * K + B * 3 * L) * limb, "CKKS encryption of NTT-form plaintexts with the public key, " + shape,
+           "shc_encrypt_many", make_enc, "one Encryptor::encrypt per call")
+    del eb
+    torch.cuda.empty_cache()
+
+    dec = dev((B, L, n))
+    ms = timed(lambda: ctx.decrypt_many(x.data_ptr(), 2, B, sk.data_ptr(), L, dec.data_ptr(), 0, 0, st))
+    # self-check: decrypt(encrypt(m)) - m is a small error (first limb, coefficient form)
+    diff = dev((1, L, n))
+    ctx.sub(dec[0].data_ptr(), plain[0].data_ptr(), diff.data_ptr(), 1, L, st)
+    ctx.ntt_inverse(diff.data_ptr(), 1, L, st)
+    d0 = diff[0, 0].cpu().numpy().view(np.uint64).astype(object)
+    if max(min(int(v), moduli[0] - int(v)) for v in d0) >= (1 << 30):
+        raise SystemExit("bench self-check failed: CKKS decrypt(encrypt(m)) - m is not small")
+    h_sk = sk.cpu().numpy().view(np.uint64).copy()
+
+    def make_dec(kind):
+        s_ = ref_sess(kind, CKKS, n, moduli)
+        if s_ is not None:
+            return lambda: s_.decrypt(h_ct, h_sk, L)
+        return lambda: ck.decrypt_phase(CKKS, h_ct, h_sk[:L], moduli, logn)
+
+    record("ckks_decrypt", "CKKS decryptions/s, " + shape, "decryptions/s", B, ms, (B * 3 * L + L) * limb,
+           "CKKS decryption to the NTT-form plaintext, " + shape, "shc_decrypt_many", make_dec,
+           "one Decryptor::decrypt per call")
+    del dec, diff, x, sk, pk
+    torch.cuda.empty_cache()
+
+    half = n // 2
+    vals = torch.rand((B, n), dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+    cw = ctx.ckks_encode_many_workspace_size(B)
+    cb = wsb(cw)
+    status = torch.zeros((B,), dtype=torch.int32, device="cuda")
+    scale = 2.0 ** 40
+    ms = timed(lambda: ctx.ckks_encode_many(vals.data_ptr(), half, n, B, scale, L, plain.data_ptr(),
+                                            status.data_ptr(), cb.data_ptr(), cw, st))
+    h_vals = vals[0].cpu().numpy()
+    h_cplx = h_vals[0::2] + 1j * h_vals[1::2]
+
+    def make_enc_ckks(kind):
+        s_ = ref_sess(kind, CKKS, n, moduli)
+        if s_ is not None:
+            return lambda: s_.ckks_encode(h_cplx, L, scale)
+        return lambda: ck.ckks_encode(h_cplx, moduli[:L], scale, logn)
+
+    slot_bytes = B * (half * 16 + L * limb)
+    record("ckks_encode", "CKKSEncoder encodes/s, N/2 = 32768 slots each, " + shape, "encodes/s", B, ms,
+           slot_bytes, "CKKSEncoder encode of 32768 complex slots to an NTT-form plaintext at level 24, batch of "
+           f"{B}", "shc_ckks_encode_many", make_enc_ckks, "one CKKSEncoder::encode per call")
+    del cb
+    back = torch.empty((B, n), dtype=torch.float64, device="cuda")
+    dw = ctx.ckks_decode_many_workspace_size(L, B)
+    db = wsb(dw)
+    ms = timed(lambda: ctx.ckks_decode_many(plain.data_ptr(), L, B, scale, back.data_ptr(), db.data_ptr(), dw, st))
+    if int(status.abs().sum()) or float((back - vals).abs().max()) > 2.0 ** -20:
+        raise SystemExit("bench self-check failed: CKKS decode(encode(x)) differs from x")
+    h_plain = plain[0].cpu().numpy().view(np.uint64).copy()
+
+    def make_dec_ckks(kind):
+        s_ = ref_sess(kind, CKKS, n, moduli)
+        if s_ is not None:
+            return lambda: s_.ckks_decode(h_plain, L, scale)
+        return lambda: ck.ckks_decode(h_plain, moduli[:L], scale, logn)
+
+    record("ckks_decode", "CKKSEncoder decodes/s, N/2 = 32768 slots each, " + shape, "decodes/s", B, ms,
+           slot_bytes, "CKKSEncoder decode of an NTT-form plaintext at level 24 to 32768 complex slots, batch of "
+           f"{B}", "shc_ckks_decode_many", make_dec_ckks, "one CKKSEncoder::decode per call")
+    del db, back, vals, plain, status
+    torch.cuda.empty_cache()
+
+    # ---------------------------------------------------------------- BFV, N = 8192
+    logn, n, t = 13, 8192, T_BFV
+    moduli = [0x7fffffd8001, 0x7fffffc8001, 0xfffffffc001, 0xffffff6c001, 0xfffffebc001]
+    K, L = len(moduli), len(moduli) - 1
+    limb = n * 8
+    shape = f"N=8192, BFVDefault(8192) (4 data + special), t=65537, batch of {B}"
+    ctx = sealhip.Context(sealhip.SCHEME_BFV, n, moduli, t)
+    sk, pk = dev((K, n)), dev((2, K, n))
+    ctx.keygen_secret(7, sk.data_ptr(), st)
+    ctx.keygen_public(sk.data_ptr(), 8, pk.data_ptr(), st)
+    values = torch.randint(0, t, (B, n), generator=gen, device="cuda", dtype=torch.int64)
+    plain = dev((B, n))
+
+    def encode_each():
+        for g in range(B):
+            ctx.batch_encode(values[g].data_ptr(), n, plain[g].data_ptr(), st)
+
+    per_call_ms = timed(encode_each)
+    ms = timed(lambda: ctx.batch_encode_many(values.data_ptr(), n, n, B, plain.data_ptr(), st))
+    h_v = values[0].cpu().numpy().view(np.uint64).copy()
+
+    def make_benc(kind):
+        s_ = ref_sess(kind, BFV_SCHEME, n, moduli, t)
+        if s_ is not None:
+            return lambda: s_.batch_encode(h_v)
+        return lambda: ck.batch_encode(h_v, t, logn)
+
+    record("bfv_batch_encode", "BatchEncoder encodes/s, " + shape, "encodes/s", B, ms, B * 2 * n * 8,
+           "BatchEncoder encode of 8192 slots mod t, " + shape, "shc_batch_encode_many", make_benc,
+           "one BatchEncoder::encode per call")
+    ops["bfv_batch_encode"]["one_call_per_plaintext"] = {"value": B / (per_call_ms * 1e-3), "ms": per_call_ms,
+                                                         "entry": "shc_batch_encode"}
+
+    cts = dev((B, 2, L, n))
+    ew = ctx.encrypt_many_workspace_size(L, B)
+    eb = wsb(ew)
+    ms = timed(lambda: ctx.encrypt_many(pk.data_ptr(), plain.data_ptr(), B, L, 123, cts.data_ptr(), eb.data_ptr(),
+                                        ew, st))
+    h_pk = pk.cpu().numpy().view(np.uint64).copy()
+    h_plain = plain[0].cpu().numpy().view(np.uint64).copy()
+
+    def make_benc2(kind):
+        s_ = ref_sess(kind, BFV_SCHEME, n, moduli, t)
+        if s_ is not None:
+            return lambda: s_.encrypt(h_pk, h_plain, L)
+        u = ck.rns_of_small(np.random.default_rng(1).integers(-1, 2, size=n), moduli)
+        e = ck.rns_of_small(np.random.default_rng(2).integers(-3, 4, size=n), moduli)
+
+        def run():
+            z = ck.rescale(BFV_SCHEME, ck.encrypt_zero_asymmetric(BFV_SCHEME, h_pk, u, e, e, moduli, logn), moduli,
+                           logn)
+            ck.bfv_add_plain_scaled(z[0], h_plain, moduli, t, logn)
+        return run
+
+    record("bfv_encrypt", "BFV public-key encryptions/s, " + shape, "encryptions/s", B, ms,
+           (2 * K + B * 2 * L) * limb + B * n * 8, "BFV encryption of batch-encoded plaintexts, " + shape,
+           "shc_encrypt_many", make_benc2, "one Encryptor::encrypt per call")
+    del eb
+    out = dev((B, n))
+    dw = ctx.decrypt_many_workspace_size(L, B)
+    db = wsb(dw)
+    ms = timed(lambda: ctx.decrypt_many(cts.data_ptr(), 2, B, sk.data_ptr(), L, out.data_ptr(), db.data_ptr(), dw,
+                                        st))
+    if not torch.equal(out, plain):
+        raise SystemExit("bench self-check failed: BFV decrypt(encrypt(m)) != m")
+    h_ct = cts[0].cpu().numpy().view(np.uint64).copy()
+    h_sk = sk.cpu().numpy().view(np.uint64).copy()
+
+    def make_bdec(kind):
+        s_ = ref_sess(kind, BFV_SCHEME, n, moduli, t)
+        if s_ is not None:
+            return lambda: s_.decrypt(h_ct, h_sk, L)
+        return lambda: ck.bfv_decrypt(h_ct, h_sk[:L], moduli, t, logn)
+
+    record("bfv_decrypt", "BFV decryptions/s, " + shape, "decryptions/s", B, ms, (B * 2 * L + L) * limb + B * n * 8,
+           "BFV decryption (scale and round) to plaintexts mod t, " + shape, "shc_decrypt_many", make_bdec,
+           "one Decryptor::decrypt per call")
+    vals_out, scratch = dev((B, n)), dev((B, n))
+
+    def decode_each():
+        for g in range(B):
+            ctx.batch_decode(out[g].data_ptr(), vals_out[g].data_ptr(), scratch.data_ptr(), st)
+
+    per_call_ms = timed(decode_each)
+    ms = timed(lambda: ctx.batch_decode_many(out.data_ptr(), B, vals_out.data_ptr(), scratch.data_ptr(), st))
+    if not torch.equal(vals_out, values):
+        raise SystemExit("bench self-check failed: BFV encode -> encrypt -> decrypt -> decode lost slots")
+    h_p = out[0].cpu().numpy().view(np.uint64).copy()
+
+    def make_bdec2(kind):
+        s_ = ref_sess(kind, BFV_SCHEME, n, moduli, t)
+        if s_ is not None:
+            return lambda: s_.batch_decode(h_p)
+        return lambda: ck.batch_decode(h_p, t, logn)
+
+    record("bfv_batch_decode", "BatchEncoder decodes/s, " + shape, "decodes/s", B, ms, B * 2 * n * 8,
+           "BatchEncoder decode of 8192 slots mod t, " + shape, "shc_batch_decode_many", make_bdec2,
+           "one BatchEncoder::decode per call")
+    ops["bfv_batch_decode"]["one_call_per_plaintext"] = {"value": B / (per_call_ms * 1e-3), "ms": per_call_ms,
+                                                         "entry": "shc_batch_decode"}
+    del cts, out, db, vals_out, plain, values
+    torch.cuda.empty_cache()
+    first = ops["ckks_encrypt"]
+    return {"metric": "per-operation throughput of the path's other entry points (rotation, rescale, encryption, "
+                      "decryption, encoders)", "value": first["value"], "unit": first["unit"],
+            "ms_per_step": first["ms_per_step"], "steps": steps, "n_gpus": 1, "warmup": warmup,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+            "data": "synthetic (seeded uniform residues and slots; device-generated keys for the round trips)",
+            "config": {"workload": "per-operation rates: CKKS N=2^16 L=24 and BFV N=8192, batches of 128",
+                       "parallelism": "single GPU"},
+            "ops": ops}
+
+
 SUB_KEYS = ("metric", "value", "unit", "ms_per_step", "steps", "config", "roofline", "step_roofline", "cpu_baseline")
 
 
 def bench_default(args):
     """The headline line (CKKS mul+relin at N=2^16, L=24, batch 128) with every other
     workload as a sub-object under "configs": the CPU-runnable BFV path (N=8192), the
-    batched NTT, the CKKS chain in a HIP graph, the hoisted rotation sums and the served
-    matvec mix -- each with its value, step time, dominant-kernel roofline and CPU
+    batched NTT, the CKKS chain in a HIP graph, the hoisted rotation sums, the served
+    matvec mix and one object per other entry point (rotation, rescale, encryption,
+    decryption, encoders) -- each with its value, step time, roofline and CPU
     baseline.  A sub-workload whose self-check fails stops the run; one that fails
     otherwise is reported with its error."""
     import copy
@@ -1378,11 +1699,16 @@ def bench_default(args):
     sub.steps, sub.warmup = args.sub_steps, min(args.warmup, 2)
     configs = {}
     for name, fn in (("bfv_n8192", bench_bfv8192), ("ntt", bench_ntt), ("chain", bench_chain),
-                     ("rotsum", bench_rotsum), ("serve", bench_serve)):
+                     ("rotsum", bench_rotsum), ("serve", bench_serve), ("primitives", bench_primitives)):
         torch.cuda.empty_cache()
         t0 = time.perf_counter()
         try:
             r = fn(sub)
+            if "ops" in r:  # one sub-object per operation
+                for k, v in r["ops"].items():
+                    configs[k] = v
+                print(f"[bench] {name} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+                continue
             d = {k: r[k] for k in SUB_KEYS if k in r}
             for k in ("batched", "plain", "capacity_per_s", "admission", "hoisted_over_separate", "graph_over_eager",
                       "latency_ms", "self_check", "per_level_graph_ms", "grouped_graph_ms", "path"):
@@ -1426,7 +1752,7 @@ def main():
         sys.exit(2)
     out = {"mulrelin": bench_default, "ntt": bench_ntt, "copy": bench_copy, "chain": bench_chain,
            "rotsum": bench_rotsum, "matvec": bench_matvec, "serve": bench_serve,
-           "bfv8192": bench_bfv8192, "bfv": bench_bfv}[args.workload](args)
+           "bfv8192": bench_bfv8192, "bfv": bench_bfv, "primitives": bench_primitives}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -272,6 +272,16 @@ int shc_encrypt_zero_asymmetric(const shc_context *ctx, const uint64_t *d_public
  * Replaces EncryptorCuda::encrypt (src/encryptor_cuda.cuh:123, src/encryptor_cuda.cu:162-237). */
 int shc_encrypt(const shc_context *ctx, const uint64_t *d_public_key, const uint64_t *d_plain, uint32_t limb_count,
                 uint64_t seed, uint64_t *d_out, void *d_workspace, size_t workspace_bytes, void *stream);
+/* `count` public-key encryptions in one pass: ciphertext g (d_out + g * 2 * L * N) is
+ * the one shc_encrypt gives for seed + g and plaintext d_plain + g * (BFV: N, CKKS:
+ * L * N) words (d_plain NULL: encryptions of zero).  The workspace needs room for one
+ * ciphertext at least; shc_encrypt_many_workspace_size(count) lets the whole batch
+ * share each launch, a smaller one runs it in slices.  The reference encrypts one
+ * ciphertext per EncryptorCuda::encrypt call (src/encryptor_cuda.cu:162-237). */
+int shc_encrypt_many_workspace_size(const shc_context *ctx, uint32_t limb_count, uint32_t count, size_t *bytes);
+int shc_encrypt_many(const shc_context *ctx, const uint64_t *d_public_key, const uint64_t *d_plain, uint32_t count,
+                     uint32_t limb_count, uint64_t seed, uint64_t *d_out, void *d_workspace, size_t workspace_bytes,
+                     void *stream);
 /* Symmetric encryption of zero with the secret key (utils/rlwe_cuda.cu:262-331). */
 int shc_encrypt_zero_symmetric(const shc_context *ctx, const uint64_t *d_secret_key, uint32_t limb_count,
                                uint64_t seed, uint64_t *d_out, void *stream);
@@ -283,12 +293,30 @@ int shc_decrypt_workspace_size(const shc_context *ctx, uint32_t limb_count, size
 int shc_decrypt(const shc_context *ctx, const uint64_t *d_ct, uint32_t ct_size, const uint64_t *d_secret_key,
                 uint32_t limb_count, uint64_t *d_out, void *d_workspace, size_t workspace_bytes, void *stream);
 
+/* `count` ciphertexts d_ct[count][size][L][N] decrypted in one pass into d_out[count][N]
+ * (BFV) or d_out[count][L][N] (CKKS), the words of shc_decrypt per ciphertext.  BFV
+ * needs size * L * N words of workspace at least (shc_decrypt_many_workspace_size(count)
+ * takes the batch at once); CKKS none.  The reference decrypts one ciphertext per
+ * DecryptorCuda::decrypt call (src/decryptor_cuda.cu:92-328). */
+int shc_decrypt_many_workspace_size(const shc_context *ctx, uint32_t limb_count, uint32_t count, size_t *bytes);
+int shc_decrypt_many(const shc_context *ctx, const uint64_t *d_ct, uint32_t ct_size, uint32_t count,
+                     const uint64_t *d_secret_key, uint32_t limb_count, uint64_t *d_out, void *d_workspace,
+                     size_t workspace_bytes, void *stream);
+
 /* BatchEncoder: d_values[count <= N] -> d_plain[N] mod t, and back (d_scratch: N words).
  * Replaces BatchEncoderCuda::encode / decode (src/batchencoder_cuda.cuh:61-75). */
 int shc_batch_encode(const shc_context *ctx, const uint64_t *d_values, uint32_t count, uint64_t *d_plain,
                      void *stream);
 int shc_batch_decode(const shc_context *ctx, const uint64_t *d_plain, uint64_t *d_values, uint64_t *d_scratch,
                      void *stream);
+/* Batches of plaintexts in one pass: plaintext g's `count` values at d_values + g *
+ * values_stride into d_plain[g][N]; decode d_plain[plaintexts][N] into
+ * d_values[plaintexts][N] through d_scratch[plaintexts][N].  The reference encodes and
+ * decodes one plaintext per BatchEncoderCuda call. */
+int shc_batch_encode_many(const shc_context *ctx, const uint64_t *d_values, uint32_t count, size_t values_stride,
+                          uint32_t plaintexts, uint64_t *d_plain, void *stream);
+int shc_batch_decode_many(const shc_context *ctx, const uint64_t *d_plain, uint32_t plaintexts, uint64_t *d_values,
+                          uint64_t *d_scratch, void *stream);
 
 /* CKKSEncoder on the device (fp64): d_values holds `count` (re, im) double pairs,
  * count <= N/2; d_plain is the NTT-form plaintext [L][N] at the level with

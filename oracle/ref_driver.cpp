@@ -230,6 +230,39 @@ int ref_decrypt(void *h, uint64_t L, const uint64_t *ct, const uint64_t *sk, uin
     } catch (const std::exception &e) { fprintf(stderr, "ref_decrypt: %s\n", e.what()); return -1; }
 }
 
+// Encryptor::encrypt (src/encryptor.cpp:172-233) with an injected key-level public key
+// ([2][K][N], NTT form).  plain: CKKS [L][N] NTT form (scale 2^40); BFV [N] mod t, at
+// the first data level (L = K - 1).  out: [2][L][N].  The reference draws u, e0, e1
+// from its own generator, so this call is timed, never compared word for word.
+int ref_encrypt(void *h, uint64_t L, const uint64_t *pk_words, const uint64_t *plain, uint64_t *out) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        auto key_parms = s.context->keyParmsID();
+        PublicKey pk;
+        pk.data().resize(*s.context, key_parms, 2);
+        pk.data().isNttForm() = true;
+        std::memcpy(pk.data().data(), pk_words, 2 * s.K * s.n * 8);
+        pk.parmsID() = key_parms;
+        Encryptor enc(*s.context, pk);
+        Plaintext p;
+        if (s.scheme == SchemeType::ckks) {
+            p.parmsID() = parmsIDZero;
+            p.resize(L * s.n);
+            std::memcpy(p.data(), plain, L * s.n * 8);
+            p.parmsID() = parms_for_level(s, L);
+            p.scale() = 1099511627776.0;  // 2^40
+        } else {
+            if (L + 1 != s.K) throw std::invalid_argument("BFV encrypts at the first data level");
+            p.resize(s.n);
+            std::memcpy(p.data(), plain, s.n * 8);
+        }
+        Ciphertext c;
+        enc.encrypt(p, c);
+        std::memcpy(out, c.data(), 2 * L * s.n * 8);
+        return 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_encrypt: %s\n", e.what()); return -1; }
+}
+
 // BatchEncoder::encode / decode (src/batchencoder.cpp:104-200)
 int ref_batch_encode(void *h, const uint64_t *values, uint64_t count, uint64_t *plain) {
     try {

@@ -457,6 +457,23 @@ int shc_encrypt(const shc_context *ctx, const uint64_t *pk, const uint64_t *plai
     });
 }
 
+int shc_encrypt_many_workspace_size(const shc_context *ctx, uint32_t limbs, uint32_t count, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = shc::encrypt_many_workspace(c, limbs, count);
+    });
+}
+
+int shc_encrypt_many(const shc_context *ctx, const uint64_t *pk, const uint64_t *plain, uint32_t count, uint32_t limbs,
+                     uint64_t seed, uint64_t *out, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::encrypt_many(c, pk, plain, count, limbs, seed, out, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
 int shc_encrypt_zero_symmetric(const shc_context *ctx, const uint64_t *sk, uint32_t limbs, uint64_t seed,
                                uint64_t *out, void *stream) {
     return guarded([&] {
@@ -484,6 +501,23 @@ int shc_decrypt(const shc_context *ctx, const uint64_t *ct, uint32_t size, const
     });
 }
 
+int shc_decrypt_many_workspace_size(const shc_context *ctx, uint32_t limbs, uint32_t count, size_t *bytes) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        *bytes = shc::decrypt_many_workspace(c, limbs, count);
+    });
+}
+
+int shc_decrypt_many(const shc_context *ctx, const uint64_t *ct, uint32_t size, uint32_t count, const uint64_t *sk,
+                     uint32_t limbs, uint64_t *out, void *ws, size_t ws_bytes, void *stream) {
+    return guarded([&] {
+        const auto &c = ctx_of(ctx);
+        check_limbs(c, limbs);
+        shc::decrypt_many(c, ct, size, count, sk, limbs, out, ws, ws_bytes, (hipStream_t)stream);
+    });
+}
+
 int shc_batch_encode(const shc_context *ctx, const uint64_t *values, uint32_t count, uint64_t *plain, void *stream) {
     return guarded([&] { shc::batch_encode(ctx_of(ctx), values, count, plain, (hipStream_t)stream); });
 }
@@ -491,6 +525,20 @@ int shc_batch_encode(const shc_context *ctx, const uint64_t *values, uint32_t co
 int shc_batch_decode(const shc_context *ctx, const uint64_t *plain, uint64_t *values, uint64_t *scratch,
                      void *stream) {
     return guarded([&] { shc::batch_decode(ctx_of(ctx), plain, values, scratch, (hipStream_t)stream); });
+}
+
+int shc_batch_encode_many(const shc_context *ctx, const uint64_t *values, uint32_t count, size_t values_stride,
+                          uint32_t plaintexts, uint64_t *plain, void *stream) {
+    return guarded([&] {
+        shc::batch_encode_many(ctx_of(ctx), values, count, values_stride, plaintexts, plain, (hipStream_t)stream);
+    });
+}
+
+int shc_batch_decode_many(const shc_context *ctx, const uint64_t *plain, uint32_t plaintexts, uint64_t *values,
+                          uint64_t *scratch, void *stream) {
+    return guarded([&] {
+        shc::batch_decode_many(ctx_of(ctx), plain, plaintexts, values, scratch, (hipStream_t)stream);
+    });
 }
 
 int shc_ckks_encode_workspace_size(const shc_context *ctx, size_t *bytes) {

@@ -48,6 +48,13 @@ void encrypt_zero_asymmetric(const Context &c, const uint64_t *pk, const uint64_
 // [N] mod t (coefficients), CKKS plain is [L][N] in NTT form; out: [2][L][N].
 void encrypt(const Context &c, const uint64_t *pk, const uint64_t *plain, uint32_t L, uint64_t seed, uint64_t *out,
              void *ws, size_t ws_bytes, hipStream_t st);
+// `count` encryptions: ciphertext g is encrypt()'s for seed + g and plain + g * (BFV: N,
+// CKKS: L * N) words (plain == nullptr: zeros), out [count][2][L][N].  The workspace
+// holds one ciphertext's samples and products at least; larger ones let more
+// ciphertexts share each launch (encrypt_many_workspace(c, L, count) takes all at once).
+size_t encrypt_many_workspace(const Context &c, uint32_t L, uint32_t count);
+void encrypt_many(const Context &c, const uint64_t *pk, const uint64_t *plain, uint32_t count, uint32_t L,
+                  uint64_t seed, uint64_t *out, void *ws, size_t ws_bytes, hipStream_t st);
 // c0 += Delta m with SEAL's rounding fix, at level L (src/utils/scalingvariant.cpp:48-86).
 void bfv_add_plain_scaled(const Context &c, uint64_t *c0, const uint64_t *plain, uint32_t L, hipStream_t st);
 // c0 +- Delta m (multiplyAdd / multiplySubPlainWithScalingVariant, scalingvariant.cpp:53-131).
@@ -58,10 +65,23 @@ void bfv_scale_add(const Context &c, uint64_t *c0, const uint64_t *plain, uint32
 size_t decrypt_workspace(const Context &c, uint32_t L);
 void decrypt(const Context &c, const uint64_t *ct, uint32_t size, const uint64_t *sk, uint32_t L, uint64_t *out,
              void *ws, size_t ws_bytes, hipStream_t st);
+// `count` ciphertexts [count][size][L][N] -> out [count][N] (BFV) or [count][L][N]
+// (CKKS); BFV needs at least size * L * N words of workspace (3 * L * N * count from
+// decrypt_many_workspace takes the batch at once), CKKS none.
+size_t decrypt_many_workspace(const Context &c, uint32_t L, uint32_t count);
+void decrypt_many(const Context &c, const uint64_t *ct, uint32_t size, uint32_t count, const uint64_t *sk, uint32_t L,
+                  uint64_t *out, void *ws, size_t ws_bytes, hipStream_t st);
 
 // BatchEncoder (src/batchencoder.cpp:104-200): values [count <= N] -> plain [N] mod t,
 // and back.  decode's output has N slots.
 void batch_encode(const Context &c, const uint64_t *values, uint32_t count, uint64_t *plain, hipStream_t st);
 void batch_decode(const Context &c, const uint64_t *plain, uint64_t *values, uint64_t *scratch, hipStream_t st);
+// Batches: plaintext g from values + g * values_stride (count values each) into
+// plain [plaintexts][N]; decode plain [plaintexts][N] into values [plaintexts][N]
+// through scratch [plaintexts][N].
+void batch_encode_many(const Context &c, const uint64_t *values, uint32_t count, size_t values_stride,
+                       uint32_t plaintexts, uint64_t *plain, hipStream_t st);
+void batch_decode_many(const Context &c, const uint64_t *plain, uint32_t plaintexts, uint64_t *values,
+                       uint64_t *scratch, hipStream_t st);
 
 }  // namespace shc

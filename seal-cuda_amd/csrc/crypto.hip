@@ -4,6 +4,7 @@
 #include "modarith.h"
 #include "ntt.h"
 #include "rns_ops.h"
+#include <algorithm>
 
 namespace shc {
 namespace {
@@ -27,7 +28,9 @@ __host__ __device__ __forceinline__ uint64_t stream_word(uint64_t seed, uint64_t
 }
 
 // Independent sub-stream seeds (u, e0, e1, per-digit keys, ...)
-inline uint64_t sub_seed(uint64_t seed, uint64_t k) { return mix64(seed ^ mix64(k + 0x5EA15EEDull)); }
+__host__ __device__ __forceinline__ uint64_t sub_seed(uint64_t seed, uint64_t k) {
+    return mix64(seed ^ mix64(k + 0x5EA15EEDull));
+}
 
 // ---- keyed streams: ChaCha20 (20 rounds, 256-bit key, 64-bit block counter, 64-bit
 // nonce; state words 12-15 = counter lo, hi, nonce lo, hi).  Word i of the stream of
@@ -196,13 +199,17 @@ __device__ __forceinline__ uint64_t div128_small(u128 x, uint64_t t, uint64_t r_
     return r >= t ? est + 1 : est;
 }
 
-// c0[l][c] += m Delta_l + floor((m (Q mod t) + floor((t+1)/2)) / t)   (mod q_l)
-__global__ void bfv_scale_add_kernel(uint64_t *__restrict__ c0, const uint64_t *__restrict__ plain, BehzDev B,
+// c0[l][c] += m Delta_l + floor((m (Q mod t) + floor((t+1)/2)) / t)   (mod q_l), for
+// `count` pairs (c0 + g * c0_stride, plain + g * n).
+__global__ void bfv_scale_add_kernel(uint64_t *__restrict__ c0_base, size_t c0_stride,
+                                     const uint64_t *__restrict__ plain_base, uint32_t count, BehzDev B,
                                      const DevPrime *__restrict__ qp, int logn, bool subtract) {
-    const size_t n = (size_t)1 << logn;
+    const size_t n = (size_t)1 << logn, total = (size_t)count << logn;
     const uint64_t t = B.t, half = (t + 1) >> 1;
-    for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < n; c += (size_t)gridDim.x * TPB) {
-        const uint64_t m = plain[c];
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t g = idx >> logn, c = idx & (n - 1);
+        uint64_t *__restrict__ c0 = c0_base + g * c0_stride;
+        const uint64_t m = plain_base[idx];
         u128 num = mul64x64(m, B.q_mod_t);
         add128(num, u128{half, 0});
         const uint64_t fix = div128_small(num, t, B.t_ratio_lo, B.t_ratio_hi);
@@ -215,31 +222,42 @@ __global__ void bfv_scale_add_kernel(uint64_t *__restrict__ c0, const uint64_t *
     }
 }
 
-// phase[l] = c0[l] + sum_{i>=1} ct_i[l] s^i (all operands in the same form)
+// phase[g][l] = c0[g][l] + sum_{k>=1} ct_k[g][l] s^k (all operands in the same form) for
+// `count` ciphertexts of `poly` = L * N words per polynomial: c0 of ciphertext g at
+// c0 + g * in_stride, its c_k at cs + g * in_stride + (k - 1) * c_stride, its phase at
+// phase + g * poly.  c0 == nullptr: the sum alone.
 __global__ void phase_kernel(const uint64_t *__restrict__ c0, const uint64_t *__restrict__ cs, size_t c_stride,
-                             uint32_t size, const uint64_t *__restrict__ sk, uint64_t *__restrict__ phase,
-                             const DevPrime *__restrict__ primes, int logn, size_t total) {
-    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < total; i += (size_t)gridDim.x * TPB) {
+                             size_t in_stride, uint32_t size, const uint64_t *__restrict__ sk,
+                             uint64_t *__restrict__ phase, const DevPrime *__restrict__ primes, int logn, size_t poly,
+                             size_t total) {
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t g = idx / poly, i = idx - g * poly, in = g * in_stride + i;
         const DevPrime &P = primes[i >> logn];
         const uint64_t s = sk[i];
         uint64_t acc = 0, sp = s;
         for (uint32_t k = 1; k < size; k++) {
-            acc = addmod(acc, mulmod(cs[(k - 1) * c_stride + i], sp, P.q, P.ratio_lo, P.ratio_hi), P.q);
+            acc = addmod(acc, mulmod(cs[(k - 1) * c_stride + in], sp, P.q, P.ratio_lo, P.ratio_hi), P.q);
             sp = mulmod(sp, s, P.q, P.ratio_lo, P.ratio_hi);
         }
-        phase[i] = c0 ? addmod(acc, c0[i], P.q) : acc;
+        phase[idx] = c0 ? addmod(acc, c0[in], P.q) : acc;
     }
 }
 
-// RNSTool::decryptScaleAndRound (src/utils/rns.cpp:1039-1095) per coefficient
-__global__ void scale_round_kernel(const uint64_t *__restrict__ phase, uint64_t *__restrict__ out, BehzDev B,
+// RNSTool::decryptScaleAndRound (src/utils/rns.cpp:1039-1095) per coefficient of
+// phase[h] + c0[h] (coefficient form, L = B.L limbs) for `count` ciphertexts: phase of
+// ciphertext h at phase + h * L * N, its c0 at c0 + h * c0_stride, output at out + h * N.
+__global__ void scale_round_kernel(const uint64_t *__restrict__ phase, const uint64_t *__restrict__ c0,
+                                   size_t c0_stride, uint64_t *__restrict__ out, uint32_t count, BehzDev B,
                                    const DevPrime *__restrict__ qp, int logn) {
-    const size_t n = (size_t)1 << logn;
+    const size_t n = (size_t)1 << logn, total = (size_t)count << logn;
     const uint64_t t = B.t, g = B.gamma;
-    for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < n; c += (size_t)gridDim.x * TPB) {
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t h = idx >> logn, c = idx & (n - 1);
+        const uint64_t *ph = phase + h * B.L * n, *x0 = c0 + h * c0_stride;
         u128 at{0, 0}, ag{0, 0};
         for (uint32_t j = 0; j < B.L; j++) {
-            const uint64_t y = shoup(phase[j * n + c], B.tg_qhat_inv[2 * j], B.tg_qhat_inv[2 * j + 1], qp[j].q);
+            const uint64_t v = addmod(ph[j * n + c], x0[j * n + c], qp[j].q);
+            const uint64_t y = shoup(v, B.tg_qhat_inv[2 * j], B.tg_qhat_inv[2 * j + 1], qp[j].q);
             add128(at, mul64x64(y, B.qhat_mod_t[j]));
             add128(ag, mul64x64(y, B.qhat_mod_g[j]));
         }
@@ -251,7 +269,7 @@ __global__ void scale_round_kernel(const uint64_t *__restrict__ phase, uint64_t 
         if (rg > (g >> 1)) d = addmod(rt, barrett64(g - rg, t, B.t_ratio_hi), t);
         else d = submod(rt, barrett64(rg, t, B.t_ratio_hi), t);
         if (d) d = shoup(d, B.inv_gamma_t, B.inv_gamma_t_shoup, t);
-        out[c] = d;
+        out[idx] = d;
     }
 }
 
@@ -268,16 +286,178 @@ __device__ __forceinline__ uint32_t slot_index(uint32_t i, int logn) {
     return __brev((uint32_t)idx) >> (32 - logn);
 }
 
-__global__ void batch_scatter_kernel(const uint64_t *__restrict__ values, uint32_t count, uint64_t *__restrict__ plain,
-                                     int logn) {
-    const uint32_t n = 1u << logn;
-    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB)
-        plain[slot_index(i, logn)] = i < count ? values[i] : 0;
+// plain[g][slot_index(i)] = values[g * stride + i] (0 past count), g < plaintexts
+__global__ void batch_scatter_kernel(const uint64_t *__restrict__ values, uint32_t count, size_t stride,
+                                     uint64_t *__restrict__ plain, int logn, size_t total) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t g = idx >> logn;
+        const uint32_t i = (uint32_t)(idx & (n - 1));
+        plain[g * n + slot_index(i, logn)] = i < count ? values[g * stride + i] : 0;
+    }
 }
 
-__global__ void batch_gather_kernel(const uint64_t *__restrict__ ntt, uint64_t *__restrict__ values, int logn) {
-    const uint32_t n = 1u << logn;
-    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) values[i] = ntt[slot_index(i, logn)];
+// values[g][i] = ntt[g][slot_index(i)]
+__global__ void batch_gather_kernel(const uint64_t *__restrict__ ntt, uint64_t *__restrict__ values, int logn,
+                                    size_t total) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t g = idx >> logn;
+        values[idx] = ntt[g * n + slot_index((uint32_t)(idx & (n - 1)), logn)];
+    }
+}
+
+
+// ---- batched public-key encryption ------------------------------------------------
+// Ciphertext g of a batch is the one encrypt() gives for the seed seed + g: u from the
+// stream of sub_seed(seed + g, 0) (ternary), e0 and e1 from sub_seed(seed + g, 1 / 2)
+// (centred binomial).  With L1 = L + 1 limbs before the divide-and-round by p = q_L
+// (the special prime at the top level), the CKKS words are
+//   c_j[i] = p^-1 (pk_j[i] NTT_i(u) + NTT_i(e_j - r_j))  (mod q_i, i < L),
+//   r_j    = ([INTT_p(pk_j[L] NTT_p(u)) + e_j + floor(p/2)]_p mod q_i) - floor(p/2),
+// which equals dividing-and-rounding pk_j NTT(u) + NTT(e_j) at L1 limbs (the transform
+// is linear modulo each q_i): L1 + 2 + 2L transforms per ciphertext instead of
+// 3 L1 + 2 + 2L.  BFV stays in coefficient form: c_j[i] = p^-1 (INTT_i(pk_j[i] NTT_i(u))
+// + e_j - r_j), r_j from limb L of the same sum.
+
+// u lifted to the L1 limbs of U[g]; e0, e1 as signed integers in Es[g][0 / 1].
+template <bool KEYED>
+__global__ void sample_encrypt_kernel(SamplerKey key, uint64_t seed, const DevPrime *__restrict__ primes, uint32_t L1,
+                                      int logn, size_t total, uint64_t *__restrict__ U, int64_t *__restrict__ Es) {
+    const size_t n = (size_t)1 << logn;
+    const size_t units = KEYED ? (n + 7) / 8 : n;  // per ciphertext: keystream blocks or coefficients
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t g = idx / units, b = idx - g * units;
+        uint64_t *Ug = U + g * L1 * n;
+        int64_t *Eg = Es + g * 2 * n;
+        for (int k = 0; k < 3; k++) {
+            const uint64_t s = sub_seed(seed + g, (uint64_t)k);
+            const int kind = k == 0 ? SAMPLE_TERNARY : SAMPLE_CBD;
+            if (KEYED) {
+                uint32_t ks[16];
+                chacha20_block(key, b, s, ks);
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    const size_t c = b * 8 + w;
+                    if (c >= n) break;
+                    const int64_t v = small_from_word(kind, block_word(ks, w));
+                    if (k == 0) {
+                        for (uint32_t l = 0; l < L1; l++) {
+                            const uint64_t q = primes[l].q;
+                            Ug[(size_t)l * n + c] = v < 0 ? q - (uint64_t)(-v) : (uint64_t)v;
+                        }
+                    } else {
+                        Eg[(size_t)(k - 1) * n + c] = v;
+                    }
+                }
+            } else {
+                const int64_t v = small_from_word(kind, stream_word(s, b));
+                if (k == 0) {
+                    for (uint32_t l = 0; l < L1; l++) {
+                        const uint64_t q = primes[l].q;
+                        Ug[(size_t)l * n + b] = v < 0 ? q - (uint64_t)(-v) : (uint64_t)v;
+                    }
+                } else {
+                    Eg[(size_t)(k - 1) * n + b] = v;
+                }
+            }
+        }
+    }
+}
+
+// dst[r][k] = src[r * src_stride + k] for rows of `row` words (total = rows * row)
+__global__ void gather_rows_kernel(const uint64_t *__restrict__ src, size_t src_stride, uint64_t *__restrict__ dst,
+                                   size_t row, size_t total) {
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t r = idx / row;
+        dst[idx] = src[r * src_stride + (idx - r * row)];
+    }
+}
+
+__device__ __forceinline__ uint64_t lift_small(int64_t v, uint64_t q) { return v < 0 ? q - (uint64_t)(-v) : (uint64_t)v; }
+
+// SEAL's rounding correction of a divide-and-round by p, from the coefficient-form last
+// limb x (< p): ([x + floor(p/2)]_p mod q_i) - floor(p/2) mod q_i.
+__device__ __forceinline__ uint64_t divround_corr(uint64_t x, uint64_t p, const DevPrime &Q, const DivRoundConst &d) {
+    uint64_t v = x + (p >> 1);
+    v = v >= p ? v - p : v;
+    return submod(barrett64(v, Q.q, Q.ratio_hi), d.half_mod, Q.q);
+}
+
+// P[g][j][l] = pk_j[l0 + l] * U[g][l0 + l] (mod q_{l0 + l}) for l < lw; U and pk limbs
+// in NTT form, U with L1 limbs per ciphertext, P with lw limbs per polynomial.
+__global__ void enc_products_kernel(const uint64_t *__restrict__ pk, uint32_t K, const uint64_t *__restrict__ U,
+                                    uint32_t L1, uint32_t l0, uint32_t lw, uint64_t *__restrict__ P,
+                                    const DevPrime *__restrict__ primes, int logn, size_t total) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t c = idx & (n - 1), r = idx >> logn;
+        const uint32_t l = (uint32_t)(r % lw) + l0;
+        const size_t gj = r / lw, g = gj >> 1, j = gj & 1;
+        const DevPrime &Q = primes[l];
+        P[idx] = mulmod(pk[(j * K + l) * n + c], U[(g * L1 + l) * n + c], Q.q, Q.ratio_lo, Q.ratio_hi);
+    }
+}
+
+// CKKS, before the forward transform: out[g][j][i] = e_j - r_j (mod q_i), i < L (r_j = 0
+// when nothing is dropped); Pl[g][j] = INTT_p(pk_j[L] NTT_p(u)), coefficient form.
+__global__ void enc_error_kernel(const int64_t *__restrict__ Es, const uint64_t *__restrict__ Pl,
+                                 uint64_t *__restrict__ out, const DevPrime *__restrict__ primes,
+                                 const DivRoundConst *__restrict__ dc, uint64_t p, uint32_t L, int logn, size_t total) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t c = idx & (n - 1), r = idx >> logn;
+        const uint32_t i = (uint32_t)(r % L);
+        const size_t gj = r / L;
+        const DevPrime &Q = primes[i];
+        const int64_t e = Es[gj * n + c];
+        uint64_t x = lift_small(e, Q.q);
+        if (Pl) x = submod(x, divround_corr(addmod(Pl[gj * n + c], lift_small(e, p), p), p, Q, dc[i]), Q.q);
+        out[idx] = x;
+    }
+}
+
+// CKKS, after the forward transform of out: out[g][j][i] = (pk_j[i] U[g][i] + out) times
+// p^-1 when a prime was dropped, plus plain[g][i] for j = 0 when plain != nullptr.
+__global__ void enc_finish_ckks_kernel(const uint64_t *__restrict__ pk, uint32_t K, const uint64_t *__restrict__ U,
+                                       uint32_t L1, uint64_t *__restrict__ out, const uint64_t *__restrict__ plain,
+                                       const DevPrime *__restrict__ primes, const DivRoundConst *__restrict__ dc,
+                                       bool drop, uint32_t L, int logn, size_t total) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t c = idx & (n - 1), r = idx >> logn;
+        const uint32_t i = (uint32_t)(r % L);
+        const size_t gj = r / L, g = gj >> 1, j = gj & 1;
+        const DevPrime &Q = primes[i];
+        uint64_t x = addmod(mulmod(pk[(j * K + i) * n + c], U[(g * L1 + i) * n + c], Q.q, Q.ratio_lo, Q.ratio_hi),
+                            out[idx], Q.q);
+        if (drop) x = shoup(x, dc[i].inv, dc[i].inv_shoup, Q.q);
+        if (plain && j == 0) x = addmod(x, plain[(g * L + i) * n + c], Q.q);
+        out[idx] = x;
+    }
+}
+
+// BFV: out[g][j][i] = (P[g][j][i] + e_j - r_j) p^-1 (mod q_i), P = INTT(pk_j NTT(u)) with
+// L1 limbs per polynomial; without a dropped prime, P[g][j][i] + e_j.
+__global__ void enc_finish_bfv_kernel(const uint64_t *__restrict__ P, const int64_t *__restrict__ Es,
+                                      uint64_t *__restrict__ out, const DevPrime *__restrict__ primes,
+                                      const DivRoundConst *__restrict__ dc, uint64_t p, bool drop, uint32_t L,
+                                      uint32_t L1, int logn, size_t total) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const size_t c = idx & (n - 1), r = idx >> logn;
+        const uint32_t i = (uint32_t)(r % L);
+        const size_t gj = r / L;
+        const DevPrime &Q = primes[i];
+        const int64_t e = Es[gj * n + c];
+        const uint64_t *Pg = P + gj * L1 * n;
+        uint64_t x = addmod(Pg[(size_t)i * n + c], lift_small(e, Q.q), Q.q);
+        if (drop) {
+            const uint64_t last = addmod(Pg[(size_t)L * n + c], lift_small(e, p), p);
+            x = shoup(submod(x, divround_corr(last, p, Q, dc[i]), Q.q), dc[i].inv, dc[i].inv_shoup, Q.q);
+        }
+        out[idx] = x;
+    }
 }
 
 }  // namespace
@@ -416,7 +596,7 @@ void encrypt_zero_asymmetric(const Context &c, const uint64_t *pk, const uint64_
 
 void bfv_scale_add(const Context &c, uint64_t *c0, const uint64_t *plain, uint32_t L, bool subtract, hipStream_t st) {
     const BehzDev &B = c.behz(L).dev;
-    bfv_scale_add_kernel<<<blocks_for(c.n), TPB, 0, st>>>(c0, plain, B, c.primes(), c.logn, subtract);
+    bfv_scale_add_kernel<<<blocks_for(c.n), TPB, 0, st>>>(c0, 0, plain, 1, B, c.primes(), c.logn, subtract);
     launch_check("bfv_scale_add");
 }
 
@@ -424,59 +604,180 @@ void bfv_add_plain_scaled(const Context &c, uint64_t *c0, const uint64_t *plain,
     bfv_scale_add(c, c0, plain, L, false, st);
 }
 
+namespace {
+
+// Words of encrypt_many's workspace per ciphertext: U [L1][N], e0 and e1 [2][N], and
+// the products P ([2][N] of the dropped limb for CKKS, [2][L1][N] for BFV).
+size_t encrypt_many_words(const Context &c, uint32_t L) {
+    const size_t L1 = upper_limbs(c, L);
+    return (L1 + 2 + (c.scheme == SCHEME_BFV ? 2 * L1 : 2)) * c.n;
+}
+
+void encrypt_chunk(const Context &c, const uint64_t *pk, const uint64_t *plain, uint32_t m, uint32_t L, uint64_t seed,
+                   uint64_t *out, void *ws, hipStream_t st) {
+    const uint32_t L1 = upper_limbs(c, L), K = (uint32_t)c.key_count();
+    const bool drop = L1 != L, ckks = c.scheme == SCHEME_CKKS;
+    const size_t n = c.n;
+    const int logn = c.logn;
+    const DevPrime *primes = c.primes();
+    uint64_t *U = static_cast<uint64_t *>(ws);
+    int64_t *Es = reinterpret_cast<int64_t *>(U + (size_t)m * L1 * n);
+    uint64_t *P = reinterpret_cast<uint64_t *>(Es + (size_t)m * 2 * n);
+    // drop: divide-and-round by q_L (row L1 of the rescale constants, entries i < L)
+    const DivRoundConst *dc = drop ? c.rescale_consts(L1) : nullptr;
+    const uint64_t p = drop ? c.moduli[L] : 0;
+
+    SamplerKey key{};
+    for (int i = 0; i < 8; i++) key.k[i] = c.sampler_key[i];
+    if (c.sampler_keyed) {
+        const size_t total = (size_t)m * ((n + 7) / 8);
+        sample_encrypt_kernel<true><<<blocks_for(total), TPB, 0, st>>>(key, seed, primes, L1, logn, total, U, Es);
+    } else {
+        const size_t total = (size_t)m * n;
+        sample_encrypt_kernel<false><<<blocks_for(total), TPB, 0, st>>>(key, seed, primes, L1, logn, total, U, Es);
+    }
+    launch_check("sample_encrypt");
+    ntt_forward(U, m * L1, L1, primes, logn, NTT_FULL, st);
+    const size_t out_words = (size_t)m * 2 * L * n;
+    if (ckks) {
+        if (drop) {
+            const size_t pw = (size_t)m * 2 * n;
+            enc_products_kernel<<<blocks_for(pw), TPB, 0, st>>>(pk, K, U, L1, L, 1, P, primes, logn, pw);
+            launch_check("enc_products");
+            ntt_inverse(P, m * 2, 1, primes + L, logn, NTT_FULL, st);
+        }
+        enc_error_kernel<<<blocks_for(out_words), TPB, 0, st>>>(Es, drop ? P : nullptr, out, primes, dc, p, L, logn,
+                                                                out_words);
+        launch_check("enc_error");
+        ntt_forward(out, m * 2 * L, L, primes, logn, NTT_FULL, st);
+        enc_finish_ckks_kernel<<<blocks_for(out_words), TPB, 0, st>>>(pk, K, U, L1, out, plain, primes, dc, drop, L,
+                                                                      logn, out_words);
+        launch_check("enc_finish");
+        return;
+    }
+    const size_t pw = (size_t)m * 2 * L1 * n;
+    enc_products_kernel<<<blocks_for(pw), TPB, 0, st>>>(pk, K, U, L1, 0, L1, P, primes, logn, pw);
+    launch_check("enc_products");
+    ntt_inverse(P, m * 2 * L1, L1, primes, logn, NTT_FULL, st);
+    enc_finish_bfv_kernel<<<blocks_for(out_words), TPB, 0, st>>>(P, Es, out, primes, dc, p, drop, L, L1, logn,
+                                                                 out_words);
+    launch_check("enc_finish");
+    if (plain) {
+        bfv_scale_add_kernel<<<blocks_for((size_t)m * n), TPB, 0, st>>>(out, (size_t)2 * L * n, plain, m,
+                                                                        c.behz(L).dev, primes, logn, false);
+        launch_check("bfv_scale_add");
+    }
+}
+
+}  // namespace
+
+size_t encrypt_many_workspace(const Context &c, uint32_t L, uint32_t count) {
+    return (size_t)(count ? count : 1) * encrypt_many_words(c, L) * 8;
+}
+
+void encrypt_many(const Context &c, const uint64_t *pk, const uint64_t *plain, uint32_t count, uint32_t L,
+                  uint64_t seed, uint64_t *out, void *ws, size_t ws_bytes, hipStream_t st) {
+    // Encryptor::encryptInternal (src/encryptor.cpp:152-233), one seed per ciphertext
+    if (L < 1 || L > c.key_count()) fail(ERR_INVALID_ARGUMENT, "level out of range for encryption");
+    if (!count) return;
+    const size_t per = encrypt_many_words(c, L) * 8;
+    if (!ws || ws_bytes < per) fail(ERR_INVALID_ARGUMENT, "workspace too small for encryption");
+    const uint32_t chunk = (uint32_t)std::min<size_t>(count, ws_bytes / per);
+    const size_t out_words = (size_t)2 * L * c.n, plain_words = c.scheme == SCHEME_BFV ? c.n : (size_t)L * c.n;
+    for (uint32_t g0 = 0; g0 < count; g0 += chunk) {
+        const uint32_t m = std::min(chunk, count - g0);
+        encrypt_chunk(c, pk, plain ? plain + g0 * plain_words : nullptr, m, L, seed + g0, out + g0 * out_words, ws, st);
+    }
+}
+
 void encrypt(const Context &c, const uint64_t *pk, const uint64_t *plain, uint32_t L, uint64_t seed, uint64_t *out,
              void *ws, size_t ws_bytes, hipStream_t st) {
     // Encryptor::encryptInternal (src/encryptor.cpp:152-233)
+    if (L < 1 || L > c.key_count()) fail(ERR_INVALID_ARGUMENT, "level out of range for encryption");
     if (!ws || ws_bytes < encrypt_workspace(c, L)) fail(ERR_INVALID_ARGUMENT, "workspace too small for encryption");
-    const uint32_t L1 = upper_limbs(c, L);
-    uint64_t *u = static_cast<uint64_t *>(ws), *e0 = u + (size_t)L1 * c.n, *e1 = e0 + (size_t)L1 * c.n;
-    sample_poly(c, SAMPLE_TERNARY, sub_seed(seed, 0), c.primes(), L1, u, st);
-    sample_poly(c, SAMPLE_CBD, sub_seed(seed, 1), c.primes(), L1, e0, st);
-    sample_poly(c, SAMPLE_CBD, sub_seed(seed, 2), c.primes(), L1, e1, st);
-    encrypt_zero_asymmetric(c, pk, u, e0, e1, L, out, ws, ws_bytes, st);
-    if (!plain) return;
-    if (c.scheme == SCHEME_BFV) bfv_add_plain_scaled(c, out, plain, L, st);
-    else limb_op(OP_ADD, out, c.n, plain, c.n, out, c.n, c.primes(), L, c.logn, st);
+    encrypt_many(c, pk, plain, 1, L, seed, out, ws, ws_bytes, st);
 }
 
 size_t decrypt_workspace(const Context &c, uint32_t L) { return (size_t)4 * L * c.n * 8; }
 
-void decrypt(const Context &c, const uint64_t *ct, uint32_t size, const uint64_t *sk, uint32_t L, uint64_t *out,
-             void *ws, size_t ws_bytes, hipStream_t st) {
+size_t decrypt_many_workspace(const Context &c, uint32_t L, uint32_t count) {
+    return c.scheme == SCHEME_CKKS ? 0 : (size_t)(count ? count : 1) * 3 * L * c.n * 8;
+}
+
+void decrypt_many(const Context &c, const uint64_t *ct, uint32_t size, uint32_t count, const uint64_t *sk, uint32_t L,
+                  uint64_t *out, void *ws, size_t ws_bytes, hipStream_t st) {
+    // Decryptor::decrypt (src/decryptor.cpp:79-153), one call per ciphertext there
     if (size < 2 || size > 3) fail(ERR_INVALID_ARGUMENT, "ciphertext size must be 2 or 3");
-    const size_t n = c.n, poly = (size_t)L * n, total = poly;
+    if (!count) return;
+    const size_t n = c.n, poly = (size_t)L * n, ct_words = size * poly;
     const DevPrime *primes = c.primes();
     if (c.scheme == SCHEME_CKKS) {
-        phase_kernel<<<blocks_for(total), TPB, 0, st>>>(ct, ct + poly, poly, size, sk, out, primes, c.logn, total);
+        const size_t total = (size_t)count * poly;
+        phase_kernel<<<blocks_for(total), TPB, 0, st>>>(ct, ct + poly, poly, ct_words, size, sk, out, primes, c.logn,
+                                                        poly, total);
         launch_check("phase");
         return;
     }
-    if (!ws || ws_bytes < decrypt_workspace(c, L)) fail(ERR_INVALID_ARGUMENT, "workspace too small for decryption");
-    uint64_t *tmp = static_cast<uint64_t *>(ws), *phase = tmp + 2 * poly;
-    SHC_HIP(hipMemcpyAsync(tmp, ct + poly, (size - 1) * poly * 8, hipMemcpyDeviceToDevice, st));
-    ntt_forward(tmp, (size - 1) * L, L, primes, c.logn, NTT_FULL, st);
-    phase_kernel<<<blocks_for(total), TPB, 0, st>>>(nullptr, tmp, poly, size, sk, phase, primes, c.logn, total);
-    launch_check("phase");
-    ntt_inverse(phase, L, L, primes, c.logn, NTT_FULL, st);
-    limb_op(OP_ADD, phase, n, ct, n, phase, n, primes, L, c.logn, st);
-    scale_round_kernel<<<blocks_for(n), TPB, 0, st>>>(phase, out, c.behz(L).dev, primes, c.logn);
-    launch_check("scale_round");
+    // BFV: phase = INTT(sum_k NTT(c_k) s^k) + c0, then scale-and-round, per chunk of m
+    const size_t per = ct_words * 8;  // (size - 1) transformed polynomials + the phase
+    if (!ws || ws_bytes < per) fail(ERR_INVALID_ARGUMENT, "workspace too small for decryption");
+    const uint32_t chunk = (uint32_t)std::min<size_t>(count, ws_bytes / per);
+    const BehzDev &B = c.behz(L).dev;
+    for (uint32_t g0 = 0; g0 < count; g0 += chunk) {
+        const uint32_t m = std::min(chunk, count - g0);
+        const uint64_t *src = ct + g0 * ct_words;
+        uint64_t *tmp = static_cast<uint64_t *>(ws), *phase = tmp + (size_t)m * (size - 1) * poly;
+        const size_t tw = (size - 1) * poly, gw = (size_t)m * tw;
+        gather_rows_kernel<<<blocks_for(gw), TPB, 0, st>>>(src + poly, ct_words, tmp, tw, gw);
+        launch_check("gather_rows");
+        ntt_forward(tmp, m * (size - 1) * L, L, primes, c.logn, NTT_FULL, st);
+        const size_t total = (size_t)m * poly;
+        phase_kernel<<<blocks_for(total), TPB, 0, st>>>(nullptr, tmp, poly, (size - 1) * poly, size, sk, phase, primes,
+                                                        c.logn, poly, total);
+        launch_check("phase");
+        ntt_inverse(phase, m * L, L, primes, c.logn, NTT_FULL, st);
+        scale_round_kernel<<<blocks_for((size_t)m * n), TPB, 0, st>>>(phase, src, ct_words, out + g0 * n, m, B, primes,
+                                                                      c.logn);
+        launch_check("scale_round");
+    }
+}
+
+void decrypt(const Context &c, const uint64_t *ct, uint32_t size, const uint64_t *sk, uint32_t L, uint64_t *out,
+             void *ws, size_t ws_bytes, hipStream_t st) {
+    if (c.scheme == SCHEME_BFV && (!ws || ws_bytes < decrypt_workspace(c, L)))
+        fail(ERR_INVALID_ARGUMENT, "workspace too small for decryption");
+    decrypt_many(c, ct, size, 1, sk, L, out, ws, ws_bytes, st);
+}
+
+void batch_encode_many(const Context &c, const uint64_t *values, uint32_t count, size_t values_stride,
+                       uint32_t plaintexts, uint64_t *plain, hipStream_t st) {
+    if (!c.plain_prime()) fail(ERR_LOGIC, "encryption parameters are not valid for batching");
+    if (count > c.n) fail(ERR_INVALID_ARGUMENT, "values_matrix size is too large");
+    if (plaintexts > 1 && values_stride < count) fail(ERR_INVALID_ARGUMENT, "values stride is below the value count");
+    if (!plaintexts) return;
+    const size_t total = (size_t)plaintexts * c.n;
+    batch_scatter_kernel<<<blocks_for(total), TPB, 0, st>>>(values, count, values_stride, plain, c.logn, total);
+    launch_check("batch_scatter");
+    ntt_inverse(plain, plaintexts, 1, c.plain_prime(), c.logn, NTT_FULL, st);
+}
+
+void batch_decode_many(const Context &c, const uint64_t *plain, uint32_t plaintexts, uint64_t *values,
+                       uint64_t *scratch, hipStream_t st) {
+    if (!c.plain_prime()) fail(ERR_LOGIC, "encryption parameters are not valid for batching");
+    if (!plaintexts) return;
+    const size_t total = (size_t)plaintexts * c.n;
+    SHC_HIP(hipMemcpyAsync(scratch, plain, total * 8, hipMemcpyDeviceToDevice, st));
+    ntt_forward(scratch, plaintexts, 1, c.plain_prime(), c.logn, NTT_FULL, st);
+    batch_gather_kernel<<<blocks_for(total), TPB, 0, st>>>(scratch, values, c.logn, total);
+    launch_check("batch_gather");
 }
 
 void batch_encode(const Context &c, const uint64_t *values, uint32_t count, uint64_t *plain, hipStream_t st) {
-    if (!c.plain_prime()) fail(ERR_LOGIC, "encryption parameters are not valid for batching");
-    if (count > c.n) fail(ERR_INVALID_ARGUMENT, "values_matrix size is too large");
-    batch_scatter_kernel<<<blocks_for(c.n), TPB, 0, st>>>(values, count, plain, c.logn);
-    launch_check("batch_scatter");
-    ntt_inverse(plain, 1, 1, c.plain_prime(), c.logn, NTT_FULL, st);
+    batch_encode_many(c, values, count, count, 1, plain, st);
 }
 
 void batch_decode(const Context &c, const uint64_t *plain, uint64_t *values, uint64_t *scratch, hipStream_t st) {
-    if (!c.plain_prime()) fail(ERR_LOGIC, "encryption parameters are not valid for batching");
-    SHC_HIP(hipMemcpyAsync(scratch, plain, c.n * 8, hipMemcpyDeviceToDevice, st));
-    ntt_forward(scratch, 1, 1, c.plain_prime(), c.logn, NTT_FULL, st);
-    batch_gather_kernel<<<blocks_for(c.n), TPB, 0, st>>>(scratch, values, c.logn);
-    launch_check("batch_gather");
+    batch_decode_many(c, plain, 1, values, scratch, st);
 }
 
 }  // namespace shc

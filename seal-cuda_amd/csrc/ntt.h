@@ -79,6 +79,9 @@ bool ntt_modup_keyprod(const uint64_t *d, size_t d_ct_stride, uint64_t *T, const
 // with s = S[ct][k] (special limb, coefficient form) and P = [ct][k][L][N].
 //   CKKS: buf = U scratch [count][2][L][N]; P in NTT form (u is transformed on the way).
 //   BFV:  buf = P itself (coefficient form after the in-place inverse transform).
+// p_lpp != 0 (CKKS only): P holds p_lpp limbs per polynomial, polynomial (ct, k) at
+// ct * p_ct_stride + k * p_lpp * N words -- a ciphertext with a limb above the L kept
+// ones, divided and rounded by its last prime (a rescale; folded).
 struct ModDownArgs {
     const uint64_t *S;
     const uint64_t *P;
@@ -88,9 +91,16 @@ struct ModDownArgs {
     uint64_t p;
     uint32_t L;
     bool folded;
+    size_t p_ct_stride = 0;
+    uint32_t p_lpp = 0;
 };
 bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &args, uint32_t count, const DevPrime *primes, int logn,
                  hipStream_t stream);
+// CKKS form over `polys` polynomials (limb (i, j), j = 2 ct + k, may be odd): the
+// rescale of polys contiguous polynomials is one call with ct_stride = 2 L N and
+// p_ct_stride = 2 p_lpp N.
+bool ntt_moddown_polys(uint64_t *U, const ModDownArgs &args, uint32_t polys, const DevPrime *primes, int logn,
+                       hipStream_t stream);
 
 // ---- grouped launches (ciphertexts at several levels, one launch per stage) ----------
 // At most MAX_GROUPS groups per call; every group is one level's batch with its own

@@ -780,6 +780,14 @@ __device__ __forceinline__ uint64_t moddown_prep(uint64_t s, uint64_t p, const D
     return submod(barrett64(v, Q.q, Q.ratio_hi), d.half_mod, Q.q);
 }
 
+// Word (limb i of polynomial j, coefficient off) of a ModDown's P.
+template <int LOGN>
+__device__ __forceinline__ uint64_t moddown_p(const ModDownArgs &a, uint32_t i, uint32_t j, size_t off) {
+    if (a.p_lpp)  // uniform
+        return a.P[(size_t)(j >> 1) * a.p_ct_stride + ((size_t)((j & 1) * a.p_lpp + i) << LOGN) + off];
+    return a.P[((size_t)(j * a.L + i) << LOGN) + off];
+}
+
 template <int LOGN>
 __device__ __forceinline__ void moddown_add(const ModDownArgs &a, uint32_t i, uint32_t j, size_t off, uint64_t pv,
                                             uint64_t u, const DevPrime &Q) {
@@ -810,7 +818,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_moddown_kernel(const
         blockIdx.x, primes, a.L, 0, jn, [&](uint32_t i, uint32_t j, size_t off) { return U[((size_t)(j * a.L + i) << LOGN) + off]; },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             const DevPrime &Q = primes[i];
-            moddown_add<LOGN>(a, i, j, off, a.P[((size_t)(j * a.L + i) << LOGN) + off], finish<false, true>(x, Q), Q);
+            moddown_add<LOGN>(a, i, j, off, moddown_p<LOGN>(a, i, j, off), finish<false, true>(x, Q), Q);
         });
 }
 
@@ -852,7 +860,7 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_moddown_grouped_kern
         [&](uint32_t i, uint32_t j, size_t off) { return U[((size_t)(j * a.L + i) << LOGN) + off]; },
         [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
             const DevPrime &Q = primes[i];
-            moddown_add<LOGN>(a, i, j, off, a.P[((size_t)(j * a.L + i) << LOGN) + off], finish<false, true>(x, Q), Q);
+            moddown_add<LOGN>(a, i, j, off, moddown_p<LOGN>(a, i, j, off), finish<false, true>(x, Q), Q);
         });
 }
 
@@ -870,14 +878,13 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_moddown_kernel(const
 }
 
 template <int LOGN>
-void launch_moddown(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t count, const DevPrime *primes,
+void launch_moddown(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t jn, const DevPrime *primes,
                     hipStream_t st) {
     constexpr int K1 = LOGN / 2;
     constexpr int K2 = LOGN - K1;
     constexpr int E = 4;
     constexpr int CS = 16;
     constexpr int RS = 4096 >> K2;
-    const uint32_t jn = 2 * count;
     const uint32_t col_blocks = a.L * jn * ((1u << K2) / CS);
     const bool mx = rows_mixed<LOGN, K2, RS>(jn);
     const uint32_t row_blocks = a.L * rows_blocks_per_prime<LOGN, K2, E, RS>(jn, mx);
@@ -1378,20 +1385,32 @@ bool ntt_modup_keyprod(const uint64_t *d, size_t d_ct_stride, uint64_t *T, const
     return true;
 }
 
-bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t count, const DevPrime *primes, int logn,
-                 hipStream_t st) {
-    if (logn < 13 || logn > 17 || count == 0) return false;
-    if ((uint64_t)2 * count * a.L > ((uint64_t(1) << 34) >> logn))
-        fail(ERR_INVALID_ARGUMENT, "key-switch batch too large for one launch");
+namespace {
+bool moddown_jn(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t jn, const DevPrime *primes, int logn,
+                hipStream_t st) {
+    if (logn < 13 || logn > 17 || jn == 0) return false;
+    if ((uint64_t)jn * a.L > ((uint64_t(1) << 34) >> logn)) fail(ERR_INVALID_ARGUMENT, "batch too large for one launch");
+    if (a.p_lpp && !ckks) fail(ERR_LOGIC, "a strided ModDown input is for the NTT form");
     switch (logn) {
-        case 13: launch_moddown<13>(ckks, buf, a, count, primes, st); break;
-        case 14: launch_moddown<14>(ckks, buf, a, count, primes, st); break;
-        case 15: launch_moddown<15>(ckks, buf, a, count, primes, st); break;
-        case 16: launch_moddown<16>(ckks, buf, a, count, primes, st); break;
-        case 17: launch_moddown<17>(ckks, buf, a, count, primes, st); break;
+        case 13: launch_moddown<13>(ckks, buf, a, jn, primes, st); break;
+        case 14: launch_moddown<14>(ckks, buf, a, jn, primes, st); break;
+        case 15: launch_moddown<15>(ckks, buf, a, jn, primes, st); break;
+        case 16: launch_moddown<16>(ckks, buf, a, jn, primes, st); break;
+        case 17: launch_moddown<17>(ckks, buf, a, jn, primes, st); break;
     }
     launch_check("ntt moddown launch");
     return true;
+}
+}  // namespace
+
+bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t count, const DevPrime *primes, int logn,
+                 hipStream_t st) {
+    return moddown_jn(ckks, buf, a, 2 * count, primes, logn, st);
+}
+
+bool ntt_moddown_polys(uint64_t *U, const ModDownArgs &a, uint32_t polys, const DevPrime *primes, int logn,
+                       hipStream_t st) {
+    return moddown_jn(true, U, a, polys, primes, logn, st);
 }
 
 void ntt_forward(uint64_t *data, uint32_t count, uint32_t lpp, const DevPrime *primes, int logn, NttOut out,

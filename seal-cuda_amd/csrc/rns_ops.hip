@@ -221,6 +221,27 @@ __global__ void rescale_finish_kernel(const uint64_t *__restrict__ in, const uin
     }
 }
 
+// Coefficient-form divide-and-round by q_Lo: out[g][i] = (in[g][i] - u_i) q_Lo^-1 with
+// u_i = ([in[g][Lo] + floor(q_Lo/2)]_{q_Lo} mod q_i) - floor(q_Lo/2) mod q_i, in with
+// Lo + 1 limbs per polynomial, out with Lo.
+__global__ void rescale_coeff_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                                     const DevPrime *__restrict__ primes, const DivRoundConst *__restrict__ dc,
+                                     uint64_t p, uint32_t Lo, int logn, size_t total) {
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+    const size_t n = (size_t)1 << logn;
+    const size_t c = idx & (n - 1);
+    const size_t r = idx >> logn;
+    const uint32_t i = (uint32_t)(r % Lo);
+    const size_t g = r / Lo;
+    const DevPrime &Q = primes[i];
+    const uint64_t *src = in + g * (Lo + 1) * n + c;
+    uint64_t v = src[(size_t)Lo * n] + (p >> 1);
+    v = v >= p ? v - p : v;
+    const uint64_t u = submod(barrett64(v, Q.q, Q.ratio_hi), dc[i].half_mod, Q.q);
+    out[idx] = shoup(submod(src[(size_t)i * n], u, Q.q), dc[i].inv, dc[i].inv_shoup, Q.q);
+    }
+}
+
 // S[g][c] = in[g][Lin-1][c]: gathers the last limb of every polynomial.
 __global__ void last_limb_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ S, uint32_t Lin, int logn,
                                  size_t total) {
@@ -523,6 +544,24 @@ void rescale_to_next(const Context &c, const uint64_t *in, uint64_t *out, uint32
     uint64_t *U = reinterpret_cast<uint64_t *>(static_cast<char *>(ws) + ((g * c.n * 8 + 255) & ~size_t(255)));
     const bool ckks = c.scheme == SCHEME_CKKS;
     const DevPrime *primes = c.primes();
+    const uint32_t Lo = L - 1;
+    if (!ckks) {
+        // coefficient form: one pass, the last limb read beside every kept one
+        const size_t total = g * Lo * c.n;
+        rescale_coeff_kernel<<<blocks_for(total), TPB, 0, st>>>(in, out, primes, c.rescale_consts(L), c.moduli[Lo], Lo,
+                                                                logn, total);
+        launch_check("rescale_coeff");
+        return;
+    }
+    if (logn >= 13 && g * Lo <= ((uint64_t(1) << 34) >> logn)) {
+        // NTT form: the last limbs' inverse transforms, then the divide-and-round fused into
+        // the forward transform of the correction (written once, never re-read)
+        ntt_inverse_from(in + (size_t)Lo * c.n, LimbGrid{1, L, 1, (uint32_t)g}, S, LimbGrid{1, 1, 1, (uint32_t)g},
+                         primes + Lo, logn, NTT_FULL, st);
+        const ModDownArgs md{S, in, out, 2 * (size_t)Lo * c.n, c.rescale_consts(L), c.moduli[Lo], Lo, true,
+                             2 * (size_t)L * c.n, L};
+        if (ntt_moddown_polys(U, md, (uint32_t)g, primes, logn, st)) return;
+    }
     const size_t swords = g * c.n;
     last_limb_kernel<<<blocks_for(swords), TPB, 0, st>>>(in, S, L, logn, swords);
     launch_check("last_limb");
@@ -665,6 +704,8 @@ void rescale_groups(const Context &c, const RescaleGroup *gs, uint32_t n, uint32
     in_chunks(n, [&](uint32_t g0, uint32_t m) {
         std::vector<ElemGroup> last, prep, fin;
         NttGroup sg[MAX_GROUPS], ug[MAX_GROUPS];
+        ModDownGroup mg[MAX_GROUPS];
+        bool fused = ckks && logn >= 13;
         uint32_t k = 0;
         for (uint32_t g = g0; g < g0 + m; g++) {
             const RescaleGroup &r = gs[g];
@@ -705,11 +746,18 @@ void rescale_groups(const Context &c, const RescaleGroup *gs, uint32_t n, uint32
             fin.push_back(f);
             sg[k] = NttGroup{S, nullptr, LimbGrid{1, 1, 1, 2 * r.count}, LimbGrid{1, 1, 1, 2 * r.count}, primes + (L - 1)};
             ug[k] = NttGroup{U, nullptr, LimbGrid{1, Lo, Lo, 2 * r.count}, LimbGrid{1, Lo, Lo, 2 * r.count}, primes};
+            // the divide-and-round fused into the correction's forward transform (folded:
+            // out written once from in, the correction never re-read)
+            mg[k] = ModDownGroup{U, ModDownArgs{S, r.in, r.out, 2 * (size_t)Lo * nn, c.rescale_consts(L),
+                                                c.moduli[L - 1], Lo, true, sa, L},
+                                 r.count};
+            if ((uint64_t)2 * r.count * Lo > ((uint64_t(1) << 34) >> logn)) fused = false;
             k++;
         }
         if (!k) return;
         launch_elem_groups<OP_LAST_LIMB>(last, primes, logn, st);
         if (ckks) ntt_inverse_groups(sg, k, logn, NTT_FULL, st);
+        if (fused && ntt_moddown_groups(mg, k, primes, logn, st)) return;
         launch_elem_groups<OP_DIVROUND_PREP>(prep, primes, logn, st);
         if (ckks) ntt_forward_groups(ug, k, logn, NTT_FULL, st);
         launch_elem_groups<OP_RESCALE_FINISH>(fin, primes, logn, st);

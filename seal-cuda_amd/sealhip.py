@@ -66,10 +66,16 @@ def lib():
         L.shc_encrypt_zero_asymmetric.argtypes = [_vp, _vp, _vp, _vp, _vp, u32, _vp, _vp, sz, _vp]
         L.shc_encrypt.argtypes = [_vp, _vp, _vp, u32, u64, _vp, _vp, sz, _vp]
         L.shc_encrypt_zero_symmetric.argtypes = [_vp, _vp, u32, u64, _vp, _vp]
+        L.shc_encrypt_many_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
+        L.shc_encrypt_many.argtypes = [_vp, _vp, _vp, u32, u32, u64, _vp, _vp, sz, _vp]
+        L.shc_decrypt_many_workspace_size.argtypes = [_vp, u32, u32, ctypes.POINTER(sz)]
+        L.shc_decrypt_many.argtypes = [_vp, _vp, u32, u32, _vp, u32, _vp, _vp, sz, _vp]
         L.shc_decrypt_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
         L.shc_decrypt.argtypes = [_vp, _vp, u32, _vp, u32, _vp, _vp, sz, _vp]
         L.shc_batch_encode.argtypes = [_vp, _vp, u32, _vp, _vp]
         L.shc_batch_decode.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.shc_batch_encode_many.argtypes = [_vp, _vp, u32, sz, u32, _vp, _vp]
+        L.shc_batch_decode_many.argtypes = [_vp, _vp, u32, _vp, _vp, _vp]
         L.shc_ckks_encode_workspace_size.argtypes = [_vp, ctypes.POINTER(sz)]
         L.shc_ckks_encode.argtypes = [_vp, _vp, u32, ctypes.c_double, u32, _vp, _vp, sz, _vp]
         L.shc_ckks_encode_many_workspace_size.argtypes = [_vp, u32, ctypes.POINTER(sz)]
@@ -346,6 +352,19 @@ class Context:
     def encrypt_zero_symmetric(self, d_sk, limbs, seed, d_out, stream=None):
         check(lib().shc_encrypt_zero_symmetric(self.handle, d_sk, limbs, seed, d_out, stream))
 
+    def encrypt_many_workspace_size(self, limbs, count):
+        return self._size(lib().shc_encrypt_many_workspace_size, limbs, count)
+
+    def encrypt_many(self, d_pk, d_plain, count, limbs, seed, d_out, d_ws, ws_bytes, stream=None):
+        """Ciphertext g = encrypt(seed + g) of plaintext g (d_plain None/0: zeros)."""
+        check(lib().shc_encrypt_many(self.handle, d_pk, d_plain, count, limbs, seed, d_out, d_ws, ws_bytes, stream))
+
+    def decrypt_many_workspace_size(self, limbs, count):
+        return self._size(lib().shc_decrypt_many_workspace_size, limbs, count)
+
+    def decrypt_many(self, d_ct, size, count, d_sk, limbs, d_out, d_ws, ws_bytes, stream=None):
+        check(lib().shc_decrypt_many(self.handle, d_ct, size, count, d_sk, limbs, d_out, d_ws, ws_bytes, stream))
+
     def decrypt_workspace_size(self, limbs):
         return self._size(lib().shc_decrypt_workspace_size, limbs)
 
@@ -357,6 +376,12 @@ class Context:
 
     def batch_decode(self, d_plain, d_values, d_scratch, stream=None):
         check(lib().shc_batch_decode(self.handle, d_plain, d_values, d_scratch, stream))
+
+    def batch_encode_many(self, d_values, count, values_stride, plaintexts, d_plain, stream=None):
+        check(lib().shc_batch_encode_many(self.handle, d_values, count, values_stride, plaintexts, d_plain, stream))
+
+    def batch_decode_many(self, d_plain, plaintexts, d_values, d_scratch, stream=None):
+        check(lib().shc_batch_decode_many(self.handle, d_plain, plaintexts, d_values, d_scratch, stream))
 
     def ckks_encode_workspace_size(self):
         return self._size(lib().shc_ckks_encode_workspace_size)

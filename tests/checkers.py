@@ -111,6 +111,7 @@ def ref():
         lib.ref_apply_galois_keyswitch.argtypes = [ctypes.c_void_p, u64, u64p, ctypes.c_uint32, u64p, u64p]
         dp = ctypes.POINTER(ctypes.c_double)
         lib.ref_decrypt.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
+        lib.ref_encrypt.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
         lib.ref_batch_encode.argtypes = [ctypes.c_void_p, u64p, u64, u64p]
         lib.ref_batch_decode.argtypes = [ctypes.c_void_p, u64p, u64p]
         lib.ref_ckks_encode.argtypes = [ctypes.c_void_p, u64, dp, u64, ctypes.c_double, u64p]
@@ -421,6 +422,13 @@ class RefSession:
         shape = (L, self.n) if self.scheme == 2 else (self.n,)
         out = np.zeros(shape, dtype=np.uint64)
         assert ref().ref_decrypt(self.h, L, ptr(np.ascontiguousarray(ct)), ptr(np.ascontiguousarray(sk)),
+                                 ptr(out)) == 0
+        return out
+
+    def encrypt(self, pk, plain, L):
+        """The reference's public-key encryption (its own randomness: for timing)."""
+        out = np.zeros((2, L, self.n), dtype=np.uint64)
+        assert ref().ref_encrypt(self.h, L, ptr(np.ascontiguousarray(pk)), ptr(np.ascontiguousarray(plain)),
                                  ptr(out)) == 0
         return out
 

@@ -321,3 +321,180 @@ def test_key_generation_in_a_captured_graph():
         gen(stream(), rk2, gk2)
         torch.cuda.synchronize()
         np.testing.assert_array_equal(from_dev(rk2), want_r)
+
+
+M64 = (1 << 64) - 1
+
+
+def mix64(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def sub_seed(seed, k):
+    """The library's derivation of an encryption's three streams (u, e0, e1 = k 0, 1, 2)."""
+    return mix64((seed ^ mix64((k + 0x5EA15EED) & M64)) & M64)
+
+
+def expected_encryption(scheme, key, pk, seed, L, moduli, logn, plain=None):
+    """Oracle words of a public-key encryption at level L with the streams of `seed`:
+    sampled u, e0, e1 at L + 1 limbs (L at the key level), pk u + e, then the
+    divide-and-round by the dropped prime, then the plaintext."""
+    K = len(moduli)
+    L1 = min(L + 1, K)
+    u = ck.sample_poly(0, key, sub_seed(seed, 0), logn, moduli[:L1])
+    e0 = ck.sample_poly(1, key, sub_seed(seed, 1), logn, moduli[:L1])
+    e1 = ck.sample_poly(1, key, sub_seed(seed, 2), logn, moduli[:L1])
+    z = ck.encrypt_zero_asymmetric(scheme, pk, u, e0, e1, moduli[:L1], logn)
+    want = ck.rescale(scheme, z, moduli[:L1], logn) if L1 != L else z
+    if plain is not None:
+        if scheme == CKKS:
+            want[0] = ck.mod_add(want[0], plain, moduli)
+        else:
+            want[0] = ck.bfv_add_plain_scaled(want[0], plain, moduli, T, logn)
+    return want
+
+
+@pytest.mark.parametrize("scheme,keyed", [(BFV, False), (CKKS, False), (BFV, True), (CKKS, True)])
+def test_encrypt_many_words_match_oracle(scheme, keyed):
+    """shc_encrypt_many: ciphertext g equals the oracle's encryption with the streams of
+    seed + g (u, e0, e1 sampled by the oracle, pk u + e, divide-and-round, plaintext)
+    and shc_encrypt(seed + g), at the top level (special prime dropped), at level 1 (a
+    data prime dropped) and at the key level (nothing dropped), with and without
+    plaintexts, in one pass and in slices of two."""
+    logn = 12
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [50, 40, 40, 50] if scheme == CKKS else [43, 43, 44, 44])
+    ctx = hip().Context(scheme, n, moduli, T if scheme == BFV else 0)
+    key = bytes(range(3, 35)) if keyed else None
+    if keyed:
+        ctx.set_sampler_key(key)
+    K = len(moduli)
+    sk, pk = empty_dev((K, n)), empty_dev((2, K, n))
+    ctx.keygen_secret(1, sk.data_ptr(), stream())
+    ctx.keygen_public(sk.data_ptr(), 2, pk.data_ptr(), stream())
+    pk_h = from_dev(pk)
+    rng = np.random.default_rng(scheme * 10 + keyed)
+    count, seed = 5, 1000 + 7 * keyed
+    for L in (K - 1, 1, K):
+        plains = [None]
+        if scheme == CKKS:
+            plains.append(ck.uniform_residues(rng, (count, L, n), moduli[:L]))
+        elif L < K:
+            plains.append(rng.integers(0, T, size=(count, n), dtype=np.uint64))
+        for plain in plains:
+            dplain = to_dev(plain) if plain is not None else None
+            pp = dplain.data_ptr() if dplain is not None else 0
+            want = np.stack([expected_encryption(scheme, key, pk_h, seed + g, L, moduli, logn,
+                                                 None if plain is None else plain[g]) for g in range(count)])
+            out = empty_dev((count, 2, L, n))
+            w = ctx.encrypt_many_workspace_size(L, count)
+            wb = ws(w)
+            ctx.encrypt_many(pk.data_ptr(), pp, count, L, seed, out.data_ptr(), wb.data_ptr(), w, stream())
+            np.testing.assert_array_equal(from_dev(out), want, err_msg=f"L {L}, plain {plain is not None}")
+            # slices of two ciphertexts
+            w2 = ctx.encrypt_many_workspace_size(L, 2)
+            out2 = empty_dev((count, 2, L, n))
+            ctx.encrypt_many(pk.data_ptr(), pp, count, L, seed, out2.data_ptr(), wb.data_ptr(), w2, stream())
+            np.testing.assert_array_equal(from_dev(out2), want)
+            # the single-ciphertext entry point gives the same words
+            ew = ctx.encrypt_workspace_size(L)
+            eb = ws(ew)
+            one = empty_dev((2, L, n))
+            for g in (0, count - 1):
+                ctx.encrypt(pk.data_ptr(), 0 if dplain is None else dplain[g].data_ptr(), L, seed + g,
+                            one.data_ptr(), eb.data_ptr(), ew, stream())
+                np.testing.assert_array_equal(from_dev(one), want[g])
+
+
+def test_encrypt_many_rejects_and_empty():
+    logn = 10
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [40, 40, 50])
+    ctx = hip().Context(CKKS, n, moduli)
+    K = len(moduli)
+    pk, out = empty_dev((2, K, n)), empty_dev((2, 2, 2, n))
+    w = ctx.encrypt_many_workspace_size(2, 2)
+    wb = ws(w)
+    ctx.encrypt_many(pk.data_ptr(), 0, 0, 2, 1, out.data_ptr(), 0, 0, stream())  # nothing to do
+    with pytest.raises(hip().SealHipError):
+        ctx.encrypt_many(pk.data_ptr(), 0, 2, 2, 1, out.data_ptr(), wb.data_ptr(), w // 2 - 8, stream())
+    with pytest.raises(hip().SealHipError):
+        ctx.encrypt_many(pk.data_ptr(), 0, 2, K + 1, 1, out.data_ptr(), wb.data_ptr(), w, stream())
+    with pytest.raises(hip().SealHipError):
+        ctx.decrypt_many(out.data_ptr(), 4, 1, pk.data_ptr(), 2, out.data_ptr(), 0, 0, stream())
+
+
+@pytest.mark.parametrize("scheme", [BFV, CKKS])
+def test_decrypt_many_words_match_oracle(scheme):
+    """shc_decrypt_many on size-2 and size-3 ciphertexts with arbitrary words: every
+    ciphertext's output is the oracle's decryption (BFV scale-and-round, CKKS phase) and
+    shc_decrypt's; BFV also in slices of two ciphertexts."""
+    logn = 12
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [50, 40, 40, 50] if scheme == CKKS else [43, 43, 44, 44])
+    ctx = hip().Context(scheme, n, moduli, T if scheme == BFV else 0)
+    K = len(moduli)
+    rng = np.random.default_rng(40 + scheme)
+    sk = ck.ntt_rns(ck.rns_of_small(rng.integers(-1, 2, size=n), moduli), moduli, logn)
+    dsk = to_dev(sk)
+    count = 5
+    for L in (K - 1, 2):
+        for size in (2, 3):
+            cts = ck.uniform_residues(rng, (count, size, L, n), moduli[:L])
+            if scheme == BFV:
+                want = np.stack([ck.bfv_decrypt(ct, sk[:L], moduli, T, logn) for ct in cts])
+                shape = (count, n)
+            else:
+                want = np.stack([ck.decrypt_phase(CKKS, ct, sk[:L], moduli, logn) for ct in cts])
+                shape = (count, L, n)
+            dct = to_dev(cts)
+            out = empty_dev(shape)
+            w = ctx.decrypt_many_workspace_size(L, count)
+            wb = ws(w)
+            ctx.decrypt_many(dct.data_ptr(), size, count, dsk.data_ptr(), L, out.data_ptr(), wb.data_ptr(), w,
+                             stream())
+            np.testing.assert_array_equal(from_dev(out), want, err_msg=f"L {L}, size {size}")
+            if scheme == BFV:
+                w2 = ctx.decrypt_many_workspace_size(L, 2)
+                out2 = empty_dev(shape)
+                ctx.decrypt_many(dct.data_ptr(), size, count, dsk.data_ptr(), L, out2.data_ptr(), wb.data_ptr(), w2,
+                                 stream())
+                np.testing.assert_array_equal(from_dev(out2), want)
+            dw = ctx.decrypt_workspace_size(L)
+            db = ws(dw)
+            one = empty_dev(shape[1:])
+            ctx.decrypt(dct[count - 1].data_ptr(), size, dsk.data_ptr(), L, one.data_ptr(), db.data_ptr(), dw, stream())
+            np.testing.assert_array_equal(from_dev(one), want[count - 1])
+
+
+def test_batch_encoder_many_matches_single_calls():
+    """shc_batch_encode_many / shc_batch_decode_many: plaintext g of a batch (strided
+    values, fewer than N) has the words of its own shc_batch_encode and the oracle's, and
+    decodes back to its values padded with zeros."""
+    ctx, moduli = bfv_ctx()
+    n = ctx.n
+    rng = np.random.default_rng(71)
+    P, count, stride = 5, 5000, 6000
+    vals = np.zeros((P, stride), dtype=np.uint64)
+    vals[:, :count] = rng.integers(0, T, size=(P, count), dtype=np.uint64)
+    dv = to_dev(vals)
+    plains = empty_dev((P, n))
+    ctx.batch_encode_many(dv.data_ptr(), count, stride, P, plains.data_ptr(), stream())
+    got = from_dev(plains)
+    one = empty_dev((n,))
+    for g in range(P):
+        padded = np.zeros(n, dtype=np.uint64)
+        padded[:count] = vals[g, :count]
+        np.testing.assert_array_equal(got[g], ck.batch_encode(padded, T, logn=13), err_msg=f"plaintext {g}")
+        ctx.batch_encode(dv[g].data_ptr(), count, one.data_ptr(), stream())
+        np.testing.assert_array_equal(got[g], from_dev(one))
+    back, scratch = empty_dev((P, n)), empty_dev((P, n))
+    ctx.batch_decode_many(plains.data_ptr(), P, back.data_ptr(), scratch.data_ptr(), stream())
+    b = from_dev(back)
+    np.testing.assert_array_equal(b[:, :count], vals[:, :count])
+    assert not b[:, count:].any()
+    ctx.batch_encode_many(dv.data_ptr(), count, stride, 0, plains.data_ptr(), stream())  # nothing to do
+    with pytest.raises(hip().SealHipError):
+        ctx.batch_encode_many(dv.data_ptr(), count, count - 1, 2, plains.data_ptr(), stream())

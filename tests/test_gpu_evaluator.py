@@ -119,15 +119,21 @@ def test_batch_equals_alone():
         np.testing.assert_array_equal(batch[c], alone[0])
 
 
-@pytest.mark.parametrize("scheme,logn,bits,L", [
-    (CKKS, 10, [60, 40, 40, 60], 3),
-    (CKKS, 15, [60] + [40] * 14 + [60], 15),
-    (BFV, 12, [43, 43, 44, 44, 44], 4),
+@pytest.mark.parametrize("scheme,logn,bits,L,count,polys", [
+    (CKKS, 10, [60, 40, 40, 60], 3, 2, 2),
+    (CKKS, 15, [60] + [40] * 14 + [60], 15, 2, 2),
+    (CKKS, 13, [60, 40, 40, 60], 3, 1, 3),       # an odd number of polynomials
+    (CKKS, 13, [60, 40, 40, 60], 4, 3, 2),       # key level: the special prime dropped
+    (CKKS, 16, [60] * 25, 24, 3, 2),             # the headline shape
+    (CKKS, 17, [60] * 4, 2, 2, 3),
+    (BFV, 12, [43, 43, 44, 44, 44], 4, 2, 2),
+    (BFV, 14, [50, 50, 50, 50], 3, 3, 3),
 ])
-def test_rescale(scheme, logn, bits, L):
+def test_rescale(scheme, logn, bits, L, count, polys):
+    """rescale_to_next words equal the oracle's per polynomial: CKKS through the
+    divide-and-round fused into the correction's transform (N >= 2^13), BFV in one pass."""
     ctx, moduli = make_ctx(scheme, logn, bits, 65537 if scheme == BFV else 0)
     n = 1 << logn
-    count, polys = 2, 2
     rng = np.random.default_rng(200 + logn)
     ct = ck.uniform_residues(rng, (count, polys, L, n), moduli[:L])
     din, dout = to_dev(ct), empty_dev((count, polys, L - 1, n))

@@ -120,6 +120,32 @@ def test_reference_decrypts_oracle_encryption():
 
 @ref_only
 @pytest.mark.ref
+def test_reference_encryption_shim_round_trips():
+    """The reference's own Encryptor behind ref_encrypt (timed by the benchmark's CPU
+    baseline) encrypts under an injected public key: BFV decrypts to the plaintext
+    exactly, CKKS to the plaintext plus a small error (|e| < 2^20 per coefficient)."""
+    logn, t = 12, 65537
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [43, 43, 44, 44, 44])
+    rng = np.random.default_rng(17)
+    s, sk, pk = keygen(rng, moduli, logn)
+    L = len(moduli) - 1
+    bfv = ck.RefSession(BFV, n, moduli, t)
+    plain = bfv.batch_encode(rng.integers(0, t, size=n, dtype=np.uint64))
+    ct = bfv.encrypt(pk, plain, L)
+    assert np.array_equal(bfv.decrypt(ct, sk, L), plain)
+    ckks = ck.RefSession(CKKS, n, moduli)
+    for level in (L, 2):
+        pl = ck.uniform_residues(rng, (level, n), moduli[:level])
+        dec = ckks.decrypt(ckks.encrypt(pk, pl, level), sk, level)
+        diff = ck.ntt_rns(ck.mod_sub(dec, pl, moduli), moduli[:level], logn, inverse=True)
+        for i, q in enumerate(moduli[:level]):
+            d = diff[i].astype(object)
+            assert all(min(int(x), q - int(x)) < (1 << 20) for x in d)
+
+
+@ref_only
+@pytest.mark.ref
 @pytest.mark.parametrize("logn,t", [(10, 65537), (13, 65537), (12, 786433)])
 def test_batch_encoder_matches_reference(logn, t):
     n = 1 << logn
