@@ -93,6 +93,23 @@ __global__ void fft_fwd_layer(double2 *__restrict__ a, const double2 *__restrict
 // src/ckks.cpp:176 and src/ckks_cuda.cu:220), reduce into each limb; record max
 // |coefficient| (positive doubles order like their bit patterns, so an integer
 // atomicMax works)
+// atomicMax of the active lanes' values: with the whole wave active, its maximum first
+// (a butterfly of lane shuffles) and one atomic from lane 0.  A non-negative double's
+// bit pattern orders like the double.
+__device__ __forceinline__ void wave_max_atomic(unsigned long long *dst, unsigned long long v) {
+    const uint64_t active = __ballot(1);
+    if (active == ~0ull) {
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            const unsigned long long o = __shfl_xor(v, m);
+            v = o > v ? o : v;
+        }
+        if (__lane_id() == 0) atomicMax(dst, v);
+    } else {
+        atomicMax(dst, v);  // partial wave (the grid's tail): one atomic per lane
+    }
+}
+
 __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__restrict__ out,
                                     const DevPrime *__restrict__ primes, uint32_t L, int logn,
                                     unsigned long long *maxbits) {
@@ -105,7 +122,7 @@ __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__r
         double d = round(a[c].x);
         const bool neg = signbit(d);
         d = fabs(d);
-        atomicMax(maxbits, (unsigned long long)__double_as_longlong(d));
+        wave_max_atomic(maxbits, (unsigned long long)__double_as_longlong(d));
         if (!(d < two64 * two64)) d = 0;  // too large, inf or NaN: rejected on the host after the range check
         const uint64_t hi = (uint64_t)(d / two64);
         const uint64_t lo = (uint64_t)fmod(d, two64);
@@ -125,7 +142,7 @@ __global__ void poly_scale_kernel(const double *__restrict__ values, uint32_t co
     for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
         const double v = i < count ? values[i] * scale : 0.0;
         a[i] = make_double2(v, 0.0);
-        atomicMax(maxbits, (unsigned long long)__double_as_longlong(fabs(v)));
+        wave_max_atomic(maxbits, (unsigned long long)__double_as_longlong(fabs(v)));
     }
 }
 

@@ -46,6 +46,15 @@ struct KernelProbe {
     hipEvent_t begin, end;
 };
 
+// Row idx >> logn of a flat index over rows of 2^logn words.  In the grid-stride loops
+// here a wave's 64 lanes hold 64 consecutive indices starting at a multiple of 64, so
+// for 2^logn >= 64 every lane is in the same row: the row is made wave-uniform and the
+// divisions that split it run on the scalar unit.  Rows are below 2^32.
+__device__ __forceinline__ uint32_t uniform_row(size_t idx, int logn) {
+    const uint32_t r = (uint32_t)(idx >> logn);
+    return logn >= 6 ? __builtin_amdgcn_readfirstlane(r) : r;
+}
+
 // Error raised by host code; carries the exception family of the reference
 // (std::invalid_argument / logic_error / out_of_range / runtime_error).
 enum ErrKind { ERR_OK = 0, ERR_INVALID_ARGUMENT = -1, ERR_LOGIC = -2, ERR_OUT_OF_RANGE = -3, ERR_RUNTIME = -4,

@@ -228,11 +228,13 @@ __global__ void bfv_scale_add_kernel(uint64_t *__restrict__ c0_base, size_t c0_s
 // phase + g * poly.  c0 == nullptr: the sum alone.
 __global__ void phase_kernel(const uint64_t *__restrict__ c0, const uint64_t *__restrict__ cs, size_t c_stride,
                              size_t in_stride, uint32_t size, const uint64_t *__restrict__ sk,
-                             uint64_t *__restrict__ phase, const DevPrime *__restrict__ primes, int logn, size_t poly,
+                             uint64_t *__restrict__ phase, const DevPrime *__restrict__ primes, int logn, uint32_t L,
                              size_t total) {
+    const size_t poly = (size_t)L << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t g = idx / poly, i = idx - g * poly, in = g * in_stride + i;
-        const DevPrime &P = primes[i >> logn];
+        const uint32_t r = uniform_row(idx, logn), g = r / L;
+        const size_t i = idx - g * poly, in = g * in_stride + i;
+        const DevPrime &P = primes[r - g * L];
         const uint64_t s = sk[i];
         uint64_t acc = 0, sp = s;
         for (uint32_t k = 1; k < size; k++) {
@@ -325,9 +327,10 @@ template <bool KEYED>
 __global__ void sample_encrypt_kernel(SamplerKey key, uint64_t seed, const DevPrime *__restrict__ primes, uint32_t L1,
                                       int logn, size_t total, uint64_t *__restrict__ U, int64_t *__restrict__ Es) {
     const size_t n = (size_t)1 << logn;
-    const size_t units = KEYED ? (n + 7) / 8 : n;  // per ciphertext: keystream blocks or coefficients
+    // per ciphertext: 2^ulog keystream blocks (8 coefficients each) or coefficients
+    const int ulog = KEYED ? (logn > 3 ? logn - 3 : 0) : logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t g = idx / units, b = idx - g * units;
+        const size_t g = idx >> ulog, b = idx & (((size_t)1 << ulog) - 1);
         uint64_t *Ug = U + g * L1 * n;
         int64_t *Eg = Es + g * 2 * n;
         for (int k = 0; k < 3; k++) {
@@ -365,11 +368,12 @@ __global__ void sample_encrypt_kernel(SamplerKey key, uint64_t seed, const DevPr
     }
 }
 
-// dst[r][k] = src[r * src_stride + k] for rows of `row` words (total = rows * row)
+// dst[r][k] = src[r * src_stride + k] for rows of row_limbs * N words (total = rows * row)
 __global__ void gather_rows_kernel(const uint64_t *__restrict__ src, size_t src_stride, uint64_t *__restrict__ dst,
-                                   size_t row, size_t total) {
+                                   uint32_t row_limbs, int logn, size_t total) {
+    const size_t row = (size_t)row_limbs << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t r = idx / row;
+        const size_t r = uniform_row(idx, logn) / row_limbs;
         dst[idx] = src[r * src_stride + (idx - r * row)];
     }
 }
@@ -391,9 +395,9 @@ __global__ void enc_products_kernel(const uint64_t *__restrict__ pk, uint32_t K,
                                     const DevPrime *__restrict__ primes, int logn, size_t total) {
     const size_t n = (size_t)1 << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t c = idx & (n - 1), r = idx >> logn;
-        const uint32_t l = (uint32_t)(r % lw) + l0;
-        const size_t gj = r / lw, g = gj >> 1, j = gj & 1;
+        const size_t c = idx & (n - 1);
+        const uint32_t r = uniform_row(idx, logn), gj = r / lw, l = r - gj * lw + l0;
+        const size_t g = gj >> 1, j = gj & 1;
         const DevPrime &Q = primes[l];
         P[idx] = mulmod(pk[(j * K + l) * n + c], U[(g * L1 + l) * n + c], Q.q, Q.ratio_lo, Q.ratio_hi);
     }
@@ -406,13 +410,12 @@ __global__ void enc_error_kernel(const int64_t *__restrict__ Es, const uint64_t 
                                  const DivRoundConst *__restrict__ dc, uint64_t p, uint32_t L, int logn, size_t total) {
     const size_t n = (size_t)1 << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t c = idx & (n - 1), r = idx >> logn;
-        const uint32_t i = (uint32_t)(r % L);
-        const size_t gj = r / L;
+        const size_t c = idx & (n - 1);
+        const uint32_t r = uniform_row(idx, logn), gj = r / L, i = r - gj * L;
         const DevPrime &Q = primes[i];
-        const int64_t e = Es[gj * n + c];
+        const int64_t e = Es[(size_t)gj * n + c];
         uint64_t x = lift_small(e, Q.q);
-        if (Pl) x = submod(x, divround_corr(addmod(Pl[gj * n + c], lift_small(e, p), p), p, Q, dc[i]), Q.q);
+        if (Pl) x = submod(x, divround_corr(addmod(Pl[(size_t)gj * n + c], lift_small(e, p), p), p, Q, dc[i]), Q.q);
         out[idx] = x;
     }
 }
@@ -425,9 +428,9 @@ __global__ void enc_finish_ckks_kernel(const uint64_t *__restrict__ pk, uint32_t
                                        bool drop, uint32_t L, int logn, size_t total) {
     const size_t n = (size_t)1 << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t c = idx & (n - 1), r = idx >> logn;
-        const uint32_t i = (uint32_t)(r % L);
-        const size_t gj = r / L, g = gj >> 1, j = gj & 1;
+        const size_t c = idx & (n - 1);
+        const uint32_t r = uniform_row(idx, logn), gj = r / L, i = r - gj * L;
+        const size_t g = gj >> 1, j = gj & 1;
         const DevPrime &Q = primes[i];
         uint64_t x = addmod(mulmod(pk[(j * K + i) * n + c], U[(g * L1 + i) * n + c], Q.q, Q.ratio_lo, Q.ratio_hi),
                             out[idx], Q.q);
@@ -445,12 +448,11 @@ __global__ void enc_finish_bfv_kernel(const uint64_t *__restrict__ P, const int6
                                       uint32_t L1, int logn, size_t total) {
     const size_t n = (size_t)1 << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t c = idx & (n - 1), r = idx >> logn;
-        const uint32_t i = (uint32_t)(r % L);
-        const size_t gj = r / L;
+        const size_t c = idx & (n - 1);
+        const uint32_t r = uniform_row(idx, logn), gj = r / L, i = r - gj * L;
         const DevPrime &Q = primes[i];
-        const int64_t e = Es[gj * n + c];
-        const uint64_t *Pg = P + gj * L1 * n;
+        const int64_t e = Es[(size_t)gj * n + c];
+        const uint64_t *Pg = P + (size_t)gj * L1 * n;
         uint64_t x = addmod(Pg[(size_t)i * n + c], lift_small(e, Q.q), Q.q);
         if (drop) {
             const uint64_t last = addmod(Pg[(size_t)L * n + c], lift_small(e, p), p);
@@ -714,7 +716,7 @@ void decrypt_many(const Context &c, const uint64_t *ct, uint32_t size, uint32_t 
     if (c.scheme == SCHEME_CKKS) {
         const size_t total = (size_t)count * poly;
         phase_kernel<<<blocks_for(total), TPB, 0, st>>>(ct, ct + poly, poly, ct_words, size, sk, out, primes, c.logn,
-                                                        poly, total);
+                                                        L, total);
         launch_check("phase");
         return;
     }
@@ -728,12 +730,12 @@ void decrypt_many(const Context &c, const uint64_t *ct, uint32_t size, uint32_t 
         const uint64_t *src = ct + g0 * ct_words;
         uint64_t *tmp = static_cast<uint64_t *>(ws), *phase = tmp + (size_t)m * (size - 1) * poly;
         const size_t tw = (size - 1) * poly, gw = (size_t)m * tw;
-        gather_rows_kernel<<<blocks_for(gw), TPB, 0, st>>>(src + poly, ct_words, tmp, tw, gw);
+        gather_rows_kernel<<<blocks_for(gw), TPB, 0, st>>>(src + poly, ct_words, tmp, (size - 1) * L, c.logn, gw);
         launch_check("gather_rows");
         ntt_forward(tmp, m * (size - 1) * L, L, primes, c.logn, NTT_FULL, st);
         const size_t total = (size_t)m * poly;
         phase_kernel<<<blocks_for(total), TPB, 0, st>>>(nullptr, tmp, poly, (size - 1) * poly, size, sk, phase, primes,
-                                                        c.logn, poly, total);
+                                                        c.logn, L, total);
         launch_check("phase");
         ntt_inverse(phase, m * L, L, primes, c.logn, NTT_FULL, st);
         scale_round_kernel<<<blocks_for((size_t)m * n), TPB, 0, st>>>(phase, src, ct_words, out + g0 * n, m, B, primes,

@@ -29,8 +29,8 @@ __global__ void galois_ntt_kernel(const uint64_t *__restrict__ in, size_t in_str
     const uint32_t n = 1u << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
         const uint32_t i = (uint32_t)(idx & (n - 1));
-        const size_t r = idx >> logn;
-        const size_t g = r / L, l = r % L;
+        const uint32_t r = uniform_row(idx, logn), g32 = r / L;
+        const size_t g = g32, l = r - g32 * L;
         const uint32_t rev = rev_bits(n + i, logn + 1);
         const uint32_t raw = (uint32_t)((((uint64_t)elt * rev) >> 1) & (n - 1));
         const uint32_t src = rev_bits(raw, logn);
@@ -44,8 +44,8 @@ __global__ void galois_coeff_kernel(const uint64_t *__restrict__ in, size_t in_s
     const uint32_t n = 1u << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
         const uint32_t k = (uint32_t)(idx & (n - 1));
-        const size_t r = idx >> logn;
-        const size_t g = r / L, l = r % L;
+        const uint32_t r = uniform_row(idx, logn), g32 = r / L;
+        const size_t g = g32, l = r - g32 * L;
         const uint32_t i = (uint32_t)(((uint64_t)k * elt_inv) & (n - 1));
         const uint64_t raw = (uint64_t)i * elt;
         uint64_t v = in[g * in_stride + (l << logn) + i];
@@ -56,9 +56,10 @@ __global__ void galois_coeff_kernel(const uint64_t *__restrict__ in, size_t in_s
 
 // c0 <- c1, c1 <- 0 for each ciphertext (a kernel rather than memcpy/memset so the
 // sequence stays capturable in a HIP graph).
-__global__ void shift_zero_kernel(uint64_t *__restrict__ ct, size_t poly, size_t total) {
+__global__ void shift_zero_kernel(uint64_t *__restrict__ ct, uint32_t L, int logn, size_t total) {
+    const size_t poly = (size_t)L << logn;
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t g = idx / poly, r = idx % poly;
+        const size_t g = uniform_row(idx, logn) / L, r = idx - g * poly;
         uint64_t *base = ct + g * 2 * poly;
         base[r] = base[poly + r];
         base[poly + r] = 0;
@@ -101,11 +102,17 @@ void apply_galois_keyswitch(const Context &c, uint64_t *ct, uint32_t count, uint
     uint64_t *sigma = static_cast<uint64_t *>(ws);
     const size_t sigma_bytes = ((size_t)count * poly * 8 + 255) & ~size_t(255);
     void *ks_ws = static_cast<char *>(ws) + sigma_bytes;
-    // sigma(c1) first (into scratch), then sigma(c0) through the same scratch row of c1
+    // sigma(c1) first (into scratch), then sigma(c0) into the row of c1
     apply_galois(c, ct + poly, 2 * poly, sigma, poly, count, L, elt, ntt, st);
     apply_galois(c, ct, 2 * poly, ct + poly, 2 * poly, count, L, elt, ntt, st);
+    if (keyswitch_folds(c)) {
+        // the fused key products read sigma(c0) where it lies and take c1 = 0; the ModDown
+        // writes both polynomials
+        switch_key(c, sigma, poly, ct, 2 * poly, key, count, L, ks_ws, ws_bytes - sigma_bytes, st, ct + poly);
+        return;
+    }
     const size_t words = (size_t)count * poly;
-    shift_zero_kernel<<<blocks_for(words), TPB, 0, st>>>(ct, poly, words);
+    shift_zero_kernel<<<blocks_for(words), TPB, 0, st>>>(ct, L, c.logn, words);
     launch_check("shift_zero");
     switch_key(c, sigma, poly, ct, 2 * poly, key, count, L, ks_ws, ws_bytes - sigma_bytes, st);
 }

@@ -61,10 +61,13 @@ bool ntt_forward_modup(const uint64_t *d, size_t d_ct_stride, uint64_t *T, uint3
 // words, polynomials k = 0, 1 of L limbs each), P[ct][k][i] also takes ct_k[i] * (p mod q_i)
 // (fold.dc: the special prime's divide-and-round constants), so that the ModDown writes
 // (P - u) p^-1 without reading the ciphertext (ModDownArgs::folded).
+// c1_zero: the added-into ciphertext is (ct_0, 0) -- a Galois switch, whose sigma(c0)
+// stays where it was written and whose c1 is never materialised as zeros.
 struct KsFold {
     const uint64_t *ct;
     size_t ct_stride;
     const DivRoundConst *dc;
+    bool c1_zero = false;
 };
 bool ntt_modup_keyprod(const uint64_t *d, size_t d_ct_stride, uint64_t *T, const uint64_t *key, uint64_t *P,
                        uint64_t *S, const uint64_t *diag, size_t diag_stride, uint32_t count, uint32_t L,

@@ -419,7 +419,7 @@ __device__ __forceinline__ void rows_keyprod_body(
             const uint64_t *cw = fold.ct + (size_t)(ct0 + min<uint32_t>(c % CG, gv - 1)) * fold.ct_stride +
                                  ((size_t)i << LOGN) + row_off(c / CG) + tid;
             mac<true>(a0[c], cw[0], pm);
-            mac<true>(a1[c], cw[(size_t)L << LOGN], pm);
+            if (!fold.c1_zero) mac<true>(a1[c], cw[(size_t)L << LOGN], pm);
         }
     }
 #pragma unroll

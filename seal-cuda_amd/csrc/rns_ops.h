@@ -23,9 +23,13 @@ size_t keyswitch_workspace(const Context &c, uint32_t count, uint32_t L);
 //           (CKKS: NTT form; BFV: coefficient form), L limbs each.
 //   ct:     c0,c1 of each ciphertext, stride `ct_stride` words between ciphertexts.
 //   key:    [K-1][2][K][N] key-switching key at key level (NTT form).
-// The target is read only.
+// The target is read only.  fold_c0 (only where keyswitch_folds(c)): the switch adds
+// into (c0, 0) with c0 at fold_c0 + ct * ct_stride, and ct_0, ct_1 are written without
+// being read -- a Galois switch, whose permuted c0 needs no copy and whose c1 no zeros.
+bool keyswitch_folds(const Context &c);
 void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
-                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st);
+                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st,
+                const uint64_t *fold_c0 = nullptr);
 
 // Stages of a key switch at level L over `count` ciphertexts, for callers that compose
 // them differently (hoisted rotations):

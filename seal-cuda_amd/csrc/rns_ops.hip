@@ -178,11 +178,9 @@ __global__ void divround_prep_kernel(const uint64_t *__restrict__ S, uint64_t *_
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t n = (size_t)1 << logn;
     const size_t c = idx & (n - 1);
-    const size_t r = idx >> logn;
-    const uint32_t i = (uint32_t)(r % Lout);
-    const size_t g = r / Lout;
+    const uint32_t r = uniform_row(idx, logn), g = r / Lout, i = r - g * Lout;
     const DevPrime &Q = primes[i];
-    uint64_t v = S[g * n + c] + (p >> 1);
+    uint64_t v = S[(size_t)g * n + c] + (p >> 1);
     v = v >= p ? v - p : v;
     v = barrett64(v, Q.q, Q.ratio_hi);
     U[idx] = submod(v, dc[i].half_mod, Q.q);
@@ -212,11 +210,9 @@ __global__ void rescale_finish_kernel(const uint64_t *__restrict__ in, const uin
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t n = (size_t)1 << logn;
     const size_t c = idx & (n - 1);
-    const size_t r = idx >> logn;
-    const uint32_t i = (uint32_t)(r % Lout);
-    const size_t g = r / Lout;
+    const uint32_t r = uniform_row(idx, logn), g = r / Lout, i = r - g * Lout;
     const DevPrime &Q = primes[i];
-    const uint64_t x = submod(in[(g * (Lout + 1) + i) * n + c], U[idx], Q.q);
+    const uint64_t x = submod(in[((size_t)g * (Lout + 1) + i) * n + c], U[idx], Q.q);
     out[idx] = shoup(x, dc[i].inv, dc[i].inv_shoup, Q.q);
     }
 }
@@ -230,11 +226,9 @@ __global__ void rescale_coeff_kernel(const uint64_t *__restrict__ in, uint64_t *
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
     const size_t n = (size_t)1 << logn;
     const size_t c = idx & (n - 1);
-    const size_t r = idx >> logn;
-    const uint32_t i = (uint32_t)(r % Lo);
-    const size_t g = r / Lo;
+    const uint32_t r = uniform_row(idx, logn), g = r / Lo, i = r - g * Lo;
     const DevPrime &Q = primes[i];
-    const uint64_t *src = in + g * (Lo + 1) * n + c;
+    const uint64_t *src = in + (size_t)g * (Lo + 1) * n + c;
     uint64_t v = src[(size_t)Lo * n] + (p >> 1);
     v = v >= p ? v - p : v;
     const uint64_t u = submod(barrett64(v, Q.q, Q.ratio_hi), dc[i].half_mod, Q.q);
@@ -399,7 +393,7 @@ namespace {
 // within the NTT's per-launch limb bound).
 void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct,
                       size_t ct_stride, const uint64_t *key, uint32_t count, uint32_t L, void *ws, hipStream_t st,
-                      const KernelProbe *probe) {
+                      const KernelProbe *probe, const uint64_t *fold_c0) {
     const KsLayout lay = ks_layout(c, count, L);
     char *base = static_cast<char *>(ws);
     uint64_t *d = reinterpret_cast<uint64_t *>(base + lay.d);
@@ -436,10 +430,11 @@ void switch_key_chunk(const Context &c, const uint64_t *target, size_t target_st
     // (CKKS: the fused products also take p times the ciphertext the switch adds into, so
     // that the ModDown writes it without reading it)
     const KsShape shape = c.ks_shape(L, count);
-    const KsFold fold = ckks ? KsFold{ct, ct_stride, c.ks_consts()} : KsFold{};
+    const KsFold fold = ckks ? KsFold{fold_c0 ? fold_c0 : ct, ct_stride, c.ks_consts(), fold_c0 != nullptr} : KsFold{};
     const bool fused = ntt_modup_keyprod(digits, dstride, T, key, P, S, diag, target_stride, count, L,
                                          (uint32_t)c.key_count(), c.keyswitch_primes(L), c.primes(), logn, st, probe,
                                          shape, fold);
+    if (fold_c0 && !fused) fail(ERR_LOGIC, "a key switch into (c0, 0) needs the fused key products");
     if (!fused) {
         const bool skipped = modup_to_T(c, digits, dstride, T, count, L, st, probe, diag != nullptr);
         keyswitch_inner_product(c, T, key, P, S, count, L, st, skipped ? diag : nullptr, target_stride);
@@ -504,8 +499,12 @@ void moddown_into(const Context &c, uint64_t *P, uint64_t *S, uint64_t *U, uint6
     launch_check("moddown_finish");
 }
 
+bool keyswitch_folds(const Context &c) { return c.scheme == SCHEME_CKKS && (c.logn == 15 || c.logn == 16); }
+
 void switch_key(const Context &c, const uint64_t *target, size_t target_stride, uint64_t *ct, size_t ct_stride,
-                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st) {
+                const uint64_t *key, uint32_t count, uint32_t L, void *ws, size_t ws_bytes, hipStream_t st,
+                const uint64_t *fold_c0) {
+    if (fold_c0 && !keyswitch_folds(c)) fail(ERR_LOGIC, "a key switch into (c0, 0) needs the fused key products");
     const size_t K = c.key_count();
     if (K < 2) fail(ERR_LOGIC, "keyswitching is not supported by the context");
     if (L < 1 || L > K - 1) fail(ERR_INVALID_ARGUMENT, "level out of range for key switching");
@@ -522,7 +521,8 @@ void switch_key(const Context &c, const uint64_t *target, size_t target_stride, 
     for (uint32_t done = 0; done < count;) {
         const uint32_t n = (uint32_t)(count - done < max_cts ? count - done : max_cts);
         switch_key_chunk(c, target + (size_t)done * target_stride, target_stride, ct + (size_t)done * ct_stride,
-                         ct_stride, key, n, L, ws, st, armed && done == 0 ? &probe : nullptr);
+                         ct_stride, key, n, L, ws, st, armed && done == 0 ? &probe : nullptr,
+                         fold_c0 ? fold_c0 + (size_t)done * ct_stride : nullptr);
         done += n;
     }
 }

@@ -56,6 +56,8 @@ def test_galois_kat_on_gpu():
 @pytest.mark.parametrize("scheme,logn,bits,L,count,steps", [
     (CKKS, 10, [60, 40, 40, 60], 3, 2, [1, -1, 0]),
     (CKKS, 16, [60] + [50] * 23 + [60], 24, 1, [4]),      # hoisted-rotation workload's shape
+    (CKKS, 15, [60] * 6, 5, 7, [3, 0]),   # fused products over groups of 4, 2, 1 reading sigma(c0) in place
+    (CKKS, 13, [60] * 4, 3, 2, [2]),      # unfused key products: c1 zeroed first
     (BFV, 12, [43, 43, 44, 44, 44], 4, 3, [1, -7]),
     (BFV, 13, [43, 43, 44, 44, 44], 4, 1, [0]),          # row swap (conjugation element)
 ])
