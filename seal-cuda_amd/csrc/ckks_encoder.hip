@@ -57,35 +57,100 @@ __global__ void scatter_kernel(const double *__restrict__ values, uint32_t count
     }
 }
 
-// one Gentleman-Sande layer of DWTHandler::transformFromRev with m groups of size 2*gap
-__global__ void fft_inv_layer(double2 *__restrict__ a, const double2 *__restrict__ r, uint32_t m, uint32_t gap,
-                              uint32_t ridx, double scalar, int last, uint32_t half) {
-    a += (size_t)blockIdx.y * 2 * half;
-    for (uint32_t b = blockIdx.x * TPB + threadIdx.x; b < half; b += gridDim.x * TPB) {
-        const uint32_t i = b / gap, j = b % gap;
-        double2 *x = a + 2 * gap * i + j, *y = x + gap;
-        const double2 u = *x, v = *y;
-        double2 w = r[ridx + i];
+// ---- DWT (DWTHandler::transformFromRev / transformToRev, src/utils/dwthandler.h) ----
+// Two passes over HBM instead of one per layer.  A block holds
+// FFT_TILE values of one plaintext (blockIdx.y) in LDS and runs every layer whose
+// butterflies stay inside its lines: the row pass takes 2^kr contiguous values
+// (kr = min(logn, 12)), the layers with gap G < 2^kr; the column pass takes W = FFT_TILE
+// >> kc neighbouring columns of 2^kc values at stride 2^kr (kc = logn - kr), the layers
+// with G >= 2^kr.  Each butterfly is the reference's (Gentleman-Sande for the inverse,
+// scale folded into the last layer; Cooley-Tukey forward) with the same twiddle, so the
+// words are those of the layer-by-layer transform.
+constexpr int FFT_TILE = 4096;
+
+// inverse (Gentleman-Sande) layer of gap G reads irp[N - N/G + 1 + group]; forward
+// (Cooley-Tukey) layer of gap G reads rp[N/(2G) + group]; group = position / (2G).
+template <bool INV>
+__device__ __forceinline__ void dwt_butterfly(double2 *t, uint32_t x, uint32_t y, double2 w, bool last,
+                                              double scalar) {
+    const double2 u = t[x];
+    if (INV) {
+        const double2 v = t[y];
         double2 s = make_double2(u.x + v.x, u.y + v.y), d = make_double2(u.x - v.x, u.y - v.y);
         if (last) {
             w = make_double2(w.x * scalar, w.y * scalar);
             s = make_double2(s.x * scalar, s.y * scalar);
         }
-        *x = s;
-        *y = cmul(d, w);
+        t[x] = s;
+        t[y] = cmul(d, w);
+    } else {
+        const double2 v = cmul(t[y], w);
+        t[x] = make_double2(u.x + v.x, u.y + v.y);
+        t[y] = make_double2(u.x - v.x, u.y - v.y);
     }
 }
 
-// one Cooley-Tukey layer of DWTHandler::transformToRev
-__global__ void fft_fwd_layer(double2 *__restrict__ a, const double2 *__restrict__ r, uint32_t gap, uint32_t ridx,
-                              uint32_t half) {
-    a += (size_t)blockIdx.y * 2 * half;
-    for (uint32_t b = blockIdx.x * TPB + threadIdx.x; b < half; b += gridDim.x * TPB) {
-        const uint32_t i = b / gap, j = b % gap;
-        double2 *x = a + 2 * gap * i + j, *y = x + gap;
-        const double2 u = *x, v = cmul(*y, r[ridx + i]);
-        *x = make_double2(u.x + v.x, u.y + v.y);
-        *y = make_double2(u.x - v.x, u.y - v.y);
+template <bool INV>
+__global__ __launch_bounds__(TPB) void fft_rows_kernel(double2 *__restrict__ a, const double2 *__restrict__ r,
+                                                      int logn, int kr, double scalar) {
+    __shared__ double2 t[FFT_TILE];
+    const uint32_t n = 1u << logn, len = 1u << kr, base = blockIdx.x << kr;
+    a += ((size_t)blockIdx.y << logn) + base;
+    for (uint32_t k = threadIdx.x; k < len; k += TPB) t[k] = a[k];
+    __syncthreads();
+    for (uint32_t step = 0; step < (uint32_t)kr; step++) {
+        const uint32_t G = INV ? 1u << step : len >> (step + 1);
+        const uint32_t ridx = INV ? n - n / G + 1 : n / (2 * G);
+        const bool last = INV && 2 * G == n;
+        for (uint32_t b = threadIdx.x; b < len / 2; b += TPB) {
+            const uint32_t il = b / G, j = b % G, x = 2 * G * il + j;
+            dwt_butterfly<INV>(t, x, x + G, r[ridx + (base + x) / (2 * G)], last, scalar);
+        }
+        __syncthreads();
+    }
+    for (uint32_t k = threadIdx.x; k < len; k += TPB) a[k] = t[k];
+}
+
+template <bool INV>
+__global__ __launch_bounds__(TPB) void fft_cols_kernel(double2 *__restrict__ a, const double2 *__restrict__ r,
+                                                      int logn, int kr, double scalar) {
+    __shared__ double2 t[FFT_TILE];
+    const int kc = logn - kr;
+    const uint32_t n = 1u << logn, rows = 1u << kc, W = FFT_TILE >> kc, c0 = blockIdx.x * W;
+    a += ((size_t)blockIdx.y << logn) + c0;
+    for (uint32_t e = threadIdx.x; e < FFT_TILE; e += TPB) {
+        const uint32_t k = e / W, c = e % W;
+        t[e] = a[((size_t)k << kr) + c];
+    }
+    __syncthreads();
+    for (uint32_t step = 0; step < (uint32_t)kc; step++) {
+        const uint32_t g = INV ? 1u << step : rows >> (step + 1), G = g << kr;
+        const uint32_t ridx = INV ? n - n / G + 1 : n / (2 * G);
+        const bool last = INV && 2 * G == n;
+        for (uint32_t b = threadIdx.x; b < FFT_TILE / 2; b += TPB) {
+            const uint32_t c = b % W, kb = b / W, il = kb / g, k = 2 * g * il + kb % g;
+            dwt_butterfly<INV>(t, k * W + c, (k + g) * W + c, r[ridx + il], last, scalar);
+        }
+        __syncthreads();
+    }
+    for (uint32_t e = threadIdx.x; e < FFT_TILE; e += TPB) {
+        const uint32_t k = e / W, c = e % W;
+        a[((size_t)k << kr) + c] = t[e];
+    }
+}
+
+// The whole DWT of `batch` plaintexts of N = 2^logn values at a (inverse: scalar folded
+// into the last layer).
+template <bool INV>
+void launch_dwt(double2 *a, const double2 *r, int logn, uint32_t batch, double scalar, hipStream_t st) {
+    const int kr = logn < 12 ? logn : 12, kc = logn - kr;
+    const dim3 rows_grid(1u << kc, batch), cols_grid(kc ? (1u << kr) / (FFT_TILE >> kc) : 0, batch);
+    if (INV) {
+        fft_rows_kernel<true><<<rows_grid, TPB, 0, st>>>(a, r, logn, kr, scalar);
+        if (kc) fft_cols_kernel<true><<<cols_grid, TPB, 0, st>>>(a, r, logn, kr, scalar);
+    } else {
+        if (kc) fft_cols_kernel<false><<<cols_grid, TPB, 0, st>>>(a, r, logn, kr, scalar);
+        fft_rows_kernel<false><<<rows_grid, TPB, 0, st>>>(a, r, logn, kr, scalar);
     }
 }
 
@@ -344,12 +409,7 @@ void encode_body(const Context &c, const double *values, uint32_t count, size_t 
     auto *maxbits = reinterpret_cast<unsigned long long *>(static_cast<char *>(ws) + batch * n * 16);
     const double2 *irp = c.ckks->roots.as<const double2>() + n;
     scatter_kernel<<<dim3(blocks_for(n / 2), batch), TPB, 0, st>>>(values, count, vstride, a, c.logn);
-    uint32_t gap = 1, ridx = 1;
-    for (uint32_t m = (uint32_t)(n >> 1); m >= 1; m >>= 1, gap <<= 1) {
-        fft_inv_layer<<<dim3(blocks_for(n / 2), batch), TPB, 0, st>>>(a, irp, m, gap, ridx, scale / (double)n, m == 1,
-                                                                      (uint32_t)(n / 2));
-        ridx += m;
-    }
+    launch_dwt<true>(a, irp, c.logn, batch, scale / (double)n, st);
     clear_words_kernel<<<1, TPB, 0, st>>>(maxbits, batch);
     round_reduce_kernel<<<dim3(blocks_for(n), batch), TPB, 0, st>>>(a, plain, c.primes(), L, c.logn, maxbits);
     launch_check("ckks encode");
@@ -407,15 +467,10 @@ void ckks_decode_many(const Context &c, const uint64_t *plain, uint32_t L, uint3
     const size_t n = c.n;
     uint64_t *cp = static_cast<uint64_t *>(ws);
     double2 *a = reinterpret_cast<double2 *>(static_cast<char *>(ws) + (((size_t)batch * L * n * 8 + 255) & ~size_t(255)));
-    SHC_HIP(hipMemcpyAsync(cp, plain, (size_t)batch * L * n * 8, hipMemcpyDeviceToDevice, st));
-    ntt_inverse(cp, batch * L, L, c.primes(), c.logn, NTT_FULL, st);
+    const LimbGrid lg{1, L, L, batch};  // the plaintexts' limbs, transformed out of place
+    ntt_inverse_from(plain, lg, cp, lg, c.primes(), c.logn, NTT_FULL, st);
     compose_kernel<<<dim3(blocks_for(n), batch), TPB, 0, st>>>(cp, a, T, c.primes(), c.logn, 1.0 / scale);
-    const double2 *rp = c.ckks->roots.as<const double2>();
-    uint32_t gap = (uint32_t)(n >> 1), ridx = 1;
-    for (uint32_t m = 1; m < n; m <<= 1, gap >>= 1) {
-        fft_fwd_layer<<<dim3(blocks_for(n / 2), batch), TPB, 0, st>>>(a, rp, gap, ridx, (uint32_t)(n / 2));
-        ridx += m;
-    }
+    launch_dwt<false>(a, c.ckks->roots.as<const double2>(), c.logn, batch, 0.0, st);
     gather_kernel<<<dim3(blocks_for(n / 2), batch), TPB, 0, st>>>(a, values, c.logn);
     launch_check("ckks decode");
 }
