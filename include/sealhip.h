@@ -335,7 +335,7 @@ int shc_ckks_encode(const shc_context *ctx, const double *d_values, uint32_t cou
 int shc_ckks_encode_deferred(const shc_context *ctx, const double *d_values, uint32_t count, double scale,
                              uint32_t limb_count, uint64_t *d_plain, int32_t *d_status, void *d_workspace,
                              size_t workspace_bytes, void *stream);
-/* Batches of plaintexts, one launch per FFT layer for the whole batch: plaintext p's
+/* Batches of plaintexts, two LDS-tiled DWT passes for the whole batch: plaintext p's
  * `count` (re, im) pairs at d_values + p * values_stride (doubles), its NTT-form words at
  * d_plain + p * limb_count * N.  d_status: NULL raises the range check like
  * shc_ckks_encode (one wait for the batch), else one int32 per plaintext as

@@ -30,7 +30,7 @@ void ckks_encode(const Context &c, const double *values, uint32_t count, double 
 // zero plaintext) instead of raised: nothing waits for the stream.
 void ckks_encode_deferred(const Context &c, const double *values, uint32_t count, double scale, uint32_t L,
                           uint64_t *plain, int *d_status, void *ws, size_t ws_bytes, hipStream_t st);
-// Batches: `batch` plaintexts in one launch per FFT layer.  Encode: values of plaintext p
+// Batches: `batch` plaintexts in each launch (the DWT in two LDS-tiled passes).  Encode: values of plaintext p
 // at values + p * vstride (count pairs each), plain [batch][L][N]; d_status (optional, one
 // int per plaintext) takes the range checks on the device.  Decode: plain [batch][L][N]
 // -> values [batch][N/2] pairs.
