@@ -648,6 +648,8 @@ void encrypt_chunk(const Context &c, const uint64_t *pk, const uint64_t *plain, 
             launch_check("enc_products");
             ntt_inverse(P, m * 2, 1, primes + L, logn, NTT_FULL, st);
         }
+        const EncArgs ea{Es, drop ? P : nullptr, pk, U, plain, out, dc, p, K, L1, L, drop};
+        if (ntt_encrypt_ckks(ea, m, primes, logn, st)) return;
         enc_error_kernel<<<blocks_for(out_words), TPB, 0, st>>>(Es, drop ? P : nullptr, out, primes, dc, p, L, logn,
                                                                 out_words);
         launch_check("enc_error");

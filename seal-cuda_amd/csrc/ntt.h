@@ -105,6 +105,27 @@ bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &args, uint32_t cou
 bool ntt_moddown_polys(uint64_t *U, const ModDownArgs &args, uint32_t polys, const DevPrime *primes, int logn,
                        hipStream_t stream);
 
+// CKKS public-key encryption of m ciphertexts fused into the forward transform of its
+// error term (N >= 2^13; false and nothing done otherwise).  Polynomial j = 2g + k of
+// ciphertext g: the columns pass computes x = e_k - r_k (mod q_i) on load -- r_k the
+// divide-and-round correction of the dropped prime p from Pl[g][k] + e_k (mod p), when
+// drop -- and the rows pass stores out[g][k][i] = (pk_k[i] U[g][i] + NTT_i(x)) (times
+// p^-1 when drop) (+ plain[g][i] for k = 0).  Es: [m][2][N] signed errors; Pl: [m][2][N]
+// coefficient form; U: [m][L1][N] NTT(u); pk: [2][K][N]; out: [m][2][L][N].
+struct EncArgs {
+    const int64_t *Es;
+    const uint64_t *Pl;
+    const uint64_t *pk;
+    const uint64_t *U;
+    const uint64_t *plain;
+    uint64_t *out;
+    const DivRoundConst *dc;
+    uint64_t p;
+    uint32_t K, L1, L;
+    bool drop;
+};
+bool ntt_encrypt_ckks(const EncArgs &args, uint32_t m, const DevPrime *primes, int logn, hipStream_t stream);
+
 // ---- grouped launches (ciphertexts at several levels, one launch per stage) ----------
 // At most MAX_GROUPS groups per call; every group is one level's batch with its own
 // buffers.  A grouped call gives each group the words of the one-level call.

@@ -822,6 +822,62 @@ __global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_moddown_kernel(const
         });
 }
 
+// ---- CKKS public-key encryption fused into the error term's forward transform ----
+__device__ __forceinline__ uint64_t enc_lift(int64_t v, uint64_t q) { return v < 0 ? q - (uint64_t)(-v) : (uint64_t)v; }
+
+template <int LOGN, int K, int E, int SUBS>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_cols_encrypt_kernel(const DevPrime *__restrict__ primes,
+                                                                          EncArgs a, uint32_t jn) {
+    // every prime of polynomial j reads the same error and last-limb words: i fastest
+    cols_body<LOGN, K, E, SUBS, false, true>(
+        blockIdx.x, primes, 0, jn,
+        [&](uint32_t i, uint32_t j, size_t off) {
+            const DevPrime &Q = primes[i];
+            const size_t w = ((size_t)j << LOGN) + off;
+            const int64_t e = a.Es[w];
+            uint64_t x = enc_lift(e, Q.q);
+            if (a.drop) x = submod(x, moddown_prep(addmod(a.Pl[w], enc_lift(e, a.p), a.p), a.p, Q, a.dc[i]), Q.q);
+            return x;
+        },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) { a.out[((size_t)(j * a.L + i) << LOGN) + off] = x; },
+        a.L);
+}
+
+template <int LOGN, int K, int E, int SUBS, bool MX>
+__global__ __launch_bounds__(SUBS << (K - E)) void ntt_rows_encrypt_kernel(const DevPrime *__restrict__ primes,
+                                                                          EncArgs a, uint32_t jn) {
+    rows_body<LOGN, K, E, SUBS, false, false, MX>(
+        blockIdx.x, primes, a.L, 0, jn,
+        [&](uint32_t i, uint32_t j, size_t off) { return a.out[((size_t)(j * a.L + i) << LOGN) + off]; },
+        [&](uint32_t i, uint32_t j, size_t off, uint64_t x) {
+            const DevPrime &Q = primes[i];
+            const uint32_t g = j >> 1, k = j & 1;
+            const uint64_t pu = mulmod(a.pk[((size_t)(k * a.K + i) << LOGN) + off],
+                                       a.U[((size_t)(g * a.L1 + i) << LOGN) + off], Q.q, Q.ratio_lo, Q.ratio_hi);
+            uint64_t v = addmod(pu, finish<false, true>(x, Q), Q.q);
+            if (a.drop) v = shoup(v, a.dc[i].inv, a.dc[i].inv_shoup, Q.q);
+            if (a.plain && k == 0) v = addmod(v, a.plain[((size_t)(g * a.L + i) << LOGN) + off], Q.q);
+            a.out[((size_t)(j * a.L + i) << LOGN) + off] = v;
+        });
+}
+
+template <int LOGN>
+void launch_encrypt(const EncArgs &a, uint32_t jn, const DevPrime *primes, hipStream_t st) {
+    constexpr int K1 = LOGN / 2;
+    constexpr int K2 = LOGN - K1;
+    constexpr int E = 4;
+    constexpr int CS = 16;
+    constexpr int RS = 4096 >> K2;
+    const uint32_t col_blocks = a.L * jn * ((1u << K2) / CS);
+    const bool mx = rows_mixed<LOGN, K2, RS>(jn);
+    const uint32_t row_blocks = a.L * rows_blocks_per_prime<LOGN, K2, E, RS>(jn, mx);
+    ntt_cols_encrypt_kernel<LOGN, K1, E, CS><<<col_blocks, CS << (K1 - E), 0, st>>>(primes, a, jn);
+    if (mx)
+        ntt_rows_encrypt_kernel<LOGN, K2, E, RS, true><<<row_blocks, RS << (K2 - E), 0, st>>>(primes, a, jn);
+    else
+        ntt_rows_encrypt_kernel<LOGN, K2, E, RS, false><<<row_blocks, RS << (K2 - E), 0, st>>>(primes, a, jn);
+}
+
 struct ModDownGroupDev {
     uint64_t *U;
     ModDownArgs a;
@@ -1406,6 +1462,21 @@ bool moddown_jn(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t jn, con
 bool ntt_moddown(bool ckks, uint64_t *buf, const ModDownArgs &a, uint32_t count, const DevPrime *primes, int logn,
                  hipStream_t st) {
     return moddown_jn(ckks, buf, a, 2 * count, primes, logn, st);
+}
+
+bool ntt_encrypt_ckks(const EncArgs &a, uint32_t m, const DevPrime *primes, int logn, hipStream_t st) {
+    if (logn < 13 || logn > 17 || m == 0) return false;
+    const uint32_t jn = 2 * m;
+    if ((uint64_t)jn * a.L > ((uint64_t(1) << 34) >> logn)) return false;
+    switch (logn) {
+        case 13: launch_encrypt<13>(a, jn, primes, st); break;
+        case 14: launch_encrypt<14>(a, jn, primes, st); break;
+        case 15: launch_encrypt<15>(a, jn, primes, st); break;
+        case 16: launch_encrypt<16>(a, jn, primes, st); break;
+        case 17: launch_encrypt<17>(a, jn, primes, st); break;
+    }
+    launch_check("ntt encrypt launch");
+    return true;
 }
 
 bool ntt_moddown_polys(uint64_t *U, const ModDownArgs &a, uint32_t polys, const DevPrime *primes, int logn,

@@ -356,14 +356,15 @@ def expected_encryption(scheme, key, pk, seed, L, moduli, logn, plain=None):
     return want
 
 
-@pytest.mark.parametrize("scheme,keyed", [(BFV, False), (CKKS, False), (BFV, True), (CKKS, True)])
-def test_encrypt_many_words_match_oracle(scheme, keyed):
+@pytest.mark.parametrize("scheme,keyed,logn", [(BFV, False, 12), (CKKS, False, 12), (BFV, True, 12), (CKKS, True, 12),
+                                               (CKKS, False, 14), (CKKS, True, 13), (BFV, False, 13)])
+def test_encrypt_many_words_match_oracle(scheme, keyed, logn):
     """shc_encrypt_many: ciphertext g equals the oracle's encryption with the streams of
     seed + g (u, e0, e1 sampled by the oracle, pk u + e, divide-and-round, plaintext)
     and shc_encrypt(seed + g), at the top level (special prime dropped), at level 1 (a
     data prime dropped) and at the key level (nothing dropped), with and without
-    plaintexts, in one pass and in slices of two."""
-    logn = 12
+    plaintexts, in one pass and in slices of two.  From N = 2^13 CKKS runs the
+    encryption fused into the error term's transform."""
     n = 1 << logn
     moduli = ck.coeff_modulus_create(n, [50, 40, 40, 50] if scheme == CKKS else [43, 43, 44, 44])
     ctx = hip().Context(scheme, n, moduli, T if scheme == BFV else 0)
