@@ -80,36 +80,84 @@ __global__ void plain_lift_kernel(const uint64_t *__restrict__ plain, uint64_t *
 }
 
 // out[h][ct][p][l][c] = sum_j plains[h][j][l][c] * cts[j][ct][p][l][c] mod q_l for the
-// H outputs h0 .. h0 + H - 1 (hn of them valid; the rest repeat the last, unstored):
-// each ciphertext word is read once for H outputs (the diagonal matrix-vector products'
-// giant steps), sums in three words (Acc3), folded mod q every 64 terms.
-template <int H>
-__global__ void plain_dot_many_kernel(const uint64_t *__restrict__ cts, const uint64_t *__restrict__ plains,
+// H outputs h0 .. h0 + H - 1 (hn of them valid; the rest repeat the last, unstored).
+// A thread owns one word position (l, c) of G consecutive polynomials (ct, p): each
+// ciphertext word it loads serves H outputs (the diagonal matrix-vector products' giant
+// steps) and each plaintext word G polynomials, so the plaintexts -- shared by every
+// ciphertext of the batch -- come from HBM polys / G times instead of polys times.  Sums
+// in three words (Acc3), folded mod q every 64 terms.  The G x H sums take ~150
+// registers: the launch bound lets them stay there (two waves per SIMD at least).
+//
+// xcd: the grid is exactly groups x (lw / TPB) blocks with lw / TPB a multiple of 8, and
+// block B (dispatched to XCD B % 8) takes position block 8 (B / 8 / groups) + B % 8 of
+// polynomial group (B / 8) % groups: consecutive blocks of one XCD share their plaintext
+// words, which then come from that XCD's L2 after the first.
+template <int H, int G>
+__global__ __launch_bounds__(TPB, 2) void plain_dot_many_kernel(const uint64_t *__restrict__ cts, const uint64_t *__restrict__ plains,
                                       uint64_t *__restrict__ out, const DevPrime *__restrict__ primes, uint32_t terms,
-                                      uint32_t h0, uint32_t hn, uint32_t L, int logn, size_t total) {
-    const size_t lw = (size_t)L << logn;
-    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
-        const size_t r = idx % lw;
-        const DevPrime &P = primes[r >> logn];
+                                      uint32_t h0, uint32_t hn, uint32_t polys, uint32_t L, int logn, size_t total,
+                                      bool xcd) {
+    const size_t lw = (size_t)L << logn, pw = (size_t)polys * lw;
+    size_t first = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (xcd) {
+        const uint32_t groups = (polys + G - 1) / G, k = blockIdx.x >> 3;
+        first = (size_t)(k % groups) * lw + ((size_t)(k / groups) * 8 + (blockIdx.x & 7)) * TPB + threadIdx.x;
+    }
+    for (size_t idx = first; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const uint32_t row = uniform_row(idx, logn), grp = row / L;  // idx = grp * lw + r
+        const size_t r = idx - (size_t)grp * lw;
+        const uint32_t p0 = grp * G, gn = min<uint32_t>(G, polys - p0);
+        const DevPrime &P = primes[row - grp * L];
         const uint64_t *pp[H];
 #pragma unroll
         for (int h = 0; h < H; h++) pp[h] = plains + (size_t)(h0 + min<uint32_t>(h, hn - 1)) * terms * lw + r;
-        Acc3 acc[H];
+        const uint64_t *cp = cts + (size_t)p0 * lw + r;
+        Acc3 acc[G][H];
 #pragma unroll
-        for (int h = 0; h < H; h++) acc[h] = Acc3{0, 0, 0};
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int h = 0; h < H; h++) acc[g][h] = Acc3{0, 0, 0};
         for (uint32_t j = 0; j < terms; j++) {
-            const uint64_t t = cts[(size_t)j * total + idx];
+            uint64_t pv[H];
 #pragma unroll
-            for (int h = 0; h < H; h++) mac(acc[h], t, pp[h][(size_t)j * lw]);
+            for (int h = 0; h < H; h++) pv[h] = pp[h][(size_t)j * lw];
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                if ((uint32_t)g < gn) {  // uniform: a wave shares its polynomial group
+                    const uint64_t t = cp[(size_t)j * pw + (size_t)g * lw];
+#pragma unroll
+                    for (int h = 0; h < H; h++) mac(acc[g][h], t, pv[h]);
+                }
+            }
             if ((j & 63) == 63) {
 #pragma unroll
-                for (int h = 0; h < H; h++) acc_fold(acc[h], P);
+                for (int g = 0; g < G; g++)
+#pragma unroll
+                    for (int h = 0; h < H; h++) acc_fold(acc[g][h], P);
             }
         }
 #pragma unroll
-        for (int h = 0; h < H; h++)
-            if ((uint32_t)h < hn) out[(size_t)(h0 + h) * total + idx] = acc_reduce(acc[h], P);
+        for (int g = 0; g < G; g++)
+            if ((uint32_t)g < gn)
+#pragma unroll
+                for (int h = 0; h < H; h++)
+                    if ((uint32_t)h < hn) out[(size_t)(h0 + h) * pw + (size_t)(p0 + g) * lw + r] = acc_reduce(acc[g][h], P);
     }
+}
+
+template <int H>
+void launch_plain_dot(uint32_t G, const uint64_t *cts, const uint64_t *plains, uint64_t *out, const DevPrime *primes,
+                      uint32_t terms, uint32_t h0, uint32_t hn, uint32_t polys, uint32_t L, int logn, hipStream_t st) {
+    const size_t groups = (polys + G - 1) / G, lw = (size_t)L << logn, total = groups * lw;
+    const size_t rblocks = lw / TPB;
+    // the XCD-aware order where the grid is one block per (group, position block)
+    const bool xcd = groups > 1 && lw % TPB == 0 && rblocks % 8 == 0 && groups * rblocks < (size_t(1) << 20);
+    if (G == 4)
+        plain_dot_many_kernel<H, 4><<<blocks_for(total), TPB, 0, st>>>(cts, plains, out, primes, terms, h0, hn, polys, L,
+                                                                        logn, total, xcd);
+    else
+        plain_dot_many_kernel<H, 1><<<blocks_for(total), TPB, 0, st>>>(cts, plains, out, primes, terms, h0, hn, polys, L,
+                                                                        logn, total, xcd);
 }
 
 void check_level(const Context &c, uint32_t L) {
@@ -193,16 +241,19 @@ void multiply_plain_accumulate_many(const Context &c, const uint64_t *cts, const
     check_level(c, L);
     if (!terms) fail(ERR_INVALID_ARGUMENT, "no terms to accumulate");
     if (!outputs) fail(ERR_INVALID_ARGUMENT, "no outputs to accumulate");
-    const size_t total = ((size_t)polys * L) << c.logn;
-    if (!total) return;
-    const uint32_t blocks = blocks_for(total);
+    if (!polys) return;
+    // four polynomials per thread once the batch has them (the plaintexts' reuse), one
+    // for a single ciphertext
+    const uint32_t G = polys >= 4 ? 4 : 1;
+    const DevPrime *primes = c.primes();
+    const int logn = c.logn;
     for (uint32_t h0 = 0; h0 < outputs;) {
         const uint32_t left = outputs - h0;
         const uint32_t hn = left >= 8 ? 8 : left;
-        if (hn > 4) plain_dot_many_kernel<8><<<blocks, TPB, 0, st>>>(cts, plains, out, c.primes(), terms, h0, hn, L, c.logn, total);
-        else if (hn > 2) plain_dot_many_kernel<4><<<blocks, TPB, 0, st>>>(cts, plains, out, c.primes(), terms, h0, hn, L, c.logn, total);
-        else if (hn == 2) plain_dot_many_kernel<2><<<blocks, TPB, 0, st>>>(cts, plains, out, c.primes(), terms, h0, hn, L, c.logn, total);
-        else plain_dot_many_kernel<1><<<blocks, TPB, 0, st>>>(cts, plains, out, c.primes(), terms, h0, hn, L, c.logn, total);
+        if (hn > 4) launch_plain_dot<8>(G, cts, plains, out, primes, terms, h0, hn, polys, L, logn, st);
+        else if (hn > 2) launch_plain_dot<4>(G, cts, plains, out, primes, terms, h0, hn, polys, L, logn, st);
+        else if (hn == 2) launch_plain_dot<2>(G, cts, plains, out, primes, terms, h0, hn, polys, L, logn, st);
+        else launch_plain_dot<1>(G, cts, plains, out, primes, terms, h0, hn, polys, L, logn, st);
         launch_check("plain_dot");
         h0 += hn;
     }

@@ -110,12 +110,19 @@ def test_plain_ops_reject_bad_arguments():
         ctx.bfv_add_plain(d.data_ptr(), d.data_ptr(), d.data_ptr(), 1, 2, 2, False, stream())   # CKKS context
 
 
-@pytest.mark.parametrize("scheme,terms,outputs", [(CKKS, 70, 11), (BFV, 5, 2)])
-def test_multiply_plain_accumulate_many(scheme, terms, outputs):
+@pytest.mark.parametrize("scheme,terms,outputs,polys,logn", [
+    (CKKS, 70, 11, 4, 10), (BFV, 5, 2, 4, 10),
+    (CKKS, 3, 3, 6, 10),    # a partial group of polynomials (4 + 2)
+    (CKKS, 65, 1, 9, 11),   # 4 + 4 + 1, across the fold
+    (BFV, 2, 5, 2, 10),     # one polynomial per thread
+    (CKKS, 4, 2, 5, 4),     # N = 16: rows shorter than a wave
+])
+def test_multiply_plain_accumulate_many(scheme, terms, outputs, polys, logn):
     """sum_j plains[o][j] * cts[j] for several outputs o at once equals the oracle's plain
     products and sums (70 terms cross the 64-term fold; 11 outputs take an 8-wide and a
-    partial 4-wide pass), and each output equals the single-output call."""
-    logn, L, polys = 10, 3, 4
+    partial 4-wide pass; polynomials in groups of four with partial groups), and each
+    output equals the single-output call."""
+    L = 3
     n = 1 << logn
     moduli = ck.coeff_modulus_create(n, [60, 50, 40, 60])
     ctx = hip().Context(scheme, n, moduli, 65537 if scheme == BFV else 0)
