@@ -15,7 +15,12 @@ shapes that the benchmarked workloads use and compare words with the oracle ther
 * baby-step giant-step matrix-vector products at the serving sizes: BFV N=8192
   (BFVDefault(8192) primes, t=65537), 4096 x 4096, 64 x 64; CKKS N=2^15, 1024 x 1024,
   32 x 32 -- words against the oracle's composition of rotations, plain products and
-  sums (app/matmul.cuh:98-121), and the decryption against M v.
+  sums (app/matmul.cuh:98-121), and the decryption against M v; the diagonal products
+  alone at the CKKS shape (polynomials in groups of four, XCD-ordered grid);
+* the per-operation workload's CKKS shape (N=2^16, level 24 of 60 x 25): batched
+  public-key encryption (fused into the error term's transform) against the oracle's
+  encryption of each seed, batched decryption, and the batched CKKSEncoder against its
+  single calls.
 
 Inputs are seeded; large keys are uniform words drawn on the device by the library's
 counter-based sampler and copied to the host for the oracle.  Oracle calls run on a
@@ -462,3 +467,106 @@ def test_bsgs_matvec_ckks_n32768_1024x1024():
     want_mv = M @ v
     for period in (0, slots // d - 1):
         np.testing.assert_allclose(vals[period * d:(period + 1) * d].real, want_mv, atol=1e-3)
+
+
+# ------------------------------------------- encryption, decryption, encoders (N=2^16, L=24)
+
+M64 = (1 << 64) - 1
+
+
+def _mix64(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def _sub_seed(seed, k):
+    return _mix64((seed ^ _mix64((k + 0x5EA15EED) & M64)) & M64)
+
+
+def test_encrypt_decrypt_encode_n65536_l24():
+    """The per-operation workload's shapes: shc_encrypt_many of three NTT-form plaintexts
+    at N=2^16, level 24 of Create(65536, {60 x 25}) (the encryption fused into the error
+    term's transform, the special prime dropped) equals the oracle's encryption with the
+    streams of each seed; shc_decrypt_many of those ciphertexts equals the oracle's
+    phase; shc_ckks_encode_many / shc_ckks_decode_many of three plaintexts equal their
+    single calls, and decode(encode(x)) is within 2^-20 of x."""
+    logn, n = 16, 1 << 16
+    moduli = ck.coeff_modulus_create(n, [60] * 25)
+    K, L = len(moduli), len(moduli) - 1
+    ctx = hip().Context(CKKS, n, moduli)
+    sk, pk = empty_dev((K, n)), empty_dev((2, K, n))
+    ctx.keygen_secret(3, sk.data_ptr(), stream())
+    ctx.keygen_public(sk.data_ptr(), 4, pk.data_ptr(), stream())
+    count, seed = 3, 77
+    plain = device_uniform(ctx, (count, L, n), L, 5)
+    out = empty_dev((count, 2, L, n))
+    w = ctx.encrypt_many_workspace_size(L, count)
+    wb = ws(w)
+    ctx.encrypt_many(pk.data_ptr(), plain.data_ptr(), count, L, seed, out.data_ptr(), wb.data_ptr(), w, stream())
+    got, pk_h, pl_h = from_dev(out), from_dev(pk), from_dev(plain)
+
+    def expected(g):
+        u = ck.sample_poly(0, None, _sub_seed(seed + g, 0), logn, moduli)
+        e0 = ck.sample_poly(1, None, _sub_seed(seed + g, 1), logn, moduli)
+        e1 = ck.sample_poly(1, None, _sub_seed(seed + g, 2), logn, moduli)
+        want = ck.rescale(CKKS, ck.encrypt_zero_asymmetric(CKKS, pk_h, u, e0, e1, moduli, logn), moduli, logn)
+        want[0] = ck.mod_add(want[0], pl_h[g], moduli)
+        return want
+
+    for g, want in enumerate(ck.parallel_map(expected, range(count))):
+        np.testing.assert_array_equal(got[g], want, err_msg=f"ciphertext {g}")
+    dec = empty_dev((count, L, n))
+    ctx.decrypt_many(out.data_ptr(), 2, count, sk.data_ptr(), L, dec.data_ptr(), 0, 0, stream())
+    sk_h = from_dev(sk)
+    phases = ck.parallel_map(lambda g: ck.decrypt_phase(CKKS, got[g], sk_h[:L], moduli, logn), range(count))
+    np.testing.assert_array_equal(from_dev(dec), np.stack(phases))
+
+    g = torch.Generator(device="cuda").manual_seed(9)
+    vals = torch.rand((count, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    cw = ctx.ckks_encode_many_workspace_size(count)
+    cb = ws(cw)
+    plains = empty_dev((count, L, n))
+    status = torch.zeros((count,), dtype=torch.int32, device="cuda")
+    ctx.ckks_encode_many(vals.data_ptr(), n // 2, n, count, 2.0 ** 40, L, plains.data_ptr(), status.data_ptr(),
+                         cb.data_ptr(), cw, stream())
+    ew1 = ctx.ckks_encode_workspace_size()
+    eb1 = ws(ew1)
+    one = empty_dev((L, n))
+    many = from_dev(plains)
+    assert not status.any()
+    for p in range(count):
+        ctx.ckks_encode(vals[p].data_ptr(), n // 2, 2.0 ** 40, L, one.data_ptr(), eb1.data_ptr(), ew1, stream())
+        np.testing.assert_array_equal(many[p], from_dev(one), err_msg=f"plaintext {p}")
+    dw = ctx.ckks_decode_many_workspace_size(L, count)
+    db = ws(dw)
+    back = torch.empty((count, n), dtype=torch.float64, device="cuda")
+    ctx.ckks_decode_many(plains.data_ptr(), L, count, 2.0 ** 40, back.data_ptr(), db.data_ptr(), dw, stream())
+    torch.cuda.synchronize()
+    assert float((back - vals).abs().max()) < 2.0 ** -20
+
+
+def test_plain_dot_many_matvec_shape():
+    """The diagonal products at the CKKS matvec's shape (N=2^15, 15 primes): 12
+    polynomials in groups of four over 4 terms, 9 outputs (an 8-wide and a 1-wide pass),
+    positions in the XCD-ordered grid, against the oracle's plain products and sums."""
+    logn, n = 15, 1 << 15
+    moduli = ck.coeff_modulus_create(n, [60] + [40] * 14 + [60])
+    L, terms, outputs, polys = 15, 4, 9, 12
+    ctx = hip().Context(CKKS, n, moduli)
+    cts = device_uniform(ctx, (terms, polys, L, n), L, 21)
+    plains = device_uniform(ctx, (outputs, terms, L, n), L, 22)
+    out = empty_dev((outputs, polys, L, n))
+    ctx.multiply_plain_accumulate_many(cts.data_ptr(), plains.data_ptr(), terms, outputs, polys, L, out.data_ptr(),
+                                       stream())
+    got, c_h, p_h = from_dev(out), from_dev(cts), from_dev(plains)
+
+    def expected(o):
+        want = None
+        for j in range(terms):
+            t = ck.multiply_plain_ntt(c_h[j], p_h[o, j], moduli[:L], logn)
+            want = t if want is None else ck.mod_add(want, t, moduli)
+        return want
+
+    for o, want in enumerate(ck.parallel_map(expected, range(outputs))):
+        np.testing.assert_array_equal(got[o], want, err_msg=f"output {o}")
