@@ -245,6 +245,56 @@ __global__ void phase_kernel(const uint64_t *__restrict__ c0, const uint64_t *__
     }
 }
 
+// phase_kernel with two words per thread and 16-byte accesses (every pointer and
+// stride 16-byte aligned; a pair never straddles two limbs as N is even).
+__global__ void phase_pairs_kernel(const uint64_t *__restrict__ c0, const uint64_t *__restrict__ cs, size_t c_stride,
+                                   size_t in_stride, uint32_t size, const uint64_t *__restrict__ sk,
+                                   uint64_t *__restrict__ phase, const DevPrime *__restrict__ primes, int logn,
+                                   uint32_t L, size_t pairs) {
+    const size_t poly = (size_t)L << logn;
+    for (size_t p = (size_t)blockIdx.x * TPB + threadIdx.x; p < pairs; p += (size_t)gridDim.x * TPB) {
+        const size_t w = 2 * p;
+        uint32_t r = (uint32_t)(w >> logn);
+        if (logn >= 7) r = __builtin_amdgcn_readfirstlane(r);  // a wave's 128 words in one limb
+        const uint32_t g = r / L;
+        const size_t i = w - g * poly, in = g * in_stride + i;
+        const DevPrime &P = primes[r - g * L];
+        const ulonglong2 s = *reinterpret_cast<const ulonglong2 *>(sk + i);
+        uint64_t acc0 = 0, acc1 = 0, sp0 = s.x, sp1 = s.y;
+        for (uint32_t k = 1; k < size; k++) {
+            const ulonglong2 x = *reinterpret_cast<const ulonglong2 *>(cs + (k - 1) * c_stride + in);
+            acc0 = addmod(acc0, mulmod(x.x, sp0, P.q, P.ratio_lo, P.ratio_hi), P.q);
+            acc1 = addmod(acc1, mulmod(x.y, sp1, P.q, P.ratio_lo, P.ratio_hi), P.q);
+            if (k + 1 < size) {
+                sp0 = mulmod(sp0, s.x, P.q, P.ratio_lo, P.ratio_hi);
+                sp1 = mulmod(sp1, s.y, P.q, P.ratio_lo, P.ratio_hi);
+            }
+        }
+        if (c0) {
+            const ulonglong2 x = *reinterpret_cast<const ulonglong2 *>(c0 + in);
+            acc0 = addmod(acc0, x.x, P.q);
+            acc1 = addmod(acc1, x.y, P.q);
+        }
+        *reinterpret_cast<ulonglong2 *>(phase + w) = make_ulonglong2(acc0, acc1);
+    }
+}
+
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+// phase_kernel's launch, in pairs where the pointers allow
+void launch_phase(const uint64_t *c0, const uint64_t *cs, size_t c_stride, size_t in_stride, uint32_t size,
+                  const uint64_t *sk, uint64_t *phase, const DevPrime *primes, int logn, uint32_t L, size_t total,
+                  hipStream_t st) {
+    if (logn >= 1 && (!c0 || aligned16(c0)) && aligned16(cs) && aligned16(sk) && aligned16(phase)) {
+        phase_pairs_kernel<<<blocks_for(total / 2), TPB, 0, st>>>(c0, cs, c_stride, in_stride, size, sk, phase, primes,
+                                                                  logn, L, total / 2);
+    } else {
+        phase_kernel<<<blocks_for(total), TPB, 0, st>>>(c0, cs, c_stride, in_stride, size, sk, phase, primes, logn, L,
+                                                        total);
+    }
+    launch_check("phase");
+}
+
 // RNSTool::decryptScaleAndRound (src/utils/rns.cpp:1039-1095) per coefficient of
 // phase[h] + c0[h] (coefficient form, L = B.L limbs) for `count` ciphertexts: phase of
 // ciphertext h at phase + h * L * N, its c0 at c0 + h * c0_stride, output at out + h * N.
@@ -717,9 +767,7 @@ void decrypt_many(const Context &c, const uint64_t *ct, uint32_t size, uint32_t 
     const DevPrime *primes = c.primes();
     if (c.scheme == SCHEME_CKKS) {
         const size_t total = (size_t)count * poly;
-        phase_kernel<<<blocks_for(total), TPB, 0, st>>>(ct, ct + poly, poly, ct_words, size, sk, out, primes, c.logn,
-                                                        L, total);
-        launch_check("phase");
+        launch_phase(ct, ct + poly, poly, ct_words, size, sk, out, primes, c.logn, L, total, st);
         return;
     }
     // BFV: phase = INTT(sum_k NTT(c_k) s^k) + c0, then scale-and-round, per chunk of m
@@ -736,9 +784,7 @@ void decrypt_many(const Context &c, const uint64_t *ct, uint32_t size, uint32_t 
         launch_check("gather_rows");
         ntt_forward(tmp, m * (size - 1) * L, L, primes, c.logn, NTT_FULL, st);
         const size_t total = (size_t)m * poly;
-        phase_kernel<<<blocks_for(total), TPB, 0, st>>>(nullptr, tmp, poly, (size - 1) * poly, size, sk, phase, primes,
-                                                        c.logn, L, total);
-        launch_check("phase");
+        launch_phase(nullptr, tmp, poly, (size - 1) * poly, size, sk, phase, primes, c.logn, L, total, st);
         ntt_inverse(phase, m * L, L, primes, c.logn, NTT_FULL, st);
         scale_round_kernel<<<blocks_for((size_t)m * n), TPB, 0, st>>>(phase, src, ct_words, out + g0 * n, m, B, primes,
                                                                       c.logn);

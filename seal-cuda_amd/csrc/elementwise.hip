@@ -44,6 +44,65 @@ __global__ void binary_kernel(const uint64_t *__restrict__ a, const uint64_t *__
     }
 }
 
+// The same two words per thread with 16-byte accesses (all pointers 16-byte aligned;
+// N is even, so a pair never straddles two limbs).  From N = 128 a wave's 128 words lie
+// in one limb, and the limb index is wave-uniform.
+__device__ __forceinline__ uint32_t pair_limb(size_t w, int logn, uint32_t L) {
+    uint32_t r = (uint32_t)(w >> logn);
+    if (logn >= 7) r = __builtin_amdgcn_readfirstlane(r);
+    return r % L;
+}
+
+template <int OP>
+__global__ void binary_pairs_kernel(const ulonglong2 *__restrict__ a, const uint64_t *__restrict__ b, ulonglong2 *out,
+                                    const DevPrime *__restrict__ primes, uint32_t L, int logn, size_t pairs) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t p = (size_t)blockIdx.x * TPB + threadIdx.x; p < pairs; p += (size_t)gridDim.x * TPB) {
+        const size_t w = 2 * p;
+        const uint32_t l = pair_limb(w, logn, L);
+        const DevPrime &P = primes[l];
+        const ulonglong2 x = a[p];
+        const ulonglong2 y = OP == BIN_MUL_PLAIN ? *reinterpret_cast<const ulonglong2 *>(b + ((size_t)l << logn) + (w & (n - 1)))
+                                                 : reinterpret_cast<const ulonglong2 *>(b)[p];
+        ulonglong2 o;
+        if (OP == BIN_ADD) {
+            o.x = addmod(x.x, y.x, P.q);
+            o.y = addmod(x.y, y.y, P.q);
+        } else if (OP == BIN_SUB) {
+            o.x = submod(x.x, y.x, P.q);
+            o.y = submod(x.y, y.y, P.q);
+        } else {
+            o.x = mulmod(x.x, y.x, P.q, P.ratio_lo, P.ratio_hi);
+            o.y = mulmod(x.y, y.y, P.q, P.ratio_lo, P.ratio_hi);
+        }
+        out[p] = o;
+    }
+}
+
+__global__ void negate_pairs_kernel(const ulonglong2 *__restrict__ a, ulonglong2 *out,
+                                    const DevPrime *__restrict__ primes, uint32_t L, int logn, size_t pairs) {
+    for (size_t p = (size_t)blockIdx.x * TPB + threadIdx.x; p < pairs; p += (size_t)gridDim.x * TPB) {
+        const uint64_t q = primes[pair_limb(2 * p, logn, L)].q;
+        const ulonglong2 x = a[p];
+        out[p] = make_ulonglong2(negmod(x.x, q), negmod(x.y, q));
+    }
+}
+
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+template <int OP>
+void launch_binary(const uint64_t *a, const uint64_t *b, uint64_t *out, const DevPrime *primes, uint32_t L, int logn,
+                   size_t total, hipStream_t st) {
+    if (logn >= 1 && aligned16(a) && aligned16(b) && aligned16(out)) {
+        const size_t pairs = total / 2;
+        binary_pairs_kernel<OP><<<blocks_for(pairs), TPB, 0, st>>>(reinterpret_cast<const ulonglong2 *>(a), b,
+                                                                    reinterpret_cast<ulonglong2 *>(out), primes, L,
+                                                                    logn, pairs);
+    } else {
+        binary_kernel<OP><<<blocks_for(total), TPB, 0, st>>>(a, b, out, primes, L, logn, total);
+    }
+}
+
 __global__ void negate_kernel(const uint64_t *__restrict__ a, uint64_t *out, const DevPrime *__restrict__ primes,
                               uint32_t L, int logn, size_t total) {
     for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB)
@@ -171,7 +230,7 @@ void poly_add(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *
     check_level(c, L);
     const size_t total = ((size_t)polys * L) << c.logn;
     if (!total) return;
-    binary_kernel<BIN_ADD><<<blocks_for(total), TPB, 0, st>>>(a, b, out, c.primes(), L, c.logn, total);
+    launch_binary<BIN_ADD>(a, b, out, c.primes(), L, c.logn, total, st);
     launch_check("add");
 }
 
@@ -180,7 +239,7 @@ void poly_sub(const Context &c, const uint64_t *a, const uint64_t *b, uint64_t *
     check_level(c, L);
     const size_t total = ((size_t)polys * L) << c.logn;
     if (!total) return;
-    binary_kernel<BIN_SUB><<<blocks_for(total), TPB, 0, st>>>(a, b, out, c.primes(), L, c.logn, total);
+    launch_binary<BIN_SUB>(a, b, out, c.primes(), L, c.logn, total, st);
     launch_check("sub");
 }
 
@@ -188,7 +247,12 @@ void poly_negate(const Context &c, const uint64_t *a, uint64_t *out, uint32_t po
     check_level(c, L);
     const size_t total = ((size_t)polys * L) << c.logn;
     if (!total) return;
-    negate_kernel<<<blocks_for(total), TPB, 0, st>>>(a, out, c.primes(), L, c.logn, total);
+    if (c.logn >= 1 && aligned16(a) && aligned16(out))
+        negate_pairs_kernel<<<blocks_for(total / 2), TPB, 0, st>>>(reinterpret_cast<const ulonglong2 *>(a),
+                                                                   reinterpret_cast<ulonglong2 *>(out), c.primes(), L,
+                                                                   c.logn, total / 2);
+    else
+        negate_kernel<<<blocks_for(total), TPB, 0, st>>>(a, out, c.primes(), L, c.logn, total);
     launch_check("negate");
 }
 
@@ -197,7 +261,7 @@ void multiply_plain_ntt(const Context &c, const uint64_t *a, const uint64_t *pla
     check_level(c, L);
     const size_t total = ((size_t)polys * L) << c.logn;
     if (!total) return;
-    binary_kernel<BIN_MUL_PLAIN><<<blocks_for(total), TPB, 0, st>>>(a, plain, out, c.primes(), L, c.logn, total);
+    launch_binary<BIN_MUL_PLAIN>(a, plain, out, c.primes(), L, c.logn, total, st);
     launch_check("multiply_plain");
 }
 

@@ -24,6 +24,7 @@ def ws(nbytes):
     (CKKS, 12, [60, 40, 40, 60], 3, 3),
     (BFV, 13, [43, 43, 44, 44, 44], 4, 2),
     (CKKS, 16, [60] * 5, 4, 2),
+    (CKKS, 3, [30, 30, 30], 3, 3),    # N = 8: limbs shorter than a wave's words
 ])
 def test_add_sub_negate(scheme, logn, bits, L, count):
     n = 1 << logn
@@ -44,6 +45,33 @@ def test_add_sub_negate(scheme, logn, bits, L, count):
     # in place: a += b
     ctx.add(da.data_ptr(), db.data_ptr(), da.data_ptr(), polys, L, stream())
     assert np.array_equal(from_dev(da), ck.mod_add(a, b, moduli))
+
+
+def test_elementwise_at_odd_word_offsets():
+    """Operands at an odd word offset (8-byte aligned only) take the one-word-per-thread
+    kernels and give the words of the 16-byte-aligned ones."""
+    logn, L, polys = 11, 2, 3
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [50, 50, 60])
+    ctx = hip().Context(CKKS, n, moduli)
+    rng = np.random.default_rng(4)
+    a = ck.uniform_residues(rng, (polys, L, n), moduli[:L])
+    b = ck.uniform_residues(rng, (polys, L, n), moduli[:L])
+    plain = ck.uniform_residues(rng, (L, n), moduli[:L])
+    words = polys * L * n
+    buf = empty_dev((4 * words + 3 * L * n + 8,))
+    da, db, dout, dpl = buf[1:1 + words], buf[3 + words:3 + 2 * words], buf[5 + 2 * words:5 + 3 * words], \
+        buf[7 + 3 * words:7 + 3 * words + L * n]
+    da.copy_(to_dev(a).view(-1))
+    db.copy_(to_dev(b).view(-1))
+    dpl.copy_(to_dev(plain).view(-1))
+    for op, want in ((ctx.add, ck.mod_add(a, b, moduli)), (ctx.sub, ck.mod_sub(a, b, moduli))):
+        op(da.data_ptr(), db.data_ptr(), dout.data_ptr(), polys, L, stream())
+        np.testing.assert_array_equal(from_dev(dout).reshape(polys, L, n), want)
+    ctx.negate(da.data_ptr(), dout.data_ptr(), polys, L, stream())
+    np.testing.assert_array_equal(from_dev(dout).reshape(polys, L, n), ck.mod_neg(a, moduli))
+    ctx.multiply_plain_ntt(da.data_ptr(), dpl.data_ptr(), dout.data_ptr(), polys, L, stream())
+    np.testing.assert_array_equal(from_dev(dout).reshape(polys, L, n), ck.multiply_plain_ntt(a, plain, moduli, logn))
 
 
 @pytest.mark.parametrize("logn,bits,L,count", [(12, [60, 40, 40, 60], 3, 3), (15, [60] + [40] * 6 + [60], 7, 2)])
