@@ -20,8 +20,9 @@ against the same sum through separate rotations.
 `--workload matvec`: CKKS 1024 x 1024 matrix-vector products by baby-step giant-step
 rotations (N=2^15, batches of 16 requests).
 
-`--workload primitives`: rotation, rescale, encryption, decryption and both encoders,
-each timed alone (CKKS at the headline shape, BFV at N=8192), batches of 128.
+`--workload primitives`: rotation, rescale, encryption, decryption, both encoders, key
+generation and serialization, each timed alone (CKKS at the headline shape, BFV at
+N=8192), batches of 128.
 
 `--workload copy`: a 1 GiB device copy, the calibration run for HBM byte counters.
 
@@ -238,6 +239,20 @@ def facade_leg(ops=128, warmup=16):
     res["path"] = ("troyn::Evaluator multiply + relinearizeInplace per ciphertext (C++ API over the C ABI, "
                    "default stream, pooled device blocks), CKKS N=2^16, 24+1 primes")
     return res
+
+
+def facade_io_leg(reps=8):
+    """Ciphertext and relinearization-key save / load and key generation through the
+    troyn:: API at the headline shape (seal-cuda_amd/bench_facade io, a separate process
+    like the drop-in timing)."""
+    import subprocess
+    exe = os.path.join(HERE, "seal-cuda_amd", "bench_facade")
+    if not os.path.exists(exe):
+        return {"error": "bench_facade not built"}
+    r = subprocess.run([exe, "io", str(reps)], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        return {"error": f"bench_facade io exited {r.returncode}: {r.stderr[-300:]}"}
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def bench_mulrelin(args):
@@ -1565,6 +1580,54 @@ def bench_primitives(args):
            f"{B}", "shc_ckks_decode_many", make_dec_ckks, "one CKKSEncoder::decode per call")
     del db, back, vals, plain, status
     torch.cuda.empty_cache()
+
+    # relinearization key generation at the key level: K - 1 digits x 2 x K limbs
+    sk = dev((K, n))
+    ctx.keygen_secret(11, sk.data_ptr(), st)
+    rk = dev((K - 1, 2, K, n))
+    kw = ctx.keygen_workspace_size()
+    kb = wsb(kw)
+    ms = timed(lambda: ctx.keygen_relin(sk.data_ptr(), 12, rk.data_ptr(), kb.data_ptr(), kw, st))
+    h_sk = sk.cpu().numpy().view(np.uint64).copy()
+
+    def make_keygen(kind):
+        s_ = ref_sess(kind, CKKS, n, moduli)
+        if s_ is not None:
+            return s_.keygen_relin
+        r_ = np.random.default_rng(3)
+        a = ck.uniform_residues(r_, (K - 1, K, n), moduli)
+        e = ck.rns_of_small(r_.integers(-3, 4, size=n), moduli)
+        e = np.ascontiguousarray(np.broadcast_to(e, (K - 1, K, n)))
+        return lambda: ck.generate_kswitch_key(h_sk, h_sk, a, e, moduli, logn)
+
+    record("ckks_relin_keygen", "relinearization keys/s (KeyGenerator::createRelinKeys), " + shape.split(", batch")[0],
+           "keys/s", 1, ms, (2 * K * (K - 1) + K) * limb,
+           "one relinearization key at N=2^16, 24+1 primes of 60 bits (24 digits x 2 x 25 limbs, 629 MB)",
+           "shc_keygen_relin", make_keygen, "one KeyGenerator + createRelinKeys per call (its secret key included)")
+    del rk, kb, sk
+    torch.cuda.empty_cache()
+    io = facade_io_leg()
+    if "error" not in io:
+        for name, what, key, nbytes in (("ckks_ciphertext_save", "Ciphertext::save", "ct_save_ms", io["ct_bytes"]),
+                                        ("ckks_ciphertext_load", "Ciphertext::load", "ct_load_ms", io["ct_bytes"])):
+            t_ms = io[key]
+            ops[name] = {"metric": f"{what} of one level-24 ciphertext (N=2^16, 2 x 24 limbs) per second, through a "
+                                   "std::stringstream (troyn:: API, device words in and out)",
+                         "value": 1e3 / t_ms, "unit": "ciphertexts/s", "ms_per_step": t_ms, "steps": io["reps"],
+                         "bytes": nbytes, "host_gb_s": nbytes / (t_ms * 1e-3) / 1e9,
+                         "bound": "the PCIe copy of the words between device and host and the host stream copy",
+                         "cpu_baseline": None,
+                         "cpu_baseline_note": "the reference's CPU Ciphertext has save / load commented out "
+                                              "(src/ciphertext.h:453-520); its serialization is in the CUDA classes "
+                                              "only, so no CPU path exists to time",
+                         "relin_key": {"bytes": io["relin_key_bytes"], "save_ms": io["relin_key_save_ms"],
+                                       "load_ms": io["relin_key_load_ms"]},
+                         "self_check": "save -> load -> save gives the same bytes (ciphertext and relinearization key)"}
+        ops["ckks_relin_keygen"]["facade"] = {"relin_keygen_ms": io["relin_keygen_ms"],
+                                              "galois_keygen_ms": io["galois_keygen_ms"],
+                                              "path": "troyn::KeyGenerator createRelinKeys / createGaloisKeys({1})"}
+    else:
+        ops["ckks_ciphertext_save"] = io
 
     # ---------------------------------------------------------------- BFV, N = 8192
     logn, n, t = 13, 8192, T_BFV

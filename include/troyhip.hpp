@@ -852,6 +852,49 @@ inline void get_words(std::istream &is, std::vector<uint64_t> &h, size_t words) 
     if (!is) throw std::invalid_argument("stream ended before the ciphertext did");
 }
 
+// Device words to and from a stream through one pinned staging buffer (hipHostMalloc,
+// 8 MiB, allocated on first use and kept for the process; calls take it in turn): an
+// object moves in chunks at pinned-copy speed instead of through a pageable host copy of
+// the whole object.
+struct PinnedStage {
+    static constexpr size_t BYTES = size_t(8) << 20;
+    std::mutex mu;
+    char *buf = nullptr;
+    static PinnedStage &get() {
+        static PinnedStage *s = new PinnedStage;  // never destroyed: outlives the runtime's teardown
+        return *s;
+    }
+    char *take() {
+        if (!buf) hip_check(hipHostMalloc(reinterpret_cast<void **>(&buf), BYTES), "pinned staging buffer");
+        return buf;
+    }
+};
+
+inline void write_device_words(std::ostream &os, const uint64_t *d, size_t words) {
+    PinnedStage &s = PinnedStage::get();
+    std::lock_guard<std::mutex> g(s.mu);
+    char *b = s.take();
+    const size_t per = PinnedStage::BYTES / 8;
+    for (size_t done = 0; done < words; done += per) {
+        const size_t k = std::min(per, words - done);
+        hip_check(hipMemcpy(b, d + done, k * 8, hipMemcpyDeviceToHost), "copy to host");
+        os.write(b, (std::streamsize)(k * 8));
+    }
+}
+
+inline void read_device_words(std::istream &is, uint64_t *d, size_t words, const char *what) {
+    PinnedStage &s = PinnedStage::get();
+    std::lock_guard<std::mutex> g(s.mu);
+    char *b = s.take();
+    const size_t per = PinnedStage::BYTES / 8;
+    for (size_t done = 0; done < words; done += per) {
+        const size_t k = std::min(per, words - done);
+        is.read(b, (std::streamsize)(k * 8));
+        if (!is) throw std::invalid_argument(what);
+        hip_check(hipMemcpy(d + done, b, k * 8, hipMemcpyHostToDevice), "copy to device");
+    }
+}
+
 // Plaintext::operator=(const std::string &) (src/plaintext.cpp:91-200): "7FFx^3 + 1x^1
 // + 3" -- hexadecimal coefficients, decimal exponents strictly decreasing, "x^1" for
 // the linear term, a bare constant term, terms joined by exactly " + ".
@@ -1104,9 +1147,8 @@ public:
         detail::put(os, parms_id_);
         detail::put(os, (uint64_t)coeffCount());
         detail::put(os, scale_);
-        const std::vector<uint64_t> h = data_.toHost();
-        detail::put(os, (uint64_t)h.size());
-        os.write(reinterpret_cast<const char *>(h.data()), (std::streamsize)(h.size() * 8));
+        detail::put(os, (uint64_t)data_.size());
+        detail::write_device_words(os, data_.get(), data_.size());
     }
     void load(std::istream &is) {
         const char *what = "stream ended before the plaintext did";
@@ -1115,10 +1157,9 @@ public:
         const double scale = detail::get<double>(is, what);
         const uint64_t words = detail::get<uint64_t>(is, what);
         if (words != count) throw std::invalid_argument("plaintext word count does not match its coefficient count");
-        std::vector<uint64_t> h(words);
-        is.read(reinterpret_cast<char *>(h.data()), (std::streamsize)(words * 8));
-        if (!is) throw std::invalid_argument(what);
-        data_.fromHost(h.data(), h.size());
+        DeviceArray<uint64_t> d(words);  // the plaintext changes only once the words are in
+        detail::read_device_words(is, d.get(), words, what);
+        data_ = std::move(d);
         parms_id_ = id;
         scale_ = scale;
     }
@@ -1215,10 +1256,9 @@ public:
     // curand stream) are not loadable here.
     void save(std::ostream &os) const {
         saveHeader(os, false);
-        std::vector<uint64_t> h = data_.toHost();
         const uint64_t words = (uint64_t)(size_ * polyWords());
         detail::put(os, words);
-        os.write(reinterpret_cast<const char *>(h.data()), (std::streamsize)(words * 8));
+        detail::write_device_words(os, data_.get(), words);
     }
     void load(std::istream &is, const SEALContext &ctx) {
         loadHeader(is, ctx, false);
@@ -1311,9 +1351,8 @@ private:
     void loadWords(std::istream &is) {
         const uint64_t words = detail::get<uint64_t>(is, "stream ended before the ciphertext did");
         if (words != size_ * polyWords()) throw std::invalid_argument("ciphertext word count does not match its shape");
-        std::vector<uint64_t> h;
-        detail::get_words(is, h, words);
-        data_.fromHost(h.data(), h.size());
+        data_.resize(words);
+        detail::read_device_words(is, data_.get(), words, "stream ended before the ciphertext did");
     }
 
     DeviceArray<uint64_t> data_;
@@ -1393,7 +1432,6 @@ public:
         detail::put(os, parms_id_);
         detail::put(os, (uint64_t)keys_.size());
         const size_t pk_words = 2 * K_ * n_;
-        std::vector<uint64_t> h;
         for (const DeviceArray<uint64_t> &k : keys_) {
             if (!k.size()) {
                 detail::put(os, (uint64_t)0);
@@ -1401,7 +1439,6 @@ public:
             }
             if (K_ < 2 || k.size() != (K_ - 1) * pk_words)
                 throw std::logic_error("key shape is not set or does not match the key");
-            h = k.toHost();
             detail::put(os, (uint64_t)(K_ - 1));
             detail::CtHeader hd;
             hd.id = parms_id_;
@@ -1412,7 +1449,7 @@ public:
             for (size_t j = 0; j + 1 < K_; j++) {
                 detail::put_header(os, hd);
                 detail::put(os, (uint64_t)pk_words);
-                os.write(reinterpret_cast<const char *>(h.data() + j * pk_words), (std::streamsize)(pk_words * 8));
+                detail::write_device_words(os, k.get() + j * pk_words, pk_words);
             }
         }
     }
@@ -1423,11 +1460,9 @@ public:
         if (slots > (1u << 18)) throw std::invalid_argument("key slot count is not valid");
         std::vector<DeviceArray<uint64_t>> keys(slots);
         size_t n = 0, K = 0;
-        std::vector<uint64_t> all, part;
         for (uint64_t s = 0; s < slots; s++) {
             const uint64_t m = detail::get<uint64_t>(is, what);
             if (!m) continue;
-            all.clear();
             for (uint64_t j = 0; j < m; j++) {
                 const detail::CtHeader hd = detail::get_header(is);
                 if (hd.seed != 0 || hd.terms || hd.size != 2 || !hd.ntt || hd.id != id)
@@ -1442,10 +1477,9 @@ public:
                     throw std::invalid_argument("key parts differ in shape or count");
                 const uint64_t words = detail::get<uint64_t>(is, what);
                 if (words != 2 * K * n) throw std::invalid_argument("key part word count does not match its shape");
-                detail::get_words(is, part, words);
-                all.insert(all.end(), part.begin(), part.end());
+                if (j == 0) keys[s].resize(m * words);
+                detail::read_device_words(is, keys[s].get() + j * words, words, what);
             }
-            keys[s].fromHost(all.data(), all.size());
         }
         keys_ = std::move(keys);
         parms_id_ = id;

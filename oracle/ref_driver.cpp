@@ -263,6 +263,18 @@ int ref_encrypt(void *h, uint64_t L, const uint64_t *pk_words, const uint64_t *p
     } catch (const std::exception &e) { fprintf(stderr, "ref_encrypt: %s\n", e.what()); return -1; }
 }
 
+// KeyGenerator(context) + createRelinKeys (src/keygenerator.cpp:36-150): a fresh secret
+// key, then its relinearization key, from the reference's own generator -- timed by the
+// benchmark's CPU baseline; the key is discarded.
+int ref_keygen_relin(void *h) {
+    try {
+        auto &s = *static_cast<RefSession *>(h);
+        KeyGenerator kg(*s.context);
+        RelinKeys rk = kg.createRelinKeys();
+        return rk.data().empty() ? -1 : 0;
+    } catch (const std::exception &e) { fprintf(stderr, "ref_keygen_relin: %s\n", e.what()); return -1; }
+}
+
 // BatchEncoder::encode / decode (src/batchencoder.cpp:104-200)
 int ref_batch_encode(void *h, const uint64_t *values, uint64_t count, uint64_t *plain) {
     try {

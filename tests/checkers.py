@@ -112,6 +112,7 @@ def ref():
         dp = ctypes.POINTER(ctypes.c_double)
         lib.ref_decrypt.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
         lib.ref_encrypt.argtypes = [ctypes.c_void_p, u64, u64p, u64p, u64p]
+        lib.ref_keygen_relin.argtypes = [ctypes.c_void_p]
         lib.ref_batch_encode.argtypes = [ctypes.c_void_p, u64p, u64, u64p]
         lib.ref_batch_decode.argtypes = [ctypes.c_void_p, u64p, u64p]
         lib.ref_ckks_encode.argtypes = [ctypes.c_void_p, u64, dp, u64, ctypes.c_double, u64p]
@@ -431,6 +432,10 @@ class RefSession:
         assert ref().ref_encrypt(self.h, L, ptr(np.ascontiguousarray(pk)), ptr(np.ascontiguousarray(plain)),
                                  ptr(out)) == 0
         return out
+
+    def keygen_relin(self):
+        """A fresh KeyGenerator's relinearization key (its own randomness: for timing)."""
+        assert ref().ref_keygen_relin(self.h) == 0
 
     def batch_encode(self, values):
         v = np.ascontiguousarray(values, dtype=np.uint64)
