@@ -1731,6 +1731,34 @@ def bench_primitives(args):
                                                          "entry": "shc_batch_decode"}
     del cts, out, db, vals_out, plain, values
     torch.cuda.empty_cache()
+    # context set-up: moduli, NTT / RNS / BEHZ / encoder tables built and uploaded
+    def setup_entry(name, scheme, nn, mods, t, what):
+        reps = 3
+        ctx_ = sealhip.Context(scheme, nn, mods, t)  # warm (first-use costs)
+        del ctx_
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx_ = sealhip.Context(scheme, nn, mods, t)
+            del ctx_
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        d = {"metric": f"context set-ups/s ({what})", "value": 1e3 / ms, "unit": "contexts/s", "ms_per_step": ms,
+             "steps": reps, "config": {"workload": what, "parallelism": "single GPU"},
+             "bound": "host table construction and upload (one-time per parameter set)"}
+        if not args.no_cpu and ck.ref() is not None:
+            def make(kind):
+                return lambda: ck.RefSession(scheme, nn, mods, t)
+            d["cpu_baseline"] = cpu_baseline(make, "contexts/s", 1, cpu_s,
+                                             "SEALContext(parms, expand_mod_chain = true) + Evaluator per call")
+        ops[name] = d
+        print(f"[bench] primitives: {name} {d['value']:.1f} contexts/s", file=sys.stderr, flush=True)
+
+    setup_entry("ckks_context_setup", sealhip.SCHEME_CKKS, 1 << 16, sealhip.coeff_modulus_create(1 << 16, [60] * 25),
+                0, "CKKS N=2^16, CoeffModulus::Create(65536, {60 x 25}): NTT tables of 25 primes, rescale / key-switch "
+                   "constants and CKKS encoder tables of every level")
+    setup_entry("bfv_context_setup", sealhip.SCHEME_BFV, n, moduli, t,
+                "BFV N=8192, BFVDefault(8192), t=65537: NTT tables, BEHZ tables of every level, batching tables")
     first = ops["ckks_encrypt"]
     return {"metric": "per-operation throughput of the path's other entry points (rotation, rescale, encryption, "
                       "decryption, encoders)", "value": first["value"], "unit": first["unit"],
