@@ -1485,6 +1485,38 @@ def bench_primitives(args):
     del out, rb
     torch.cuda.empty_cache()
 
+    # ciphertext addition and plaintext multiplication (NTT form), HBM-bound
+    y = dev((B, 2, L, n))
+    fill_uniform(torch, y, moduli[:L], 2, gen)
+    pl1 = dev((L, n))
+    fill_uniform(torch, pl1, moduli[:L], 0, gen)
+    zo = dev((B, 2, L, n))
+    ms = timed(lambda: ctx.add(x.data_ptr(), y.data_ptr(), zo.data_ptr(), 2 * B, L, st))
+    h_y = ck.uniform_residues(rng, (2, L, n), moduli[:L])
+    h_pl = ck.uniform_residues(rng, (L, n), moduli[:L])
+
+    def make_add(kind):
+        s_ = ref_sess(kind, CKKS, n, moduli)
+        if s_ is not None:
+            return lambda: s_.ct_binary(0, h_ct, h_y, L)
+        return lambda: ck.mod_add(h_ct, h_y, moduli)
+
+    record("ckks_add", "CKKS ciphertext additions/s, " + shape, "additions/s", B, ms, B * 6 * L * limb,
+           "CKKS ciphertext + ciphertext, " + shape, "shc_add", make_add, "one Evaluator::add per call")
+    ms = timed(lambda: ctx.multiply_plain_ntt(x.data_ptr(), pl1.data_ptr(), zo.data_ptr(), 2 * B, L, st))
+
+    def make_mulp(kind):
+        s_ = ref_sess(kind, CKKS, n, moduli)
+        if s_ is not None:
+            return lambda: s_.plain_op(2, h_ct, h_pl, L)
+        return lambda: ck.multiply_plain_ntt(h_ct, h_pl, moduli, logn)
+
+    record("ckks_multiply_plain", "CKKS ciphertext x plaintext products/s, " + shape, "products/s", B, ms,
+           (B * 4 * L + L) * limb, "CKKS ciphertext x NTT-form plaintext, " + shape, "shc_multiply_plain_ntt",
+           make_mulp, "one Evaluator::multiplyPlain per call")
+    del y, zo, pl1
+    torch.cuda.empty_cache()
+
     # real keys for the round trip
     sk, pk = dev((K, n)), dev((2, K, n))
     ctx.keygen_secret(5, sk.data_ptr(), st)
