@@ -360,6 +360,82 @@ __global__ void batch_gather_kernel(const uint64_t *__restrict__ ntt, uint64_t *
 }
 
 
+// ---- key-switching keys, every digit in one launch per stage -----------------------
+// Digit j (blockIdx.y) of a key draws the streams encrypt_zero_symmetric draws for seed_j
+// = sub_seed(seed, 100 + j): a (uniform, K limbs) from sub_seed(seed_j, 0) into
+// key[j][1], e (centred binomial) from sub_seed(seed_j, 1) into key[j][0] -- the words of
+// the single-stream samplers above, word for word.
+template <bool KEYED>
+__global__ void keygen_uniform_kernel(SamplerKey key, uint64_t seed, const DevPrime *__restrict__ primes, uint32_t K,
+                                      int logn, uint64_t *__restrict__ out) {
+    const size_t total = (size_t)K << logn;
+    const uint64_t s = sub_seed(sub_seed(seed, 100 + (uint64_t)blockIdx.y), 0);
+    uint64_t *c1 = out + ((size_t)(2 * blockIdx.y + 1) * K << logn);
+    const size_t units = KEYED ? (total + 3) / 4 : total;
+    for (size_t u = (size_t)blockIdx.x * TPB + threadIdx.x; u < units; u += (size_t)gridDim.x * TPB) {
+        if (KEYED) {
+            uint32_t ks[16];
+            chacha20_block(key, u, s, ks);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const size_t i = u * 4 + k;
+                if (i >= total) break;
+                const DevPrime &P = primes[i >> logn];
+                c1[i] = barrett128(u128{block_word(ks, 2 * k + 1), block_word(ks, 2 * k)}, P.q, P.ratio_lo, P.ratio_hi);
+            }
+        } else {
+            const DevPrime &P = primes[u >> logn];
+            c1[u] = barrett128(u128{stream_word(s, 2 * u + 1), stream_word(s, 2 * u)}, P.q, P.ratio_lo, P.ratio_hi);
+        }
+    }
+}
+
+template <bool KEYED>
+__global__ void keygen_cbd_kernel(SamplerKey key, uint64_t seed, const DevPrime *__restrict__ primes, uint32_t K,
+                                  int logn, uint64_t *__restrict__ out) {
+    const size_t n = (size_t)1 << logn;
+    const uint64_t s = sub_seed(sub_seed(seed, 100 + (uint64_t)blockIdx.y), 1);
+    uint64_t *c0 = out + ((size_t)(2 * blockIdx.y) * K << logn);
+    const size_t units = KEYED ? (n + 7) / 8 : n;
+    for (size_t u = (size_t)blockIdx.x * TPB + threadIdx.x; u < units; u += (size_t)gridDim.x * TPB) {
+        if (KEYED) {
+            uint32_t ks[16];
+            chacha20_block(key, u, s, ks);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const size_t c = u * 8 + k;
+                if (c >= n) break;
+                const int64_t v = small_from_word(SAMPLE_CBD, block_word(ks, k));
+                for (uint32_t l = 0; l < K; l++) {
+                    const uint64_t q = primes[l].q;
+                    c0[(size_t)l * n + c] = v < 0 ? q - (uint64_t)(-v) : (uint64_t)v;
+                }
+            }
+        } else {
+            const int64_t v = small_from_word(SAMPLE_CBD, stream_word(s, u));
+            for (uint32_t l = 0; l < K; l++) {
+                const uint64_t q = primes[l].q;
+                c0[(size_t)l * n + u] = v < 0 ? q - (uint64_t)(-v) : (uint64_t)v;
+            }
+        }
+    }
+}
+
+// key[j][0][l] = -(key[j][1][l] s[l] + key[j][0][l]) for every digit j < K - 1
+// (c0 = -(a s + NTT(e)), the operation order of encrypt_zero_symmetric's NTT form).
+__global__ void keygen_combine_kernel(uint64_t *__restrict__ key, const uint64_t *__restrict__ sk,
+                                      const DevPrime *__restrict__ primes, uint32_t K, int logn, size_t total) {
+    const size_t n = (size_t)1 << logn;
+    for (size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x; idx < total; idx += (size_t)gridDim.x * TPB) {
+        const uint32_t r = uniform_row(idx, logn), j = r / K, l = r - j * K;
+        const size_t c = idx & (n - 1);
+        uint64_t *c0 = key + (((size_t)j * 2 * K + l) << logn) + c;
+        const uint64_t a = c0[(size_t)K << logn];
+        const DevPrime &P = primes[l];
+        *c0 = negmod(addmod(mulmod(a, sk[((size_t)l << logn) + c], P.q, P.ratio_lo, P.ratio_hi), *c0, P.q), P.q);
+    }
+}
+
 // ---- batched public-key encryption ------------------------------------------------
 // Ciphertext g of a batch is the one encrypt() gives for the seed seed + g: u from the
 // stream of sub_seed(seed + g, 0) (ternary), e0 and e1 from sub_seed(seed + g, 1 / 2)
@@ -576,9 +652,26 @@ void keygen_kswitch(const Context &c, const uint64_t *sk, const uint64_t *new_ke
                     hipStream_t st) {
     const uint32_t K = (uint32_t)c.key_count();
     if (K < 2) fail(ERR_LOGIC, "keyswitching is not supported by the context");
-    const size_t n = c.n;
-    for (uint32_t j = 0; j + 1 < K; j++)
-        encrypt_zero_symmetric(c, sk, c.primes(), K, true, sub_seed(seed, 100 + j), key + (size_t)j * 2 * K * n, st);
+    // every digit's symmetric encryption of zero (encrypt_zero_symmetric with seed
+    // sub_seed(seed, 100 + j)) in one launch per stage
+    const DevPrime *primes = c.primes();
+    SamplerKey skey{};
+    for (int i = 0; i < 8; i++) skey.k[i] = c.sampler_key[i];
+    const size_t words = (size_t)K << c.logn;
+    const dim3 ug(blocks_for(c.sampler_keyed ? (words + 3) / 4 : words), K - 1);
+    const dim3 eg(blocks_for(c.sampler_keyed ? (c.n + 7) / 8 : c.n), K - 1);
+    if (c.sampler_keyed) {
+        keygen_uniform_kernel<true><<<ug, TPB, 0, st>>>(skey, seed, primes, K, c.logn, key);
+        keygen_cbd_kernel<true><<<eg, TPB, 0, st>>>(skey, seed, primes, K, c.logn, key);
+    } else {
+        keygen_uniform_kernel<false><<<ug, TPB, 0, st>>>(skey, seed, primes, K, c.logn, key);
+        keygen_cbd_kernel<false><<<eg, TPB, 0, st>>>(skey, seed, primes, K, c.logn, key);
+    }
+    launch_check("keygen sample");
+    ntt_forward_grid(key, LimbGrid{1, 2 * K, K, K - 1}, primes, c.logn, NTT_FULL, st);
+    const size_t cw = (size_t)(K - 1) * words;
+    keygen_combine_kernel<<<blocks_for(cw), TPB, 0, st>>>(key, sk, primes, K, c.logn, cw);
+    launch_check("keygen combine");
     const size_t total = (size_t)(K - 1) << c.logn;
     kswitch_fold_kernel<<<blocks_for(total), TPB, 0, st>>>(key, new_key, c.d_kswitch_factors.as<uint64_t>(),
                                                            c.primes(), K, c.logn, total);

@@ -499,3 +499,40 @@ def test_batch_encoder_many_matches_single_calls():
     ctx.batch_encode_many(dv.data_ptr(), count, stride, 0, plains.data_ptr(), stream())  # nothing to do
     with pytest.raises(hip().SealHipError):
         ctx.batch_encode_many(dv.data_ptr(), count, count - 1, 2, plains.data_ptr(), stream())
+
+
+@pytest.mark.parametrize("keyed,logn", [(False, 11), (True, 11), (False, 13)])
+def test_relin_key_words_match_oracle(keyed, logn):
+    """Relinearization and Galois key words equal the oracle's key-switching key built from
+    the library's documented streams: digit j encrypts with seed_j = sub_seed(seed, 100 +
+    j), a from sub_seed(seed_j, 0) (uniform), e from sub_seed(seed_j, 1) (centred
+    binomial), then limb j of c0 takes new_key * (P mod q_j)."""
+    n = 1 << logn
+    moduli = ck.coeff_modulus_create(n, [40, 40, 40, 50])
+    ctx = hip().Context(CKKS, n, moduli)
+    key = bytes(range(40, 72)) if keyed else None
+    if keyed:
+        ctx.set_sampler_key(key)
+    K = len(moduli)
+    sk, rk, gk = empty_dev((K, n)), empty_dev((K - 1, 2, K, n)), empty_dev((K - 1, 2, K, n))
+    ctx.keygen_secret(3, sk.data_ptr(), stream())
+    w = ctx.keygen_workspace_size()
+    wb = ws(w)
+    seed = 4
+    ctx.keygen_relin(sk.data_ptr(), seed, rk.data_ptr(), wb.data_ptr(), w, stream())
+    elt = ctx.galois_elt_from_step(1)
+    ctx.keygen_galois(sk.data_ptr(), elt, seed + 1, gk.data_ptr(), wb.data_ptr(), w, stream())
+    s = from_dev(sk)
+
+    def oracle_key(sd, new_key):
+        seeds = [sub_seed(sd, 100 + j) for j in range(K - 1)]
+        a = np.stack([ck.sample_poly(2, key, sub_seed(sj, 0), logn, moduli) for sj in seeds])
+        e = np.stack([ck.sample_poly(1, key, sub_seed(sj, 1), logn, moduli) for sj in seeds])
+        return ck.generate_kswitch_key(s, new_key, a, e, moduli, logn)
+
+    s2 = (s.astype(object) * s.astype(object) % np.array(moduli, dtype=object)[:, None]).astype(np.uint64)
+    np.testing.assert_array_equal(from_dev(rk), oracle_key(seed, s2))
+    sg = np.zeros_like(s)
+    for i in range(K):
+        ck.oracle().or_apply_galois_ntt(logn, elt, ck.ptr(np.ascontiguousarray(s[i])), ck.ptr(sg[i]))
+    np.testing.assert_array_equal(from_dev(gk), oracle_key(seed + 1, sg))
