@@ -1272,7 +1272,8 @@ def bench_bfv8192(args):
     0x7fffffc8001, 0xfffffffc001, 0xffffff6c001, 0xfffffebc001} (4 data + special),
     t = 65537, real keys (device key generation).  Two vectors of 8192 slots uniform mod t
     are batch-encoded and encrypted once; a step is multiply -> relinearize -> decrypt ->
-    batch-decode of that pair (one request, latency-bound).  Self-checks: the decoded
+    batch-decode of that pair (one request, latency-bound), timed eagerly and then replayed
+    from a captured HIP graph (value: the graph replays).  Self-checks: the decoded
     slots are the slot-wise products mod t, and the multiply + relinearize words equal the
     oracle's on the same ciphertexts and key."""
     import numpy as np
@@ -1313,13 +1314,14 @@ def bench_bfv8192(args):
     mb, kb, db = wsb(mw), wsb(kw1), wsb(dw)
     out, vals, scratch = dev((n,)), dev((n,)), dev((n,))
 
-    def step(probe=None):
-        ctx.multiply(cts[0].data_ptr(), cts[1].data_ptr(), prod.data_ptr(), 1, L, mb.data_ptr(), mw, st)
+    def step(probe=None, stream=None):
+        s_ = stream or st
+        ctx.multiply(cts[0].data_ptr(), cts[1].data_ptr(), prod.data_ptr(), 1, L, mb.data_ptr(), mw, s_)
         if probe:
             ctx.probe_keyswitch(*probe)
-        ctx.relinearize(prod.data_ptr(), rk.data_ptr(), 1, L, kb.data_ptr(), kw1, st)
-        ctx.decrypt(prod.data_ptr(), 2, sk.data_ptr(), L, out.data_ptr(), db.data_ptr(), dw, st)
-        ctx.batch_decode(out.data_ptr(), vals.data_ptr(), scratch.data_ptr(), st)
+        ctx.relinearize(prod.data_ptr(), rk.data_ptr(), 1, L, kb.data_ptr(), kw1, s_)
+        ctx.decrypt(prod.data_ptr(), 2, sk.data_ptr(), L, out.data_ptr(), db.data_ptr(), dw, s_)
+        ctx.batch_decode(out.data_ptr(), vals.data_ptr(), scratch.data_ptr(), s_)
 
     for _ in range(max(args.warmup, 1)):
         step()
@@ -1339,6 +1341,30 @@ def bench_bfv8192(args):
                         moduli, L, logn)
     if not np.array_equal(prod.cpu().numpy().view(np.uint64)[0, :2], o2):
         raise SystemExit("bench self-check failed: BFV N=8192 multiply+relinearize words differ from the oracle")
+    # the same pipeline captured in a HIP graph (its ~30 launches replayed as one): every
+    # replay, with eager work between, must decode to the slot products
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(graph, stream=cap):
+            step(stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.current_stream().wait_stream(cap)
+    want_t = torch.from_numpy(want.view(np.int64).copy()).cuda()
+    for _ in range(3):
+        vals.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        if not torch.equal(vals, want_t):
+            raise SystemExit("bench self-check failed: a graph replay of the BFV pipeline lost the slot products")
+        step()
+        torch.cuda.synchronize()
+    with Timer(torch) as tg:
+        for _ in range(steps):
+            graph.replay()
+    graph_ms = tg.ms / steps
+    eager_ms = ms
+    ms = graph_ms
     limb = n * 8
     # the probed launch: the ModUp rows pass over the L(L+1) limbs of one ciphertext (read
     # and written once) and the L+1 twiddle tables
@@ -1348,16 +1374,19 @@ def bench_bfv8192(args):
                 "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                 "dtype": "u64", "data": "synthetic (seeded slots mod t; device-generated keys)",
                 "config": {"workload": "BFV N=8192, BFVDefault(8192) primes (4 data + special), t=65537: one "
-                                       "multiply -> relinearize -> decrypt -> batch decode per step",
+                                       "multiply -> relinearize -> decrypt -> batch decode per step, captured in a "
+                                       "HIP graph and replayed",
                            "parallelism": "single GPU"},
-                "latency_ms": ms, "encrypt_two_ms": enc_ms,
+                "latency_ms": ms, "eager_latency_ms": eager_ms, "graph_over_eager": eager_ms / ms,
+                "encrypt_two_ms": enc_ms,
                 "roofline": {"bound": "hbm", "achieved": rows_bytes / (rows_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": rows_bytes / (rows_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                              "traffic": None,
                              "kernel": "ntt_rows_modup_kernel<13,...>: the key switch's ModUp rows pass "
                                        f"({L * (L + 1)} limbs; a latency-bound launch)",
                              "algorithmic_bytes": rows_bytes, "avg_ms": rows_ms},
-                "self_check": "decrypts to the slot products mod t; multiply+relinearize words == oracle"}
+                "self_check": "decrypts to the slot products mod t (eager and every graph replay); "
+                              "multiply+relinearize words == oracle"}
     if not args.no_cpu:
         ct2 = [np.ascontiguousarray(c[0]), np.ascontiguousarray(c[1])]
         key_h = rk.cpu().numpy().view(np.uint64).copy()
@@ -1834,6 +1863,7 @@ def bench_default(args):
                 continue
             d = {k: r[k] for k in SUB_KEYS if k in r}
             for k in ("batched", "plain", "capacity_per_s", "admission", "hoisted_over_separate", "graph_over_eager",
+                      "eager_latency_ms",
                       "latency_ms", "self_check", "per_level_graph_ms", "grouped_graph_ms", "path"):
                 if k in r:
                     d[k] = r[k]
