@@ -189,8 +189,11 @@ __global__ void round_reduce_kernel(const double2 *__restrict__ a, uint64_t *__r
         d = fabs(d);
         wave_max_atomic(maxbits, (unsigned long long)__double_as_longlong(d));
         if (!(d < two64 * two64)) d = 0;  // too large, inf or NaN: rejected on the host after the range check
-        const uint64_t hi = (uint64_t)(d / two64);
-        const uint64_t lo = (uint64_t)fmod(d, two64);
+        // d = hi 2^64 + lo exactly: scaling by a power of two is exact, and d mod 2^64 (the
+        // low bits of a 53-bit mantissa) is representable, so the subtraction is exact
+        const double hid = floor(d * (1.0 / two64));
+        const uint64_t hi = (uint64_t)hid;
+        const uint64_t lo = (uint64_t)(d - hid * two64);
         for (uint32_t l = 0; l < L; l++) {
             const DevPrime &P = primes[l];
             const uint64_t r = barrett128(u128{lo, hi}, P.q, P.ratio_lo, P.ratio_hi);
