@@ -103,7 +103,7 @@ class Timer:
 
 def cpu_threads():
     """Host threads for the all-core CPU baseline: every core the process may use, at
-    most 16 (one GPU box's CPU share)."""
+    most 16 (the CPU share of one GPU's host)."""
     try:
         avail = len(os.sched_getaffinity(0))
     except AttributeError:

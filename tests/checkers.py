@@ -88,7 +88,7 @@ def oracle():
 
 
 def ref():
-    """The reference CPU library, or None where it was not built (e.g. on a GPU box)."""
+    """The reference CPU library, or None where it was not built (a machine without the reference sources)."""
     global _ref
     if _ref is None and os.path.exists(REF_SO):
         lib = ctypes.CDLL(REF_SO)
@@ -491,7 +491,7 @@ class RefSession:
 def parallel_map(fn, items, workers=None):
     """[fn(x) for x in items] on a thread pool.  The oracle keeps no mutable global
     state and ctypes releases the GIL during a call, so oracle calls on large shapes
-    run side by side; at most 16 threads (one GPU box's CPU share)."""
+    run side by side; at most 16 threads (the CPU share of one GPU's host)."""
     from concurrent.futures import ThreadPoolExecutor
     items = list(items)
     workers = workers or max(1, min(16, os.cpu_count() or 1, len(items)))
