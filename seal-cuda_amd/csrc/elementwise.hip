@@ -143,16 +143,17 @@ __global__ void plain_lift_kernel(const uint64_t *__restrict__ plain, uint64_t *
 // A thread owns one word position (l, c) of G consecutive polynomials (ct, p): each
 // ciphertext word it loads serves H outputs (the diagonal matrix-vector products' giant
 // steps) and each plaintext word G polynomials, so the plaintexts -- shared by every
-// ciphertext of the batch -- come from HBM polys / G times instead of polys times.  Sums
-// in three words (Acc3), folded mod q every 64 terms.  The G x H sums take ~150
-// registers: the launch bound lets them stay there (two waves per SIMD at least).
+// ciphertext of the batch -- are loaded polys / G times instead of polys times (and the
+// block order below serves the repeats from L2).  Sums in three words (Acc3), folded mod
+// q every 64 terms.  With G = 2 and H = 8 the sums fit three waves per SIMD (the launch
+// bound), which measured faster than G = 4 at two waves.
 //
 // xcd: the grid is exactly groups x (lw / TPB) blocks with lw / TPB a multiple of 8, and
 // block B (dispatched to XCD B % 8) takes position block 8 (B / 8 / groups) + B % 8 of
 // polynomial group (B / 8) % groups: consecutive blocks of one XCD share their plaintext
 // words, which then come from that XCD's L2 after the first.
 template <int H, int G>
-__global__ __launch_bounds__(TPB, 2) void plain_dot_many_kernel(const uint64_t *__restrict__ cts, const uint64_t *__restrict__ plains,
+__global__ __launch_bounds__(TPB, 3) void plain_dot_many_kernel(const uint64_t *__restrict__ cts, const uint64_t *__restrict__ plains,
                                       uint64_t *__restrict__ out, const DevPrime *__restrict__ primes, uint32_t terms,
                                       uint32_t h0, uint32_t hn, uint32_t polys, uint32_t L, int logn, size_t total,
                                       bool xcd) {
@@ -211,8 +212,8 @@ void launch_plain_dot(uint32_t G, const uint64_t *cts, const uint64_t *plains, u
     const size_t rblocks = lw / TPB;
     // the XCD-aware order where the grid is one block per (group, position block)
     const bool xcd = groups > 1 && lw % TPB == 0 && rblocks % 8 == 0 && groups * rblocks < (size_t(1) << 20);
-    if (G == 4)
-        plain_dot_many_kernel<H, 4><<<blocks_for(total), TPB, 0, st>>>(cts, plains, out, primes, terms, h0, hn, polys, L,
+    if (G == 2)
+        plain_dot_many_kernel<H, 2><<<blocks_for(total), TPB, 0, st>>>(cts, plains, out, primes, terms, h0, hn, polys, L,
                                                                         logn, total, xcd);
     else
         plain_dot_many_kernel<H, 1><<<blocks_for(total), TPB, 0, st>>>(cts, plains, out, primes, terms, h0, hn, polys, L,
@@ -306,9 +307,9 @@ void multiply_plain_accumulate_many(const Context &c, const uint64_t *cts, const
     if (!terms) fail(ERR_INVALID_ARGUMENT, "no terms to accumulate");
     if (!outputs) fail(ERR_INVALID_ARGUMENT, "no outputs to accumulate");
     if (!polys) return;
-    // four polynomials per thread once the batch has them (the plaintexts' reuse), one
-    // for a single ciphertext
-    const uint32_t G = polys >= 4 ? 4 : 1;
+    // two polynomials per thread (the plaintexts' reuse) -- a ciphertext's pair -- one
+    // for a lone polynomial
+    const uint32_t G = polys >= 2 ? 2 : 1;
     const DevPrime *primes = c.primes();
     const int logn = c.logn;
     for (uint32_t h0 = 0; h0 < outputs;) {
